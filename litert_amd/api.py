@@ -1,0 +1,360 @@
+"""Python binding of liblrt_hip.so (ctypes) for tests and benchmarks.
+
+Mirrors the reference renderer API (Renderer/eye_ray.h:77-252):
+``CreateMultiRenderer(DEVICE_GPU)`` returns a :class:`MultiRenderer` with
+``SetScene``, ``SetPreset``, ``SetViewport``, ``UpdateCamera``, ``Render``,
+``RenderFloat`` and batched ``RayQuery_NearestHit``. All computation runs in
+the HIP library; importing this module fails loudly when the library is not
+built, and no call falls back to a CPU path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))  # litert_amd/
+LIB_PATH = os.path.join(_HERE, "liblrt_hip.so")
+
+# ---- constants (include/lrt_types.h) ---------------------------------------
+DEVICE_CPU, DEVICE_GPU, DEVICE_GPU_RTX = 0, 1, 2
+SDF_OCTREE_NODE_INTERSECT_ST, SDF_OCTREE_NODE_INTERSECT_ANALYTIC, SDF_OCTREE_NODE_INTERSECT_NEWTON, \
+    SDF_OCTREE_NODE_INTERSECT_BBOX, SDF_OCTREE_NODE_INTERSECT_IT = range(5)
+MULTI_RENDER_MODE_MASK = 0
+MULTI_RENDER_MODE_LAMBERT_NO_TEX = 1
+MULTI_RENDER_MODE_DEPTH = 2
+MULTI_RENDER_MODE_LINEAR_DEPTH = 3
+MULTI_RENDER_MODE_INVERSE_LINEAR_DEPTH = 4
+MULTI_RENDER_MODE_PRIMITIVE = 5
+MULTI_RENDER_MODE_TYPE = 6
+MULTI_RENDER_MODE_GEOM = 7
+MULTI_RENDER_MODE_NORMAL = 8
+MULTI_RENDER_MODE_ST_ITERATIONS = 10
+MULTI_RENDER_MODE_DIFFUSE = 16
+NORMAL_MODE_GEOMETRY, NORMAL_MODE_VERTEX, NORMAL_MODE_SDF_SMOOTHED = 0, 1, 2
+RAY_GEN_MODE_REGULAR, RAY_GEN_MODE_RANDOM = 0, 1
+REPRESENTATION_MODE_SURFACE, REPRESENTATION_MODE_VOLUME = 0, 1
+SDF_SBS_NODE_LAYOUT_DX = 1 << 24
+SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F = 4 << 24
+SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F_IN = 5 << 24
+INVALID_IDX = 1 << 31
+
+
+class SdfSBSHeader(C.Structure):
+    _fields_ = [("brick_size", C.c_uint32), ("brick_pad", C.c_uint32), ("bytes_per_value", C.c_uint32), ("aux_data", C.c_uint32)]
+
+
+class MultiRenderPreset(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("render_mode", "sdf_node_intersect", "normal_mode", "ray_gen_mode",
+                                           "interpolation_mode", "representation_mode", "spp")]
+
+
+class CRT_Hit(C.Structure):
+    _fields_ = [("t", C.c_float), ("primId", C.c_uint32), ("instId", C.c_uint32), ("geomId", C.c_uint32),
+                ("coords", C.c_float * 4)]
+
+
+class MultiRendererDRPreset(C.Structure):
+    _fields_ = [
+        ("dr_loss_function", C.c_uint32), ("dr_render_mode", C.c_uint32), ("dr_diff_mode", C.c_uint32),
+        ("dr_reconstruction_flags", C.c_uint32), ("dr_input_type", C.c_uint32), ("dr_border_sampling", C.c_uint32),
+        ("dr_raycasting_mask", C.c_uint32), ("dr_atomic_ders", C.c_uint32), ("spp", C.c_uint32),
+        ("render_width", C.c_uint32), ("render_height", C.c_uint32), ("border_spp", C.c_uint32),
+        ("border_relax_eps", C.c_float), ("border_integral_mult", C.c_float), ("opt_lr", C.c_float),
+        ("opt_beta_1", C.c_float), ("opt_beta_2", C.c_float), ("opt_eps", C.c_float), ("opt_iterations", C.c_uint32),
+        ("image_batch_size", C.c_uint32), ("reg_function", C.c_uint32), ("reg_lambda", C.c_float),
+        ("reg_power", C.c_float), ("redistancing_enable", C.c_bool), ("redistancing_interval", C.c_uint32),
+        ("dr_raycasting_thickness", C.c_uint32), ("debug_print", C.c_bool), ("debug_print_interval", C.c_uint32),
+        ("debug_progress_images", C.c_uint32), ("debug_progress_interval", C.c_uint32),
+        ("debug_render_mode", C.c_uint32), ("finite_diff_delta", C.c_float), ("finite_diff_brightness", C.c_float),
+        ("debug_forced_border", C.c_bool), ("debug_pd_images", C.c_bool), ("debug_pd_brightness", C.c_float),
+        ("debug_border_samples", C.c_bool), ("debug_border_samples_mega_image", C.c_bool),
+        ("debug_border_save_normals", C.c_bool),
+    ]
+
+
+def getDefaultPreset():
+    """render_settings.h:94-108."""
+    return MultiRenderPreset(MULTI_RENDER_MODE_LAMBERT_NO_TEX, SDF_OCTREE_NODE_INTERSECT_ST, NORMAL_MODE_GEOMETRY,
+                             RAY_GEN_MODE_REGULAR, 0, REPRESENTATION_MODE_VOLUME, 1)
+
+
+def getDefaultPresetDR():
+    """diff_render/MultiRendererDR.h:14-69."""
+    p = MultiRendererDRPreset()
+    p.spp = 1
+    p.dr_reconstruction_flags = 1
+    p.dr_atomic_ders = 0xFFFFFFFF
+    p.dr_raycasting_thickness = 5
+    p.border_spp = 256
+    p.border_relax_eps = 1e-3
+    p.border_integral_mult = 1.0
+    p.opt_lr = 0.01
+    p.opt_beta_1 = 0.9
+    p.opt_beta_2 = 0.999
+    p.opt_eps = 1e-8
+    p.opt_iterations = 500
+    p.image_batch_size = 1
+    p.reg_lambda = 0.25
+    p.reg_power = 2.0
+    p.redistancing_interval = 1
+    p.debug_print_interval = 10
+    p.debug_progress_images = 1001
+    p.debug_progress_interval = 100
+    p.finite_diff_delta = 0.001
+    p.finite_diff_brightness = 100.0
+    p.debug_pd_brightness = 0.1
+    return p
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (python -m litert_amd.build)")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    u32, u64, f32, i32 = C.c_uint32, C.c_uint64, C.c_float, C.c_int
+    sig = {
+        "lrt_last_error": (C.c_char_p, []),
+        "lrt_create": (i32, [u32, u32, C.POINTER(P)]),
+        "lrt_destroy": (i32, [P]),
+        "lrt_set_scene_sbs": (i32, [P, C.POINTER(SdfSBSHeader), P, u32, P, u64, P, u64]),
+        "lrt_get_bvh": (i32, [P, P, u32, C.POINTER(u32), C.POINTER(u32)]),
+        "lrt_build_bvh": (i32, [P, u32, P, u32, C.POINTER(u32), C.POINTER(u32)]),
+        "lrt_set_preset": (i32, [P, C.POINTER(MultiRenderPreset)]),
+        "lrt_get_preset": (i32, [P, C.POINTER(MultiRenderPreset)]),
+        "lrt_set_viewport": (i32, [P, i32, i32, i32, i32]),
+        "lrt_update_camera": (i32, [P, P, P]),
+        "lrt_set_seed": (i32, [P, u32]),
+        "lrt_render": (i32, [P, P, u32, u32, i32]),
+        "lrt_render_float": (i32, [P, P, u32, u32, i32]),
+        "lrt_render_gbuffer": (i32, [P, P, P, P, u32, u32]),
+        "lrt_cast_rays": (i32, [P, P, u32, P]),
+        "lrt_get_execution_time": (i32, [P, C.c_char_p, P]),
+        "lrt_render_views_device": (i32, [P, P, P, u32, u32, u32, i32, C.POINTER(f32)]),
+        "lrt_download_view": (i32, [P, u32, P, P, P, P]),
+        "lrt_count_work": (i32, [P, P, P, u32, u32, u32, P]),
+        "lrt_sync": (i32, [P]),
+        "lrt_set_launch_blocks": (i32, [P, u32]),
+        "lrt_sbs_displaced_icosphere": (i32, [u32, f32, f32, u32, u32, u32, u32, u32, u32, i32, i32, C.POINTER(P)]),
+        "lrt_sbs_grid": (i32, [u32, f32, f32, f32, u32, u32, u32, u32, i32, C.POINTER(P)]),
+        "lrt_sbs_info": (i32, [P, C.POINTER(SdfSBSHeader), C.POINTER(u32), C.POINTER(u64), C.POINTER(u64)]),
+        "lrt_sbs_copy": (i32, [P, P, P, P]),
+        "lrt_sbs_free": (i32, [P]),
+        "lrt_orbit_camera": (i32, [u32, u32, f32, f32, f32, f32, f32, P, P]),
+    }
+    optional = {}
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype, f.argtypes = res, args
+    for name, (res, args) in optional.items():
+        if hasattr(lib, name):
+            f = getattr(lib, name)
+            f.restype, f.argtypes = res, args
+    return lib
+
+
+lib = _load()
+
+
+class LrtError(RuntimeError):
+    pass
+
+
+def _check(code, what=""):
+    if code != 0:
+        raise LrtError(f"{what} failed ({code}): {lib.lrt_last_error().decode()}")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class SdfSBS:
+    """Reference SdfSBS (sdfScene/sdf_scene.h:226-232): header, nodes (n,4)
+    uint32 (pos_xy, pos_z_lod_size, data_offset, pad), values, values_f."""
+
+    def __init__(self, header, nodes, values, values_f=None):
+        self.header = header
+        self.nodes = np.ascontiguousarray(nodes, dtype=np.uint32).reshape(-1, 4)
+        self.values = np.ascontiguousarray(values, dtype=np.uint32)
+        self.values_f = np.zeros(0, np.float32) if values_f is None else _f32(values_f)
+
+    @staticmethod
+    def _from_handle(h):
+        hdr = SdfSBSHeader()
+        nn, nv, nf = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        _check(lib.lrt_sbs_info(h, C.byref(hdr), C.byref(nn), C.byref(nv), C.byref(nf)), "lrt_sbs_info")
+        nodes = np.zeros((nn.value, 4), np.uint32)
+        values = np.zeros(nv.value, np.uint32)
+        vf = np.zeros(nf.value, np.float32)
+        _check(lib.lrt_sbs_copy(h, _ptr(nodes), _ptr(values), _ptr(vf) if nf.value else None), "lrt_sbs_copy")
+        lib.lrt_sbs_free(h)
+        return SdfSBS(hdr, nodes, values, vf)
+
+    def lod(self):
+        return int(self.nodes[0, 1] & 0xFFFF)
+
+
+def displaced_icosphere(subdiv=5, radius=0.8, amplitude=0.05, octaves=3, seed=1, lod=32, brick_size=4, brick_pad=1,
+                        bytes_per_value=4, voxel_diag_scale=False, threads=0):
+    h = C.c_void_p()
+    _check(lib.lrt_sbs_displaced_icosphere(subdiv, radius, amplitude, octaves, seed, lod, brick_size, brick_pad,
+                                           bytes_per_value, int(voxel_diag_scale), threads, C.byref(h)),
+           "lrt_sbs_displaced_icosphere")
+    return SdfSBS._from_handle(h)
+
+
+def grid_sbs(shape="torus", R=0.6, r=0.2, amplitude=0.0, octaves=3, seed=1, brick_count=32, brick_size=4,
+             with_neighbors=False):
+    h = C.c_void_p()
+    _check(lib.lrt_sbs_grid(1 if shape == "torus" else 0, R, r, amplitude, octaves, seed, brick_count, brick_size,
+                            int(with_neighbors), C.byref(h)), "lrt_sbs_grid")
+    return SdfSBS._from_handle(h)
+
+
+def build_bvh(sbs):
+    """Host-side BVH2 build for the bricks of `sbs` (same tree as SetScene)."""
+    n, depth = C.c_uint32(), C.c_uint32()
+    _check(lib.lrt_build_bvh(_ptr(sbs.nodes), sbs.nodes.shape[0], None, 0, C.byref(n), C.byref(depth)), "lrt_build_bvh")
+    pairs = np.zeros((n.value, 16), np.uint32)
+    _check(lib.lrt_build_bvh(_ptr(sbs.nodes), sbs.nodes.shape[0], _ptr(pairs), n.value, None, None), "lrt_build_bvh")
+    return pairs, depth.value
+
+
+def orbit_camera(seed, index, distance=3.0, fov=60.0, aspect=1.0, znear=0.01, zfar=100.0):
+    wv = np.zeros(16, np.float32)
+    pr = np.zeros(16, np.float32)
+    _check(lib.lrt_orbit_camera(seed, index, distance, fov, aspect, znear, zfar, _ptr(wv), _ptr(pr)), "lrt_orbit_camera")
+    return wv, pr
+
+
+class MultiRenderer:
+    """GPU MultiRenderer (Renderer/eye_ray.h:77-250) backed by liblrt_hip.so."""
+
+    def __init__(self, device=DEVICE_GPU, max_primitives=10_000_000):
+        self._h = C.c_void_p()
+        _check(lib.lrt_create(device, max_primitives, C.byref(self._h)), "lrt_create")
+        self.width = self.height = 0
+
+    def close(self):
+        if self._h:
+            lib.lrt_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def SetScene(self, sbs):
+        nv = sbs.values.size
+        nf = sbs.values_f.size
+        _check(lib.lrt_set_scene_sbs(self._h, C.byref(sbs.header), _ptr(sbs.nodes), sbs.nodes.shape[0], _ptr(sbs.values),
+                                     nv, _ptr(sbs.values_f) if nf else None, nf), "SetScene")
+
+    def GetBVH(self):
+        n, depth = C.c_uint32(), C.c_uint32()
+        _check(lib.lrt_get_bvh(self._h, None, 0, C.byref(n), C.byref(depth)), "GetBVH")
+        pairs = np.zeros((n.value, 16), np.uint32)  # 2 x (boxMin[3], leftOffset, boxMax[3], escapeIndex)
+        _check(lib.lrt_get_bvh(self._h, _ptr(pairs), n.value, None, None), "GetBVH")
+        return pairs, depth.value
+
+    def SetPreset(self, preset):
+        _check(lib.lrt_set_preset(self._h, C.byref(preset)), "SetPreset")
+
+    def GetPreset(self):
+        p = MultiRenderPreset()
+        _check(lib.lrt_get_preset(self._h, C.byref(p)), "GetPreset")
+        return p
+
+    def SetViewport(self, x0, y0, w, h):
+        _check(lib.lrt_set_viewport(self._h, x0, y0, w, h), "SetViewport")
+        self.width, self.height = w, h
+
+    def UpdateCamera(self, world_view, proj):
+        _check(lib.lrt_update_camera(self._h, _ptr(_f32(world_view)), _ptr(_f32(proj))), "UpdateCamera")
+
+    def setSeed(self, seed):
+        _check(lib.lrt_set_seed(self._h, seed), "setSeed")
+
+    def RenderFloat(self, width, height, world_view=None, proj=None, preset=None, passes=1):
+        """MultiRenderer::RenderFloat (eye_ray_host.cpp:575-585); returns (H, W, 4) float32."""
+        if world_view is not None:
+            self.SetViewport(0, 0, width, height)
+            self.UpdateCamera(world_view, proj)
+        if preset is not None:
+            self.SetPreset(preset)
+        img = np.zeros((height, width, 4), np.float32)
+        _check(lib.lrt_render_float(self._h, _ptr(img), width, height, passes), "RenderFloat")
+        return img
+
+    def Render(self, width, height, world_view=None, proj=None, preset=None, passes=1):
+        """MultiRenderer::Render (eye_ray_host.cpp:563-573); returns (H, W) uint32 RGBA8."""
+        if world_view is not None:
+            self.SetViewport(0, 0, width, height)
+            self.UpdateCamera(world_view, proj)
+        if preset is not None:
+            self.SetPreset(preset)
+        img = np.zeros((height, width), np.uint32)
+        _check(lib.lrt_render(self._h, _ptr(img), width, height, passes), "Render")
+        return img
+
+    def RenderGBuffer(self, width, height, world_view=None, proj=None):
+        if world_view is not None:
+            self.UpdateCamera(world_view, proj)
+        t = np.zeros((height, width), np.float32)
+        n = np.zeros((height, width, 2), np.float32)
+        prim = np.zeros((height, width), np.uint32)
+        _check(lib.lrt_render_gbuffer(self._h, _ptr(t), _ptr(n), _ptr(prim), width, height), "RenderGBuffer")
+        return t, n, prim
+
+    def RayQuery_NearestHit(self, rays):
+        """Batched RayQuery_NearestHit; rays (n, 8) float32 -> CRT_Hit array."""
+        rays = _f32(rays).reshape(-1, 8)
+        out = (CRT_Hit * rays.shape[0])()
+        _check(lib.lrt_cast_rays(self._h, _ptr(rays), rays.shape[0], C.cast(out, C.c_void_p)), "RayQuery_NearestHit")
+        return out
+
+    def GetExecutionTime(self, name):
+        out = (C.c_float * 4)()
+        _check(lib.lrt_get_execution_time(self._h, name.encode(), C.cast(out, C.c_void_p)), "GetExecutionTime")
+        return list(out)
+
+    def RenderViewsDevice(self, world_views, projs, width, height, out_mode=0, timed=True):
+        wv = _f32(world_views).reshape(-1, 16)
+        pr = _f32(projs).reshape(-1, 16)
+        ms = C.c_float(0.0)
+        _check(lib.lrt_render_views_device(self._h, _ptr(wv), _ptr(pr), wv.shape[0], width, height, out_mode,
+                                           C.byref(ms) if timed else None), "RenderViewsDevice")
+        return ms.value
+
+    def DownloadView(self, view, width, height, rgba=False):
+        t = np.zeros((height, width), np.float32)
+        n = np.zeros((height, width, 2), np.float32)
+        prim = np.zeros((height, width), np.uint32)
+        col = np.zeros((height, width, 4), np.float32) if rgba else None
+        _check(lib.lrt_download_view(self._h, view, None if rgba else _ptr(t), None if rgba else _ptr(n),
+                                     None if rgba else _ptr(prim), _ptr(col)), "DownloadView")
+        return col if rgba else (t, n, prim)
+
+    def CountWork(self, world_views, projs, width, height):
+        wv = _f32(world_views).reshape(-1, 16)
+        pr = _f32(projs).reshape(-1, 16)
+        out = np.zeros(3, np.uint64)
+        _check(lib.lrt_count_work(self._h, _ptr(wv), _ptr(pr), wv.shape[0], width, height, _ptr(out)), "CountWork")
+        return out
+
+    def Sync(self):
+        _check(lib.lrt_sync(self._h), "Sync")
+
+    def SetLaunchBlocks(self, blocks):
+        _check(lib.lrt_set_launch_blocks(self._h, blocks), "SetLaunchBlocks")
+
+
+def CreateMultiRenderer(device=DEVICE_GPU, max_primitives=10_000_000):
+    """CreateMultiRenderer (Renderer/eye_ray.h:252)."""
+    return MultiRenderer(device, max_primitives)

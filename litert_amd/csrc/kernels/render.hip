@@ -1,0 +1,192 @@
+// render.hip - primary-ray render kernels (MultiRenderer::CastRaySingleBlock /
+// CastRayFloatSingleBlock, Renderer/eye_ray_host.cpp:246-282) and batched
+// RayQuery_NearestHit for gfx950.
+//
+// One lane per packed pixel: the packed index space groups each 8x8 pixel
+// super-block (eye_ray.cpp:537-555) into 64 consecutive indices, i.e. one
+// wavefront per screen tile, so neighbouring lanes walk the same BVH nodes
+// and bricks. Blocks are persistent and pull 256-pixel chunks from a global
+// counter, so uneven per-ray work does not leave CUs idle at the tail.
+#include "render.hpp"
+#include "traverse_device.hpp"
+
+namespace lrtd {
+
+// Render modes that apply to SBS geometry (eye_ray.cpp:162-489) with the
+// renderer's default lights: direct normalize(1,1,1) colour 2/3 and ambient
+// 0.25 (eye_ray_host.cpp:40).
+__device__ __forceinline__ float4 shade(uint32_t mode, const Hit& hit, v3 dir) {
+  if (hit.primId == 0xFFFFFFFFu) return make_float4(0, 0, 0, 1);
+  const float z = hit.t, z_near = 0.1f, z_far = 10.0f;
+  const uint32_t type = hit.geomId >> LRT_SH_TYPE;
+  const uint32_t geomId = hit.geomId & LRT_GEOM_ID_MASK;
+  v3 norm = decode_normal(hit.coords[2], hit.coords[3]);
+  const float ns = sgn(dot(-1.0f * dir, norm));
+  norm = norm * ns;
+  const uint32_t pal[20] = {0xffe6194b, 0xff3cb44b, 0xffffe119, 0xff0082c8, 0xfff58231, 0xff911eb4, 0xff46f0f0,
+                            0xfff032e6, 0xffd2f53c, 0xfffabebe, 0xff008080, 0xffe6beff, 0xffaa6e28, 0xfffffac8,
+                            0xff800000, 0xffaaffc3, 0xff808000, 0xffffd8b1, 0xff000080, 0xff808080};
+  auto palc = [](uint32_t c) {
+    return make_float4((float)(c & 0xFF) * (1.0f / 255.0f), (float)((c >> 8) & 0xFF) * (1.0f / 255.0f),
+                       (float)((c >> 16) & 0xFF) * (1.0f / 255.0f), (float)((c >> 24) & 0xFF) * (1.0f / 255.0f));
+  };
+  switch (mode) {
+    case LRT_RENDER_MODE_MASK: return make_float4(1, 1, 1, 1);
+    case LRT_RENDER_MODE_DEPTH: { float d = (1 / z - 1 / z_near) / (1 / z_far - 1 / z_near); return make_float4(d, d, d, 1); }
+    case LRT_RENDER_MODE_LINEAR_DEPTH: { float d = ((z - z_near) / (z_far - z_near)); return make_float4(d, d, d, 1); }
+    case LRT_RENDER_MODE_INVERSE_LINEAR_DEPTH: { float d = 1 - ((z - z_near) / (z_far - z_near)); return make_float4(d, d, d, 1); }
+    case LRT_RENDER_MODE_PRIMITIVE: return palc(pal[hit.primId % 20]);
+    case LRT_RENDER_MODE_TYPE: return palc(pal[type % 20]);
+    case LRT_RENDER_MODE_GEOM: return palc(pal[geomId % 20]);
+    case LRT_RENDER_MODE_NORMAL: return make_float4(fabsf(norm.x), fabsf(norm.y), fabsf(norm.z), 1);
+    case LRT_RENDER_MODE_ST_ITERATIONS: {
+      if (hit.primId == 0) return make_float4(0.5f, 0.5f, 0.5f, 1);
+      if (hit.primId <= 64) {
+        float q = clampf_((float)hit.primId / 64, 0.0f, 1.0f);
+        return make_float4(0, (1 - q), q, q + (1 - q));
+      }
+      float q = clampf_((float)(hit.primId - 64) / (256 - 64), 0.0f, 1.0f);
+      return make_float4(q, 0, (1 - q), q + (1 - q));
+    }
+    case LRT_RENDER_MODE_DIFFUSE: {
+      if (type == LRT_TYPE_SDF_SBS_COL)
+        return make_float4(roundf(hit.coords[0]) / 255.0f, hit.coords[0] - floorf(hit.coords[0]), roundf(hit.coords[1]) / 255.0f, 1);
+      return make_float4(0, 1, 1, 1);
+    }
+    case LRT_RENDER_MODE_LAMBERT_NO_TEX: {
+      v3 ld = normalize(v3(1, 1, 1));
+      float q = mx(0.0f, dot(norm, ld));
+      float c = (2.0f / 3.0f) * q + 0.25f;
+      return make_float4(c, c, c, 1);
+    }
+    default: return make_float4(1, 0, 1, 1);
+  }
+}
+
+__device__ __forceinline__ float2 sample_offset(const RenderArgs& A, uint32_t x, uint32_t y, uint32_t i) {
+  if (A.ray_gen_mode == LRT_RAY_GEN_MODE_RANDOM) return rand2(x, y, i + A.seed % 2, A.seed);
+  const uint32_t spp_sqrt = (uint32_t)sqrt((double)A.spp);
+  const float is = 1.0f / (float)spp_sqrt;
+  return make_float2(is * (float)((double)(i / spp_sqrt) + 0.5), is * (float)((double)(i % spp_sqrt) + 0.5));
+}
+
+template <bool COUNT, bool RGBA>
+__device__ __forceinline__ void render_pixel(const DevScene& S, const TraceParams& P, const Camera* __restrict__ cams, const RenderArgs& A,
+                                             const RenderOut& O, uint64_t gtid, Stack& st, WorkCount& wc) {
+  const uint32_t view = (uint32_t)(gtid / A.n_packed);
+  const uint32_t tid = (uint32_t)(gtid - (uint64_t)view * A.n_packed);
+  uint32_t x, y;
+  tid_to_xy(tid, A.packed_w, x, y);
+  if (x >= A.W || y >= A.H) return;
+  const Camera& C = cams[view];
+  const size_t pix = (size_t)view * A.W * A.H + (size_t)y * A.W + x;
+  float4 acc = make_float4(0, 0, 0, 0);
+  for (uint32_t i = 0; i < A.spp; i++) {
+    const float2 d = sample_offset(A, x, y, i);
+    v3 pos, dir;
+    eye_ray(C, x, y, d.x, d.y, A.W, A.H, pos, dir);
+    Hit hit = RayQuery_NearestHit<COUNT>(S, P, pos, 0.0f, dir, 1e9f, st, &wc);
+    if (i == 0) {
+      if (O.t) O.t[pix] = hit.t;
+      if (O.prim) O.prim[pix] = hit.primId;
+      if (O.n) O.n[pix] = make_float2(hit.coords[2], hit.coords[3]);
+    }
+    if (RGBA) {
+      float4 c = shade(P.render_mode, hit, dir);
+      acc = make_float4(acc.x + c.x, acc.y + c.y, acc.z + c.z, acc.w + c.w);
+    }
+  }
+  if (RGBA) {
+    const float s = (float)A.spp;
+    O.rgba[pix] = make_float4(acc.x / s, acc.y / s, acc.z / s, acc.w / s);
+  }
+}
+
+template <bool COUNT, bool RGBA>
+__global__ __launch_bounds__(256) void k_render(DevScene S, TraceParams P, const Camera* __restrict__ cams, RenderArgs A,
+                                                RenderOut O, uint32_t* work_counter, uint32_t* spill, unsigned long long* counts) {
+  __shared__ uint32_t lds_stack[kStackLds * 256];
+  __shared__ uint32_t chunk_base;
+  Stack st;
+  st.lds = lds_stack + threadIdx.x;
+  st.lds_stride = 256;
+  st.lds_entries = kStackLds;
+  const uint32_t gl = blockIdx.x * 256 + threadIdx.x;
+  st.spill = spill + gl;
+  st.spill_stride = gridDim.x * 256;
+  WorkCount wc;
+  const uint64_t total = (uint64_t)A.n_views * A.n_packed;
+  for (;;) {
+    if (threadIdx.x == 0) chunk_base = atomicAdd(work_counter, 1u);
+    __syncthreads();
+    const uint64_t base = (uint64_t)chunk_base * 256;
+    __syncthreads();
+    if (base >= total) break;  // uniform across the block
+    const uint64_t gtid = base + threadIdx.x;
+    if (gtid < total) render_pixel<COUNT, RGBA>(S, P, cams, A, O, gtid, st, wc);
+  }
+  if (COUNT) {
+    atomicAdd(&counts[0], (unsigned long long)wc.pairs);
+    atomicAdd(&counts[1], (unsigned long long)wc.bricks);
+    atomicAdd(&counts[2], (unsigned long long)wc.voxels);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cast_rays(DevScene S, TraceParams P, const float4* __restrict__ rays, uint32_t n,
+                                                   CRT_Hit* __restrict__ out, uint32_t* spill) {
+  __shared__ uint32_t lds_stack[kStackLds * 256];
+  Stack st;
+  st.lds = lds_stack + threadIdx.x;
+  st.lds_stride = 256;
+  st.lds_entries = kStackLds;
+  const uint32_t gl = blockIdx.x * 256 + threadIdx.x;
+  st.spill = spill + gl;
+  st.spill_stride = gridDim.x * 256;
+  if (gl >= n) return;
+  const float4 pn = rays[2 * gl], df = rays[2 * gl + 1];
+  WorkCount wc;
+  Hit h = RayQuery_NearestHit<false>(S, P, v3(pn.x, pn.y, pn.z), pn.w, v3(df.x, df.y, df.z), df.w, st, &wc);
+  CRT_Hit r;
+  r.t = h.t; r.primId = h.primId; r.instId = h.instId; r.geomId = h.geomId;
+  r.coords[0] = h.coords[0]; r.coords[1] = h.coords[1]; r.coords[2] = h.coords[2]; r.coords[3] = h.coords[3];
+  out[gl] = r;
+}
+
+// Clears small counters with a kernel rather than hipMemsetAsync, so the
+// launch sequence stays valid inside a captured HIP graph.
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+}  // namespace lrtd
+
+namespace lrt {
+
+hipError_t zero_u32(uint32_t* d, uint32_t n, hipStream_t stream) {
+  hipLaunchKernelGGL(lrtd::k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, stream, d, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_render(const lrtd::DevScene& S, const lrtd::TraceParams& P, const lrtd::Camera* d_cams, const lrtd::RenderArgs& A,
+                         const lrtd::RenderOut& O, uint32_t* d_work_counter, uint32_t* d_spill, uint32_t blocks, bool rgba,
+                         unsigned long long* d_counts, hipStream_t stream) {
+  hipError_t e = zero_u32(d_work_counter, 1, stream);
+  if (e != hipSuccess) return e;
+  if (d_counts) {
+    if (rgba) hipLaunchKernelGGL((lrtd::k_render<true, true>), dim3(blocks), dim3(256), 0, stream, S, P, d_cams, A, O, d_work_counter, d_spill, d_counts);
+    else hipLaunchKernelGGL((lrtd::k_render<true, false>), dim3(blocks), dim3(256), 0, stream, S, P, d_cams, A, O, d_work_counter, d_spill, d_counts);
+  } else {
+    if (rgba) hipLaunchKernelGGL((lrtd::k_render<false, true>), dim3(blocks), dim3(256), 0, stream, S, P, d_cams, A, O, d_work_counter, d_spill, d_counts);
+    else hipLaunchKernelGGL((lrtd::k_render<false, false>), dim3(blocks), dim3(256), 0, stream, S, P, d_cams, A, O, d_work_counter, d_spill, d_counts);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_cast_rays(const lrtd::DevScene& S, const lrtd::TraceParams& P, const float4* d_rays, uint32_t n, CRT_Hit* d_out,
+                            uint32_t* d_spill, uint32_t blocks, hipStream_t stream) {
+  hipLaunchKernelGGL(lrtd::k_cast_rays, dim3(blocks), dim3(256), 0, stream, S, P, d_rays, n, d_out, d_spill);
+  return hipGetLastError();
+}
+
+}  // namespace lrt

@@ -1,0 +1,36 @@
+// render.hpp - launch interface of the render kernels (host + device).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "trace_device.hpp"
+
+namespace lrtd {
+
+// LDS traversal-stack entries per lane (x 256 lanes x 4 B = 24 KiB per block);
+// deeper entries spill to global memory.
+constexpr uint32_t kStackLds = 24;
+
+struct RenderArgs {
+  uint32_t W, H;
+  uint32_t packed_w, n_packed;  // packed dims are rounded up to multiples of 8
+  uint32_t n_views;
+  uint32_t spp, ray_gen_mode, seed;
+};
+
+struct RenderOut {
+  float* t;        // [view][y][x] hit distance (1e9 on miss)
+  float2* n;       // octahedral-encoded normal (coords[2], coords[3])
+  uint32_t* prim;  // primId (~0 on miss)
+  float4* rgba;    // RenderFloat colour
+};
+
+}  // namespace lrtd
+
+namespace lrt {
+hipError_t zero_u32(uint32_t* d, uint32_t n, hipStream_t stream);
+hipError_t launch_render(const lrtd::DevScene& S, const lrtd::TraceParams& P, const lrtd::Camera* d_cams, const lrtd::RenderArgs& A,
+                         const lrtd::RenderOut& O, uint32_t* d_work_counter, uint32_t* d_spill, uint32_t blocks, bool rgba,
+                         unsigned long long* d_counts, hipStream_t stream);
+hipError_t launch_cast_rays(const lrtd::DevScene& S, const lrtd::TraceParams& P, const float4* d_rays, uint32_t n, CRT_Hit* d_out,
+                            uint32_t* d_spill, uint32_t blocks, hipStream_t stream);
+}  // namespace lrt

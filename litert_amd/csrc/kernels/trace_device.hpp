@@ -1,0 +1,475 @@
+// trace_device.hpp - device-side SBS ray casting for gfx950.
+//
+// The arithmetic of each function follows the reference function named in
+// its comment in the same operation order, and the library is compiled with
+// -ffp-contract=off, so FP32 results match a CPU FP32 evaluation of the same
+// steps bit for bit (division and square root are correctly rounded in HIP by
+// default). Layout-specific loads are the MI355X part: brick values are read
+// straight from the packed 8/16/32-bit words or the indexed float array.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../../include/lrt_types.h"
+
+namespace lrtd {
+
+struct v3 {
+  float x, y, z;
+  __device__ __forceinline__ v3() {}
+  __device__ __forceinline__ v3(float a, float b, float c) : x(a), y(b), z(c) {}
+};
+__device__ __forceinline__ v3 operator+(v3 a, v3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ v3 operator-(v3 a, v3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ v3 operator*(v3 a, v3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ v3 operator*(float s, v3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ v3 operator*(v3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float mn(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float mx(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ float sgn(float a) { return a > 0.f ? 1.f : (a < 0.f ? -1.f : 0.f); }
+__device__ __forceinline__ float clampf_(float u, float a, float b) { return mn(mx(u, a), b); }
+__device__ __forceinline__ float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ v3 normalize(v3 a) {
+  float l = sqrtf(dot(a, a));
+  return v3(a.x / l, a.y / l, a.z / l);
+}
+
+struct Mat4 { float m[16]; };  // column-major
+
+// Scene description passed by value to every kernel.
+struct DevScene {
+  const uint4* pairs;       // BVHNodePair as 4 x uint4 (boxMin|leftOffset, boxMax|escape) x 2
+  const uint4* nodes;       // SdfSBSNode
+  const uint32_t* values;   // packed DX words or indices (ID32F layouts)
+  const float* values_f;    // shared floats of indexed layouts
+  uint32_t brick_size, brick_pad, bytes_per_value, layout;
+  uint32_t v_size, quant_voxel_diag, type, n_nodes;
+};
+
+struct Hit {
+  float t;
+  uint32_t primId, instId, geomId;
+  float coords[4];
+};
+
+// ---- raytrace_common.h:59-82 ----------------------------------------------
+__device__ __forceinline__ v3 SafeInverse(v3 d) {
+  const float ooeps = 1.0e-36f;
+  v3 r;
+  r.x = 1.0f / (fabsf(d.x) > ooeps ? d.x : fabsf(ooeps) * sgn(d.x));
+  r.y = 1.0f / (fabsf(d.y) > ooeps ? d.y : fabsf(ooeps) * sgn(d.y));
+  r.z = 1.0f / (fabsf(d.z) > ooeps ? d.z : fabsf(ooeps) * sgn(d.z));
+  return r;
+}
+
+__device__ __forceinline__ float2 RayBox(v3 o, v3 inv, v3 bmin, v3 bmax) {
+  const float lo = inv.x * (bmin.x - o.x);
+  const float hi = inv.x * (bmax.x - o.x);
+  const float lo1 = inv.y * (bmin.y - o.y);
+  const float hi1 = inv.y * (bmax.y - o.y);
+  const float lo2 = inv.z * (bmin.z - o.z);
+  const float hi2 = inv.z * (bmax.z - o.z);
+  const float tmin = mx(mn(lo, hi), mn(lo1, hi1));
+  const float tmax = mn(mx(lo, hi), mx(lo1, hi1));
+  return make_float2(mx(tmin, mn(lo2, hi2)), mn(tmax, mx(lo2, hi2)));
+}
+
+// ---- BVH2Common.cpp:62-72 --------------------------------------------------
+__device__ __forceinline__ float trilinear(const float v[8], v3 dp) {
+  return (1 - dp.x) * (1 - dp.y) * (1 - dp.z) * v[0] + (1 - dp.x) * (1 - dp.y) * (dp.z) * v[1] +
+         (1 - dp.x) * (dp.y) * (1 - dp.z) * v[2] + (1 - dp.x) * (dp.y) * (dp.z) * v[3] +
+         (dp.x) * (1 - dp.y) * (1 - dp.z) * v[4] + (dp.x) * (1 - dp.y) * (dp.z) * v[5] +
+         (dp.x) * (dp.y) * (1 - dp.z) * v[6] + (dp.x) * (dp.y) * (dp.z) * v[7];
+}
+
+// ---- BVH2Common.cpp:657-687 ------------------------------------------------
+__device__ __forceinline__ v3 trilinear_diff(const float v[8], v3 dp) {
+  float ddx = -(1 - dp.y) * (1 - dp.z) * v[0] + -(1 - dp.y) * (dp.z) * v[1] + -(dp.y) * (1 - dp.z) * v[2] + -(dp.y) * (dp.z) * v[3] +
+              (1 - dp.y) * (1 - dp.z) * v[4] + (1 - dp.y) * (dp.z) * v[5] + (dp.y) * (1 - dp.z) * v[6] + (dp.y) * (dp.z) * v[7];
+  float ddy = -(1 - dp.x) * (1 - dp.z) * v[0] + -(1 - dp.x) * (dp.z) * v[1] + (1 - dp.x) * (1 - dp.z) * v[2] + (1 - dp.x) * (dp.z) * v[3] +
+              -(dp.x) * (1 - dp.z) * v[4] + -(dp.x) * (dp.z) * v[5] + (dp.x) * (1 - dp.z) * v[6] + (dp.x) * (dp.z) * v[7];
+  float ddz = -(1 - dp.x) * (1 - dp.y) * v[0] + (1 - dp.x) * (1 - dp.y) * v[1] + -(1 - dp.x) * (dp.y) * v[2] + (1 - dp.x) * (dp.y) * v[3] +
+              -(dp.x) * (1 - dp.y) * v[4] + (dp.x) * (1 - dp.y) * v[5] + -(dp.x) * (dp.y) * v[6] + (dp.x) * (dp.y) * v[7];
+  return v3(ddx, ddy, ddz);
+}
+
+// ---- BVH2Common.cpp:30-35 --------------------------------------------------
+__device__ __forceinline__ float2 encode_normal(v3 v) {
+  float s = 1.0f / (fabsf(v.x) + fabsf(v.y) + fabsf(v.z));
+  float px = v.x * s, py = v.y * s;
+  float snx = (px >= 0.0f) ? 1.0f : -1.0f, sny = (py >= 0.0f) ? 1.0f : -1.0f;
+  if (v.z <= 0.0f) return make_float2((1.0f - fabsf(py)) * snx, (1.0f - fabsf(px)) * sny);
+  return make_float2(px, py);
+}
+
+// ---- eye_ray.cpp:22-32 -----------------------------------------------------
+__device__ __forceinline__ v3 decode_normal(float ex, float ey) {
+  v3 v(ex, ey, 1.0f - fabsf(ex) - fabsf(ey));
+  if (v.z < 0) {
+    float vx = v.x;
+    v.x = (1.0f - fabsf(v.y)) * ((v.x >= 0.0f) ? 1.0f : -1.0f);
+    v.y = (1.0f - fabsf(vx)) * ((v.y >= 0.0f) ? 1.0f : -1.0f);
+  }
+  return normalize(v);
+}
+
+__device__ __forceinline__ bool is_indexed(const DevScene& S) {
+  return S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F || S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F_IN;
+}
+
+// ---- BVH2Common.cpp:725-762, trilinear branch -----------------------------
+__device__ __forceinline__ float load_voxel(const DevScene& S, uint32_t v_off, uint32_t vx, uint32_t vy, uint32_t vz, float sz_inv,
+                                            float values[8]) {
+  float vmin = 1e6f;
+  const uint32_t vs = S.v_size, pad = S.brick_pad;
+  if (is_indexed(S)) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      uint32_t vId = (vx + ((i & 4) >> 2) + pad) * vs * vs + (vy + ((i & 2) >> 1) + pad) * vs + (vz + (i & 1) + pad);
+      values[i] = S.values_f[S.values[v_off + vId]];
+      vmin = mn(vmin, values[i]);
+    }
+  } else {
+    const uint32_t bpv = S.bytes_per_value;
+    const uint32_t vals_per_int = 4 / bpv;
+    const uint32_t bits = 8 * bpv;
+    const uint32_t max_val = bpv == 4 ? 0xFFFFFFFFu : ((1u << bits) - 1);
+    const float d_max = S.quant_voxel_diag ? 1.73205081f * (sz_inv / (float)S.brick_size) : 1.73205081f * sz_inv;
+    const float mult = 2.0f * d_max / (float)max_val;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      uint32_t vId = (vx + ((i & 4) >> 2) + pad) * vs * vs + (vy + ((i & 2) >> 1) + pad) * vs + (vz + (i & 1) + pad);
+      uint32_t q = (S.values[v_off + vId / vals_per_int] >> (bits * (vId % vals_per_int))) & max_val;
+      values[i] = -d_max + mult * (float)q;
+      vmin = mn(vmin, values[i]);
+    }
+  }
+  return vmin;
+}
+
+struct TraceParams {
+  uint32_t sdf_node_intersect;
+  uint32_t representation_mode;
+  uint32_t render_mode;
+  uint32_t normal_mode;
+  uint32_t need_normal;
+};
+
+// ---- BVH2Common.cpp:274-631 ------------------------------------------------
+// Returns true when *pHit was updated. d_inv = 1/d is passed in (same value
+// the reference computes per call).
+__device__ __forceinline__ bool LocalSurfaceIntersection(const TraceParams& P, v3 ray_dir, const float values[8], uint32_t primId,
+                                                         uint32_t geomType, float d, float d_inv, float qFar, float fNearX,
+                                                         v3 start_q, Hit* pHit) {
+  const float EPS = 1e-6f;
+  const unsigned mode = P.sdf_node_intersect;
+  float t = 0.0f;
+  bool hit = false;
+  unsigned iter = 0;
+  float start_sign = 1;
+  float start_dist = trilinear(values, start_q + t * ray_dir);
+  if (P.representation_mode == LRT_REPRESENTATION_MODE_SURFACE) start_sign = sgn(start_dist);
+  start_dist *= start_sign;
+
+  if (start_dist <= EPS || mode == LRT_SDF_NODE_INTERSECT_BBOX) {
+    hit = true;
+  } else if (mode == LRT_SDF_NODE_INTERSECT_ST) {
+    float dist = start_dist;
+    while (t < qFar && dist > EPS && iter < 256u) {
+      t += dist * d_inv;
+      dist = start_sign * trilinear(values, start_q + t * ray_dir);
+      iter++;
+    }
+    hit = (dist <= EPS);
+  } else {
+    float s000 = values[0] * d_inv, s001 = values[1] * d_inv, s010 = values[2] * d_inv, s011 = values[3] * d_inv;
+    float s100 = values[4] * d_inv, s101 = values[5] * d_inv, s110 = values[6] * d_inv, s111 = values[7] * d_inv;
+    float a = s101 - s001;
+    float k0 = s000;
+    float k1 = s100 - s000;
+    float k2 = s010 - s000;
+    float k3 = s110 - s010 - k1;
+    float k4 = k0 - s001;
+    float k5 = k1 - a;
+    float k6 = k2 - (s011 - s001);
+    float k7 = k3 - (s111 - s011 - a);
+    v3 o = start_q, d3 = ray_dir;
+    float m0 = o.x * o.y;
+    float m1 = d3.x * d3.y;
+    float m2 = o.x * d3.y + o.y * d3.x;
+    float m3 = k5 * o.z - k1;
+    float m4 = k6 * o.z - k2;
+    float m5 = k7 * o.z - k3;
+    float c0 = (k4 * o.z - k0) + o.x * m3 + o.y * m4 + m0 * m5;
+    float c1 = d3.x * m3 + d3.y * m4 + m2 * m5 + d3.z * (k4 + k5 * o.x + k6 * o.y + k7 * m0);
+    float c2 = m1 * m5 + d3.z * (k5 * d3.x + k6 * d3.y + k7 * m2);
+    float c3 = k7 * m1 * d3.z;
+
+    if (mode == LRT_SDF_NODE_INTERSECT_NEWTON) {
+      float qa = 3 * c3, qb = 2 * c2, qc = c1;
+      float t0 = 0, t1 = qFar, t2 = qFar, t3 = qFar;
+      float D = qb * qb - 4 * qa * qc;
+      if (D >= 0) {
+        float q = -0.5f * (qb + sgn(qb) * sqrtf(D));
+        t1 = fabsf(qa) > EPS ? q / qa : t0;
+        t2 = fabsf(q) > EPS ? qc / q : qFar;
+        float tmp = mn(t1, t2);
+        t2 = mx(t1, t2);
+        t1 = tmp;
+        t1 = clampf_(t1, t0, t3);
+        t2 = clampf_(t2, t0, t3);
+      }
+      bool s0 = c0 > 0;
+      bool s1 = (c0 + t1 * (c1 + t1 * (c2 + t1 * c3))) > 0;
+      bool s2 = (c0 + t2 * (c1 + t2 * (c2 + t2 * c3))) > 0;
+      bool s3 = (c0 + t3 * (c1 + t3 * (c2 + t3 * c3))) > 0;
+      float nwt_min = t0, nwt_max = t0;
+      if (s0 != s1) { nwt_min = t0; nwt_max = t1; }
+      else if (s1 != s2) { nwt_min = t1; nwt_max = t2; }
+      else if (s2 != s3) { nwt_min = t2; nwt_max = t3; }
+      if (nwt_min < nwt_max) {
+        unsigned it = 0;
+        float rtn = 0.5f * (nwt_min + nwt_max);
+        float f = 1000;
+        while (it < 10u && fabsf(f) >= EPS) {
+          f = c0 + rtn * (c1 + rtn * (c2 + rtn * c3));
+          float df = c1 + rtn * (2 * c2 + rtn * 3 * c3);
+          float dx = f / (df + sgn(df) * 1e-9f);
+          rtn -= dx;
+          it++;
+        }
+        t = rtn;
+        hit = (t >= 0 && t <= qFar && fabsf(f) < EPS);
+      } else {
+        hit = false;
+      }
+    } else if (mode == LRT_SDF_NODE_INTERSECT_ANALYTIC) {
+      float x1 = 1000, x2 = 1000, x3 = 1000;
+      if ((double)fabsf(c3) > 1e-2) {
+        float A_ = c2 / c3, B_ = c1 / c3, C_ = c0 / c3;
+        float Q = (A_ * A_ - 3 * B_) / 9;
+        float R = (2 * A_ * A_ - 9 * A_ * B_ + 27 * C_) / 54;
+        float Q3 = Q * Q * Q;
+        if (R * R < Q3) {
+          float theta = acosf(R / sqrtf(Q3));
+          x1 = -2 * sqrtf(Q) * cosf(theta / 3) - A_ / 3;
+          x2 = (float)(-2 * sqrtf(Q) * cos(((double)theta + 2 * M_PI) / 3) - A_ / 3);
+          x3 = (float)(-2 * sqrtf(Q) * cos(((double)theta - 2 * M_PI) / 3) - A_ / 3);
+        } else {
+          float Am = -sgn(R) * powf(fabsf(R) + sqrtf(R * R - Q3), 1.0f / 3.0f);
+          float Bm = fabsf(Am) > EPS ? Q / Am : 0;
+          x1 = Am + Bm - A_ / 3;
+        }
+      } else if ((double)fabsf(c2) > 1e-4) {
+        float D = c1 * c1 - 4 * c2 * c0;
+        if (D > 0) {
+          float q = -0.5f * (c1 + sgn(c1) * sqrtf(D));
+          x1 = q / c2;
+          if (fabsf(q) > EPS) x2 = c0 / q;
+        }
+      } else if (fabsf(c1) > EPS) {
+        x1 = -c0 / c1;
+      }
+      x1 = x1 < 0 ? 1000 : x1;
+      x2 = x2 < 0 ? 1000 : x2;
+      x3 = x3 < 0 ? 1000 : x3;
+      t = mn(x1, mn(x2, x3));
+      hit = (t >= 0 && t <= qFar);
+    } else {  // IT
+      const float k = 2;
+      float e = 0.1f * qFar;
+      float t_max = fabsf(c3) < EPS ? 1e6f : -c2 / (3 * c3);
+      float df_max = 3 * c3 * t_max * t_max + 2 * c2 * t_max + c1;
+      float dist = start_dist;
+      while (t < qFar && dist > EPS && iter < 256u) {
+        float df_1 = 3 * c3 * t * t + 2 * c2 * t + c1;
+        float df_2 = 3 * c3 * (t + e) * (t + e) + 2 * c2 * (t + e) + c1;
+        float L = (t_max > t && t_max < t + e) ? mx(df_max, mx(df_1, df_2)) : mx(df_1, df_2);
+        L = mx(L, EPS);
+        float s = mn((dist * d_inv) / L, e);
+        t += s;
+        e = k * s;
+        dist = start_sign * trilinear(values, start_q + t * ray_dir);
+        iter++;
+      }
+      hit = (dist <= EPS);
+    }
+  }
+
+  float tReal = fNearX + d * t;
+  if (t <= qFar && hit && tReal < pHit->t) {
+    v3 norm(0, 0, 1);
+    if (P.need_normal) {
+      v3 p0 = start_q + t * ray_dir;
+      const float h = 0.001f;
+      float ddx = (trilinear(values, p0 + v3(h, 0, 0)) - trilinear(values, p0 + v3(-h, 0, 0))) / (2 * h);
+      float ddy = (trilinear(values, p0 + v3(0, h, 0)) - trilinear(values, p0 + v3(0, -h, 0))) / (2 * h);
+      float ddz = (trilinear(values, p0 + v3(0, 0, h)) - trilinear(values, p0 + v3(0, 0, -h))) / (2 * h);
+      norm = start_sign * normalize(v3(ddx, ddy, ddz));
+    }
+    float2 en = encode_normal(norm);
+    pHit->t = tReal;
+    pHit->primId = (P.render_mode == LRT_RENDER_MODE_ST_ITERATIONS) ? iter : primId;
+    pHit->instId = 0;
+    pHit->geomId = 0u | (geomType << LRT_SH_TYPE);
+    pHit->coords[0] = 0;
+    pHit->coords[1] = 0;
+    pHit->coords[2] = en.x;
+    pHit->coords[3] = en.y;
+    return true;
+  }
+  return false;
+}
+
+__device__ __forceinline__ float step01(float e0, float e1, float x) { return clampf_((x - e0) / (e1 - e0), 0.0f, 1.0f); }
+
+// ---- BVH2Common.cpp:764-984 ------------------------------------------------
+// rayDirInv is SafeInverse(ray_dir), evaluated once per ray by the caller
+// (the reference recomputes the identical value for every box test).
+// Counters (optional) receive voxel and root-find counts.
+__device__ __forceinline__ void BrickIntersect(const DevScene& S, const TraceParams& P, v3 ray_pos, v3 ray_dir, v3 rayDirInv,
+                                               float tNear, uint32_t brick, Hit* pHit, uint32_t* n_vox) {
+  float values[8];
+  const uint4 N = S.nodes[brick];
+  const float px = (float)(N.x >> 16), py = (float)(N.x & 0xFFFFu), pz = (float)(N.y >> 16), sz = (float)(N.y & 0xFFFFu);
+  const float bsf = (float)S.brick_size;
+  const float sz_inv = 2.0f / sz;
+  const float d = 2.0f / (sz * bsf);
+  const float d_inv = 1.0f / d;
+  const v3 bmin = v3(-1, -1, -1) + sz_inv * v3(px, py, pz);
+  const v3 bmax = bmin + sz_inv * v3(1, 1, 1);
+  const float qscale = 0.5f * sz * bsf;
+  const float hi_clamp = bsf - 1e-6f;
+  float2 bnf = RayBox(ray_pos, rayDirInv, bmin, bmax);
+  const float old_t = pHit->t;
+  // A ray crosses at most 3*brick_size+1 voxels of a brick; the cap only
+  // guarantees termination where the 1e-6 step could vanish in rounding.
+  const uint32_t max_steps = 16u * (S.brick_size + 1u) + 16u;
+  uint32_t steps = 0;
+  while (bnf.x < bnf.y && pHit->t == old_t && steps++ < max_steps) {
+    v3 hit_pos = ray_pos + bnf.x * ray_dir;
+    v3 local = (hit_pos - bmin) * qscale;
+    v3 vp(floorf(clampf_(local.x, 1e-6f, hi_clamp)), floorf(clampf_(local.y, 1e-6f, hi_clamp)), floorf(clampf_(local.z, 1e-6f, hi_clamp)));
+    v3 min_pos = bmin + d * vp;
+    v3 max_pos = min_pos + d * v3(1, 1, 1);
+    if (n_vox) (*n_vox)++;
+    float vmin = load_voxel(S, N.z, (uint32_t)vp.x, (uint32_t)vp.y, (uint32_t)vp.z, sz_inv, values);
+    float2 fnf = RayBox(ray_pos, rayDirInv, min_pos, max_pos);
+    if (tNear < fnf.x && vmin <= 0.0f) {
+      v3 start_pos = ray_pos + fnf.x * ray_dir;
+      v3 start_q = (start_pos - min_pos) * qscale;
+      float qFar = (fnf.y - fnf.x) * qscale;
+      bool updated = LocalSurfaceIntersection(P, ray_dir, values, brick, S.type, d, d_inv, qFar, fnf.x, start_q, pHit);
+      if (updated && S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F_IN && P.normal_mode == LRT_NORMAL_MODE_SDF_SMOOTHED && P.need_normal) {
+        // smoothed normal, BVH2Common.cpp:838-906
+        const float beta = 0.5f;
+        v3 dp = start_q + ((pHit->t - fnf.x) / d) * ray_dir;
+        v3 nmq;
+        nmq.x = dp.x < 0.5f ? step01(-beta, beta, dp.x) : step01(-beta, beta, dp.x - 1.0f);
+        nmq.y = dp.y < 0.5f ? step01(-beta, beta, dp.y) : step01(-beta, beta, dp.y - 1.0f);
+        nmq.z = dp.z < 0.5f ? step01(-beta, beta, dp.z) : step01(-beta, beta, dp.z - 1.0f);
+        const int bs = (int)S.brick_size;
+        const int vpi[3] = {(int)vp.x, (int)vp.y, (int)vp.z};
+        const int vp0[3] = {dp.x < 0.5f ? vpi[0] - 1 : vpi[0], dp.y < 0.5f ? vpi[1] - 1 : vpi[1], dp.z < 0.5f ? vpi[2] - 1 : vpi[2]};
+        v3 sm(0, 0, 0);
+        for (int i = 0; i < 8; i++) {
+          int VI[3] = {vp0[0] + ((i & 4) >> 2), vp0[1] + ((i & 2) >> 1), vp0[2] + (i & 1)};
+          int BP[3];
+          for (int a = 0; a < 3; a++) BP[a] = VI[a] >= 0 ? (VI[a] < bs ? 0 : 1) : -1;
+          uint32_t nid = 9 * (BP[0] + 1) + 3 * (BP[1] + 1) + (BP[2] + 1);
+          v3 dvp = v3((float)VI[0], (float)VI[1], (float)VI[2]) - vp;
+          uint32_t nb = brick;
+          uint32_t nv[3];
+          if (nid == 13) {
+            for (int a = 0; a < 3; a++) nv[a] = (uint32_t)VI[a];
+          } else {
+            nb = S.values[N.z + S.v_size * S.v_size * S.v_size + 8 + nid];
+            for (int a = 0; a < 3; a++) nv[a] = (uint32_t)(VI[a] - bs * BP[a]);
+          }
+          v3 ni(0, 0, 0);
+          if (nb != LRT_INVALID_IDX) {
+            float vn[8];
+            load_voxel(S, S.nodes[nb].z, nv[0], nv[1], nv[2], sz_inv, vn);
+            ni = normalize(trilinear_diff(vn, dp - dvp));
+          }
+          float w = ((i & 4) ? (nmq.x) : (1 - nmq.x)) * ((i & 2) ? (nmq.y) : (1 - nmq.y)) * ((i & 1) ? (nmq.z) : (1 - nmq.z));
+          sm = (i == 0) ? w * ni : sm + w * ni;
+        }
+        sm = normalize(sm);
+        float2 en = encode_normal(sm);
+        pHit->coords[2] = en.x;
+        pHit->coords[3] = en.y;
+      }
+    }
+    bnf.x += mx(0.0f, fnf.y - bnf.x) + 1e-6f;
+  }
+  if (pHit->t < old_t && is_indexed(S)) {  // colour, BVH2Common.cpp:962-982
+    v3 pos = ray_pos + pHit->t * ray_dir;
+    v3 dp = (pos - bmin) * (0.5f * sz);
+    uint32_t t_off = N.z + S.v_size * S.v_size * S.v_size;
+    v3 col(0, 0, 0);
+    for (int i = 0; i < 8; i++) {
+      float w = ((i & 4) ? dp.x : 1 - dp.x) * ((i & 2) ? dp.y : 1 - dp.y) * ((i & 1) ? dp.z : 1 - dp.z);
+      uint32_t ci = S.values[t_off + i];
+      col = col + w * v3(S.values_f[ci], S.values_f[ci + 1], S.values_f[ci + 2]);
+    }
+    col = v3(clampf_(floorf(255.0f * col.x + 0.5f), 0.0f, 255.0f), clampf_(floorf(255.0f * col.y + 0.5f), 0.0f, 255.0f),
+             clampf_(floorf(255.0f * col.z + 0.5f), 0.0f, 255.0f));
+    pHit->coords[0] = col.x + col.y / 256.0f;
+    pHit->coords[1] = col.z;
+  }
+}
+
+// ---- camera, eye_ray.cpp:146-160, render_common.h:25-55 -------------------
+__device__ __forceinline__ void mul4(const float* m, float x, float y, float z, float w, float* r) {
+  r[0] = m[0] * x + m[4] * y + m[8] * z + m[12] * w;
+  r[1] = m[1] * x + m[5] * y + m[9] * z + m[13] * w;
+  r[2] = m[2] * x + m[6] * y + m[10] * z + m[14] * w;
+  r[3] = m[3] * x + m[7] * y + m[11] * z + m[15] * w;
+}
+
+struct Camera {
+  float projInv[16];
+  float worldViewInv[16];
+};
+
+__device__ __forceinline__ void eye_ray(const Camera& C, uint32_t x, uint32_t y, float dx, float dy, uint32_t W, uint32_t H, v3& pos,
+                                        v3& dir) {
+  float fx = ((float)x + dx) / (float)W;
+  float fy = ((float)y + dy) / (float)H;
+  float p[4];
+  mul4(C.projInv, 2.0f * fx - 1.0f, -2.0f * fy + 1.0f, 1.0f, 1.0f, p);
+  v3 rd = normalize(v3(p[0] / p[3], p[1] / p[3], p[2] / p[3]));
+  float a[4], b[4];
+  mul4(C.worldViewInv, 0.0f, 0.0f, 0.0f, 1.0f, a);
+  v3 q = v3(0, 0, 0) + 100.0f * rd;
+  mul4(C.worldViewInv, q.x, q.y, q.z, 1.0f, b);
+  pos = v3(a[0], a[1], a[2]);
+  dir = normalize(v3(b[0], b[1], b[2]) - pos);
+}
+
+// ---- PCG3D jitter, eye_ray.cpp:53-81 ----------------------------------------
+__device__ __forceinline__ float2 rand2(uint32_t x, uint32_t y, uint32_t iter, uint32_t seed) {
+  x = x + 1233u * (iter + seed) % 171u;
+  y = y + 453u * (iter + seed) % 765u;
+  uint32_t a = x, b = y, c = x ^ y;
+  a = a * 1664525u + 1013904223u;
+  b = b * 1664525u + 1013904223u;
+  c = c * 1664525u + 1013904223u;
+  a += b * c; b += c * a; c += a * b;
+  a ^= a >> 16u; b ^= b >> 16u; c ^= c >> 16u;
+  a += b * c; b += c * a; c += a * b;
+  const float s = 1.0f / 4294967295.0f;
+  return make_float2((float)a * s, (float)b * s);
+}
+
+// inverse of SuperBlockIndex2DOpt (eye_ray.cpp:537-555): packed index -> pixel
+__device__ __forceinline__ void tid_to_xy(uint32_t tid, uint32_t packed_w, uint32_t& x, uint32_t& y) {
+  const uint32_t sb = tid >> 6, r = tid & 63u;
+  const uint32_t wbh = packed_w >> 3;
+  const uint32_t bhx = sb % wbh, bhy = sb / wbh;
+  const uint32_t lh = r >> 4, l = r & 15u;
+  x = ((bhx * 2 + (lh & 1u)) << 2) + (l & 3u);
+  y = ((bhy * 2 + (lh >> 1)) << 2) + (l >> 2);
+}
+
+}  // namespace lrtd

@@ -1,0 +1,567 @@
+// lrt_capi.hip - C ABI implementation: scene ingest, device buffers, launches.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/lrt.h"
+#include "host/bvh_build.hpp"
+#include "host/hmath.hpp"
+#include "host/scene_gen.hpp"
+#include "kernels/render.hpp"
+#include "lrt_internal.hpp"
+
+namespace lrt {
+
+static thread_local std::string g_err;
+int set_err(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+int hip_err(hipError_t e, const char* what) { return set_err(LRT_E_HIP, std::string(what) + ": " + hipGetErrorString(e)); }
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+hipError_t DevBuf::ensure(size_t n) {
+  if (n <= bytes && p) return hipSuccess;
+  release();
+  hipError_t e = hipMalloc(&p, n ? n : 16);
+  if (e == hipSuccess) bytes = n ? n : 16;
+  return e;
+}
+hipError_t DevBuf::upload(const void* src, size_t n, hipStream_t s) {
+  hipError_t e = ensure(n);
+  if (e != hipSuccess) return e;
+  if (n) e = hipMemcpyAsync(p, src, n, hipMemcpyHostToDevice, s);
+  return e;
+}
+
+void inverse4x4(const float in[16], float out[16]) {
+  double m[16], inv[16];
+  for (int i = 0; i < 16; i++) m[i] = in[i];
+  inv[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] + m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+  inv[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] - m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+  inv[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] + m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+  inv[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] - m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+  inv[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] - m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+  inv[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] + m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+  inv[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] - m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+  inv[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] + m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+  inv[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] + m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+  inv[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] - m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+  inv[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] + m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+  inv[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] - m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+  inv[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] - m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+  inv[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] + m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+  inv[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] - m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+  inv[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] + m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+  double det = m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12];
+  double idet = (det != 0.0) ? 1.0 / det : 0.0;
+  for (int i = 0; i < 16; i++) out[i] = (float)(inv[i] * idet);
+}
+
+lrtd::Camera make_camera(const float wv[16], const float pr[16]) {
+  lrtd::Camera c;
+  inverse4x4(pr, c.projInv);
+  inverse4x4(wv, c.worldViewInv);
+  return c;
+}
+
+lrtd::DevScene Renderer::dev_scene() const {
+  lrtd::DevScene S;
+  S.pairs = (const uint4*)d_pairs.p;
+  S.nodes = (const uint4*)d_nodes.p;
+  S.values = (const uint32_t*)d_values.p;
+  S.values_f = (const float*)d_values_f.p;
+  S.brick_size = header.brick_size;
+  S.brick_pad = header.brick_pad;
+  S.bytes_per_value = header.bytes_per_value;
+  S.layout = header.aux_data & LRT_SBS_LAYOUT_MASK;
+  S.v_size = header.brick_size + 2 * header.brick_pad + 1;
+  S.quant_voxel_diag = quant_voxel_diag;
+  S.type = geom_type;
+  S.n_nodes = (uint32_t)nodes.size();
+  return S;
+}
+
+lrtd::TraceParams Renderer::trace_params(bool force_normal) const {
+  lrtd::TraceParams P;
+  P.sdf_node_intersect = preset.sdf_node_intersect;
+  P.representation_mode = preset.representation_mode;
+  P.render_mode = preset.render_mode;
+  P.normal_mode = preset.normal_mode;
+  const uint32_t m = preset.render_mode;
+  P.need_normal = (force_normal || m == LRT_RENDER_MODE_LAMBERT_NO_TEX || m == LRT_RENDER_MODE_NORMAL || m == LRT_RENDER_MODE_PHONG_NO_TEX ||
+                   m == LRT_RENDER_MODE_LAMBERT || m == LRT_RENDER_MODE_PHONG)
+                      ? 1u
+                      : 0u;
+  return P;
+}
+
+uint32_t Renderer::grid_blocks() const {
+  if (launch_blocks) return launch_blocks;
+  return (uint32_t)std::max(1, cu_count) * 8u;
+}
+
+hipError_t Renderer::ensure_spill(uint32_t blocks) {
+  uint32_t extra = (uint32_t)std::max(0, pair_depth + 2 - (int)lrtd::kStackLds);
+  size_t n = (size_t)blocks * 256 * std::max<uint32_t>(extra, 1u);
+  return d_spill.ensure(n * sizeof(uint32_t));
+}
+
+static int check_ready(Renderer* r) {
+  if (!r) return set_err(LRT_E_INVALID, "null handle");
+  if (!r->has_scene) return set_err(LRT_E_NOSCENE, "no scene: call lrt_set_scene_sbs first");
+  return LRT_OK;
+}
+
+// Launches n_views cameras at W x H into the handle's output buffers.
+int Renderer::render_device(const lrtd::Camera* cams, uint32_t n_views, uint32_t W, uint32_t H, bool rgba, bool gbuf, bool force_normal,
+                            const char* timer_name, unsigned long long* d_counts) {
+  if (W == 0 || H == 0 || W > 65535 || H > 65535 || n_views == 0) return set_err(LRT_E_INVALID, "bad image size");
+  if (preset.spp == 0) return set_err(LRT_E_INVALID, "spp must be >= 1");
+  if (preset.interpolation_mode != LRT_INTERPOLATION_MODE_TRILINEAR)
+    return set_err(LRT_E_UNSUPPORTED, "only trilinear interpolation is supported");
+  hipError_t e;
+  if ((e = hipSetDevice(device)) != hipSuccess) return hip_err(e, "hipSetDevice");
+  const size_t npix = (size_t)n_views * W * H;
+  if ((e = d_cams.upload(cams, sizeof(lrtd::Camera) * n_views, stream)) != hipSuccess) return hip_err(e, "upload cameras");
+  if ((e = d_counter.ensure(64)) != hipSuccess) return hip_err(e, "alloc counter");
+  lrtd::RenderOut O{};
+  if (gbuf) {
+    if ((e = d_out_t.ensure(npix * 4)) != hipSuccess) return hip_err(e, "alloc t");
+    if ((e = d_out_n.ensure(npix * 8)) != hipSuccess) return hip_err(e, "alloc n");
+    if ((e = d_out_prim.ensure(npix * 4)) != hipSuccess) return hip_err(e, "alloc prim");
+    O.t = (float*)d_out_t.p;
+    O.n = (float2*)d_out_n.p;
+    O.prim = (uint32_t*)d_out_prim.p;
+  }
+  if (rgba) {
+    if ((e = d_out_rgba.ensure(npix * 16)) != hipSuccess) return hip_err(e, "alloc rgba");
+    O.rgba = (float4*)d_out_rgba.p;
+  }
+  lrtd::RenderArgs A;
+  A.W = W;
+  A.H = H;
+  A.packed_w = 8 * ((W + 7) / 8);
+  A.n_packed = A.packed_w * (8 * ((H + 7) / 8));
+  A.n_views = n_views;
+  A.spp = preset.spp;
+  A.ray_gen_mode = preset.ray_gen_mode;
+  A.seed = seed;
+  const uint32_t blocks = grid_blocks();
+  if ((e = ensure_spill(blocks)) != hipSuccess) return hip_err(e, "alloc spill");
+  if ((e = hipEventRecord(ev0, stream)) != hipSuccess) return hip_err(e, "event");
+  e = launch_render(dev_scene(), trace_params(force_normal), (const lrtd::Camera*)d_cams.p, A, O, (uint32_t*)d_counter.p,
+                    (uint32_t*)d_spill.p, blocks, rgba, d_counts, stream);
+  if (e != hipSuccess) return hip_err(e, "render launch");
+  if ((e = hipEventRecord(ev1, stream)) != hipSuccess) return hip_err(e, "event");
+  last_w = W;
+  last_h = H;
+  last_views = n_views;
+  pending_timer = timer_name ? timer_name : "";
+  return LRT_OK;
+}
+
+int Renderer::finish_timer() {
+  hipError_t e = hipEventSynchronize(ev1);
+  if (e != hipSuccess) return hip_err(e, "kernel execution");
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess && !pending_timer.empty()) times[pending_timer] = ms;
+  last_ms = ms;
+  return LRT_OK;
+}
+
+}  // namespace lrt
+
+using namespace lrt;
+
+struct lrt_sbs {
+  SbsScene s;
+};
+
+extern "C" {
+
+const char* lrt_last_error(void) { return g_err.c_str(); }
+
+int lrt_create(uint32_t device, uint32_t max_primitives, lrt_handle* out) {
+  (void)max_primitives;
+  if (!out) return set_err(LRT_E_INVALID, "null out");
+  if (device != LRT_DEVICE_GPU && device != LRT_DEVICE_GPU_RTX)
+    return set_err(LRT_E_UNSUPPORTED, "this library renders on the GPU only (DEVICE_GPU)");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(LRT_E_NODEVICE, "no HIP device visible");
+  std::unique_ptr<lrt_renderer> r(new lrt_renderer);
+  hipError_t e;
+  if ((e = hipGetDevice(&r->device)) != hipSuccess) return hip_err(e, "hipGetDevice");
+  hipDeviceProp_t prop;
+  if ((e = hipGetDeviceProperties(&prop, r->device)) != hipSuccess) return hip_err(e, "hipGetDeviceProperties");
+  r->cu_count = prop.multiProcessorCount;
+  if ((e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking)) != hipSuccess) return hip_err(e, "hipStreamCreate");
+  if ((e = hipEventCreate(&r->ev0)) != hipSuccess) return hip_err(e, "hipEventCreate");
+  if ((e = hipEventCreate(&r->ev1)) != hipSuccess) return hip_err(e, "hipEventCreate");
+  r->preset = MultiRenderPreset{LRT_RENDER_MODE_LAMBERT_NO_TEX, LRT_SDF_NODE_INTERSECT_ST, LRT_NORMAL_MODE_GEOMETRY,
+                                LRT_RAY_GEN_MODE_REGULAR, LRT_INTERPOLATION_MODE_TRILINEAR, LRT_REPRESENTATION_MODE_VOLUME, 1};
+  *out = r.release();
+  return LRT_OK;
+}
+
+int lrt_destroy(lrt_handle h) {
+  if (!h) return LRT_OK;
+  lrt_renderer* r = h;
+  (void)hipSetDevice(r->device);
+  if (r->stream) (void)hipStreamSynchronize(r->stream);
+  r->release_all();
+  if (r->ev0) (void)hipEventDestroy(r->ev0);
+  if (r->ev1) (void)hipEventDestroy(r->ev1);
+  if (r->stream) (void)hipStreamDestroy(r->stream);
+  delete r;
+  return LRT_OK;
+}
+
+int lrt_set_scene_sbs(lrt_handle h, const SdfSBSHeader* header, const SdfSBSNode* nodes, uint32_t n_nodes, const uint32_t* values,
+                      uint64_t n_values, const float* values_f, uint64_t n_values_f) {
+  if (!h || !header || !nodes || !values) return set_err(LRT_E_INVALID, "null argument");
+  if (n_nodes == 0 || n_values == 0) return set_err(LRT_E_INVALID, "empty scene (reference asserts size > 0, BVH2Common_host.cpp:569)");
+  if (n_nodes >= (1u << 24)) return set_err(LRT_E_INVALID, "too many bricks for the 24-bit leaf index");
+  const uint32_t layout = header->aux_data & LRT_SBS_LAYOUT_MASK;
+  const bool indexed = layout == LRT_SBS_LAYOUT_ID32F_IRGB32F || layout == LRT_SBS_LAYOUT_ID32F_IRGB32F_IN;
+  if (header->brick_size < 1 || header->brick_size > 16 || header->brick_pad > 1)
+    return set_err(LRT_E_INVALID, "brick_size must be 1..16 and brick_pad 0..1");
+  if (!indexed && layout != LRT_SBS_LAYOUT_DX && layout != LRT_SBS_LAYOUT_UNDEFINED && layout != LRT_SBS_LAYOUT_DX_RGB8 &&
+      layout != LRT_SBS_LAYOUT_DX_UV16)
+    return set_err(LRT_E_UNSUPPORTED, "unsupported node layout");
+  if (!indexed && header->bytes_per_value != 1 && header->bytes_per_value != 2 && header->bytes_per_value != 4)
+    return set_err(LRT_E_INVALID, "bytes_per_value must be 1, 2 or 4");
+  if (indexed && (!values_f || n_values_f == 0)) return set_err(LRT_E_INVALID, "indexed layout needs values_f");
+  // bounds: every brick's words must lie inside `values`
+  const uint32_t v_size = header->brick_size + 2 * header->brick_pad + 1;
+  const uint64_t nv = (uint64_t)v_size * v_size * v_size;
+  uint64_t words = indexed ? nv + 8 + (layout == LRT_SBS_LAYOUT_ID32F_IRGB32F_IN ? 27 : 0)
+                           : (nv + (4 / header->bytes_per_value) - 1) / (4 / header->bytes_per_value);
+  for (uint32_t i = 0; i < n_nodes; i++) {
+    if ((uint64_t)nodes[i].data_offset + words > n_values) return set_err(LRT_E_INVALID, "brick data_offset out of range");
+    uint32_t sz = nodes[i].pos_z_lod_size & 0xFFFF;
+    if (sz == 0 || (nodes[i].pos_xy >> 16) >= sz || (nodes[i].pos_xy & 0xFFFF) >= sz || (nodes[i].pos_z_lod_size >> 16) >= sz)
+      return set_err(LRT_E_INVALID, "brick position outside its LOD");
+  }
+  if (indexed) {
+    for (uint32_t i = 0; i < n_nodes; i++) {
+      const uint32_t* w = values + nodes[i].data_offset;
+      for (uint64_t k = 0; k < nv; k++)
+        if (w[k] >= n_values_f) return set_err(LRT_E_INVALID, "distance index out of range");
+      for (int k = 0; k < 8; k++)
+        if ((uint64_t)w[nv + k] + 3 > n_values_f) return set_err(LRT_E_INVALID, "colour index out of range");
+      if (layout == LRT_SBS_LAYOUT_ID32F_IRGB32F_IN)
+        for (int k = 0; k < 27; k++)
+          if (w[nv + 8 + k] != LRT_INVALID_IDX && w[nv + 8 + k] >= n_nodes) return set_err(LRT_E_INVALID, "neighbour index out of range");
+    }
+  }
+  Renderer* r = h;
+  hipError_t e;
+  if ((e = hipSetDevice(r->device)) != hipSuccess) return hip_err(e, "hipSetDevice");
+  r->header = *header;
+  r->header.aux_data = layout;
+  r->quant_voxel_diag = header->aux_data & 1u;
+  r->geom_type = (layout == LRT_SBS_LAYOUT_DX_RGB8 || indexed) ? LRT_TYPE_SDF_SBS_COL
+                                                               : (layout == LRT_SBS_LAYOUT_DX_UV16 ? LRT_TYPE_SDF_SBS_TEX : LRT_TYPE_SDF_SBS);
+  r->nodes.assign(nodes, nodes + n_nodes);
+  r->values.assign(values, values + n_values);
+  if (indexed) r->values_f.assign(values_f, values_f + n_values_f);
+  else r->values_f.clear();
+  std::vector<f3> bmin, bmax;
+  sbs_brick_boxes(nodes, n_nodes, bmin, bmax);
+  BuildTree tree = build_sah(bmin, bmax, 1);
+  r->pairs = to_fat_pairs(tree, bmin, bmax, &r->pair_depth);
+  if ((e = r->d_pairs.upload(r->pairs.data(), r->pairs.size() * sizeof(BVHNodePair), r->stream)) != hipSuccess) return hip_err(e, "upload bvh");
+  if ((e = r->d_nodes.upload(r->nodes.data(), r->nodes.size() * sizeof(SdfSBSNode), r->stream)) != hipSuccess) return hip_err(e, "upload nodes");
+  if ((e = r->d_values.upload(r->values.data(), r->values.size() * 4, r->stream)) != hipSuccess) return hip_err(e, "upload values");
+  if ((e = r->d_values_f.upload(r->values_f.data(), r->values_f.size() * 4, r->stream)) != hipSuccess) return hip_err(e, "upload values_f");
+  if ((e = hipStreamSynchronize(r->stream)) != hipSuccess) return hip_err(e, "scene upload");
+  r->has_scene = true;
+  return LRT_OK;
+}
+
+int lrt_get_bvh(lrt_handle h, BVHNodePair* out, uint32_t capacity, uint32_t* count, uint32_t* depth) {
+  int c = check_ready(h);
+  if (c) return c;
+  if (count) *count = (uint32_t)h->pairs.size();
+  if (depth) *depth = (uint32_t)h->pair_depth;
+  if (out) {
+    if (capacity < h->pairs.size()) return set_err(LRT_E_INVALID, "capacity too small");
+    std::memcpy(out, h->pairs.data(), h->pairs.size() * sizeof(BVHNodePair));
+  }
+  return LRT_OK;
+}
+
+int lrt_build_bvh(const SdfSBSNode* nodes, uint32_t n_nodes, BVHNodePair* out, uint32_t capacity, uint32_t* count, uint32_t* depth) {
+  if (!nodes || n_nodes == 0) return set_err(LRT_E_INVALID, "empty brick list");
+  std::vector<f3> bmin, bmax;
+  sbs_brick_boxes(nodes, n_nodes, bmin, bmax);
+  BuildTree tree = build_sah(bmin, bmax, 1);
+  int d = 0;
+  std::vector<BVHNodePair> pairs = to_fat_pairs(tree, bmin, bmax, &d);
+  if (count) *count = (uint32_t)pairs.size();
+  if (depth) *depth = (uint32_t)d;
+  if (out) {
+    if (capacity < pairs.size()) return set_err(LRT_E_INVALID, "capacity too small");
+    std::memcpy(out, pairs.data(), pairs.size() * sizeof(BVHNodePair));
+  }
+  return LRT_OK;
+}
+
+int lrt_set_preset(lrt_handle h, const MultiRenderPreset* p) {
+  if (!h || !p) return set_err(LRT_E_INVALID, "null argument");
+  if (p->spp == 0) return set_err(LRT_E_INVALID, "spp must be >= 1");
+  if (p->sdf_node_intersect > LRT_SDF_NODE_INTERSECT_IT) return set_err(LRT_E_INVALID, "bad sdf_node_intersect");
+  h->preset = *p;
+  return LRT_OK;
+}
+int lrt_get_preset(lrt_handle h, MultiRenderPreset* p) {
+  if (!h || !p) return set_err(LRT_E_INVALID, "null argument");
+  *p = h->preset;
+  return LRT_OK;
+}
+int lrt_set_viewport(lrt_handle h, int x0, int y0, int width, int height) {
+  (void)x0; (void)y0;
+  if (!h || width <= 0 || height <= 0 || width > 65535 || height > 65535) return set_err(LRT_E_INVALID, "bad viewport");
+  h->width = (uint32_t)width;
+  h->height = (uint32_t)height;
+  return LRT_OK;
+}
+int lrt_update_camera(lrt_handle h, const float wv[16], const float pr[16]) {
+  if (!h || !wv || !pr) return set_err(LRT_E_INVALID, "null argument");
+  std::memcpy(h->world_view, wv, 64);
+  std::memcpy(h->proj, pr, 64);
+  h->cam = make_camera(wv, pr);
+  h->has_camera = true;
+  return LRT_OK;
+}
+int lrt_set_seed(lrt_handle h, uint32_t seed) {
+  if (!h) return set_err(LRT_E_INVALID, "null handle");
+  h->seed = seed;
+  return LRT_OK;
+}
+
+static int render_host(lrt_handle h, uint32_t W, uint32_t H, int passes, bool rgba, bool gbuf, const char* name) {
+  int c = check_ready(h);
+  if (c) return c;
+  if (!h->has_camera) return set_err(LRT_E_INVALID, "no camera: call lrt_update_camera");
+  if (passes < 1) passes = 1;
+  h->width = W;
+  h->height = H;
+  for (int i = 0; i < passes; i++) {
+    if ((c = h->render_device(&h->cam, 1, W, H, rgba, gbuf, gbuf, name, nullptr))) return c;
+    if ((c = h->finish_timer())) return c;
+  }
+  return LRT_OK;
+}
+
+int lrt_render_float(lrt_handle h, float* image, uint32_t W, uint32_t H, int passes) {
+  if (!image) return set_err(LRT_E_INVALID, "null image");
+  int c = render_host(h, W, H, passes, true, false, "CastRayFloatSingleBlock");
+  if (c) return c;
+  hipError_t e = hipMemcpy(image, h->d_out_rgba.p, (size_t)W * H * 16, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? LRT_OK : hip_err(e, "download");
+}
+
+int lrt_render(lrt_handle h, uint32_t* image, uint32_t W, uint32_t H, int passes) {
+  if (!image) return set_err(LRT_E_INVALID, "null image");
+  int c = render_host(h, W, H, passes, true, false, "CastRaySingleBlock");
+  if (c) return c;
+  std::vector<float> tmp((size_t)W * H * 4);
+  hipError_t e = hipMemcpy(tmp.data(), h->d_out_rgba.p, tmp.size() * 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_err(e, "download");
+  for (size_t i = 0; i < (size_t)W * H; i++) {  // uint4(255 * clamp(c, 0, 1)), eye_ray.cpp:111-115
+    uint32_t col[4];
+    for (int k = 0; k < 4; k++) col[k] = (uint32_t)(255.0f * std::min(std::max(tmp[4 * i + k], 0.0f), 1.0f));
+    image[i] = (col[3] << 24) | (col[2] << 16) | (col[1] << 8) | col[0];
+  }
+  return LRT_OK;
+}
+
+int lrt_render_gbuffer(lrt_handle h, float* t, float* n, uint32_t* prim, uint32_t W, uint32_t H) {
+  int c = render_host(h, W, H, 1, false, true, "CastRayGBuffer");
+  if (c) return c;
+  return lrt_download_view(h, 0, t, n, prim, nullptr);
+}
+
+int lrt_cast_rays(lrt_handle h, const float* rays, uint32_t n, CRT_Hit* out) {
+  int c = check_ready(h);
+  if (c) return c;
+  if (!rays || !out) return set_err(LRT_E_INVALID, "null argument");
+  if (n == 0) return LRT_OK;
+  hipError_t e;
+  if ((e = hipSetDevice(h->device)) != hipSuccess) return hip_err(e, "hipSetDevice");
+  if ((e = h->d_rays.upload(rays, (size_t)n * 32, h->stream)) != hipSuccess) return hip_err(e, "upload rays");
+  if ((e = h->d_hits.ensure((size_t)n * sizeof(CRT_Hit))) != hipSuccess) return hip_err(e, "alloc hits");
+  uint32_t blocks = (n + 255) / 256;
+  if ((e = h->ensure_spill(blocks)) != hipSuccess) return hip_err(e, "alloc spill");
+  e = launch_cast_rays(h->dev_scene(), h->trace_params(false), (const float4*)h->d_rays.p, n, (CRT_Hit*)h->d_hits.p,
+                       (uint32_t*)h->d_spill.p, blocks, h->stream);
+  if (e != hipSuccess) return hip_err(e, "cast launch");
+  if ((e = hipMemcpyAsync(out, h->d_hits.p, (size_t)n * sizeof(CRT_Hit), hipMemcpyDeviceToHost, h->stream)) != hipSuccess)
+    return hip_err(e, "download hits");
+  if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_err(e, "cast rays");
+  return LRT_OK;
+}
+
+int lrt_get_execution_time(lrt_handle h, const char* name, float out[4]) {
+  if (!h || !name || !out) return set_err(LRT_E_INVALID, "null argument");
+  auto it = h->times.find(name);
+  if (it == h->times.end()) return LRT_OK;  // reference leaves out untouched when unknown
+  out[0] = it->second;
+  return LRT_OK;
+}
+
+int lrt_render_views_device(lrt_handle h, const float* wvs, const float* prs, uint32_t n_views, uint32_t W, uint32_t H, int out_mode,
+                            float* kernel_ms) {
+  int c = check_ready(h);
+  if (c) return c;
+  if (!wvs || !prs || n_views == 0) return set_err(LRT_E_INVALID, "bad cameras");
+  std::vector<lrtd::Camera> cams(n_views);
+  for (uint32_t v = 0; v < n_views; v++) cams[v] = make_camera(wvs + 16 * v, prs + 16 * v);
+  bool rgba = out_mode == 1;
+  if ((c = h->render_device(cams.data(), n_views, W, H, rgba, !rgba, !rgba, "RenderViews", nullptr))) return c;
+  if (kernel_ms) {
+    if ((c = h->finish_timer())) return c;
+    *kernel_ms = h->last_ms;
+  }
+  return LRT_OK;
+}
+
+int lrt_count_work(lrt_handle h, const float* wvs, const float* prs, uint32_t n_views, uint32_t W, uint32_t H, uint64_t out[3]) {
+  int c = check_ready(h);
+  if (c) return c;
+  if (!wvs || !prs || !out || n_views == 0) return set_err(LRT_E_INVALID, "bad arguments");
+  std::vector<lrtd::Camera> cams(n_views);
+  for (uint32_t v = 0; v < n_views; v++) cams[v] = make_camera(wvs + 16 * v, prs + 16 * v);
+  hipError_t e;
+  if ((e = h->d_counts.ensure(64)) != hipSuccess) return hip_err(e, "alloc counts");
+  if ((e = zero_u32((uint32_t*)h->d_counts.p, 16, h->stream)) != hipSuccess) return hip_err(e, "clear counts");
+  if ((c = h->render_device(cams.data(), n_views, W, H, false, true, true, "CountWork", (unsigned long long*)h->d_counts.p))) return c;
+  if ((c = h->finish_timer())) return c;
+  unsigned long long tmp[4];
+  if ((e = hipMemcpy(tmp, h->d_counts.p, sizeof(tmp), hipMemcpyDeviceToHost)) != hipSuccess) return hip_err(e, "download counts");
+  for (int i = 0; i < 3; i++) out[i] = tmp[i];
+  return LRT_OK;
+}
+
+int lrt_download_view(lrt_handle h, uint32_t view, float* t, float* n, uint32_t* prim, float* rgba) {
+  if (!h) return set_err(LRT_E_INVALID, "null handle");
+  if (view >= h->last_views) return set_err(LRT_E_INVALID, "view index out of range");
+  hipError_t e;
+  if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_err(e, "render");
+  const size_t np = (size_t)h->last_w * h->last_h, off = np * view;
+  if (t && h->d_out_t.p && (e = hipMemcpy(t, (float*)h->d_out_t.p + off, np * 4, hipMemcpyDeviceToHost)) != hipSuccess) return hip_err(e, "t");
+  if (n && h->d_out_n.p && (e = hipMemcpy(n, (float*)h->d_out_n.p + 2 * off, np * 8, hipMemcpyDeviceToHost)) != hipSuccess) return hip_err(e, "n");
+  if (prim && h->d_out_prim.p && (e = hipMemcpy(prim, (uint32_t*)h->d_out_prim.p + off, np * 4, hipMemcpyDeviceToHost)) != hipSuccess)
+    return hip_err(e, "prim");
+  if (rgba && h->d_out_rgba.p && (e = hipMemcpy(rgba, (float*)h->d_out_rgba.p + 4 * off, np * 16, hipMemcpyDeviceToHost)) != hipSuccess)
+    return hip_err(e, "rgba");
+  return LRT_OK;
+}
+
+int lrt_sync(lrt_handle h) {
+  if (!h) return set_err(LRT_E_INVALID, "null handle");
+  hipError_t e = hipStreamSynchronize(h->stream);
+  return e == hipSuccess ? LRT_OK : hip_err(e, "sync");
+}
+
+int lrt_set_launch_blocks(lrt_handle h, uint32_t blocks) {
+  if (!h) return set_err(LRT_E_INVALID, "null handle");
+  h->launch_blocks = blocks;
+  return LRT_OK;
+}
+
+// ---- synthetic scenes -----------------------------------------------------
+
+int lrt_sbs_displaced_icosphere(uint32_t subdiv, float radius, float amplitude, uint32_t octaves, uint32_t seed, uint32_t lod,
+                                uint32_t brick_size, uint32_t brick_pad, uint32_t bytes_per_value, int voxel_diag_scale, int threads,
+                                lrt_sbs_handle* out) {
+  if (!out || subdiv > 8) return set_err(LRT_E_INVALID, "bad arguments");
+  try {
+    TriMesh m = displaced_icosphere((int)subdiv, radius, amplitude, (int)octaves, seed);
+    MeshSdf sdf(m);
+    std::unique_ptr<lrt_sbs> s(new lrt_sbs);
+    s->s = build_sbs([&](f3 p) { return sdf(p); }, lod, brick_size, brick_pad, bytes_per_value, voxel_diag_scale != 0, threads);
+    if (s->s.nodes.empty()) return set_err(LRT_E_INVALID, "scene produced no bricks");
+    *out = s.release();
+  } catch (const std::exception& ex) {
+    return set_err(LRT_E_INVALID, ex.what());
+  }
+  return LRT_OK;
+}
+
+int lrt_sbs_grid(uint32_t shape, float R, float r, float amplitude, uint32_t octaves, uint32_t seed, uint32_t brick_count,
+                 uint32_t brick_size, int with_neighbors, lrt_sbs_handle* out) {
+  if (!out || brick_count == 0 || brick_count > 256 || brick_size == 0 || brick_size > 16) return set_err(LRT_E_INVALID, "bad arguments");
+  std::function<float(f3)> sdf;
+  if (shape == 1) {
+    sdf = [=](f3 p) {  // torus in the xy plane (DR_common.cpp:212-222 orientation)
+      float qx = std::sqrt(p.x * p.x + p.y * p.y) - R;
+      float d = std::sqrt(qx * qx + p.z * p.z) - r;
+      return amplitude != 0.f ? d - amplitude * value_noise_fbm(p, seed, (int)octaves, 2.0f) : d;
+    };
+  } else {
+    sdf = [=](f3 p) {
+      float d = length(p) - R;
+      return amplitude != 0.f ? d - amplitude * value_noise_fbm(p, seed, (int)octaves, 2.0f) : d;
+    };
+  }
+  std::unique_ptr<lrt_sbs> s(new lrt_sbs);
+  s->s = build_grid_sbs(brick_count, brick_size, sdf, [](f3) { return f3(1.0f, 0.44f, 0.0f); }, with_neighbors != 0);
+  *out = s.release();
+  return LRT_OK;
+}
+
+int lrt_sbs_info(lrt_sbs_handle s, SdfSBSHeader* header, uint32_t* n_nodes, uint64_t* n_values, uint64_t* n_values_f) {
+  if (!s) return set_err(LRT_E_INVALID, "null scene");
+  if (header) *header = s->s.header;
+  if (n_nodes) *n_nodes = (uint32_t)s->s.nodes.size();
+  if (n_values) *n_values = s->s.values.size();
+  if (n_values_f) *n_values_f = s->s.values_f.size();
+  return LRT_OK;
+}
+int lrt_sbs_copy(lrt_sbs_handle s, SdfSBSNode* nodes, uint32_t* values, float* values_f) {
+  if (!s) return set_err(LRT_E_INVALID, "null scene");
+  if (nodes) std::memcpy(nodes, s->s.nodes.data(), s->s.nodes.size() * sizeof(SdfSBSNode));
+  if (values) std::memcpy(values, s->s.values.data(), s->s.values.size() * 4);
+  if (values_f && !s->s.values_f.empty()) std::memcpy(values_f, s->s.values_f.data(), s->s.values_f.size() * 4);
+  return LRT_OK;
+}
+int lrt_sbs_free(lrt_sbs_handle s) {
+  delete s;
+  return LRT_OK;
+}
+
+int lrt_orbit_camera(uint32_t seed, uint32_t index, float distance, float fov_deg, float aspect, float znear, float zfar,
+                     float world_view[16], float proj[16]) {
+  if (!world_view || !proj) return set_err(LRT_E_INVALID, "null output");
+  uint64_t z = ((uint64_t)seed << 32) ^ (0x9E3779B97F4A7C15ull * (index + 1));
+  auto next = [&]() {  // splitmix64
+    z += 0x9E3779B97F4A7C15ull;
+    uint64_t x = z;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return (double)((x ^ (x >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+  };
+  double az = 2.0 * 3.14159265358979323846 * next();
+  double el = (next() - 0.5) * (3.14159265358979323846 / 2.0);  // elevation in [-45, 45] degrees
+  f3 eye((float)(distance * std::cos(el) * std::sin(az)), (float)(distance * std::sin(el)), (float)(distance * std::cos(el) * std::cos(az)));
+  m4 v = look_at(eye, f3(0, 0, 0), f3(0, 1, 0));
+  m4 p = perspective(fov_deg, aspect, znear, zfar);
+  std::memcpy(world_view, v.m, 64);
+  std::memcpy(proj, p.m, 64);
+  return LRT_OK;
+}
+
+}  // extern "C"

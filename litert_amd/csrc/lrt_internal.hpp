@@ -1,0 +1,73 @@
+// lrt_internal.hpp - state behind an lrt_handle.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <map>
+#include <string>
+#include <vector>
+#include "../../include/lrt.h"
+#include "kernels/render.hpp"
+
+namespace lrt {
+
+int set_err(int code, const std::string& msg);
+int hip_err(hipError_t e, const char* what);
+lrtd::Camera make_camera(const float wv[16], const float pr[16]);
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t n);
+  hipError_t upload(const void* src, size_t n, hipStream_t s);
+  void release();
+};
+
+struct Renderer {
+  int device = 0;
+  int cu_count = 256;
+  uint32_t launch_blocks = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  MultiRenderPreset preset{};
+  uint32_t width = 0, height = 0, seed = 0;
+  float world_view[16] = {}, proj[16] = {};
+  lrtd::Camera cam{};
+  bool has_camera = false;
+
+  // scene as ingested (host copy, reference layout)
+  bool has_scene = false;
+  SdfSBSHeader header{};
+  uint32_t quant_voxel_diag = 0, geom_type = LRT_TYPE_SDF_SBS;
+  std::vector<SdfSBSNode> nodes;
+  std::vector<uint32_t> values;
+  std::vector<float> values_f;
+  std::vector<BVHNodePair> pairs;
+  int pair_depth = 0;
+
+  // device scene
+  DevBuf d_pairs, d_nodes, d_values, d_values_f;
+  // work buffers
+  DevBuf d_cams, d_counter, d_spill, d_counts, d_rays, d_hits;
+  DevBuf d_out_t, d_out_n, d_out_prim, d_out_rgba;
+  uint32_t last_w = 0, last_h = 0, last_views = 0;
+  float last_ms = 0.f;
+  std::string pending_timer;
+  std::map<std::string, float> times;
+
+  lrtd::DevScene dev_scene() const;
+  lrtd::TraceParams trace_params(bool force_normal) const;
+  uint32_t grid_blocks() const;
+  hipError_t ensure_spill(uint32_t blocks);
+  int render_device(const lrtd::Camera* cams, uint32_t n_views, uint32_t W, uint32_t H, bool rgba, bool gbuf, bool force_normal,
+                    const char* timer_name, unsigned long long* d_counts);
+  int finish_timer();
+  void release_all() {
+    DevBuf* all[] = {&d_pairs, &d_nodes, &d_values, &d_values_f, &d_cams, &d_counter, &d_spill, &d_counts,
+                     &d_rays, &d_hits, &d_out_t, &d_out_n, &d_out_prim, &d_out_rgba};
+    for (DevBuf* b : all) b->release();
+  }
+};
+
+}  // namespace lrt
+
+struct lrt_renderer : lrt::Renderer {};

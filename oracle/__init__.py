@@ -1,0 +1,184 @@
+"""CPU oracle: restatement of the reference LiteRT SBS path (TEST INFRASTRUCTURE).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this package. It is the checker of the HIP library and the CPU
+baseline; the product library never calls it.
+
+The reference itself cannot be compiled here (its HydraCore3 submodule --
+LiteMath, CrossRT -- is empty and the Embree runtime is missing), so there is
+no oracle/_ref build; see lrt_oracle.h for the restated functions.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD_DIR = os.path.join(_HERE, "_build")
+LIB_PATH = os.path.join(BUILD_DIR, "liblrt_oracle.so")
+SOURCES = ["oracle_api.cpp", "oracle_dr.cpp", "oracle_mlp.cpp"]
+# -ffp-contract=off matches the reference's ISO C++17 build (no FP contraction).
+FLAGS = ["-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC", "-shared",
+         "-Wall", "-Wno-unused-function"]
+
+
+def build():
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    srcs = [os.path.join(_HERE, s) for s in SOURCES if os.path.exists(os.path.join(_HERE, s))]
+    tmp = LIB_PATH + ".tmp"
+    r = subprocess.run(["g++", *FLAGS, *srcs, "-o", tmp], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        u32, u64, i32, f32, f64 = C.c_uint32, C.c_uint64, C.c_int, C.c_float, C.c_double
+        sig = {
+            "orc_scene_create": (P, [P, P, u32, P, u64, P, u64, P, u32]),
+            "orc_scene_destroy": (None, [P]),
+            "orc_scene_set_values_f": (i32, [P, P, u64]),
+            "orc_render": (i32, [P, P, P, P, u32, u32, u32, i32, i32, i32, P, P, P, P, P]),
+            "orc_cast_rays": (i32, [P, P, P, u32, i32, i32, i32, P]),
+            "orc_brute_nearest": (i32, [P, P, P, u32, i32, i32, P, P]),
+            "orc_eval_trilinear": (f64, [P, P, i32]),
+            "orc_eval_trilinear_diff": (None, [P, P, i32, P]),
+            "orc_voxel_root": (i32, [u32, P, P, P, f64, f64, i32, P, P]),
+            "orc_encode_normal": (None, [P, i32, P]),
+            "orc_decode_normal": (None, [P, i32, P]),
+            "orc_pcg3d": (None, [P]),
+            "orc_rand2": (None, [u32, u32, u32, u32, P]),
+            "orc_pack_xy": (None, [u32, u32, P]),
+            "orc_safe_inverse": (None, [P, i32, P]),
+            "orc_ray_box": (None, [P, P, P, P, i32, P]),
+            "orc_load_voxel": (f64, [P, u32, u32, u32, u32, i32, P]),
+            "orc_eye_ray": (None, [P, P, u32, u32, f32, f32, u32, u32, i32, P]),
+        }
+        optional = {
+            "orc_dr_render_grad": (i32, [P, P, P, P, u32, u32, u32, P, i32, i32, i32, P, P, P]),
+            "orc_adam_step": (None, [u32, P, P, P, P, u32, f32, f32, f32, f32]),
+            "orc_mlp_forward": (None, [P, P, u32, i32, P]),
+            "orc_mlp_loss_grad": (f64, [P, P, P, u32, i32, P]),
+        }
+        for name, (res, args) in list(sig.items()) + list(optional.items()):
+            if name in optional and not hasattr(L, name):
+                continue
+            f = getattr(L, name)
+            f.restype, f.argtypes = res, args
+        _lib = L
+    return _lib
+
+
+class Counters(C.Structure):
+    _fields_ = [("node_pairs", C.c_uint64), ("bricks", C.c_uint64), ("voxels", C.c_uint64),
+                ("root_finds", C.c_uint64), ("rays", C.c_uint64)]
+
+
+class DRStats(C.Structure):
+    _fields_ = [("loss", C.c_double), ("border_pixels", C.c_uint32), ("border_rays_hit", C.c_uint32),
+                ("rays", C.c_uint64)]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Scene:
+    """Oracle copy of an SBS scene plus the fat BVH2 pairs (uint32 (n,16))."""
+
+    def __init__(self, sbs, pairs):
+        self._keep = (np.ascontiguousarray(sbs.nodes, np.uint32), np.ascontiguousarray(sbs.values, np.uint32),
+                      np.ascontiguousarray(sbs.values_f, np.float32), np.ascontiguousarray(pairs, np.uint32))
+        n, v, vf, pr = self._keep
+        self.h = lib().orc_scene_create(C.byref(sbs.header), _p(n), n.shape[0], _p(v), v.size,
+                                        _p(vf) if vf.size else None, vf.size, _p(pr), pr.shape[0])
+        if not self.h:
+            raise ValueError("orc_scene_create rejected the scene")
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().orc_scene_destroy(self.h)
+        except Exception:
+            pass
+
+    def render(self, world_view, proj, preset, W, H, seed=0, precision=32, threads=0, force_normal=True, rgba=False,
+               counters=False):
+        t = np.zeros((H, W), np.float64)
+        n = np.zeros((H, W, 2), np.float64)
+        prim = np.zeros((H, W), np.uint32)
+        col = np.zeros((H, W, 4), np.float64) if rgba else None
+        cnt = Counters() if counters else None
+        wv = np.ascontiguousarray(world_view, np.float32)
+        pr = np.ascontiguousarray(proj, np.float32)
+        rc = lib().orc_render(self.h, _p(wv), _p(pr), C.byref(preset), W, H, seed, precision, threads, int(force_normal),
+                              _p(col), _p(t), _p(prim), _p(n), C.byref(cnt) if cnt is not None else None)
+        if rc != 0:
+            raise ValueError(f"orc_render failed {rc}")
+        out = {"t": t, "n": n, "prim": prim}
+        if rgba:
+            out["rgba"] = col
+        if counters:
+            out["counters"] = {k: getattr(cnt, k) for k, _ in Counters._fields_}
+        return out
+
+    def cast_rays(self, preset, rays, precision=32, threads=0, force_normal=True):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros((rays.shape[0], 6), np.float64)
+        rc = lib().orc_cast_rays(self.h, C.byref(preset), _p(rays), rays.shape[0], precision, threads, int(force_normal),
+                                 _p(out))
+        if rc != 0:
+            raise ValueError("orc_cast_rays failed")
+        return out
+
+    def brute_nearest(self, preset, rays, precision=32, threads=0):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        t = np.zeros(rays.shape[0], np.float64)
+        prim = np.zeros(rays.shape[0], np.uint32)
+        rc = lib().orc_brute_nearest(self.h, C.byref(preset), _p(rays), rays.shape[0], precision, threads, _p(t), _p(prim))
+        if rc != 0:
+            raise ValueError("orc_brute_nearest failed")
+        return t, prim
+
+    def load_voxel(self, node, vx, vy, vz, precision=32):
+        out = np.zeros(8, np.float64)
+        vmin = lib().orc_load_voxel(self.h, node, vx, vy, vz, precision, _p(out))
+        return out, vmin
+
+    def set_values_f(self, vf):
+        vf = np.ascontiguousarray(vf, np.float32)
+        self._keep = self._keep[:2] + (vf,) + self._keep[3:]
+        if lib().orc_scene_set_values_f(self.h, _p(vf), vf.size) != 0:
+            raise ValueError("size mismatch")
+
+    def dr_render_grad(self, dr_preset, world_view, proj, W, H, ref, n_params, seed=0, precision=32, threads=0,
+                       dtdv_mode=0):
+        img = np.zeros((H, W, 4), np.float64)
+        grad = np.zeros(n_params, np.float64)
+        st = DRStats()
+        wv = np.ascontiguousarray(world_view, np.float32)
+        pr = np.ascontiguousarray(proj, np.float32)
+        ref = np.ascontiguousarray(ref, np.float32)
+        rc = lib().orc_dr_render_grad(self.h, C.byref(dr_preset), _p(wv), _p(pr), W, H, seed, _p(ref), precision, threads,
+                                      dtdv_mode, _p(img), _p(grad), C.byref(st))
+        if rc != 0:
+            raise ValueError(f"orc_dr_render_grad failed {rc}")
+        return img, grad, {k: getattr(st, k) for k, _ in DRStats._fields_}
+
+
+def eye_ray(world_view, proj, x, y, dx, dy, W, H, precision=32):
+    out = np.zeros(8, np.float64)
+    lib().orc_eye_ray(_p(np.ascontiguousarray(world_view, np.float32)), _p(np.ascontiguousarray(proj, np.float32)),
+                      x, y, dx, dy, W, H, precision, _p(out))
+    return out

@@ -1,0 +1,212 @@
+"""GPU parity of the plain primary-ray path against the CPU oracle.
+
+Tolerances (north star): hit mask may differ on at most 1e-4 of the pixels
+and only on grazing/seam pixels; depth within 1e-5 relative and normals
+within 1e-4 rad of the FP64 oracle. The FP32 oracle is the reference CPU
+path restated operation for operation, so the GPU is also compared with it
+directly (it is expected to agree bit for bit on almost every pixel).
+"""
+import numpy as np
+import pytest
+
+import litert_amd.api as L
+import oracle
+from helpers import MISS, angle, c1_scene, c2_scene, oracle_scene, preset
+
+pytestmark = pytest.mark.gpu
+
+MODES = [L.SDF_OCTREE_NODE_INTERSECT_ST, L.SDF_OCTREE_NODE_INTERSECT_ANALYTIC, L.SDF_OCTREE_NODE_INTERSECT_NEWTON,
+         L.SDF_OCTREE_NODE_INTERSECT_IT]
+
+
+@pytest.fixture(scope="module")
+def r_c1():
+    r = L.CreateMultiRenderer(L.DEVICE_GPU)
+    r.SetScene(c1_scene())
+    yield r
+    r.close()
+
+
+def _grazing_or_seam(wv, pr, W, H, ys, xs, t, n, voxels_per_unit):
+    """|cos(ray, n)| < 1e-2, or the hit lies within 1e-3 voxel of a voxel face."""
+    out = []
+    for y, x, tt, nn in zip(ys, xs, t, n):
+        ray = oracle.eye_ray(wv, pr, int(x), int(y), 0.5, 0.5, W, H, 64)
+        o, d = ray[:3], ray[4:7]
+        c = abs(float(np.dot(d, _dec(nn))))
+        p = (o + tt * d + 1.0) * voxels_per_unit
+        seam = np.min(np.abs(p - np.round(p))) < 1e-3
+        out.append(c < 1e-2 or seam)
+    return np.array(out, bool)
+
+
+def _dec(n):
+    from helpers import decode_oct
+    return decode_oct(np.asarray(n, np.float64)[None])[0]
+
+
+def _compare(r, sc, p, wv, pr, W, H, voxels_per_unit):
+    r.SetViewport(0, 0, W, H)
+    r.SetPreset(p)
+    r.UpdateCamera(wv, pr)
+    t, n, prim = r.RenderGBuffer(W, H)
+    o32 = sc.render(wv, pr, p, W, H, precision=32)
+    o64 = sc.render(wv, pr, p, W, H, precision=64)
+    g_hit, c_hit, d_hit = prim != MISS, o32["prim"] != MISS, o64["prim"] != MISS
+    res = {}
+    # bit-level agreement with the FP32 twin
+    both32 = g_hit & c_hit
+    res["mask_mismatch_fp32"] = int((g_hit != c_hit).sum())
+    res["exact_t_fp32"] = float(np.mean(t[both32] == o32["t"][both32].astype(np.float32))) if both32.any() else 1.0
+    # error against FP64
+    mism = np.argwhere(g_hit != d_hit)
+    res["mask_mismatch_fp64"] = len(mism)
+    if len(mism):
+        ys, xs = mism[:, 0], mism[:, 1]
+        tt = np.where(g_hit[ys, xs], t[ys, xs], o64["t"][ys, xs])
+        nn = np.where(g_hit[ys, xs][:, None], n[ys, xs], o64["n"][ys, xs])
+        res["mismatch_all_grazing"] = bool(_grazing_or_seam(wv, pr, W, H, ys, xs, tt, nn, voxels_per_unit).all())
+    else:
+        res["mismatch_all_grazing"] = True
+    both = g_hit & d_hit
+    rel = np.abs(t[both].astype(np.float64) - o64["t"][both]) / o64["t"][both]
+    res["depth_rel_max"] = float(rel.max()) if rel.size else 0.0
+    ang = angle(n[both], o64["n"][both])
+    ang32 = angle(o32["n"][both], o64["n"][both]) if (c_hit[both]).all() else np.zeros_like(ang)
+    # seam pixels: the FP32 CPU path itself leaves the 1e-4 band there (normal
+    # field of the trilinear interpolant is discontinuous across voxel faces)
+    bad = (ang > 1e-4) & ~(ang32 > 1e-4)
+    res["normal_bad"] = int(bad.sum())
+    res["normal_p999"] = float(np.quantile(ang, 0.999)) if ang.size else 0.0
+    res["n_pix"] = W * H
+    res["hits"] = int(g_hit.sum())
+    return res
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("view", [0, 1, 2])
+def test_c1_gbuffer_parity(r_c1, mode, view):
+    sc = oracle_scene("c1")
+    wv, pr = L.orbit_camera(7, view)
+    res = _compare(r_c1, sc, preset(mode), wv, pr, 256, 256, voxels_per_unit=32 * 4 / 2.0)
+    print(mode, view, res)
+    assert res["hits"] > 5000
+    assert res["mask_mismatch_fp32"] <= 1e-4 * res["n_pix"]
+    assert res["mask_mismatch_fp64"] <= 1e-4 * res["n_pix"] and res["mismatch_all_grazing"]
+    assert res["depth_rel_max"] <= 1e-5
+    assert res["normal_bad"] == 0
+    if mode != L.SDF_OCTREE_NODE_INTERSECT_ANALYTIC:  # transcendental calls differ between libm and the device
+        assert res["exact_t_fp32"] > 0.999
+
+
+def test_c2_gbuffer_parity():
+    sbs = c2_scene()
+    r = L.CreateMultiRenderer(L.DEVICE_GPU)
+    r.SetScene(sbs)
+    sc = oracle_scene("c2")
+    wv, pr = L.orbit_camera(11, 0)
+    res = _compare(r, sc, preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON), wv, pr, 512, 512, voxels_per_unit=128 * 4 / 2.0)
+    print(res)
+    assert res["mask_mismatch_fp32"] <= 1e-4 * res["n_pix"]
+    assert res["mask_mismatch_fp64"] <= 1e-4 * res["n_pix"] and res["mismatch_all_grazing"]
+    assert res["depth_rel_max"] <= 1e-5
+    assert res["normal_bad"] == 0
+    assert res["exact_t_fp32"] > 0.999
+
+
+@pytest.mark.parametrize("rmode", [L.MULTI_RENDER_MODE_MASK, L.MULTI_RENDER_MODE_LINEAR_DEPTH, L.MULTI_RENDER_MODE_DEPTH,
+                                   L.MULTI_RENDER_MODE_NORMAL, L.MULTI_RENDER_MODE_LAMBERT_NO_TEX,
+                                   L.MULTI_RENDER_MODE_PRIMITIVE, L.MULTI_RENDER_MODE_ST_ITERATIONS])
+def test_render_float_modes(r_c1, rmode):
+    """RenderFloat colour per render mode vs the FP32 oracle (eye_ray.cpp:162-489)."""
+    sc = oracle_scene("c1")
+    wv, pr = L.orbit_camera(3, 1)
+    p = preset(L.SDF_OCTREE_NODE_INTERSECT_ST, render_mode=rmode)
+    img = r_c1.RenderFloat(128, 128, wv, pr, p)
+    ref = sc.render(wv, pr, p, 128, 128, precision=32, force_normal=False, rgba=True)["rgba"]
+    diff = np.abs(img - ref).max(axis=-1)
+    assert np.mean(diff <= 1e-5) > 0.9995, (rmode, np.sort(diff.ravel())[-10:])
+
+
+def test_render_rgba8_and_timer(r_c1):
+    wv, pr = L.orbit_camera(3, 2)
+    p = preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON, render_mode=L.MULTI_RENDER_MODE_LAMBERT_NO_TEX)
+    f = r_c1.RenderFloat(64, 64, wv, pr, p)
+    u = r_c1.Render(64, 64, wv, pr, p)
+    exp = (255 * np.clip(f, 0, 1)).astype(np.uint32)
+    packed = exp[..., 0] | (exp[..., 1] << 8) | (exp[..., 2] << 16) | (exp[..., 3] << 24)
+    assert np.array_equal(u, packed)
+    assert r_c1.GetExecutionTime("CastRaySingleBlock")[0] > 0
+
+
+def test_spp_random_rays(r_c1):
+    """spp 4 with PCG3D jitter (RAY_GEN_MODE_RANDOM) matches the oracle."""
+    sc = oracle_scene("c1")
+    wv, pr = L.orbit_camera(5, 0)
+    p = preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON, render_mode=L.MULTI_RENDER_MODE_LINEAR_DEPTH, spp=4,
+               ray_gen=L.RAY_GEN_MODE_RANDOM)
+    r_c1.setSeed(12345)
+    img = r_c1.RenderFloat(96, 96, wv, pr, p)
+    ref = sc.render(wv, pr, p, 96, 96, seed=12345, precision=32, rgba=True)["rgba"]
+    assert np.mean(np.abs(img - ref).max(-1) <= 1e-6) > 0.999
+
+
+def test_cast_rays_random(r_c1):
+    """Batched RayQuery_NearestHit on rays from outside, inside and on axis."""
+    sc = oracle_scene("c1")
+    rng = np.random.default_rng(0)
+    n = 4096
+    o = rng.uniform(-1.5, 1.5, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d[:256, 1:] = 0.0  # axis-aligned directions exercise SafeInverse with zeros
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3], rays[:, 4:7], rays[:, 7] = o, d, 1e9
+    hits = r_c1.RayQuery_NearestHit(rays)
+    g = np.array([(h.t, h.primId, h.coords[2], h.coords[3]) for h in hits])
+    ref = sc.cast_rays(preset(), rays, precision=32)
+    gh, rh = g[:, 1] != MISS, ref[:, 1] != MISS
+    assert (gh != rh).sum() <= 2
+    both = gh & rh
+    assert np.mean(g[both, 0].astype(np.float32) == ref[both, 0].astype(np.float32)) > 0.999
+
+
+def test_work_counters_match_oracle(r_c1):
+    """GPU per-ray work counters equal the FP32 oracle's (same traversal order)."""
+    sc = oracle_scene("c1")
+    wv, pr = L.orbit_camera(7, 0)
+    r_c1.SetPreset(preset())
+    g = r_c1.CountWork(wv[None], pr[None], 256, 256)
+    o = sc.render(wv, pr, preset(), 256, 256, precision=32, counters=True)["counters"]
+    assert abs(int(g[0]) - o["node_pairs"]) <= 1e-4 * o["node_pairs"]
+    assert abs(int(g[1]) - o["bricks"]) <= 1e-4 * o["bricks"]
+    assert abs(int(g[2]) - o["voxels"]) <= 1e-4 * o["voxels"]
+
+
+def test_brute_force_topology_independent():
+    """On a small scene the BVH result equals the nearest hit over all bricks
+    except where stop-at-first-SDF-hit returns a farther brick."""
+    sbs = L.displaced_icosphere(3, 0.7, 0.05, 3, seed=4, lod=8, brick_size=4, brick_pad=0, bytes_per_value=2)
+    r = L.CreateMultiRenderer(L.DEVICE_GPU)
+    r.SetScene(sbs)
+    pairs, _ = r.GetBVH()
+    sc = oracle.Scene(sbs, pairs)
+    wv, pr = L.orbit_camera(9, 0)
+    rays = np.array([oracle.eye_ray(wv, pr, x, y, 0.5, 0.5, 64, 64) for y in range(64) for x in range(64)], np.float32)
+    hits = r.RayQuery_NearestHit(rays)
+    gt = np.array([h.t for h in hits])
+    bt, bp = sc.brute_nearest(preset(), rays)
+    hit = bp != MISS
+    assert np.array_equal(np.array([h.primId for h in hits]) != MISS, hit)
+    assert np.mean(np.abs(gt[hit] - bt[hit]) <= 1e-6 * bt[hit]) > 0.99
+
+
+def test_bad_inputs_raise():
+    r = L.CreateMultiRenderer(L.DEVICE_GPU)
+    with pytest.raises(L.LrtError):
+        r.RenderFloat(16, 16)  # no scene
+    sbs = c1_scene()
+    bad = L.SdfSBS(sbs.header, sbs.nodes.copy(), sbs.values)
+    bad.nodes[0, 2] = sbs.values.size  # data_offset out of range
+    with pytest.raises(L.LrtError):
+        r.SetScene(bad)
