@@ -96,6 +96,35 @@ int lrt_sync(lrt_handle h);
 /* Persistent-grid size of the render kernels (0 = device default). */
 int lrt_set_launch_blocks(lrt_handle h, uint32_t blocks);
 
+/* ---- differentiable rendering (dr::MultiRendererDR) --------------------- */
+/* MultiRendererDR::SetReference(images, worldView, proj),
+ * diff_render/MultiRendererDR.h:76-82, MultiRendererDR.cpp:147-185: n_views
+ * RGBA images of W x H (W*H*4 floats each). masks may be NULL, then the mask is
+ * 1 where |rgb| >= 0.001 (CreateRefImageMasks); the alpha used for the loss is
+ * the mask. Images must already have the render size (the reference's
+ * bilinear resampling, PreprocessRefImages :187-226, is not done). */
+int lrt_dr_set_reference(lrt_handle h, const float* images, const float* masks, uint32_t n_views, uint32_t width, uint32_t height,
+                         const float* world_views, const float* projs);
+/* MultiRendererDR::RenderDR (diff_render/MultiRendererDR.cpp:578-750) for one
+ * reference view with the current scene (indexed ID32F layout): image, loss
+ * and dLoss/dS (values_f-sized, already scaled by getBaseGradientMult) of that
+ * view. dtdv_mode 0 = analytic dt/dS (implicit function theorem), 1 = the
+ * reference's central difference (BVH2DR.cpp:1075-1090). Outputs may be NULL. */
+int lrt_dr_render_grad(lrt_handle h, const MultiRendererDRPreset* preset, uint32_t view, uint32_t seed, int dtdv_mode, float* out_image,
+                       float* out_grad, float* loss, float* kernel_ms);
+/* MultiRendererDR::OptimizeFixedStructure(preset, sbs), MultiRendererDR.cpp:278-576:
+ * sets the scene, runs opt_iterations Adam steps over the reference views and
+ * writes the optimised values_f back. losses (opt_iterations floats) and
+ * total_ms may be NULL. View choice when image_batch_size < n_views uses a
+ * PCG hash of (seed, iteration, batch slot) in place of rand(). */
+int lrt_dr_optimize(lrt_handle h, const MultiRendererDRPreset* preset, const SdfSBSHeader* header, const SdfSBSNode* nodes,
+                    uint32_t n_nodes, const uint32_t* values, uint64_t n_values, float* values_f, uint64_t n_values_f, uint32_t seed,
+                    int dtdv_mode, float* losses, float* total_ms);
+/* MultiRendererDR::getLastdLoss_dS, MultiRendererDR.h:88. */
+int lrt_dr_get_last_dloss_ds(lrt_handle h, float* out, uint64_t n);
+/* Border statistics of the last RenderDR: border pixels and border rays that found a relaxation point. */
+int lrt_dr_last_stats(lrt_handle h, uint32_t* border_pixels, uint32_t* border_rays_hit);
+
 /* ---- synthetic scenes (no reference counterpart; the closest reference
  * builders are utils/sparse_octree_builder.cpp:291-444 and
  * diff_render/DR_common.cpp:47-146) ---------------------------------------- */
@@ -111,6 +140,17 @@ int lrt_sbs_grid(uint32_t shape, float R, float r, float amplitude, uint32_t oct
 int lrt_sbs_info(lrt_sbs_handle s, SdfSBSHeader* header, uint32_t* n_nodes, uint64_t* n_values, uint64_t* n_values_f);
 int lrt_sbs_copy(lrt_sbs_handle s, SdfSBSNode* nodes, uint32_t* values, float* values_f);
 int lrt_sbs_free(lrt_sbs_handle s);
+/* load_sdf_SBS / save_sdf_SBS (sdfScene/sdf_scene.cpp:151-175): header,
+ * uint32 node count, nodes, uint32 value count, values. The reference format
+ * drops values_f; save appends uint32 count + floats when values_f is present
+ * and load reads that tail if the file has one. */
+int lrt_sbs_load_file(const char* path, lrt_sbs_handle* out);
+int lrt_sbs_save_file(const char* path, const SdfSBSHeader* header, const SdfSBSNode* nodes, uint32_t n_nodes, const uint32_t* values,
+                      uint32_t n_values, const float* values_f, uint32_t n_values_f);
+/* LiteMath lookAt(eye, target, up) and perspectiveMatrix(fov, aspect, near, far)
+ * as the reference tests build cameras (tests/tests.cpp:29-41). */
+int lrt_camera_lookat(const float eye[3], const float target[3], const float up[3], float fov_deg, float aspect, float znear,
+                      float zfar, float world_view[16], float proj[16]);
 /* Seeded orbit camera looking at the origin: azimuth/elevation drawn from
  * `seed` and `index`, eye at `distance`, up (0,1,0), perspective fov (deg). */
 int lrt_orbit_camera(uint32_t seed, uint32_t index, float distance, float fov_deg, float aspect, float znear, float zfar,

@@ -140,8 +140,19 @@ def _load():
         "lrt_sbs_copy": (i32, [P, P, P, P]),
         "lrt_sbs_free": (i32, [P]),
         "lrt_orbit_camera": (i32, [u32, u32, f32, f32, f32, f32, f32, P, P]),
+        "lrt_sbs_load_file": (i32, [C.c_char_p, C.POINTER(P)]),
+        "lrt_sbs_save_file": (i32, [C.c_char_p, C.POINTER(SdfSBSHeader), P, u32, P, u32, P, u32]),
+        "lrt_camera_lookat": (i32, [P, P, P, f32, f32, f32, f32, P, P]),
     }
     optional = {}
+    sig.update({
+        "lrt_dr_set_reference": (i32, [P, P, P, u32, u32, u32, P, P]),
+        "lrt_dr_render_grad": (i32, [P, C.POINTER(MultiRendererDRPreset), u32, u32, i32, P, P, C.POINTER(f32), C.POINTER(f32)]),
+        "lrt_dr_optimize": (i32, [P, C.POINTER(MultiRendererDRPreset), C.POINTER(SdfSBSHeader), P, u32, P, u64, P, u64, u32, i32, P,
+                                  C.POINTER(f32)]),
+        "lrt_dr_get_last_dloss_ds": (i32, [P, P, u64]),
+        "lrt_dr_last_stats": (i32, [P, C.POINTER(u32), C.POINTER(u32)]),
+    })
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
         f.restype, f.argtypes = res, args
@@ -222,6 +233,27 @@ def build_bvh(sbs):
     pairs = np.zeros((n.value, 16), np.uint32)
     _check(lib.lrt_build_bvh(_ptr(sbs.nodes), sbs.nodes.shape[0], _ptr(pairs), n.value, None, None), "lrt_build_bvh")
     return pairs, depth.value
+
+
+def load_sbs(path):
+    """load_sdf_SBS (sdfScene/sdf_scene.cpp:163-175)."""
+    h = C.c_void_p()
+    _check(lib.lrt_sbs_load_file(os.fsencode(path), C.byref(h)), "lrt_sbs_load_file")
+    return SdfSBS._from_handle(h)
+
+
+def save_sbs(path, sbs):
+    nf = sbs.values_f.size
+    _check(lib.lrt_sbs_save_file(os.fsencode(path), C.byref(sbs.header), _ptr(sbs.nodes), sbs.nodes.shape[0], _ptr(sbs.values),
+                                 sbs.values.size, _ptr(sbs.values_f) if nf else None, nf), "lrt_sbs_save_file")
+
+
+def look_at_camera(eye, target, up, fov=60.0, aspect=1.0, znear=0.1, zfar=100.0):
+    wv = np.zeros(16, np.float32)
+    pr = np.zeros(16, np.float32)
+    _check(lib.lrt_camera_lookat(_ptr(_f32(eye)), _ptr(_f32(target)), _ptr(_f32(up)), fov, aspect, znear, zfar, _ptr(wv), _ptr(pr)),
+           "lrt_camera_lookat")
+    return wv, pr
 
 
 def orbit_camera(seed, index, distance=3.0, fov=60.0, aspect=1.0, znear=0.01, zfar=100.0):
@@ -353,6 +385,47 @@ class MultiRenderer:
 
     def SetLaunchBlocks(self, blocks):
         _check(lib.lrt_set_launch_blocks(self._h, blocks), "SetLaunchBlocks")
+
+
+class MultiRendererDR(MultiRenderer):
+    """dr::MultiRendererDR (diff_render/MultiRendererDR.h:71-155) on the GPU."""
+
+    def SetReference(self, images, world_views, projs, masks=None):
+        imgs = _f32(images)
+        n, H, W = imgs.shape[0], imgs.shape[1], imgs.shape[2]
+        m = None if masks is None else _f32(masks)
+        _check(lib.lrt_dr_set_reference(self._h, _ptr(imgs), _ptr(m), n, W, H, _ptr(_f32(world_views)), _ptr(_f32(projs))),
+               "SetReference")
+        self.ref_shape = (n, H, W)
+
+    def RenderDR(self, preset, view, seed=0, dtdv_mode=0, n_params=None):
+        """One RenderDR (MultiRendererDR.cpp:578-750): returns (image, dLoss/dS, loss, kernel_ms)."""
+        n, H, W = self.ref_shape
+        img = np.zeros((H, W, 4), np.float32)
+        g = None if n_params is None else np.zeros(n_params, np.float32)
+        loss, ms = C.c_float(), C.c_float()
+        _check(lib.lrt_dr_render_grad(self._h, C.byref(preset), view, seed, dtdv_mode, _ptr(img), _ptr(g), C.byref(loss),
+                                      C.byref(ms)), "RenderDR")
+        return img, g, loss.value, ms.value
+
+    def LastStats(self):
+        a, b = C.c_uint32(), C.c_uint32()
+        _check(lib.lrt_dr_last_stats(self._h, C.byref(a), C.byref(b)), "LastStats")
+        return a.value, b.value
+
+    def OptimizeFixedStructure(self, preset, sbs, seed=0, dtdv_mode=0):
+        """Runs preset.opt_iterations Adam steps; updates sbs.values_f in place; returns (losses, total_ms)."""
+        losses = np.zeros(preset.opt_iterations, np.float32)
+        ms = C.c_float()
+        _check(lib.lrt_dr_optimize(self._h, C.byref(preset), C.byref(sbs.header), _ptr(sbs.nodes), sbs.nodes.shape[0],
+                                   _ptr(sbs.values), sbs.values.size, _ptr(sbs.values_f), sbs.values_f.size, seed, dtdv_mode,
+                                   _ptr(losses), C.byref(ms)), "OptimizeFixedStructure")
+        return losses, ms.value
+
+    def getLastdLoss_dS(self, n):
+        out = np.zeros(n, np.float32)
+        _check(lib.lrt_dr_get_last_dloss_ds(self._h, _ptr(out), n), "getLastdLoss_dS")
+        return out
 
 
 def CreateMultiRenderer(device=DEVICE_GPU, max_primitives=10_000_000):

@@ -1,0 +1,288 @@
+// dr.hip - MultiRendererDR::RenderDR (diff_render/MultiRendererDR.cpp:578-750)
+// and OptimizeStepAdam (:828-853) on gfx950.
+//
+//   k_dr_primary      phase I: per-pixel primary rays, loss, dLoss/dS scatter
+//   k_dr_border_find  phase II: 3x3 alpha / depth-discontinuity border pixels
+//   k_dr_border_rays  phase III: border_spp jittered rays per border pixel,
+//                     relaxed-boundary (silhouette) term
+//   k_dr_scale        phase IV: getBaseGradientMult scaling
+//   k_adam            Adam step with clamp to +-1000
+// Gradients are scattered with hardware float atomics into one dLoss/dS
+// buffer (the reference keeps a buffer per CPU thread and sums them).
+#include "dr_device.hpp"
+#include "dr.hpp"
+
+namespace lrtd {
+
+__device__ __forceinline__ float lin_depth(float z) { return (z - 0.1f) / (10.0f - 0.1f); }
+
+__device__ __forceinline__ v3 dr_color(uint32_t mode, const DRHit& h) {  // MultiRendererDR.cpp:855-898
+  if (mode == LRT_DR_RENDER_MODE_DIFFUSE) return h.color;
+  if (mode == LRT_DR_RENDER_MODE_MASK) return v3(1, 1, 1);
+  float d = lin_depth(h.t);
+  return v3(d, d, d);
+}
+
+__device__ __forceinline__ float loss_grad1(uint32_t fn, float c, float r) {
+  return fn == LRT_DR_LOSS_FUNCTION_MSE ? 2.0f * (c - r) : sgn(c - r);
+}
+
+// t_samples_to_threshold (MultiRendererDR.cpp:997-1063)
+__device__ float samples_to_threshold(const float* s, unsigned count, float dv, unsigned dc) {
+  float mnv = s[0], mxv = s[0];
+  for (unsigned i = 1; i < count; i++) { mnv = fminf(mnv, s[i]); mxv = fmaxf(mxv, s[i]); }
+  if (dc > 0) { mnv = fminf(mnv, dv); mxv = fmaxf(mxv, dv); }
+  float diff = fmaxf(1e-9f, mxv - mnv);
+  unsigned g[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (unsigned i = 0; i < count; i++) g[min(max(int((s[i] - mnv) / diff * 10), 0), 9)]++;
+  g[min(max(int((dv - mnv) / diff * 10), 0), 9)] += dc;
+  int best_len = 0, best_start = -1, start = -1;
+  for (int i = 1; i < 10; i++) {
+    if (start != -1) {
+      if (g[i] != 0) {
+        int len = i - start;
+        if (len > best_len) { best_len = len; best_start = start; }
+        start = -1;
+      }
+    } else if (g[i] == 0) {
+      start = i;
+    }
+  }
+  if (best_len > 4) return mnv + diff * (best_start + 0.5f * best_len) / 10;
+  return 0.0f;
+}
+
+__device__ __forceinline__ void scatter_pd(const DRArgs& A, const DRPayload& py, v3 dLdC) {
+  const uint32_t f = A.Q.ray_flags;
+  if ((f & LRT_DR_RAY_FLAG_DDIFFUSE_DCOLOR) && A.mode != LRT_DR_RENDER_MODE_MASK) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (py.col_idx[k] == 0xFFFFFFFFu) continue;
+      float w = py.col_w[k];
+      atomicAdd(&A.grad[py.col_idx[k] + 0], dLdC.x * w);
+      atomicAdd(&A.grad[py.col_idx[k] + 1], dLdC.y * w);
+      atomicAdd(&A.grad[py.col_idx[k] + 2], dLdC.z * w);
+    }
+  }
+  if (f & LRT_DR_RAY_FLAG_DDIFFUSE_DPOS) {
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (py.pd_idx[k] != 0xFFFFFFFFu) atomicAdd(&A.grad[py.pd_idx[k]], dot(dLdC, py.pd_ddiffuse[k]));
+  } else if (f & LRT_DR_RAY_FLAG_DDIST_DPOS) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (py.pd_idx[k] == 0xFFFFFFFFu) continue;
+      float v = py.pd_ddist[k] / (10.0f - 0.1f);
+      atomicAdd(&A.grad[py.pd_idx[k]], dot(dLdC, v3(v, v, v)));
+    }
+  }
+}
+
+// Phase I for one pixel (MultiRendererDR.cpp:1084-1294). With spp > 1 the
+// colour is needed before any sample's gradient can be weighted, so the
+// samples are traced twice (first for colour, then with gradients); the
+// second pass recomputes identical rays and hits.
+__device__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, Stack& st, double& loss_acc) {
+  uint32_t x, y;
+  tid_to_xy(packed_idx, A.packed_w, x, y);
+  if (x >= A.W || y >= A.H) return;
+  const size_t pix = (size_t)y * A.W + x;
+  DRPayload py;
+  v3 res(0, 0, 0);
+  int hit_count = 0;
+  float samples[16];
+  const uint32_t spp = A.spp;
+  const DRParams Qc = {spp == 1 ? A.Q.ray_flags : 0u, A.Q.dtdv_fd};
+  const DRParams Qg = {A.Q.ray_flags, A.Q.dtdv_fd};
+  for (uint32_t s = 0; s < spp; s++) {
+    float2 d = rand2(x, y, s, A.seed);
+    v3 pos, dir;
+    eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
+    DRHit hit = dr_nearest_hit(A.S, Qc, pos, dir, 1e9f, &py, st);
+    if (hit.primId == 0xFFFFFFFFu) continue;
+    v3 c = dr_color(A.mode, hit);
+    res = res + v3(c.x / (float)spp, c.y / (float)spp, c.z / (float)spp);
+    if (hit_count < 16) samples[hit_count] = hit.t;
+    hit_count++;
+  }
+  const float cm[4] = {res.x, res.y, res.z, (float)hit_count / (float)spp};
+  A.out_image[pix] = make_float4(cm[0], cm[1], cm[2], cm[3]);
+  A.depth_thr[pix] = (hit_count == 0 || hit_count > 16) ? 0.0f : samples_to_threshold(samples, hit_count, 1e9f, spp - hit_count);
+  const float4 r = A.ref[pix];
+  const float rr[4] = {r.x, r.y, r.z, r.w};
+  float l = 0.0f;
+  for (int k = 0; k < 4; k++) {
+    float df = cm[k] - rr[k];
+    l += A.loss_fn == LRT_DR_LOSS_FUNCTION_MSE ? df * df : fabsf(df);
+  }
+  loss_acc += (double)l;
+  const v3 dLdC(loss_grad1(A.loss_fn, cm[0], rr[0]) / (float)spp, loss_grad1(A.loss_fn, cm[1], rr[1]) / (float)spp,
+                loss_grad1(A.loss_fn, cm[2], rr[2]) / (float)spp);
+  if (A.Q.ray_flags == 0) return;
+  if (spp == 1) {
+    if (hit_count) scatter_pd(A, py, dLdC);
+    return;
+  }
+  for (uint32_t s = 0; s < spp; s++) {
+    float2 d = rand2(x, y, s, A.seed);
+    v3 pos, dir;
+    eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
+    DRHit hit = dr_nearest_hit(A.S, Qg, pos, dir, 1e9f, &py, st);
+    if (hit.primId != 0xFFFFFFFFu) scatter_pd(A, py, dLdC);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_dr_primary(DRArgs A, uint32_t* work_counter, uint32_t* spill, double* loss) {
+  __shared__ uint32_t lds_stack[kStackLds * 256];
+  __shared__ uint32_t chunk_base;
+  Stack st;
+  st.lds = lds_stack + threadIdx.x;
+  st.lds_stride = 256;
+  st.lds_entries = kStackLds;
+  st.spill = spill + blockIdx.x * 256 + threadIdx.x;
+  st.spill_stride = gridDim.x * 256;
+  double acc = 0.0;
+  const uint32_t total = A.W * A.H;  // the reference walks the first W*H packed indices
+  for (;;) {
+    if (threadIdx.x == 0) chunk_base = atomicAdd(work_counter, 1u);
+    __syncthreads();
+    const uint64_t base = (uint64_t)chunk_base * 256;
+    __syncthreads();
+    if (base >= total) break;
+    const uint64_t i = base + threadIdx.x;
+    if (i < total) dr_pixel(A, (uint32_t)i, st, acc);
+  }
+  if (acc != 0.0) atomicAdd(loss, acc);
+}
+
+// Phase II (MultiRendererDR.cpp:609-685)
+__global__ void k_dr_border_find(DRArgs A, uint32_t* list, uint32_t* count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.W * A.H) return;
+  uint32_t x, y;
+  tid_to_xy(i, A.packed_w, x, y);
+  if (x >= A.W || y >= A.H) return;
+  const float d0 = A.out_image[(size_t)y * A.W + x].w;
+  float max_diff = 0.0f, max_thr = 0.0f;
+  if (x >= 1 && x < A.W - 1 && y >= 1 && y < A.H - 1) {
+    for (int dx = -1; dx <= 1; dx++)
+      for (int dy = -1; dy <= 1; dy++) {
+        size_t q = (size_t)(y + dy) * A.W + x + dx;
+        max_diff = fmaxf(max_diff, fabsf(d0 - A.out_image[q].w));
+        max_thr = fmaxf(max_thr, A.depth_thr[q]);
+      }
+  }
+  bool border = A.mode == LRT_DR_RENDER_MODE_MASK ? max_diff > 0 : (max_diff > 0 || max_thr > 0);
+  if (border || A.forced_border) list[atomicAdd(count, 1u)] = i;
+}
+
+// Phase III (MultiRendererDR.cpp:1319-1499): one lane per (border pixel, sample)
+__global__ __launch_bounds__(256) void k_dr_border_rays(DRArgs A, const uint32_t* list, const uint32_t* count, uint32_t* spill,
+                                                        uint32_t* hits) {
+  __shared__ uint32_t lds_stack[kStackLds * 256];
+  Stack st;
+  st.lds = lds_stack + threadIdx.x;
+  st.lds_stride = 256;
+  st.lds_entries = kStackLds;
+  st.spill = spill + blockIdx.x * 256 + threadIdx.x;
+  st.spill_stride = gridDim.x * 256;
+  const uint64_t n = (uint64_t)(*count) * A.border_spp;
+  const DRParams Qb = {LRT_DR_RAY_FLAG_BORDER, A.Q.dtdv_fd};
+  uint32_t local_hits = 0;
+  DRPayload py;
+  for (uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x; g < n; g += (uint64_t)gridDim.x * 256) {
+    const uint32_t item = (uint32_t)(g / A.border_spp), s = (uint32_t)(g % A.border_spp);
+    uint32_t x, y;
+    tid_to_xy(list[item], A.packed_w, x, y);
+    float2 d = rand2(x, y, s, A.seed);
+    v3 pos, dir;
+    eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
+    DRHit hit = dr_nearest_hit(A.S, Qb, pos, dir, 1e9f, &py, st);
+    if (!(py.missed_hit.sdf < A.relax_eps)) continue;
+    local_hits++;
+    float delta[4];
+    if (A.input_type == LRT_DR_INPUT_TYPE_LINEAR_DEPTH) {
+      float din = lin_depth(py.missed_hit.t);
+      float dout = hit.primId == 0xFFFFFFFFu ? 0.0f : lin_depth(hit.t);
+      delta[0] = delta[1] = delta[2] = din - dout;
+    } else {
+      v3 fin = dr_color(A.mode, py.missed_hit);
+      v3 fout = hit.primId == 0xFFFFFFFFu ? v3(0, 0, 0) : dr_color(A.mode, hit);
+      delta[0] = fin.x - fout.x; delta[1] = fin.y - fout.y; delta[2] = fin.z - fout.z;
+    }
+    delta[3] = hit.primId == 0xFFFFFFFFu ? 1.0f : 0.0f;
+    const float pdf = 1.0f / (float)A.border_spp;
+    for (int j = 0; j < 8; j++) {
+      if (py.missed_idx[j] == 0xFFFFFFFFu) continue;
+      float bd = mx(0.0f, py.missed_hit.sdf);
+      v3 ystar = pos + py.missed_hit.t * dir;
+      v3 xstar = ystar - bd * py.missed_hit.normal;
+      float ps[4];
+      mul4(A.proj_view, xstar.x, xstar.y, xstar.z, 1.0f, ps);
+      float sx = 0.5f * (ps[0] / ps[3] + 1.0f), sy = 1.0f - 0.5f * (ps[1] / ps[3] + 1.0f);
+      // the reference converts to unsigned without a range check; clamped here
+      float fx = floorf(sx * (float)A.W), fy = floorf(sy * (float)A.H);
+      uint32_t xp = (uint32_t)fminf(fmaxf(fx, 0.0f), (float)(A.W - 1)), yp = (uint32_t)fminf(fmaxf(fy, 0.0f), (float)(A.H - 1));
+      size_t q = (size_t)yp * A.W + xp;
+      float4 c = A.out_image[q], r = A.ref[q];
+      float dd = loss_grad1(A.loss_fn, c.x, r.x) * delta[0] + loss_grad1(A.loss_fn, c.y, r.y) * delta[1] +
+                 loss_grad1(A.loss_fn, c.z, r.z) * delta[2] + loss_grad1(A.loss_fn, c.w, r.w) * delta[3];
+      float diff = pdf * (1.0f / A.relax_eps) * dd * py.missed_dsdf[j];
+      atomicAdd(&A.grad[py.missed_idx[j]], diff);
+    }
+  }
+  if (local_hits) atomicAdd(hits, local_hits);
+}
+
+__global__ void k_dr_scale(float* grad, uint32_t n, float mult) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) grad[i] *= mult;
+}
+
+// MultiRendererDR.cpp:828-853; c1 = 1 - beta1^(iter+1), c2 = 1 - beta2^(iter+1)
+// in double as the reference evaluates them.
+__global__ void k_adam(const float* __restrict__ g, float* X, float* V, float* Sq, uint32_t n, float lr, float b1, float b2, float eps,
+                       double c1, double c2, float batch) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float gi = g[i] / batch;  // m_dLoss_dS_tmp[j] /= image_batch_size, MultiRendererDR.cpp:474-475
+  float v = b1 * V[i] + (1 - b1) * gi;
+  V[i] = v;
+  float vh = (float)((double)v / c1);
+  float s = b2 * Sq[i] + (1 - b2) * gi * gi;
+  Sq[i] = s;
+  float sh = (float)((double)s / c2);
+  float x = X[i] - lr * vh / (sqrtf(sh) + eps);
+  X[i] = fminf(fmaxf(x, -1000.0f), 1000.0f);
+}
+
+}  // namespace lrtd
+
+namespace lrt {
+
+hipError_t launch_dr_primary(const lrtd::DRArgs& A, uint32_t* d_counter, uint32_t* d_spill, double* d_loss, uint32_t blocks,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(lrtd::k_dr_primary, dim3(blocks), dim3(256), 0, s, A, d_counter, d_spill, d_loss);
+  return hipGetLastError();
+}
+hipError_t launch_dr_border_find(const lrtd::DRArgs& A, uint32_t* d_list, uint32_t* d_count, hipStream_t s) {
+  uint32_t n = A.W * A.H;
+  hipLaunchKernelGGL(lrtd::k_dr_border_find, dim3((n + 255) / 256), dim3(256), 0, s, A, d_list, d_count);
+  return hipGetLastError();
+}
+hipError_t launch_dr_border_rays(const lrtd::DRArgs& A, const uint32_t* d_list, const uint32_t* d_count, uint32_t* d_spill,
+                                 uint32_t* d_hits, uint32_t blocks, hipStream_t s) {
+  hipLaunchKernelGGL(lrtd::k_dr_border_rays, dim3(blocks), dim3(256), 0, s, A, d_list, d_count, d_spill, d_hits);
+  return hipGetLastError();
+}
+hipError_t launch_dr_scale(float* d_grad, uint32_t n, float mult, hipStream_t s) {
+  hipLaunchKernelGGL(lrtd::k_dr_scale, dim3((n + 255) / 256), dim3(256), 0, s, d_grad, n, mult);
+  return hipGetLastError();
+}
+hipError_t launch_adam(const float* g, float* X, float* V, float* S, uint32_t n, float lr, float b1, float b2, float eps, double c1,
+                       double c2, float batch, hipStream_t s) {
+  hipLaunchKernelGGL(lrtd::k_adam, dim3((n + 255) / 256), dim3(256), 0, s, g, X, V, S, n, lr, b1, b2, eps, c1, c2, batch);
+  return hipGetLastError();
+}
+
+}  // namespace lrt

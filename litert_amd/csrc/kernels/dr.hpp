@@ -1,0 +1,37 @@
+// dr.hpp - launch interface of the differentiable-rendering kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "render.hpp"
+
+namespace lrtd {
+
+struct DRParams;
+
+struct DRArgs {
+  DevScene S;
+  Camera cam;
+  float proj_view[16];  // proj * worldView (TransformWorldToScreenSpace, MultiRendererDR.cpp:1296-1301)
+  struct {
+    uint32_t ray_flags;
+    uint32_t dtdv_fd;
+  } Q;
+  uint32_t W, H, packed_w, spp, seed;
+  uint32_t mode, input_type, loss_fn, border_spp, forced_border;
+  float relax_eps;
+  const float4* ref;   // reference image, alpha = mask
+  float4* out_image;   // rendered colour + coverage
+  float* depth_thr;    // per-pixel depth-discontinuity threshold
+  float* grad;         // dLoss/dS, one float per values_f entry
+};
+
+}  // namespace lrtd
+
+namespace lrt {
+hipError_t launch_dr_primary(const lrtd::DRArgs& A, uint32_t* d_counter, uint32_t* d_spill, double* d_loss, uint32_t blocks, hipStream_t s);
+hipError_t launch_dr_border_find(const lrtd::DRArgs& A, uint32_t* d_list, uint32_t* d_count, hipStream_t s);
+hipError_t launch_dr_border_rays(const lrtd::DRArgs& A, const uint32_t* d_list, const uint32_t* d_count, uint32_t* d_spill,
+                                 uint32_t* d_hits, uint32_t blocks, hipStream_t s);
+hipError_t launch_dr_scale(float* d_grad, uint32_t n, float mult, hipStream_t s);
+hipError_t launch_adam(const float* g, float* X, float* V, float* S, uint32_t n, float lr, float b1, float b2, float eps, double c1,
+                       double c2, float batch, hipStream_t s);
+}  // namespace lrt

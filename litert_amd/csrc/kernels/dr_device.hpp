@@ -1,0 +1,324 @@
+// dr_device.hpp - differentiable SBS ray casting (forward with payload) on gfx950.
+//
+// Same steps as BVHDR::RayQuery_NearestHitWithGrad / BVH2TraverseF32WithGrad
+// / OctreeBrickIntersectWithGrad / Intersect (diff_render/BVH2DR.cpp:12-190,
+// 623-1117, 1130-1610) for the Newton root find that MultiRendererDR forces,
+// indexed (ID32F) layouts, and the partial derivatives of depth and diffuse
+// colour. dt/dv is either the implicit-function-theorem derivative of the
+// root (default) or the reference's central difference of the root find.
+#pragma once
+#include "traverse_device.hpp"
+
+namespace lrtd {
+
+struct DRHit {
+  float t;
+  uint32_t primId;
+  v3 color;
+  float sdf;
+  v3 normal;
+  uint32_t check_vox_border;
+};
+
+struct DRPayload {
+  DRHit missed_hit, missed_cand;
+  uint32_t missed_idx[8];
+  float missed_dsdf[8];
+  uint32_t pd_idx[8];    // distance parameters of the hit voxel
+  float pd_ddist[8];     // d(tReal)/dS
+  v3 pd_ddiffuse[8];     // dDiffuse/dS
+  uint32_t col_idx[8];   // colour parameter base indices of the hit brick
+  float col_w[8];        // dDiffuse/dColour (trilinear weight)
+};
+
+struct DRParams {
+  uint32_t ray_flags;
+  uint32_t dtdv_fd;  // 1 = central difference (reference), 0 = analytic
+};
+static_assert(sizeof(DRParams) == 8, "DRArgs::Q layout");
+
+// BVH2DR.cpp:623-1073, Newton branch. Returns hit ? t : qFar + EPS.
+__device__ __forceinline__ float dr_intersect(uint32_t ray_flags, v3 ray_dir, const float values[8], float d, float d_inv, float qFar,
+                                              v3 start_q, DRPayload* relax) {
+  const float EPS = 1e-6f, SDF_EPS = 1e-7f;
+  float t = 0.0f;
+  bool hit = false;
+  if (relax) relax->missed_cand.sdf = 1000.f;
+  float start_dist = trilinear(values, start_q + t * ray_dir);
+  if (start_dist <= SDF_EPS) {
+    hit = true;
+  } else {
+    float s000 = values[0] * d_inv, s001 = values[1] * d_inv, s010 = values[2] * d_inv, s011 = values[3] * d_inv;
+    float s100 = values[4] * d_inv, s101 = values[5] * d_inv, s110 = values[6] * d_inv, s111 = values[7] * d_inv;
+    float a = s101 - s001;
+    float k0 = s000, k1 = s100 - s000, k2 = s010 - s000, k3 = s110 - s010 - k1, k4 = k0 - s001, k5 = k1 - a;
+    float k6 = k2 - (s011 - s001), k7 = k3 - (s111 - s011 - a);
+    v3 o = start_q, d3 = ray_dir;
+    float m0 = o.x * o.y, m1 = d3.x * d3.y, m2 = o.x * d3.y + o.y * d3.x;
+    float m3 = k5 * o.z - k1, m4 = k6 * o.z - k2, m5 = k7 * o.z - k3;
+    float c0 = (k4 * o.z - k0) + o.x * m3 + o.y * m4 + m0 * m5;
+    float c1 = d3.x * m3 + d3.y * m4 + m2 * m5 + d3.z * (k4 + k5 * o.x + k6 * o.y + k7 * m0);
+    float c2 = m1 * m5 + d3.z * (k5 * d3.x + k6 * d3.y + k7 * m2);
+    float c3 = k7 * m1 * d3.z;
+    float qa = 3 * c3, qb = 2 * c2, qc = c1;
+    float t0 = 0, t1 = qFar, t2 = qFar, t3 = qFar;
+    float D = qb * qb - 4 * qa * qc;
+    if (D >= 0) {
+      float q = -0.5f * (qb + sgn(qb) * sqrtf(D));
+      t1 = fabsf(qa) > EPS ? q / qa : t0;
+      t2 = fabsf(q) > EPS ? qc / q : qFar;
+      float tmp = mn(t1, t2);
+      t2 = mx(t1, t2);
+      t1 = tmp;
+      t1 = clampf_(t1, t0, t3);
+      t2 = clampf_(t2, t0, t3);
+    }
+    float sdf0 = c0;
+    float sdf1 = (c0 + t1 * (c1 + t1 * (c2 + t1 * c3)));
+    float sdf2 = (c0 + t2 * (c1 + t2 * (c2 + t2 * c3)));
+    float sdf3 = (c0 + t3 * (c1 + t3 * (c2 + t3 * c3)));
+    bool s0 = sdf0 > 0, s1 = sdf1 > 0, s2 = sdf2 > 0, s3 = sdf3 > 0;
+    float nwt_min = t0, nwt_max = t0;
+    if (s0 != s1) { nwt_min = t0; nwt_max = t1; }
+    else if (s1 != s2) { nwt_min = t1; nwt_max = t2; }
+    else if (s2 != s3) { nwt_min = t2; nwt_max = t3; }
+    if (nwt_min < nwt_max) {
+      unsigned it = 0;
+      float rtn = 0.5f * (nwt_min + nwt_max);
+      float f = 1000;
+      while (it < 10u && fabsf(f) >= SDF_EPS) {
+        f = c0 + rtn * (c1 + rtn * (c2 + rtn * c3));
+        float df = c1 + rtn * (2 * c2 + rtn * 3 * c3);
+        float dx = f / (df + sgn(df) * 1e-9f);
+        rtn -= dx;
+        it++;
+      }
+      t = rtn;
+      hit = (t >= 0 && t <= qFar && fabsf(f) < SDF_EPS);
+    }
+    if (relax && (ray_flags & LRT_DR_RAY_FLAG_BORDER)) {  // BVH2DR.cpp:960-1035
+      float t_min = qFar + EPS;
+      if (fabsf(qa) > EPS) {
+        float root1 = -0.5f * (qb + sqrtf(D)) / qa, root2 = -0.5f * (qb - sqrtf(D)) / qa;
+        t_min = (sgn(qa) > 0) == (root1 < root2) ? root1 : root2;
+      } else if (qb < -EPS) {
+        t_min = -qc / qb;
+      }
+      float sdf_min = (c0 + t_min * (c1 + t_min * (c2 + t_min * c3)));
+      DRHit& mh = relax->missed_hit;
+      if (mh.check_vox_border == 1) {
+        if (c1 <= 0) {
+          if (sdf0 < 0.f && -sdf0 * d < mh.sdf) { mh.t = t0; mh.sdf = -sdf0 * d; }
+        }
+        mh.check_vox_border = 0;
+      }
+      if (t_min >= 0 && t_min <= qFar && sdf_min < 0.f && -sdf_min * d < mh.sdf) { mh.t = t_min; mh.sdf = -sdf_min * d; }
+      if (sdf3 < 0.f && -sdf3 * d < mh.sdf && (c1 + qFar * (2 * c2 + qFar * 3 * c3)) >= 0.f) {
+        mh.check_vox_border = 1;
+        relax->missed_cand.t = qFar;
+        relax->missed_cand.sdf = -sdf3 * d;
+      }
+    }
+  }
+  return hit ? t : qFar + EPS;
+}
+
+__device__ __forceinline__ float corner_w(int i, v3 q) {
+  return ((i & 4) ? q.x : 1 - q.x) * ((i & 2) ? q.y : 1 - q.y) * ((i & 1) ? q.z : 1 - q.z);
+}
+
+// BVH2DR.cpp:1130-1610 for ID32F layouts without normal derivatives.
+__device__ __forceinline__ void dr_brick(const DevScene& S, const DRParams& Q, v3 ray_pos, v3 ray_dir, v3 inv, float tNear, uint32_t brick,
+                                         DRPayload* relax, DRHit* pHit) {
+  float values[8];
+  uint32_t idx[8];
+  const uint4 N = S.nodes[brick];
+  const float px = (float)(N.x >> 16), py = (float)(N.x & 0xFFFFu), pz = (float)(N.y >> 16), sz = (float)(N.y & 0xFFFFu);
+  const float bsf = (float)S.brick_size;
+  const float sz_inv = 2.0f / sz;
+  const float d = 2.0f / (sz * bsf);
+  const float d_inv = 1.0f / d;
+  const v3 bmin = v3(-1, -1, -1) + sz_inv * v3(px, py, pz);
+  const v3 bmax = bmin + sz_inv * v3(1, 1, 1);
+  const float qscale = 0.5f * sz * bsf;
+  const uint32_t vs = S.v_size;
+  float2 bnf = RayBox(ray_pos, inv, bmin, bmax);
+  const float old_t = pHit->t;
+  const uint32_t max_steps = 16u * (S.brick_size + 1u) + 16u;
+  uint32_t steps = 0;
+  const uint32_t flags = Q.ray_flags;
+  while (bnf.x < bnf.y && pHit->t == old_t && steps++ < max_steps) {
+    v3 hit_pos = ray_pos + bnf.x * ray_dir;
+    v3 local = (hit_pos - bmin) * qscale;
+    const float hi = bsf - 1e-6f;
+    v3 vp(floorf(clampf_(local.x, 1e-6f, hi)), floorf(clampf_(local.y, 1e-6f, hi)), floorf(clampf_(local.z, 1e-6f, hi)));
+    const uint32_t vx = (uint32_t)vp.x, vy = (uint32_t)vp.y, vz = (uint32_t)vp.z;
+    v3 min_pos = bmin + d * vp;
+    v3 max_pos = min_pos + d * v3(1, 1, 1);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      uint32_t vId = (vx + ((i & 4) >> 2)) * vs * vs + (vy + ((i & 2) >> 1)) * vs + (vz + (i & 1));
+      idx[i] = S.values[N.z + vId];
+      values[i] = S.values_f[idx[i]];
+    }
+    float2 fnf = RayBox(ray_pos, inv, min_pos, max_pos);
+    if (tNear < fnf.x) {
+      v3 start_pos = ray_pos + fnf.x * ray_dir;
+      v3 start_q = (start_pos - min_pos) * qscale;
+      float qFar = (fnf.y - fnf.x) * qscale;
+      float prev_sdf = relax->missed_hit.sdf;
+      float t = dr_intersect(flags, ray_dir, values, d, d_inv, qFar, start_q, relax);
+      float tReal = fnf.x + d * t;
+      if (flags & LRT_DR_RAY_FLAG_BORDER) {  // BVH2DR.cpp:1223-1275
+        if (relax->missed_hit.sdf < prev_sdf && relax->missed_hit.t < t) {
+          float missed_t = relax->missed_hit.t;
+          relax->missed_hit.t = fnf.x + d * relax->missed_hit.t;
+          v3 q_ast = start_q + missed_t * ray_dir;
+          v3 y_ast = ray_pos + (fnf.x + (d * missed_t)) * ray_dir;
+          v3 dp_ast = (y_ast - bmin) * (0.5f * sz);
+          v3 g = trilinear_diff(values, q_ast) * (1.0f / d);
+          v3 mul(fabsf(q_ast.x - 0.5f) > 0.5f - 1e-5f ? 0.0f : 1.0f, fabsf(q_ast.y - 0.5f) > 0.5f - 1e-5f ? 0.0f : 1.0f,
+                 fabsf(q_ast.z - 0.5f) > 0.5f - 1e-5f ? 0.0f : 1.0f);
+          g = g * mul;
+          float glen = sqrtf(dot(g, g));
+          if (glen > 1e-9f) {
+            const uint32_t t_off = N.z + vs * vs * vs;
+            v3 col(0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+              relax->missed_dsdf[i] = -corner_w(i, q_ast) / glen;
+              uint32_t ci = S.values[t_off + i];
+              col = col + corner_w(i, dp_ast) * v3(S.values_f[ci], S.values_f[ci + 1], S.values_f[ci + 2]);
+              relax->missed_idx[i] = idx[i];
+            }
+            relax->missed_hit.color = col;
+            relax->missed_hit.normal = v3(g.x / glen, g.y / glen, g.z / glen);
+          }
+        }
+      }
+      if (t <= qFar && tReal < pHit->t) {  // BVH2DR.cpp:1277-1600
+        pHit->t = tReal;
+        pHit->primId = brick;
+        float dtdv[8];
+        if (flags & (LRT_DR_RAY_FLAG_DDIFFUSE_DPOS | LRT_DR_RAY_FLAG_DDIST_DPOS)) {
+          if (Q.dtdv_fd) {  // BVH2DR.cpp:1075-1090
+            float vv[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) vv[i] = values[i];
+            const float delta = 0.001f;
+            for (int i = 0; i < 8; i++) {
+              vv[i] += delta;
+              float d1 = dr_intersect(flags, ray_dir, vv, d, d_inv, qFar, start_q, nullptr);
+              vv[i] -= 2 * delta;
+              float d2 = dr_intersect(flags, ray_dir, vv, d, d_inv, qFar, start_q, nullptr);
+              vv[i] += delta;
+              dtdv[i] = (d1 - d2) / (2 * delta);
+            }
+          } else {  // dt/dv_i = -w_i(q*) / (grad f(q*) . dir)
+            v3 q = start_q + t * ray_dir;
+            float fd = dot(trilinear_diff(values, q), ray_dir);
+#pragma unroll
+            for (int i = 0; i < 8; i++) dtdv[i] = fd != 0.0f ? -corner_w(i, q) / fd : 0.0f;
+          }
+#pragma unroll
+          for (int i = 0; i < 8; i++) {
+            relax->pd_idx[i] = idx[i];
+            relax->pd_ddist[i] = d * dtdv[i];
+          }
+        }
+        v3 pos = ray_pos + pHit->t * ray_dir;
+        v3 dp = (pos - bmin) * (0.5f * sz);
+        const uint32_t t_off = N.z + vs * vs * vs;
+        v3 col(0, 0, 0), J0(0, 0, 0), J1(0, 0, 0), J2(0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          uint32_t ci = S.values[t_off + i];
+          v3 c(S.values_f[ci], S.values_f[ci + 1], S.values_f[ci + 2]);
+          float wx = (i & 4) ? dp.x : 1 - dp.x, wy = (i & 2) ? dp.y : 1 - dp.y, wz = (i & 1) ? dp.z : 1 - dp.z;
+          col = col + (wx * wy * wz) * c;
+          if (flags & LRT_DR_RAY_FLAG_DDIFFUSE_DPOS) {
+            float ax = (i & 4) ? 1.0f : -1.0f, ay = (i & 2) ? 1.0f : -1.0f, az = (i & 1) ? 1.0f : -1.0f;
+            J0 = J0 + (ax * wy * wz) * c;
+            J1 = J1 + (wx * ay * wz) * c;
+            J2 = J2 + (wx * wy * az) * c;
+          }
+          relax->col_idx[i] = ci;
+          relax->col_w[i] = wx * wy * wz;
+        }
+        pHit->color = col;
+        if (flags & LRT_DR_RAY_FLAG_DDIFFUSE_DPOS) {
+#pragma unroll
+          for (int i = 0; i < 8; i++) {
+            v3 v(0.5f * sz * ray_dir.x * dtdv[i], 0.5f * sz * ray_dir.y * dtdv[i], 0.5f * sz * ray_dir.z * dtdv[i]);
+            relax->pd_ddiffuse[i] = J0 * v.x + J1 * v.y + J2 * v.z;
+          }
+        }
+        break;
+      }
+    }
+    bnf.x += mx(0.0f, fnf.y - bnf.x) + 1e-6f;
+  }
+}
+
+// BVH2DR.cpp:12-167 (stop at the first hit, forced in the reference)
+__device__ __forceinline__ DRHit dr_nearest_hit(const DevScene& S, const DRParams& Q, v3 pos, v3 dir, float tFar, DRPayload* relax,
+                                                Stack& st) {
+  if (tFar <= 0.0f) tFar *= -1.0f;
+  const v3 inv = SafeInverse(dir);
+  DRHit hit;
+  hit.t = tFar;
+  hit.primId = 0xFFFFFFFFu;
+  hit.color = v3(0, 0, 0);
+  hit.sdf = 1000.0f;
+  hit.normal = v3(1, 0, 0);
+  hit.check_vox_border = 0;
+  relax->missed_hit = hit;
+  relax->missed_hit.check_vox_border = 1;
+  relax->missed_cand = relax->missed_hit;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    relax->missed_idx[i] = 0xFFFFFFFFu;
+    relax->missed_dsdf[i] = 0.0f;
+    relax->pd_idx[i] = 0xFFFFFFFFu;
+    relax->pd_ddist[i] = 0.0f;
+    relax->pd_ddiffuse[i] = v3(0, 0, 0);
+    relax->col_idx[i] = 0xFFFFFFFFu;
+    relax->col_w[i] = 0.0f;
+  }
+  const float2 bh = RayBox(pos, inv, v3(-1, -1, -1), v3(1, 1, 1));
+  if ((bh.x <= bh.y) && (bh.y > 0.0f) && (bh.x < hit.t)) {
+    const float tNear = 0.0f;
+    int top = 0;
+    uint32_t left = 0;
+    while (top >= 0 && hit.primId == 0xFFFFFFFFu) {
+      while (top >= 0 && ((left & LRT_LEAF_BIT) == 0)) {
+        const uint4* pp = S.pairs + 4 * (size_t)left;
+        const uint4 a = pp[0], b = pp[1], c = pp[2], e = pp[3];
+        const float2 tm0 = RayBox(pos, inv, v3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)),
+                                  v3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z)));
+        const float2 tm1 = RayBox(pos, inv, v3(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z)),
+                                  v3(__uint_as_float(e.x), __uint_as_float(e.y), __uint_as_float(e.z)));
+        const bool h0 = (tm0.x <= tm0.y) && (tm0.y >= tNear) && (tm0.x <= hit.t);
+        const bool h1 = (tm1.x <= tm1.y) && (tm1.y >= tNear) && (tm1.x <= hit.t);
+        left = h0 ? a.w : c.w;
+        if (h0 && h1) {
+          const bool near0 = tm0.x <= tm1.x;
+          left = near0 ? a.w : c.w;
+          st.put(top, near0 ? c.w : a.w);
+          top++;
+        }
+        if (!h0 && !h1) {
+          top--;
+          left = st.get(top > 0 ? top : 0);
+        }
+      }
+      if (top >= 0 && left != 0xFFFFFFFFu) dr_brick(S, Q, pos, dir, inv, mx(tNear, 0.1f), left & LRT_START_MASK, relax, &hit);
+      top--;
+      left = st.get(top > 0 ? top : 0);
+    }
+  }
+  if (relax->missed_cand.sdf < relax->missed_hit.sdf) relax->missed_hit = relax->missed_cand;
+  return hit;
+}
+
+}  // namespace lrtd

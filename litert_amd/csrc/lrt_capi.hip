@@ -543,6 +543,56 @@ int lrt_sbs_free(lrt_sbs_handle s) {
   return LRT_OK;
 }
 
+int lrt_sbs_load_file(const char* path, lrt_sbs_handle* out) {
+  if (!path || !out) return set_err(LRT_E_INVALID, "null argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return set_err(LRT_E_INVALID, std::string("cannot open ") + path);
+  std::unique_ptr<lrt_sbs> s(new lrt_sbs);
+  uint32_t n = 0, cnt = 0;
+  bool ok = std::fread(&s->s.header, sizeof(SdfSBSHeader), 1, f) == 1 && std::fread(&n, 4, 1, f) == 1 && n < (1u << 28);
+  if (ok) {
+    s->s.nodes.resize(n);
+    ok = (n == 0 || std::fread(s->s.nodes.data(), sizeof(SdfSBSNode), n, f) == n) && std::fread(&cnt, 4, 1, f) == 1 && cnt < (1u << 30);
+  }
+  if (ok) {
+    s->s.values.resize(cnt);
+    ok = cnt == 0 || std::fread(s->s.values.data(), 4, cnt, f) == cnt;
+  }
+  uint32_t nf = 0;
+  if (ok && std::fread(&nf, 4, 1, f) == 1 && nf > 0 && nf < (1u << 30)) {
+    s->s.values_f.resize(nf);
+    ok = std::fread(s->s.values_f.data(), 4, nf, f) == nf;
+  }
+  std::fclose(f);
+  if (!ok) return set_err(LRT_E_INVALID, std::string("truncated or invalid SBS file ") + path);
+  *out = s.release();
+  return LRT_OK;
+}
+
+int lrt_sbs_save_file(const char* path, const SdfSBSHeader* header, const SdfSBSNode* nodes, uint32_t n_nodes, const uint32_t* values,
+                      uint32_t n_values, const float* values_f, uint32_t n_values_f) {
+  if (!path || !header || (!nodes && n_nodes) || (!values && n_values)) return set_err(LRT_E_INVALID, "null argument");
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return set_err(LRT_E_INVALID, std::string("cannot write ") + path);
+  bool ok = std::fwrite(header, sizeof(SdfSBSHeader), 1, f) == 1 && std::fwrite(&n_nodes, 4, 1, f) == 1 &&
+            (n_nodes == 0 || std::fwrite(nodes, sizeof(SdfSBSNode), n_nodes, f) == n_nodes) && std::fwrite(&n_values, 4, 1, f) == 1 &&
+            (n_values == 0 || std::fwrite(values, 4, n_values, f) == n_values);
+  if (ok && values_f && n_values_f)
+    ok = std::fwrite(&n_values_f, 4, 1, f) == 1 && std::fwrite(values_f, 4, n_values_f, f) == n_values_f;
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? LRT_OK : set_err(LRT_E_INVALID, std::string("write failed ") + path);
+}
+
+int lrt_camera_lookat(const float eye[3], const float target[3], const float up[3], float fov_deg, float aspect, float znear, float zfar,
+                      float world_view[16], float proj[16]) {
+  if (!eye || !target || !up || !world_view || !proj) return set_err(LRT_E_INVALID, "null argument");
+  m4 v = look_at(f3(eye[0], eye[1], eye[2]), f3(target[0], target[1], target[2]), f3(up[0], up[1], up[2]));
+  m4 p = perspective(fov_deg, aspect, znear, zfar);
+  std::memcpy(world_view, v.m, 64);
+  std::memcpy(proj, p.m, 64);
+  return LRT_OK;
+}
+
 int lrt_orbit_camera(uint32_t seed, uint32_t index, float distance, float fov_deg, float aspect, float znear, float zfar,
                      float world_view[16], float proj[16]) {
   if (!world_view || !proj) return set_err(LRT_E_INVALID, "null output");

@@ -1,0 +1,227 @@
+// lrt_dr_capi.hip - C ABI of the differentiable renderer (dr::MultiRendererDR).
+#include <hip/hip_runtime.h>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/lrt.h"
+#include "kernels/dr.hpp"
+#include "lrt_internal.hpp"
+
+namespace lrt {
+
+static uint32_t pcg_hash(uint32_t v) {
+  uint32_t state = v * 747796405u + 2891336453u;
+  uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return (word >> 22u) ^ word;
+}
+
+// CastRayWithGrad ray flags, MultiRendererDR.cpp:1119-1142
+static uint32_t dr_ray_flags(const MultiRendererDRPreset& p) {
+  if (p.dr_diff_mode == LRT_DR_DIFF_MODE_FINITE_DIFF) return LRT_DR_RAY_FLAG_NO_DIFF;
+  if (p.dr_input_type == LRT_DR_INPUT_TYPE_LINEAR_DEPTH) return LRT_DR_RAY_FLAG_DDIST_DPOS;
+  uint32_t f = LRT_DR_RAY_FLAG_NO_DIFF;
+  if (p.dr_reconstruction_flags & LRT_DR_RECONSTRUCTION_FLAG_COLOR) f |= LRT_DR_RAY_FLAG_DDIFFUSE_DCOLOR;
+  if ((p.dr_reconstruction_flags & LRT_DR_RECONSTRUCTION_FLAG_GEOMETRY) && p.dr_render_mode == LRT_DR_RENDER_MODE_DIFFUSE)
+    f |= LRT_DR_RAY_FLAG_DDIFFUSE_DPOS;
+  return f;
+}
+
+static int dr_check(Renderer* r, const MultiRendererDRPreset& p) {
+  if (!r->has_scene) return set_err(LRT_E_NOSCENE, "no scene");
+  const uint32_t lay = r->header.aux_data & LRT_SBS_LAYOUT_MASK;
+  if (lay != LRT_SBS_LAYOUT_ID32F_IRGB32F && lay != LRT_SBS_LAYOUT_ID32F_IRGB32F_IN)
+    return set_err(LRT_E_UNSUPPORTED, "differentiable rendering needs an indexed ID32F_IRGB32F(_IN) layout (MultiRendererDR.cpp:283-284)");
+  if (r->header.brick_pad != 0) return set_err(LRT_E_UNSUPPORTED, "indexed DR layouts have brick_pad 0");
+  if (p.dr_render_mode != LRT_DR_RENDER_MODE_LINEAR_DEPTH && p.dr_render_mode != LRT_DR_RENDER_MODE_MASK &&
+      p.dr_render_mode != LRT_DR_RENDER_MODE_DIFFUSE)
+    return set_err(LRT_E_UNSUPPORTED, "DR render modes LAMBERT and NORMAL (normal derivatives) are not built");
+  if (p.dr_loss_function > LRT_DR_LOSS_FUNCTION_MAE) return set_err(LRT_E_INVALID, "bad loss function");
+  if (p.dr_border_sampling != LRT_DR_BORDER_SAMPLING_RANDOM) return set_err(LRT_E_UNSUPPORTED, "SVM border sampling is not built");
+  if (p.dr_diff_mode != LRT_DR_DIFF_MODE_DEFAULT) return set_err(LRT_E_UNSUPPORTED, "finite-difference DR mode is not built");
+  if (p.spp < 1 || p.spp > 16) return set_err(LRT_E_INVALID, "DR spp must be 1..16");
+  if (p.border_relax_eps <= 0.f) return set_err(LRT_E_INVALID, "border_relax_eps must be > 0");
+  if (r->dr_n_ref == 0) return set_err(LRT_E_INVALID, "no reference images: call lrt_dr_set_reference");
+  if ((p.render_width && p.render_width != r->dr_w) || (p.render_height && p.render_height != r->dr_h))
+    return set_err(LRT_E_UNSUPPORTED, "render size must equal the reference image size");
+  return LRT_OK;
+}
+
+// one RenderDR call; gradient left in d_dr_grad (scaled)
+static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t view, uint32_t seed, int dtdv_mode, float* kernel_ms) {
+  int c = dr_check(r, p);
+  if (c) return c;
+  if (view >= r->dr_n_ref) return set_err(LRT_E_INVALID, "view out of range");
+  hipError_t e;
+  if ((e = hipSetDevice(r->device)) != hipSuccess) return hip_err(e, "hipSetDevice");
+  const uint32_t W = r->dr_w, H = r->dr_h;
+  const size_t np = (size_t)W * H;
+  const uint32_t P = (uint32_t)r->values_f.size();
+  if ((e = r->d_dr_img.ensure(np * 16)) != hipSuccess) return hip_err(e, "alloc");
+  if ((e = r->d_dr_thr.ensure(np * 4)) != hipSuccess) return hip_err(e, "alloc");
+  if ((e = r->d_dr_grad.ensure((size_t)P * 4)) != hipSuccess) return hip_err(e, "alloc");
+  if ((e = r->d_dr_list.ensure(np * 4)) != hipSuccess) return hip_err(e, "alloc");
+  if ((e = r->d_dr_misc.ensure(64)) != hipSuccess) return hip_err(e, "alloc");
+  const uint32_t blocks = r->grid_blocks();
+  if ((e = r->ensure_spill(blocks)) != hipSuccess) return hip_err(e, "alloc spill");
+  uint32_t* misc = (uint32_t*)r->d_dr_misc.p;  // [0] work counter [1] border count [2] border hits [4..5] loss (double)
+  if ((e = zero_u32(misc, 16, r->stream)) != hipSuccess) return hip_err(e, "clear");
+  if ((e = zero_u32((uint32_t*)r->d_dr_grad.p, P, r->stream)) != hipSuccess) return hip_err(e, "clear grad");
+
+  lrtd::DRArgs A{};
+  A.S = r->dev_scene();
+  const float* wv = &r->dr_world_views[16 * view];
+  const float* pr = &r->dr_projs[16 * view];
+  A.cam = make_camera(wv, pr);
+  for (int cc = 0; cc < 4; cc++)
+    for (int rr = 0; rr < 4; rr++) {
+      float s = 0.f;
+      for (int k = 0; k < 4; k++) s += pr[k * 4 + rr] * wv[cc * 4 + k];
+      A.proj_view[cc * 4 + rr] = s;
+    }
+  A.Q.ray_flags = dr_ray_flags(p);
+  A.Q.dtdv_fd = dtdv_mode == 1 ? 1u : 0u;
+  A.W = W;
+  A.H = H;
+  A.packed_w = 8 * ((W + 7) / 8);
+  A.spp = p.spp;
+  A.seed = seed;
+  A.mode = p.dr_render_mode;
+  A.input_type = p.dr_input_type;
+  A.loss_fn = p.dr_loss_function;
+  A.border_spp = p.border_spp;
+  A.forced_border = p.debug_forced_border ? 1u : 0u;
+  A.relax_eps = p.border_relax_eps;
+  A.ref = (const float4*)r->d_dr_ref.p + np * view;
+  A.out_image = (float4*)r->d_dr_img.p;
+  A.depth_thr = (float*)r->d_dr_thr.p;
+  A.grad = (float*)r->d_dr_grad.p;
+
+  if ((e = hipEventRecord(r->ev0, r->stream)) != hipSuccess) return hip_err(e, "event");
+  if ((e = launch_dr_primary(A, misc, (uint32_t*)r->d_spill.p, (double*)(misc + 4), blocks, r->stream)) != hipSuccess)
+    return hip_err(e, "dr primary");
+  if ((e = launch_dr_border_find(A, (uint32_t*)r->d_dr_list.p, misc + 1, r->stream)) != hipSuccess) return hip_err(e, "dr border");
+  if ((p.dr_reconstruction_flags & LRT_DR_RECONSTRUCTION_FLAG_GEOMETRY) && p.border_spp > 0) {
+    if ((e = launch_dr_border_rays(A, (uint32_t*)r->d_dr_list.p, misc + 1, (uint32_t*)r->d_spill.p, misc + 2, blocks, r->stream)) != hipSuccess)
+      return hip_err(e, "dr border rays");
+  }
+  // getBaseGradientMult, MultiRendererDR.cpp:140-145
+  const float g_mult = (float)(std::pow((double)P + 1.0, 2.0 / 3.0) / (double)(W * H));
+  if ((e = launch_dr_scale((float*)r->d_dr_grad.p, P, g_mult, r->stream)) != hipSuccess) return hip_err(e, "dr scale");
+  if ((e = hipEventRecord(r->ev1, r->stream)) != hipSuccess) return hip_err(e, "event");
+  uint32_t h_misc[6];
+  if ((e = hipMemcpyAsync(h_misc, misc, sizeof(h_misc), hipMemcpyDeviceToHost, r->stream)) != hipSuccess) return hip_err(e, "stats");
+  if ((e = hipStreamSynchronize(r->stream)) != hipSuccess) return hip_err(e, "RenderDR");
+  double loss;
+  std::memcpy(&loss, &h_misc[4], sizeof(double));
+  r->dr_last_loss = (float)(loss / (double)np);
+  r->dr_last_border = h_misc[1];
+  r->dr_last_border_hits = h_misc[2];
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, r->ev0, r->ev1);
+  r->times["RenderDR"] = ms;
+  if (kernel_ms) *kernel_ms = ms;
+  return LRT_OK;
+}
+
+}  // namespace lrt
+
+using namespace lrt;
+
+extern "C" {
+
+int lrt_dr_set_reference(lrt_handle h, const float* images, const float* masks, uint32_t n, uint32_t W, uint32_t H, const float* wvs,
+                         const float* prs) {
+  if (!h || !images || !wvs || !prs || n == 0 || W == 0 || H == 0 || W > 65535 || H > 65535) return set_err(LRT_E_INVALID, "bad arguments");
+  const size_t np = (size_t)W * H;
+  std::vector<float> ref(np * 4 * n);
+  for (size_t i = 0; i < np * n; i++) {
+    const float* c = images + 4 * i;
+    float m;
+    if (masks) m = masks[4 * i + 3];
+    else m = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) < 0.001f ? 0.0f : 1.0f;  // CreateRefImageMasks, :169-185
+    ref[4 * i + 0] = c[0];
+    ref[4 * i + 1] = c[1];
+    ref[4 * i + 2] = c[2];
+    ref[4 * i + 3] = m;
+  }
+  hipError_t e;
+  if ((e = hipSetDevice(h->device)) != hipSuccess) return hip_err(e, "hipSetDevice");
+  if ((e = h->d_dr_ref.upload(ref.data(), ref.size() * 4, h->stream)) != hipSuccess) return hip_err(e, "upload reference");
+  if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_err(e, "upload reference");
+  h->dr_n_ref = n;
+  h->dr_w = W;
+  h->dr_h = H;
+  h->dr_world_views.assign(wvs, wvs + 16 * n);
+  h->dr_projs.assign(prs, prs + 16 * n);
+  return LRT_OK;
+}
+
+int lrt_dr_render_grad(lrt_handle h, const MultiRendererDRPreset* p, uint32_t view, uint32_t seed, int dtdv_mode, float* out_image,
+                       float* out_grad, float* loss, float* kernel_ms) {
+  if (!h || !p) return set_err(LRT_E_INVALID, "null argument");
+  int c = dr_render_view(h, *p, view, seed, dtdv_mode, kernel_ms);
+  if (c) return c;
+  hipError_t e;
+  if (out_image && (e = hipMemcpy(out_image, h->d_dr_img.p, (size_t)h->dr_w * h->dr_h * 16, hipMemcpyDeviceToHost)) != hipSuccess)
+    return hip_err(e, "download image");
+  if (out_grad && (e = hipMemcpy(out_grad, h->d_dr_grad.p, h->values_f.size() * 4, hipMemcpyDeviceToHost)) != hipSuccess)
+    return hip_err(e, "download grad");
+  if (loss) *loss = h->dr_last_loss;
+  return LRT_OK;
+}
+
+int lrt_dr_optimize(lrt_handle h, const MultiRendererDRPreset* p, const SdfSBSHeader* header, const SdfSBSNode* nodes, uint32_t n_nodes,
+                    const uint32_t* values, uint64_t n_values, float* values_f, uint64_t n_values_f, uint32_t seed, int dtdv_mode,
+                    float* losses, float* total_ms) {
+  if (!h || !p || !values_f) return set_err(LRT_E_INVALID, "null argument");
+  if (p->opt_iterations == 0 || p->image_batch_size == 0) return set_err(LRT_E_INVALID, "opt_iterations and image_batch_size must be > 0");
+  int c = lrt_set_scene_sbs(h, header, nodes, n_nodes, values, n_values, values_f, n_values_f);
+  if (c) return c;
+  if ((c = dr_check(h, *p))) return c;
+  const uint32_t P = (uint32_t)n_values_f;
+  hipError_t e;
+  if ((e = h->d_adam_v.ensure((size_t)P * 4)) != hipSuccess) return hip_err(e, "alloc adam");
+  if ((e = h->d_adam_s.ensure((size_t)P * 4)) != hipSuccess) return hip_err(e, "alloc adam");
+  if ((e = zero_u32((uint32_t*)h->d_adam_v.p, P, h->stream)) != hipSuccess) return hip_err(e, "clear");
+  if ((e = zero_u32((uint32_t*)h->d_adam_s.p, P, h->stream)) != hipSuccess) return hip_err(e, "clear");
+  auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t it = 0; it < p->opt_iterations; it++) {
+    double lsum = 0;
+    for (uint32_t b = 0; b < p->image_batch_size; b++) {
+      // MultiRendererDR.cpp:395-406; the gradient buffer is cleared per image as there
+      uint32_t view = p->image_batch_size == h->dr_n_ref ? b : pcg_hash(seed ^ pcg_hash(it * 131u + b)) % h->dr_n_ref;
+      if ((c = dr_render_view(h, *p, view, seed, dtdv_mode, nullptr))) return c;
+      lsum += h->dr_last_loss;
+    }
+    if (losses) losses[it] = (float)(lsum / p->image_batch_size);
+    const double c1 = 1.0 - std::pow((double)p->opt_beta_1, (double)(it + 1));
+    const double c2 = 1.0 - std::pow((double)p->opt_beta_2, (double)(it + 1));
+    e = launch_adam((const float*)h->d_dr_grad.p, (float*)h->d_values_f.p, (float*)h->d_adam_v.p, (float*)h->d_adam_s.p, P, p->opt_lr,
+                    p->opt_beta_1, p->opt_beta_2, p->opt_eps, c1, c2, (float)p->image_batch_size, h->stream);
+    if (e != hipSuccess) return hip_err(e, "adam");
+  }
+  if ((e = hipMemcpyAsync(values_f, h->d_values_f.p, (size_t)P * 4, hipMemcpyDeviceToHost, h->stream)) != hipSuccess)
+    return hip_err(e, "download values_f");
+  if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_err(e, "optimize");
+  h->values_f.assign(values_f, values_f + P);
+  if (total_ms) *total_ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return LRT_OK;
+}
+
+int lrt_dr_get_last_dloss_ds(lrt_handle h, float* out, uint64_t n) {
+  if (!h || !out) return set_err(LRT_E_INVALID, "null argument");
+  if (!h->d_dr_grad.p || n > h->values_f.size()) return set_err(LRT_E_INVALID, "no gradient or size too large");
+  hipError_t e = hipMemcpy(out, h->d_dr_grad.p, n * 4, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? LRT_OK : hip_err(e, "download grad");
+}
+
+int lrt_dr_last_stats(lrt_handle h, uint32_t* border_pixels, uint32_t* border_rays_hit) {
+  if (!h) return set_err(LRT_E_INVALID, "null handle");
+  if (border_pixels) *border_pixels = h->dr_last_border;
+  if (border_rays_hit) *border_rays_hit = h->dr_last_border_hits;
+  return LRT_OK;
+}
+
+}  // extern "C"

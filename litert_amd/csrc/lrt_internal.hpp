@@ -50,6 +50,13 @@ struct Renderer {
   DevBuf d_cams, d_counter, d_spill, d_counts, d_rays, d_hits;
   DevBuf d_out_t, d_out_n, d_out_prim, d_out_rgba;
   uint32_t last_w = 0, last_h = 0, last_views = 0;
+
+  // differentiable rendering state (MultiRendererDR)
+  uint32_t dr_n_ref = 0, dr_w = 0, dr_h = 0;
+  std::vector<float> dr_world_views, dr_projs;  // 16 floats per reference view
+  DevBuf d_dr_ref, d_dr_img, d_dr_thr, d_dr_grad, d_dr_list, d_dr_misc, d_adam_v, d_adam_s;
+  float dr_last_loss = 0.f;
+  uint32_t dr_last_border = 0, dr_last_border_hits = 0;
   float last_ms = 0.f;
   std::string pending_timer;
   std::map<std::string, float> times;
@@ -63,7 +70,8 @@ struct Renderer {
   int finish_timer();
   void release_all() {
     DevBuf* all[] = {&d_pairs, &d_nodes, &d_values, &d_values_f, &d_cams, &d_counter, &d_spill, &d_counts,
-                     &d_rays, &d_hits, &d_out_t, &d_out_n, &d_out_prim, &d_out_rgba};
+                     &d_rays, &d_hits, &d_out_t, &d_out_n, &d_out_prim, &d_out_rgba, &d_dr_ref, &d_dr_img,
+                     &d_dr_thr, &d_dr_grad, &d_dr_list, &d_dr_misc, &d_adam_v, &d_adam_s};
     for (DevBuf* b : all) b->release();
   }
 };

@@ -1,14 +1,11 @@
 // oracle_api.cpp - C entry points of the CPU oracle (plain primary-ray path).
 // TEST INFRASTRUCTURE ONLY; see lrt_oracle.h.
 #include "lrt_oracle.h"
-#include "oracle_render.hpp"
+#include "oracle_scene.hpp"
 #include <omp.h>
 
 using namespace orc;
 
-struct orc_scene {
-  Scene s;
-};
 
 extern "C" orc_scene* orc_scene_create(const SdfSBSHeader* h, const SdfSBSNode* nodes, uint32_t n_nodes, const uint32_t* values,
                                        uint64_t n_values, const float* values_f, uint64_t n_values_f, const BVHNodePair* pairs,

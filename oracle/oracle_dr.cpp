@@ -1,0 +1,646 @@
+// oracle_dr.cpp - CPU restatement of LiteRT's differentiable SBS renderer
+// (diff_render/BVH2DR.cpp, diff_render/MultiRendererDR.cpp).
+//
+// TEST INFRASTRUCTURE ONLY (see lrt_oracle.h). Covers what RenderDR does for
+// the Newton root find that OptimizeFixedStructure forces
+// (MultiRendererDR.cpp:324) and the render modes LINEAR_DEPTH, MASK and
+// DIFFUSE; the other modes return an error. Gradients are accumulated in
+// double whatever the precision of the per-ray arithmetic (the reference sums
+// per-thread float buffers in a thread-dependent order).
+#include <omp.h>
+#include <cmath>
+#include <vector>
+#include "lrt_oracle.h"
+#include "oracle_scene.hpp"
+
+using namespace orc;
+
+
+namespace {
+
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+constexpr float kZNear = 0.1f, kZFar = 10.0f;  // MultiRendererDR.cpp:22-23
+
+template <typename T> struct HitDR {  // CRT_HitDR, DR_common.h:47-57
+  T t;
+  uint32_t primId, instId, geomId;
+  V3<T> color;
+  T sdf;
+  V3<T> normal;
+  uint32_t check_vox_border;
+};
+
+template <typename T> struct PDDist {
+  V3<T> dDiffuse;
+  uint32_t index;
+  V3<T> dNorm;
+  T dDist;
+};
+
+template <typename T> struct Payload {  // RayDiffPayload, DR_common.h:59-72
+  HitDR<T> missed_hit, missed_hit_candidate;
+  uint32_t missed_indices[8];
+  T missed_dSDF_dtheta[8];
+  uint32_t dSc_index[8];
+  T dSc_value[8];
+  PDDist<T> dSd[64];
+};
+
+template <typename T> struct DRTracer {
+  const Scene& S;
+  MultiRenderPreset preset;
+  int dtdv_mode;  // 0 analytic (implicit function theorem), 1 central differences (BVH2DR.cpp:1075-1090)
+  DRTracer(const Scene& s, const MultiRenderPreset& p, int m) : S(s), preset(p), dtdv_mode(m) {}
+
+  // BVH2DR.cpp:623-1073, Newton branch; returns hit ? t : qFar + EPS
+  T Intersect(uint32_t ray_flags, V3<T> ray_dir, const T values[8], T d, T qFar, V3<T> start_q, Payload<T>* relax) const {
+    const T EPS = T(1e-6f), SDF_EPS = T(1e-7f);
+    T d_inv = T(1) / d;
+    T t = 0;
+    bool hit = false;
+    if (relax) relax->missed_hit_candidate.sdf = T(1000.f);
+    T start_dist = eval_dist_trilinear(values, start_q + t * ray_dir);
+    if (start_dist <= SDF_EPS) {
+      hit = true;
+    } else {
+      T s000 = values[0] * d_inv, s001 = values[1] * d_inv, s010 = values[2] * d_inv, s011 = values[3] * d_inv;
+      T s100 = values[4] * d_inv, s101 = values[5] * d_inv, s110 = values[6] * d_inv, s111 = values[7] * d_inv;
+      T a = s101 - s001;
+      T k0 = s000, k1 = s100 - s000, k2 = s010 - s000, k3 = s110 - s010 - k1, k4 = k0 - s001, k5 = k1 - a;
+      T k6 = k2 - (s011 - s001), k7 = k3 - (s111 - s011 - a);
+      V3<T> o = start_q, d3 = ray_dir;
+      T m0 = o.x * o.y, m1 = d3.x * d3.y, m2 = o.x * d3.y + o.y * d3.x;
+      T m3 = k5 * o.z - k1, m4 = k6 * o.z - k2, m5 = k7 * o.z - k3;
+      T c0 = (k4 * o.z - k0) + o.x * m3 + o.y * m4 + m0 * m5;
+      T c1 = d3.x * m3 + d3.y * m4 + m2 * m5 + d3.z * (k4 + k5 * o.x + k6 * o.y + k7 * m0);
+      T c2 = m1 * m5 + d3.z * (k5 * d3.x + k6 * d3.y + k7 * m2);
+      T c3 = k7 * m1 * d3.z;
+      T qa = 3 * c3, qb = 2 * c2, qc = c1;
+      T t0 = 0, t1 = qFar, t2 = qFar, t3 = qFar;
+      T D = qb * qb - 4 * qa * qc;
+      if (D >= 0) {
+        T q = T(-0.5f) * (qb + sgn(qb) * std::sqrt(D));
+        t1 = std::abs(qa) > EPS ? q / qa : t0;
+        t2 = std::abs(q) > EPS ? qc / q : qFar;
+        T tmp = mn(t1, t2);
+        t2 = mx(t1, t2);
+        t1 = tmp;
+        t1 = clampv(t1, t0, t3);
+        t2 = clampv(t2, t0, t3);
+      }
+      T sdf0 = c0;
+      T sdf1 = (c0 + t1 * (c1 + t1 * (c2 + t1 * c3)));
+      T sdf2 = (c0 + t2 * (c1 + t2 * (c2 + t2 * c3)));
+      T sdf3 = (c0 + t3 * (c1 + t3 * (c2 + t3 * c3)));
+      bool s0 = sdf0 > 0, s1 = sdf1 > 0, s2 = sdf2 > 0, s3 = sdf3 > 0;
+      T nwt_min = t0, nwt_max = t0;
+      if (s0 != s1) { nwt_min = t0; nwt_max = t1; }
+      else if (s1 != s2) { nwt_min = t1; nwt_max = t2; }
+      else if (s2 != s3) { nwt_min = t2; nwt_max = t3; }
+      if (nwt_min < nwt_max) {
+        unsigned it = 0;
+        T rtn = T(0.5f) * (nwt_min + nwt_max);
+        T f = 1000;
+        while (it < 10u && std::abs(f) >= SDF_EPS) {
+          f = c0 + rtn * (c1 + rtn * (c2 + rtn * c3));
+          T df = c1 + rtn * (2 * c2 + rtn * 3 * c3);
+          T dx = f / (df + sgn(df) * T(1e-9f));
+          rtn -= dx;
+          it++;
+        }
+        t = rtn;
+        hit = (t >= 0 && t <= qFar && std::abs(f) < SDF_EPS);
+      }
+      if (relax && (ray_flags & LRT_DR_RAY_FLAG_BORDER)) {  // relaxation point, BVH2DR.cpp:960-1035
+        T t_min = qFar + EPS;
+        if (std::abs(qa) > EPS) {
+          T root1 = T(-0.5f) * (qb + std::sqrt(D)) / qa, root2 = T(-0.5f) * (qb - std::sqrt(D)) / qa;
+          t_min = (sgn(qa) > 0) == (root1 < root2) ? root1 : root2;
+        } else if (qb < -EPS) {
+          t_min = -qc / qb;
+        }
+        T sdf_min = (c0 + t_min * (c1 + t_min * (c2 + t_min * c3)));
+        HitDR<T>& mh = relax->missed_hit;
+        if (mh.check_vox_border == 1) {
+          if (c1 <= 0) {
+            if (sdf0 < T(0) && -sdf0 * d < mh.sdf) { mh.t = t0; mh.sdf = -sdf0 * d; }
+          }
+          mh.check_vox_border = 0;
+        }
+        if (t_min >= 0 && t_min <= qFar && sdf_min < T(0) && -sdf_min * d < mh.sdf) { mh.t = t_min; mh.sdf = -sdf_min * d; }
+        if (sdf3 < T(0) && -sdf3 * d < mh.sdf && (c1 + qFar * (2 * c2 + qFar * 3 * c3)) >= T(0)) {
+          mh.check_vox_border = 1;
+          relax->missed_hit_candidate.t = qFar;
+          relax->missed_hit_candidate.sdf = -sdf3 * d;
+        }
+      }
+    }
+    return hit ? t : qFar + EPS;
+  }
+
+  // dt/dvalues in voxel units of t
+  void dIntersect_dValues(uint32_t ray_flags, V3<T> ray_dir, const T values[8], T d, T qFar, V3<T> start_q, T t, T out[8]) const {
+    if (dtdv_mode == 1) {  // BVH2DR.cpp:1075-1090
+      T v[8];
+      for (int i = 0; i < 8; i++) v[i] = values[i];
+      const T delta = T(0.001f);
+      for (int i = 0; i < 8; i++) {
+        v[i] += delta;
+        T d1 = Intersect(ray_flags, ray_dir, v, d, qFar, start_q, nullptr);
+        v[i] -= 2 * delta;
+        T d2 = Intersect(ray_flags, ray_dir, v, d, qFar, start_q, nullptr);
+        v[i] += delta;
+        out[i] = (d1 - d2) / (2 * delta);
+      }
+      return;
+    }
+    // implicit function theorem on f(q(t); v) = 0, q(t) = start_q + t*dir:
+    // dt/dv_i = -w_i(q*) / (grad f(q*) . dir)
+    V3<T> q = start_q + t * ray_dir;
+    V3<T> g = eval_dist_trilinear_diff(values, q);
+    T fd = dot(g, ray_dir);
+    for (int i = 0; i < 8; i++) {
+      T w = ((i & 4) ? q.x : 1 - q.x) * ((i & 2) ? q.y : 1 - q.y) * ((i & 1) ? q.z : 1 - q.z);
+      out[i] = fd != T(0) ? -w / fd : T(0);
+    }
+  }
+
+  static V3<T> color_lerp(const V3<T> c[8], V3<T> dp) {
+    V3<T> r(0, 0, 0);
+    for (int i = 0; i < 8; i++) {
+      T w = ((i & 4) ? dp.x : 1 - dp.x) * ((i & 2) ? dp.y : 1 - dp.y) * ((i & 1) ? dp.z : 1 - dp.z);
+      r = r + w * c[i];
+    }
+    return r;
+  }
+  // dLerp/d(dp) as three columns (BVH2DR.cpp:257-337)
+  static void color_lerp_ddp(const V3<T> c[8], V3<T> dp, V3<T> col[3]) {
+    col[0] = col[1] = col[2] = V3<T>(0, 0, 0);
+    for (int i = 0; i < 8; i++) {
+      T ax = (i & 4) ? T(1) : T(-1), ay = (i & 2) ? T(1) : T(-1), az = (i & 1) ? T(1) : T(-1);
+      T wx = (i & 4) ? dp.x : 1 - dp.x, wy = (i & 2) ? dp.y : 1 - dp.y, wz = (i & 1) ? dp.z : 1 - dp.z;
+      col[0] = col[0] + (ax * wy * wz) * c[i];
+      col[1] = col[1] + (wx * ay * wz) * c[i];
+      col[2] = col[2] + (wx * wy * az) * c[i];
+    }
+  }
+
+  // BVH2DR.cpp:1130-1610 (ID32F layouts, no normal derivatives)
+  void OctreeBrickIntersectWithGrad(uint32_t ray_flags, V3<T> ray_pos, V3<T> ray_dir, T tNear, uint32_t brick, Payload<T>* relax,
+                                    HitDR<T>* pHit) const {
+    T values[8];
+    const SdfSBSHeader& h = S.header;
+    const uint32_t v_size = h.brick_size + 2 * h.brick_pad + 1;
+    const SdfSBSNode& N = S.nodes[brick];
+    T px = T(N.pos_xy >> 16), py = T(N.pos_xy & 0xFFFF), pz = T(N.pos_z_lod_size >> 16), sz = T(N.pos_z_lod_size & 0xFFFF);
+    T sz_inv = T(2) / sz;
+    T d = T(2) / (sz * T(h.brick_size));
+    V3<T> bmin = V3<T>(-1, -1, -1) + sz_inv * V3<T>(px, py, pz);
+    V3<T> bmax = bmin + sz_inv * V3<T>(1, 1, 1);
+    const T qscale = T(0.5f) * sz * T(h.brick_size);
+    V2<T> bnf = RayBoxIntersection2(ray_pos, SafeInverse(ray_dir), bmin, bmax);
+    T old_t = pHit->t;
+    const uint32_t max_steps = 16u * (h.brick_size + 1u) + 16u;  // termination cap as in the plain path
+    uint32_t steps = 0;
+    while (bnf.x < bnf.y && pHit->t == old_t && steps++ < max_steps) {
+      V3<T> hit_pos = ray_pos + bnf.x * ray_dir;
+      V3<T> local = (hit_pos - bmin) * qscale;
+      V3<T> vp = vfloor(vclamp(local, T(1e-6f), T(h.brick_size) - T(1e-6f)));
+      uint32_t vx = (uint32_t)vp.x, vy = (uint32_t)vp.y, vz = (uint32_t)vp.z;
+      V3<T> min_pos = bmin + d * vp;
+      V3<T> max_pos = min_pos + d * V3<T>(1, 1, 1);
+      uint32_t idx[8];
+      for (int i = 0; i < 8; i++) {
+        uint32_t vId = (vx + ((i & 4) >> 2)) * v_size * v_size + (vy + ((i & 2) >> 1)) * v_size + (vz + (i & 1));
+        idx[i] = S.values[N.data_offset + vId];
+        values[i] = T(S.values_f[idx[i]]);
+      }
+      V2<T> fnf = RayBoxIntersection2(ray_pos, SafeInverse(ray_dir), min_pos, max_pos);
+      if (tNear < fnf.x) {
+        V3<T> start_pos = ray_pos + fnf.x * ray_dir;
+        V3<T> start_q = (start_pos - min_pos) * qscale;
+        T qFar = (fnf.y - fnf.x) * qscale;
+        T prev_sdf = relax->missed_hit.sdf;
+        T t = Intersect(ray_flags, ray_dir, values, d, qFar, start_q, relax);
+        T tReal = fnf.x + d * t;
+        if (ray_flags & LRT_DR_RAY_FLAG_BORDER) {  // BVH2DR.cpp:1223-1275
+          if (relax->missed_hit.sdf < prev_sdf && relax->missed_hit.t < t) {
+            T missed_t = relax->missed_hit.t;
+            relax->missed_hit.t = fnf.x + d * relax->missed_hit.t;
+            V3<T> q_ast = start_q + missed_t * ray_dir;
+            V3<T> y_ast = ray_pos + (fnf.x + (d * missed_t)) * ray_dir;
+            V3<T> dp_ast = (y_ast - bmin) * (T(0.5f) * sz);
+            V3<T> g = eval_dist_trilinear_diff(values, q_ast) * (T(1) / d);
+            V3<T> mult(std::abs(q_ast.x - T(0.5f)) > T(0.5f) - T(1e-5f) ? T(0) : T(1),
+                       std::abs(q_ast.y - T(0.5f)) > T(0.5f) - T(1e-5f) ? T(0) : T(1),
+                       std::abs(q_ast.z - T(0.5f)) > T(0.5f) - T(1e-5f) ? T(0) : T(1));
+            g = g * mult;
+            T glen = length(g);
+            if (glen > T(1e-9f)) {
+              for (int i = 0; i < 8; i++) {
+                T w = ((i & 4) ? q_ast.x : 1 - q_ast.x) * ((i & 2) ? q_ast.y : 1 - q_ast.y) * ((i & 1) ? q_ast.z : 1 - q_ast.z);
+                relax->missed_dSDF_dtheta[i] = -w / glen;
+              }
+              uint32_t t_off = N.data_offset + v_size * v_size * v_size;
+              V3<T> cols[8];
+              for (int i = 0; i < 8; i++) {
+                uint32_t ci = S.values[t_off + i];
+                cols[i] = V3<T>(T(S.values_f[ci]), T(S.values_f[ci + 1]), T(S.values_f[ci + 2]));
+              }
+              relax->missed_hit.color = color_lerp(cols, dp_ast);
+              relax->missed_hit.normal = g / glen;
+              for (int i = 0; i < 8; i++) relax->missed_indices[i] = idx[i];
+            }
+          }
+        }
+        if (t <= qFar && tReal < pHit->t) {  // BVH2DR.cpp:1277-1600
+          pHit->t = tReal;
+          pHit->primId = brick;
+          pHit->instId = 0;
+          pHit->geomId = 0u | (S.type << LRT_SH_TYPE);
+          unsigned dvo = 0;
+          if ((S.header.aux_data & LRT_SBS_LAYOUT_MASK) == LRT_SBS_LAYOUT_ID32F_IRGB32F_IN) {
+            V3<T> p0 = start_q + t * ray_dir;
+            dvo += p0.x < T(0.5f) ? 4 : 0;
+            dvo += p0.y < T(0.5f) ? 2 : 0;
+            dvo += p0.z < T(0.5f) ? 1 : 0;
+            dvo *= 8;
+          }
+          T dtdv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+          if (ray_flags & (LRT_DR_RAY_FLAG_DDIFFUSE_DPOS | LRT_DR_RAY_FLAG_DNORM_DPOS | LRT_DR_RAY_FLAG_DDIST_DPOS)) {
+            dIntersect_dValues(ray_flags, ray_dir, values, d, qFar, start_q, t, dtdv);
+            for (int i = 0; i < 8; i++) {
+              relax->dSd[dvo + i].index = idx[i];
+              relax->dSd[dvo + i].dDist = d * dtdv[i];
+            }
+          }
+          V3<T> pos = ray_pos + pHit->t * ray_dir;
+          V3<T> dp = (pos - bmin) * (T(0.5f) * sz);
+          uint32_t t_off = N.data_offset + v_size * v_size * v_size;
+          V3<T> cols[8];
+          for (int i = 0; i < 8; i++) {
+            uint32_t ci = S.values[t_off + i];
+            cols[i] = V3<T>(T(S.values_f[ci]), T(S.values_f[ci + 1]), T(S.values_f[ci + 2]));
+          }
+          pHit->color = color_lerp(cols, dp);
+          if (ray_flags & LRT_DR_RAY_FLAG_DDIFFUSE_DCOLOR) {
+            for (int i = 0; i < 8; i++) {
+              T lx = (i & 4) ? dp.x : 1 - dp.x, ly = (i & 2) ? dp.y : 1 - dp.y, lz = (i & 1) ? dp.z : 1 - dp.z;
+              relax->dSc_index[i] = S.values[t_off + i];
+              relax->dSc_value[i] = lx * ly * lz;
+            }
+          }
+          if (ray_flags & LRT_DR_RAY_FLAG_DDIFFUSE_DPOS) {
+            V3<T> J[3];
+            color_lerp_ddp(cols, dp, J);
+            for (int i = 0; i < 8; i++) {
+              V3<T> v(T(0.5f) * sz * ray_dir.x * dtdv[i], T(0.5f) * sz * ray_dir.y * dtdv[i], T(0.5f) * sz * ray_dir.z * dtdv[i]);
+              relax->dSd[dvo + i].dDiffuse = J[0] * v.x + J[1] * v.y + J[2] * v.z;
+            }
+          }
+          break;
+        }
+      }
+      bnf.x += mx(T(0), fnf.y - bnf.x) + T(1e-6f);
+    }
+  }
+
+  // BVH2DR.cpp:106-167
+  void TraverseWithGrad(uint32_t ray_flags, V3<T> ray_pos, V3<T> ray_dir, T tNear, Payload<T>* relax, HitDR<T>* pHit) const {
+    uint32_t stack[LRT_STACK_SIZE];
+    int top = 0;
+    uint32_t left = 0;
+    const V3<T> inv = SafeInverse(ray_dir);
+    while (top >= 0 && !(pHit->primId != kInvalid)) {
+      while (top >= 0 && ((left & LRT_LEAF_BIT) == 0)) {
+        const BVHNodePair& fat = S.pairs[left];
+        V2<T> tm0 = RayBoxIntersection2(ray_pos, inv, V3<T>(fat.left.boxMin[0], fat.left.boxMin[1], fat.left.boxMin[2]),
+                                        V3<T>(fat.left.boxMax[0], fat.left.boxMax[1], fat.left.boxMax[2]));
+        V2<T> tm1 = RayBoxIntersection2(ray_pos, inv, V3<T>(fat.right.boxMin[0], fat.right.boxMin[1], fat.right.boxMin[2]),
+                                        V3<T>(fat.right.boxMax[0], fat.right.boxMax[1], fat.right.boxMax[2]));
+        const bool h0 = (tm0.x <= tm0.y) && (tm0.y >= tNear) && (tm0.x <= pHit->t);
+        const bool h1 = (tm1.x <= tm1.y) && (tm1.y >= tNear) && (tm1.x <= pHit->t);
+        left = h0 ? fat.left.leftOffset : fat.right.leftOffset;
+        if (h0 && h1) {
+          left = (tm0.x <= tm1.x) ? fat.left.leftOffset : fat.right.leftOffset;
+          stack[top] = (tm0.x <= tm1.x) ? fat.right.leftOffset : fat.left.leftOffset;
+          top++;
+        }
+        if (!h0 && !h1) {
+          top--;
+          left = stack[std::max(top, 0)];
+        }
+      }
+      if (top >= 0 && left != kInvalid) {
+        const T tNearSdf = mx(tNear, T(0.1f));  // BVH2DR.cpp:175-176
+        OctreeBrickIntersectWithGrad(ray_flags, ray_pos, ray_dir, tNearSdf, left & LRT_START_MASK, relax, pHit);
+      }
+      top--;
+      left = stack[std::max(top, 0)];
+    }
+  }
+
+  // BVH2DR.cpp:12-102
+  HitDR<T> NearestHitWithGrad(uint32_t ray_flags, V4<T> pn, V4<T> df, Payload<T>* relax) const {
+    if (df.w <= T(0)) df.w *= T(-1);
+    V3<T> pos(pn.x, pn.y, pn.z), dir(df.x, df.y, df.z);
+    HitDR<T> hit;
+    hit.t = df.w;
+    hit.primId = hit.instId = hit.geomId = kInvalid;
+    hit.color = V3<T>(0, 0, 0);
+    hit.sdf = T(1000.0f);
+    hit.normal = V3<T>(1, 0, 0);
+    hit.check_vox_border = 0;
+    relax->missed_hit = hit;
+    relax->missed_hit.check_vox_border = 1;
+    relax->missed_hit_candidate = relax->missed_hit;
+    for (int i = 0; i < 8; i++) {
+      relax->missed_indices[i] = kInvalid;
+      relax->missed_dSDF_dtheta[i] = 0;
+      relax->dSc_index[i] = kInvalid;
+      relax->dSc_value[i] = 0;
+    }
+    for (int i = 0; i < 64; i++) {
+      relax->dSd[i].index = kInvalid;
+      relax->dSd[i].dDiffuse = V3<T>(0, 0, 0);
+      relax->dSd[i].dNorm = V3<T>(0, 0, 0);
+      relax->dSd[i].dDist = 0;
+    }
+    V2<T> bh = RayBoxIntersection2(pos, SafeInverse(dir), V3<T>(-1, -1, -1), V3<T>(1, 1, 1));
+    if ((bh.x <= bh.y) && (bh.y > pn.w) && (bh.x < hit.t)) TraverseWithGrad(ray_flags, pos, dir, pn.w, relax, &hit);
+    if (relax->missed_hit_candidate.sdf < relax->missed_hit.sdf) relax->missed_hit = relax->missed_hit_candidate;
+    return hit;
+  }
+};
+
+template <typename T> static T lin_depth(T z) { return (z - T(kZNear)) / (T(kZFar) - T(kZNear)); }
+
+// CalculateColor (MultiRendererDR.cpp:855-898) for the supported modes
+template <typename T> static V3<T> calc_color(uint32_t mode, const HitDR<T>& h) {
+  switch (mode) {
+    case LRT_DR_RENDER_MODE_DIFFUSE: return h.color;
+    case LRT_DR_RENDER_MODE_MASK: return V3<T>(1, 1, 1);
+    case LRT_DR_RENDER_MODE_LINEAR_DEPTH: { T d = lin_depth(h.t); return V3<T>(d, d, d); }
+    default: return V3<T>(1, 0, 1);
+  }
+}
+
+// t_samples_to_threshold (MultiRendererDR.cpp:997-1063)
+static float samples_to_threshold(const float* s, unsigned count, float dv, unsigned dc) {
+  const int G = 10, GAP = 4;
+  float mnv = s[0], mxv = s[0];
+  for (unsigned i = 1; i < count; i++) { mnv = std::min(mnv, s[i]); mxv = std::max(mxv, s[i]); }
+  if (dc > 0) { mnv = std::min(mnv, dv); mxv = std::max(mxv, dv); }
+  float diff = std::max(1e-9f, mxv - mnv);
+  unsigned g[G] = {0};
+  for (unsigned i = 0; i < count; i++) g[std::min(std::max(int((s[i] - mnv) / diff * G), 0), G - 1)]++;
+  g[std::min(std::max(int((dv - mnv) / diff * G), 0), G - 1)] += dc;
+  int best_len = 0, best_start = -1, start = -1;
+  for (int i = 1; i < G; i++) {
+    if (start != -1) {
+      if (g[i] != 0) {
+        int len = i - start;
+        if (len > best_len) { best_len = len; best_start = start; }
+        start = -1;
+      }
+    } else if (g[i] == 0) {
+      start = i;
+    }
+  }
+  if (best_len > GAP) return mnv + diff * (best_start + 0.5f * best_len) / G;
+  return 0.0f;
+}
+
+template <typename T>
+static int dr_impl(const Scene& S, const MultiRendererDRPreset& dr, const Camera& cam, const float* wv, const float* pr, uint32_t W,
+                   uint32_t H, uint32_t seed, const float* ref, int nthreads, int dtdv_mode, double* out_image, double* out_grad,
+                   orc_dr_stats* stats) {
+  const uint32_t mode = dr.dr_render_mode;
+  if (mode != LRT_DR_RENDER_MODE_LINEAR_DEPTH && mode != LRT_DR_RENDER_MODE_MASK && mode != LRT_DR_RENDER_MODE_DIFFUSE) return -3;
+  if (dr.dr_loss_function != LRT_DR_LOSS_FUNCTION_MSE && dr.dr_loss_function != LRT_DR_LOSS_FUNCTION_MAE) return -3;
+  if (S.type != LRT_TYPE_SDF_SBS_COL) return -4;  // DR needs indexed colour layouts (MultiRendererDR.cpp:283-284)
+  MultiRenderPreset p{};
+  p.sdf_node_intersect = LRT_SDF_NODE_INTERSECT_NEWTON;
+  p.ray_gen_mode = LRT_RAY_GEN_MODE_RANDOM;
+  p.spp = dr.spp;
+  DRTracer<T> tr(S, p, dtdv_mode);
+  const uint32_t spp = dr.spp;
+  const size_t np = (size_t)W * H;
+  std::vector<uint32_t> packed = PackXY(W, H);
+  std::vector<float> img(np * 4, 0.f), depth_thr(np, 0.f);
+  const size_t P = S.values_f.size();
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+  std::vector<std::vector<double>> grads(nthreads, std::vector<double>(P, 0.0));
+  std::vector<double> loss_v(nthreads, 0.0);
+  uint32_t ray_flags = LRT_DR_RAY_FLAG_NO_DIFF;  // CastRayWithGrad, MultiRendererDR.cpp:1119-1142
+  if (dr.dr_diff_mode == LRT_DR_DIFF_MODE_FINITE_DIFF) ray_flags = LRT_DR_RAY_FLAG_NO_DIFF;
+  else if (dr.dr_input_type == LRT_DR_INPUT_TYPE_LINEAR_DEPTH) ray_flags = LRT_DR_RAY_FLAG_DDIST_DPOS;
+  else {
+    if (dr.dr_reconstruction_flags & LRT_DR_RECONSTRUCTION_FLAG_COLOR) ray_flags |= LRT_DR_RAY_FLAG_DDIFFUSE_DCOLOR;
+    if ((dr.dr_reconstruction_flags & LRT_DR_RECONSTRUCTION_FLAG_GEOMETRY) && mode == LRT_DR_RENDER_MODE_DIFFUSE)
+      ray_flags |= LRT_DR_RAY_FLAG_DDIFFUSE_DPOS;
+  }
+  const bool mse = dr.dr_loss_function == LRT_DR_LOSS_FUNCTION_MSE;
+  auto loss_grad = [&](const float c[4], const float r[4], T g[4]) {
+    for (int k = 0; k < 4; k++) g[k] = mse ? T(2.0f) * (T(c[k]) - T(r[k])) : sgn(T(c[k]) - T(r[k]));
+  };
+
+  // I: primary rays with gradients (MultiRendererDR.cpp:596-607, 1084-1294)
+#pragma omp parallel num_threads(nthreads)
+  {
+    const int tid_omp = omp_get_thread_num();
+    std::vector<double>& G = grads[tid_omp];
+    std::vector<Payload<T>> pl(spp);
+    std::vector<HitDR<T>> hits(spp);
+    std::vector<float> samples(spp);
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)np; i++) {
+      const uint32_t XY = packed[i];
+      const uint32_t x = XY & 0xFFFF, y = XY >> 16;
+      if (x >= W || y >= H) continue;
+      V3<T> res(0, 0, 0);
+      int hit_count = 0;
+      std::vector<std::pair<uint32_t, V3<T>>> pds;
+      for (uint32_t s = 0; s < spp; s++) {
+        V2<float> d = rand2(x, y, s, seed);
+        V4<T> pn, df;
+        InitEyeRay<T>(cam, x, y, d, W, H, &pn, &df);
+        Payload<T>& py = pl[s];
+        HitDR<T> hit = tr.NearestHitWithGrad(ray_flags, pn, df, &py);
+        if (hit.primId == kInvalid) continue;
+        V3<T> color = calc_color<T>(mode, hit);
+        res = res + color / T(spp);
+        // dColor/dDiffuse is I for DIFFUSE (and left at I for LINEAR_DEPTH), 0 for MASK
+        if ((ray_flags & LRT_DR_RAY_FLAG_DDIFFUSE_DCOLOR) && mode != LRT_DR_RENDER_MODE_MASK)
+          for (int k = 0; k < 8; k++)
+            if (py.dSc_index[k] != kInvalid) pds.push_back({py.dSc_index[k] | 0x80000000u, V3<T>(py.dSc_value[k], py.dSc_value[k], py.dSc_value[k])});
+        if (ray_flags & LRT_DR_RAY_FLAG_DDIFFUSE_DPOS) {
+          for (int k = 0; k < 64; k++)
+            if (py.dSd[k].index != kInvalid) pds.push_back({py.dSd[k].index, py.dSd[k].dDiffuse});  // dColor_dDiffuse = I
+        } else if (ray_flags & LRT_DR_RAY_FLAG_DDIST_DPOS) {
+          for (int k = 0; k < 64; k++)
+            if (py.dSd[k].index != kInvalid) {
+              T v = py.dSd[k].dDist / (T(kZFar) - T(kZNear));
+              pds.push_back({py.dSd[k].index, V3<T>(v, v, v)});
+            }
+        }
+        samples[hit_count] = (float)hit.t;
+        hit_count++;
+      }
+      const size_t pi = (size_t)y * W + x;
+      float cm[4] = {(float)res.x, (float)res.y, (float)res.z, float(hit_count) / float(spp)};
+      for (int k = 0; k < 4; k++) img[4 * pi + k] = cm[k];
+      depth_thr[pi] = hit_count == 0 ? 0.f : samples_to_threshold(samples.data(), hit_count, 1e9f, spp - hit_count);
+      const float* r = ref + 4 * pi;
+      T g4[4];
+      loss_grad(cm, r, g4);
+      T l = 0;
+      for (int k = 0; k < 4; k++) {
+        T df = T(cm[k]) - T(r[k]);
+        l += mse ? df * df : std::abs(df);
+      }
+      loss_v[tid_omp] += (double)l;
+      V3<T> dLdC(g4[0] / T(spp), g4[1] / T(spp), g4[2] / T(spp));
+      for (auto& pd : pds) {
+        if (pd.first & 0x80000000u) {  // colour parameter triple
+          uint32_t base = pd.first & 0x7FFFFFFFu;
+          G[base + 0] += (double)(dLdC.x * pd.second.x);
+          G[base + 1] += (double)(dLdC.y * pd.second.y);
+          G[base + 2] += (double)(dLdC.z * pd.second.z);
+        } else {
+          G[pd.first] += (double)dot(dLdC, pd.second);
+        }
+      }
+    }
+  }
+
+  // II: border pixels (MultiRendererDR.cpp:609-685), in packed order
+  std::vector<uint32_t> border;
+  for (size_t i = 0; i < np; i++) {
+    const uint32_t XY = packed[i];
+    const uint32_t x = XY & 0xFFFF, y = XY >> 16;
+    if (x >= W || y >= H) continue;
+    float d0 = img[4 * ((size_t)y * W + x) + 3];
+    float max_diff = 0.f, max_thr = 0.f;
+    if (x >= 1 && x < W - 1 && y >= 1 && y < H - 1) {
+      for (int dx = -1; dx <= 1; dx++)
+        for (int dy = -1; dy <= 1; dy++) {
+          size_t q = (size_t)(y + dy) * W + x + dx;
+          max_diff = std::max(max_diff, std::abs(d0 - img[4 * q + 3]));
+          max_thr = std::max(max_thr, depth_thr[q]);
+        }
+    }
+    bool is_border = mode == LRT_DR_RENDER_MODE_MASK ? max_diff > 0 : (max_diff > 0 || max_thr > 0);
+    if (is_border || dr.debug_forced_border) border.push_back((uint32_t)i);
+  }
+
+  // III: border rays (MultiRendererDR.cpp:703-729, 1319-1499)
+  uint32_t border_hits = 0;
+  float PV[16];  // proj * worldView (TransformWorldToScreenSpace, MultiRendererDR.cpp:1296-1301)
+  for (int c = 0; c < 4; c++)
+    for (int r = 0; r < 4; r++) {
+      float s = 0.f;
+      for (int k = 0; k < 4; k++) s += pr[k * 4 + r] * wv[c * 4 + k];
+      PV[c * 4 + r] = s;
+    }
+  if (!border.empty() && (dr.dr_reconstruction_flags & LRT_DR_RECONSTRUCTION_FLAG_GEOMETRY) &&
+      dr.dr_diff_mode == LRT_DR_DIFF_MODE_DEFAULT) {
+    const uint32_t bspp = dr.border_spp;
+    const T eps = T(dr.border_relax_eps);
+#pragma omp parallel num_threads(nthreads) reduction(+ : border_hits)
+    {
+      std::vector<double>& G = grads[omp_get_thread_num()];
+      Payload<T> py;
+#pragma omp for schedule(dynamic, 4)
+      for (int64_t bi = 0; bi < (int64_t)border.size(); bi++) {
+        const uint32_t XY = packed[border[bi]];
+        const uint32_t x = XY & 0xFFFF, y = XY >> 16;
+        for (uint32_t s = 0; s < bspp; s++) {
+          V2<float> d = rand2(x, y, s, seed);
+          V4<T> pn, df;
+          InitEyeRay<T>(cam, x, y, d, W, H, &pn, &df);
+          HitDR<T> hit = tr.NearestHitWithGrad(LRT_DR_RAY_FLAG_BORDER, pn, df, &py);
+          if (!(py.missed_hit.sdf < eps)) continue;
+          border_hits++;
+          T delta[4];
+          if (dr.dr_input_type == LRT_DR_INPUT_TYPE_LINEAR_DEPTH) {
+            T din = lin_depth(py.missed_hit.t);
+            T dout = hit.primId == kInvalid ? T(0) : lin_depth(hit.t);
+            delta[0] = delta[1] = delta[2] = din - dout;
+          } else {
+            V3<T> fin = calc_color<T>(mode, py.missed_hit);
+            V3<T> fout = hit.primId == kInvalid ? V3<T>(0, 0, 0) : calc_color<T>(mode, hit);
+            delta[0] = fin.x - fout.x; delta[1] = fin.y - fout.y; delta[2] = fin.z - fout.z;
+          }
+          delta[3] = hit.primId == kInvalid ? T(1) : T(0);
+          const T pdf = T(1.0f) / T(bspp);
+          for (int j = 0; j < 8; j++) {
+            if (py.missed_indices[j] == kInvalid) continue;
+            T bd = mx(T(0), py.missed_hit.sdf);
+            V3<T> ystar = V3<T>(pn.x, pn.y, pn.z) + py.missed_hit.t * V3<T>(df.x, df.y, df.z);
+            V3<T> xstar = ystar - bd * py.missed_hit.normal;
+            V4<T> ps = mul(M4<T>{{T(PV[0]), T(PV[1]), T(PV[2]), T(PV[3]), T(PV[4]), T(PV[5]), T(PV[6]), T(PV[7]), T(PV[8]), T(PV[9]),
+                                  T(PV[10]), T(PV[11]), T(PV[12]), T(PV[13]), T(PV[14]), T(PV[15])}},
+                           V4<T>{xstar.x, xstar.y, xstar.z, T(1)});
+            T sx = T(0.5f) * (ps.x / ps.w + T(1)), sy = T(1) - T(0.5f) * (ps.y / ps.w + T(1));
+            // the reference converts to unsigned without a range check; the
+            // pixel is clamped into the image here
+            double fx = std::floor((double)(sx * T(W))), fy = std::floor((double)(sy * T(H)));
+            uint32_t xp = (uint32_t)std::min(std::max(fx, 0.0), (double)W - 1), yp = (uint32_t)std::min(std::max(fy, 0.0), (double)H - 1);
+            size_t q = (size_t)yp * W + xp;
+            T g4[4];
+            loss_grad(&img[4 * q], ref + 4 * q, g4);
+            T dd = g4[0] * delta[0] + g4[1] * delta[1] + g4[2] * delta[2] + g4[3] * delta[3];
+            T diff = pdf * (T(1) / eps) * dd * py.missed_dSDF_dtheta[j];
+            G[py.missed_indices[j]] += (double)diff;
+          }
+        }
+      }
+    }
+  }
+
+  // IV: scale (getBaseGradientMult, MultiRendererDR.cpp:140-145) and loss
+  const double g_mult = (double)(float)(std::pow((double)P + 1.0, 2.0 / 3.0) / (double)(W * H));
+  for (size_t k = 0; k < P; k++) {
+    double s = 0;
+    for (int t = 0; t < nthreads; t++) s += grads[t][k];
+    out_grad[k] += s * g_mult;
+  }
+  double loss = 0;
+  for (double l : loss_v) loss += l;
+  if (out_image)
+    for (size_t k = 0; k < np * 4; k++) out_image[k] = img[k];
+  if (stats) {
+    stats->loss = loss / (double)np;
+    stats->border_pixels = (uint32_t)border.size();
+    stats->border_rays_hit = border_hits;
+    stats->rays = (uint64_t)np * spp + (uint64_t)border.size() * dr.border_spp;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int orc_dr_render_grad(const orc_scene* s, const MultiRendererDRPreset* dr, const float worldView[16], const float proj[16],
+                                  uint32_t W, uint32_t H, uint32_t seed, const float* ref, int precision, int nthreads, int dtdv_mode,
+                                  double* out_image, double* out_grad, orc_dr_stats* stats) {
+  if (!s || !dr || !ref || !out_grad || W < 1 || H < 1 || dr->spp == 0) return -1;
+  Camera cam = Camera::from(worldView, proj);
+  if (precision == 64)
+    return dr_impl<double>(s->s, *dr, cam, worldView, proj, W, H, seed, ref, nthreads, dtdv_mode, out_image, out_grad, stats);
+  return dr_impl<float>(s->s, *dr, cam, worldView, proj, W, H, seed, ref, nthreads, dtdv_mode, out_image, out_grad, stats);
+}
+
+// MultiRendererDR.cpp:828-853
+extern "C" void orc_adam_step(uint32_t iter, const float* dX, float* X, float* V, float* S, uint32_t n, float lr, float beta1,
+                              float beta2, float eps) {
+  for (uint32_t i = 0; i < n; i++) {
+    float g = dX[i];
+    V[i] = beta1 * V[i] + (1 - beta1) * g;
+    float Vh = (float)(V[i] / (1 - std::pow(beta1, iter + 1)));
+    S[i] = beta2 * S[i] + (1 - beta2) * g * g;
+    float Sh = (float)(S[i] / (1 - std::pow(beta2, iter + 1)));
+    float x = X[i] - lr * Vh / (std::sqrt(Sh) + eps);
+    X[i] = std::min(std::max(x, -1000.0f), 1000.0f);
+  }
+}

@@ -1,0 +1,96 @@
+"""CPU-only checks of the library's host side: exported C ABI, scene builders,
+cameras and argument validation (no GPU calls)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+
+import litert_amd.api as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    return set(re.findall(r"^\s*(?:int|const char\*|void)\s+(lrt_\w+)\s*\(", text, re.M))
+
+
+def test_library_exports_every_declared_symbol():
+    declared = _declared("lrt.h")
+    assert len(declared) > 30
+    out = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], stdout=subprocess.PIPE, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = declared - exported
+    assert not missing, missing
+    for name in declared:  # and ctypes can resolve each one
+        assert hasattr(L.lib, name)
+
+
+def test_struct_layouts_match_reference_sizes():
+    # sdf_scene.h:136-161, render_settings.h:86-95, cbvh.h:21-33
+    assert C.sizeof(L.SdfSBSHeader) == 16
+    assert C.sizeof(L.MultiRenderPreset) == 28
+    assert C.sizeof(L.CRT_Hit) == 32
+
+
+def test_sbs_builder_keeps_reference_border_rule():
+    s = L.displaced_icosphere(3, 0.7, 0.05, 2, seed=5, lod=16, brick_size=4, brick_pad=1, bytes_per_value=4)
+    assert s.header.aux_data == L.SDF_SBS_NODE_LAYOUT_DX
+    v = 4 + 2 + 1
+    words = v ** 3
+    assert np.array_equal(s.nodes[:, 2], words * np.arange(s.nodes.shape[0]))
+    d_max = 2 * np.sqrt(3) / 16
+    q = s.values.reshape(-1, words).astype(np.float64)
+    vals = -d_max + (2 * d_max / 4294967295.0) * q
+    # sparse_octree_builder.cpp:49-53: min <= 0 and max > -d_max
+    assert np.all(vals.min(axis=1) <= 1e-9)
+    assert np.all(vals.max(axis=1) > -d_max)
+    # and every cell crossing the zero level set was found: bricks along a radius
+    pos = np.stack([s.nodes[:, 0] >> 16, s.nodes[:, 0] & 0xFFFF, s.nodes[:, 1] >> 16], 1)
+    centre = -1 + (pos + 0.5) * (2.0 / 16)
+    r = np.linalg.norm(centre, axis=1)
+    band = d_max + np.sqrt(3) * (2.0 / 16)  # keep band + brick (with pad) half-diagonals
+    assert r.min() > 0.7 - 0.05 - band and r.max() < 0.7 + 0.05 + band
+
+
+def test_voxel_diagonal_quantisation_flag():
+    s = L.displaced_icosphere(3, 0.7, 0.05, 2, seed=5, lod=16, brick_size=4, brick_pad=0, bytes_per_value=1, voxel_diag_scale=True)
+    assert s.header.aux_data == (L.SDF_SBS_NODE_LAYOUT_DX | 1)
+    assert s.values.size == s.nodes.shape[0] * 32  # 125 bytes -> 32 words per brick
+
+
+def test_grid_sbs_layout():
+    s = L.grid_sbs("torus", 0.6, 0.2, 0.0, 1, seed=1, brick_count=3, brick_size=2, with_neighbors=True)
+    assert s.header.aux_data == L.SDF_SBS_NODE_LAYOUT_ID32F_IRGB32F_IN
+    per = 27 + 8 + 27
+    assert s.values.size == 27 * per
+    p_count = 3 * 2 + 1
+    assert s.values_f.size == p_count ** 3 + 3 * 4 ** 3
+    # distance index of brick (1,1,1) corner (0,0,0) -> lattice point (2,2,2)
+    b = (1 * 3 + 1) * 3 + 1
+    assert s.values[b * per] == (2 * p_count + 2) * p_count + 2
+    # neighbour table: slot 9*(dx+1)+3*(dy+1)+(dz+1)
+    nb = s.values[b * per + 27 + 8: b * per + 27 + 8 + 27]
+    assert nb[0] == 0 and nb[26] == 26 and nb[13] == b
+    corner = s.values[27 + 8: 27 + 8 + 27]  # brick (0,0,0): negative offsets are missing
+    assert corner[0] == L.INVALID_IDX and corner[26] == (1 * 3 + 1) * 3 + 1
+
+
+def test_orbit_cameras_look_at_origin():
+    for i in range(8):
+        wv, pr = L.orbit_camera(2, i)
+        M = wv.reshape(4, 4).T  # column-major -> row-major
+        eye = -M[:3, :3].T @ M[:3, 3]
+        assert abs(np.linalg.norm(eye) - 3.0) < 1e-4
+        fwd = -M[2, :3]  # camera looks down -z
+        assert np.dot(fwd, -eye / np.linalg.norm(eye)) > 1 - 1e-5
+
+
+def test_invalid_arguments_are_rejected():
+    h = C.c_void_p()
+    assert L.lib.lrt_create(L.DEVICE_CPU, 0, C.byref(h)) != 0  # CPU device is not served by this library
+    assert L.lib.lrt_sbs_grid(1, 0.6, 0.2, 0.0, 1, 1, 0, 4, 0, C.byref(h)) != 0
+    assert b"bad" in L.lib.lrt_last_error()
+    assert L.lib.lrt_sbs_load_file(b"/nonexistent/file.bin", C.byref(h)) != 0

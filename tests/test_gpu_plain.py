@@ -189,6 +189,7 @@ def test_brute_force_topology_independent():
     sbs = L.displaced_icosphere(3, 0.7, 0.05, 3, seed=4, lod=8, brick_size=4, brick_pad=0, bytes_per_value=2)
     r = L.CreateMultiRenderer(L.DEVICE_GPU)
     r.SetScene(sbs)
+    r.SetPreset(preset())
     pairs, _ = r.GetBVH()
     sc = oracle.Scene(sbs, pairs)
     wv, pr = L.orbit_camera(9, 0)
