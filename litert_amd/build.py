@@ -68,5 +68,15 @@ def build(verbose=False, jobs=None):
     return LIB
 
 
+def build_cpp_tests(oracle_lib):
+    """C++ tests that use the facade header (include/litert_amd/MultiRenderer.hpp)."""
+    out_dir = os.path.join(ROOT, "tests", "cpp", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, "test_facade_render")
+    _run(["g++", "-O2", "-std=c++17", os.path.join(ROOT, "tests", "cpp", "test_facade_render.cpp"), "-o", exe,
+          LIB, oracle_lib, "-Wl,-rpath," + HERE, "-Wl,-rpath," + os.path.dirname(oracle_lib), "-fopenmp"])
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
