@@ -7,13 +7,14 @@ icosphere (4 noise octaves, 512^3 effective = brick LOD 128 x 4^3 voxels,
 G-buffer written to HBM. Scene build and upload happen before the timed
 region; each step is one kernel launch.
 
-Prints one JSON line (see the driver contract). `roofline` uses the
-algorithmic bytes per launch from the kernel's own work counters:
-64 B per BVH node pair, 16 B per brick record, 8 x bytes_per_value per voxel
-corner fetch and 16 B of output per ray, divided by the kernel's average
-duration (HIP events on the launch stream). `cpu_baseline` times the CPU
-oracle (the reference CPU path restated in C++, FP32, OpenMP over all host
-cores) on a bounded sample of the same views.
+Prints one JSON line. `roofline` uses the algorithmic bytes per launch from
+the kernel's own work counters: 64 B per BVH node pair, 16 B per brick record,
+8 x bytes_per_value per voxel-corner fetch and 16 B of output per ray, divided
+by the kernel's average duration (HIP events on the launch stream).
+`cpu_baseline` times the CPU oracle (the reference CPU path restated in C++,
+FP32, OpenMP over the host cores) on the same workload. The DR leg
+(`dr_fwd_bwd_mrays_s`) times RenderDR + Adam on config 4 (torus target,
+sphere start, 128^3 indexed SBS, 16 views at 512x512, border_spp 256).
 """
 import argparse
 import json
@@ -35,13 +36,53 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic():
+    """HBM bytes per plain-render launch from the FETCH_SIZE / WRITE_SIZE counter
+    passes (scaled by a 1 GiB copy), recorded in profiles/k_render_c2_hbm.json."""
+    path = os.path.join(ROOT, "profiles", "k_render_c2_hbm.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def plain_scene(L):
+    return L.displaced_icosphere(5, 0.8, 0.05, 4, seed=SCENE_SEED, lod=128, brick_size=4, brick_pad=0, bytes_per_value=1,
+                                 voxel_diag_scale=True)
+
+
+def dr_problem(L, np, views=16, res=512, brick_count=32, brick_size=4):
+    """Config 4: noisy torus (R 0.6, r 0.2) target, sphere r 0.5 start, dense
+    indexed SBS of 128^3 voxels, `views` seeded views at res x res."""
+    tgt = L.grid_sbs("torus", 0.6, 0.2, 0.02, 3, seed=4, brick_count=brick_count, brick_size=brick_size)
+    ini = L.grid_sbs("sphere", 0.5, 0.0, 0.0, 3, seed=4, brick_count=brick_count, brick_size=brick_size)
+    cams = [L.orbit_camera(4, i) for i in range(views)]
+    r = L.CreateMultiRenderer(L.DEVICE_GPU)
+    r.SetScene(tgt)
+    p = L.getDefaultPreset()
+    p.sdf_node_intersect = L.SDF_OCTREE_NODE_INTERSECT_NEWTON
+    p.render_mode = L.MULTI_RENDER_MODE_LINEAR_DEPTH
+    refs = np.stack([r.RenderFloat(res, res, wv, pr, p) for wv, pr in cams])
+    r.close()
+    dr = L.getDefaultPresetDR()
+    dr.dr_render_mode = 3  # DR_RENDER_MODE_LINEAR_DEPTH
+    dr.dr_input_type = 1   # DR_INPUT_TYPE_LINEAR_DEPTH
+    dr.dr_reconstruction_flags = 2  # geometry
+    dr.border_spp = 256
+    dr.opt_lr = 1e-3
+    return tgt, ini, cams, refs, dr
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-views", type=int, default=1, help="views in the CPU baseline sample")
+    ap.add_argument("--no-dr", action="store_true", help="skip the DR leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU work for the baseline sample")
+    ap.add_argument("--calib-copy-mib", type=int, default=0, help="append a device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); every GPU box here has a single MI355X "
@@ -52,8 +93,7 @@ def main():
     import litert_amd.api as L
 
     t0 = time.time()
-    sbs = L.displaced_icosphere(5, 0.8, 0.05, 4, seed=SCENE_SEED, lod=128, brick_size=4, brick_pad=0,
-                                bytes_per_value=1, voxel_diag_scale=True)
+    sbs = plain_scene(L)
     log(f"scene: {sbs.nodes.shape[0]} bricks, {sbs.values.size * 4 / 1e6:.1f} MB values, built in {time.time() - t0:.1f}s")
     r = L.CreateMultiRenderer(L.DEVICE_GPU)
     r.SetScene(sbs)
@@ -85,24 +125,83 @@ def main():
     bpv = sbs.header.bytes_per_value
     alg_bytes = 64 * int(counts[0]) + 16 * int(counts[1]) + 8 * bpv * int(counts[2]) + 16 * rays
     achieved = alg_bytes / (avg_kernel_ms * 1e-3) / 1e9
-    log(f"work/ray: pairs {counts[0] / rays:.2f} bricks {counts[1] / rays:.3f} voxels {counts[2] / rays:.3f}; "
-        f"algorithmic {alg_bytes / 1e9:.2f} GB/launch -> {achieved:.0f} GB/s")
+    log(f"work/ray: pairs {counts[0] / rays:.3f} bricks {counts[1] / rays:.4f} voxels {counts[2] / rays:.4f}; "
+        f"algorithmic {alg_bytes / 1e9:.3f} GB/launch -> {achieved:.0f} GB/s")
+
+    dr_mrays = None
+    dr_info = None
+    if not args.no_dr:
+        tgt, ini, dcams, refs, dr = dr_problem(L, np)
+        g = L.MultiRendererDR()
+        g.SetScene(ini)
+        dwv = np.stack([c[0] for c in dcams])
+        dpr = np.stack([c[1] for c in dcams])
+        g.SetReference(refs, dwv, dpr)
+        g.RenderDR(dr, 0, seed=1)  # warm-up
+        dr.opt_iterations = 16
+        dr.image_batch_size = 1
+        work = L.SdfSBS(ini.header, ini.nodes, ini.values, ini.values_f.copy())
+        losses, total_ms = g.OptimizeFixedStructure(dr, work, seed=1)
+        dres = refs.shape[1]
+        dr_mrays = dres * dres * dr.spp * dr.opt_iterations / (total_ms * 1e-3) / 1e6
+        view_ms = [g.RenderDR(dr, v, seed=1)[3] for v in range(4)]
+        bp, bh = g.LastStats()
+        dr_info = {"views": int(refs.shape[0]), "res": int(dres), "border_spp": int(dr.border_spp), "iterations": int(dr.opt_iterations),
+                   "ms_per_iteration": round(total_ms / dr.opt_iterations, 3), "renderdr_kernel_ms": round(float(np.mean(view_ms)), 3),
+                   "last_view_border_pixels": int(bp), "loss_first_last": [float(losses[0]), float(losses[-1])]}
+        log(f"dr: {dr_mrays:.1f} Mrays/s (fwd+bwd+Adam, {total_ms / dr.opt_iterations:.2f} ms/iter), RenderDR kernels "
+            f"{np.mean(view_ms):.2f} ms/view, border pixels {bp}")
 
     cpu = None
     if not args.no_cpu:
         import oracle
         sc = oracle.Scene(sbs, pairs)
-        nthreads = os.cpu_count() or 1
-        nv = max(1, min(args.cpu_views, N_VIEWS))
+        nthreads = min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "0")) or 1 << 30)
         sc.render(wvs[0], prs[0], p, 256, 256, precision=32, threads=nthreads)  # warm-up
+        reps, tcpu = 0, 0.0
         tc = time.perf_counter()
-        for v in range(nv):
-            sc.render(wvs[v], prs[v], p, W, H, precision=32, threads=nthreads)
-        tcpu = time.perf_counter() - tc
-        cpu = {"value": nv * W * H / tcpu / 1e6, "unit": "Mrays/s", "cores": nthreads, "kind": "port",
-               "sample": f"{nv} of the {N_VIEWS} views at {W}x{H} (same scene, cameras and preset), FP32 oracle, "
-                         f"OpenMP {nthreads} threads, {tcpu:.1f}s"}
-        log(f"cpu baseline: {cpu['value']:.2f} Mrays/s on {nthreads} threads")
+        while tcpu < args.cpu_seconds and reps < 64:
+            for v in range(N_VIEWS):
+                sc.render(wvs[v], prs[v], p, W, H, precision=32, threads=nthreads)
+            reps += 1
+            tcpu = time.perf_counter() - tc
+        cpu_mrays = reps * rays / tcpu / 1e6
+        t1 = time.perf_counter()
+        sc.render(wvs[0], prs[0], p, 1024, 1024, precision=32, threads=1)
+        one_thread = 1024 * 1024 / (time.perf_counter() - t1) / 1e6
+        cpu = {"value": round(cpu_mrays, 2), "unit": "Mrays/s", "cores": nthreads, "kind": "port",
+               "sample": f"the full workload (8 views x {W}x{H}, same scene, cameras, Newton preset) repeated {reps}x "
+                         f"({tcpu:.1f}s), FP32 oracle, OpenMP {nthreads} threads; 1 thread: {one_thread:.2f} Mrays/s on one "
+                         f"1024x1024 view"}
+        log(f"cpu baseline: {cpu_mrays:.2f} Mrays/s on {nthreads} threads, {one_thread:.2f} on 1 thread")
+        if dr_info is not None:
+            tgt, ini, dcams, refs, dr = dr_problem(L, np, views=1)
+            ip, _ = L.build_bvh(ini)
+            si = oracle.Scene(ini, ip)
+            ref = refs[0].copy()
+            ref[..., 3] = np.linalg.norm(ref[..., :3], axis=-1) >= 0.001
+            t2 = time.perf_counter()
+            si.dr_render_grad(dr, dcams[0][0], dcams[0][1], refs.shape[2], refs.shape[1], ref, ini.values_f.size, seed=1,
+                              precision=32, threads=nthreads)
+            dt = time.perf_counter() - t2
+            dr_info["cpu_baseline_mrays_s"] = round(refs.shape[1] * refs.shape[2] / dt / 1e6, 3)
+            dr_info["cpu_baseline_sample"] = f"one RenderDR view (fwd+bwd incl. border rays), FP32 oracle, {nthreads} threads, {dt:.2f}s"
+            log(f"dr cpu baseline: {dr_info['cpu_baseline_mrays_s']} Mrays/s")
+
+    if args.calib_copy_mib:
+        # plain device-to-device copy through the HIP runtime's blit kernel:
+        # n bytes read and n bytes written, the scale for HBM byte counters
+        import ctypes as C
+        hip = C.CDLL("libamdhip64.so.7")
+        n = C.c_size_t(args.calib_copy_mib << 20)
+        a, b = C.c_void_p(), C.c_void_p()
+        for call in (lambda: hip.hipMalloc(C.byref(a), n), lambda: hip.hipMalloc(C.byref(b), n),
+                     lambda: hip.hipMemset(a, 1, n), lambda: hip.hipMemcpy(b, a, n, 3), lambda: hip.hipDeviceSynchronize(),
+                     lambda: hip.hipFree(a), lambda: hip.hipFree(b)):
+            rc = call()
+            if rc != 0:
+                raise RuntimeError(f"HIP call failed with {rc} in the calibration copy")
+        log(f"calibration copy: {n.value} bytes read + written")
 
     out = {
         "metric": "primary Mrays/s on SBS (plain vs MLP-seeded); DR fwd+bwd Mrays/s",
@@ -121,10 +220,12 @@ def main():
                    "bricks": int(sbs.nodes.shape[0]), "rays_per_step": rays, "parallelism": "single GPU"},
         "kernel_ms_per_step": round(avg_kernel_ms, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
+                     "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "mlp_seeded_mrays_s": None,
-        "dr_fwd_bwd_mrays_s": None,
+        "dr_fwd_bwd_mrays_s": None if dr_mrays is None else round(dr_mrays, 2),
+        "dr": dr_info,
     }
     print(json.dumps(out), flush=True)
 

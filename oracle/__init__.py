@@ -82,7 +82,7 @@ def lib():
 
 class Counters(C.Structure):
     _fields_ = [("node_pairs", C.c_uint64), ("bricks", C.c_uint64), ("voxels", C.c_uint64),
-                ("root_finds", C.c_uint64), ("rays", C.c_uint64)]
+                ("root_finds", C.c_uint64), ("rays", C.c_uint64), ("root_iters", C.c_uint64)]
 
 
 class DRStats(C.Structure):

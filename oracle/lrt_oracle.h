@@ -22,7 +22,7 @@ extern "C" {
 typedef struct orc_scene orc_scene;
 
 typedef struct orc_counters {
-  uint64_t node_pairs, bricks, voxels, root_finds, rays;
+  uint64_t node_pairs, bricks, voxels, root_finds, rays, root_iters;
 } orc_counters;
 
 /* Copies the SBS arrays and the fat BVH2 pairs (root pair at index 0). */

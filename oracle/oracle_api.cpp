@@ -88,7 +88,7 @@ static void render_impl(const Scene& S, const Camera& cam, const MultiRenderPres
   }
   if (cnt) {
     cnt->node_pairs = total.node_pairs; cnt->bricks = total.bricks; cnt->voxels = total.voxels;
-    cnt->root_finds = total.root_finds; cnt->rays = total.rays;
+    cnt->root_finds = total.root_finds; cnt->rays = total.rays; cnt->root_iters = total.root_iters;
   }
 }
 

@@ -18,7 +18,11 @@ struct Counters {
   uint64_t voxels = 0;       // 8-corner loads (BVH2Common.cpp:801-817)
   uint64_t root_finds = 0;   // LocalSurfaceIntersection calls
   uint64_t rays = 0;
-  void add(const Counters& o) { node_pairs += o.node_pairs; bricks += o.bricks; voxels += o.voxels; root_finds += o.root_finds; rays += o.rays; }
+  uint64_t root_iters = 0;   // ST / IT / Newton iterations inside LocalSurfaceIntersection
+  void add(const Counters& o) {
+    node_pairs += o.node_pairs; bricks += o.bricks; voxels += o.voxels; root_finds += o.root_finds; rays += o.rays;
+    root_iters += o.root_iters;
+  }
 };
 
 // Scene as the reference BVHRT holds it after AddGeom_SdfSBS + CommitScene
@@ -282,6 +286,7 @@ template <typename T> struct Tracer {
             rtn -= dx;
             it++;
           }
+          iter = it;
           t = rtn;
           hit = (t >= 0 && t <= qFar && std::abs(f) < EPS);
         } else {
@@ -313,6 +318,7 @@ template <typename T> struct Tracer {
     last_t = t;
     last_hit = (t <= qFar && hit);
     last_iter = iter;
+    if (cnt) cnt->root_iters += iter;
     if (t <= qFar && hit && tReal < pHit->t) {  // BVH2Common.cpp:572-602
       V3<T> norm{0, 0, 1};
       if (need_normal(O)) {
