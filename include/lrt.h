@@ -125,6 +125,59 @@ int lrt_dr_get_last_dloss_ds(lrt_handle h, float* out, uint64_t n);
 /* Border statistics of the last RenderDR: border pixels and border rays that found a relaxation point. */
 int lrt_dr_last_stats(lrt_handle h, uint32_t* border_pixels, uint32_t* border_rays_hit);
 
+/* ---- learned root-find seeder (no reference counterpart) ----------------
+ * A 24 -> 64 -> 64 -> 64 -> 3 SiLU network read at every brick entry predicts
+ * where the ray meets the surface inside the brick (z0 = t_hit - margin,
+ * z1 = t_hit + margin, brick units from the brick entry) and a hit logit z2.
+ * With seeding on, a prediction whose logit clears logit(threshold) becomes
+ * the start point of the exact root finder in the candidate voxel (Newton:
+ * when inside the exact bracket; sphere tracing: when the cubic is shown to
+ * stay above the stopping threshold up to it). The exact intersector decides
+ * every hit; the BVH order and voxel walk are unchanged. Parameters are
+ * LRT_MLP_PARAMS floats, row-major out x in: W1[64x24] b1 W2[64x64] b2
+ * W3[64x64] b3 W4[3x64] b4. Inference runs in BF16 on MFMA, training in FP32. */
+#define LRT_MLP_PARAMS 10115u
+#define LRT_MLP_FEATURES 24u
+/* Uniform Glorot initialisation from a seeded generator, zero biases. */
+int lrt_mlp_init(lrt_handle h, uint32_t seed);
+int lrt_mlp_set_params(lrt_handle h, const float* params, uint32_t n);
+int lrt_mlp_get_params(lrt_handle h, float* params, uint32_t n);
+/* Appends training events from a plain render of the current scene (first
+ * sample of each pixel): one event per brick the traversal enters, with its
+ * 24 features, target (t_hit in brick units from the brick entry or 0, hit
+ * 0/1, margin = half a voxel) and meta (ray = view*W*H + y*W + x, brick).
+ * At most max_events are kept; *n_added receives the number kept and
+ * *n_seen (may be NULL) every event the render produced. */
+int lrt_mlp_collect(lrt_handle h, const float* world_views, const float* projs, uint32_t n_views, uint32_t width, uint32_t height,
+                    uint64_t max_events, uint64_t* n_added, uint64_t* n_seen);
+int lrt_mlp_dataset_size(lrt_handle h, uint64_t* n);
+/* Copies events [first, first+count) to host arrays (feats 24, tgts 3,
+ * meta 2 uint32 per event); any output may be NULL. */
+int lrt_mlp_dataset_read(lrt_handle h, uint64_t first, uint64_t count, float* feats, float* tgts, uint32_t* meta);
+int lrt_mlp_dataset_clear(lrt_handle h);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) over the collected events for `epochs`
+ * passes in minibatches of `batch` (a multiple of 64), sample order a seeded
+ * affine permutation per epoch. epoch_losses (epochs floats) and total_ms
+ * may be NULL. */
+int lrt_mlp_train(lrt_handle h, uint32_t epochs, uint32_t batch, float lr, uint32_t seed, float* epoch_losses, float* total_ms);
+/* Mean loss and its gradient (FP32 training kernels) over n host samples at
+ * the current parameters; n <= 65536. */
+int lrt_mlp_loss_grad(lrt_handle h, const float* feats, const float* tgts, uint32_t n, float* grad, float* loss);
+/* BF16 MFMA inference of n host feature rows -> out (n x 3). */
+int lrt_mlp_infer(lrt_handle h, const float* feats, uint64_t n, float* out, float* kernel_ms);
+/* BF16 MFMA inference over the whole collected dataset on the device;
+ * kernel_ms is the average of `repeats` launches; the first n_out rows of the
+ * result are copied to out (may be NULL). */
+int lrt_mlp_infer_dataset(lrt_handle h, uint32_t repeats, float* kernel_ms, float* out, uint64_t n_out);
+/* Turns seeded rendering on or off for every render entry point of h;
+ * threshold is the hit-probability gate in (0, 1) (0.9 by default). */
+int lrt_set_seeding(lrt_handle h, int enabled, float threshold);
+/* lrt_count_work with more counters: out[0] node pairs, [1] bricks, [2]
+ * voxels, [3] root-finder iterations, [4] 16-sample MLP tiles, [5] brick
+ * entries whose prediction cleared the gate. n_out <= 6. */
+int lrt_count_work_ex(lrt_handle h, const float* world_views, const float* projs, uint32_t n_views, uint32_t width, uint32_t height,
+                      uint64_t* out, uint32_t n_out);
+
 /* ---- synthetic scenes (no reference counterpart; the closest reference
  * builders are utils/sparse_octree_builder.cpp:291-444 and
  * diff_render/DR_common.cpp:47-146) ---------------------------------------- */

@@ -72,6 +72,10 @@ class MultiRendererDRPreset(C.Structure):
     ]
 
 
+MLP_PARAMS = 10115
+MLP_FEATURES = 24
+
+
 def getDefaultPreset():
     """render_settings.h:94-108."""
     return MultiRenderPreset(MULTI_RENDER_MODE_LAMBERT_NO_TEX, SDF_OCTREE_NODE_INTERSECT_ST, NORMAL_MODE_GEOMETRY,
@@ -152,6 +156,19 @@ def _load():
                                   C.POINTER(f32)]),
         "lrt_dr_get_last_dloss_ds": (i32, [P, P, u64]),
         "lrt_dr_last_stats": (i32, [P, C.POINTER(u32), C.POINTER(u32)]),
+        "lrt_mlp_init": (i32, [P, u32]),
+        "lrt_mlp_set_params": (i32, [P, P, u32]),
+        "lrt_mlp_get_params": (i32, [P, P, u32]),
+        "lrt_mlp_collect": (i32, [P, P, P, u32, u32, u32, u64, C.POINTER(u64), C.POINTER(u64)]),
+        "lrt_mlp_dataset_size": (i32, [P, C.POINTER(u64)]),
+        "lrt_mlp_dataset_read": (i32, [P, u64, u64, P, P, P]),
+        "lrt_mlp_dataset_clear": (i32, [P]),
+        "lrt_mlp_train": (i32, [P, u32, u32, f32, u32, P, C.POINTER(f32)]),
+        "lrt_mlp_loss_grad": (i32, [P, P, P, u32, P, C.POINTER(f32)]),
+        "lrt_mlp_infer": (i32, [P, P, u64, P, C.POINTER(f32)]),
+        "lrt_mlp_infer_dataset": (i32, [P, u32, C.POINTER(f32), P, u64]),
+        "lrt_set_seeding": (i32, [P, i32, f32]),
+        "lrt_count_work_ex": (i32, [P, P, P, u32, u32, u32, P, u32]),
     })
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -379,6 +396,80 @@ class MultiRenderer:
         out = np.zeros(3, np.uint64)
         _check(lib.lrt_count_work(self._h, _ptr(wv), _ptr(pr), wv.shape[0], width, height, _ptr(out)), "CountWork")
         return out
+
+    def CountWorkEx(self, world_views, projs, width, height):
+        """(pairs, bricks, voxels, root iterations, MLP tiles, gated seeds) of one counted launch."""
+        wv, pr = _f32(world_views).reshape(-1, 16), _f32(projs).reshape(-1, 16)
+        out = np.zeros(6, np.uint64)
+        _check(lib.lrt_count_work_ex(self._h, _ptr(wv), _ptr(pr), wv.shape[0], width, height, _ptr(out), 6), "CountWorkEx")
+        return out
+
+    # ---- learned root-find seeder (no reference counterpart) ----
+    def MlpInit(self, seed=0):
+        _check(lib.lrt_mlp_init(self._h, seed), "MlpInit")
+
+    def MlpSetParams(self, params):
+        p = _f32(params).ravel()
+        _check(lib.lrt_mlp_set_params(self._h, _ptr(p), p.size), "MlpSetParams")
+
+    def MlpGetParams(self):
+        out = np.zeros(MLP_PARAMS, np.float32)
+        _check(lib.lrt_mlp_get_params(self._h, _ptr(out), out.size), "MlpGetParams")
+        return out
+
+    def MlpCollect(self, world_views, projs, width, height, max_events):
+        """Appends training events of one plain render; returns (kept, produced)."""
+        wv, pr = _f32(world_views).reshape(-1, 16), _f32(projs).reshape(-1, 16)
+        a, b = C.c_uint64(), C.c_uint64()
+        _check(lib.lrt_mlp_collect(self._h, _ptr(wv), _ptr(pr), wv.shape[0], width, height, max_events, C.byref(a), C.byref(b)),
+               "MlpCollect")
+        return a.value, b.value
+
+    def MlpDatasetSize(self):
+        n = C.c_uint64()
+        _check(lib.lrt_mlp_dataset_size(self._h, C.byref(n)), "MlpDatasetSize")
+        return n.value
+
+    def MlpDataset(self, first=0, count=None):
+        n = self.MlpDatasetSize()
+        count = n - first if count is None else count
+        f = np.zeros((count, MLP_FEATURES), np.float32)
+        t = np.zeros((count, 3), np.float32)
+        m = np.zeros((count, 2), np.uint32)
+        _check(lib.lrt_mlp_dataset_read(self._h, first, count, _ptr(f), _ptr(t), _ptr(m)), "MlpDataset")
+        return f, t, m
+
+    def MlpDatasetClear(self):
+        _check(lib.lrt_mlp_dataset_clear(self._h), "MlpDatasetClear")
+
+    def MlpTrain(self, epochs, batch=16384, lr=1e-3, seed=0):
+        losses = np.zeros(epochs, np.float32)
+        ms = C.c_float()
+        _check(lib.lrt_mlp_train(self._h, epochs, batch, lr, seed, _ptr(losses), C.byref(ms)), "MlpTrain")
+        return losses, ms.value
+
+    def MlpLossGrad(self, feats, tgts):
+        f, t = _f32(feats).reshape(-1, MLP_FEATURES), _f32(tgts).reshape(-1, 3)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        _check(lib.lrt_mlp_loss_grad(self._h, _ptr(f), _ptr(t), f.shape[0], _ptr(g), C.byref(loss)), "MlpLossGrad")
+        return loss.value, g
+
+    def MlpInfer(self, feats):
+        f = _f32(feats).reshape(-1, MLP_FEATURES)
+        out = np.zeros((f.shape[0], 3), np.float32)
+        ms = C.c_float()
+        _check(lib.lrt_mlp_infer(self._h, _ptr(f), f.shape[0], _ptr(out), C.byref(ms)), "MlpInfer")
+        return out, ms.value
+
+    def MlpInferDataset(self, repeats=10, n_out=0):
+        out = np.zeros((n_out, 3), np.float32)
+        ms = C.c_float()
+        _check(lib.lrt_mlp_infer_dataset(self._h, repeats, C.byref(ms), _ptr(out) if n_out else None, n_out), "MlpInferDataset")
+        return ms.value, out
+
+    def SetSeeding(self, enabled, threshold=0.9):
+        _check(lib.lrt_set_seeding(self._h, int(bool(enabled)), threshold), "SetSeeding")
 
     def Sync(self):
         _check(lib.lrt_sync(self._h), "Sync")

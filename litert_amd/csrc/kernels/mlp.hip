@@ -1,0 +1,522 @@
+// mlp.hip - kernels of the learned root-find seeder (no reference
+// counterpart; the seeded render keeps the reference traversal order and
+// hit test, BVH2Common.cpp:3407-3488 and 764-984):
+//   k_brick_features  per-brick corner / octant-minimum table
+//   k_mlp_collect     training events from the plain traversal
+//   k_mlp_infer       BF16 MFMA inference over a feature batch
+//   k_mlp_grad        FP32 forward + backward, per-block partial gradients
+//   k_mlp_reduce_adam deterministic gradient reduction + Adam
+//   k_render_seeded   primary rays with the network consulted at every brick
+#include "mlp.hpp"
+#include "mlp_device.hpp"
+#include "render_device.hpp"
+#include "traverse_device.hpp"
+
+namespace lrtd {
+
+// LDS writes of one lane become visible to the other lanes of its wave.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// ---------------------------------------------------------------- features --
+__global__ __launch_bounds__(256) void k_brick_features(DevScene S, float4* __restrict__ out) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= S.n_nodes) return;
+  const uint4 N = S.nodes[b];
+  const float sz = (float)(N.y & 0xFFFFu);
+  const float sz_inv = 2.0f / sz;
+  const float s = 0.5f * sz;
+  const uint32_t bs = S.brick_size;
+  float corner[8], omin[8], vals[8];
+  for (int i = 0; i < 8; i++) { corner[i] = 0.0f; omin[i] = 1e30f; }
+  float all_min = 1e30f;
+  for (uint32_t vx = 0; vx < bs; vx++)
+    for (uint32_t vy = 0; vy < bs; vy++)
+      for (uint32_t vz = 0; vz < bs; vz++) {
+        const float vmin = load_voxel(S, N.z, vx, vy, vz, sz_inv, vals);
+        const int o = ((2 * vx >= bs) ? 4 : 0) | ((2 * vy >= bs) ? 2 : 0) | ((2 * vz >= bs) ? 1 : 0);
+        omin[o] = mn(omin[o], vmin);
+        all_min = mn(all_min, vmin);
+        for (int c = 0; c < 8; c++) {
+          const uint32_t cx = (c >> 2) & 1, cy = (c >> 1) & 1, cz = c & 1;
+          if (vx == cx * (bs - 1) && vy == cy * (bs - 1) && vz == cz * (bs - 1)) corner[c] = vals[c];
+        }
+      }
+  for (int o = 0; o < 8; o++)
+    if (omin[o] > 1e29f) omin[o] = all_min;  // octant without voxels (brick_size 1)
+  out[4 * (size_t)b + 0] = make_float4(corner[0] * s, corner[1] * s, corner[2] * s, corner[3] * s);
+  out[4 * (size_t)b + 1] = make_float4(corner[4] * s, corner[5] * s, corner[6] * s, corner[7] * s);
+  out[4 * (size_t)b + 2] = make_float4(omin[0] * s, omin[1] * s, omin[2] * s, omin[3] * s);
+  out[4 * (size_t)b + 3] = make_float4(omin[4] * s, omin[5] * s, omin[6] * s, omin[7] * s);
+}
+
+// ------------------------------------------------------------- collection --
+// One lane per packed pixel (spp 1, the preset's first sample offset). Every
+// brick the plain traversal enters yields one event: its features, whether
+// the exact intersector hit inside it and where (brick units from the brick
+// entry), and the half-voxel margin used by the interval targets.
+__global__ __launch_bounds__(256) void k_mlp_collect(DevScene S, TraceParams P, const float4* __restrict__ bfeat,
+                                                     const Camera* __restrict__ cams, RenderArgs A, float* __restrict__ feat,
+                                                     float* __restrict__ tgt, MlpEventMeta* __restrict__ meta, uint64_t cap,
+                                                     unsigned long long* count, uint32_t* work_counter, uint32_t* spill) {
+  __shared__ uint32_t lds_stack[kStackLds * 256];
+  __shared__ uint32_t chunk_base;
+  Stack st;
+  st.lds = lds_stack + threadIdx.x;
+  st.lds_stride = 256;
+  st.lds_entries = kStackLds;
+  st.spill = spill + blockIdx.x * 256 + threadIdx.x;
+  st.spill_stride = gridDim.x * 256;
+  const uint64_t total = (uint64_t)A.n_views * A.n_packed;
+  const float margin = 0.5f / (float)S.brick_size;
+  for (;;) {
+    if (threadIdx.x == 0) chunk_base = atomicAdd(work_counter, 1u);
+    __syncthreads();
+    const uint64_t base = (uint64_t)chunk_base * 256;
+    __syncthreads();
+    if (base >= total) break;
+    const uint64_t gtid = base + threadIdx.x;
+    if (gtid >= total) continue;
+    const uint32_t view = (uint32_t)(gtid / A.n_packed);
+    const uint32_t tid = (uint32_t)(gtid - (uint64_t)view * A.n_packed);
+    uint32_t x, y;
+    tid_to_xy(tid, A.packed_w, x, y);
+    if (x >= A.W || y >= A.H) continue;
+    const float2 d = sample_offset(A, x, y, 0);
+    v3 pos, dir;
+    eye_ray(cams[view], x, y, d.x, d.y, A.W, A.H, pos, dir);
+    const v3 inv = SafeInverse(dir);
+    Hit hit;
+    hit.t = 1e9f;
+    hit.primId = hit.instId = hit.geomId = 0xFFFFFFFFu;
+    hit.coords[0] = 1.0f; hit.coords[1] = 0.0f; hit.coords[2] = 0.0f; hit.coords[3] = 0.0f;
+    const float2 bh = RayBox(pos, inv, v3(-1, -1, -1), v3(1, 1, 1));
+    if (!((bh.x <= bh.y) && (bh.y > 0.0f) && (bh.x < hit.t))) continue;
+    int top = 0;
+    uint32_t left = 0;
+    const uint32_t ray_id = view * A.W * A.H + y * A.W + x;
+    while (NextLeaf<false>(S, pos, inv, 0.0f, false, hit.t, top, left, st, nullptr)) {
+      const uint32_t brick = left & LRT_START_MASK;
+      float f[24], t0, s;
+      mlp_features(S, bfeat, pos, dir, inv, brick, f, &t0, &s);
+      const float tPrev = hit.t;
+      BrickIntersect(S, P, pos, dir, inv, 0.1f, brick, &hit, nullptr);  // tNear for SDF leaves = max(0, 0.1)
+      const bool hb = hit.t < tPrev;
+      const unsigned long long idx = atomicAdd(count, 1ull);
+      if (idx < cap) {
+        float4* fo = (float4*)(feat + idx * 24);
+#pragma unroll
+        for (int q = 0; q < 6; q++) fo[q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
+        tgt[idx * 3 + 0] = hb ? (hit.t - t0) * s : 0.0f;
+        tgt[idx * 3 + 1] = hb ? 1.0f : 0.0f;
+        tgt[idx * 3 + 2] = margin;
+        meta[idx].ray = ray_id;
+        meta[idx].brick = brick;
+      }
+      if (hb) break;
+      top--;
+      left = st.get(top > 0 ? top : 0);
+    }
+  }
+}
+
+// -------------------------------------------------------------- inference --
+__global__ __launch_bounds__(256) void k_mlp_infer(const void* __restrict__ packed, const float* __restrict__ feat, uint64_t n,
+                                                   float* __restrict__ out) {
+  __shared__ __align__(16) uint16_t fl[4][64 * kFeatStride];
+  __shared__ float ol[4][64 * 3];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (uint64_t base = ((uint64_t)blockIdx.x * 4 + wave) * 64; base < n; base += (uint64_t)gridDim.x * 4 * 64) {
+    const uint64_t s = base + lane;
+    float f[24];
+    if (s < n) {
+      const float4* fi = (const float4*)(feat + s * 24);
+#pragma unroll
+      for (int q = 0; q < 6; q++) {
+        const float4 v = fi[q];
+        f[4 * q] = v.x; f[4 * q + 1] = v.y; f[4 * q + 2] = v.z; f[4 * q + 3] = v.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 24; k++) f[k] = 0.0f;
+    }
+    store_features_bf16(fl[wave] + lane * kFeatStride, f);
+    wave_lds_sync();
+    const uint64_t left = n - base;
+    const int cnt = left < 64 ? (int)left : 64;
+    mlp_wave_forward(packed, fl[wave], ol[wave], (cnt + 15) >> 4, lane);
+    wave_lds_sync();
+    if (s < n) {
+      out[s * 3 + 0] = ol[wave][3 * lane + 0];
+      out[s * 3 + 1] = ol[wave][3 * lane + 1];
+      out[s * 3 + 2] = ol[wave][3 * lane + 2];
+    }
+    wave_lds_sync();
+  }
+}
+
+// --------------------------------------------------------------- training --
+// One 64-lane block = 64 samples in FP32. Activations live in LDS as
+// [unit][sample] rows of 65 floats (the odd stride keeps both the per-lane
+// column walks and the per-parameter row sums free of bank conflicts).
+// Weights are read with wave-uniform addresses.
+constexpr int kRS = 65;
+
+__device__ __forceinline__ float silu_acc(float z) { return z / (1.0f + expf(-z)); }
+__device__ __forceinline__ float dsilu_acc(float z) {
+  const float sg = 1.0f / (1.0f + expf(-z));
+  return sg * (1.0f + z * (1.0f - sg));
+}
+
+// partial[o*cols + i] = sum_s D[o][s] * X[i][s]; bias[o] = sum_s D[o][s]
+__device__ __forceinline__ void block_outer(const float* D, const float* X, int rows, int cols, float* __restrict__ gW,
+                                            float* __restrict__ gB, int lane) {
+  for (int p = lane; p < rows * cols; p += 64) {
+    const int o = p / cols, i = p - o * cols;
+    float acc = 0.0f;
+    for (int s = 0; s < 64; s++) acc += D[o * kRS + s] * X[i * kRS + s];
+    gW[p] = acc;
+  }
+  for (int o = lane; o < rows; o += 64) {
+    float acc = 0.0f;
+    for (int s = 0; s < 64; s++) acc += D[o * kRS + s];
+    gB[o] = acc;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ prm, MlpTrainArgs T, float* __restrict__ partial,
+                                                 float* __restrict__ partial_loss) {
+  __shared__ float A0[kMlpIn * kRS], Z1[kMlpH * kRS], Z2[kMlpH * kRS], Z3[kMlpH * kRS], DZ[kMlpOut * kRS], ACT[kMlpH * kRS];
+  __shared__ float red[64];
+  const int lane = threadIdx.x;
+  const uint64_t local = (uint64_t)blockIdx.x * 64 + lane;
+  const bool valid = local < T.batch;
+  const uint64_t idx = valid ? (T.perm_a * (T.first + local) + T.perm_b) % T.n : 0;
+  float a[kMlpH];
+  float x[kMlpIn];
+#pragma unroll
+  for (int k = 0; k < kMlpIn; k++) {
+    x[k] = valid ? T.feats[idx * kMlpIn + k] : 0.0f;
+    A0[k * kRS + lane] = x[k];
+  }
+  // forward
+  for (int o = 0; o < kMlpH; o++) {
+    float z = prm[kOffB1 + o];
+#pragma unroll
+    for (int i = 0; i < kMlpIn; i++) z += prm[kOffW1 + o * kMlpIn + i] * x[i];
+    Z1[o * kRS + lane] = z;
+  }
+#pragma unroll
+  for (int i = 0; i < kMlpH; i++) a[i] = silu_acc(Z1[i * kRS + lane]);
+  for (int o = 0; o < kMlpH; o++) {
+    float z = prm[kOffB2 + o];
+#pragma unroll
+    for (int i = 0; i < kMlpH; i++) z += prm[kOffW2 + o * kMlpH + i] * a[i];
+    Z2[o * kRS + lane] = z;
+  }
+#pragma unroll
+  for (int i = 0; i < kMlpH; i++) a[i] = silu_acc(Z2[i * kRS + lane]);
+  for (int o = 0; o < kMlpH; o++) {
+    float z = prm[kOffB3 + o];
+#pragma unroll
+    for (int i = 0; i < kMlpH; i++) z += prm[kOffW3 + o * kMlpH + i] * a[i];
+    Z3[o * kRS + lane] = z;
+  }
+#pragma unroll
+  for (int i = 0; i < kMlpH; i++) a[i] = silu_acc(Z3[i * kRS + lane]);
+  float z4[kMlpOut];
+  for (int o = 0; o < kMlpOut; o++) {
+    float z = prm[kOffB4 + o];
+#pragma unroll
+    for (int i = 0; i < kMlpH; i++) z += prm[kOffW4 + o * kMlpH + i] * a[i];
+    z4[o] = z;
+  }
+  // loss: BCE(sigmoid(z2), hit) + hit * ((z0 - (t - m))^2 + (z1 - (t + m))^2)
+  float dz[kMlpOut] = {0.0f, 0.0f, 0.0f};
+  float loss = 0.0f;
+  if (valid) {
+    const float th = T.tgts[idx * 3 + 0], h = T.tgts[idx * 3 + 1] > 0.5f ? 1.0f : 0.0f, m = T.tgts[idx * 3 + 2];
+    const float z2 = z4[2];
+    loss = fmaxf(z2, 0.0f) - z2 * h + log1pf(expf(-fabsf(z2)));
+    dz[2] = 1.0f / (1.0f + expf(-z2)) - h;
+    const float e0 = z4[0] - (th - m), e1 = z4[1] - (th + m);
+    loss += h * (e0 * e0 + e1 * e1);
+    dz[0] = h * 2.0f * e0;
+    dz[1] = h * 2.0f * e1;
+  }
+  for (int o = 0; o < kMlpOut; o++) DZ[o * kRS + lane] = dz[o];
+  red[lane] = loss;
+  // layer 4 gradient: ACT = silu(Z3)
+#pragma unroll
+  for (int i = 0; i < kMlpH; i++) ACT[i * kRS + lane] = a[i];
+  __syncthreads();
+  float* gp = partial + (size_t)blockIdx.x * kMlpParams;
+  block_outer(DZ, ACT, kMlpOut, kMlpH, gp + kOffW4, gp + kOffB4, lane);
+  if (lane == 0) {
+    float acc = 0.0f;
+    for (int s = 0; s < 64; s++) acc += red[s];
+    partial_loss[blockIdx.x] = acc;
+  }
+  // d3 = (W4^T dz) * silu'(z3), into Z3 (own column); every register array
+  // below is indexed with compile-time indices, the cross-unit sums read the
+  // lane's own LDS column.
+  float dv[kMlpH];
+#pragma unroll
+  for (int i = 0; i < kMlpH; i++) {
+    float sacc = 0.0f;
+#pragma unroll
+    for (int o = 0; o < kMlpOut; o++) sacc += prm[kOffW4 + o * kMlpH + i] * dz[o];
+    dv[i] = sacc * dsilu_acc(Z3[i * kRS + lane]);
+  }
+  __syncthreads();  // layer-4 sums are done with ACT
+#pragma unroll
+  for (int i = 0; i < kMlpH; i++) {
+    Z3[i * kRS + lane] = dv[i];
+    ACT[i * kRS + lane] = silu_acc(Z2[i * kRS + lane]);
+  }
+  __syncthreads();
+  block_outer(Z3, ACT, kMlpH, kMlpH, gp + kOffW3, gp + kOffB3, lane);
+  // d2 = (W3^T d3) * silu'(z2)
+#pragma unroll
+  for (int i = 0; i < kMlpH; i++) {
+    float sacc = 0.0f;
+    for (int o = 0; o < kMlpH; o++) sacc += prm[kOffW3 + o * kMlpH + i] * Z3[o * kRS + lane];
+    dv[i] = sacc * dsilu_acc(Z2[i * kRS + lane]);
+  }
+  __syncthreads();  // layer-3 sums are done with ACT
+#pragma unroll
+  for (int i = 0; i < kMlpH; i++) {
+    Z2[i * kRS + lane] = dv[i];
+    ACT[i * kRS + lane] = silu_acc(Z1[i * kRS + lane]);
+  }
+  __syncthreads();
+  block_outer(Z2, ACT, kMlpH, kMlpH, gp + kOffW2, gp + kOffB2, lane);
+  // d1 = (W2^T d2) * silu'(z1)
+#pragma unroll
+  for (int i = 0; i < kMlpH; i++) {
+    float sacc = 0.0f;
+    for (int o = 0; o < kMlpH; o++) sacc += prm[kOffW2 + o * kMlpH + i] * Z2[o * kRS + lane];
+    dv[i] = sacc * dsilu_acc(Z1[i * kRS + lane]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kMlpH; i++) Z1[i * kRS + lane] = dv[i];
+  __syncthreads();
+  block_outer(Z1, A0, kMlpH, kMlpIn, gp + kOffW1, gp + kOffB1, lane);
+}
+
+__global__ __launch_bounds__(256) void k_mlp_reduce_adam(float* __restrict__ prm, float* __restrict__ m, float* __restrict__ v,
+                                                         const float* __restrict__ partial, const float* __restrict__ partial_loss,
+                                                         uint32_t parts, uint32_t batch, AdamArgs adam, int apply,
+                                                         float* __restrict__ grad_out, float* __restrict__ loss_out) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  const float inv_b = 1.0f / (float)batch;
+  if (p == 0 && loss_out) {
+    float acc = 0.0f;
+    for (uint32_t b = 0; b < parts; b++) acc += partial_loss[b];
+    *loss_out = acc * inv_b;
+  }
+  if (p >= (uint32_t)kMlpParams) return;
+  float g = 0.0f;
+  for (uint32_t b = 0; b < parts; b++) g += partial[(size_t)b * kMlpParams + p];
+  g *= inv_b;
+  if (grad_out) grad_out[p] = g;
+  if (apply) {
+    const float mm = adam.b1 * m[p] + (1.0f - adam.b1) * g;
+    const float vv = adam.b2 * v[p] + (1.0f - adam.b2) * g * g;
+    m[p] = mm;
+    v[p] = vv;
+    prm[p] -= adam.lr * (mm * adam.c1) / (sqrtf(vv * adam.c2) + adam.eps);
+  }
+}
+
+// ----------------------------------------------------------- seeded render --
+// Same pixels, camera rays, traversal order and hit test as k_render; at
+// every brick entry the wave stops, packs the features of its lanes that sit
+// at a leaf into LDS (compacted, so the MFMA work scales with those lanes),
+// runs the network, and each such lane hands the predicted hit distance to
+// the exact brick intersector when the hit logit clears the confidence gate.
+constexpr uint32_t kStackLdsSeeded = 16;
+
+template <bool COUNT, bool RGBA>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_render_seeded(DevScene S, TraceParams P, const float4* __restrict__ bfeat,
+                                                       const void* __restrict__ packed, float seed_logit, const Camera* __restrict__ cams,
+                                                       RenderArgs A, RenderOut O, uint32_t* work_counter, uint32_t* spill,
+                                                       unsigned long long* counts) {
+  __shared__ uint32_t lds_stack[kStackLdsSeeded * 256];
+  __shared__ __align__(16) uint16_t fl[4][64 * kFeatStride];
+  __shared__ float ol[4][64 * 3];
+  __shared__ uint32_t chunk_base;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  Stack st;
+  st.lds = lds_stack + threadIdx.x;
+  st.lds_stride = 256;
+  st.lds_entries = kStackLdsSeeded;
+  st.spill = spill + blockIdx.x * 256 + threadIdx.x;
+  st.spill_stride = gridDim.x * 256;
+  WorkCount wc;
+  const uint64_t total = (uint64_t)A.n_views * A.n_packed;
+  for (;;) {
+    if (threadIdx.x == 0) chunk_base = atomicAdd(work_counter, 1u);
+    __syncthreads();
+    const uint64_t base = (uint64_t)chunk_base * 256;
+    __syncthreads();
+    if (base >= total) break;  // uniform across the block
+    const uint64_t gtid = base + threadIdx.x;
+    uint32_t view = 0, x = 0, y = 0;
+    bool valid = gtid < total;
+    if (valid) {
+      view = (uint32_t)(gtid / A.n_packed);
+      const uint32_t tid = (uint32_t)(gtid - (uint64_t)view * A.n_packed);
+      tid_to_xy(tid, A.packed_w, x, y);
+      valid = x < A.W && y < A.H;
+    }
+    float4 acc = make_float4(0, 0, 0, 0);
+    for (uint32_t i = 0; i < A.spp; i++) {
+      v3 pos(0, 0, 0), dir(0, 0, 1);
+      if (valid) {
+        const float2 d = sample_offset(A, x, y, i);
+        eye_ray(cams[view], x, y, d.x, d.y, A.W, A.H, pos, dir);
+      }
+      const v3 inv = SafeInverse(dir);
+      Hit hit;
+      hit.t = 1e9f;
+      hit.primId = hit.instId = hit.geomId = 0xFFFFFFFFu;
+      hit.coords[0] = 1.0f; hit.coords[1] = 0.0f; hit.coords[2] = 0.0f; hit.coords[3] = 0.0f;
+      bool active = false;
+      if (valid) {
+        const float2 bh = RayBox(pos, inv, v3(-1, -1, -1), v3(1, 1, 1));
+        active = (bh.x <= bh.y) && (bh.y > 0.0f) && (bh.x < hit.t);
+      }
+      int top = 0;
+      uint32_t left = 0;
+      for (;;) {
+        bool at_leaf = false;
+        if (active) {
+          at_leaf = NextLeaf<COUNT>(S, pos, inv, 0.0f, false, hit.t, top, left, st, &wc);
+          active = at_leaf;
+        }
+        const uint64_t leaf_mask = __ballot(at_leaf);
+        if (leaf_mask == 0ull) break;  // wave-uniform
+        const int n_at = __popcll(leaf_mask);
+        const int slot = __popcll(leaf_mask & ((1ull << lane) - 1ull));
+        const uint32_t brick = left & LRT_START_MASK;
+        float t0 = 0.0f, s = 1.0f;
+        if (at_leaf) {
+          float f[24];
+          mlp_features(S, bfeat, pos, dir, inv, brick, f, &t0, &s);
+          store_features_bf16(fl[wave] + slot * kFeatStride, f);
+        }
+        wave_lds_sync();
+        mlp_wave_forward(packed, fl[wave], ol[wave], (n_at + 15) >> 4, lane);
+        if (COUNT && lane == 0) wc.tiles += (uint32_t)((n_at + 15) >> 4);
+        wave_lds_sync();
+        if (at_leaf) {
+          const float z0 = ol[wave][3 * slot + 0], z1 = ol[wave][3 * slot + 1], z2 = ol[wave][3 * slot + 2];
+          // confident hit: centre of the predicted interval; confident miss:
+          // beyond every voxel of the brick; otherwise no prediction
+          const bool confident = z2 >= seed_logit;
+          const float seed_world = confident ? t0 + 0.5f * (z0 + z1) / s : (z2 <= -seed_logit ? 3.0e38f : -1.0f);
+          if (COUNT) wc.bricks++, wc.seeds += confident ? 1u : 0u;
+          const float tPrev = hit.t;
+          BrickIntersect<true>(S, P, pos, dir, inv, 0.1f, brick, &hit, COUNT ? &wc.voxels : nullptr, seed_world,
+                               COUNT ? &wc.iters : nullptr);
+          if (hit.t < tPrev) {
+            active = false;
+          } else {
+            top--;
+            left = st.get(top > 0 ? top : 0);
+          }
+        }
+        wave_lds_sync();  // the next round rewrites this wave's LDS rows
+      }
+      if (hit.primId != 0xFFFFFFFFu) hit.instId = 0;
+      if (valid) {
+        const size_t pix = (size_t)view * A.W * A.H + (size_t)y * A.W + x;
+        if (i == 0) {
+          if (O.t) O.t[pix] = hit.t;
+          if (O.prim) O.prim[pix] = hit.primId;
+          if (O.n) O.n[pix] = make_float2(hit.coords[2], hit.coords[3]);
+        }
+        if (RGBA) {
+          const float4 c = shade(P.render_mode, hit, dir);
+          acc = make_float4(acc.x + c.x, acc.y + c.y, acc.z + c.z, acc.w + c.w);
+        }
+      }
+    }
+    if (RGBA && valid) {
+      const size_t pix = (size_t)view * A.W * A.H + (size_t)y * A.W + x;
+      const float sp = (float)A.spp;
+      O.rgba[pix] = make_float4(acc.x / sp, acc.y / sp, acc.z / sp, acc.w / sp);
+    }
+  }
+  if (COUNT) {
+    atomicAdd(&counts[0], (unsigned long long)wc.pairs);
+    atomicAdd(&counts[1], (unsigned long long)wc.bricks);
+    atomicAdd(&counts[2], (unsigned long long)wc.voxels);
+    atomicAdd(&counts[3], (unsigned long long)wc.iters);
+    atomicAdd(&counts[4], (unsigned long long)wc.tiles);
+    atomicAdd(&counts[5], (unsigned long long)wc.seeds);
+  }
+}
+
+}  // namespace lrtd
+
+namespace lrt {
+
+hipError_t launch_brick_features(const lrtd::DevScene& S, float4* d_bfeat, hipStream_t stream) {
+  hipLaunchKernelGGL(lrtd::k_brick_features, dim3((S.n_nodes + 255) / 256), dim3(256), 0, stream, S, d_bfeat);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_collect(const lrtd::DevScene& S, const lrtd::TraceParams& P, const float4* d_bfeat, const lrtd::Camera* d_cams,
+                              const lrtd::RenderArgs& A, float* d_feat, float* d_tgt, lrtd::MlpEventMeta* d_meta, uint64_t cap,
+                              unsigned long long* d_count, uint32_t* d_spill, uint32_t blocks, hipStream_t stream) {
+  // d_count[1] (as uint32) serves as the chunk counter
+  uint32_t* work_counter = (uint32_t*)(d_count + 1);
+  hipError_t e = zero_u32(work_counter, 1, stream);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(lrtd::k_mlp_collect, dim3(blocks), dim3(256), 0, stream, S, P, d_bfeat, d_cams, A, d_feat, d_tgt, d_meta, cap,
+                     d_count, work_counter, d_spill);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_infer(const void* d_packed, const float* d_feat, uint64_t n, float* d_out, uint32_t blocks, hipStream_t stream) {
+  hipLaunchKernelGGL(lrtd::k_mlp_infer, dim3(blocks), dim3(256), 0, stream, d_packed, d_feat, n, d_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_grad(const float* d_params, const lrtd::MlpTrainArgs& T, float* d_partial, float* d_partial_loss, uint32_t blocks,
+                           hipStream_t stream) {
+  hipLaunchKernelGGL(lrtd::k_mlp_grad, dim3(blocks), dim3(64), 0, stream, d_params, T, d_partial, d_partial_loss);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_reduce_adam(float* d_params, float* d_m, float* d_v, const float* d_partial, const float* d_partial_loss,
+                                  uint32_t parts, uint32_t batch, const lrtd::AdamArgs& adam, bool apply, float* d_grad_out,
+                                  float* d_loss_out, hipStream_t stream) {
+  hipLaunchKernelGGL(lrtd::k_mlp_reduce_adam, dim3((lrtd::kMlpParams + 255) / 256), dim3(256), 0, stream, d_params, d_m, d_v, d_partial,
+                     d_partial_loss, parts, batch, adam, apply ? 1 : 0, d_grad_out, d_loss_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_render_seeded(const lrtd::DevScene& S, const lrtd::TraceParams& P, const float4* d_bfeat, const void* d_packed,
+                                float seed_logit, const lrtd::Camera* d_cams, const lrtd::RenderArgs& A, const lrtd::RenderOut& O,
+                                uint32_t* d_work_counter, uint32_t* d_spill, uint32_t blocks, bool rgba, unsigned long long* d_counts,
+                                hipStream_t stream) {
+  hipError_t e = zero_u32(d_work_counter, 1, stream);
+  if (e != hipSuccess) return e;
+  if (d_counts) {
+    if (rgba) hipLaunchKernelGGL((lrtd::k_render_seeded<true, true>), dim3(blocks), dim3(256), 0, stream, S, P, d_bfeat, d_packed, seed_logit, d_cams, A, O, d_work_counter, d_spill, d_counts);
+    else hipLaunchKernelGGL((lrtd::k_render_seeded<true, false>), dim3(blocks), dim3(256), 0, stream, S, P, d_bfeat, d_packed, seed_logit, d_cams, A, O, d_work_counter, d_spill, d_counts);
+  } else {
+    if (rgba) hipLaunchKernelGGL((lrtd::k_render_seeded<false, true>), dim3(blocks), dim3(256), 0, stream, S, P, d_bfeat, d_packed, seed_logit, d_cams, A, O, d_work_counter, d_spill, d_counts);
+    else hipLaunchKernelGGL((lrtd::k_render_seeded<false, false>), dim3(blocks), dim3(256), 0, stream, S, P, d_bfeat, d_packed, seed_logit, d_cams, A, O, d_work_counter, d_spill, d_counts);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace lrt

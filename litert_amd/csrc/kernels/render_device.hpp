@@ -1,0 +1,66 @@
+// render_device.hpp - per-pixel helpers shared by the render kernels:
+// shading of a hit for RenderFloat and the sub-pixel sample offsets.
+#pragma once
+#include "render.hpp"
+
+namespace lrtd {
+
+// Render modes that apply to SBS geometry (eye_ray.cpp:162-489) with the
+// renderer's default lights: direct normalize(1,1,1) colour 2/3 and ambient
+// 0.25 (eye_ray_host.cpp:40).
+__device__ __forceinline__ float4 shade(uint32_t mode, const Hit& hit, v3 dir) {
+  if (hit.primId == 0xFFFFFFFFu) return make_float4(0, 0, 0, 1);
+  const float z = hit.t, z_near = 0.1f, z_far = 10.0f;
+  const uint32_t type = hit.geomId >> LRT_SH_TYPE;
+  const uint32_t geomId = hit.geomId & LRT_GEOM_ID_MASK;
+  v3 norm = decode_normal(hit.coords[2], hit.coords[3]);
+  const float ns = sgn(dot(-1.0f * dir, norm));
+  norm = norm * ns;
+  const uint32_t pal[20] = {0xffe6194b, 0xff3cb44b, 0xffffe119, 0xff0082c8, 0xfff58231, 0xff911eb4, 0xff46f0f0,
+                            0xfff032e6, 0xffd2f53c, 0xfffabebe, 0xff008080, 0xffe6beff, 0xffaa6e28, 0xfffffac8,
+                            0xff800000, 0xffaaffc3, 0xff808000, 0xffffd8b1, 0xff000080, 0xff808080};
+  auto palc = [](uint32_t c) {
+    return make_float4((float)(c & 0xFF) * (1.0f / 255.0f), (float)((c >> 8) & 0xFF) * (1.0f / 255.0f),
+                       (float)((c >> 16) & 0xFF) * (1.0f / 255.0f), (float)((c >> 24) & 0xFF) * (1.0f / 255.0f));
+  };
+  switch (mode) {
+    case LRT_RENDER_MODE_MASK: return make_float4(1, 1, 1, 1);
+    case LRT_RENDER_MODE_DEPTH: { float d = (1 / z - 1 / z_near) / (1 / z_far - 1 / z_near); return make_float4(d, d, d, 1); }
+    case LRT_RENDER_MODE_LINEAR_DEPTH: { float d = ((z - z_near) / (z_far - z_near)); return make_float4(d, d, d, 1); }
+    case LRT_RENDER_MODE_INVERSE_LINEAR_DEPTH: { float d = 1 - ((z - z_near) / (z_far - z_near)); return make_float4(d, d, d, 1); }
+    case LRT_RENDER_MODE_PRIMITIVE: return palc(pal[hit.primId % 20]);
+    case LRT_RENDER_MODE_TYPE: return palc(pal[type % 20]);
+    case LRT_RENDER_MODE_GEOM: return palc(pal[geomId % 20]);
+    case LRT_RENDER_MODE_NORMAL: return make_float4(fabsf(norm.x), fabsf(norm.y), fabsf(norm.z), 1);
+    case LRT_RENDER_MODE_ST_ITERATIONS: {
+      if (hit.primId == 0) return make_float4(0.5f, 0.5f, 0.5f, 1);
+      if (hit.primId <= 64) {
+        float q = clampf_((float)hit.primId / 64, 0.0f, 1.0f);
+        return make_float4(0, (1 - q), q, q + (1 - q));
+      }
+      float q = clampf_((float)(hit.primId - 64) / (256 - 64), 0.0f, 1.0f);
+      return make_float4(q, 0, (1 - q), q + (1 - q));
+    }
+    case LRT_RENDER_MODE_DIFFUSE: {
+      if (type == LRT_TYPE_SDF_SBS_COL)
+        return make_float4(roundf(hit.coords[0]) / 255.0f, hit.coords[0] - floorf(hit.coords[0]), roundf(hit.coords[1]) / 255.0f, 1);
+      return make_float4(0, 1, 1, 1);
+    }
+    case LRT_RENDER_MODE_LAMBERT_NO_TEX: {
+      v3 ld = normalize(v3(1, 1, 1));
+      float q = mx(0.0f, dot(norm, ld));
+      float c = (2.0f / 3.0f) * q + 0.25f;
+      return make_float4(c, c, c, 1);
+    }
+    default: return make_float4(1, 0, 1, 1);
+  }
+}
+
+__device__ __forceinline__ float2 sample_offset(const RenderArgs& A, uint32_t x, uint32_t y, uint32_t i) {
+  if (A.ray_gen_mode == LRT_RAY_GEN_MODE_RANDOM) return rand2(x, y, i + A.seed % 2, A.seed);
+  const uint32_t spp_sqrt = (uint32_t)sqrt((double)A.spp);
+  const float is = 1.0f / (float)spp_sqrt;
+  return make_float2(is * (float)((double)(i / spp_sqrt) + 0.5), is * (float)((double)(i % spp_sqrt) + 0.5));
+}
+
+}  // namespace lrtd

@@ -154,12 +154,71 @@ struct TraceParams {
   uint32_t need_normal;
 };
 
+// Cubic along the ray inside one voxel (JCGT 2022 coefficients, the same
+// expressions as the Newton/analytic branches below, BVH2Common.cpp:322-361).
+// Note the sign: -trilinear(start_q + t*dir) * d_inv = c0 + c1 t + c2 t^2 + c3 t^3.
+__device__ __forceinline__ void cubic_coeffs(const float values[8], float d_inv, v3 o, v3 d3, float& c0, float& c1, float& c2,
+                                             float& c3) {
+  float s000 = values[0] * d_inv, s001 = values[1] * d_inv, s010 = values[2] * d_inv, s011 = values[3] * d_inv;
+  float s100 = values[4] * d_inv, s101 = values[5] * d_inv, s110 = values[6] * d_inv, s111 = values[7] * d_inv;
+  float a = s101 - s001;
+  float k0 = s000;
+  float k1 = s100 - s000;
+  float k2 = s010 - s000;
+  float k3 = s110 - s010 - k1;
+  float k4 = k0 - s001;
+  float k5 = k1 - a;
+  float k6 = k2 - (s011 - s001);
+  float k7 = k3 - (s111 - s011 - a);
+  float m0 = o.x * o.y;
+  float m1 = d3.x * d3.y;
+  float m2 = o.x * d3.y + o.y * d3.x;
+  float m3 = k5 * o.z - k1;
+  float m4 = k6 * o.z - k2;
+  float m5 = k7 * o.z - k3;
+  c0 = (k4 * o.z - k0) + o.x * m3 + o.y * m4 + m0 * m5;
+  c1 = d3.x * m3 + d3.y * m4 + m2 * m5 + d3.z * (k4 + k5 * o.x + k6 * o.y + k7 * m0);
+  c2 = m1 * m5 + d3.z * (k5 * d3.x + k6 * d3.y + k7 * m2);
+  c3 = k7 * m1 * d3.z;
+}
+
+// g(t) = c0 + c1 t + c2 t^2 + c3 t^3 > margin on all of [0, ts]? Checks both
+// ends and every critical point inside (the extrema of g on the interval).
+__device__ __forceinline__ bool cubic_above_on(float c0, float c1, float c2, float c3, float ts, float margin) {
+  auto g = [&](float t) { return c0 + t * (c1 + t * (c2 + t * c3)); };
+  if (!(g(0.0f) > margin) || !(g(ts) > margin)) return false;
+  const float qa = 3 * c3, qb = 2 * c2, qc = c1;
+  if (fabsf(qa) < 1e-20f) {
+    if (fabsf(qb) < 1e-20f) return true;
+    const float tc = -qc / qb;
+    return !(tc > 0.0f && tc < ts) || g(tc) > margin;
+  }
+  const float D = qb * qb - 4 * qa * qc;
+  if (D < 0.0f) return true;
+  const float sq = sqrtf(D);
+  const float q = -0.5f * (qb + (qb >= 0.0f ? sq : -sq));
+  const float r1 = q / qa;
+  const float r2 = q != 0.0f ? qc / q : r1;
+  if (r1 > 0.0f && r1 < ts && !(g(r1) > margin)) return false;
+  if (r2 > 0.0f && r2 < ts && !(g(r2) > margin)) return false;
+  return true;
+}
+
 // ---- BVH2Common.cpp:274-631 ------------------------------------------------
 // Returns true when *pHit was updated. d_inv = 1/d is passed in (same value
 // the reference computes per call).
+// SEEDED: seed_q (voxel units from the voxel entry) is a predicted start for
+// the iterative root finders; seed_q >= qFar predicts no surface in this
+// voxel, which lets sphere tracing skip a voxel it provably cannot stop in. Newton starts there instead of the bracket
+// midpoint when it lies strictly inside the exact bracket; sphere tracing
+// starts there (or, failing that, half a voxel before it) only after the
+// cubic along the ray is shown to stay above the stopping threshold up to the
+// start, so the root it converges to is the first one either way. Any other mode, or a seed that fails the checks,
+// runs unseeded. n_iter (optional) accumulates root-finder iterations.
+template <bool SEEDED = false>
 __device__ __forceinline__ bool LocalSurfaceIntersection(const TraceParams& P, v3 ray_dir, const float values[8], uint32_t primId,
                                                          uint32_t geomType, float d, float d_inv, float qFar, float fNearX,
-                                                         v3 start_q, Hit* pHit) {
+                                                         v3 start_q, Hit* pHit, float seed_q = -1.0f, uint32_t* n_iter = nullptr) {
   const float EPS = 1e-6f;
   const unsigned mode = P.sdf_node_intersect;
   float t = 0.0f;
@@ -174,12 +233,65 @@ __device__ __forceinline__ bool LocalSurfaceIntersection(const TraceParams& P, v
     hit = true;
   } else if (mode == LRT_SDF_NODE_INTERSECT_ST) {
     float dist = start_dist;
-    while (t < qFar && dist > EPS && iter < 256u) {
+    bool seeded_start = false;
+    bool skip = false;
+    float c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+    if (SEEDED && seed_q >= qFar && start_sign == 1.0f) {
+      // predicted: no surface in this voxel. If the distance provably stays
+      // above twice the stopping threshold along the whole segment, the
+      // trace could not stop inside it, so the voxel is skipped untraced.
+      cubic_coeffs(values, d_inv, start_q, ray_dir, c0, c1, c2, c3);
+      skip = cubic_above_on(-c0, -c1, -c2, -c3, qFar, 2.0f * EPS * d_inv);
+    } else if (SEEDED && seed_q > 0.0f && seed_q < qFar && start_sign == 1.0f) {
+      // the predicted hit first, then half a voxel before it
+      cubic_coeffs(values, d_inv, start_q, ray_dir, c0, c1, c2, c3);
+      c0 = -c0; c1 = -c1; c2 = -c2; c3 = -c3;  // now the distance itself (voxel units)
+      float ts = seed_q;
+      bool ok = cubic_above_on(c0, c1, c2, c3, ts, EPS * d_inv);
+      if (!ok && seed_q > 0.5f) {
+        ts = seed_q - 0.5f;
+        ok = cubic_above_on(c0, c1, c2, c3, ts, EPS * d_inv);
+      }
+      if (ok) {
+        t = ts;
+        dist = trilinear(values, start_q + t * ray_dir);
+        seeded_start = true;
+      }
+    }
+    while (!skip && t < qFar && dist > EPS && iter < 256u) {
       t += dist * d_inv;
       dist = start_sign * trilinear(values, start_q + t * ray_dir);
       iter++;
     }
-    hit = (dist <= EPS);
+    // Sphere tracing stops anywhere in the band where the distance is below
+    // EPS (EPS / |slope| long along the ray), and where the distance changes
+    // faster than 1 per unit length a step can overshoot the root: in both
+    // cases a different start point lands elsewhere. A seeded result is kept
+    // only where neither can move it: |slope| <= 1 on the whole segment the
+    // unseeded trace would cover (no overshoot from any start) and
+    // slope <= -0.25 at the stop (band far inside the depth tolerance);
+    // otherwise the trace is redone from the voxel entry, as unseeded.
+    if (SEEDED && seeded_start) {
+      const float slope = c1 + t * (2 * c2 + t * 3 * c3);
+      auto dg = [&](float u) { return fabsf(c1 + u * (2 * c2 + u * 3 * c3)); };
+      float smax = fmaxf(dg(0.0f), dg(t));
+      if (fabsf(c3) > 1e-20f) {
+        const float tv = -c2 / (3 * c3);
+        if (tv > 0.0f && tv < t) smax = fmaxf(smax, dg(tv));
+      }
+      if (!(dist <= EPS && dist >= -EPS && slope <= -0.25f && smax <= 1.0f)) {
+        t = 0.0f;
+        dist = start_dist;
+        unsigned it2 = 0;
+        while (t < qFar && dist > EPS && it2 < 256u) {
+          t += dist * d_inv;
+          dist = start_sign * trilinear(values, start_q + t * ray_dir);
+          it2++;
+        }
+        iter += it2;
+      }
+    }
+    hit = !skip && (dist <= EPS);
   } else {
     float s000 = values[0] * d_inv, s001 = values[1] * d_inv, s010 = values[2] * d_inv, s011 = values[3] * d_inv;
     float s100 = values[4] * d_inv, s101 = values[5] * d_inv, s110 = values[6] * d_inv, s111 = values[7] * d_inv;
@@ -228,7 +340,7 @@ __device__ __forceinline__ bool LocalSurfaceIntersection(const TraceParams& P, v
       else if (s2 != s3) { nwt_min = t2; nwt_max = t3; }
       if (nwt_min < nwt_max) {
         unsigned it = 0;
-        float rtn = 0.5f * (nwt_min + nwt_max);
+        float rtn = (SEEDED && seed_q > nwt_min && seed_q < nwt_max) ? seed_q : 0.5f * (nwt_min + nwt_max);
         float f = 1000;
         while (it < 10u && fabsf(f) >= EPS) {
           f = c0 + rtn * (c1 + rtn * (c2 + rtn * c3));
@@ -237,6 +349,7 @@ __device__ __forceinline__ bool LocalSurfaceIntersection(const TraceParams& P, v
           rtn -= dx;
           it++;
         }
+        if (n_iter) *n_iter += it;  // the reference keeps Newton's count local (iter stays 0)
         t = rtn;
         hit = (t >= 0 && t <= qFar && fabsf(f) < EPS);
       } else {
@@ -295,6 +408,7 @@ __device__ __forceinline__ bool LocalSurfaceIntersection(const TraceParams& P, v
     }
   }
 
+  if (n_iter) *n_iter += iter;
   float tReal = fNearX + d * t;
   if (t <= qFar && hit && tReal < pHit->t) {
     v3 norm(0, 0, 1);
@@ -326,8 +440,12 @@ __device__ __forceinline__ float step01(float e0, float e1, float x) { return cl
 // rayDirInv is SafeInverse(ray_dir), evaluated once per ray by the caller
 // (the reference recomputes the identical value for every box test).
 // Counters (optional) receive voxel and root-find counts.
+// SEEDED: seed_world is a predicted hit distance along the ray (world t);
+// each candidate voxel hands it to the root finder in its own units.
+template <bool SEEDED = false>
 __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TraceParams& P, v3 ray_pos, v3 ray_dir, v3 rayDirInv,
-                                               float tNear, uint32_t brick, Hit* pHit, uint32_t* n_vox) {
+                                               float tNear, uint32_t brick, Hit* pHit, uint32_t* n_vox, float seed_world = -1.0f,
+                                               uint32_t* n_iter = nullptr) {
   float values[8];
   const uint4 N = S.nodes[brick];
   const float px = (float)(N.x >> 16), py = (float)(N.x & 0xFFFFu), pz = (float)(N.y >> 16), sz = (float)(N.y & 0xFFFFu);
@@ -358,7 +476,8 @@ __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TracePar
       v3 start_pos = ray_pos + fnf.x * ray_dir;
       v3 start_q = (start_pos - min_pos) * qscale;
       float qFar = (fnf.y - fnf.x) * qscale;
-      bool updated = LocalSurfaceIntersection(P, ray_dir, values, brick, S.type, d, d_inv, qFar, fnf.x, start_q, pHit);
+      const float seed_q = SEEDED ? (seed_world - fnf.x) * d_inv : -1.0f;
+      bool updated = LocalSurfaceIntersection<SEEDED>(P, ray_dir, values, brick, S.type, d, d_inv, qFar, fnf.x, start_q, pHit, seed_q, n_iter);
       if (updated && S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F_IN && P.normal_mode == LRT_NORMAL_MODE_SDF_SMOOTHED && P.need_normal) {
         // smoothed normal, BVH2Common.cpp:838-906
         const float beta = 0.5f;

@@ -27,6 +27,9 @@ struct Stack {
 
 struct WorkCount {
   uint32_t pairs = 0, bricks = 0, voxels = 0;
+  uint32_t iters = 0;   // root-finder iterations
+  uint32_t tiles = 0;   // 16-sample MLP tiles evaluated (seeded kernel)
+  uint32_t seeds = 0;   // brick entries that passed the confidence gate
 };
 
 // BVH2Common.cpp:3407-3488. Returns through *hit; stack capacity is
@@ -65,7 +68,8 @@ __device__ __forceinline__ void BVH2Traverse(const DevScene& S, const TraceParam
       const float tNearSdf = mx(tNear, 0.1f);  // SDF_BIAS, BVH2Common.cpp:3472-3473
       const float tPrev = hit->t;
       if (COUNT) wc->bricks++;
-      BrickIntersect(S, P, ray_pos, ray_dir, inv, tNearSdf, left & LRT_START_MASK, hit, COUNT ? &wc->voxels : nullptr);
+      BrickIntersect(S, P, ray_pos, ray_dir, inv, tNearSdf, left & LRT_START_MASK, hit, COUNT ? &wc->voxels : nullptr, -1.0f,
+                     COUNT ? &wc->iters : nullptr);
       hitFound = hit->t < tPrev;
     }
     top--;
@@ -88,6 +92,45 @@ __device__ __forceinline__ Hit RayQuery_NearestHit(const DevScene& S, const Trac
   if ((bh.x <= bh.y) && (bh.y > tNear) && (bh.x < hit.t)) BVH2Traverse<COUNT>(S, P, pos, dir, inv, tNear, stopOnFirstHit, &hit, st, wc);
   if (hit.primId != 0xFFFFFFFFu) hit.instId = 0;
   return hit;
+}
+
+// Resumable form of the same traversal, for kernels that do wave-wide work
+// at every leaf (the learned seeder, the training-event collector). Advances
+// (top, left) to the next brick leaf in exactly the order BVH2Traverse visits
+// them and returns true, or returns false when the stack is exhausted. After
+// a leaf that did not end the ray the caller pops: top--; left = get(top).
+template <bool COUNT>
+__device__ __forceinline__ bool NextLeaf(const DevScene& S, v3 ray_pos, v3 inv, float tNear, bool stopOnFirstHit, float hit_t, int& top,
+                                         uint32_t& left, Stack& st, WorkCount* wc) {
+  for (;;) {
+    while (top >= 0 && ((left & LRT_LEAF_BIT) == 0)) {
+      const uint4* pp = S.pairs + 4 * (size_t)left;
+      const uint4 a = pp[0], b = pp[1], c = pp[2], e = pp[3];
+      if (COUNT) wc->pairs++;
+      const float2 tm0 = RayBox(ray_pos, inv, v3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)),
+                                v3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z)));
+      const float2 tm1 = RayBox(ray_pos, inv, v3(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z)),
+                                v3(__uint_as_float(e.x), __uint_as_float(e.y), __uint_as_float(e.z)));
+      const uint32_t n0 = a.w, n1 = c.w;
+      const bool h0 = (tm0.x <= tm0.y) && (tm0.y >= tNear) && (tm0.x <= hit_t || !stopOnFirstHit);
+      const bool h1 = (tm1.x <= tm1.y) && (tm1.y >= tNear) && (tm1.x <= hit_t || !stopOnFirstHit);
+      left = h0 ? n0 : n1;
+      if (h0 && h1) {
+        const bool near0 = tm0.x <= tm1.x;
+        left = near0 ? n0 : n1;
+        st.put(top, near0 ? n1 : n0);
+        top++;
+      }
+      if (!h0 && !h1) {
+        top--;
+        left = st.get(top > 0 ? top : 0);
+      }
+    }
+    if (top < 0) return false;
+    if (left != 0xFFFFFFFFu) return true;
+    top--;
+    left = st.get(top > 0 ? top : 0);
+  }
 }
 
 }  // namespace lrtd

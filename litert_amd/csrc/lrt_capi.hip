@@ -1,6 +1,7 @@
 // lrt_capi.hip - C ABI implementation: scene ingest, device buffers, launches.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -111,8 +112,8 @@ uint32_t Renderer::grid_blocks() const {
   return (uint32_t)std::max(1, cu_count) * 8u;
 }
 
-hipError_t Renderer::ensure_spill(uint32_t blocks) {
-  uint32_t extra = (uint32_t)std::max(0, pair_depth + 2 - (int)lrtd::kStackLds);
+hipError_t Renderer::ensure_spill(uint32_t blocks, uint32_t lds_entries) {
+  uint32_t extra = (uint32_t)std::max(0, pair_depth + 2 - (int)lds_entries);
   size_t n = (size_t)blocks * 256 * std::max<uint32_t>(extra, 1u);
   return d_spill.ensure(n * sizeof(uint32_t));
 }
@@ -158,10 +159,19 @@ int Renderer::render_device(const lrtd::Camera* cams, uint32_t n_views, uint32_t
   A.ray_gen_mode = preset.ray_gen_mode;
   A.seed = seed;
   const uint32_t blocks = grid_blocks();
-  if ((e = ensure_spill(blocks)) != hipSuccess) return hip_err(e, "alloc spill");
+  const bool seeded = seeding && mlp_ready;
+  if ((e = ensure_spill(blocks, seeded ? 16u : lrtd::kStackLds)) != hipSuccess) return hip_err(e, "alloc spill");
+  if (seeded && (e = ensure_brick_features()) != hipSuccess) return hip_err(e, "brick features");
   if ((e = hipEventRecord(ev0, stream)) != hipSuccess) return hip_err(e, "event");
-  e = launch_render(dev_scene(), trace_params(force_normal), (const lrtd::Camera*)d_cams.p, A, O, (uint32_t*)d_counter.p,
-                    (uint32_t*)d_spill.p, blocks, rgba, d_counts, stream);
+  if (seeded) {
+    const float logit = std::log(seed_threshold / (1.0f - seed_threshold));
+    e = launch_render_seeded(dev_scene(), trace_params(force_normal), (const float4*)d_bfeat.p, d_mlp_packed.p, logit,
+                             (const lrtd::Camera*)d_cams.p, A, O, (uint32_t*)d_counter.p, (uint32_t*)d_spill.p, blocks, rgba, d_counts,
+                             stream);
+  } else {
+    e = launch_render(dev_scene(), trace_params(force_normal), (const lrtd::Camera*)d_cams.p, A, O, (uint32_t*)d_counter.p,
+                      (uint32_t*)d_spill.p, blocks, rgba, d_counts, stream);
+  }
   if (e != hipSuccess) return hip_err(e, "render launch");
   if ((e = hipEventRecord(ev1, stream)) != hipSuccess) return hip_err(e, "event");
   last_w = W;
@@ -287,6 +297,7 @@ int lrt_set_scene_sbs(lrt_handle h, const SdfSBSHeader* header, const SdfSBSNode
   if ((e = r->d_values_f.upload(r->values_f.data(), r->values_f.size() * 4, r->stream)) != hipSuccess) return hip_err(e, "upload values_f");
   if ((e = hipStreamSynchronize(r->stream)) != hipSuccess) return hip_err(e, "scene upload");
   r->has_scene = true;
+  r->bfeat_valid = false;
   return LRT_OK;
 }
 
@@ -438,10 +449,11 @@ int lrt_render_views_device(lrt_handle h, const float* wvs, const float* prs, ui
   return LRT_OK;
 }
 
-int lrt_count_work(lrt_handle h, const float* wvs, const float* prs, uint32_t n_views, uint32_t W, uint32_t H, uint64_t out[3]) {
+int lrt_count_work_ex(lrt_handle h, const float* wvs, const float* prs, uint32_t n_views, uint32_t W, uint32_t H, uint64_t* out,
+                      uint32_t n_out) {
   int c = check_ready(h);
   if (c) return c;
-  if (!wvs || !prs || !out || n_views == 0) return set_err(LRT_E_INVALID, "bad arguments");
+  if (!wvs || !prs || !out || n_views == 0 || n_out == 0 || n_out > 6) return set_err(LRT_E_INVALID, "bad arguments");
   std::vector<lrtd::Camera> cams(n_views);
   for (uint32_t v = 0; v < n_views; v++) cams[v] = make_camera(wvs + 16 * v, prs + 16 * v);
   hipError_t e;
@@ -449,10 +461,14 @@ int lrt_count_work(lrt_handle h, const float* wvs, const float* prs, uint32_t n_
   if ((e = zero_u32((uint32_t*)h->d_counts.p, 16, h->stream)) != hipSuccess) return hip_err(e, "clear counts");
   if ((c = h->render_device(cams.data(), n_views, W, H, false, true, true, "CountWork", (unsigned long long*)h->d_counts.p))) return c;
   if ((c = h->finish_timer())) return c;
-  unsigned long long tmp[4];
+  unsigned long long tmp[8];
   if ((e = hipMemcpy(tmp, h->d_counts.p, sizeof(tmp), hipMemcpyDeviceToHost)) != hipSuccess) return hip_err(e, "download counts");
-  for (int i = 0; i < 3; i++) out[i] = tmp[i];
+  for (uint32_t i = 0; i < n_out; i++) out[i] = tmp[i];
   return LRT_OK;
+}
+
+int lrt_count_work(lrt_handle h, const float* wvs, const float* prs, uint32_t n_views, uint32_t W, uint32_t H, uint64_t out[3]) {
+  return lrt_count_work_ex(h, wvs, prs, n_views, W, H, out, 3);
 }
 
 int lrt_download_view(lrt_handle h, uint32_t view, float* t, float* n, uint32_t* prim, float* rgba) {

@@ -206,6 +206,7 @@ int lrt_dr_optimize(lrt_handle h, const MultiRendererDRPreset* p, const SdfSBSHe
     return hip_err(e, "download values_f");
   if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_err(e, "optimize");
   h->values_f.assign(values_f, values_f + P);
+  h->bfeat_valid = false;  // brick features follow the optimised distances
   if (total_ms) *total_ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return LRT_OK;
 }

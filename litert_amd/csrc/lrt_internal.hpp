@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 #include "../../include/lrt.h"
+#include "kernels/mlp.hpp"
 #include "kernels/render.hpp"
 
 namespace lrt {
@@ -57,6 +58,18 @@ struct Renderer {
   DevBuf d_dr_ref, d_dr_img, d_dr_thr, d_dr_grad, d_dr_list, d_dr_misc, d_adam_v, d_adam_s;
   float dr_last_loss = 0.f;
   uint32_t dr_last_border = 0, dr_last_border_hits = 0;
+  // learned root-find seeder
+  bool mlp_ready = false;            // parameters set and packed
+  std::vector<float> mlp_params;     // FP32 master copy
+  DevBuf d_mlp_params, d_mlp_m, d_mlp_v, d_mlp_packed, d_mlp_partial, d_mlp_ploss, d_mlp_grad, d_mlp_loss;
+  uint64_t mlp_adam_t = 0;
+  DevBuf d_bfeat;                    // per-brick features of the current scene
+  bool bfeat_valid = false;
+  DevBuf d_ds_feat, d_ds_tgt, d_ds_meta, d_ds_out;
+  uint64_t ds_n = 0, ds_cap = 0;
+  bool seeding = false;
+  float seed_threshold = 0.9f;
+
   float last_ms = 0.f;
   std::string pending_timer;
   std::map<std::string, float> times;
@@ -64,14 +77,18 @@ struct Renderer {
   lrtd::DevScene dev_scene() const;
   lrtd::TraceParams trace_params(bool force_normal) const;
   uint32_t grid_blocks() const;
-  hipError_t ensure_spill(uint32_t blocks);
+  hipError_t ensure_spill(uint32_t blocks, uint32_t lds_entries = lrtd::kStackLds);
+  hipError_t ensure_brick_features();
+  int pack_mlp();
   int render_device(const lrtd::Camera* cams, uint32_t n_views, uint32_t W, uint32_t H, bool rgba, bool gbuf, bool force_normal,
                     const char* timer_name, unsigned long long* d_counts);
   int finish_timer();
   void release_all() {
     DevBuf* all[] = {&d_pairs, &d_nodes, &d_values, &d_values_f, &d_cams, &d_counter, &d_spill, &d_counts,
                      &d_rays, &d_hits, &d_out_t, &d_out_n, &d_out_prim, &d_out_rgba, &d_dr_ref, &d_dr_img,
-                     &d_dr_thr, &d_dr_grad, &d_dr_list, &d_dr_misc, &d_adam_v, &d_adam_s};
+                     &d_dr_thr, &d_dr_grad, &d_dr_list, &d_dr_misc, &d_adam_v, &d_adam_s, &d_mlp_params,
+                     &d_mlp_m, &d_mlp_v, &d_mlp_packed, &d_mlp_partial, &d_mlp_ploss, &d_mlp_grad, &d_mlp_loss,
+                     &d_bfeat, &d_ds_feat, &d_ds_tgt, &d_ds_meta, &d_ds_out};
     for (DevBuf* b : all) b->release();
   }
 };

@@ -70,6 +70,8 @@ def lib():
             "orc_adam_step": (None, [u32, P, P, P, P, u32, f32, f32, f32, f32]),
             "orc_mlp_forward": (None, [P, P, u32, i32, P]),
             "orc_mlp_loss_grad": (f64, [P, P, P, u32, i32, P]),
+            "orc_mlp_brick_features": (i32, [P, u32, P]),
+            "orc_mlp_features": (i32, [P, P, u32, P, P]),
         }
         for name, (res, args) in list(sig.items()) + list(optional.items()):
             if name in optional and not hasattr(L, name):
@@ -162,6 +164,18 @@ class Scene:
         if lib().orc_scene_set_values_f(self.h, _p(vf), vf.size) != 0:
             raise ValueError("size mismatch")
 
+    def mlp_features(self, rays, bricks):
+        """Seeder features for rays (n x 6: pos, dir) entering bricks (n): (n x 24 float32, t0 float32)."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        out = np.zeros((rays.shape[0], 24), np.float32)
+        t0 = np.zeros(rays.shape[0], np.float32)
+        for i in range(rays.shape[0]):
+            t = C.c_float()
+            if lib().orc_mlp_features(self.h, _p(rays[i]), int(bricks[i]), _p(out[i]), C.byref(t)) != 0:
+                raise ValueError("orc_mlp_features failed")
+            t0[i] = t.value
+        return out, t0
+
     def dr_render_grad(self, dr_preset, world_view, proj, W, H, ref, n_params, seed=0, precision=32, threads=0,
                        dtdv_mode=0):
         img = np.zeros((H, W, 4), np.float64)
@@ -182,3 +196,22 @@ def eye_ray(world_view, proj, x, y, dx, dy, W, H, precision=32):
     lib().orc_eye_ray(_p(np.ascontiguousarray(world_view, np.float32)), _p(np.ascontiguousarray(proj, np.float32)),
                       x, y, dx, dy, W, H, precision, _p(out))
     return out
+
+
+def mlp_forward(params, x, bf16=False):
+    """FP64 forward of the seeder network (optionally emulating BF16 operands): (n x 3)."""
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, 24)
+    y = np.zeros((x.shape[0], 3), np.float64)
+    lib().orc_mlp_forward(_p(params), _p(x), x.shape[0], int(bf16), _p(y))
+    return y
+
+
+def mlp_loss_grad(params, x, tgt, bf16=False):
+    """FP64 mean loss and gradient of the seeder network over n samples."""
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, 24)
+    tgt = np.ascontiguousarray(tgt, np.float32).reshape(-1, 3)
+    g = np.zeros(params.size, np.float64)
+    loss = lib().orc_mlp_loss_grad(_p(params), _p(x), _p(tgt), x.shape[0], int(bf16), _p(g))
+    return loss, g

@@ -99,6 +99,11 @@ void orc_mlp_forward(const float* params, const float* x, uint32_t n, int bf16_e
 double orc_mlp_loss_grad(const float* params, const float* x, const float* target, uint32_t n, int bf16_emulate,
                          double* grad);
 
+/* Seeder feature table of one brick (16 floats) and the 24 features of a
+ * ray (pos xyz, dir xyz) entering it; *t0 = brick entry distance. */
+int orc_mlp_brick_features(const orc_scene* s, uint32_t brick, float out[16]);
+int orc_mlp_features(const orc_scene* s, const float ray[6], uint32_t brick, float out[24], float* t0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <vector>
 #include "lrt_oracle.h"
+#include "oracle_scene.hpp"
 
 namespace {
 
@@ -127,4 +128,68 @@ extern "C" double orc_mlp_loss_grad(const float* params, const float* x, const f
   const double inv = n ? 1.0 / n : 0.0;
   for (int k = 0; k < kParams; k++) grad[k] = g[k] * inv;
   return loss * inv;
+}
+
+// ---- seeder features (restating this build's own definition, FP32) --------
+// Per-brick table: distances at the 8 brick corners, then each octant's
+// minimum voxel-corner distance (voxels with 2*v >= brick_size on an axis are
+// in that axis's upper half; an octant without voxels takes the brick
+// minimum), all multiplied by 0.5*lod_size (brick units).
+extern "C" int orc_mlp_brick_features(const orc_scene* s, uint32_t brick, float out[16]) {
+  using namespace orc;
+  if (!s || !out || brick >= s->s.nodes.size()) return -1;
+  Options O;
+  O.preset = MultiRenderPreset{0, 0, 0, 0, 0, 1, 1};
+  Tracer<float> tr(s->s, O, nullptr);
+  const SdfSBSHeader& h = s->s.header;
+  const uint32_t bs = h.brick_size, v_size = h.brick_size + 2 * h.brick_pad + 1;
+  const float sz = float(s->s.nodes[brick].pos_z_lod_size & 0xFFFF);
+  const float sz_inv = 2.0f / sz, sc = 0.5f * sz;
+  float corner[8], omin[8], all_min = 1e30f;
+  for (int i = 0; i < 8; i++) { corner[i] = 0.0f; omin[i] = 1e30f; }
+  for (uint32_t vx = 0; vx < bs; vx++)
+    for (uint32_t vy = 0; vy < bs; vy++)
+      for (uint32_t vz = 0; vz < bs; vz++) {
+        const uint32_t vp[3] = {vx, vy, vz};
+        float v[8];
+        const float vmin = tr.load_distance_values(brick, vp, v_size, sz_inv, v);
+        const int o = ((2 * vx >= bs) ? 4 : 0) | ((2 * vy >= bs) ? 2 : 0) | ((2 * vz >= bs) ? 1 : 0);
+        omin[o] = std::min(omin[o], vmin);
+        all_min = std::min(all_min, vmin);
+        for (int c = 0; c < 8; c++) {
+          const uint32_t cx = (c >> 2) & 1, cy = (c >> 1) & 1, cz = c & 1;
+          if (vx == cx * (bs - 1) && vy == cy * (bs - 1) && vz == cz * (bs - 1)) corner[c] = v[c];
+        }
+      }
+  for (int o = 0; o < 8; o++)
+    if (omin[o] > 1e29f) omin[o] = all_min;
+  for (int i = 0; i < 8; i++) { out[i] = corner[i] * sc; out[8 + i] = omin[i] * sc; }
+  return 0;
+}
+
+// 24 features of a ray entering `brick`: direction; entry point in the brick
+// ([0,1]^3); segment length; entry t relative to the closest approach to the
+// brick centre (both in brick units); then the per-brick table. ray = pos
+// xyz, dir xyz. *t0 receives the brick entry distance (world).
+extern "C" int orc_mlp_features(const orc_scene* s, const float ray[6], uint32_t brick, float out[24], float* t0) {
+  using namespace orc;
+  if (!s || !ray || !out || brick >= s->s.nodes.size()) return -1;
+  const SdfSBSNode& N = s->s.nodes[brick];
+  const float px = float(N.pos_xy >> 16), py = float(N.pos_xy & 0xFFFF), pz = float(N.pos_z_lod_size >> 16);
+  const float sz = float(N.pos_z_lod_size & 0xFFFF);
+  const float sz_inv = 2.0f / sz;
+  const V3<float> pos{ray[0], ray[1], ray[2]}, dir{ray[3], ray[4], ray[5]};
+  const V3<float> bmin = V3<float>{-1, -1, -1} + sz_inv * V3<float>{px, py, pz};
+  const V3<float> bmax = bmin + sz_inv * V3<float>{1, 1, 1};
+  const float sc = 0.5f * sz;
+  const V2<float> bnf = RayBoxIntersection2(pos, SafeInverse(dir), bmin, bmax);
+  const V3<float> e = (pos + bnf.x * dir - bmin) * sc;
+  const V3<float> c = bmin + (0.5f / sc) * V3<float>{1, 1, 1};
+  const float tc = dot(c - pos, dir);
+  out[0] = dir.x; out[1] = dir.y; out[2] = dir.z;
+  out[3] = e.x; out[4] = e.y; out[5] = e.z;
+  out[6] = (bnf.y - bnf.x) * sc;
+  out[7] = (bnf.x - tc) * sc;
+  if (t0) *t0 = bnf.x;
+  return orc_mlp_brick_features(s, brick, out + 8);
 }

@@ -286,7 +286,7 @@ template <typename T> struct Tracer {
             rtn -= dx;
             it++;
           }
-          iter = it;
+          if (cnt) cnt->root_iters += it;  // the reference keeps Newton's count local
           t = rtn;
           hit = (t >= 0 && t <= qFar && std::abs(f) < EPS);
         } else {

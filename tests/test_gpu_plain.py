@@ -11,7 +11,7 @@ import pytest
 
 import litert_amd.api as L
 import oracle
-from helpers import MISS, angle, c1_scene, c2_scene, oracle_scene, preset
+from helpers import MISS, angle, c1_scene, c2_scene, compare_gbuffer as _compare, oracle_scene, preset
 
 pytestmark = pytest.mark.gpu
 
@@ -25,62 +25,6 @@ def r_c1():
     r.SetScene(c1_scene())
     yield r
     r.close()
-
-
-def _grazing_or_seam(wv, pr, W, H, ys, xs, t, n, voxels_per_unit):
-    """|cos(ray, n)| < 1e-2, or the hit lies within 1e-3 voxel of a voxel face."""
-    out = []
-    for y, x, tt, nn in zip(ys, xs, t, n):
-        ray = oracle.eye_ray(wv, pr, int(x), int(y), 0.5, 0.5, W, H, 64)
-        o, d = ray[:3], ray[4:7]
-        c = abs(float(np.dot(d, _dec(nn))))
-        p = (o + tt * d + 1.0) * voxels_per_unit
-        seam = np.min(np.abs(p - np.round(p))) < 1e-3
-        out.append(c < 1e-2 or seam)
-    return np.array(out, bool)
-
-
-def _dec(n):
-    from helpers import decode_oct
-    return decode_oct(np.asarray(n, np.float64)[None])[0]
-
-
-def _compare(r, sc, p, wv, pr, W, H, voxels_per_unit):
-    r.SetViewport(0, 0, W, H)
-    r.SetPreset(p)
-    r.UpdateCamera(wv, pr)
-    t, n, prim = r.RenderGBuffer(W, H)
-    o32 = sc.render(wv, pr, p, W, H, precision=32)
-    o64 = sc.render(wv, pr, p, W, H, precision=64)
-    g_hit, c_hit, d_hit = prim != MISS, o32["prim"] != MISS, o64["prim"] != MISS
-    res = {}
-    # bit-level agreement with the FP32 twin
-    both32 = g_hit & c_hit
-    res["mask_mismatch_fp32"] = int((g_hit != c_hit).sum())
-    res["exact_t_fp32"] = float(np.mean(t[both32] == o32["t"][both32].astype(np.float32))) if both32.any() else 1.0
-    # error against FP64
-    mism = np.argwhere(g_hit != d_hit)
-    res["mask_mismatch_fp64"] = len(mism)
-    if len(mism):
-        ys, xs = mism[:, 0], mism[:, 1]
-        tt = np.where(g_hit[ys, xs], t[ys, xs], o64["t"][ys, xs])
-        nn = np.where(g_hit[ys, xs][:, None], n[ys, xs], o64["n"][ys, xs])
-        res["mismatch_all_grazing"] = bool(_grazing_or_seam(wv, pr, W, H, ys, xs, tt, nn, voxels_per_unit).all())
-    else:
-        res["mismatch_all_grazing"] = True
-    both = g_hit & d_hit
-    rel = np.abs(t[both].astype(np.float64) - o64["t"][both]) / o64["t"][both]
-    res["depth_rel_max"] = float(rel.max()) if rel.size else 0.0
-    ang = angle(n[both], o64["n"][both])
-    ang32 = angle(o32["n"][both], o64["n"][both]) if (c_hit[both]).all() else np.zeros_like(ang)
-    # seam pixels: the FP32 CPU path itself leaves the 1e-4 band there (normal
-    # field of the trilinear interpolant is discontinuous across voxel faces)
-    bad = (ang > 1e-4) & ~(ang32 > 1e-4)
-    res["normal_bad"] = int(bad.sum())
-    res["normal_p999"] = float(np.quantile(ang, 0.999)) if ang.size else 0.0
-    res["n_pix"] = W * H
-    res["hits"] = int(g_hit.sum())
-    return res
 
 
 @pytest.mark.parametrize("mode", MODES)
