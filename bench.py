@@ -74,6 +74,84 @@ def dr_problem(L, np, views=16, res=512, brick_count=32, brick_size=4):
     return tgt, ini, cams, refs, dr
 
 
+def time_views(r, wvs, prs, steps, warmup):
+    """Wall-clock Mrays/s and mean kernel ms of `steps` launches of all views."""
+    for _ in range(warmup):
+        r.RenderViewsDevice(wvs, prs, W, H, 0, timed=True)
+    r.Sync()
+    ks = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ks.append(r.RenderViewsDevice(wvs, prs, W, H, 0, timed=True))
+    r.Sync()
+    el = time.perf_counter() - t0
+    return wvs.shape[0] * W * H * steps / el / 1e6, sum(ks) / len(ks)
+
+
+def mlp_leg(L, np, r, wvs, prs, args):
+    """Learned seeder: collect (ray, t_hit) events from plain renders of 32
+    other seeded config-1-style scenes, train on the GPU, then time seeded vs
+    plain on (a) the config-1 scene with the reference's default sphere-tracing
+    root finder (config 3) and (b) the config-2 workload (Newton)."""
+    MLP_FLOP = 19840
+    t0 = time.perf_counter()
+    r.MlpDatasetClear()
+    per_scene = args.mlp_events // args.mlp_scenes
+    for k in range(args.mlp_scenes):
+        s = L.displaced_icosphere(5, 0.8, 0.05, 3, seed=1000 + k, lod=32, brick_size=4, brick_pad=1, bytes_per_value=4)
+        r.SetScene(s)
+        cams = [L.orbit_camera(5000 + k, i) for i in range(2)]
+        r.MlpCollect(np.stack([c[0] for c in cams]), np.stack([c[1] for c in cams]), 512, 512, per_scene)
+    n = r.MlpDatasetSize()
+    collect_s = time.perf_counter() - t0
+    r.MlpInit(1)
+    losses, train_ms = r.MlpTrain(args.mlp_epochs, batch=16384, lr=2e-3, seed=1)
+    f, t, _ = r.MlpDataset(0, min(n, 200000))
+    out, _ = r.MlpInfer(f)
+    acc = float(np.mean((out[:, 2] > 0) == (t[:, 1] > 0.5)))
+    hit = t[:, 1] > 0.5
+    t_err = float(np.median(np.abs(0.5 * (out[hit, 0] + out[hit, 1]) - t[hit, 0]))) if hit.any() else None
+    inf_ms, _ = r.MlpInferDataset(repeats=5)
+    tflops = MLP_FLOP * n / (inf_ms * 1e-3) / 1e12
+    log(f"mlp: {n} events from {args.mlp_scenes} scenes ({collect_s:.1f}s), train {train_ms / 1e3:.2f}s, loss "
+        f"{losses[0]:.4f}->{losses[-1]:.4f}, hit acc {acc:.4f}, inference {inf_ms:.3f} ms ({tflops:.1f} TFLOP/s)")
+    info = {"train_scenes": args.mlp_scenes, "train_events": int(n), "epochs": args.mlp_epochs, "train_ms": round(train_ms, 1),
+            "train_samples_per_s": round(n * args.mlp_epochs / (train_ms * 1e-3), 0),
+            "loss_first_last": [float(losses[0]), float(losses[-1])], "hit_accuracy": round(acc, 4),
+            "median_abs_t_err_brick_units": t_err,
+            "inference": {"samples": int(n), "kernel_ms": round(inf_ms, 4), "tflops": round(tflops, 2),
+                          "mfma_bf16_peak_tflops": 2500.0, "mfma_frac": round(tflops / 2500.0, 5)}}
+    res = {}
+    for name, scene, mode in (("config3_st", "c1", L.SDF_OCTREE_NODE_INTERSECT_ST),
+                              ("config2_newton", "c2", L.SDF_OCTREE_NODE_INTERSECT_NEWTON)):
+        sbs = (L.displaced_icosphere(5, 0.8, 0.05, 3, seed=1, lod=32, brick_size=4, brick_pad=1, bytes_per_value=4)
+               if scene == "c1" else plain_scene(L))
+        r.SetScene(sbs)
+        pp = L.getDefaultPreset()
+        pp.sdf_node_intersect = mode
+        pp.render_mode = L.MULTI_RENDER_MODE_NORMAL
+        r.SetPreset(pp)
+        row = {}
+        for seeded in (False, True):
+            r.SetSeeding(seeded, 0.9)
+            mr, kms = time_views(r, wvs, prs, args.steps, 1)
+            cnt = r.CountWorkEx(wvs, prs, W, H)
+            key = "seeded" if seeded else "plain"
+            row[key + "_mrays_s"] = round(mr, 1)
+            row[key + "_kernel_ms"] = round(kms, 3)
+            row[key + "_root_iters_per_ray"] = round(float(cnt[3]) / (wvs.shape[0] * W * H), 4)
+            if seeded:
+                row["mlp_tiles"] = int(cnt[4])
+                row["gated_bricks"] = int(cnt[5])
+        r.SetSeeding(False)
+        row["bricks"] = int(sbs.nodes.shape[0])
+        res[name] = row
+        log(f"{name}: plain {row['plain_mrays_s']} Mrays/s, seeded {row['seeded_mrays_s']} Mrays/s; root iters/ray "
+            f"{row['plain_root_iters_per_ray']} -> {row['seeded_root_iters_per_ray']}")
+    info.update(res)
+    return res["config2_newton"]["seeded_mrays_s"], info
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +159,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-dr", action="store_true", help="skip the DR leg")
+    ap.add_argument("--no-mlp", action="store_true", help="skip the learned-seeder leg")
+    ap.add_argument("--mlp-scenes", type=int, default=32)
+    ap.add_argument("--mlp-events", type=int, default=16_000_000)
+    ap.add_argument("--mlp-epochs", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU work for the baseline sample")
     ap.add_argument("--calib-copy-mib", type=int, default=0, help="append a device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -127,6 +209,12 @@ def main():
     achieved = alg_bytes / (avg_kernel_ms * 1e-3) / 1e9
     log(f"work/ray: pairs {counts[0] / rays:.3f} bricks {counts[1] / rays:.4f} voxels {counts[2] / rays:.4f}; "
         f"algorithmic {alg_bytes / 1e9:.3f} GB/launch -> {achieved:.0f} GB/s")
+
+    mlp_mrays, mlp_info = None, None
+    if not args.no_mlp:
+        rm = L.CreateMultiRenderer(L.DEVICE_GPU)
+        mlp_mrays, mlp_info = mlp_leg(L, np, rm, wvs, prs, args)
+        rm.close()
 
     dr_mrays = None
     dr_info = None
@@ -223,7 +311,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
-        "mlp_seeded_mrays_s": None,
+        "mlp_seeded_mrays_s": mlp_mrays,
+        "mlp": mlp_info,
         "dr_fwd_bwd_mrays_s": None if dr_mrays is None else round(dr_mrays, 2),
         "dr": dr_info,
     }
