@@ -38,19 +38,21 @@ def _run(cmd):
     return r.stdout
 
 
-def build(verbose=False, jobs=None):
-    os.makedirs(BUILD, exist_ok=True)
+def build(verbose=False, jobs=None, extra_flags=(), out=None, build_dir=None):
+    """Compiles the HIP library; extra_flags / out / build_dir allow variant builds."""
+    bdir = build_dir or BUILD
+    os.makedirs(bdir, exist_ok=True)
     hipcc = os.path.join(ROCM, "bin", "hipcc")
     cmds = []
     objs = []
     for src in HOST_SOURCES:
-        obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+        obj = os.path.join(bdir, os.path.basename(src) + ".o")
         cmds.append(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     for src in HIP_SOURCES:
         path = os.path.join(CSRC, src)
-        obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-        cmds.append([hipcc, *HIP_FLAGS, "-c", path, "-o", obj])
+        obj = os.path.join(bdir, os.path.basename(src) + ".o")
+        cmds.append([hipcc, *HIP_FLAGS, *extra_flags, "-c", path, "-o", obj])
         objs.append(obj)
     jobs = jobs or min(8, os.cpu_count() or 4)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -60,11 +62,12 @@ def build(verbose=False, jobs=None):
             print(" ".join(c))
             if o.strip():
                 print(o)
-    tmp = LIB + ".tmp"
+    lib = out or LIB
+    tmp = lib + ".tmp"
     _run(["g++", "-shared", "-fopenmp", "-o", tmp, *objs, "-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
           "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-Wl,--no-undefined"])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 def build_cpp_tests(oracle_lib):

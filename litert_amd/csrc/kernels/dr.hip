@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void k_dr_primary(DRArgs A, uint32_t* work_cou
   __shared__ uint32_t lds_stack[kStackLds * 256];
   __shared__ uint32_t chunk_base;
   Stack st;
-  st.lds = lds_stack + threadIdx.x;
+  st.lds = (lds_u32*)(lds_stack + threadIdx.x);
   st.lds_stride = 256;
   st.lds_entries = kStackLds;
   st.spill = spill + blockIdx.x * 256 + threadIdx.x;
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(256) void k_dr_border_rays(DRArgs A, const uint32_t
                                                         uint32_t* hits) {
   __shared__ uint32_t lds_stack[kStackLds * 256];
   Stack st;
-  st.lds = lds_stack + threadIdx.x;
+  st.lds = (lds_u32*)(lds_stack + threadIdx.x);
   st.lds_stride = 256;
   st.lds_entries = kStackLds;
   st.spill = spill + blockIdx.x * 256 + threadIdx.x;

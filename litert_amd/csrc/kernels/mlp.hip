@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void k_mlp_collect(DevScene S, TraceParams P, 
   __shared__ uint32_t lds_stack[kStackLds * 256];
   __shared__ uint32_t chunk_base;
   Stack st;
-  st.lds = lds_stack + threadIdx.x;
+  st.lds = (lds_u32*)(lds_stack + threadIdx.x);
   st.lds_stride = 256;
   st.lds_entries = kStackLds;
   st.spill = spill + blockIdx.x * 256 + threadIdx.x;
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
   __shared__ uint32_t chunk_base;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   Stack st;
-  st.lds = lds_stack + threadIdx.x;
+  st.lds = (lds_u32*)(lds_stack + threadIdx.x);
   st.lds_stride = 256;
   st.lds_entries = kStackLdsSeeded;
   st.spill = spill + blockIdx.x * 256 + threadIdx.x;

@@ -13,7 +13,14 @@
 
 namespace lrtd {
 
-template <bool COUNT, bool RGBA>
+#ifndef LRT_SPEC_WAVES
+#define LRT_SPEC_WAVES 6
+#endif
+// Occupancy target (waves per SIMD) of the render kernels specialised for one
+// root finder on DX-layout scenes; the register budget is 512 / waves.
+constexpr int kSpecWaves = LRT_SPEC_WAVES;
+
+template <bool COUNT, bool RGBA, int RF, bool IDX>
 __device__ __forceinline__ void render_pixel(const DevScene& S, const TraceParams& P, const Camera* __restrict__ cams, const RenderArgs& A,
                                              const RenderOut& O, uint64_t gtid, Stack& st, WorkCount& wc) {
   const uint32_t view = (uint32_t)(gtid / A.n_packed);
@@ -28,7 +35,7 @@ __device__ __forceinline__ void render_pixel(const DevScene& S, const TraceParam
     const float2 d = sample_offset(A, x, y, i);
     v3 pos, dir;
     eye_ray(C, x, y, d.x, d.y, A.W, A.H, pos, dir);
-    Hit hit = RayQuery_NearestHit<COUNT>(S, P, pos, 0.0f, dir, 1e9f, st, &wc);
+    Hit hit = RayQuery_NearestHit<COUNT, RF, IDX>(S, P, pos, 0.0f, dir, 1e9f, st, &wc);
     if (i == 0) {
       if (O.t) O.t[pix] = hit.t;
       if (O.prim) O.prim[pix] = hit.primId;
@@ -45,13 +52,13 @@ __device__ __forceinline__ void render_pixel(const DevScene& S, const TraceParam
   }
 }
 
-template <bool COUNT, bool RGBA>
-__global__ __launch_bounds__(256) void k_render(DevScene S, TraceParams P, const Camera* __restrict__ cams, RenderArgs A,
+template <bool COUNT, bool RGBA, int RF, bool IDX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RF >= 0 && !IDX ? kSpecWaves : 1, 8))) void k_render(DevScene S, TraceParams P, const Camera* __restrict__ cams, RenderArgs A,
                                                 RenderOut O, uint32_t* work_counter, uint32_t* spill, unsigned long long* counts) {
   __shared__ uint32_t lds_stack[kStackLds * 256];
   __shared__ uint32_t chunk_base;
   Stack st;
-  st.lds = lds_stack + threadIdx.x;
+  st.lds = (lds_u32*)(lds_stack + threadIdx.x);
   st.lds_stride = 256;
   st.lds_entries = kStackLds;
   const uint32_t gl = blockIdx.x * 256 + threadIdx.x;
@@ -66,7 +73,7 @@ __global__ __launch_bounds__(256) void k_render(DevScene S, TraceParams P, const
     __syncthreads();
     if (base >= total) break;  // uniform across the block
     const uint64_t gtid = base + threadIdx.x;
-    if (gtid < total) render_pixel<COUNT, RGBA>(S, P, cams, A, O, gtid, st, wc);
+    if (gtid < total) render_pixel<COUNT, RGBA, RF, IDX>(S, P, cams, A, O, gtid, st, wc);
   }
   if (COUNT) {
     atomicAdd(&counts[0], (unsigned long long)wc.pairs);
@@ -80,7 +87,7 @@ __global__ __launch_bounds__(256) void k_cast_rays(DevScene S, TraceParams P, co
                                                    CRT_Hit* __restrict__ out, uint32_t* spill) {
   __shared__ uint32_t lds_stack[kStackLds * 256];
   Stack st;
-  st.lds = lds_stack + threadIdx.x;
+  st.lds = (lds_u32*)(lds_stack + threadIdx.x);
   st.lds_stride = 256;
   st.lds_entries = kStackLds;
   const uint32_t gl = blockIdx.x * 256 + threadIdx.x;
@@ -112,19 +119,41 @@ hipError_t zero_u32(uint32_t* d, uint32_t n, hipStream_t stream) {
   return hipGetLastError();
 }
 
+// Kernel variants: the root finder is fixed at compile time for the two
+// modes the benchmarks use on DX-layout scenes (fewer live registers, more
+// waves per SIMD); other modes and indexed layouts use the generic build.
+template <int RF, bool IDX, typename F>
+static hipError_t with_counts_rgba(bool counts, bool rgba, F&& f) {
+  if (counts) return rgba ? f.template go<true, true, RF, IDX>() : f.template go<true, false, RF, IDX>();
+  return rgba ? f.template go<false, true, RF, IDX>() : f.template go<false, false, RF, IDX>();
+}
+
+template <typename F>
+static hipError_t dispatch_variant(const lrtd::DevScene& S, const lrtd::TraceParams& P, bool counts, bool rgba, F&& f) {
+  const bool indexed = S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F || S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F_IN;
+  if (indexed) return with_counts_rgba<-1, true>(counts, rgba, f);
+  if (P.sdf_node_intersect == LRT_SDF_NODE_INTERSECT_NEWTON) return with_counts_rgba<(int)LRT_SDF_NODE_INTERSECT_NEWTON, false>(counts, rgba, f);
+  if (P.sdf_node_intersect == LRT_SDF_NODE_INTERSECT_ST) return with_counts_rgba<(int)LRT_SDF_NODE_INTERSECT_ST, false>(counts, rgba, f);
+  return with_counts_rgba<-1, false>(counts, rgba, f);
+}
+
+struct RenderLaunch {
+  const lrtd::DevScene& S; const lrtd::TraceParams& P; const lrtd::Camera* cams; const lrtd::RenderArgs& A; const lrtd::RenderOut& O;
+  uint32_t* counter; uint32_t* spill; uint32_t blocks; unsigned long long* counts; hipStream_t stream;
+  template <bool C, bool R, int RF, bool IDX>
+  hipError_t go() {
+    hipLaunchKernelGGL((lrtd::k_render<C, R, RF, IDX>), dim3(blocks), dim3(256), 0, stream, S, P, cams, A, O, counter, spill, counts);
+    return hipGetLastError();
+  }
+};
+
 hipError_t launch_render(const lrtd::DevScene& S, const lrtd::TraceParams& P, const lrtd::Camera* d_cams, const lrtd::RenderArgs& A,
                          const lrtd::RenderOut& O, uint32_t* d_work_counter, uint32_t* d_spill, uint32_t blocks, bool rgba,
                          unsigned long long* d_counts, hipStream_t stream) {
   hipError_t e = zero_u32(d_work_counter, 1, stream);
   if (e != hipSuccess) return e;
-  if (d_counts) {
-    if (rgba) hipLaunchKernelGGL((lrtd::k_render<true, true>), dim3(blocks), dim3(256), 0, stream, S, P, d_cams, A, O, d_work_counter, d_spill, d_counts);
-    else hipLaunchKernelGGL((lrtd::k_render<true, false>), dim3(blocks), dim3(256), 0, stream, S, P, d_cams, A, O, d_work_counter, d_spill, d_counts);
-  } else {
-    if (rgba) hipLaunchKernelGGL((lrtd::k_render<false, true>), dim3(blocks), dim3(256), 0, stream, S, P, d_cams, A, O, d_work_counter, d_spill, d_counts);
-    else hipLaunchKernelGGL((lrtd::k_render<false, false>), dim3(blocks), dim3(256), 0, stream, S, P, d_cams, A, O, d_work_counter, d_spill, d_counts);
-  }
-  return hipGetLastError();
+  RenderLaunch L{S, P, d_cams, A, O, d_work_counter, d_spill, blocks, d_counts, stream};
+  return dispatch_variant(S, P, d_counts != nullptr, rgba, L);
 }
 
 hipError_t launch_cast_rays(const lrtd::DevScene& S, const lrtd::TraceParams& P, const float4* d_rays, uint32_t n, CRT_Hit* d_out,

@@ -68,9 +68,12 @@ __device__ __forceinline__ float2 RayBox(v3 o, v3 inv, v3 bmin, v3 bmax) {
   const float hi1 = inv.y * (bmax.y - o.y);
   const float lo2 = inv.z * (bmin.z - o.z);
   const float hi2 = inv.z * (bmax.z - o.z);
-  const float tmin = mx(mn(lo, hi), mn(lo1, hi1));
-  const float tmax = mn(mx(lo, hi), mx(lo1, hi1));
-  return make_float2(mx(tmin, mn(lo2, hi2)), mn(tmax, mx(lo2, hi2)));
+  // fminf/fmaxf equal the reference's min/max here: the operands are finite
+  // (SafeInverse keeps |inv| <= 1e36) and a signed-zero difference cannot
+  // change any later comparison. They map to single v_min/v_max instructions.
+  const float tmin = fmaxf(fminf(lo, hi), fminf(lo1, hi1));
+  const float tmax = fminf(fmaxf(lo, hi), fmaxf(lo1, hi1));
+  return make_float2(fmaxf(tmin, fminf(lo2, hi2)), fminf(tmax, fmaxf(lo2, hi2)));
 }
 
 // ---- BVH2Common.cpp:62-72 --------------------------------------------------
@@ -117,11 +120,12 @@ __device__ __forceinline__ bool is_indexed(const DevScene& S) {
 }
 
 // ---- BVH2Common.cpp:725-762, trilinear branch -----------------------------
+template <bool IDX = true>
 __device__ __forceinline__ float load_voxel(const DevScene& S, uint32_t v_off, uint32_t vx, uint32_t vy, uint32_t vz, float sz_inv,
                                             float values[8]) {
   float vmin = 1e6f;
   const uint32_t vs = S.v_size, pad = S.brick_pad;
-  if (is_indexed(S)) {
+  if (IDX && is_indexed(S)) {
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       uint32_t vId = (vx + ((i & 4) >> 2) + pad) * vs * vs + (vy + ((i & 2) >> 1) + pad) * vs + (vz + (i & 1) + pad);
@@ -129,17 +133,33 @@ __device__ __forceinline__ float load_voxel(const DevScene& S, uint32_t v_off, u
       vmin = mn(vmin, values[i]);
     }
   } else {
+    // q = (values[v_off + vId / (4/bpv)] >> (8*bpv * (vId % (4/bpv)))) & max_val
+    // is the little-endian bpv-byte element vId of the brick's words, read
+    // here directly with a byte / short / dword load.
     const uint32_t bpv = S.bytes_per_value;
-    const uint32_t vals_per_int = 4 / bpv;
     const uint32_t bits = 8 * bpv;
     const uint32_t max_val = bpv == 4 ? 0xFFFFFFFFu : ((1u << bits) - 1);
     const float d_max = S.quant_voxel_diag ? 1.73205081f * (sz_inv / (float)S.brick_size) : 1.73205081f * sz_inv;
     const float mult = 2.0f * d_max / (float)max_val;
+    const uint32_t b0 = (vx + pad) * vs * vs + (vy + pad) * vs + (vz + pad);
+    const uint32_t dx = vs * vs, dy = vs;
+    uint32_t q[8];
+    if (bpv == 1) {
+      const uint8_t* p = (const uint8_t*)(S.values + v_off) + b0;
+#pragma unroll
+      for (int i = 0; i < 8; i++) q[i] = p[((i & 4) ? dx : 0u) + ((i & 2) ? dy : 0u) + (i & 1)];
+    } else if (bpv == 2) {
+      const uint16_t* p = (const uint16_t*)(S.values + v_off) + b0;
+#pragma unroll
+      for (int i = 0; i < 8; i++) q[i] = p[((i & 4) ? dx : 0u) + ((i & 2) ? dy : 0u) + (i & 1)];
+    } else {
+      const uint32_t* p = S.values + v_off + b0;
+#pragma unroll
+      for (int i = 0; i < 8; i++) q[i] = p[((i & 4) ? dx : 0u) + ((i & 2) ? dy : 0u) + (i & 1)];
+    }
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-      uint32_t vId = (vx + ((i & 4) >> 2) + pad) * vs * vs + (vy + ((i & 2) >> 1) + pad) * vs + (vz + (i & 1) + pad);
-      uint32_t q = (S.values[v_off + vId / vals_per_int] >> (bits * (vId % vals_per_int))) & max_val;
-      values[i] = -d_max + mult * (float)q;
+      values[i] = -d_max + mult * (float)q[i];
       vmin = mn(vmin, values[i]);
     }
   }
@@ -215,12 +235,12 @@ __device__ __forceinline__ bool cubic_above_on(float c0, float c1, float c2, flo
 // cubic along the ray is shown to stay above the stopping threshold up to the
 // start, so the root it converges to is the first one either way. Any other mode, or a seed that fails the checks,
 // runs unseeded. n_iter (optional) accumulates root-finder iterations.
-template <bool SEEDED = false>
+template <bool SEEDED = false, int RF = -1>
 __device__ __forceinline__ bool LocalSurfaceIntersection(const TraceParams& P, v3 ray_dir, const float values[8], uint32_t primId,
                                                          uint32_t geomType, float d, float d_inv, float qFar, float fNearX,
                                                          v3 start_q, Hit* pHit, float seed_q = -1.0f, uint32_t* n_iter = nullptr) {
   const float EPS = 1e-6f;
-  const unsigned mode = P.sdf_node_intersect;
+  const unsigned mode = RF >= 0 ? (unsigned)RF : P.sdf_node_intersect;  // RF >= 0: root finder fixed at compile time
   float t = 0.0f;
   bool hit = false;
   unsigned iter = 0;
@@ -442,7 +462,9 @@ __device__ __forceinline__ float step01(float e0, float e1, float x) { return cl
 // Counters (optional) receive voxel and root-find counts.
 // SEEDED: seed_world is a predicted hit distance along the ray (world t);
 // each candidate voxel hands it to the root finder in its own units.
-template <bool SEEDED = false>
+// RF >= 0 fixes the root finder at compile time; IDX = false compiles out the
+// indexed-layout paths (colour, smoothed normals) for DX-layout scenes.
+template <bool SEEDED = false, int RF = -1, bool IDX = true>
 __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TraceParams& P, v3 ray_pos, v3 ray_dir, v3 rayDirInv,
                                                float tNear, uint32_t brick, Hit* pHit, uint32_t* n_vox, float seed_world = -1.0f,
                                                uint32_t* n_iter = nullptr) {
@@ -470,15 +492,16 @@ __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TracePar
     v3 min_pos = bmin + d * vp;
     v3 max_pos = min_pos + d * v3(1, 1, 1);
     if (n_vox) (*n_vox)++;
-    float vmin = load_voxel(S, N.z, (uint32_t)vp.x, (uint32_t)vp.y, (uint32_t)vp.z, sz_inv, values);
+    float vmin = load_voxel<IDX>(S, N.z, (uint32_t)vp.x, (uint32_t)vp.y, (uint32_t)vp.z, sz_inv, values);
     float2 fnf = RayBox(ray_pos, rayDirInv, min_pos, max_pos);
     if (tNear < fnf.x && vmin <= 0.0f) {
       v3 start_pos = ray_pos + fnf.x * ray_dir;
       v3 start_q = (start_pos - min_pos) * qscale;
       float qFar = (fnf.y - fnf.x) * qscale;
       const float seed_q = SEEDED ? (seed_world - fnf.x) * d_inv : -1.0f;
-      bool updated = LocalSurfaceIntersection<SEEDED>(P, ray_dir, values, brick, S.type, d, d_inv, qFar, fnf.x, start_q, pHit, seed_q, n_iter);
-      if (updated && S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F_IN && P.normal_mode == LRT_NORMAL_MODE_SDF_SMOOTHED && P.need_normal) {
+      bool updated =
+          LocalSurfaceIntersection<SEEDED, RF>(P, ray_dir, values, brick, S.type, d, d_inv, qFar, fnf.x, start_q, pHit, seed_q, n_iter);
+      if (IDX && updated && S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F_IN && P.normal_mode == LRT_NORMAL_MODE_SDF_SMOOTHED && P.need_normal) {
         // smoothed normal, BVH2Common.cpp:838-906
         const float beta = 0.5f;
         v3 dp = start_q + ((pHit->t - fnf.x) / d) * ray_dir;
@@ -521,7 +544,7 @@ __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TracePar
     }
     bnf.x += mx(0.0f, fnf.y - bnf.x) + 1e-6f;
   }
-  if (pHit->t < old_t && is_indexed(S)) {  // colour, BVH2Common.cpp:962-982
+  if (IDX && pHit->t < old_t && is_indexed(S)) {  // colour, BVH2Common.cpp:962-982
     v3 pos = ray_pos + pHit->t * ray_dir;
     v3 dp = (pos - bmin) * (0.5f * sz);
     uint32_t t_off = N.z + S.v_size * S.v_size * S.v_size;

@@ -10,8 +10,10 @@
 
 namespace lrtd {
 
+typedef __attribute__((address_space(3))) uint32_t lds_u32;  // an LDS pointer: ds_read / ds_write, never flat
+
 struct Stack {
-  uint32_t* lds;        // base for this lane: entry k at lds[k * lds_stride]
+  lds_u32* lds;         // base for this lane: entry k at lds[k * lds_stride]
   uint32_t lds_stride;  // lanes per block
   uint32_t* spill;      // base for this lane: entry k at spill[k * spill_stride]
   uint32_t spill_stride;
@@ -34,7 +36,7 @@ struct WorkCount {
 
 // BVH2Common.cpp:3407-3488. Returns through *hit; stack capacity is
 // lds_entries + spill entries (>= the tree's pair depth, checked on the host).
-template <bool COUNT>
+template <bool COUNT, int RF = -1, bool IDX = true>
 __device__ __forceinline__ void BVH2Traverse(const DevScene& S, const TraceParams& P, v3 ray_pos, v3 ray_dir, v3 inv, float tNear,
                                              bool stopOnFirstHit, Hit* hit, Stack& st, WorkCount* wc) {
   int top = 0;
@@ -68,7 +70,7 @@ __device__ __forceinline__ void BVH2Traverse(const DevScene& S, const TraceParam
       const float tNearSdf = mx(tNear, 0.1f);  // SDF_BIAS, BVH2Common.cpp:3472-3473
       const float tPrev = hit->t;
       if (COUNT) wc->bricks++;
-      BrickIntersect(S, P, ray_pos, ray_dir, inv, tNearSdf, left & LRT_START_MASK, hit, COUNT ? &wc->voxels : nullptr, -1.0f,
+      BrickIntersect<false, RF, IDX>(S, P, ray_pos, ray_dir, inv, tNearSdf, left & LRT_START_MASK, hit, COUNT ? &wc->voxels : nullptr, -1.0f,
                      COUNT ? &wc->iters : nullptr);
       hitFound = hit->t < tPrev;
     }
@@ -78,7 +80,7 @@ __device__ __forceinline__ void BVH2Traverse(const DevScene& S, const TraceParam
 }
 
 // BVH2Common.cpp:3490-3559 for the single identity instance.
-template <bool COUNT>
+template <bool COUNT, int RF = -1, bool IDX = true>
 __device__ __forceinline__ Hit RayQuery_NearestHit(const DevScene& S, const TraceParams& P, v3 pos, float tNear, v3 dir, float tFar,
                                                    Stack& st, WorkCount* wc) {
   bool stopOnFirstHit = (tFar <= 0.0f);
@@ -89,7 +91,7 @@ __device__ __forceinline__ Hit RayQuery_NearestHit(const DevScene& S, const Trac
   hit.primId = hit.instId = hit.geomId = 0xFFFFFFFFu;
   hit.coords[0] = 1.0f; hit.coords[1] = 0.0f; hit.coords[2] = 0.0f; hit.coords[3] = 0.0f;
   const float2 bh = RayBox(pos, inv, v3(-1, -1, -1), v3(1, 1, 1));
-  if ((bh.x <= bh.y) && (bh.y > tNear) && (bh.x < hit.t)) BVH2Traverse<COUNT>(S, P, pos, dir, inv, tNear, stopOnFirstHit, &hit, st, wc);
+  if ((bh.x <= bh.y) && (bh.y > tNear) && (bh.x < hit.t)) BVH2Traverse<COUNT, RF, IDX>(S, P, pos, dir, inv, tNear, stopOnFirstHit, &hit, st, wc);
   if (hit.primId != 0xFFFFFFFFu) hit.instId = 0;
   return hit;
 }

@@ -155,3 +155,4 @@ def test_bad_inputs_raise():
     bad.nodes[0, 2] = sbs.values.size  # data_offset out of range
     with pytest.raises(L.LrtError):
         r.SetScene(bad)
+
