@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256) void k_mlp_reduce_adam(float* __restrict__ prm
 // the exact brick intersector when the hit logit clears the confidence gate.
 constexpr uint32_t kStackLdsSeeded = 16;
 
-template <bool COUNT, bool RGBA>
+template <bool COUNT, bool RGBA, int RF, bool IDX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_render_seeded(DevScene S, TraceParams P, const float4* __restrict__ bfeat,
                                                        const void* __restrict__ packed, float seed_logit, const Camera* __restrict__ cams,
                                                        RenderArgs A, RenderOut O, uint32_t* work_counter, uint32_t* spill,
@@ -422,7 +422,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
           const float seed_world = confident ? t0 + 0.5f * (z0 + z1) / s : (z2 <= -seed_logit ? 3.0e38f : -1.0f);
           if (COUNT) wc.bricks++, wc.seeds += confident ? 1u : 0u;
           const float tPrev = hit.t;
-          BrickIntersect<true>(S, P, pos, dir, inv, 0.1f, brick, &hit, COUNT ? &wc.voxels : nullptr, seed_world,
+          BrickIntersect<true, RF, IDX>(S, P, pos, dir, inv, 0.1f, brick, &hit, COUNT ? &wc.voxels : nullptr, seed_world,
                                COUNT ? &wc.iters : nullptr);
           if (hit.t < tPrev) {
             active = false;
@@ -503,20 +503,37 @@ hipError_t launch_mlp_reduce_adam(float* d_params, float* d_m, float* d_v, const
   return hipGetLastError();
 }
 
+// Same variant split as the plain render: root finder fixed at compile time
+// for Newton / sphere tracing on DX-layout scenes, generic otherwise.
+struct SeededLaunch {
+  const lrtd::DevScene& S; const lrtd::TraceParams& P; const float4* bfeat; const void* packed; float logit; const lrtd::Camera* cams;
+  const lrtd::RenderArgs& A; const lrtd::RenderOut& O; uint32_t* counter; uint32_t* spill; uint32_t blocks; unsigned long long* counts;
+  hipStream_t stream;
+  template <bool C, bool R, int RF, bool IDX>
+  hipError_t go() {
+    hipLaunchKernelGGL((lrtd::k_render_seeded<C, R, RF, IDX>), dim3(blocks), dim3(256), 0, stream, S, P, bfeat, packed, logit, cams, A, O,
+                       counter, spill, counts);
+    return hipGetLastError();
+  }
+  template <int RF, bool IDX>
+  hipError_t pick(bool rgba) {
+    if (counts) return rgba ? go<true, true, RF, IDX>() : go<true, false, RF, IDX>();
+    return rgba ? go<false, true, RF, IDX>() : go<false, false, RF, IDX>();
+  }
+};
+
 hipError_t launch_render_seeded(const lrtd::DevScene& S, const lrtd::TraceParams& P, const float4* d_bfeat, const void* d_packed,
                                 float seed_logit, const lrtd::Camera* d_cams, const lrtd::RenderArgs& A, const lrtd::RenderOut& O,
                                 uint32_t* d_work_counter, uint32_t* d_spill, uint32_t blocks, bool rgba, unsigned long long* d_counts,
                                 hipStream_t stream) {
   hipError_t e = zero_u32(d_work_counter, 1, stream);
   if (e != hipSuccess) return e;
-  if (d_counts) {
-    if (rgba) hipLaunchKernelGGL((lrtd::k_render_seeded<true, true>), dim3(blocks), dim3(256), 0, stream, S, P, d_bfeat, d_packed, seed_logit, d_cams, A, O, d_work_counter, d_spill, d_counts);
-    else hipLaunchKernelGGL((lrtd::k_render_seeded<true, false>), dim3(blocks), dim3(256), 0, stream, S, P, d_bfeat, d_packed, seed_logit, d_cams, A, O, d_work_counter, d_spill, d_counts);
-  } else {
-    if (rgba) hipLaunchKernelGGL((lrtd::k_render_seeded<false, true>), dim3(blocks), dim3(256), 0, stream, S, P, d_bfeat, d_packed, seed_logit, d_cams, A, O, d_work_counter, d_spill, d_counts);
-    else hipLaunchKernelGGL((lrtd::k_render_seeded<false, false>), dim3(blocks), dim3(256), 0, stream, S, P, d_bfeat, d_packed, seed_logit, d_cams, A, O, d_work_counter, d_spill, d_counts);
-  }
-  return hipGetLastError();
+  SeededLaunch L{S, P, d_bfeat, d_packed, seed_logit, d_cams, A, O, d_work_counter, d_spill, blocks, d_counts, stream};
+  const bool indexed = S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F || S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F_IN;
+  if (indexed) return L.pick<-1, true>(rgba);
+  if (P.sdf_node_intersect == LRT_SDF_NODE_INTERSECT_NEWTON) return L.pick<(int)LRT_SDF_NODE_INTERSECT_NEWTON, false>(rgba);
+  if (P.sdf_node_intersect == LRT_SDF_NODE_INTERSECT_ST) return L.pick<(int)LRT_SDF_NODE_INTERSECT_ST, false>(rgba);
+  return L.pick<-1, false>(rgba);
 }
 
 }  // namespace lrt

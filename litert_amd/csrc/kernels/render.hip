@@ -23,10 +23,20 @@ constexpr int kSpecWaves = LRT_SPEC_WAVES;
 template <bool COUNT, bool RGBA, int RF, bool IDX>
 __device__ __forceinline__ void render_pixel(const DevScene& S, const TraceParams& P, const Camera* __restrict__ cams, const RenderArgs& A,
                                              const RenderOut& O, uint64_t gtid, Stack& st, WorkCount& wc) {
-  const uint32_t view = (uint32_t)(gtid / A.n_packed);
+  // A wave's 64 packed indices form one 8x8 super-block of one view (each
+  // view spans a multiple of 64 packed indices and chunks start on multiples
+  // of 256), so the view, the super-block and the camera are wave-uniform:
+  // computed on the scalar unit and read with scalar loads into SGPRs.
+  const uint64_t g0 = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(gtid >> 32)) << 32) |
+                      (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)gtid);
+  const uint32_t view = (uint32_t)(g0 / A.n_packed);
   const uint32_t tid = (uint32_t)(gtid - (uint64_t)view * A.n_packed);
-  uint32_t x, y;
-  tid_to_xy(tid, A.packed_w, x, y);
+  const uint32_t sb = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const uint32_t wbh = A.packed_w >> 3;
+  const uint32_t bhx = sb % wbh, bhy = sb / wbh;
+  const uint32_t r = tid & 63u, lh = r >> 4, l = r & 15u;
+  const uint32_t x = ((bhx * 2 + (lh & 1u)) << 2) + (l & 3u);  // tid_to_xy with the uniform part hoisted
+  const uint32_t y = ((bhy * 2 + (lh >> 1)) << 2) + (l >> 2);
   if (x >= A.W || y >= A.H) return;
   const Camera& C = cams[view];
   const size_t pix = (size_t)view * A.W * A.H + (size_t)y * A.W + x;

@@ -41,6 +41,7 @@ struct DevScene {
   const uint4* nodes;       // SdfSBSNode
   const uint32_t* values;   // packed DX words or indices (ID32F layouts)
   const float* values_f;    // shared floats of indexed layouts
+  const float4* brecs;      // per brick: {bmin.xyz, data_offset}, {sz_inv, d, d_inv, qscale}, {d_max, mult, 0.5*sz, -}
   uint32_t brick_size, brick_pad, bytes_per_value, layout;
   uint32_t v_size, quant_voxel_diag, type, n_nodes;
 };
@@ -120,9 +121,11 @@ __device__ __forceinline__ bool is_indexed(const DevScene& S) {
 }
 
 // ---- BVH2Common.cpp:725-762, trilinear branch -----------------------------
+// d_max / mult: the brick's dequantisation constants (ignored for indexed
+// layouts), the same values load_voxel below derives from sz_inv.
 template <bool IDX = true>
-__device__ __forceinline__ float load_voxel(const DevScene& S, uint32_t v_off, uint32_t vx, uint32_t vy, uint32_t vz, float sz_inv,
-                                            float values[8]) {
+__device__ __forceinline__ float load_voxel_q(const DevScene& S, uint32_t v_off, uint32_t vx, uint32_t vy, uint32_t vz, float d_max,
+                                              float mult, float values[8]) {
   float vmin = 1e6f;
   const uint32_t vs = S.v_size, pad = S.brick_pad;
   if (IDX && is_indexed(S)) {
@@ -137,10 +140,6 @@ __device__ __forceinline__ float load_voxel(const DevScene& S, uint32_t v_off, u
     // is the little-endian bpv-byte element vId of the brick's words, read
     // here directly with a byte / short / dword load.
     const uint32_t bpv = S.bytes_per_value;
-    const uint32_t bits = 8 * bpv;
-    const uint32_t max_val = bpv == 4 ? 0xFFFFFFFFu : ((1u << bits) - 1);
-    const float d_max = S.quant_voxel_diag ? 1.73205081f * (sz_inv / (float)S.brick_size) : 1.73205081f * sz_inv;
-    const float mult = 2.0f * d_max / (float)max_val;
     const uint32_t b0 = (vx + pad) * vs * vs + (vy + pad) * vs + (vz + pad);
     const uint32_t dx = vs * vs, dy = vs;
     uint32_t q[8];
@@ -164,6 +163,22 @@ __device__ __forceinline__ float load_voxel(const DevScene& S, uint32_t v_off, u
     }
   }
   return vmin;
+}
+
+// BVH2Common.cpp:749-750 dequantisation constants of a brick of size 2/sz_inv.
+__device__ __forceinline__ void dequant_consts(const DevScene& S, float sz_inv, float& d_max, float& mult) {
+  const uint32_t bits = 8 * S.bytes_per_value;
+  const uint32_t max_val = S.bytes_per_value == 4 ? 0xFFFFFFFFu : ((1u << bits) - 1);
+  d_max = S.quant_voxel_diag ? 1.73205081f * (sz_inv / (float)S.brick_size) : 1.73205081f * sz_inv;
+  mult = 2.0f * d_max / (float)max_val;
+}
+
+template <bool IDX = true>
+__device__ __forceinline__ float load_voxel(const DevScene& S, uint32_t v_off, uint32_t vx, uint32_t vy, uint32_t vz, float sz_inv,
+                                            float values[8]) {
+  float d_max = 0.0f, mult = 0.0f;
+  if (!(IDX && is_indexed(S))) dequant_consts(S, sz_inv, d_max, mult);
+  return load_voxel_q<IDX>(S, v_off, vx, vy, vz, d_max, mult, values);
 }
 
 struct TraceParams {
@@ -469,15 +484,15 @@ __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TracePar
                                                float tNear, uint32_t brick, Hit* pHit, uint32_t* n_vox, float seed_world = -1.0f,
                                                uint32_t* n_iter = nullptr) {
   float values[8];
-  const uint4 N = S.nodes[brick];
-  const float px = (float)(N.x >> 16), py = (float)(N.x & 0xFFFFu), pz = (float)(N.y >> 16), sz = (float)(N.y & 0xFFFFu);
+  // Per-brick constants precomputed on the host with the same float
+  // operations (brick_records): bmin = -1 + sz_inv * pos, sz_inv = 2/sz,
+  // d = 2/(sz*bs), d_inv = 1/d, qscale = 0.5*sz*bs, d_max, mult, 0.5*sz.
+  const float4 r0 = S.brecs[3 * (size_t)brick], r1 = S.brecs[3 * (size_t)brick + 1], r2 = S.brecs[3 * (size_t)brick + 2];
+  const uint32_t v_off = __float_as_uint(r0.w);
   const float bsf = (float)S.brick_size;
-  const float sz_inv = 2.0f / sz;
-  const float d = 2.0f / (sz * bsf);
-  const float d_inv = 1.0f / d;
-  const v3 bmin = v3(-1, -1, -1) + sz_inv * v3(px, py, pz);
+  const float sz_inv = r1.x, d = r1.y, d_inv = r1.z, qscale = r1.w;
+  const v3 bmin(r0.x, r0.y, r0.z);
   const v3 bmax = bmin + sz_inv * v3(1, 1, 1);
-  const float qscale = 0.5f * sz * bsf;
   const float hi_clamp = bsf - 1e-6f;
   float2 bnf = RayBox(ray_pos, rayDirInv, bmin, bmax);
   const float old_t = pHit->t;
@@ -492,7 +507,7 @@ __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TracePar
     v3 min_pos = bmin + d * vp;
     v3 max_pos = min_pos + d * v3(1, 1, 1);
     if (n_vox) (*n_vox)++;
-    float vmin = load_voxel<IDX>(S, N.z, (uint32_t)vp.x, (uint32_t)vp.y, (uint32_t)vp.z, sz_inv, values);
+    float vmin = load_voxel_q<IDX>(S, v_off, (uint32_t)vp.x, (uint32_t)vp.y, (uint32_t)vp.z, r2.x, r2.y, values);
     float2 fnf = RayBox(ray_pos, rayDirInv, min_pos, max_pos);
     if (tNear < fnf.x && vmin <= 0.0f) {
       v3 start_pos = ray_pos + fnf.x * ray_dir;
@@ -524,7 +539,7 @@ __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TracePar
           if (nid == 13) {
             for (int a = 0; a < 3; a++) nv[a] = (uint32_t)VI[a];
           } else {
-            nb = S.values[N.z + S.v_size * S.v_size * S.v_size + 8 + nid];
+            nb = S.values[v_off + S.v_size * S.v_size * S.v_size + 8 + nid];
             for (int a = 0; a < 3; a++) nv[a] = (uint32_t)(VI[a] - bs * BP[a]);
           }
           v3 ni(0, 0, 0);
@@ -546,8 +561,8 @@ __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TracePar
   }
   if (IDX && pHit->t < old_t && is_indexed(S)) {  // colour, BVH2Common.cpp:962-982
     v3 pos = ray_pos + pHit->t * ray_dir;
-    v3 dp = (pos - bmin) * (0.5f * sz);
-    uint32_t t_off = N.z + S.v_size * S.v_size * S.v_size;
+    v3 dp = (pos - bmin) * r2.z;  // 0.5 * sz
+    uint32_t t_off = v_off + S.v_size * S.v_size * S.v_size;
     v3 col(0, 0, 0);
     for (int i = 0; i < 8; i++) {
       float w = ((i & 4) ? dp.x : 1 - dp.x) * ((i & 2) ? dp.y : 1 - dp.y) * ((i & 1) ? dp.z : 1 - dp.z);

@@ -82,6 +82,7 @@ lrtd::DevScene Renderer::dev_scene() const {
   S.nodes = (const uint4*)d_nodes.p;
   S.values = (const uint32_t*)d_values.p;
   S.values_f = (const float*)d_values_f.p;
+  S.brecs = (const float4*)d_brecs.p;
   S.brick_size = header.brick_size;
   S.brick_pad = header.brick_pad;
   S.bytes_per_value = header.bytes_per_value;
@@ -116,6 +117,36 @@ hipError_t Renderer::ensure_spill(uint32_t blocks, uint32_t lds_entries) {
   uint32_t extra = (uint32_t)std::max(0, pair_depth + 2 - (int)lds_entries);
   size_t n = (size_t)blocks * 256 * std::max<uint32_t>(extra, 1u);
   return d_spill.ensure(n * sizeof(uint32_t));
+}
+
+// Per-brick constants of the voxel walk (BVH2Common.cpp:785-795, 749-750),
+// evaluated once here with exactly the float operations the per-ray code
+// used to repeat: 12 floats per brick, {bmin.xyz, data_offset bits},
+// {sz_inv, d, d_inv, qscale}, {d_max, mult, 0.5*sz, 0}.
+std::vector<float> brick_records(const std::vector<SdfSBSNode>& nodes, const SdfSBSHeader& h, uint32_t quant_voxel_diag) {
+  std::vector<float> out(nodes.size() * 12, 0.0f);
+  const float bsf = (float)h.brick_size;
+  const uint32_t bits = 8 * h.bytes_per_value;
+  const uint32_t max_val = h.bytes_per_value == 4 ? 0xFFFFFFFFu : ((1u << bits) - 1);
+  for (size_t i = 0; i < nodes.size(); i++) {
+    const SdfSBSNode& n = nodes[i];
+    const float px = (float)(n.pos_xy >> 16), py = (float)(n.pos_xy & 0xFFFFu), pz = (float)(n.pos_z_lod_size >> 16);
+    const float sz = (float)(n.pos_z_lod_size & 0xFFFFu);
+    const float sz_inv = 2.0f / sz;
+    const float d = 2.0f / (sz * bsf);
+    const float d_inv = 1.0f / d;
+    const float qscale = 0.5f * sz * bsf;
+    const float d_max = quant_voxel_diag ? 1.73205081f * (sz_inv / bsf) : 1.73205081f * sz_inv;
+    const float mult = 2.0f * d_max / (float)max_val;
+    float* o = out.data() + 12 * i;
+    o[0] = -1.0f + sz_inv * px;
+    o[1] = -1.0f + sz_inv * py;
+    o[2] = -1.0f + sz_inv * pz;
+    std::memcpy(&o[3], &n.data_offset, 4);
+    o[4] = sz_inv; o[5] = d; o[6] = d_inv; o[7] = qscale;
+    o[8] = d_max; o[9] = mult; o[10] = 0.5f * sz;
+  }
+  return out;
 }
 
 static int check_ready(Renderer* r) {
@@ -293,6 +324,8 @@ int lrt_set_scene_sbs(lrt_handle h, const SdfSBSHeader* header, const SdfSBSNode
   r->pairs = to_fat_pairs(tree, bmin, bmax, &r->pair_depth);
   if ((e = r->d_pairs.upload(r->pairs.data(), r->pairs.size() * sizeof(BVHNodePair), r->stream)) != hipSuccess) return hip_err(e, "upload bvh");
   if ((e = r->d_nodes.upload(r->nodes.data(), r->nodes.size() * sizeof(SdfSBSNode), r->stream)) != hipSuccess) return hip_err(e, "upload nodes");
+  std::vector<float> recs = brick_records(r->nodes, r->header, r->quant_voxel_diag);
+  if ((e = r->d_brecs.upload(recs.data(), recs.size() * sizeof(float), r->stream)) != hipSuccess) return hip_err(e, "upload bricks");
   if ((e = r->d_values.upload(r->values.data(), r->values.size() * 4, r->stream)) != hipSuccess) return hip_err(e, "upload values");
   if ((e = r->d_values_f.upload(r->values_f.data(), r->values_f.size() * 4, r->stream)) != hipSuccess) return hip_err(e, "upload values_f");
   if ((e = hipStreamSynchronize(r->stream)) != hipSuccess) return hip_err(e, "scene upload");

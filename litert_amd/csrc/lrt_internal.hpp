@@ -13,6 +13,7 @@ namespace lrt {
 int set_err(int code, const std::string& msg);
 int hip_err(hipError_t e, const char* what);
 lrtd::Camera make_camera(const float wv[16], const float pr[16]);
+std::vector<float> brick_records(const std::vector<SdfSBSNode>& nodes, const SdfSBSHeader& h, uint32_t quant_voxel_diag);
 
 struct DevBuf {
   void* p = nullptr;
@@ -46,7 +47,7 @@ struct Renderer {
   int pair_depth = 0;
 
   // device scene
-  DevBuf d_pairs, d_nodes, d_values, d_values_f;
+  DevBuf d_pairs, d_nodes, d_values, d_values_f, d_brecs;
   // work buffers
   DevBuf d_cams, d_counter, d_spill, d_counts, d_rays, d_hits;
   DevBuf d_out_t, d_out_n, d_out_prim, d_out_rgba;
@@ -84,7 +85,7 @@ struct Renderer {
                     const char* timer_name, unsigned long long* d_counts);
   int finish_timer();
   void release_all() {
-    DevBuf* all[] = {&d_pairs, &d_nodes, &d_values, &d_values_f, &d_cams, &d_counter, &d_spill, &d_counts,
+    DevBuf* all[] = {&d_pairs, &d_nodes, &d_values, &d_values_f, &d_brecs, &d_cams, &d_counter, &d_spill, &d_counts,
                      &d_rays, &d_hits, &d_out_t, &d_out_n, &d_out_prim, &d_out_rgba, &d_dr_ref, &d_dr_img,
                      &d_dr_thr, &d_dr_grad, &d_dr_list, &d_dr_misc, &d_adam_v, &d_adam_s, &d_mlp_params,
                      &d_mlp_m, &d_mlp_v, &d_mlp_packed, &d_mlp_partial, &d_mlp_ploss, &d_mlp_grad, &d_mlp_loss,
