@@ -76,14 +76,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RF >= 0 && 
   st.spill_stride = gridDim.x * 256;
   WorkCount wc;
   const uint64_t total = (uint64_t)A.n_views * A.n_packed;
-  for (;;) {
-    if (threadIdx.x == 0) chunk_base = atomicAdd(work_counter, 1u);
-    __syncthreads();
-    const uint64_t base = (uint64_t)chunk_base * 256;
-    __syncthreads();
-    if (base >= total) break;  // uniform across the block
-    const uint64_t gtid = base + threadIdx.x;
-    if (gtid < total) render_pixel<COUNT, RGBA, RF, IDX>(S, P, cams, A, O, gtid, st, wc);
+  // Work distribution, XCD-aware: workgroups are dispatched round-robin over
+  // the 8 XCDs, so blockIdx.x % 8 names this block's XCD (and its L2). The
+  // packed pixel range is cut into 8 contiguous slices (8 views -> one view
+  // per XCD), each with its own chunk counter on its own 128-B line; a block
+  // drains its XCD's slice first, then helps the others in turn.
+  const uint32_t xcd = blockIdx.x & (kXcds - 1u);
+  const uint64_t n_chunks = (total + 255) / 256;
+  const uint64_t per = (n_chunks + kXcds - 1) / kXcds;
+  for (uint32_t k = 0; k < kXcds; k++) {
+    const uint32_t sl = (xcd + k) & (kXcds - 1u);
+    const uint64_t c0 = per * sl, c1 = c0 + per < n_chunks ? c0 + per : n_chunks;
+    for (;;) {
+      if (threadIdx.x == 0) chunk_base = atomicAdd(work_counter + kCounterStride * sl, 1u);
+      __syncthreads();
+      const uint64_t c = c0 + chunk_base;
+      __syncthreads();
+      if (c >= c1) break;  // uniform across the block
+      const uint64_t gtid = c * 256 + threadIdx.x;
+      if (gtid < total) render_pixel<COUNT, RGBA, RF, IDX>(S, P, cams, A, O, gtid, st, wc);
+    }
   }
   if (COUNT) {
     atomicAdd(&counts[0], (unsigned long long)wc.pairs);
@@ -160,7 +172,7 @@ struct RenderLaunch {
 hipError_t launch_render(const lrtd::DevScene& S, const lrtd::TraceParams& P, const lrtd::Camera* d_cams, const lrtd::RenderArgs& A,
                          const lrtd::RenderOut& O, uint32_t* d_work_counter, uint32_t* d_spill, uint32_t blocks, bool rgba,
                          unsigned long long* d_counts, hipStream_t stream) {
-  hipError_t e = zero_u32(d_work_counter, 1, stream);
+  hipError_t e = zero_u32(d_work_counter, lrtd::kXcds * lrtd::kCounterStride, stream);
   if (e != hipSuccess) return e;
   RenderLaunch L{S, P, d_cams, A, O, d_work_counter, d_spill, blocks, d_counts, stream};
   return dispatch_variant(S, P, d_counts != nullptr, rgba, L);

@@ -10,6 +10,11 @@ namespace lrtd {
 // deeper entries spill to global memory.
 constexpr uint32_t kStackLds = 24;
 
+// Chunk counters of the render kernel: one per XCD, each on its own 128-B
+// line (kCounterStride uint32 apart); the work-counter buffer holds
+// kXcds * kCounterStride words.
+constexpr uint32_t kXcds = 8, kCounterStride = 32;
+
 struct RenderArgs {
   uint32_t W, H;
   uint32_t packed_w, n_packed;  // packed dims are rounded up to multiples of 8

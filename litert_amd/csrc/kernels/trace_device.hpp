@@ -159,7 +159,7 @@ __device__ __forceinline__ float load_voxel_q(const DevScene& S, uint32_t v_off,
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       values[i] = -d_max + mult * (float)q[i];
-      vmin = mn(vmin, values[i]);
+      vmin = fminf(vmin, values[i]);  // finite values: equal to the reference's min, maps to v_min3
     }
   }
   return vmin;
@@ -503,7 +503,9 @@ __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TracePar
   while (bnf.x < bnf.y && pHit->t == old_t && steps++ < max_steps) {
     v3 hit_pos = ray_pos + bnf.x * ray_dir;
     v3 local = (hit_pos - bmin) * qscale;
-    v3 vp(floorf(clampf_(local.x, 1e-6f, hi_clamp)), floorf(clampf_(local.y, 1e-6f, hi_clamp)), floorf(clampf_(local.z, 1e-6f, hi_clamp)));
+    // clamp of finite values with fminf/fmaxf: same result, one v_med3 per axis
+    v3 vp(floorf(fminf(fmaxf(local.x, 1e-6f), hi_clamp)), floorf(fminf(fmaxf(local.y, 1e-6f), hi_clamp)),
+          floorf(fminf(fmaxf(local.z, 1e-6f), hi_clamp)));
     v3 min_pos = bmin + d * vp;
     v3 max_pos = min_pos + d * v3(1, 1, 1);
     if (n_vox) (*n_vox)++;

@@ -166,7 +166,7 @@ int Renderer::render_device(const lrtd::Camera* cams, uint32_t n_views, uint32_t
   if ((e = hipSetDevice(device)) != hipSuccess) return hip_err(e, "hipSetDevice");
   const size_t npix = (size_t)n_views * W * H;
   if ((e = d_cams.upload(cams, sizeof(lrtd::Camera) * n_views, stream)) != hipSuccess) return hip_err(e, "upload cameras");
-  if ((e = d_counter.ensure(64)) != hipSuccess) return hip_err(e, "alloc counter");
+  if ((e = d_counter.ensure(lrtd::kXcds * lrtd::kCounterStride * 4)) != hipSuccess) return hip_err(e, "alloc counter");
   lrtd::RenderOut O{};
   if (gbuf) {
     if ((e = d_out_t.ensure(npix * 4)) != hipSuccess) return hip_err(e, "alloc t");
