@@ -13,12 +13,6 @@
 
 namespace lrtd {
 
-#ifndef LRT_BLOCK
-#define LRT_BLOCK 128
-#endif
-// Render block size and the pixels it takes per chunk-counter atomic.
-constexpr uint32_t kBlock = LRT_BLOCK;
-constexpr uint32_t kChunk = LRT_BLOCK;
 #ifndef LRT_SPEC_WAVES
 #define LRT_SPEC_WAVES 6
 #endif
@@ -69,17 +63,17 @@ __device__ __forceinline__ void render_pixel(const DevScene& S, const TraceParam
 }
 
 template <bool COUNT, bool RGBA, int RF, bool IDX>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RF >= 0 && !IDX ? kSpecWaves : 1, 8))) void k_render(DevScene S, TraceParams P, const Camera* __restrict__ cams, RenderArgs A,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RF >= 0 && !IDX ? kSpecWaves : 1, 8))) void k_render(DevScene S, TraceParams P, const Camera* __restrict__ cams, RenderArgs A,
                                                 RenderOut O, uint32_t* work_counter, uint32_t* spill, unsigned long long* counts) {
-  __shared__ uint32_t lds_stack[kStackLds * kBlock];
+  __shared__ uint32_t lds_stack[kStackLds * 256];
   __shared__ uint32_t chunk_base;
   Stack st;
   st.lds = (lds_u32*)(lds_stack + threadIdx.x);
-  st.lds_stride = kBlock;
+  st.lds_stride = 256;
   st.lds_entries = kStackLds;
-  const uint32_t gl = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t gl = blockIdx.x * 256 + threadIdx.x;
   st.spill = spill + gl;
-  st.spill_stride = gridDim.x * kBlock;
+  st.spill_stride = gridDim.x * 256;
   WorkCount wc;
   const uint64_t total = (uint64_t)A.n_views * A.n_packed;
   // Work distribution, XCD-aware: workgroups are dispatched round-robin over
@@ -88,7 +82,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RF >= 0 
   // per XCD), each with its own chunk counter on its own 128-B line; a block
   // drains its XCD's slice first, then helps the others in turn.
   const uint32_t xcd = blockIdx.x & (kXcds - 1u);
-  const uint64_t n_chunks = (total + kChunk - 1) / kChunk;
+  const uint64_t n_chunks = (total + 255) / 256;
   const uint64_t per = (n_chunks + kXcds - 1) / kXcds;
   for (uint32_t k = 0; k < kXcds; k++) {
     const uint32_t sl = (xcd + k) & (kXcds - 1u);
@@ -99,10 +93,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RF >= 0 
       const uint64_t c = c0 + chunk_base;
       __syncthreads();
       if (c >= c1) break;  // uniform across the block
-      for (uint32_t j = 0; j < kChunk; j += kBlock) {
-        const uint64_t gtid = c * kChunk + j + threadIdx.x;
-        if (gtid < total) render_pixel<COUNT, RGBA, RF, IDX>(S, P, cams, A, O, gtid, st, wc);
-      }
+      const uint64_t gtid = c * 256 + threadIdx.x;
+      if (gtid < total) render_pixel<COUNT, RGBA, RF, IDX>(S, P, cams, A, O, gtid, st, wc);
     }
   }
   if (COUNT) {
@@ -172,8 +164,7 @@ struct RenderLaunch {
   uint32_t* counter; uint32_t* spill; uint32_t blocks; unsigned long long* counts; hipStream_t stream;
   template <bool C, bool R, int RF, bool IDX>
   hipError_t go() {
-    hipLaunchKernelGGL((lrtd::k_render<C, R, RF, IDX>), dim3(blocks * (256 / lrtd::kBlock)), dim3(lrtd::kBlock), 0, stream, S, P, cams, A, O,
-                       counter, spill, counts);
+    hipLaunchKernelGGL((lrtd::k_render<C, R, RF, IDX>), dim3(blocks), dim3(256), 0, stream, S, P, cams, A, O, counter, spill, counts);
     return hipGetLastError();
   }
 };
