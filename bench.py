@@ -1,4 +1,4 @@
-"""Throughput benchmark: primary rays per second on the SBS scene of config 2.
+"""Throughput benchmark: primary rays per second on the 512^3 displaced-icosphere SBS.
 
 Workload (one step): 8 seeded orbit views x 2048x2048 primary rays
 (33,554,432 rays) through the BVH2 over ~1e5 bricks of the displaced
@@ -13,7 +13,7 @@ the kernel's own work counters: 64 B per BVH node pair, 16 B per brick record,
 by the kernel's average duration (HIP events on the launch stream).
 `cpu_baseline` times the CPU oracle (the reference CPU path restated in C++,
 FP32, OpenMP over the host cores) on the same workload. The DR leg
-(`dr_fwd_bwd_mrays_s`) times RenderDR + Adam on config 4 (torus target,
+(`dr_fwd_bwd_mrays_s`) times RenderDR + Adam on the torus-from-sphere problem (torus target,
 sphere start, 128^3 indexed SBS, 16 views at 512x512, border_spp 256).
 """
 import argparse
@@ -29,6 +29,7 @@ W = H = 2048
 N_VIEWS = 8
 SCENE_SEED = 2
 CAM_SEED = 2
+BATCH_SEED = 5
 HBM_PEAK_GBS = 8000.0
 
 
@@ -53,7 +54,7 @@ def plain_scene(L):
 
 
 def dr_problem(L, np, views=16, res=512, brick_count=32, brick_size=4):
-    """Config 4: noisy torus (R 0.6, r 0.2) target, sphere r 0.5 start, dense
+    """DR problem: noisy torus (R 0.6, r 0.2) target, sphere r 0.5 start, dense
     indexed SBS of 128^3 voxels, `views` seeded views at res x res."""
     tgt = L.grid_sbs("torus", 0.6, 0.2, 0.02, 3, seed=4, brick_count=brick_count, brick_size=brick_size)
     ini = L.grid_sbs("sphere", 0.5, 0.0, 0.0, 3, seed=4, brick_count=brick_count, brick_size=brick_size)
@@ -90,9 +91,9 @@ def time_views(r, wvs, prs, steps, warmup):
 
 def mlp_leg(L, np, r, wvs, prs, args):
     """Learned seeder: collect (ray, t_hit) events from plain renders of 32
-    other seeded config-1-style scenes, train on the GPU, then time seeded vs
-    plain on (a) the config-1 scene with the reference's default sphere-tracing
-    root finder (config 3) and (b) the config-2 workload (Newton)."""
+    other seeded 128^3 FP32 displaced icospheres, train on the GPU, then time
+    seeded vs plain on (a) the 128^3 FP32 icosphere with the reference's default
+    sphere-tracing root finder and (b) the 512^3 8-bit icosphere (Newton)."""
     MLP_FLOP = 19840
     t0 = time.perf_counter()
     r.MlpDatasetClear()
@@ -122,10 +123,10 @@ def mlp_leg(L, np, r, wvs, prs, args):
             "inference": {"samples": int(n), "kernel_ms": round(inf_ms, 4), "tflops": round(tflops, 2),
                           "mfma_bf16_peak_tflops": 2500.0, "mfma_frac": round(tflops / 2500.0, 5)}}
     res = {}
-    for name, scene, mode in (("config3_st", "c1", L.SDF_OCTREE_NODE_INTERSECT_ST),
-                              ("config2_newton", "c2", L.SDF_OCTREE_NODE_INTERSECT_NEWTON)):
+    for name, scene, mode in (("ico128_fp32_st", "ico128", L.SDF_OCTREE_NODE_INTERSECT_ST),
+                              ("ico512_u8_newton", "ico512", L.SDF_OCTREE_NODE_INTERSECT_NEWTON)):
         sbs = (L.displaced_icosphere(5, 0.8, 0.05, 3, seed=1, lod=32, brick_size=4, brick_pad=1, bytes_per_value=4)
-               if scene == "c1" else plain_scene(L))
+               if scene == "ico128" else plain_scene(L))
         r.SetScene(sbs)
         pp = L.getDefaultPreset()
         pp.sdf_node_intersect = mode
@@ -149,7 +150,82 @@ def mlp_leg(L, np, r, wvs, prs, args):
         log(f"{name}: plain {row['plain_mrays_s']} Mrays/s, seeded {row['seeded_mrays_s']} Mrays/s; root iters/ray "
             f"{row['plain_root_iters_per_ray']} -> {row['seeded_root_iters_per_ray']}")
     info.update(res)
-    return res["config2_newton"]["seeded_mrays_s"], info
+    return res["ico512_u8_newton"]["seeded_mrays_s"], info
+
+
+def batch_leg(L, np, args, mlp_params):
+    """Uneven batch: `args.batch_scenes` seeded scenes (displaced icosphere or
+    torus; octree depth d ~ U{5..9}, i.e. brick LOD 2^(d-2) x 4^3 voxels; 8-bit
+    distances over the voxel diagonal or FP32), each with its own seeded orbit
+    camera and image of W x H ~ U[128, 2048]^2, rendered in one persistent
+    launch (Newton, depth + normal) with the chunk and grid sizes picked by the
+    autotuner; compared with the same items rendered one launch per scene.
+    With a trained network the seeded form of the same batch is timed too."""
+    rng = np.random.default_rng(BATCH_SEED)
+    p = L.getDefaultPreset()
+    p.sdf_node_intersect = L.SDF_OCTREE_NODE_INTERSECT_NEWTON
+    p.render_mode = L.MULTI_RENDER_MODE_NORMAL
+    rs, wvs, prs, ws, hs, bricks = [], [], [], [], [], []
+    t0 = time.perf_counter()
+    for k in range(args.batch_scenes):
+        depth = int(rng.integers(5, 10))
+        bpv = int(rng.choice([1, 4]))
+        torus = bool(rng.integers(0, 2))
+        w, h = int(rng.integers(128, 2049)), int(rng.integers(128, 2049))
+        kw = dict(seed=BATCH_SEED * 1000 + k, lod=1 << (depth - 2), brick_size=4, brick_pad=0, bytes_per_value=bpv,
+                  voxel_diag_scale=bpv == 1)
+        s = L.displaced_torus(0.6, 0.2, 0.02, 3, **kw) if torus else L.displaced_icosphere(5, 0.8, 0.05, 3, **kw)
+        r = L.CreateMultiRenderer(L.DEVICE_GPU)
+        r.SetScene(s)
+        r.SetPreset(p)
+        bricks.append(int(s.nodes.shape[0]))
+        del s
+        wv, pr = L.orbit_camera(BATCH_SEED * 1000 + k, 0, aspect=w / h)
+        rs.append(r), wvs.append(wv), prs.append(pr), ws.append(w), hs.append(h)
+    build_s = time.perf_counter() - t0
+    wvs, prs = np.stack(wvs), np.stack(prs)
+    rays = int(sum(w * h for w, h in zip(ws, hs)))
+    owner = L.CreateMultiRenderer(L.DEVICE_GPU)
+    owner.SetPreset(p)
+
+    def timed_batch():
+        owner.RenderBatch(rs, wvs, prs, ws, hs, autotune=True)
+        chunk, blocks = owner.BatchConfig()
+        ks = []
+        tb = time.perf_counter()
+        for _ in range(args.steps):
+            ks.append(owner.RenderBatch(rs, wvs, prs, ws, hs))
+        el = time.perf_counter() - tb
+        return {"mrays_s": round(rays * args.steps / el / 1e6, 1), "kernel_ms": round(float(np.mean(ks)), 3),
+                "kernel_mrays_s": round(rays / float(np.mean(ks)) / 1e3, 1), "tuned_chunk": chunk, "tuned_blocks": blocks}
+
+    plain = timed_batch()
+    for r, wv, pr, w, h in zip(rs, wvs, prs, ws, hs):  # warm-up of the per-scene launches
+        r.RenderViewsDevice(wv[None], pr[None], w, h, 0, timed=True)
+    seq_ms = []
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        seq_ms.append(sum(r.RenderViewsDevice(wv[None], pr[None], w, h, 0, timed=True)
+                          for r, wv, pr, w, h in zip(rs, wvs, prs, ws, hs)))
+    seq_el = time.perf_counter() - ts
+    info = {"scenes": len(rs), "rays": rays, "min_rays": int(min(w * h for w, h in zip(ws, hs))),
+            "max_rays": int(max(w * h for w, h in zip(ws, hs))), "bricks_total": int(sum(bricks)),
+            "min_bricks": int(min(bricks)), "max_bricks": int(max(bricks)), "scene_build_s": round(build_s, 1),
+            "one_launch": plain,
+            "launch_per_scene": {"mrays_s": round(rays * args.steps / seq_el / 1e6, 1),
+                                 "kernel_ms_sum": round(float(np.mean(seq_ms)), 3)}}
+    log(f"batch: {len(rs)} scenes, {rays / 1e6:.1f} Mrays, one launch {plain['kernel_ms']} ms ({plain['mrays_s']} Mrays/s wall, "
+        f"chunk {plain['tuned_chunk']}, {plain['tuned_blocks']} blocks); per-scene launches {info['launch_per_scene']['kernel_ms_sum']} "
+        f"ms of kernels ({info['launch_per_scene']['mrays_s']} Mrays/s wall)")
+    if mlp_params is not None:
+        owner.MlpSetParams(mlp_params)
+        owner.SetSeeding(True, 0.9)
+        info["one_launch_seeded"] = timed_batch()
+        owner.SetSeeding(False)
+        log(f"batch seeded: {info['one_launch_seeded']['kernel_ms']} ms ({info['one_launch_seeded']['mrays_s']} Mrays/s wall)")
+    for r in rs + [owner]:
+        r.close()
+    return info
 
 
 def main():
@@ -160,6 +236,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-dr", action="store_true", help="skip the DR leg")
     ap.add_argument("--no-mlp", action="store_true", help="skip the learned-seeder leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched many-scene leg")
+    ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--mlp-scenes", type=int, default=32)
     ap.add_argument("--mlp-events", type=int, default=16_000_000)
     ap.add_argument("--mlp-epochs", type=int, default=3)
@@ -210,11 +288,14 @@ def main():
     log(f"work/ray: pairs {counts[0] / rays:.3f} bricks {counts[1] / rays:.4f} voxels {counts[2] / rays:.4f}; "
         f"algorithmic {alg_bytes / 1e9:.3f} GB/launch -> {achieved:.0f} GB/s")
 
-    mlp_mrays, mlp_info = None, None
+    mlp_mrays, mlp_info, mlp_params = None, None, None
     if not args.no_mlp:
         rm = L.CreateMultiRenderer(L.DEVICE_GPU)
         mlp_mrays, mlp_info = mlp_leg(L, np, rm, wvs, prs, args)
+        mlp_params = rm.MlpGetParams()
         rm.close()
+
+    batch_info = None if args.no_batch else batch_leg(L, np, args, mlp_params)
 
     dr_mrays = None
     dr_info = None
@@ -304,7 +385,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded displaced icosphere SBS, seeded orbit cameras)",
-        "config": {"workload": "config 2: SBS 512^3 effective (LOD 128 x 4^3 bricks, 8-bit), 8 views x 2048x2048 primary rays, Newton",
+        "config": {"workload": "displaced icosphere SBS 512^3 effective (LOD 128 x 4^3 bricks, 8-bit), 8 views x 2048x2048 primary rays, Newton",
                    "bricks": int(sbs.nodes.shape[0]), "rays_per_step": rays, "parallelism": "single GPU"},
         "kernel_ms_per_step": round(avg_kernel_ms, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -313,6 +394,7 @@ def main():
         "cpu_baseline": cpu,
         "mlp_seeded_mrays_s": mlp_mrays,
         "mlp": mlp_info,
+        "batch": batch_info,
         "dr_fwd_bwd_mrays_s": None if dr_mrays is None else round(dr_mrays, 2),
         "dr": dr_info,
     }

@@ -96,6 +96,24 @@ int lrt_sync(lrt_handle h);
 /* Persistent-grid size of the render kernels (0 = device default). */
 int lrt_set_launch_blocks(lrt_handle h, uint32_t blocks);
 
+/* ---- batched uneven workload (no reference counterpart: the reference
+ * renders one scene per call, eye_ray_host.cpp:246-282) ------------------
+ * One persistent launch renders n_items (scene, camera, image) items: item i
+ * traces the scene held by handle scenes[i] (same device as owner) with the
+ * camera world_views[16*i], projs[16*i] into a widths[i] x heights[i]
+ * G-buffer, using owner's preset (single sample). With autotune != 0 the
+ * chunk size (pixels per work-counter atomic) and the grid size are chosen
+ * by timing a small candidate set first, and kept in owner for later calls.
+ * When owner has seeding on (lrt_set_seeding) and a network, every item is
+ * traced with the learned seeder of owner (same hits as without it).
+ * kernel_ms (may be NULL) receives the timed launch. */
+int lrt_render_batch(lrt_handle owner, uint32_t n_items, const lrt_handle* scenes, const float* world_views, const float* projs,
+                     const uint32_t* widths, const uint32_t* heights, int autotune, float* kernel_ms);
+/* G-buffer of one item of the last lrt_render_batch (any output may be NULL). */
+int lrt_batch_download(lrt_handle owner, uint32_t item, float* t, float* normal_oct, uint32_t* prim);
+/* The work-distribution settings lrt_render_batch uses (after autotuning). */
+int lrt_batch_config(lrt_handle owner, uint32_t* chunk, uint32_t* blocks);
+
 /* ---- differentiable rendering (dr::MultiRendererDR) --------------------- */
 /* MultiRendererDR::SetReference(images, worldView, proj),
  * diff_render/MultiRendererDR.h:76-82, MultiRendererDR.cpp:147-185: n_views
@@ -186,6 +204,10 @@ int lrt_count_work_ex(lrt_handle h, const float* world_views, const float* projs
 int lrt_sbs_displaced_icosphere(uint32_t subdiv, float radius, float amplitude, uint32_t octaves, uint32_t seed, uint32_t lod,
                                 uint32_t brick_size, uint32_t brick_pad, uint32_t bytes_per_value, int voxel_diag_scale,
                                 int threads, lrt_sbs_handle* out);
+/* Torus (major R, minor r, in the xy plane) displaced by seeded value noise,
+ * as a sparse DX-layout SBS at brick LOD `lod` (same builder as the icosphere). */
+int lrt_sbs_displaced_torus(float R, float r, float amplitude, uint32_t octaves, uint32_t seed, uint32_t lod, uint32_t brick_size,
+                            uint32_t brick_pad, uint32_t bytes_per_value, int voxel_diag_scale, int threads, lrt_sbs_handle* out);
 /* Dense indexed grid (layout ID32F_IRGB32F[_IN]) of a torus (major R, minor r)
  * with noise displacement amp (shape = 1) or a sphere of radius R (shape = 0). */
 int lrt_sbs_grid(uint32_t shape, float R, float r, float amplitude, uint32_t octaves, uint32_t seed, uint32_t brick_count,

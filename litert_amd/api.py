@@ -168,6 +168,10 @@ def _load():
         "lrt_mlp_infer": (i32, [P, P, u64, P, C.POINTER(f32)]),
         "lrt_mlp_infer_dataset": (i32, [P, u32, C.POINTER(f32), P, u64]),
         "lrt_set_seeding": (i32, [P, i32, f32]),
+        "lrt_render_batch": (i32, [P, u32, P, P, P, P, P, i32, C.POINTER(f32)]),
+        "lrt_batch_download": (i32, [P, u32, P, P, P]),
+        "lrt_batch_config": (i32, [P, C.POINTER(u32), C.POINTER(u32)]),
+        "lrt_sbs_displaced_torus": (i32, [f32, f32, f32, u32, u32, u32, u32, u32, u32, i32, i32, C.POINTER(P)]),
         "lrt_count_work_ex": (i32, [P, P, P, u32, u32, u32, P, u32]),
     })
     for name, (res, args) in sig.items():
@@ -232,6 +236,15 @@ def displaced_icosphere(subdiv=5, radius=0.8, amplitude=0.05, octaves=3, seed=1,
     _check(lib.lrt_sbs_displaced_icosphere(subdiv, radius, amplitude, octaves, seed, lod, brick_size, brick_pad,
                                            bytes_per_value, int(voxel_diag_scale), threads, C.byref(h)),
            "lrt_sbs_displaced_icosphere")
+    return SdfSBS._from_handle(h)
+
+
+def displaced_torus(R=0.6, r=0.2, amplitude=0.02, octaves=3, seed=1, lod=32, brick_size=4, brick_pad=0, bytes_per_value=1,
+                    voxel_diag_scale=False, threads=0):
+    """Noise-displaced torus as a sparse DX-layout SBS (no reference counterpart)."""
+    h = C.c_void_p()
+    _check(lib.lrt_sbs_displaced_torus(R, r, amplitude, octaves, seed, lod, brick_size, brick_pad, bytes_per_value,
+                                       int(voxel_diag_scale), threads, C.byref(h)), "displaced_torus")
     return SdfSBS._from_handle(h)
 
 
@@ -467,6 +480,32 @@ class MultiRenderer:
         ms = C.c_float()
         _check(lib.lrt_mlp_infer_dataset(self._h, repeats, C.byref(ms), _ptr(out) if n_out else None, n_out), "MlpInferDataset")
         return ms.value, out
+
+    # ---- batched uneven workload (no reference counterpart) ----
+    def RenderBatch(self, scenes, world_views, projs, widths, heights, autotune=False):
+        """One launch over len(scenes) items (renderers holding scenes); returns kernel ms."""
+        hs = (C.c_void_p * len(scenes))(*[s._h for s in scenes])
+        wv, pr = _f32(world_views).reshape(-1, 16), _f32(projs).reshape(-1, 16)
+        w = np.ascontiguousarray(widths, np.uint32)
+        h = np.ascontiguousarray(heights, np.uint32)
+        ms = C.c_float()
+        _check(lib.lrt_render_batch(self._h, len(scenes), hs, _ptr(wv), _ptr(pr), _ptr(w), _ptr(h), int(bool(autotune)),
+                                    C.byref(ms)), "RenderBatch")
+        self._batch_dims = list(zip(w.tolist(), h.tolist()))
+        return ms.value
+
+    def BatchDownload(self, item):
+        W, H = self._batch_dims[item]
+        t = np.zeros((H, W), np.float32)
+        n = np.zeros((H, W, 2), np.float32)
+        prim = np.zeros((H, W), np.uint32)
+        _check(lib.lrt_batch_download(self._h, item, _ptr(t), _ptr(n), _ptr(prim)), "BatchDownload")
+        return t, n, prim
+
+    def BatchConfig(self):
+        a, b = C.c_uint32(), C.c_uint32()
+        _check(lib.lrt_batch_config(self._h, C.byref(a), C.byref(b)), "BatchConfig")
+        return a.value, b.value
 
     def SetSeeding(self, enabled, threshold=0.9):
         _check(lib.lrt_set_seeding(self._h, int(bool(enabled)), threshold), "SetSeeding")

@@ -339,7 +339,102 @@ __global__ __launch_bounds__(256) void k_mlp_reduce_adam(float* __restrict__ prm
 // at a leaf into LDS (compacted, so the MFMA work scales with those lanes),
 // runs the network, and each such lane hands the predicted hit distance to
 // the exact brick intersector when the hit logit clears the confidence gate.
-constexpr uint32_t kStackLdsSeeded = 16;
+// One pixel per lane, all 64 lanes of a wave together (the network runs on
+// the wave's compacted leaf lanes, so lanes outside the image take part with
+// valid = false). fl/ol are this wave's LDS feature and output rows.
+template <bool COUNT, bool RGBA, int RF, bool IDX>
+__device__ __forceinline__ void seeded_pixel(const DevScene& S, const TraceParams& P, const float4* __restrict__ bfeat,
+                                             const void* __restrict__ packed, float seed_logit, const Camera& cam, const RenderArgs& A,
+                                             const RenderOut& O, uint32_t view, uint32_t x, uint32_t y, bool valid, Stack& st,
+                                             WorkCount& wc, uint16_t* fl, float* ol, int lane) {
+  float4 acc = make_float4(0, 0, 0, 0);
+  for (uint32_t i = 0; i < A.spp; i++) {
+    v3 pos(0, 0, 0), dir(0, 0, 1);
+    if (valid) {
+      const float2 d = sample_offset(A, x, y, i);
+      eye_ray(cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
+    }
+    const v3 inv = SafeInverse(dir);
+    Hit hit;
+    hit.t = 1e9f;
+    hit.primId = hit.instId = hit.geomId = 0xFFFFFFFFu;
+    hit.coords[0] = 1.0f; hit.coords[1] = 0.0f; hit.coords[2] = 0.0f; hit.coords[3] = 0.0f;
+    bool active = false;
+    if (valid) {
+      const float2 bh = RayBox(pos, inv, v3(-1, -1, -1), v3(1, 1, 1));
+      active = (bh.x <= bh.y) && (bh.y > 0.0f) && (bh.x < hit.t);
+    }
+    int top = 0;
+    uint32_t left = 0;
+    for (;;) {
+      bool at_leaf = false;
+      if (active) {
+        at_leaf = NextLeaf<COUNT>(S, pos, inv, 0.0f, false, hit.t, top, left, st, &wc);
+        active = at_leaf;
+      }
+      const uint64_t leaf_mask = __ballot(at_leaf);
+      if (leaf_mask == 0ull) break;  // wave-uniform
+      const int n_at = __popcll(leaf_mask);
+      const int slot = __popcll(leaf_mask & ((1ull << lane) - 1ull));
+      const uint32_t brick = left & LRT_START_MASK;
+      float t0 = 0.0f, s = 1.0f;
+      if (at_leaf) {
+        float f[24];
+        mlp_features(S, bfeat, pos, dir, inv, brick, f, &t0, &s);
+        store_features_bf16(fl + slot * kFeatStride, f);
+      }
+      wave_lds_sync();
+      mlp_wave_forward(packed, fl, ol, (n_at + 15) >> 4, lane);
+      if (COUNT && lane == 0) wc.tiles += (uint32_t)((n_at + 15) >> 4);
+      wave_lds_sync();
+      if (at_leaf) {
+        const float z0 = ol[3 * slot + 0], z1 = ol[3 * slot + 1], z2 = ol[3 * slot + 2];
+        // confident hit: centre of the predicted interval; confident miss:
+        // beyond every voxel of the brick; otherwise no prediction
+        const bool confident = z2 >= seed_logit;
+        const float seed_world = confident ? t0 + 0.5f * (z0 + z1) / s : (z2 <= -seed_logit ? 3.0e38f : -1.0f);
+        if (COUNT) wc.bricks++, wc.seeds += confident ? 1u : 0u;
+        const float tPrev = hit.t;
+        BrickIntersect<true, RF, IDX>(S, P, pos, dir, inv, 0.1f, brick, &hit, COUNT ? &wc.voxels : nullptr, seed_world,
+                             COUNT ? &wc.iters : nullptr);
+        if (hit.t < tPrev) {
+          active = false;
+        } else {
+          top--;
+          left = st.get(top > 0 ? top : 0);
+        }
+      }
+      wave_lds_sync();  // the next round rewrites this wave's LDS rows
+    }
+    if (hit.primId != 0xFFFFFFFFu) hit.instId = 0;
+    if (valid) {
+      const size_t pix = (size_t)view * A.W * A.H + (size_t)y * A.W + x;
+      if (i == 0) {
+        if (O.t) O.t[pix] = hit.t;
+        if (O.prim) O.prim[pix] = hit.primId;
+        if (O.n) O.n[pix] = make_float2(hit.coords[2], hit.coords[3]);
+      }
+      if (RGBA) {
+        const float4 c = shade(P.render_mode, hit, dir);
+        acc = make_float4(acc.x + c.x, acc.y + c.y, acc.z + c.z, acc.w + c.w);
+      }
+    }
+  }
+  if (RGBA && valid) {
+    const size_t pix = (size_t)view * A.W * A.H + (size_t)y * A.W + x;
+    const float sp = (float)A.spp;
+    O.rgba[pix] = make_float4(acc.x / sp, acc.y / sp, acc.z / sp, acc.w / sp);
+  }
+}
+
+__device__ __forceinline__ void flush_counts(const WorkCount& wc, unsigned long long* counts) {
+  atomicAdd(&counts[0], (unsigned long long)wc.pairs);
+  atomicAdd(&counts[1], (unsigned long long)wc.bricks);
+  atomicAdd(&counts[2], (unsigned long long)wc.voxels);
+  atomicAdd(&counts[3], (unsigned long long)wc.iters);
+  atomicAdd(&counts[4], (unsigned long long)wc.tiles);
+  atomicAdd(&counts[5], (unsigned long long)wc.seeds);
+}
 
 template <bool COUNT, bool RGBA, int RF, bool IDX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_render_seeded(DevScene S, TraceParams P, const float4* __restrict__ bfeat,
@@ -374,93 +469,67 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
       tid_to_xy(tid, A.packed_w, x, y);
       valid = x < A.W && y < A.H;
     }
-    float4 acc = make_float4(0, 0, 0, 0);
-    for (uint32_t i = 0; i < A.spp; i++) {
-      v3 pos(0, 0, 0), dir(0, 0, 1);
-      if (valid) {
-        const float2 d = sample_offset(A, x, y, i);
-        eye_ray(cams[view], x, y, d.x, d.y, A.W, A.H, pos, dir);
-      }
-      const v3 inv = SafeInverse(dir);
-      Hit hit;
-      hit.t = 1e9f;
-      hit.primId = hit.instId = hit.geomId = 0xFFFFFFFFu;
-      hit.coords[0] = 1.0f; hit.coords[1] = 0.0f; hit.coords[2] = 0.0f; hit.coords[3] = 0.0f;
-      bool active = false;
-      if (valid) {
-        const float2 bh = RayBox(pos, inv, v3(-1, -1, -1), v3(1, 1, 1));
-        active = (bh.x <= bh.y) && (bh.y > 0.0f) && (bh.x < hit.t);
-      }
-      int top = 0;
-      uint32_t left = 0;
-      for (;;) {
-        bool at_leaf = false;
-        if (active) {
-          at_leaf = NextLeaf<COUNT>(S, pos, inv, 0.0f, false, hit.t, top, left, st, &wc);
-          active = at_leaf;
-        }
-        const uint64_t leaf_mask = __ballot(at_leaf);
-        if (leaf_mask == 0ull) break;  // wave-uniform
-        const int n_at = __popcll(leaf_mask);
-        const int slot = __popcll(leaf_mask & ((1ull << lane) - 1ull));
-        const uint32_t brick = left & LRT_START_MASK;
-        float t0 = 0.0f, s = 1.0f;
-        if (at_leaf) {
-          float f[24];
-          mlp_features(S, bfeat, pos, dir, inv, brick, f, &t0, &s);
-          store_features_bf16(fl[wave] + slot * kFeatStride, f);
-        }
-        wave_lds_sync();
-        mlp_wave_forward(packed, fl[wave], ol[wave], (n_at + 15) >> 4, lane);
-        if (COUNT && lane == 0) wc.tiles += (uint32_t)((n_at + 15) >> 4);
-        wave_lds_sync();
-        if (at_leaf) {
-          const float z0 = ol[wave][3 * slot + 0], z1 = ol[wave][3 * slot + 1], z2 = ol[wave][3 * slot + 2];
-          // confident hit: centre of the predicted interval; confident miss:
-          // beyond every voxel of the brick; otherwise no prediction
-          const bool confident = z2 >= seed_logit;
-          const float seed_world = confident ? t0 + 0.5f * (z0 + z1) / s : (z2 <= -seed_logit ? 3.0e38f : -1.0f);
-          if (COUNT) wc.bricks++, wc.seeds += confident ? 1u : 0u;
-          const float tPrev = hit.t;
-          BrickIntersect<true, RF, IDX>(S, P, pos, dir, inv, 0.1f, brick, &hit, COUNT ? &wc.voxels : nullptr, seed_world,
-                               COUNT ? &wc.iters : nullptr);
-          if (hit.t < tPrev) {
-            active = false;
-          } else {
-            top--;
-            left = st.get(top > 0 ? top : 0);
+    seeded_pixel<COUNT, RGBA, RF, IDX>(S, P, bfeat, packed, seed_logit, cams[view], A, O, view, x, y, valid, st, wc, fl[wave], ol[wave],
+                                       lane);
+  }
+  if (COUNT) flush_counts(wc, counts);
+}
+
+// Batched seeded render: the item layout and XCD-sliced chunk counters of
+// k_render_batch. Item ranges are multiples of 64 packed pixels and chunks
+// multiples of 256, so each wave lies in one item and either covers 64 real
+// packed pixels or lies wholly past the end.
+template <bool COUNT, int RF, bool IDX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_render_batch_seeded(
+    const BatchItem* __restrict__ items, const uint64_t* __restrict__ offsets, uint32_t n_items, uint64_t total, TraceParams P,
+    const void* __restrict__ packed, float seed_logit, uint32_t chunk, uint32_t* work_counter, uint32_t* spill,
+    unsigned long long* counts) {
+  __shared__ uint32_t lds_stack[kStackLdsSeeded * 256];
+  __shared__ __align__(16) uint16_t fl[4][64 * kFeatStride];
+  __shared__ float ol[4][64 * 3];
+  __shared__ uint32_t chunk_base;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  Stack st;
+  st.lds = (lds_u32*)(lds_stack + threadIdx.x);
+  st.lds_stride = 256;
+  st.lds_entries = kStackLdsSeeded;
+  st.spill = spill + blockIdx.x * 256 + threadIdx.x;
+  st.spill_stride = gridDim.x * 256;
+  WorkCount wc;
+  const uint32_t xcd = blockIdx.x & (kXcds - 1u);
+  const uint64_t n_chunks = (total + chunk - 1) / chunk;
+  const uint64_t per = (n_chunks + kXcds - 1) / kXcds;
+  for (uint32_t k = 0; k < kXcds; k++) {
+    const uint32_t sl = (xcd + k) & (kXcds - 1u);
+    const uint64_t c0 = per * sl, c1 = c0 + per < n_chunks ? c0 + per : n_chunks;
+    for (;;) {
+      if (threadIdx.x == 0) chunk_base = atomicAdd(work_counter + kCounterStride * sl, 1u);
+      __syncthreads();
+      const uint64_t c = c0 + chunk_base;
+      __syncthreads();
+      if (c >= c1) break;  // uniform across the block
+      for (uint32_t j = 0; j < chunk; j += 256) {
+        const uint64_t gtid = c * chunk + j + threadIdx.x;
+        if (gtid < total) {  // wave-uniform: total is a multiple of 64
+          const uint64_t g0 = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(gtid >> 32)) << 32) |
+                              (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)gtid);
+          uint32_t lo = 0, hi = n_items;  // offsets[lo] <= g0 < offsets[hi]
+          while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (offsets[mid] <= g0) lo = mid;
+            else hi = mid;
           }
-        }
-        wave_lds_sync();  // the next round rewrites this wave's LDS rows
-      }
-      if (hit.primId != 0xFFFFFFFFu) hit.instId = 0;
-      if (valid) {
-        const size_t pix = (size_t)view * A.W * A.H + (size_t)y * A.W + x;
-        if (i == 0) {
-          if (O.t) O.t[pix] = hit.t;
-          if (O.prim) O.prim[pix] = hit.primId;
-          if (O.n) O.n[pix] = make_float2(hit.coords[2], hit.coords[3]);
-        }
-        if (RGBA) {
-          const float4 c = shade(P.render_mode, hit, dir);
-          acc = make_float4(acc.x + c.x, acc.y + c.y, acc.z + c.z, acc.w + c.w);
+          const BatchItem& it = items[lo];
+          uint32_t x = 0, y = 0;
+          tid_to_xy((uint32_t)(gtid - offsets[lo]), it.A.packed_w, x, y);
+          const bool valid = x < it.A.W && y < it.A.H;
+          seeded_pixel<COUNT, false, RF, IDX>(it.S, P, it.bfeat, packed, seed_logit, it.cam, it.A, it.O, 0, x, y, valid, st, wc,
+                                              fl[wave], ol[wave], lane);
         }
       }
     }
-    if (RGBA && valid) {
-      const size_t pix = (size_t)view * A.W * A.H + (size_t)y * A.W + x;
-      const float sp = (float)A.spp;
-      O.rgba[pix] = make_float4(acc.x / sp, acc.y / sp, acc.z / sp, acc.w / sp);
-    }
   }
-  if (COUNT) {
-    atomicAdd(&counts[0], (unsigned long long)wc.pairs);
-    atomicAdd(&counts[1], (unsigned long long)wc.bricks);
-    atomicAdd(&counts[2], (unsigned long long)wc.voxels);
-    atomicAdd(&counts[3], (unsigned long long)wc.iters);
-    atomicAdd(&counts[4], (unsigned long long)wc.tiles);
-    atomicAdd(&counts[5], (unsigned long long)wc.seeds);
-  }
+  if (COUNT) flush_counts(wc, counts);
 }
 
 }  // namespace lrtd
@@ -534,6 +603,44 @@ hipError_t launch_render_seeded(const lrtd::DevScene& S, const lrtd::TraceParams
   if (P.sdf_node_intersect == LRT_SDF_NODE_INTERSECT_NEWTON) return L.pick<(int)LRT_SDF_NODE_INTERSECT_NEWTON, false>(rgba);
   if (P.sdf_node_intersect == LRT_SDF_NODE_INTERSECT_ST) return L.pick<(int)LRT_SDF_NODE_INTERSECT_ST, false>(rgba);
   return L.pick<-1, false>(rgba);
+}
+
+
+template <bool C, int RF, bool IDX>
+static hipError_t batch_seeded_go(const lrtd::BatchItem* items, const uint64_t* offsets, uint32_t n_items, uint64_t total,
+                                  const lrtd::TraceParams& P, const void* packed, float logit, uint32_t chunk, uint32_t* counter,
+                                  uint32_t* spill, uint32_t blocks, unsigned long long* counts, hipStream_t stream) {
+  hipLaunchKernelGGL((lrtd::k_render_batch_seeded<C, RF, IDX>), dim3(blocks), dim3(256), 0, stream, items, offsets, n_items, total, P,
+                     packed, logit, chunk, counter, spill, counts);
+  return hipGetLastError();
+}
+
+template <int RF, bool IDX>
+static hipError_t batch_seeded_pick(const lrtd::BatchItem* items, const uint64_t* offsets, uint32_t n_items, uint64_t total,
+                                    const lrtd::TraceParams& P, const void* packed, float logit, uint32_t chunk, uint32_t* counter,
+                                    uint32_t* spill, uint32_t blocks, unsigned long long* counts, hipStream_t stream) {
+  return counts ? batch_seeded_go<true, RF, IDX>(items, offsets, n_items, total, P, packed, logit, chunk, counter, spill, blocks, counts, stream)
+                : batch_seeded_go<false, RF, IDX>(items, offsets, n_items, total, P, packed, logit, chunk, counter, spill, blocks, counts, stream);
+}
+
+hipError_t launch_render_batch_seeded(const lrtd::BatchItem* d_items, const uint64_t* d_offsets, uint32_t n_items, uint64_t total,
+                                      const lrtd::TraceParams& P, bool indexed, const void* d_packed, float seed_logit, uint32_t chunk,
+                                      uint32_t* d_work_counter, uint32_t* d_spill, uint32_t blocks, unsigned long long* d_counts,
+                                      hipStream_t stream) {
+  if (chunk == 0 || chunk % 256 != 0) return hipErrorInvalidValue;
+  hipError_t e = zero_u32(d_work_counter, lrtd::kXcds * lrtd::kCounterStride, stream);
+  if (e != hipSuccess) return e;
+  if (indexed)
+    return batch_seeded_pick<-1, true>(d_items, d_offsets, n_items, total, P, d_packed, seed_logit, chunk, d_work_counter, d_spill, blocks,
+                                       d_counts, stream);
+  if (P.sdf_node_intersect == LRT_SDF_NODE_INTERSECT_NEWTON)
+    return batch_seeded_pick<(int)LRT_SDF_NODE_INTERSECT_NEWTON, false>(d_items, d_offsets, n_items, total, P, d_packed, seed_logit, chunk,
+                                                                        d_work_counter, d_spill, blocks, d_counts, stream);
+  if (P.sdf_node_intersect == LRT_SDF_NODE_INTERSECT_ST)
+    return batch_seeded_pick<(int)LRT_SDF_NODE_INTERSECT_ST, false>(d_items, d_offsets, n_items, total, P, d_packed, seed_logit, chunk,
+                                                                    d_work_counter, d_spill, blocks, d_counts, stream);
+  return batch_seeded_pick<-1, false>(d_items, d_offsets, n_items, total, P, d_packed, seed_logit, chunk, d_work_counter, d_spill, blocks,
+                                      d_counts, stream);
 }
 
 }  // namespace lrt

@@ -60,6 +60,10 @@ struct MlpEventMeta {
   uint32_t brick;  // brick index
 };
 
+// LDS stack entries per lane of the seeded render kernels (the LDS feature
+// and output rows take the rest of the budget).
+constexpr uint32_t kStackLdsSeeded = 16;
+
 }  // namespace lrtd
 
 namespace lrt {
@@ -77,4 +81,8 @@ hipError_t launch_render_seeded(const lrtd::DevScene& S, const lrtd::TraceParams
                                 float seed_logit, const lrtd::Camera* d_cams, const lrtd::RenderArgs& A, const lrtd::RenderOut& O,
                                 uint32_t* d_work_counter, uint32_t* d_spill, uint32_t blocks, bool rgba, unsigned long long* d_counts,
                                 hipStream_t stream);
+hipError_t launch_render_batch_seeded(const lrtd::BatchItem* d_items, const uint64_t* d_offsets, uint32_t n_items, uint64_t total,
+                                      const lrtd::TraceParams& P, bool indexed, const void* d_packed, float seed_logit, uint32_t chunk,
+                                      uint32_t* d_work_counter, uint32_t* d_spill, uint32_t blocks, unsigned long long* d_counts,
+                                      hipStream_t stream);
 }  // namespace lrt

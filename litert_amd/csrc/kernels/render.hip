@@ -105,6 +105,61 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RF >= 0 && 
   }
 }
 
+// Batched render: the concatenated packed pixel ranges of many
+// (scene, camera, image) items are drained by one persistent launch with the
+// same XCD-sliced chunk counters as k_render; each wave looks its item up
+// (wave-uniform, item ranges are 64-aligned) and runs render_pixel on it.
+template <bool COUNT, int RF, bool IDX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RF >= 0 && !IDX ? kSpecWaves : 1, 8))) void k_render_batch(
+    const BatchItem* __restrict__ items, const uint64_t* __restrict__ offsets, uint32_t n_items, uint64_t total, TraceParams P,
+    uint32_t chunk, uint32_t* work_counter, uint32_t* spill, unsigned long long* counts) {
+  __shared__ uint32_t lds_stack[kStackLds * 256];
+  __shared__ uint32_t chunk_base;
+  Stack st;
+  st.lds = (lds_u32*)(lds_stack + threadIdx.x);
+  st.lds_stride = 256;
+  st.lds_entries = kStackLds;
+  st.spill = spill + blockIdx.x * 256 + threadIdx.x;
+  st.spill_stride = gridDim.x * 256;
+  WorkCount wc;
+  const uint32_t xcd = blockIdx.x & (kXcds - 1u);
+  const uint64_t n_chunks = (total + chunk - 1) / chunk;
+  const uint64_t per = (n_chunks + kXcds - 1) / kXcds;
+  for (uint32_t k = 0; k < kXcds; k++) {
+    const uint32_t sl = (xcd + k) & (kXcds - 1u);
+    const uint64_t c0 = per * sl, c1 = c0 + per < n_chunks ? c0 + per : n_chunks;
+    for (;;) {
+      if (threadIdx.x == 0) chunk_base = atomicAdd(work_counter + kCounterStride * sl, 1u);
+      __syncthreads();
+      const uint64_t c = c0 + chunk_base;
+      __syncthreads();
+      if (c >= c1) break;  // uniform across the block
+      for (uint32_t j = 0; j < chunk; j += 256) {
+        const uint64_t gtid = c * chunk + j + threadIdx.x;
+        if (gtid < total) {
+          // item of this wave: largest i with offsets[i] <= first active gtid
+          const uint64_t g0 = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(gtid >> 32)) << 32) |
+                              (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)gtid);
+          uint32_t lo = 0, hi = n_items;  // offsets[lo] <= g0 < offsets[hi]
+          while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (offsets[mid] <= g0) lo = mid;
+            else hi = mid;
+          }
+          const BatchItem& it = items[lo];
+          render_pixel<COUNT, false, RF, IDX>(it.S, P, &it.cam, it.A, it.O, gtid - offsets[lo], st, wc);
+        }
+      }
+    }
+  }
+  if (COUNT) {
+    atomicAdd(&counts[0], (unsigned long long)wc.pairs);
+    atomicAdd(&counts[1], (unsigned long long)wc.bricks);
+    atomicAdd(&counts[2], (unsigned long long)wc.voxels);
+    atomicAdd(&counts[3], (unsigned long long)wc.iters);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_cast_rays(DevScene S, TraceParams P, const float4* __restrict__ rays, uint32_t n,
                                                    CRT_Hit* __restrict__ out, uint32_t* spill) {
   __shared__ uint32_t lds_stack[kStackLds * 256];
@@ -176,6 +231,39 @@ hipError_t launch_render(const lrtd::DevScene& S, const lrtd::TraceParams& P, co
   if (e != hipSuccess) return e;
   RenderLaunch L{S, P, d_cams, A, O, d_work_counter, d_spill, blocks, d_counts, stream};
   return dispatch_variant(S, P, d_counts != nullptr, rgba, L);
+}
+
+template <bool C, int RF, bool IDX>
+static hipError_t batch_go(const lrtd::BatchItem* items, const uint64_t* offsets, uint32_t n_items, uint64_t total, const lrtd::TraceParams& P,
+                           uint32_t chunk, uint32_t* counter, uint32_t* spill, uint32_t blocks, unsigned long long* counts, hipStream_t stream) {
+  hipLaunchKernelGGL((lrtd::k_render_batch<C, RF, IDX>), dim3(blocks), dim3(256), 0, stream, items, offsets, n_items, total, P, chunk, counter,
+                     spill, counts);
+  return hipGetLastError();
+}
+
+template <int RF, bool IDX>
+static hipError_t batch_pick(bool counts, const lrtd::BatchItem* items, const uint64_t* offsets, uint32_t n_items, uint64_t total,
+                             const lrtd::TraceParams& P, uint32_t chunk, uint32_t* counter, uint32_t* spill, uint32_t blocks,
+                             unsigned long long* d_counts, hipStream_t stream) {
+  return counts ? batch_go<true, RF, IDX>(items, offsets, n_items, total, P, chunk, counter, spill, blocks, d_counts, stream)
+                : batch_go<false, RF, IDX>(items, offsets, n_items, total, P, chunk, counter, spill, blocks, d_counts, stream);
+}
+
+hipError_t launch_render_batch(const lrtd::BatchItem* d_items, const uint64_t* d_offsets, uint32_t n_items, uint64_t total,
+                               const lrtd::TraceParams& P, bool indexed, uint32_t chunk, uint32_t* d_work_counter, uint32_t* d_spill,
+                               uint32_t blocks, unsigned long long* d_counts, hipStream_t stream) {
+  if (chunk == 0 || chunk % 256 != 0) return hipErrorInvalidValue;
+  hipError_t e = zero_u32(d_work_counter, lrtd::kXcds * lrtd::kCounterStride, stream);
+  if (e != hipSuccess) return e;
+  const bool c = d_counts != nullptr;
+  if (indexed) return batch_pick<-1, true>(c, d_items, d_offsets, n_items, total, P, chunk, d_work_counter, d_spill, blocks, d_counts, stream);
+  if (P.sdf_node_intersect == LRT_SDF_NODE_INTERSECT_NEWTON)
+    return batch_pick<(int)LRT_SDF_NODE_INTERSECT_NEWTON, false>(c, d_items, d_offsets, n_items, total, P, chunk, d_work_counter, d_spill,
+                                                                  blocks, d_counts, stream);
+  if (P.sdf_node_intersect == LRT_SDF_NODE_INTERSECT_ST)
+    return batch_pick<(int)LRT_SDF_NODE_INTERSECT_ST, false>(c, d_items, d_offsets, n_items, total, P, chunk, d_work_counter, d_spill, blocks,
+                                                              d_counts, stream);
+  return batch_pick<-1, false>(c, d_items, d_offsets, n_items, total, P, chunk, d_work_counter, d_spill, blocks, d_counts, stream);
 }
 
 hipError_t launch_cast_rays(const lrtd::DevScene& S, const lrtd::TraceParams& P, const float4* d_rays, uint32_t n, CRT_Hit* d_out,

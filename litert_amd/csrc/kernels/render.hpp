@@ -29,6 +29,17 @@ struct RenderOut {
   float4* rgba;    // RenderFloat colour
 };
 
+// One scene + camera + image of a batched launch (many scenes,
+// uneven ray counts, one persistent launch). Pixel ranges are concatenated in
+// the packed order; each item's range is a multiple of 64 (8x8 tiles).
+struct BatchItem {
+  DevScene S;
+  Camera cam;
+  RenderArgs A;  // n_views = 1, spp = 1
+  RenderOut O;
+  const float4* bfeat;  // per-brick seeder features (seeded launch only)
+};
+
 }  // namespace lrtd
 
 namespace lrt {
@@ -36,6 +47,11 @@ hipError_t zero_u32(uint32_t* d, uint32_t n, hipStream_t stream);
 hipError_t launch_render(const lrtd::DevScene& S, const lrtd::TraceParams& P, const lrtd::Camera* d_cams, const lrtd::RenderArgs& A,
                          const lrtd::RenderOut& O, uint32_t* d_work_counter, uint32_t* d_spill, uint32_t blocks, bool rgba,
                          unsigned long long* d_counts, hipStream_t stream);
+// offsets: n_items + 1 packed-pixel offsets (offsets[n_items] = total);
+// chunk: pixels per chunk-counter atomic (a multiple of 256).
+hipError_t launch_render_batch(const lrtd::BatchItem* d_items, const uint64_t* d_offsets, uint32_t n_items, uint64_t total,
+                               const lrtd::TraceParams& P, bool indexed, uint32_t chunk, uint32_t* d_work_counter, uint32_t* d_spill,
+                               uint32_t blocks, unsigned long long* d_counts, hipStream_t stream);
 hipError_t launch_cast_rays(const lrtd::DevScene& S, const lrtd::TraceParams& P, const float4* d_rays, uint32_t n, CRT_Hit* d_out,
                             uint32_t* d_spill, uint32_t blocks, hipStream_t stream);
 }  // namespace lrt

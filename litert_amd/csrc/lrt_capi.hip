@@ -191,7 +191,7 @@ int Renderer::render_device(const lrtd::Camera* cams, uint32_t n_views, uint32_t
   A.seed = seed;
   const uint32_t blocks = grid_blocks();
   const bool seeded = seeding && mlp_ready;
-  if ((e = ensure_spill(blocks, seeded ? 16u : lrtd::kStackLds)) != hipSuccess) return hip_err(e, "alloc spill");
+  if ((e = ensure_spill(blocks, seeded ? lrtd::kStackLdsSeeded : lrtd::kStackLds)) != hipSuccess) return hip_err(e, "alloc spill");
   if (seeded && (e = ensure_brick_features()) != hipSuccess) return hip_err(e, "brick features");
   if ((e = hipEventRecord(ev0, stream)) != hipSuccess) return hip_err(e, "event");
   if (seeded) {
@@ -519,6 +519,132 @@ int lrt_download_view(lrt_handle h, uint32_t view, float* t, float* n, uint32_t*
   return LRT_OK;
 }
 
+int lrt_render_batch(lrt_handle owner, uint32_t n_items, const lrt_handle* scenes, const float* wvs, const float* prs,
+                     const uint32_t* widths, const uint32_t* heights, int autotune, float* kernel_ms) {
+  if (!owner || !scenes || !wvs || !prs || !widths || !heights || n_items == 0) return set_err(LRT_E_INVALID, "bad arguments");
+  if (n_items > (1u << 20)) return set_err(LRT_E_INVALID, "too many items");
+  Renderer* r = owner;
+  if (r->preset.interpolation_mode != LRT_INTERPOLATION_MODE_TRILINEAR)
+    return set_err(LRT_E_UNSUPPORTED, "only trilinear interpolation is supported");
+  std::vector<lrtd::BatchItem> items(n_items);
+  std::vector<uint64_t> offsets(n_items + 1, 0);
+  r->b_pix_off.assign(n_items + 1, 0);
+  r->b_w.assign(widths, widths + n_items);
+  r->b_h.assign(heights, heights + n_items);
+  bool indexed = false;
+  int max_depth = 0;
+  // the owner's seeding switch and network apply to every item; each item's
+  // per-brick features come from its own renderer
+  const bool seeded = r->seeding && r->mlp_ready;
+  hipError_t e;
+  for (uint32_t i = 0; i < n_items; i++) {
+    Renderer* s = scenes[i];
+    if (!s || !s->has_scene) return set_err(LRT_E_NOSCENE, "batch item without a scene");
+    if (s->device != r->device) return set_err(LRT_E_INVALID, "batch items must live on the owner's device");
+    const uint32_t W = widths[i], H = heights[i];
+    if (W == 0 || H == 0 || W > 65535 || H > 65535) return set_err(LRT_E_INVALID, "bad image size");
+    lrtd::BatchItem& it = items[i];
+    it.S = s->dev_scene();
+    it.cam = make_camera(wvs + 16 * (size_t)i, prs + 16 * (size_t)i);
+    it.A.W = W;
+    it.A.H = H;
+    it.A.packed_w = 8 * ((W + 7) / 8);
+    it.A.n_packed = it.A.packed_w * (8 * ((H + 7) / 8));
+    it.A.n_views = 1;
+    it.A.spp = 1;
+    it.A.ray_gen_mode = r->preset.ray_gen_mode;
+    it.A.seed = r->seed;
+    it.bfeat = nullptr;
+    if (seeded) {
+      if ((e = hipSetDevice(s->device)) != hipSuccess) return hip_err(e, "hipSetDevice");
+      if ((e = s->ensure_brick_features()) != hipSuccess) return hip_err(e, "brick features");
+      if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) return hip_err(e, "brick features");
+      it.bfeat = (const float4*)s->d_bfeat.p;
+    }
+    offsets[i + 1] = offsets[i] + it.A.n_packed;  // multiples of 64
+    r->b_pix_off[i + 1] = r->b_pix_off[i] + (uint64_t)W * H;
+    indexed = indexed || it.S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F || it.S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F_IN;
+    max_depth = std::max(max_depth, s->pair_depth);
+  }
+  const uint64_t total = offsets[n_items], npix = r->b_pix_off[n_items];
+  if ((e = hipSetDevice(r->device)) != hipSuccess) return hip_err(e, "hipSetDevice");
+  if ((e = r->d_b_t.ensure(npix * 4)) != hipSuccess) return hip_err(e, "alloc t");
+  if ((e = r->d_b_n.ensure(npix * 8)) != hipSuccess) return hip_err(e, "alloc n");
+  if ((e = r->d_b_prim.ensure(npix * 4)) != hipSuccess) return hip_err(e, "alloc prim");
+  for (uint32_t i = 0; i < n_items; i++) {
+    items[i].O.t = (float*)r->d_b_t.p + r->b_pix_off[i];
+    items[i].O.n = (float2*)r->d_b_n.p + r->b_pix_off[i];
+    items[i].O.prim = (uint32_t*)r->d_b_prim.p + r->b_pix_off[i];
+    items[i].O.rgba = nullptr;
+  }
+  if ((e = r->d_b_items.upload(items.data(), items.size() * sizeof(lrtd::BatchItem), r->stream)) != hipSuccess) return hip_err(e, "upload items");
+  if ((e = r->d_b_offsets.upload(offsets.data(), offsets.size() * 8, r->stream)) != hipSuccess) return hip_err(e, "upload offsets");
+  if ((e = r->d_counter.ensure(lrtd::kXcds * lrtd::kCounterStride * 4)) != hipSuccess) return hip_err(e, "alloc counter");
+  const lrtd::TraceParams P = r->trace_params(true);
+  const float logit = std::log(r->seed_threshold / (1.0f - r->seed_threshold));
+  auto run = [&](uint32_t chunk, uint32_t blocks, float* ms) -> int {
+    const int saved_depth = r->pair_depth;
+    r->pair_depth = max_depth;  // the spill area must cover the deepest item tree
+    hipError_t e2 = r->ensure_spill(blocks, seeded ? lrtd::kStackLdsSeeded : lrtd::kStackLds);
+    r->pair_depth = saved_depth;
+    if (e2 != hipSuccess) return hip_err(e2, "alloc spill");
+    if ((e2 = hipEventRecord(r->ev0, r->stream)) != hipSuccess) return hip_err(e2, "event");
+    const lrtd::BatchItem* d_items = (const lrtd::BatchItem*)r->d_b_items.p;
+    const uint64_t* d_offsets = (const uint64_t*)r->d_b_offsets.p;
+    e2 = seeded ? launch_render_batch_seeded(d_items, d_offsets, n_items, total, P, indexed, r->d_mlp_packed.p, logit, chunk,
+                                             (uint32_t*)r->d_counter.p, (uint32_t*)r->d_spill.p, blocks, nullptr, r->stream)
+                : launch_render_batch(d_items, d_offsets, n_items, total, P, indexed, chunk, (uint32_t*)r->d_counter.p,
+                                      (uint32_t*)r->d_spill.p, blocks, nullptr, r->stream);
+    if (e2 != hipSuccess) return hip_err(e2, "batch launch");
+    if ((e2 = hipEventRecord(r->ev1, r->stream)) != hipSuccess) return hip_err(e2, "event");
+    if ((e2 = hipEventSynchronize(r->ev1)) != hipSuccess) return hip_err(e2, "batch render");
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, r->ev0, r->ev1);
+    if (ms) *ms = t;
+    return LRT_OK;
+  };
+  const uint32_t cu = (uint32_t)std::max(1, r->cu_count);
+  int c;
+  if (autotune) {
+    // the tuner: every (chunk, grid) pair of a small candidate set, fastest wins
+    const uint32_t chunks[3] = {256, 512, 1024}, grids[3] = {cu * 4, cu * 8, cu * 16};
+    float best = 1e30f;
+    if ((c = run(256, cu * 8, nullptr))) return c;  // warm-up
+    for (uint32_t ch : chunks)
+      for (uint32_t g : grids) {
+        float ms = 0.f;
+        if ((c = run(ch, g, &ms))) return c;
+        if (ms < best) { best = ms; r->b_chunk = ch; r->b_blocks = g; }
+      }
+  }
+  const uint32_t blocks = r->b_blocks ? r->b_blocks : cu * 8;
+  float ms = 0.f;
+  if ((c = run(r->b_chunk, blocks, &ms))) return c;
+  r->times["RenderBatch"] = ms;
+  if (kernel_ms) *kernel_ms = ms;
+  return LRT_OK;
+}
+
+int lrt_batch_download(lrt_handle owner, uint32_t item, float* t, float* normal_oct, uint32_t* prim) {
+  if (!owner) return set_err(LRT_E_INVALID, "null handle");
+  Renderer* r = owner;
+  if (r->b_w.empty() || item >= r->b_w.size()) return set_err(LRT_E_INVALID, "no such batch item");
+  const uint64_t off = r->b_pix_off[item], n = (uint64_t)r->b_w[item] * r->b_h[item];
+  hipError_t e;
+  if ((e = hipSetDevice(r->device)) != hipSuccess) return hip_err(e, "hipSetDevice");
+  if (t && (e = hipMemcpy(t, (float*)r->d_b_t.p + off, n * 4, hipMemcpyDeviceToHost)) != hipSuccess) return hip_err(e, "download");
+  if (normal_oct && (e = hipMemcpy(normal_oct, (float2*)r->d_b_n.p + off, n * 8, hipMemcpyDeviceToHost)) != hipSuccess) return hip_err(e, "download");
+  if (prim && (e = hipMemcpy(prim, (uint32_t*)r->d_b_prim.p + off, n * 4, hipMemcpyDeviceToHost)) != hipSuccess) return hip_err(e, "download");
+  return LRT_OK;
+}
+
+int lrt_batch_config(lrt_handle owner, uint32_t* chunk, uint32_t* blocks) {
+  if (!owner) return set_err(LRT_E_INVALID, "null handle");
+  if (chunk) *chunk = owner->b_chunk;
+  if (blocks) *blocks = owner->b_blocks ? owner->b_blocks : (uint32_t)std::max(1, owner->cu_count) * 8;
+  return LRT_OK;
+}
+
 int lrt_sync(lrt_handle h) {
   if (!h) return set_err(LRT_E_INVALID, "null handle");
   hipError_t e = hipStreamSynchronize(h->stream);
@@ -542,6 +668,25 @@ int lrt_sbs_displaced_icosphere(uint32_t subdiv, float radius, float amplitude, 
     MeshSdf sdf(m);
     std::unique_ptr<lrt_sbs> s(new lrt_sbs);
     s->s = build_sbs([&](f3 p) { return sdf(p); }, lod, brick_size, brick_pad, bytes_per_value, voxel_diag_scale != 0, threads);
+    if (s->s.nodes.empty()) return set_err(LRT_E_INVALID, "scene produced no bricks");
+    *out = s.release();
+  } catch (const std::exception& ex) {
+    return set_err(LRT_E_INVALID, ex.what());
+  }
+  return LRT_OK;
+}
+
+int lrt_sbs_displaced_torus(float R, float r, float amplitude, uint32_t octaves, uint32_t seed, uint32_t lod, uint32_t brick_size,
+                            uint32_t brick_pad, uint32_t bytes_per_value, int voxel_diag_scale, int threads, lrt_sbs_handle* out) {
+  if (!out || !(R > 0.0f) || !(r > 0.0f) || R + r >= 1.0f) return set_err(LRT_E_INVALID, "bad arguments");
+  try {
+    auto sdf = [=](f3 p) {  // torus in the xy plane, displaced by value noise like lrt_sbs_grid's
+      float qx = std::sqrt(p.x * p.x + p.y * p.y) - R;
+      float d = std::sqrt(qx * qx + p.z * p.z) - r;
+      return amplitude != 0.f ? d - amplitude * value_noise_fbm(p, seed, (int)octaves, 2.0f) : d;
+    };
+    std::unique_ptr<lrt_sbs> s(new lrt_sbs);
+    s->s = build_sbs(sdf, lod, brick_size, brick_pad, bytes_per_value, voxel_diag_scale != 0, threads);
     if (s->s.nodes.empty()) return set_err(LRT_E_INVALID, "scene produced no bricks");
     *out = s.release();
   } catch (const std::exception& ex) {

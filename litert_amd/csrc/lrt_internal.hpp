@@ -59,6 +59,12 @@ struct Renderer {
   DevBuf d_dr_ref, d_dr_img, d_dr_thr, d_dr_grad, d_dr_list, d_dr_misc, d_adam_v, d_adam_s;
   float dr_last_loss = 0.f;
   uint32_t dr_last_border = 0, dr_last_border_hits = 0;
+  // batched render (lrt_render_batch)
+  DevBuf d_b_items, d_b_offsets, d_b_t, d_b_n, d_b_prim;
+  std::vector<uint64_t> b_pix_off;            // first pixel of each item in the batch outputs
+  std::vector<uint32_t> b_w, b_h;
+  uint32_t b_chunk = 256, b_blocks = 0;        // 0 = device default grid
+
   // learned root-find seeder
   bool mlp_ready = false;            // parameters set and packed
   std::vector<float> mlp_params;     // FP32 master copy
@@ -89,7 +95,8 @@ struct Renderer {
                      &d_rays, &d_hits, &d_out_t, &d_out_n, &d_out_prim, &d_out_rgba, &d_dr_ref, &d_dr_img,
                      &d_dr_thr, &d_dr_grad, &d_dr_list, &d_dr_misc, &d_adam_v, &d_adam_s, &d_mlp_params,
                      &d_mlp_m, &d_mlp_v, &d_mlp_packed, &d_mlp_partial, &d_mlp_ploss, &d_mlp_grad, &d_mlp_loss,
-                     &d_bfeat, &d_ds_feat, &d_ds_tgt, &d_ds_meta, &d_ds_out};
+                     &d_bfeat, &d_ds_feat, &d_ds_tgt, &d_ds_meta, &d_ds_out, &d_b_items, &d_b_offsets, &d_b_t,
+                     &d_b_n, &d_b_prim};
     for (DevBuf* b : all) b->release();
   }
 };

@@ -10,15 +10,15 @@ MISS = 0xFFFFFFFF
 
 
 @functools.lru_cache(maxsize=None)
-def c1_scene(seed=1):
-    """Config C1: icosphere(5) r 0.8, 3-octave noise amp 0.05, 128^3 effective
+def ico128_scene(seed=1):
+    """Displaced icosphere(5) r 0.8, 3-octave noise amp 0.05, 128^3 effective
     (brick LOD 32 x 4^3 voxels), pad 1, 32-bit values."""
     return L.displaced_icosphere(5, 0.8, 0.05, 3, seed=seed, lod=32, brick_size=4, brick_pad=1, bytes_per_value=4)
 
 
 @functools.lru_cache(maxsize=None)
-def c2_scene(seed=2):
-    """Config C2: 4 octaves, 512^3 effective (LOD 128 x 4^3), pad 0, 8-bit
+def ico512_scene(seed=2):
+    """Displaced icosphere(5), 4 octaves, 512^3 effective (LOD 128 x 4^3), pad 0, 8-bit
     values quantised over the voxel diagonal."""
     return L.displaced_icosphere(5, 0.8, 0.05, 4, seed=seed, lod=128, brick_size=4, brick_pad=0, bytes_per_value=1,
                                  voxel_diag_scale=True)
@@ -26,7 +26,7 @@ def c2_scene(seed=2):
 
 @functools.lru_cache(maxsize=None)
 def oracle_scene(which):
-    sbs = c1_scene() if which == "c1" else c2_scene()
+    sbs = ico128_scene() if which == "ico128" else ico512_scene()
     pairs, _ = L.build_bvh(sbs)
     return oracle.Scene(sbs, pairs)
 

@@ -94,3 +94,14 @@ def test_invalid_arguments_are_rejected():
     assert L.lib.lrt_sbs_grid(1, 0.6, 0.2, 0.0, 1, 1, 0, 4, 0, C.byref(h)) != 0
     assert b"bad" in L.lib.lrt_last_error()
     assert L.lib.lrt_sbs_load_file(b"/nonexistent/file.bin", C.byref(h)) != 0
+
+
+def test_displaced_torus_builder():
+    s = L.displaced_torus(0.6, 0.2, 0.02, 2, seed=3, lod=16, brick_size=4, brick_pad=0, bytes_per_value=1)
+    assert s.header.aux_data == L.SDF_SBS_NODE_LAYOUT_DX and s.nodes.shape[0] > 50
+    # brick centres lie near the torus surface: |dist to the tube centre circle - r| within the keep band
+    pos = np.stack([s.nodes[:, 0] >> 16, s.nodes[:, 0] & 0xFFFF, s.nodes[:, 1] >> 16], 1)
+    c = -1 + (pos + 0.5) * (2.0 / 16)
+    q = np.sqrt(np.hypot(c[:, 0], c[:, 1]) ** 2) - 0.6
+    d = np.abs(np.hypot(q, c[:, 2]) - 0.2)
+    assert d.max() < 0.02 + 2 * np.sqrt(3) / 16 + np.sqrt(3) / 16

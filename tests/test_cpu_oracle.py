@@ -11,7 +11,7 @@ import pytest
 
 import litert_amd.api as L
 import oracle
-from helpers import MISS, c1_scene, decode_oct, preset
+from helpers import MISS, ico128_scene, decode_oct, preset
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 O = oracle.lib()
@@ -260,7 +260,7 @@ def test_oracle_bvh_matches_brute_force():
 
 # ---- BVH builder invariants (format of cbvh.h:21-33, raytrace_common.h:48-57)
 def test_bvh_fat_pairs_cover_every_brick_once():
-    sbs = c1_scene()
+    sbs = ico128_scene()
     pairs, depth = L.build_bvh(sbs)
     f = pairs.view(np.float32)
     leaves = []

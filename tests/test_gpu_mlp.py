@@ -15,7 +15,7 @@ import pytest
 
 import litert_amd.api as L
 import oracle
-from helpers import MISS, c1_scene, compare_gbuffer, oracle_scene, preset
+from helpers import MISS, ico128_scene, compare_gbuffer, oracle_scene, preset
 
 pytestmark = pytest.mark.gpu
 
@@ -25,7 +25,7 @@ ST, NEWTON = L.SDF_OCTREE_NODE_INTERSECT_ST, L.SDF_OCTREE_NODE_INTERSECT_NEWTON
 @pytest.fixture(scope="module")
 def r_c1():
     r = L.CreateMultiRenderer(L.DEVICE_GPU)
-    r.SetScene(c1_scene())
+    r.SetScene(ico128_scene())
     r.SetPreset(preset(NEWTON))
     yield r
     r.close()
@@ -42,7 +42,7 @@ def _collect(r, view_seed, W, max_events=2_000_000):
 def test_collected_features_match_oracle(r_c1):
     wv, pr, f, t, m, kept, seen = _collect(r_c1, 21, 64)
     assert kept == seen == f.shape[0] > 500
-    sc = oracle_scene("c1")
+    sc = oracle_scene("ico128")
     rng = np.random.default_rng(0)
     pick = rng.choice(f.shape[0], 300, replace=False)
     rays = np.zeros((len(pick), 6), np.float32)
@@ -64,7 +64,7 @@ def test_collected_targets_match_render(r_c1):
     tg, _, prim = r_c1.RenderGBuffer(96, 96)
     counts = r_c1.CountWorkEx(wv[None], pr[None], 96, 96)
     assert seen == counts[1]  # one event per brick entry
-    sc = oracle_scene("c1")
+    sc = oracle_scene("ico128")
     hit_rays = set()
     last_event = {}
     for e in range(f.shape[0]):
@@ -84,7 +84,7 @@ def test_collected_targets_match_render(r_c1):
         rr = oracle.eye_ray(wv, pr, ray % 96, ray // 96, 0.5, 0.5, 96, 96, 32)
         rays[k, :3], rays[k, 3:] = rr[:3], rr[4:7]
     _, t0 = sc.mlp_features(rays, m[sel, 1])
-    s = 0.5 * (c1_scene().nodes[m[sel, 1], 1] & 0xFFFF).astype(np.float64)
+    s = 0.5 * (ico128_scene().nodes[m[sel, 1], 1] & 0xFFFF).astype(np.float64)
     depth = t0 + t[sel, 0] / s
     ref = tg.ravel()[m[sel, 0]]
     assert np.max(np.abs(depth - ref) / ref) <= 1e-6
@@ -149,7 +149,7 @@ def _adversarial_params(seed):
 
 @pytest.mark.parametrize("mode", [ST, NEWTON])
 def test_seeded_render_exact_with_adversarial_seeds(r_c1, mode):
-    sc = oracle_scene("c1")
+    sc = oracle_scene("ico128")
     r_c1.MlpSetParams(_adversarial_params(7))
     r_c1.SetSeeding(True, 0.9)
     try:
@@ -173,7 +173,7 @@ def _train_on_other_scenes(r, n_scenes=4, W=192, epochs=12):
         r.SetScene(s)
         cams = [L.orbit_camera(300 + k, i) for i in range(4)]
         r.MlpCollect(np.stack([c[0] for c in cams]), np.stack([c[1] for c in cams]), W, W, 4_000_000)
-    r.SetScene(c1_scene())
+    r.SetScene(ico128_scene())
     r.MlpInit(11)
     losses, ms = r.MlpTrain(epochs, batch=4096, lr=2e-3, seed=3)
     return losses, ms
@@ -189,7 +189,7 @@ def test_trained_seeder_keeps_tolerances_and_saves_iterations(r_c1):
     acc = np.mean((out[:, 2] > 0) == (t[:, 1] > 0.5))
     print("hit accuracy", acc)
     assert acc > 0.85
-    sc = oracle_scene("c1")
+    sc = oracle_scene("ico128")
     for mode in (ST, NEWTON):
         wv, pr = L.orbit_camera(7, 2)
         r_c1.SetPreset(preset(mode))

@@ -11,7 +11,7 @@ import pytest
 
 import litert_amd.api as L
 import oracle
-from helpers import MISS, angle, c1_scene, c2_scene, compare_gbuffer as _compare, oracle_scene, preset
+from helpers import MISS, angle, ico128_scene, ico512_scene, compare_gbuffer as _compare, oracle_scene, preset
 
 pytestmark = pytest.mark.gpu
 
@@ -22,7 +22,7 @@ MODES = [L.SDF_OCTREE_NODE_INTERSECT_ST, L.SDF_OCTREE_NODE_INTERSECT_ANALYTIC, L
 @pytest.fixture(scope="module")
 def r_c1():
     r = L.CreateMultiRenderer(L.DEVICE_GPU)
-    r.SetScene(c1_scene())
+    r.SetScene(ico128_scene())
     yield r
     r.close()
 
@@ -30,7 +30,7 @@ def r_c1():
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("view", [0, 1, 2])
 def test_c1_gbuffer_parity(r_c1, mode, view):
-    sc = oracle_scene("c1")
+    sc = oracle_scene("ico128")
     wv, pr = L.orbit_camera(7, view)
     res = _compare(r_c1, sc, preset(mode), wv, pr, 256, 256, voxels_per_unit=32 * 4 / 2.0)
     print(mode, view, res)
@@ -44,10 +44,10 @@ def test_c1_gbuffer_parity(r_c1, mode, view):
 
 
 def test_c2_gbuffer_parity():
-    sbs = c2_scene()
+    sbs = ico512_scene()
     r = L.CreateMultiRenderer(L.DEVICE_GPU)
     r.SetScene(sbs)
-    sc = oracle_scene("c2")
+    sc = oracle_scene("ico512")
     wv, pr = L.orbit_camera(11, 0)
     res = _compare(r, sc, preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON), wv, pr, 512, 512, voxels_per_unit=128 * 4 / 2.0)
     print(res)
@@ -63,7 +63,7 @@ def test_c2_gbuffer_parity():
                                    L.MULTI_RENDER_MODE_PRIMITIVE, L.MULTI_RENDER_MODE_ST_ITERATIONS])
 def test_render_float_modes(r_c1, rmode):
     """RenderFloat colour per render mode vs the FP32 oracle (eye_ray.cpp:162-489)."""
-    sc = oracle_scene("c1")
+    sc = oracle_scene("ico128")
     wv, pr = L.orbit_camera(3, 1)
     p = preset(L.SDF_OCTREE_NODE_INTERSECT_ST, render_mode=rmode)
     img = r_c1.RenderFloat(128, 128, wv, pr, p)
@@ -85,7 +85,7 @@ def test_render_rgba8_and_timer(r_c1):
 
 def test_spp_random_rays(r_c1):
     """spp 4 with PCG3D jitter (RAY_GEN_MODE_RANDOM) matches the oracle."""
-    sc = oracle_scene("c1")
+    sc = oracle_scene("ico128")
     wv, pr = L.orbit_camera(5, 0)
     p = preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON, render_mode=L.MULTI_RENDER_MODE_LINEAR_DEPTH, spp=4,
                ray_gen=L.RAY_GEN_MODE_RANDOM)
@@ -97,7 +97,7 @@ def test_spp_random_rays(r_c1):
 
 def test_cast_rays_random(r_c1):
     """Batched RayQuery_NearestHit on rays from outside, inside and on axis."""
-    sc = oracle_scene("c1")
+    sc = oracle_scene("ico128")
     rng = np.random.default_rng(0)
     n = 4096
     o = rng.uniform(-1.5, 1.5, (n, 3))
@@ -117,7 +117,7 @@ def test_cast_rays_random(r_c1):
 
 def test_work_counters_match_oracle(r_c1):
     """GPU per-ray work counters equal the FP32 oracle's (same traversal order)."""
-    sc = oracle_scene("c1")
+    sc = oracle_scene("ico128")
     wv, pr = L.orbit_camera(7, 0)
     r_c1.SetPreset(preset())
     g = r_c1.CountWork(wv[None], pr[None], 256, 256)
@@ -150,7 +150,7 @@ def test_bad_inputs_raise():
     r = L.CreateMultiRenderer(L.DEVICE_GPU)
     with pytest.raises(L.LrtError):
         r.RenderFloat(16, 16)  # no scene
-    sbs = c1_scene()
+    sbs = ico128_scene()
     bad = L.SdfSBS(sbs.header, sbs.nodes.copy(), sbs.values)
     bad.nodes[0, 2] = sbs.values.size  # data_offset out of range
     with pytest.raises(L.LrtError):
