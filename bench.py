@@ -14,7 +14,8 @@ by the kernel's average duration (HIP events on the launch stream).
 `cpu_baseline` times the CPU oracle (the reference CPU path restated in C++,
 FP32, OpenMP over the host cores) on the same workload. The DR leg
 (`dr_fwd_bwd_mrays_s`) times RenderDR + Adam on the torus-from-sphere problem (torus target,
-sphere start, 128^3 indexed SBS, 16 views at 512x512, border_spp 256).
+sphere start, 128^3 indexed SBS, 16 views at 512x512, border_spp 256, 200
+Adam steps at lr 1e-3).
 """
 import argparse
 import json
@@ -307,7 +308,7 @@ def main():
         dpr = np.stack([c[1] for c in dcams])
         g.SetReference(refs, dwv, dpr)
         g.RenderDR(dr, 0, seed=1)  # warm-up
-        dr.opt_iterations = 16
+        dr.opt_iterations = 200  # one view per Adam step (MultiRendererDR.h:40)
         dr.image_batch_size = 1
         work = L.SdfSBS(ini.header, ini.nodes, ini.values, ini.values_f.copy())
         losses, total_ms = g.OptimizeFixedStructure(dr, work, seed=1)

@@ -52,8 +52,9 @@ __device__ float samples_to_threshold(const float* s, unsigned count, float dv, 
   return 0.0f;
 }
 
+template <int SPEC>
 __device__ __forceinline__ void scatter_pd(const DRArgs& A, const DRPayload& py, v3 dLdC) {
-  const uint32_t f = A.Q.ray_flags;
+  const uint32_t f = dr_flags<SPEC>(DRParams{A.Q.ray_flags, A.Q.dtdv_fd});
   if ((f & LRT_DR_RAY_FLAG_DDIFFUSE_DCOLOR) && A.mode != LRT_DR_RENDER_MODE_MASK) {
 #pragma unroll
     for (int k = 0; k < 8; k++) {
@@ -81,8 +82,10 @@ __device__ __forceinline__ void scatter_pd(const DRArgs& A, const DRPayload& py,
 // Phase I for one pixel (MultiRendererDR.cpp:1084-1294). With spp > 1 the
 // colour is needed before any sample's gradient can be weighted, so the
 // samples are traced twice (first for colour, then with gradients); the
-// second pass recomputes identical rays and hits.
-__device__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, Stack& st, double& loss_acc) {
+// second pass recomputes identical rays and hits. SPEC kDrDepth is used with
+// spp 1 only (one pass, gradient flags on).
+template <int SPEC>
+__device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, Stack& st, double& loss_acc) {
   uint32_t x, y;
   tid_to_xy(packed_idx, A.packed_w, x, y);
   if (x >= A.W || y >= A.H) return;
@@ -91,14 +94,14 @@ __device__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, Stack& st, double
   v3 res(0, 0, 0);
   int hit_count = 0;
   float samples[16];
-  const uint32_t spp = A.spp;
+  const uint32_t spp = SPEC == kDrDepth ? 1u : A.spp;
   const DRParams Qc = {spp == 1 ? A.Q.ray_flags : 0u, A.Q.dtdv_fd};
   const DRParams Qg = {A.Q.ray_flags, A.Q.dtdv_fd};
   for (uint32_t s = 0; s < spp; s++) {
     float2 d = rand2(x, y, s, A.seed);
     v3 pos, dir;
     eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
-    DRHit hit = dr_nearest_hit(A.S, Qc, pos, dir, 1e9f, &py, st);
+    DRHit hit = dr_nearest_hit<SPEC>(A.S, Qc, pos, dir, 1e9f, &py, st);
     if (hit.primId == 0xFFFFFFFFu) continue;
     v3 c = dr_color(A.mode, hit);
     res = res + v3(c.x / (float)spp, c.y / (float)spp, c.z / (float)spp);
@@ -120,18 +123,19 @@ __device__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, Stack& st, double
                 loss_grad1(A.loss_fn, cm[2], rr[2]) / (float)spp);
   if (A.Q.ray_flags == 0) return;
   if (spp == 1) {
-    if (hit_count) scatter_pd(A, py, dLdC);
+    if (hit_count) scatter_pd<SPEC>(A, py, dLdC);
     return;
   }
   for (uint32_t s = 0; s < spp; s++) {
     float2 d = rand2(x, y, s, A.seed);
     v3 pos, dir;
     eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
-    DRHit hit = dr_nearest_hit(A.S, Qg, pos, dir, 1e9f, &py, st);
-    if (hit.primId != 0xFFFFFFFFu) scatter_pd(A, py, dLdC);
+    DRHit hit = dr_nearest_hit<SPEC>(A.S, Qg, pos, dir, 1e9f, &py, st);
+    if (hit.primId != 0xFFFFFFFFu) scatter_pd<SPEC>(A, py, dLdC);
   }
 }
 
+template <int SPEC>
 __global__ __launch_bounds__(256) void k_dr_primary(DRArgs A, uint32_t* work_counter, uint32_t* spill, double* loss) {
   __shared__ uint32_t lds_stack[kStackLds * 256];
   __shared__ uint32_t chunk_base;
@@ -150,7 +154,7 @@ __global__ __launch_bounds__(256) void k_dr_primary(DRArgs A, uint32_t* work_cou
     __syncthreads();
     if (base >= total) break;
     const uint64_t i = base + threadIdx.x;
-    if (i < total) dr_pixel(A, (uint32_t)i, st, acc);
+    if (i < total) dr_pixel<SPEC>(A, (uint32_t)i, st, acc);
   }
   if (acc != 0.0) atomicAdd(loss, acc);
 }
@@ -177,6 +181,7 @@ __global__ void k_dr_border_find(DRArgs A, uint32_t* list, uint32_t* count) {
 }
 
 // Phase III (MultiRendererDR.cpp:1319-1499): one lane per (border pixel, sample)
+template <int SPEC>
 __global__ __launch_bounds__(256) void k_dr_border_rays(DRArgs A, const uint32_t* list, const uint32_t* count, uint32_t* spill,
                                                         uint32_t* hits) {
   __shared__ uint32_t lds_stack[kStackLds * 256];
@@ -197,7 +202,7 @@ __global__ __launch_bounds__(256) void k_dr_border_rays(DRArgs A, const uint32_t
     float2 d = rand2(x, y, s, A.seed);
     v3 pos, dir;
     eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
-    DRHit hit = dr_nearest_hit(A.S, Qb, pos, dir, 1e9f, &py, st);
+    DRHit hit = dr_nearest_hit<SPEC>(A.S, Qb, pos, dir, 1e9f, &py, st);
     if (!(py.missed_hit.sdf < A.relax_eps)) continue;
     local_hits++;
     float delta[4];
@@ -260,9 +265,17 @@ __global__ void k_adam(const float* __restrict__ g, float* X, float* V, float* S
 
 namespace lrt {
 
+// kDrDepth: linear-depth reconstruction (distance gradients only), analytic
+// dt/dv, one sample per pixel, a render mode whose colour is not the diffuse
+// colour. Everything else runs the generic form.
 hipError_t launch_dr_primary(const lrtd::DRArgs& A, uint32_t* d_counter, uint32_t* d_spill, double* d_loss, uint32_t blocks,
                              hipStream_t s) {
-  hipLaunchKernelGGL(lrtd::k_dr_primary, dim3(blocks), dim3(256), 0, s, A, d_counter, d_spill, d_loss);
+  const bool depth = A.Q.ray_flags == LRT_DR_RAY_FLAG_DDIST_DPOS && A.Q.dtdv_fd == 0 && A.spp == 1 &&
+                     A.mode != LRT_DR_RENDER_MODE_DIFFUSE;
+  if (depth)
+    hipLaunchKernelGGL(lrtd::k_dr_primary<lrtd::kDrDepth>, dim3(blocks), dim3(256), 0, s, A, d_counter, d_spill, d_loss);
+  else
+    hipLaunchKernelGGL(lrtd::k_dr_primary<lrtd::kDrGeneric>, dim3(blocks), dim3(256), 0, s, A, d_counter, d_spill, d_loss);
   return hipGetLastError();
 }
 hipError_t launch_dr_border_find(const lrtd::DRArgs& A, uint32_t* d_list, uint32_t* d_count, hipStream_t s) {
@@ -272,7 +285,11 @@ hipError_t launch_dr_border_find(const lrtd::DRArgs& A, uint32_t* d_list, uint32
 }
 hipError_t launch_dr_border_rays(const lrtd::DRArgs& A, const uint32_t* d_list, const uint32_t* d_count, uint32_t* d_spill,
                                  uint32_t* d_hits, uint32_t blocks, hipStream_t s) {
-  hipLaunchKernelGGL(lrtd::k_dr_border_rays, dim3(blocks), dim3(256), 0, s, A, d_list, d_count, d_spill, d_hits);
+  // linear-depth input reads only depths and hit flags of border rays
+  if (A.input_type == LRT_DR_INPUT_TYPE_LINEAR_DEPTH)
+    hipLaunchKernelGGL(lrtd::k_dr_border_rays<lrtd::kDrDepthBorder>, dim3(blocks), dim3(256), 0, s, A, d_list, d_count, d_spill, d_hits);
+  else
+    hipLaunchKernelGGL(lrtd::k_dr_border_rays<lrtd::kDrGeneric>, dim3(blocks), dim3(256), 0, s, A, d_list, d_count, d_spill, d_hits);
   return hipGetLastError();
 }
 hipError_t launch_dr_scale(float* d_grad, uint32_t n, float mult, hipStream_t s) {

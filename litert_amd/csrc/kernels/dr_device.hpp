@@ -37,6 +37,19 @@ struct DRParams {
 };
 static_assert(sizeof(DRParams) == 8, "DRArgs::Q layout");
 
+// Compile-time forms of the DR ray casts. kDrGeneric reads the ray flags and
+// the dt/dv mode at run time and computes every payload part. kDrDepth is the
+// primary pass of linear-depth reconstruction (flags = DDIST_DPOS, analytic
+// dt/dv, colour unused); kDrDepthBorder the border pass of linear-depth input
+// (flags = BORDER, colour unused). The specialised forms evaluate the same
+// expressions for what they keep and compile the rest of the payload out.
+enum : int { kDrGeneric = 0, kDrDepth = 1, kDrDepthBorder = 2 };
+
+template <int SPEC>
+__device__ __forceinline__ uint32_t dr_flags(const DRParams& Q) {
+  return SPEC == kDrDepth ? LRT_DR_RAY_FLAG_DDIST_DPOS : SPEC == kDrDepthBorder ? LRT_DR_RAY_FLAG_BORDER : Q.ray_flags;
+}
+
 // BVH2DR.cpp:623-1073, Newton branch. Returns hit ? t : qFar + EPS.
 __device__ __forceinline__ float dr_intersect(uint32_t ray_flags, v3 ray_dir, const float values[8], float d, float d_inv, float qFar,
                                               v3 start_q, DRPayload* relax) {
@@ -128,8 +141,10 @@ __device__ __forceinline__ float corner_w(int i, v3 q) {
 }
 
 // BVH2DR.cpp:1130-1610 for ID32F layouts without normal derivatives.
+template <int SPEC>
 __device__ __forceinline__ void dr_brick(const DevScene& S, const DRParams& Q, v3 ray_pos, v3 ray_dir, v3 inv, float tNear, uint32_t brick,
                                          DRPayload* relax, DRHit* pHit) {
+  constexpr bool kColour = SPEC == kDrGeneric;
   float values[8];
   uint32_t idx[8];
   const uint4 N = S.nodes[brick];
@@ -146,7 +161,8 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const DRParams& Q, v
   const float old_t = pHit->t;
   const uint32_t max_steps = 16u * (S.brick_size + 1u) + 16u;
   uint32_t steps = 0;
-  const uint32_t flags = Q.ray_flags;
+  const uint32_t flags = dr_flags<SPEC>(Q);
+  const bool dtdv_fd = SPEC == kDrGeneric && Q.dtdv_fd != 0;
   while (bnf.x < bnf.y && pHit->t == old_t && steps++ < max_steps) {
     v3 hit_pos = ray_pos + bnf.x * ray_dir;
     v3 local = (hit_pos - bmin) * qscale;
@@ -187,8 +203,10 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const DRParams& Q, v
 #pragma unroll
             for (int i = 0; i < 8; i++) {
               relax->missed_dsdf[i] = -corner_w(i, q_ast) / glen;
-              uint32_t ci = S.values[t_off + i];
-              col = col + corner_w(i, dp_ast) * v3(S.values_f[ci], S.values_f[ci + 1], S.values_f[ci + 2]);
+              if (kColour) {
+                uint32_t ci = S.values[t_off + i];
+                col = col + corner_w(i, dp_ast) * v3(S.values_f[ci], S.values_f[ci + 1], S.values_f[ci + 2]);
+              }
               relax->missed_idx[i] = idx[i];
             }
             relax->missed_hit.color = col;
@@ -201,7 +219,7 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const DRParams& Q, v
         pHit->primId = brick;
         float dtdv[8];
         if (flags & (LRT_DR_RAY_FLAG_DDIFFUSE_DPOS | LRT_DR_RAY_FLAG_DDIST_DPOS)) {
-          if (Q.dtdv_fd) {  // BVH2DR.cpp:1075-1090
+          if (dtdv_fd) {  // BVH2DR.cpp:1075-1090
             float vv[8];
 #pragma unroll
             for (int i = 0; i < 8; i++) vv[i] = values[i];
@@ -226,6 +244,7 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const DRParams& Q, v
             relax->pd_ddist[i] = d * dtdv[i];
           }
         }
+        if (!kColour) break;  // colour and its derivatives are not read
         v3 pos = ray_pos + pHit->t * ray_dir;
         v3 dp = (pos - bmin) * (0.5f * sz);
         const uint32_t t_off = N.z + vs * vs * vs;
@@ -261,6 +280,7 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const DRParams& Q, v
 }
 
 // BVH2DR.cpp:12-167 (stop at the first hit, forced in the reference)
+template <int SPEC>
 __device__ __forceinline__ DRHit dr_nearest_hit(const DevScene& S, const DRParams& Q, v3 pos, v3 dir, float tFar, DRPayload* relax,
                                                 Stack& st) {
   if (tFar <= 0.0f) tFar *= -1.0f;
@@ -312,7 +332,7 @@ __device__ __forceinline__ DRHit dr_nearest_hit(const DevScene& S, const DRParam
           left = st.get(top > 0 ? top : 0);
         }
       }
-      if (top >= 0 && left != 0xFFFFFFFFu) dr_brick(S, Q, pos, dir, inv, mx(tNear, 0.1f), left & LRT_START_MASK, relax, &hit);
+      if (top >= 0 && left != 0xFFFFFFFFu) dr_brick<SPEC>(S, Q, pos, dir, inv, mx(tNear, 0.1f), left & LRT_START_MASK, relax, &hit);
       top--;
       left = st.get(top > 0 ? top : 0);
     }
