@@ -64,6 +64,10 @@ int lrt_update_camera(lrt_handle h, const float world_view[16], const float proj
 /* MultiRenderer::setSeed, Renderer/eye_ray.h:173. */
 int lrt_set_seed(lrt_handle h, uint32_t seed);
 
+/* Image sizes (all render calls): 1..65535 per side, as the reference's packed
+ * (y<<16)|x pixel coordinates allow, and the pixel count with both sides
+ * rounded up to multiples of 8 at most 2^32 - 1; larger sizes return
+ * LRT_E_INVALID. */
 /* MultiRenderer::Render(imageData, w, h, "color", passNum), eye_ray_host.cpp:246-263:
  * RGBA8 packed (a<<24)|(b<<16)|(g<<8)|r. */
 int lrt_render(lrt_handle h, uint32_t* image_rgba8, uint32_t width, uint32_t height, int passes);

@@ -155,10 +155,17 @@ static int check_ready(Renderer* r) {
   return LRT_OK;
 }
 
+// Packed pixel count of one W x H image (both sides rounded up to multiples
+// of 8); the kernels index a view's packed pixels with 32-bit integers.
+static bool packed_fits(uint32_t W, uint32_t H) {
+  return (uint64_t)(8 * ((W + 7) / 8)) * (uint64_t)(8 * ((H + 7) / 8)) <= 0xFFFFFFFFull;
+}
+
 // Launches n_views cameras at W x H into the handle's output buffers.
 int Renderer::render_device(const lrtd::Camera* cams, uint32_t n_views, uint32_t W, uint32_t H, bool rgba, bool gbuf, bool force_normal,
                             const char* timer_name, unsigned long long* d_counts) {
   if (W == 0 || H == 0 || W > 65535 || H > 65535 || n_views == 0) return set_err(LRT_E_INVALID, "bad image size");
+  if (!packed_fits(W, H)) return set_err(LRT_E_INVALID, "image too large: the 8-aligned pixel count must fit 32 bits");
   if (preset.spp == 0) return set_err(LRT_E_INVALID, "spp must be >= 1");
   if (preset.interpolation_mode != LRT_INTERPOLATION_MODE_TRILINEAR)
     return set_err(LRT_E_UNSUPPORTED, "only trilinear interpolation is supported");
@@ -543,6 +550,7 @@ int lrt_render_batch(lrt_handle owner, uint32_t n_items, const lrt_handle* scene
     if (s->device != r->device) return set_err(LRT_E_INVALID, "batch items must live on the owner's device");
     const uint32_t W = widths[i], H = heights[i];
     if (W == 0 || H == 0 || W > 65535 || H > 65535) return set_err(LRT_E_INVALID, "bad image size");
+    if (!packed_fits(W, H)) return set_err(LRT_E_INVALID, "image too large: the 8-aligned pixel count must fit 32 bits");
     lrtd::BatchItem& it = items[i];
     it.S = s->dev_scene();
     it.cam = make_camera(wvs + 16 * (size_t)i, prs + 16 * (size_t)i);

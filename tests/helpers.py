@@ -102,6 +102,9 @@ def compare_gbuffer(r, sc, p, wv, pr, W, H, voxels_per_unit):
     both = g_hit & d_hit
     rel = np.abs(t[both].astype(np.float64) - o64["t"][both]) / o64["t"][both]
     res["depth_rel_max"] = float(rel.max()) if rel.size else 0.0
+    both3 = both & c_hit  # the FP32 twin's own distance from FP64 on the same pixels
+    rel32 = np.abs(o32["t"][both3].astype(np.float32).astype(np.float64) - o64["t"][both3]) / o64["t"][both3]
+    res["depth_rel_max_fp32_twin"] = float(rel32.max()) if rel32.size else 0.0
     ang = angle(n[both], o64["n"][both])
     ang32 = angle(o32["n"][both], o64["n"][both]) if (c_hit[both]).all() else np.zeros_like(ang)
     # seam pixels: the FP32 CPU path itself leaves the 1e-4 band there (normal

@@ -6,6 +6,8 @@ within 1e-4 rad of the FP64 oracle. The FP32 oracle is the reference CPU
 path restated operation for operation, so the GPU is also compared with it
 directly (it is expected to agree bit for bit on almost every pixel).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -20,7 +22,7 @@ MODES = [L.SDF_OCTREE_NODE_INTERSECT_ST, L.SDF_OCTREE_NODE_INTERSECT_ANALYTIC, L
 
 
 @pytest.fixture(scope="module")
-def r_c1():
+def r_ico128():
     r = L.CreateMultiRenderer(L.DEVICE_GPU)
     r.SetScene(ico128_scene())
     yield r
@@ -29,10 +31,10 @@ def r_c1():
 
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("view", [0, 1, 2])
-def test_c1_gbuffer_parity(r_c1, mode, view):
+def test_ico128_gbuffer_parity(r_ico128, mode, view):
     sc = oracle_scene("ico128")
     wv, pr = L.orbit_camera(7, view)
-    res = _compare(r_c1, sc, preset(mode), wv, pr, 256, 256, voxels_per_unit=32 * 4 / 2.0)
+    res = _compare(r_ico128, sc, preset(mode), wv, pr, 256, 256, voxels_per_unit=32 * 4 / 2.0)
     print(mode, view, res)
     assert res["hits"] > 5000
     assert res["mask_mismatch_fp32"] <= 1e-4 * res["n_pix"]
@@ -43,7 +45,7 @@ def test_c1_gbuffer_parity(r_c1, mode, view):
         assert res["exact_t_fp32"] > 0.999
 
 
-def test_c2_gbuffer_parity():
+def test_ico512_gbuffer_parity():
     sbs = ico512_scene()
     r = L.CreateMultiRenderer(L.DEVICE_GPU)
     r.SetScene(sbs)
@@ -61,41 +63,41 @@ def test_c2_gbuffer_parity():
 @pytest.mark.parametrize("rmode", [L.MULTI_RENDER_MODE_MASK, L.MULTI_RENDER_MODE_LINEAR_DEPTH, L.MULTI_RENDER_MODE_DEPTH,
                                    L.MULTI_RENDER_MODE_NORMAL, L.MULTI_RENDER_MODE_LAMBERT_NO_TEX,
                                    L.MULTI_RENDER_MODE_PRIMITIVE, L.MULTI_RENDER_MODE_ST_ITERATIONS])
-def test_render_float_modes(r_c1, rmode):
+def test_render_float_modes(r_ico128, rmode):
     """RenderFloat colour per render mode vs the FP32 oracle (eye_ray.cpp:162-489)."""
     sc = oracle_scene("ico128")
     wv, pr = L.orbit_camera(3, 1)
     p = preset(L.SDF_OCTREE_NODE_INTERSECT_ST, render_mode=rmode)
-    img = r_c1.RenderFloat(128, 128, wv, pr, p)
+    img = r_ico128.RenderFloat(128, 128, wv, pr, p)
     ref = sc.render(wv, pr, p, 128, 128, precision=32, force_normal=False, rgba=True)["rgba"]
     diff = np.abs(img - ref).max(axis=-1)
     assert np.mean(diff <= 1e-5) > 0.9995, (rmode, np.sort(diff.ravel())[-10:])
 
 
-def test_render_rgba8_and_timer(r_c1):
+def test_render_rgba8_and_timer(r_ico128):
     wv, pr = L.orbit_camera(3, 2)
     p = preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON, render_mode=L.MULTI_RENDER_MODE_LAMBERT_NO_TEX)
-    f = r_c1.RenderFloat(64, 64, wv, pr, p)
-    u = r_c1.Render(64, 64, wv, pr, p)
+    f = r_ico128.RenderFloat(64, 64, wv, pr, p)
+    u = r_ico128.Render(64, 64, wv, pr, p)
     exp = (255 * np.clip(f, 0, 1)).astype(np.uint32)
     packed = exp[..., 0] | (exp[..., 1] << 8) | (exp[..., 2] << 16) | (exp[..., 3] << 24)
     assert np.array_equal(u, packed)
-    assert r_c1.GetExecutionTime("CastRaySingleBlock")[0] > 0
+    assert r_ico128.GetExecutionTime("CastRaySingleBlock")[0] > 0
 
 
-def test_spp_random_rays(r_c1):
+def test_spp_random_rays(r_ico128):
     """spp 4 with PCG3D jitter (RAY_GEN_MODE_RANDOM) matches the oracle."""
     sc = oracle_scene("ico128")
     wv, pr = L.orbit_camera(5, 0)
     p = preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON, render_mode=L.MULTI_RENDER_MODE_LINEAR_DEPTH, spp=4,
                ray_gen=L.RAY_GEN_MODE_RANDOM)
-    r_c1.setSeed(12345)
-    img = r_c1.RenderFloat(96, 96, wv, pr, p)
+    r_ico128.setSeed(12345)
+    img = r_ico128.RenderFloat(96, 96, wv, pr, p)
     ref = sc.render(wv, pr, p, 96, 96, seed=12345, precision=32, rgba=True)["rgba"]
     assert np.mean(np.abs(img - ref).max(-1) <= 1e-6) > 0.999
 
 
-def test_cast_rays_random(r_c1):
+def test_cast_rays_random(r_ico128):
     """Batched RayQuery_NearestHit on rays from outside, inside and on axis."""
     sc = oracle_scene("ico128")
     rng = np.random.default_rng(0)
@@ -106,7 +108,7 @@ def test_cast_rays_random(r_c1):
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     rays = np.zeros((n, 8), np.float32)
     rays[:, :3], rays[:, 4:7], rays[:, 7] = o, d, 1e9
-    hits = r_c1.RayQuery_NearestHit(rays)
+    hits = r_ico128.RayQuery_NearestHit(rays)
     g = np.array([(h.t, h.primId, h.coords[2], h.coords[3]) for h in hits])
     ref = sc.cast_rays(preset(), rays, precision=32)
     gh, rh = g[:, 1] != MISS, ref[:, 1] != MISS
@@ -115,12 +117,12 @@ def test_cast_rays_random(r_c1):
     assert np.mean(g[both, 0].astype(np.float32) == ref[both, 0].astype(np.float32)) > 0.999
 
 
-def test_work_counters_match_oracle(r_c1):
+def test_work_counters_match_oracle(r_ico128):
     """GPU per-ray work counters equal the FP32 oracle's (same traversal order)."""
     sc = oracle_scene("ico128")
     wv, pr = L.orbit_camera(7, 0)
-    r_c1.SetPreset(preset())
-    g = r_c1.CountWork(wv[None], pr[None], 256, 256)
+    r_ico128.SetPreset(preset())
+    g = r_ico128.CountWork(wv[None], pr[None], 256, 256)
     o = sc.render(wv, pr, preset(), 256, 256, precision=32, counters=True)["counters"]
     assert abs(int(g[0]) - o["node_pairs"]) <= 1e-4 * o["node_pairs"]
     assert abs(int(g[1]) - o["bricks"]) <= 1e-4 * o["bricks"]
@@ -146,6 +148,82 @@ def test_brute_force_topology_independent():
     assert np.mean(np.abs(gt[hit] - bt[hit]) <= 1e-6 * bt[hit]) > 0.99
 
 
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.mark.parametrize("mode", [L.SDF_OCTREE_NODE_INTERSECT_ST, L.SDF_OCTREE_NODE_INTERSECT_NEWTON])
+def test_reference_fixture_gbuffer_parity(mode):
+    """The reference's own SBS (saves/test_10_sbs.bin: 49,487 one-voxel bricks
+    at LODs 16..128, 8-bit DX) rendered from the reference test's camera
+    (tests.cpp render helper: lookAt from (0,0,3), fov 60, near 0.1, far 100)
+    and from two orbit views, against the FP32 and FP64 oracle."""
+    sbs = L.load_sbs(os.path.join(HERE, "golden", "test_10_sbs.bin"))
+    r = L.CreateMultiRenderer(L.DEVICE_GPU)
+    r.SetScene(sbs)
+    pairs, _ = r.GetBVH()
+    sc = oracle.Scene(sbs, pairs)
+    cams = [L.look_at_camera([0, 0, 3], [0, 0, 0], [0, 1, 0], fov=60.0, znear=0.1, zfar=100.0)]
+    cams += [L.orbit_camera(21, i) for i in range(2)]
+    for wv, pr in cams:
+        res = _compare(r, sc, preset(mode), wv, pr, 256, 256, voxels_per_unit=128 * 1 / 2.0)
+        print(mode, res)
+        assert res["hits"] > 1000
+        assert res["mask_mismatch_fp32"] <= 1e-4 * res["n_pix"]
+        assert res["mask_mismatch_fp64"] <= 1e-4 * res["n_pix"] and res["mismatch_all_grazing"]
+        assert res["depth_rel_max"] <= 1e-5
+        assert res["normal_bad"] == 0
+        assert res["exact_t_fp32"] > 0.999
+    r.close()
+
+
+@pytest.mark.parametrize("W,H", [(1, 1), (7, 13), (129, 65), (1000, 3), (3, 700)])
+def test_ragged_image_sizes(r_ico128, W, H):
+    """Image sides that are not multiples of the 8x8 tile (partial tiles,
+    single rows / columns) against the FP32 and FP64 oracle."""
+    sc = oracle_scene("ico128")
+    wv, pr = L.orbit_camera(31, 0, aspect=W / H)
+    res = _compare(r_ico128, sc, preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON), wv, pr, W, H, voxels_per_unit=32 * 4 / 2.0)
+    print(W, H, res)
+    assert res["mask_mismatch_fp32"] == 0
+    assert res["mask_mismatch_fp64"] <= max(1, 1e-4 * res["n_pix"]) and res["mismatch_all_grazing"]
+    assert res["depth_rel_max"] <= 1e-5
+    assert res["normal_bad"] == 0
+
+
+@pytest.mark.parametrize("eye", [[0.95, 0.1, 0.05], [0.0, 0.0, 0.0], [0.3, -0.85, 0.2]])
+def test_camera_inside_the_scene_box(r_ico128, eye):
+    """Eye inside the [-1, 1]^3 scene box: near a box face among the bricks
+    (the 0.1 SDF bias on tNear decides which bricks count), at the centre
+    (every ray starts inside the solid), and just outside the surface. With
+    the eye ~0.1 from the surface the reference arithmetic in FP32 (the
+    oracle's twin) is itself up to ~1.5e-5 from FP64 in relative depth, so
+    the depth bar is 1e-5 or the twin's own error, whichever is larger."""
+    sc = oracle_scene("ico128")
+    wv, pr = L.look_at_camera(eye, [0.2, 0.3, -0.4], [0, 1, 0], fov=90.0, znear=0.01, zfar=100.0)
+    res = _compare(r_ico128, sc, preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON), wv, pr, 128, 128, voxels_per_unit=32 * 4 / 2.0)
+    print(eye, res)
+    assert res["mask_mismatch_fp32"] <= 2
+    assert res["mask_mismatch_fp64"] <= 2 and res["mismatch_all_grazing"]
+    assert res["exact_t_fp32"] > 0.999
+    assert res["depth_rel_max"] <= max(1e-5, res["depth_rel_max_fp32_twin"])
+    assert res["normal_bad"] == 0
+
+
+def test_single_brick_scene():
+    """A one-brick scene (its BVH is one pair with an empty second child)."""
+    sbs = L.grid_sbs("sphere", 0.5, 0.0, 0.0, 1, seed=1, brick_count=1, brick_size=4)
+    r = L.CreateMultiRenderer(L.DEVICE_GPU)
+    r.SetScene(sbs)
+    pairs, _ = r.GetBVH()
+    assert pairs.shape[0] == 1
+    sc = oracle.Scene(sbs, pairs)
+    wv, pr = L.orbit_camera(2, 0)
+    res = _compare(r, sc, preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON), wv, pr, 64, 64, voxels_per_unit=1 * 4 / 2.0)
+    print(res)
+    assert res["hits"] > 100 and res["mask_mismatch_fp32"] == 0 and res["exact_t_fp32"] > 0.999
+    r.close()
+
+
 def test_bad_inputs_raise():
     r = L.CreateMultiRenderer(L.DEVICE_GPU)
     with pytest.raises(L.LrtError):
@@ -155,4 +233,8 @@ def test_bad_inputs_raise():
     bad.nodes[0, 2] = sbs.values.size  # data_offset out of range
     with pytest.raises(L.LrtError):
         r.SetScene(bad)
+    r.SetScene(sbs)
+    # sizes checked before any work; outputs NULL so no host buffers are needed
+    for w, h in ((65535, 65535), (0, 16), (65536, 1)):  # 65535^2 rounds up to 65536^2 packed pixels, past 32 bits
+        assert L.lib.lrt_render_gbuffer(r._h, None, None, None, w, h) != 0, (w, h)
 
