@@ -178,7 +178,24 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const DRParams& Q, v
       values[i] = S.values_f[idx[i]];
     }
     float2 fnf = RayBox(ray_pos, inv, min_pos, max_pos);
-    if (tNear < fnf.x) {
+    bool candidate = tNear < fnf.x;
+    if (SPEC == kDrDepth && candidate) {
+      // Without border tracking a voxel matters only if it can hold a root.
+      // Along the segment the trilinear distance is a convex combination of
+      // the 8 corners, so with every corner above 1e-6 and above 1e-3 of the
+      // largest one it stays far above SDF_EPS, and the FP32 cubic's rounding
+      // (~1e-5 of the largest coefficient) cannot flip a sign: dr_intersect
+      // would return "no hit". The reference evaluates such voxels anyway
+      // (BVH2DR.cpp:1199); skipping them changes no result.
+      float vmin = values[0], vmax = values[0];
+#pragma unroll
+      for (int i = 1; i < 8; i++) {
+        vmin = fminf(vmin, values[i]);
+        vmax = fmaxf(vmax, values[i]);
+      }
+      candidate = !(vmin > 1e-6f && vmin > 1e-3f * vmax);
+    }
+    if (candidate) {
       v3 start_pos = ray_pos + fnf.x * ray_dir;
       v3 start_q = (start_pos - min_pos) * qscale;
       float qFar = (fnf.y - fnf.x) * qscale;

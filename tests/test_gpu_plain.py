@@ -60,6 +60,45 @@ def test_ico512_gbuffer_parity():
     assert res["exact_t_fp32"] > 0.999
 
 
+def test_full_size_views_sampled_against_oracle():
+    """The benchmark's own shape: 8 orbit views x 2048x2048 of the 512^3
+    scene in one launch. 512 seeded pixels per view (image corners and
+    edges included) against the FP32 oracle's RayQuery_NearestHit on the
+    same eye rays: hit mask equal but for at most one grazing sample,
+    t bit-identical on >= 99.9 % of the hits and within 1e-6 relative on
+    all, normals within 1e-5 rad."""
+    sbs = ico512_scene()
+    r = L.CreateMultiRenderer(L.DEVICE_GPU)
+    r.SetScene(sbs)
+    r.SetPreset(preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON, render_mode=L.MULTI_RENDER_MODE_NORMAL))
+    sc = oracle_scene("ico512")
+    W = H = 2048
+    cams = [L.orbit_camera(2, i) for i in range(8)]
+    wvs, prs = np.stack([c[0] for c in cams]), np.stack([c[1] for c in cams])
+    r.RenderViewsDevice(wvs, prs, W, H, 0)
+    rng = np.random.default_rng(8)
+    n_hits = n_exact = n_mask = 0
+    for v in range(8):
+        t, n, prim = r.DownloadView(v, W, H)
+        xs = np.concatenate([[0, W - 1, 0, W - 1], rng.integers(0, W, 508)])
+        ys = np.concatenate([[0, 0, H - 1, H - 1], rng.integers(0, H, 508)])
+        rays = np.array([oracle.eye_ray(wvs[v], prs[v], int(x), int(y), 0.5, 0.5, W, H) for x, y in zip(xs, ys)], np.float32)
+        ref = sc.cast_rays(preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON), rays, precision=32)
+        g_prim, r_prim = prim[ys, xs], ref[:, 1].astype(np.uint32)
+        n_mask += int(((g_prim != MISS) != (r_prim != MISS)).sum())
+        hit = (g_prim != MISS) & (g_prim == r_prim)
+        assert hit.sum() > 50, v
+        gt, rt = t[ys, xs][hit], ref[hit, 0].astype(np.float32)
+        n_hits += int(hit.sum())
+        n_exact += int((gt == rt).sum())
+        assert np.max(np.abs(gt.astype(np.float64) - rt) / rt) <= 1e-6, v
+        assert np.max(angle(n[ys, xs][hit], ref[hit][:, 3:5].astype(np.float32))) <= 1e-5, v
+    print("hits", n_hits, "bit-exact", n_exact, "mask mismatches", n_mask)
+    assert n_mask <= 1
+    assert n_exact >= 0.999 * n_hits
+    r.close()
+
+
 @pytest.mark.parametrize("rmode", [L.MULTI_RENDER_MODE_MASK, L.MULTI_RENDER_MODE_LINEAR_DEPTH, L.MULTI_RENDER_MODE_DEPTH,
                                    L.MULTI_RENDER_MODE_NORMAL, L.MULTI_RENDER_MODE_LAMBERT_NO_TEX,
                                    L.MULTI_RENDER_MODE_PRIMITIVE, L.MULTI_RENDER_MODE_ST_ITERATIONS])
