@@ -9,6 +9,8 @@
 //   k_adam            Adam step with clamp to +-1000
 // Gradients are scattered with hardware float atomics into one dLoss/dS
 // buffer (the reference keeps a buffer per CPU thread and sums them).
+#include <algorithm>
+
 #include "dr_device.hpp"
 #include "dr.hpp"
 
@@ -101,7 +103,7 @@ __device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, S
     float2 d = rand2(x, y, s, A.seed);
     v3 pos, dir;
     eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
-    DRHit hit = dr_nearest_hit<SPEC>(A.S, Qc, pos, dir, 1e9f, &py, st);
+    DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, Qc, pos, dir, 1e9f, &py, st);
     if (hit.primId == 0xFFFFFFFFu) continue;
     v3 c = dr_color(A.mode, hit);
     res = res + v3(c.x / (float)spp, c.y / (float)spp, c.z / (float)spp);
@@ -130,7 +132,7 @@ __device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, S
     float2 d = rand2(x, y, s, A.seed);
     v3 pos, dir;
     eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
-    DRHit hit = dr_nearest_hit<SPEC>(A.S, Qg, pos, dir, 1e9f, &py, st);
+    DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, Qg, pos, dir, 1e9f, &py, st);
     if (hit.primId != 0xFFFFFFFFu) scatter_pd<SPEC>(A, py, dLdC);
   }
 }
@@ -202,7 +204,7 @@ __global__ __launch_bounds__(256) void k_dr_border_rays(DRArgs A, const uint32_t
     float2 d = rand2(x, y, s, A.seed);
     v3 pos, dir;
     eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
-    DRHit hit = dr_nearest_hit<SPEC>(A.S, Qb, pos, dir, 1e9f, &py, st);
+    DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, Qb, pos, dir, 1e9f, &py, st);
     if (!(py.missed_hit.sdf < A.relax_eps)) continue;
     local_hits++;
     float delta[4];
@@ -237,6 +239,20 @@ __global__ __launch_bounds__(256) void k_dr_border_rays(DRArgs A, const uint32_t
     }
   }
   if (local_hits) atomicAdd(hits, local_hits);
+}
+
+// Per-brick copy of the corner values the voxel walk reads:
+// out[b * v^3 + k] = values_f[values[data_offset(b) + k]] (indices checked
+// against values_f when the scene was set). One dependent load per corner in
+// the ray casts instead of two; refreshed before every RenderDR, so it always
+// equals the current parameters.
+__global__ void k_dr_gather(DevScene S, float* __restrict__ out) {
+  const uint32_t nv = S.v_size * S.v_size * S.v_size;
+  const uint64_t total = (uint64_t)S.n_nodes * nv;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t b = (uint32_t)(g / nv), k = (uint32_t)(g - (uint64_t)b * nv);
+    out[g] = S.values_f[S.values[S.nodes[b].z + k]];
+  }
 }
 
 __global__ void k_dr_scale(float* grad, uint32_t n, float mult) {
@@ -290,6 +306,12 @@ hipError_t launch_dr_border_rays(const lrtd::DRArgs& A, const uint32_t* d_list, 
     hipLaunchKernelGGL(lrtd::k_dr_border_rays<lrtd::kDrDepthBorder>, dim3(blocks), dim3(256), 0, s, A, d_list, d_count, d_spill, d_hits);
   else
     hipLaunchKernelGGL(lrtd::k_dr_border_rays<lrtd::kDrGeneric>, dim3(blocks), dim3(256), 0, s, A, d_list, d_count, d_spill, d_hits);
+  return hipGetLastError();
+}
+hipError_t launch_dr_gather(const lrtd::DevScene& S, float* d_out, hipStream_t s) {
+  const uint64_t total = (uint64_t)S.n_nodes * S.v_size * S.v_size * S.v_size;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(lrtd::k_dr_gather, dim3(blocks), dim3(256), 0, s, S, d_out);
   return hipGetLastError();
 }
 hipError_t launch_dr_scale(float* d_grad, uint32_t n, float mult, hipStream_t s) {

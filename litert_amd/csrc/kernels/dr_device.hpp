@@ -142,11 +142,10 @@ __device__ __forceinline__ float corner_w(int i, v3 q) {
 
 // BVH2DR.cpp:1130-1610 for ID32F layouts without normal derivatives.
 template <int SPEC>
-__device__ __forceinline__ void dr_brick(const DevScene& S, const DRParams& Q, v3 ray_pos, v3 ray_dir, v3 inv, float tNear, uint32_t brick,
-                                         DRPayload* relax, DRHit* pHit) {
+__device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restrict__ bvals, const DRParams& Q, v3 ray_pos, v3 ray_dir,
+                                         v3 inv, float tNear, uint32_t brick, DRPayload* relax, DRHit* pHit) {
   constexpr bool kColour = SPEC == kDrGeneric;
   float values[8];
-  uint32_t idx[8];
   const uint4 N = S.nodes[brick];
   const float px = (float)(N.x >> 16), py = (float)(N.x & 0xFFFFu), pz = (float)(N.y >> 16), sz = (float)(N.y & 0xFFFFu);
   const float bsf = (float)S.brick_size;
@@ -157,6 +156,7 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const DRParams& Q, v
   const v3 bmax = bmin + sz_inv * v3(1, 1, 1);
   const float qscale = 0.5f * sz * bsf;
   const uint32_t vs = S.v_size;
+  const float* bv = bvals + (size_t)brick * vs * vs * vs;  // this brick's corner values, gathered from values_f
   float2 bnf = RayBox(ray_pos, inv, bmin, bmax);
   const float old_t = pHit->t;
   const uint32_t max_steps = 16u * (S.brick_size + 1u) + 16u;
@@ -171,12 +171,10 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const DRParams& Q, v
     const uint32_t vx = (uint32_t)vp.x, vy = (uint32_t)vp.y, vz = (uint32_t)vp.z;
     v3 min_pos = bmin + d * vp;
     v3 max_pos = min_pos + d * v3(1, 1, 1);
+    // corner i of the voxel: values[i] = values_f[values[data_offset + vid(i)]]
+    auto vid = [&](int i) { return (vx + ((i & 4) >> 2)) * vs * vs + (vy + ((i & 2) >> 1)) * vs + (vz + (i & 1)); };
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      uint32_t vId = (vx + ((i & 4) >> 2)) * vs * vs + (vy + ((i & 2) >> 1)) * vs + (vz + (i & 1));
-      idx[i] = S.values[N.z + vId];
-      values[i] = S.values_f[idx[i]];
-    }
+    for (int i = 0; i < 8; i++) values[i] = bv[vid(i)];
     float2 fnf = RayBox(ray_pos, inv, min_pos, max_pos);
     bool candidate = tNear < fnf.x;
     if (SPEC == kDrDepth && candidate) {
@@ -224,7 +222,7 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const DRParams& Q, v
                 uint32_t ci = S.values[t_off + i];
                 col = col + corner_w(i, dp_ast) * v3(S.values_f[ci], S.values_f[ci + 1], S.values_f[ci + 2]);
               }
-              relax->missed_idx[i] = idx[i];
+              relax->missed_idx[i] = S.values[N.z + vid(i)];
             }
             relax->missed_hit.color = col;
             relax->missed_hit.normal = v3(g.x / glen, g.y / glen, g.z / glen);
@@ -257,7 +255,7 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const DRParams& Q, v
           }
 #pragma unroll
           for (int i = 0; i < 8; i++) {
-            relax->pd_idx[i] = idx[i];
+            relax->pd_idx[i] = S.values[N.z + vid(i)];
             relax->pd_ddist[i] = d * dtdv[i];
           }
         }
@@ -298,7 +296,8 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const DRParams& Q, v
 
 // BVH2DR.cpp:12-167 (stop at the first hit, forced in the reference)
 template <int SPEC>
-__device__ __forceinline__ DRHit dr_nearest_hit(const DevScene& S, const DRParams& Q, v3 pos, v3 dir, float tFar, DRPayload* relax,
+__device__ __forceinline__ DRHit dr_nearest_hit(const DevScene& S, const float* __restrict__ bvals, const DRParams& Q, v3 pos, v3 dir,
+                                                float tFar, DRPayload* relax,
                                                 Stack& st) {
   if (tFar <= 0.0f) tFar *= -1.0f;
   const v3 inv = SafeInverse(dir);
@@ -349,7 +348,7 @@ __device__ __forceinline__ DRHit dr_nearest_hit(const DevScene& S, const DRParam
           left = st.get(top > 0 ? top : 0);
         }
       }
-      if (top >= 0 && left != 0xFFFFFFFFu) dr_brick<SPEC>(S, Q, pos, dir, inv, mx(tNear, 0.1f), left & LRT_START_MASK, relax, &hit);
+      if (top >= 0 && left != 0xFFFFFFFFu) dr_brick<SPEC>(S, bvals, Q, pos, dir, inv, mx(tNear, 0.1f), left & LRT_START_MASK, relax, &hit);
       top--;
       left = st.get(top > 0 ? top : 0);
     }
