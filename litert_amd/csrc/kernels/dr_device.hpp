@@ -177,8 +177,7 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restr
     for (int i = 0; i < 8; i++) values[i] = bv[vid(i)];
     float2 fnf = RayBox(ray_pos, inv, min_pos, max_pos);
     bool candidate = tNear < fnf.x;
-    if (SPEC == kDrDepth && candidate) {
-      // Without border tracking a voxel matters only if it can hold a root.
+    if ((SPEC == kDrDepth || SPEC == kDrDepthBorder) && candidate) {
       // Along the segment the trilinear distance is a convex combination of
       // the 8 corners, so with every corner above 1e-6 and above 1e-3 of the
       // largest one it stays far above SDF_EPS, and the FP32 cubic's rounding
@@ -191,7 +190,19 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restr
         vmin = fminf(vmin, values[i]);
         vmax = fmaxf(vmax, values[i]);
       }
-      candidate = !(vmin > 1e-6f && vmin > 1e-3f * vmax);
+      bool skip = vmin > 1e-6f && vmin > 1e-3f * vmax;
+      if (SPEC == kDrDepthBorder) {
+        // Border rays also track the smallest distance along the ray
+        // (missed_hit.sdf). Every update in dr_intersect needs a distance
+        // below it, and none along this segment is (same bound), so the
+        // voxel's only effects are the two state resets done here.
+        skip = skip && vmin > relax->missed_hit.sdf + 1e-3f * vmax;
+        if (skip) {
+          relax->missed_cand.sdf = 1000.f;
+          relax->missed_hit.check_vox_border = 0;
+        }
+      }
+      candidate = !skip;
     }
     if (candidate) {
       v3 start_pos = ray_pos + fnf.x * ray_dir;
