@@ -103,7 +103,7 @@ __device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, S
     float2 d = rand2(x, y, s, A.seed);
     v3 pos, dir;
     eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
-    DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, Qc, pos, dir, 1e9f, &py, st);
+    DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, A.bminmax, Qc, pos, dir, 1e9f, &py, st);
     if (hit.primId == 0xFFFFFFFFu) continue;
     v3 c = dr_color(A.mode, hit);
     res = res + v3(c.x / (float)spp, c.y / (float)spp, c.z / (float)spp);
@@ -132,7 +132,7 @@ __device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, S
     float2 d = rand2(x, y, s, A.seed);
     v3 pos, dir;
     eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
-    DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, Qg, pos, dir, 1e9f, &py, st);
+    DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, A.bminmax, Qg, pos, dir, 1e9f, &py, st);
     if (hit.primId != 0xFFFFFFFFu) scatter_pd<SPEC>(A, py, dLdC);
   }
 }
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(256) void k_dr_border_rays(DRArgs A, const uint32_t
     float2 d = rand2(x, y, s, A.seed);
     v3 pos, dir;
     eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
-    DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, Qb, pos, dir, 1e9f, &py, st);
+    DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, A.bminmax, Qb, pos, dir, 1e9f, &py, st);
     if (!(py.missed_hit.sdf < A.relax_eps)) continue;
     local_hits++;
     float delta[4];
@@ -253,6 +253,20 @@ __global__ void k_dr_gather(DevScene S, float* __restrict__ out) {
     const uint32_t b = (uint32_t)(g / nv), k = (uint32_t)(g - (uint64_t)b * nv);
     out[g] = S.values_f[S.values[S.nodes[b].z + k]];
   }
+}
+
+// Smallest and largest gathered corner value of every brick.
+__global__ void k_dr_brick_minmax(DevScene S, const float* __restrict__ bvals, float2* __restrict__ out) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= S.n_nodes) return;
+  const uint32_t nv = S.v_size * S.v_size * S.v_size;
+  const float* v = bvals + (size_t)b * nv;
+  float lo = v[0], hi = v[0];
+  for (uint32_t k = 1; k < nv; k++) {
+    lo = fminf(lo, v[k]);
+    hi = fmaxf(hi, v[k]);
+  }
+  out[b] = make_float2(lo, hi);
 }
 
 __global__ void k_dr_scale(float* grad, uint32_t n, float mult) {
@@ -308,10 +322,11 @@ hipError_t launch_dr_border_rays(const lrtd::DRArgs& A, const uint32_t* d_list, 
     hipLaunchKernelGGL(lrtd::k_dr_border_rays<lrtd::kDrGeneric>, dim3(blocks), dim3(256), 0, s, A, d_list, d_count, d_spill, d_hits);
   return hipGetLastError();
 }
-hipError_t launch_dr_gather(const lrtd::DevScene& S, float* d_out, hipStream_t s) {
+hipError_t launch_dr_gather(const lrtd::DevScene& S, float* d_out, float2* d_minmax, hipStream_t s) {
   const uint64_t total = (uint64_t)S.n_nodes * S.v_size * S.v_size * S.v_size;
   const uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
   hipLaunchKernelGGL(lrtd::k_dr_gather, dim3(blocks), dim3(256), 0, s, S, d_out);
+  hipLaunchKernelGGL(lrtd::k_dr_brick_minmax, dim3((S.n_nodes + 255) / 256), dim3(256), 0, s, S, (const float*)d_out, d_minmax);
   return hipGetLastError();
 }
 hipError_t launch_dr_scale(float* d_grad, uint32_t n, float mult, hipStream_t s) {

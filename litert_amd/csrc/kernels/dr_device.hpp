@@ -142,8 +142,9 @@ __device__ __forceinline__ float corner_w(int i, v3 q) {
 
 // BVH2DR.cpp:1130-1610 for ID32F layouts without normal derivatives.
 template <int SPEC>
-__device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restrict__ bvals, const DRParams& Q, v3 ray_pos, v3 ray_dir,
-                                         v3 inv, float tNear, uint32_t brick, DRPayload* relax, DRHit* pHit) {
+__device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restrict__ bvals, const float2* __restrict__ bminmax,
+                                         const DRParams& Q, v3 ray_pos, v3 ray_dir, v3 inv, float tNear, uint32_t brick, DRPayload* relax,
+                                         DRHit* pHit) {
   constexpr bool kColour = SPEC == kDrGeneric;
   float values[8];
   const uint4 N = S.nodes[brick];
@@ -158,6 +159,24 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restr
   const uint32_t vs = S.v_size;
   const float* bv = bvals + (size_t)brick * vs * vs * vs;  // this brick's corner values, gathered from values_f
   float2 bnf = RayBox(ray_pos, inv, bmin, bmax);
+  if (SPEC == kDrDepth || SPEC == kDrDepthBorder) {
+    // Whole-brick form of the voxel skip below: the brick's smallest and
+    // largest corner values bound every voxel's, so if they pass the test
+    // every voxel of the walk would be skipped. Border rays apply the skip's
+    // state resets when the walk would have reached a voxel: it starts
+    // (bnf.x < bnf.y) past tNear (1e-5 margin over the few-ulp difference
+    // between the brick's and its first voxel's entry distance).
+    const float2 mm = bminmax[brick];
+    bool skip = mm.x > 1e-6f && mm.x > 1e-3f * mm.y;
+    if (SPEC == kDrDepthBorder) {
+      skip = skip && mm.x > relax->missed_hit.sdf + 1e-3f * mm.y && (!(bnf.x < bnf.y) || bnf.x > tNear + 1e-5f);
+      if (skip && bnf.x < bnf.y) {
+        relax->missed_cand.sdf = 1000.f;
+        relax->missed_hit.check_vox_border = 0;
+      }
+    }
+    if (skip) return;
+  }
   const float old_t = pHit->t;
   const uint32_t max_steps = 16u * (S.brick_size + 1u) + 16u;
   uint32_t steps = 0;
@@ -307,8 +326,8 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restr
 
 // BVH2DR.cpp:12-167 (stop at the first hit, forced in the reference)
 template <int SPEC>
-__device__ __forceinline__ DRHit dr_nearest_hit(const DevScene& S, const float* __restrict__ bvals, const DRParams& Q, v3 pos, v3 dir,
-                                                float tFar, DRPayload* relax,
+__device__ __forceinline__ DRHit dr_nearest_hit(const DevScene& S, const float* __restrict__ bvals, const float2* __restrict__ bminmax,
+                                                const DRParams& Q, v3 pos, v3 dir, float tFar, DRPayload* relax,
                                                 Stack& st) {
   if (tFar <= 0.0f) tFar *= -1.0f;
   const v3 inv = SafeInverse(dir);
@@ -359,7 +378,7 @@ __device__ __forceinline__ DRHit dr_nearest_hit(const DevScene& S, const float* 
           left = st.get(top > 0 ? top : 0);
         }
       }
-      if (top >= 0 && left != 0xFFFFFFFFu) dr_brick<SPEC>(S, bvals, Q, pos, dir, inv, mx(tNear, 0.1f), left & LRT_START_MASK, relax, &hit);
+      if (top >= 0 && left != 0xFFFFFFFFu) dr_brick<SPEC>(S, bvals, bminmax, Q, pos, dir, inv, mx(tNear, 0.1f), left & LRT_START_MASK, relax, &hit);
       top--;
       left = st.get(top > 0 ? top : 0);
     }

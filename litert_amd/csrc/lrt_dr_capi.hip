@@ -65,6 +65,7 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
   if ((e = r->d_dr_misc.ensure(64)) != hipSuccess) return hip_err(e, "alloc");
   const uint64_t vs = r->header.brick_size + 2 * r->header.brick_pad + 1;
   if ((e = r->d_dr_bvals.ensure((size_t)r->nodes.size() * vs * vs * vs * 4)) != hipSuccess) return hip_err(e, "alloc");
+  if ((e = r->d_dr_bminmax.ensure((size_t)r->nodes.size() * 8)) != hipSuccess) return hip_err(e, "alloc");
   const uint32_t blocks = r->grid_blocks();
   if ((e = r->ensure_spill(blocks)) != hipSuccess) return hip_err(e, "alloc spill");
   uint32_t* misc = (uint32_t*)r->d_dr_misc.p;  // [0] work counter [1] border count [2] border hits [4..5] loss (double)
@@ -100,9 +101,10 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
   A.depth_thr = (float*)r->d_dr_thr.p;
   A.grad = (float*)r->d_dr_grad.p;
   A.bvals = (const float*)r->d_dr_bvals.p;
+  A.bminmax = (const float2*)r->d_dr_bminmax.p;
 
   if ((e = hipEventRecord(r->ev0, r->stream)) != hipSuccess) return hip_err(e, "event");
-  if ((e = launch_dr_gather(A.S, (float*)r->d_dr_bvals.p, r->stream)) != hipSuccess) return hip_err(e, "dr gather");
+  if ((e = launch_dr_gather(A.S, (float*)r->d_dr_bvals.p, (float2*)r->d_dr_bminmax.p, r->stream)) != hipSuccess) return hip_err(e, "dr gather");
   if ((e = launch_dr_primary(A, misc, (uint32_t*)r->d_spill.p, (double*)(misc + 4), blocks, r->stream)) != hipSuccess)
     return hip_err(e, "dr primary");
   if ((e = launch_dr_border_find(A, (uint32_t*)r->d_dr_list.p, misc + 1, r->stream)) != hipSuccess) return hip_err(e, "dr border");

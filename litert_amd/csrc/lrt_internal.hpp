@@ -57,7 +57,8 @@ struct Renderer {
   uint32_t dr_n_ref = 0, dr_w = 0, dr_h = 0;
   std::vector<float> dr_world_views, dr_projs;  // 16 floats per reference view
   DevBuf d_dr_ref, d_dr_img, d_dr_thr, d_dr_grad, d_dr_list, d_dr_misc, d_adam_v, d_adam_s;
-  DevBuf d_dr_bvals;  // per-brick corner values gathered from values_f before every RenderDR
+  DevBuf d_dr_bvals;    // per-brick corner values gathered from values_f before every RenderDR
+  DevBuf d_dr_bminmax;  // per-brick smallest / largest gathered value
   float dr_last_loss = 0.f;
   uint32_t dr_last_border = 0, dr_last_border_hits = 0;
   // batched render (lrt_render_batch)
@@ -94,7 +95,7 @@ struct Renderer {
   void release_all() {
     DevBuf* all[] = {&d_pairs, &d_nodes, &d_values, &d_values_f, &d_brecs, &d_cams, &d_counter, &d_spill, &d_counts,
                      &d_rays, &d_hits, &d_out_t, &d_out_n, &d_out_prim, &d_out_rgba, &d_dr_ref, &d_dr_img,
-                     &d_dr_thr, &d_dr_grad, &d_dr_list, &d_dr_misc, &d_adam_v, &d_adam_s, &d_dr_bvals, &d_mlp_params,
+                     &d_dr_thr, &d_dr_grad, &d_dr_list, &d_dr_misc, &d_adam_v, &d_adam_s, &d_dr_bvals, &d_dr_bminmax, &d_mlp_params,
                      &d_mlp_m, &d_mlp_v, &d_mlp_packed, &d_mlp_partial, &d_mlp_ploss, &d_mlp_grad, &d_mlp_loss,
                      &d_bfeat, &d_ds_feat, &d_ds_tgt, &d_ds_meta, &d_ds_out, &d_b_items, &d_b_offsets, &d_b_t,
                      &d_b_n, &d_b_prim};
