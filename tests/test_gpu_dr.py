@@ -1,6 +1,6 @@
 """GPU parity of the differentiable renderer (RenderDR / Adam) against the oracle.
 
-Gradient tolerance (north star): within 1e-4 relative (L2 norm over all
+Gradient tolerance: within 1e-4 relative (L2 norm over all
 parameters) of the FP64 oracle; the FP32 oracle restates the same arithmetic,
 so GPU and FP32 oracle images agree bit for bit on almost every pixel.
 """

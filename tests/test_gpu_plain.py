@@ -1,6 +1,6 @@
 """GPU parity of the plain primary-ray path against the CPU oracle.
 
-Tolerances (north star): hit mask may differ on at most 1e-4 of the pixels
+Tolerances: hit mask may differ on at most 1e-4 of the pixels
 and only on grazing/seam pixels; depth within 1e-5 relative and normals
 within 1e-4 rad of the FP64 oracle. The FP32 oracle is the reference CPU
 path restated operation for operation, so the GPU is also compared with it
@@ -246,6 +246,41 @@ def test_camera_inside_the_scene_box(r_ico128, eye):
     assert res["exact_t_fp32"] > 0.999
     assert res["depth_rel_max"] <= max(1e-5, res["depth_rel_max_fp32_twin"])
     assert res["normal_bad"] == 0
+
+
+def test_indexed_neighbour_layout_smoothed_normals():
+    """ID32F_IRGB32F_IN scene (27 neighbour-brick indices per brick) with the
+    geometric and the smoothed normal mode (BVH2Common.cpp:838-906: the
+    normals of the 8 voxels around the hit, across brick borders through the
+    neighbour table, blended by the smoothstep weights), against the FP32
+    and FP64 oracle; the smoothed normals must differ from the geometric ones
+    on many pixels, which shows the blend ran."""
+    sbs = L.grid_sbs("torus", 0.6, 0.2, 0.02, 3, seed=3, brick_count=16, brick_size=4, with_neighbors=True)
+    r = L.CreateMultiRenderer(L.DEVICE_GPU)
+    r.SetScene(sbs)
+    pairs, _ = r.GetBVH()
+    sc = oracle.Scene(sbs, pairs)
+    normals = {}
+    for nm in (L.NORMAL_MODE_GEOMETRY, L.NORMAL_MODE_SDF_SMOOTHED):
+        p = preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON)
+        p.normal_mode = nm
+        for view in range(2):
+            wv, pr = L.orbit_camera(41, view)
+            res = _compare(r, sc, p, wv, pr, 160, 160, voxels_per_unit=16 * 4 / 2.0)
+            print(nm, view, res)
+            assert res["hits"] > 2000
+            assert res["mask_mismatch_fp32"] <= 1e-4 * res["n_pix"]
+            assert res["mask_mismatch_fp64"] <= 1e-4 * res["n_pix"] and res["mismatch_all_grazing"]
+            assert res["depth_rel_max"] <= 1e-5
+            assert res["normal_bad"] == 0
+            normals[(nm, view)] = r.RenderGBuffer(160, 160, wv, pr)
+    for view in range(2):
+        tg, ng, pg = normals[(L.NORMAL_MODE_GEOMETRY, view)]
+        ts, ns, ps = normals[(L.NORMAL_MODE_SDF_SMOOTHED, view)]
+        assert np.array_equal(tg, ts) and np.array_equal(pg, ps)  # same hits
+        hit = pg != MISS
+        assert np.mean(angle(ng[hit], ns[hit]) > 1e-3) > 0.2
+    r.close()
 
 
 def test_single_brick_scene():
