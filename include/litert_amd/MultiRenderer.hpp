@@ -192,16 +192,25 @@ class MultiRendererDR : public MultiRenderer {
   // OptimizeFixedStructure(preset, sbs): sbs.values_f receives the result
   bool OptimizeFixedStructure(MultiRendererDRPreset preset, SdfSBS& sbs, uint32_t seed = 0) {
     last_grad_.resize(sbs.values_f.size());
+    losses_.assign(preset.opt_iterations, 0.0f);
     bool r = ok(lrt_dr_optimize(h_, &preset, &sbs.header, sbs.nodes.data(), (uint32_t)sbs.nodes.size(), sbs.values.data(), sbs.values.size(),
-                                sbs.values_f.data(), sbs.values_f.size(), seed, 0, nullptr, nullptr));
+                                sbs.values_f.data(), sbs.values_f.size(), seed, 0, losses_.data(), nullptr));
     if (r) lrt_dr_get_last_dloss_ds(h_, last_grad_.data(), last_grad_.size());
     return r;
   }
   const float* getLastdLoss_dS() const { return last_grad_.data(); }
+  // per-step losses of the last OptimizeFixedStructure (the reference prints them)
+  const std::vector<float>& getLosses() const { return losses_; }
+  // getLastImage(view), MultiRendererDR.h:86: W * H float4 (colour + coverage)
+  const std::vector<float>& getLastImage(uint32_t view) {
+    last_img_.assign((size_t)w_ * h_img_ * 4, 0.0f);
+    if (!ok(lrt_dr_get_last_image(h_, view, last_img_.data()))) last_img_.clear();
+    return last_img_;
+  }
 
  private:
   uint32_t w_ = 0, h_img_ = 0;
-  std::vector<float> last_grad_;
+  std::vector<float> last_grad_, last_img_, losses_;
 };
 
 }  // namespace dr

@@ -142,7 +142,12 @@ int lrt_dr_render_grad(lrt_handle h, const MultiRendererDRPreset* preset, uint32
 int lrt_dr_optimize(lrt_handle h, const MultiRendererDRPreset* preset, const SdfSBSHeader* header, const SdfSBSNode* nodes,
                     uint32_t n_nodes, const uint32_t* values, uint64_t n_values, float* values_f, uint64_t n_values_f, uint32_t seed,
                     int dtdv_mode, float* losses, float* total_ms);
-/* MultiRendererDR::getLastdLoss_dS, MultiRendererDR.h:88. */
+/* MultiRendererDR::getLastImage(view), MultiRendererDR.h:86: the image the last
+ * RenderDR of that reference view produced (W*H*4 floats: colour + coverage). */
+int lrt_dr_get_last_image(lrt_handle h, uint32_t view, float* out);
+/* MultiRendererDR::getLastdLoss_dS, MultiRendererDR.h:88: the gradient of the
+ * last image rendered, divided by the step's image_batch_size as the reference
+ * leaves it (MultiRendererDR.cpp:405, 475-476). */
 int lrt_dr_get_last_dloss_ds(lrt_handle h, float* out, uint64_t n);
 /* Border statistics of the last RenderDR: border pixels and border rays that found a relaxation point. */
 int lrt_dr_last_stats(lrt_handle h, uint32_t* border_pixels, uint32_t* border_rays_hit);

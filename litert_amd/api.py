@@ -155,6 +155,7 @@ def _load():
         "lrt_dr_optimize": (i32, [P, C.POINTER(MultiRendererDRPreset), C.POINTER(SdfSBSHeader), P, u32, P, u64, P, u64, u32, i32, P,
                                   C.POINTER(f32)]),
         "lrt_dr_get_last_dloss_ds": (i32, [P, P, u64]),
+        "lrt_dr_get_last_image": (i32, [P, u32, P]),
         "lrt_dr_last_stats": (i32, [P, C.POINTER(u32), C.POINTER(u32)]),
         "lrt_mlp_init": (i32, [P, u32]),
         "lrt_mlp_set_params": (i32, [P, P, u32]),
@@ -555,6 +556,13 @@ class MultiRendererDR(MultiRenderer):
     def getLastdLoss_dS(self, n):
         out = np.zeros(n, np.float32)
         _check(lib.lrt_dr_get_last_dloss_ds(self._h, _ptr(out), n), "getLastdLoss_dS")
+        return out
+
+    def getLastImage(self, view):
+        """MultiRendererDR::getLastImage (MultiRendererDR.h:86): (H, W, 4)."""
+        n, H, W = self.ref_shape
+        out = np.zeros((H, W, 4), np.float32)
+        _check(lib.lrt_dr_get_last_image(self._h, view, _ptr(out)), "getLastImage")
         return out
 
 

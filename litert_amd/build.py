@@ -74,10 +74,13 @@ def build_cpp_tests(oracle_lib):
     """C++ tests that use the facade header (include/litert_amd/MultiRenderer.hpp)."""
     out_dir = os.path.join(ROOT, "tests", "cpp", "_build")
     os.makedirs(out_dir, exist_ok=True)
-    exe = os.path.join(out_dir, "test_facade_render")
-    _run(["g++", "-O2", "-std=c++17", os.path.join(ROOT, "tests", "cpp", "test_facade_render.cpp"), "-o", exe,
-          LIB, oracle_lib, "-Wl,-rpath," + HERE, "-Wl,-rpath," + os.path.dirname(oracle_lib), "-fopenmp"])
-    return exe
+    exes = []
+    for name in ("test_facade_render", "test_facade_dr"):
+        exe = os.path.join(out_dir, name)
+        _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(ROOT, "tests", "cpp", name + ".cpp"), "-o", exe,
+              LIB, oracle_lib, "-Wl,-rpath," + HERE, "-Wl,-rpath," + os.path.dirname(oracle_lib), "-fopenmp"])
+        exes.append(exe)
+    return exes
 
 
 if __name__ == "__main__":

@@ -58,7 +58,7 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
   const uint32_t W = r->dr_w, H = r->dr_h;
   const size_t np = (size_t)W * H;
   const uint32_t P = (uint32_t)r->values_f.size();
-  if ((e = r->d_dr_img.ensure(np * 16)) != hipSuccess) return hip_err(e, "alloc");
+  if ((e = r->d_dr_img.ensure(np * 16 * r->dr_n_ref)) != hipSuccess) return hip_err(e, "alloc");  // one slot per view
   if ((e = r->d_dr_thr.ensure(np * 4)) != hipSuccess) return hip_err(e, "alloc");
   if ((e = r->d_dr_grad.ensure((size_t)P * 4)) != hipSuccess) return hip_err(e, "alloc");
   if ((e = r->d_dr_list.ensure(np * 4)) != hipSuccess) return hip_err(e, "alloc");
@@ -97,7 +97,7 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
   A.forced_border = p.debug_forced_border ? 1u : 0u;
   A.relax_eps = p.border_relax_eps;
   A.ref = (const float4*)r->d_dr_ref.p + np * view;
-  A.out_image = (float4*)r->d_dr_img.p;
+  A.out_image = (float4*)r->d_dr_img.p + np * view;
   A.depth_thr = (float*)r->d_dr_thr.p;
   A.grad = (float*)r->d_dr_grad.p;
   A.bvals = (const float*)r->d_dr_bvals.p;
@@ -123,6 +123,7 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
   std::memcpy(&loss, &h_misc[4], sizeof(double));
   r->dr_last_loss = (float)(loss / (double)np);
   r->dr_last_border = h_misc[1];
+  r->dr_img_valid[view] = 1;
   r->dr_last_border_hits = h_misc[2];
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, r->ev0, r->ev1);
@@ -159,6 +160,7 @@ int lrt_dr_set_reference(lrt_handle h, const float* images, const float* masks, 
   h->dr_n_ref = n;
   h->dr_w = W;
   h->dr_h = H;
+  h->dr_img_valid.assign(n, 0);
   h->dr_world_views.assign(wvs, wvs + 16 * n);
   h->dr_projs.assign(prs, prs + 16 * n);
   return LRT_OK;
@@ -169,8 +171,10 @@ int lrt_dr_render_grad(lrt_handle h, const MultiRendererDRPreset* p, uint32_t vi
   if (!h || !p) return set_err(LRT_E_INVALID, "null argument");
   int c = dr_render_view(h, *p, view, seed, dtdv_mode, kernel_ms);
   if (c) return c;
+  h->dr_last_batch = 1;
   hipError_t e;
-  if (out_image && (e = hipMemcpy(out_image, h->d_dr_img.p, (size_t)h->dr_w * h->dr_h * 16, hipMemcpyDeviceToHost)) != hipSuccess)
+  const size_t np = (size_t)h->dr_w * h->dr_h;
+  if (out_image && (e = hipMemcpy(out_image, (const float4*)h->d_dr_img.p + np * view, np * 16, hipMemcpyDeviceToHost)) != hipSuccess)
     return hip_err(e, "download image");
   if (out_grad && (e = hipMemcpy(out_grad, h->d_dr_grad.p, h->values_f.size() * 4, hipMemcpyDeviceToHost)) != hipSuccess)
     return hip_err(e, "download grad");
@@ -193,6 +197,7 @@ int lrt_dr_optimize(lrt_handle h, const MultiRendererDRPreset* p, const SdfSBSHe
   if ((e = zero_u32((uint32_t*)h->d_adam_v.p, P, h->stream)) != hipSuccess) return hip_err(e, "clear");
   if ((e = zero_u32((uint32_t*)h->d_adam_s.p, P, h->stream)) != hipSuccess) return hip_err(e, "clear");
   auto t0 = std::chrono::steady_clock::now();
+  h->dr_last_batch = p->image_batch_size;
   for (uint32_t it = 0; it < p->opt_iterations; it++) {
     double lsum = 0;
     for (uint32_t b = 0; b < p->image_batch_size; b++) {
@@ -221,7 +226,20 @@ int lrt_dr_get_last_dloss_ds(lrt_handle h, float* out, uint64_t n) {
   if (!h || !out) return set_err(LRT_E_INVALID, "null argument");
   if (!h->d_dr_grad.p || n > h->values_f.size()) return set_err(LRT_E_INVALID, "no gradient or size too large");
   hipError_t e = hipMemcpy(out, h->d_dr_grad.p, n * 4, hipMemcpyDeviceToHost);
-  return e == hipSuccess ? LRT_OK : hip_err(e, "download grad");
+  if (e != hipSuccess) return hip_err(e, "download grad");
+  // the reference clears the gradient before every image of a step and divides
+  // the last one by image_batch_size (MultiRendererDR.cpp:405, 475-476)
+  if (h->dr_last_batch > 1)
+    for (uint64_t i = 0; i < n; i++) out[i] /= (float)h->dr_last_batch;
+  return LRT_OK;
+}
+
+int lrt_dr_get_last_image(lrt_handle h, uint32_t view, float* out) {
+  if (!h || !out) return set_err(LRT_E_INVALID, "null argument");
+  if (view >= h->dr_n_ref || !h->dr_img_valid[view]) return set_err(LRT_E_INVALID, "no RenderDR image for this view");
+  const size_t np = (size_t)h->dr_w * h->dr_h;
+  hipError_t e = hipMemcpy(out, (const float4*)h->d_dr_img.p + np * view, np * 16, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? LRT_OK : hip_err(e, "download image");
 }
 
 int lrt_dr_last_stats(lrt_handle h, uint32_t* border_pixels, uint32_t* border_rays_hit) {

@@ -55,11 +55,13 @@ struct Renderer {
 
   // differentiable rendering state (MultiRendererDR)
   uint32_t dr_n_ref = 0, dr_w = 0, dr_h = 0;
+  std::vector<uint8_t> dr_img_valid;  // per reference view: its slot of d_dr_img holds the last RenderDR image
   std::vector<float> dr_world_views, dr_projs;  // 16 floats per reference view
   DevBuf d_dr_ref, d_dr_img, d_dr_thr, d_dr_grad, d_dr_list, d_dr_misc, d_adam_v, d_adam_s;
   DevBuf d_dr_bvals;    // per-brick corner values gathered from values_f before every RenderDR
   DevBuf d_dr_bminmax;  // per-brick smallest / largest gathered value
   float dr_last_loss = 0.f;
+  uint32_t dr_last_batch = 1;  // images per step of the last gradient (lrt_dr_get_last_dloss_ds divides by it)
   uint32_t dr_last_border = 0, dr_last_border_hits = 0;
   // batched render (lrt_render_batch)
   DevBuf d_b_items, d_b_offsets, d_b_t, d_b_n, d_b_prim;
