@@ -135,6 +135,43 @@ def test_render_dr_diffuse_colour_and_geometry_matches_oracle(problem):
     assert np.linalg.norm(grad - o64) / nz <= max(1e-4, 1.2 * fp32_err)
 
 
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_render_dr_on_noisy_distances_matches_fp32_twin(problem, seed):
+    """The sphere's distance values perturbed by seeded noise of up to 0.4
+    voxel (rough surfaces, thin shells, voxels whose corners straddle the
+    skip margins): image, border statistics and dLoss/dS against the FP32
+    twin, which evaluates every voxel as the reference does."""
+    tgt, ini, cams, refs = problem
+    vs = ini.header.brick_size + 2 * ini.header.brick_pad + 1
+    off = ini.nodes[:, 2].astype(np.int64)
+    dist = np.unique(ini.values[(off[:, None] + np.arange(vs ** 3)[None, :]).ravel()].astype(np.int64))
+    rng = np.random.default_rng(seed)
+    vf = ini.values_f.copy()
+    voxel = 2.0 / (8 * ini.header.brick_size)
+    vf[dist] += rng.uniform(-0.4, 0.4, dist.size).astype(np.float32) * voxel
+    noisy = L.SdfSBS(ini.header, ini.nodes, ini.values, vf)
+    dr = _preset(border_spp=64)
+    g = L.MultiRendererDR()
+    g.SetScene(noisy)
+    wvs = np.stack([c[0] for c in cams])
+    prs = np.stack([c[1] for c in cams])
+    g.SetReference(refs, wvs, prs)
+    P = vf.size
+    pairs, _ = g.GetBVH()
+    sc = oracle.Scene(noisy, pairs)
+    view = seed % len(cams)
+    ref = refs[view].copy()
+    ref[..., 3] = (np.linalg.norm(ref[..., :3], axis=-1) >= 0.001)
+    img, grad, loss, ms = g.RenderDR(dr, view, seed=40 + seed, n_params=P)
+    o_img, o32, st32 = sc.dr_render_grad(dr, wvs[view], prs[view], W, H, ref, P, seed=40 + seed, precision=32)
+    bp, bh = g.LastStats()
+    print(seed, "loss", loss, st32, "border", bp, bh)
+    assert np.mean(np.all(img == o_img.astype(np.float32), axis=-1)) > 0.999
+    assert bp == st32["border_pixels"] and abs(bh - st32["border_rays_hit"]) <= 2
+    assert abs(loss - st32["loss"]) <= 1e-5 * max(1.0, abs(st32["loss"]))
+    assert np.linalg.norm(grad - o32) / np.linalg.norm(o32) <= 1e-5
+
+
 def test_adam_step_matches_oracle():
     rng = np.random.default_rng(3)
     n = 1000
