@@ -143,11 +143,9 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const void* __restrict__ pack
 #pragma unroll
       for (int k = 0; k < 24; k++) f[k] = 0.0f;
     }
-    store_features_bf16(fl[wave] + lane * kFeatStride, f);
+    store_features_bf16(fl[wave] + lane * kFeatStride, f);  // zero rows past n
     wave_lds_sync();
-    const uint64_t left = n - base;
-    const int cnt = left < 64 ? (int)left : 64;
-    mlp_wave_forward(packed, fl[wave], ol[wave], (cnt + 15) >> 4, lane);
+    mlp_wave_forward64(packed, fl[wave], ol[wave], lane);
     wave_lds_sync();
     if (s < n) {
       out[s * 3 + 0] = ol[wave][3 * lane + 0];

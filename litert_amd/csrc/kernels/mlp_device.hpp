@@ -9,11 +9,12 @@ namespace lrtd {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// FP32 -> BF16, round to nearest even (finite inputs).
-__device__ __forceinline__ uint16_t bf16_bits(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+// Two FP32 values -> packed BF16 (lo in bits 0..15), round to nearest even:
+// one v_cvt_pk_bf16_f32 on gfx950.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+  const bf16x2 v = {(__bf16)lo, (__bf16)hi};
+  return __builtin_bit_cast(uint32_t, v);
 }
 
 // SiLU with the hardware exp2 / reciprocal: x * 1 / (1 + 2^(-x log2 e)).
@@ -87,10 +88,10 @@ __device__ __forceinline__ f32x4 load_bias(const float4* __restrict__ bias, int 
 // silu of two accumulator tiles -> the 8-half B operand of the next layer
 __device__ __forceinline__ bf16x8 act_pair(f32x4 lo, f32x4 hi) {
   uint4 u;
-  u.x = (uint32_t)bf16_bits(silu_fast(lo[0])) | ((uint32_t)bf16_bits(silu_fast(lo[1])) << 16);
-  u.y = (uint32_t)bf16_bits(silu_fast(lo[2])) | ((uint32_t)bf16_bits(silu_fast(lo[3])) << 16);
-  u.z = (uint32_t)bf16_bits(silu_fast(hi[0])) | ((uint32_t)bf16_bits(silu_fast(hi[1])) << 16);
-  u.w = (uint32_t)bf16_bits(silu_fast(hi[2])) | ((uint32_t)bf16_bits(silu_fast(hi[3])) << 16);
+  u.x = pack_bf16(silu_fast(lo[0]), silu_fast(lo[1]));
+  u.y = pack_bf16(silu_fast(lo[2]), silu_fast(lo[3]));
+  u.z = pack_bf16(silu_fast(hi[0]), silu_fast(hi[1]));
+  u.w = pack_bf16(silu_fast(hi[2]), silu_fast(hi[3]));
   return __builtin_bit_cast(bf16x8, u);
 }
 
@@ -146,16 +147,77 @@ __device__ __forceinline__ void mlp_wave_forward(const void* __restrict__ packed
   }
 }
 
+// The same forward pass for all 64 slots (4 tiles) at once: each A tile and
+// bias tile is loaded once per wave and applied to the 4 sample tiles, a
+// quarter of the weight traffic of mlp_wave_forward. Per sample the MFMA
+// operands and accumulation order are those of mlp_wave_forward, so the
+// outputs are identical. Holds 16 accumulator tiles (64 VGPRs); meant for
+// kernels with registers to spare (batch inference), not the render loop.
+__device__ __forceinline__ void mlp_wave_forward64(const void* __restrict__ packed, const uint16_t* feat_lds, float* out_lds, int lane) {
+  const int row = lane & 15, grp = lane >> 4;
+  const uint4* tiles = (const uint4*)packed;
+  const float4* bias = (const float4*)((const char*)packed + (size_t)kTilesTotal * 64 * 16);
+  f32x4 h[4][4];  // [sample tile][row tile]
+  bf16x8 b0[4], b1[4];
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) {
+    const uint4 xb = *(const uint4*)(feat_lds + (16 * nt + row) * kFeatStride + 8 * grp);
+    b0[nt] = __builtin_bit_cast(bf16x8, xb);
+  }
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+    const bf16x8 a = load_a_tile(tiles, kTileOffL1 + mt, lane);
+    const f32x4 bb = load_bias(bias, mt, lane);
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) h[nt][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0[nt], bb, 0, 0, 0);
+  }
+#pragma unroll
+  for (int layer = 0; layer < 2; layer++) {
+    const int toff = layer == 0 ? kTileOffL2 : kTileOffL3;
+    const int boff = layer == 0 ? 4 : 8;
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+      b0[nt] = act_pair(h[nt][0], h[nt][1]);
+      b1[nt] = act_pair(h[nt][2], h[nt][3]);
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+      const bf16x8 a0 = load_a_tile(tiles, toff + 2 * mt + 0, lane);
+      const bf16x8 a1 = load_a_tile(tiles, toff + 2 * mt + 1, lane);
+      const f32x4 bb = load_bias(bias, boff + mt, lane);
+#pragma unroll
+      for (int nt = 0; nt < 4; nt++) {
+        f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0[nt], bb, 0, 0, 0);
+        h[nt][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1[nt], acc, 0, 0, 0);
+      }
+    }
+  }
+  const bf16x8 a0 = load_a_tile(tiles, kTileOffL4 + 0, lane);
+  const bf16x8 a1 = load_a_tile(tiles, kTileOffL4 + 1, lane);
+  const f32x4 bb = load_bias(bias, 12, lane);
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) {
+    f32x4 o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, act_pair(h[nt][0], h[nt][1]), bb, 0, 0, 0);
+    o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, act_pair(h[nt][2], h[nt][3]), o, 0, 0, 0);
+    if (grp == 0) {
+      const int slot = 16 * nt + row;
+      out_lds[3 * slot + 0] = o[0];
+      out_lds[3 * slot + 1] = o[1];
+      out_lds[3 * slot + 2] = o[2];
+    }
+  }
+}
+
 // Write one lane's features into its LDS row as bf16 (slots 24..39 zero).
 __device__ __forceinline__ void store_features_bf16(uint16_t* row, const float f[24]) {
   uint4* r4 = (uint4*)row;
 #pragma unroll
   for (int q = 0; q < 3; q++) {
     uint4 u;
-    u.x = (uint32_t)bf16_bits(f[8 * q + 0]) | ((uint32_t)bf16_bits(f[8 * q + 1]) << 16);
-    u.y = (uint32_t)bf16_bits(f[8 * q + 2]) | ((uint32_t)bf16_bits(f[8 * q + 3]) << 16);
-    u.z = (uint32_t)bf16_bits(f[8 * q + 4]) | ((uint32_t)bf16_bits(f[8 * q + 5]) << 16);
-    u.w = (uint32_t)bf16_bits(f[8 * q + 6]) | ((uint32_t)bf16_bits(f[8 * q + 7]) << 16);
+    u.x = pack_bf16(f[8 * q + 0], f[8 * q + 1]);
+    u.y = pack_bf16(f[8 * q + 2], f[8 * q + 3]);
+    u.z = pack_bf16(f[8 * q + 4], f[8 * q + 5]);
+    u.w = pack_bf16(f[8 * q + 6], f[8 * q + 7]);
     r4[q] = u;
   }
   r4[3] = make_uint4(0, 0, 0, 0);
