@@ -87,7 +87,7 @@ __device__ __forceinline__ void scatter_pd(const DRArgs& A, const DRPayload& py,
 // second pass recomputes identical rays and hits. SPEC kDrDepth is used with
 // spp 1 only (one pass, gradient flags on).
 template <int SPEC>
-__device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, Stack& st, double& loss_acc) {
+__device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, Stack& st, float& loss_acc) {
   uint32_t x, y;
   tid_to_xy(packed_idx, A.packed_w, x, y);
   if (x >= A.W || y >= A.H) return;
@@ -120,7 +120,7 @@ __device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, S
     float df = cm[k] - rr[k];
     l += A.loss_fn == LRT_DR_LOSS_FUNCTION_MSE ? df * df : fabsf(df);
   }
-  loss_acc += (double)l;
+  loss_acc += l;
   const v3 dLdC(loss_grad1(A.loss_fn, cm[0], rr[0]) / (float)spp, loss_grad1(A.loss_fn, cm[1], rr[1]) / (float)spp,
                 loss_grad1(A.loss_fn, cm[2], rr[2]) / (float)spp);
   if (A.Q.ray_flags == 0) return;
@@ -138,7 +138,7 @@ __device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, S
 }
 
 template <int SPEC>
-__global__ __launch_bounds__(256) void k_dr_primary(DRArgs A, uint32_t* work_counter, uint32_t* spill, double* loss) {
+__global__ __launch_bounds__(256) void k_dr_primary(DRArgs A, uint32_t* work_counter, uint32_t* spill, float* loss) {
   __shared__ uint32_t lds_stack[kStackLds * 256];
   __shared__ uint32_t chunk_base;
   Stack st;
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void k_dr_primary(DRArgs A, uint32_t* work_cou
   st.lds_entries = kStackLds;
   st.spill = spill + blockIdx.x * 256 + threadIdx.x;
   st.spill_stride = gridDim.x * 256;
-  double acc = 0.0;
+  float acc = 0.0f;
   const uint32_t total = A.W * A.H;  // the reference walks the first W*H packed indices
   for (;;) {
     if (threadIdx.x == 0) chunk_base = atomicAdd(work_counter, 1u);
@@ -158,7 +158,9 @@ __global__ __launch_bounds__(256) void k_dr_primary(DRArgs A, uint32_t* work_cou
     const uint64_t i = base + threadIdx.x;
     if (i < total) dr_pixel<SPEC>(A, (uint32_t)i, st, acc);
   }
-  if (acc != 0.0) atomicAdd(loss, acc);
+  // loss: per-lane partial sums, a wave sum, one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0 && acc != 0.0f) atomicAdd(loss, acc);
 }
 
 // Phase II (MultiRendererDR.cpp:609-685)
@@ -274,19 +276,20 @@ __global__ void k_dr_scale(float* grad, uint32_t n, float mult) {
   if (i < n) grad[i] *= mult;
 }
 
-// MultiRendererDR.cpp:828-853; c1 = 1 - beta1^(iter+1), c2 = 1 - beta2^(iter+1)
-// in double as the reference evaluates them.
+// MultiRendererDR.cpp:828-853; c1 = 1 - beta1^(iter+1), c2 = 1 - beta2^(iter+1).
+// The reference divides by them in double and rounds to float; here the
+// division is FP32 (the corrected moments may differ by one ulp).
 __global__ void k_adam(const float* __restrict__ g, float* X, float* V, float* Sq, uint32_t n, float lr, float b1, float b2, float eps,
-                       double c1, double c2, float batch) {
+                       float c1, float c2, float batch) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float gi = g[i] / batch;  // m_dLoss_dS_tmp[j] /= image_batch_size, MultiRendererDR.cpp:474-475
   float v = b1 * V[i] + (1 - b1) * gi;
   V[i] = v;
-  float vh = (float)((double)v / c1);
+  float vh = v / c1;
   float s = b2 * Sq[i] + (1 - b2) * gi * gi;
   Sq[i] = s;
-  float sh = (float)((double)s / c2);
+  float sh = s / c2;
   float x = X[i] - lr * vh / (sqrtf(sh) + eps);
   X[i] = fminf(fmaxf(x, -1000.0f), 1000.0f);
 }
@@ -298,7 +301,7 @@ namespace lrt {
 // kDrDepth: linear-depth reconstruction (distance gradients only), analytic
 // dt/dv, one sample per pixel, a render mode whose colour is not the diffuse
 // colour. Everything else runs the generic form.
-hipError_t launch_dr_primary(const lrtd::DRArgs& A, uint32_t* d_counter, uint32_t* d_spill, double* d_loss, uint32_t blocks,
+hipError_t launch_dr_primary(const lrtd::DRArgs& A, uint32_t* d_counter, uint32_t* d_spill, float* d_loss, uint32_t blocks,
                              hipStream_t s) {
   const bool depth = A.Q.ray_flags == LRT_DR_RAY_FLAG_DDIST_DPOS && A.Q.dtdv_fd == 0 && A.spp == 1 &&
                      A.mode != LRT_DR_RENDER_MODE_DIFFUSE;
@@ -333,8 +336,8 @@ hipError_t launch_dr_scale(float* d_grad, uint32_t n, float mult, hipStream_t s)
   hipLaunchKernelGGL(lrtd::k_dr_scale, dim3((n + 255) / 256), dim3(256), 0, s, d_grad, n, mult);
   return hipGetLastError();
 }
-hipError_t launch_adam(const float* g, float* X, float* V, float* S, uint32_t n, float lr, float b1, float b2, float eps, double c1,
-                       double c2, float batch, hipStream_t s) {
+hipError_t launch_adam(const float* g, float* X, float* V, float* S, uint32_t n, float lr, float b1, float b2, float eps, float c1,
+                       float c2, float batch, hipStream_t s) {
   hipLaunchKernelGGL(lrtd::k_adam, dim3((n + 255) / 256), dim3(256), 0, s, g, X, V, S, n, lr, b1, b2, eps, c1, c2, batch);
   return hipGetLastError();
 }

@@ -29,12 +29,12 @@ struct DRArgs {
 }  // namespace lrtd
 
 namespace lrt {
-hipError_t launch_dr_primary(const lrtd::DRArgs& A, uint32_t* d_counter, uint32_t* d_spill, double* d_loss, uint32_t blocks, hipStream_t s);
+hipError_t launch_dr_primary(const lrtd::DRArgs& A, uint32_t* d_counter, uint32_t* d_spill, float* d_loss, uint32_t blocks, hipStream_t s);
 hipError_t launch_dr_border_find(const lrtd::DRArgs& A, uint32_t* d_list, uint32_t* d_count, hipStream_t s);
 hipError_t launch_dr_border_rays(const lrtd::DRArgs& A, const uint32_t* d_list, const uint32_t* d_count, uint32_t* d_spill,
                                  uint32_t* d_hits, uint32_t blocks, hipStream_t s);
 hipError_t launch_dr_gather(const lrtd::DevScene& S, float* d_out, float2* d_minmax, hipStream_t s);
 hipError_t launch_dr_scale(float* d_grad, uint32_t n, float mult, hipStream_t s);
-hipError_t launch_adam(const float* g, float* X, float* V, float* S, uint32_t n, float lr, float b1, float b2, float eps, double c1,
-                       double c2, float batch, hipStream_t s);
+hipError_t launch_adam(const float* g, float* X, float* V, float* S, uint32_t n, float lr, float b1, float b2, float eps, float c1,
+                       float c2, float batch, hipStream_t s);
 }  // namespace lrt

@@ -56,11 +56,23 @@ __device__ __forceinline__ float4 shade(uint32_t mode, const Hit& hit, v3 dir) {
   }
 }
 
+// floor(sqrt(n)), exact for every uint32 n
+__device__ __forceinline__ uint32_t isqrt_u32(uint32_t n) {
+  uint64_t r = (uint64_t)sqrtf((float)n);
+  while (r * r > n) r--;
+  while ((r + 1) * (r + 1) <= n) r++;
+  return (uint32_t)r;
+}
+
+// The reference computes uint(sqrt(spp)) and i / spp_sqrt + 0.5 in double;
+// both are exact in the integer / FP32 forms here (k + 0.5 is exact in float
+// for k < 2^23, and the float product of two floats rounds the exact product
+// once, as the double product converted to float does).
 __device__ __forceinline__ float2 sample_offset(const RenderArgs& A, uint32_t x, uint32_t y, uint32_t i) {
   if (A.ray_gen_mode == LRT_RAY_GEN_MODE_RANDOM) return rand2(x, y, i + A.seed % 2, A.seed);
-  const uint32_t spp_sqrt = (uint32_t)sqrt((double)A.spp);
+  const uint32_t spp_sqrt = isqrt_u32(A.spp);
   const float is = 1.0f / (float)spp_sqrt;
-  return make_float2(is * (float)((double)(i / spp_sqrt) + 0.5), is * (float)((double)(i % spp_sqrt) + 0.5));
+  return make_float2(is * ((float)(i / spp_sqrt) + 0.5f), is * ((float)(i % spp_sqrt) + 0.5f));
 }
 
 }  // namespace lrtd

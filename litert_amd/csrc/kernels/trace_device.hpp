@@ -392,7 +392,9 @@ __device__ __forceinline__ bool LocalSurfaceIntersection(const TraceParams& P, v
       }
     } else if (mode == LRT_SDF_NODE_INTERSECT_ANALYTIC) {
       float x1 = 1000, x2 = 1000, x3 = 1000;
-      if ((double)fabsf(c3) > 1e-2) {
+      // The reference compares float against double constants; 1e-2 and 1e-4
+      // round down to float, so "> 1e-2f" is the same test in FP32.
+      if (fabsf(c3) > 1e-2f) {
         float A_ = c2 / c3, B_ = c1 / c3, C_ = c0 / c3;
         float Q = (A_ * A_ - 3 * B_) / 9;
         float R = (2 * A_ * A_ - 9 * A_ * B_ + 27 * C_) / 54;
@@ -400,14 +402,16 @@ __device__ __forceinline__ bool LocalSurfaceIntersection(const TraceParams& P, v
         if (R * R < Q3) {
           float theta = acosf(R / sqrtf(Q3));
           x1 = -2 * sqrtf(Q) * cosf(theta / 3) - A_ / 3;
-          x2 = (float)(-2 * sqrtf(Q) * cos(((double)theta + 2 * M_PI) / 3) - A_ / 3);
-          x3 = (float)(-2 * sqrtf(Q) * cos(((double)theta - 2 * M_PI) / 3) - A_ / 3);
+          // the reference evaluates these two in double (M_PI promotes them); FP32 here
+          const float two_pi = 6.28318530717958647692f;
+          x2 = -2 * sqrtf(Q) * cosf((theta + two_pi) / 3) - A_ / 3;
+          x3 = -2 * sqrtf(Q) * cosf((theta - two_pi) / 3) - A_ / 3;
         } else {
           float Am = -sgn(R) * powf(fabsf(R) + sqrtf(R * R - Q3), 1.0f / 3.0f);
           float Bm = fabsf(Am) > EPS ? Q / Am : 0;
           x1 = Am + Bm - A_ / 3;
         }
-      } else if ((double)fabsf(c2) > 1e-4) {
+      } else if (fabsf(c2) > 1e-4f) {
         float D = c1 * c1 - 4 * c2 * c0;
         if (D > 0) {
           float q = -0.5f * (c1 + sgn(c1) * sqrtf(D));

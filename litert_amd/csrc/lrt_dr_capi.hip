@@ -68,7 +68,7 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
   if ((e = r->d_dr_bminmax.ensure((size_t)r->nodes.size() * 8)) != hipSuccess) return hip_err(e, "alloc");
   const uint32_t blocks = r->grid_blocks();
   if ((e = r->ensure_spill(blocks)) != hipSuccess) return hip_err(e, "alloc spill");
-  uint32_t* misc = (uint32_t*)r->d_dr_misc.p;  // [0] work counter [1] border count [2] border hits [4..5] loss (double)
+  uint32_t* misc = (uint32_t*)r->d_dr_misc.p;  // [0] work counter [1] border count [2] border hits [4] loss (float)
   if ((e = zero_u32(misc, 16, r->stream)) != hipSuccess) return hip_err(e, "clear");
   if ((e = zero_u32((uint32_t*)r->d_dr_grad.p, P, r->stream)) != hipSuccess) return hip_err(e, "clear grad");
 
@@ -105,7 +105,7 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
 
   if ((e = hipEventRecord(r->ev0, r->stream)) != hipSuccess) return hip_err(e, "event");
   if ((e = launch_dr_gather(A.S, (float*)r->d_dr_bvals.p, (float2*)r->d_dr_bminmax.p, r->stream)) != hipSuccess) return hip_err(e, "dr gather");
-  if ((e = launch_dr_primary(A, misc, (uint32_t*)r->d_spill.p, (double*)(misc + 4), blocks, r->stream)) != hipSuccess)
+  if ((e = launch_dr_primary(A, misc, (uint32_t*)r->d_spill.p, (float*)(misc + 4), blocks, r->stream)) != hipSuccess)
     return hip_err(e, "dr primary");
   if ((e = launch_dr_border_find(A, (uint32_t*)r->d_dr_list.p, misc + 1, r->stream)) != hipSuccess) return hip_err(e, "dr border");
   if ((p.dr_reconstruction_flags & LRT_DR_RECONSTRUCTION_FLAG_GEOMETRY) && p.border_spp > 0) {
@@ -119,9 +119,9 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
   uint32_t h_misc[6];
   if ((e = hipMemcpyAsync(h_misc, misc, sizeof(h_misc), hipMemcpyDeviceToHost, r->stream)) != hipSuccess) return hip_err(e, "stats");
   if ((e = hipStreamSynchronize(r->stream)) != hipSuccess) return hip_err(e, "RenderDR");
-  double loss;
-  std::memcpy(&loss, &h_misc[4], sizeof(double));
-  r->dr_last_loss = (float)(loss / (double)np);
+  float loss;
+  std::memcpy(&loss, &h_misc[4], sizeof(float));
+  r->dr_last_loss = (float)((double)loss / (double)np);
   r->dr_last_border = h_misc[1];
   r->dr_img_valid[view] = 1;
   r->dr_last_border_hits = h_misc[2];
@@ -207,8 +207,8 @@ int lrt_dr_optimize(lrt_handle h, const MultiRendererDRPreset* p, const SdfSBSHe
       lsum += h->dr_last_loss;
     }
     if (losses) losses[it] = (float)(lsum / p->image_batch_size);
-    const double c1 = 1.0 - std::pow((double)p->opt_beta_1, (double)(it + 1));
-    const double c2 = 1.0 - std::pow((double)p->opt_beta_2, (double)(it + 1));
+    const float c1 = (float)(1.0 - std::pow((double)p->opt_beta_1, (double)(it + 1)));
+    const float c2 = (float)(1.0 - std::pow((double)p->opt_beta_2, (double)(it + 1)));
     e = launch_adam((const float*)h->d_dr_grad.p, (float*)h->d_values_f.p, (float*)h->d_adam_v.p, (float*)h->d_adam_s.p, P, p->opt_lr,
                     p->opt_beta_1, p->opt_beta_2, p->opt_eps, c1, c2, (float)p->image_batch_size, h->stream);
     if (e != hipSuccess) return hip_err(e, "adam");
