@@ -38,10 +38,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic():
+def measured_traffic(lod):
     """HBM bytes per plain-render launch from the FETCH_SIZE / WRITE_SIZE counter
-    passes (scaled by a 1 GiB copy), recorded in profiles/k_render_ico512_hbm.json."""
-    path = os.path.join(ROOT, "profiles", "k_render_ico512_hbm.json")
+    passes (scaled by a 1 GiB copy), recorded per scene size in
+    profiles/k_render_ico<4*lod>_hbm.json (None when no pass was recorded)."""
+    path = os.path.join(ROOT, "profiles", f"k_render_ico{4 * lod}_hbm.json")
     try:
         with open(path) as f:
             return json.load(f)["traffic_bytes"]
@@ -49,9 +50,12 @@ def measured_traffic():
         return None
 
 
-def plain_scene(L):
-    return L.displaced_icosphere(5, 0.8, 0.05, 4, seed=SCENE_SEED, lod=128, brick_size=4, brick_pad=0, bytes_per_value=1,
-                                 voxel_diag_scale=True)
+def plain_scene(L, lod=128):
+    """The displaced icosphere (4 noise octaves, 8-bit distances over the voxel
+    diagonal) at brick LOD `lod` x 4^3 voxels; 5 subdivisions up to LOD 128,
+    6 above so the mesh stays finer than the voxels."""
+    return L.displaced_icosphere(5 if lod <= 128 else 6, 0.8, 0.05, 4, seed=SCENE_SEED, lod=lod, brick_size=4, brick_pad=0,
+                                 bytes_per_value=1, voxel_diag_scale=True)
 
 
 def dr_problem(L, np, views=16, res=512, brick_count=32, brick_size=4):
@@ -242,6 +246,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-dr", action="store_true", help="skip the DR leg")
     ap.add_argument("--no-mlp", action="store_true", help="skip the learned-seeder leg")
+    ap.add_argument("--lod", type=int, default=128,
+                    help="brick LOD of the plain workload's scene (128: 512^3 effective, the headline workload)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched many-scene leg")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--mlp-scenes", type=int, default=32)
@@ -259,7 +265,7 @@ def main():
     import litert_amd.api as L
 
     t0 = time.time()
-    sbs = plain_scene(L)
+    sbs = plain_scene(L, args.lod)
     log(f"scene: {sbs.nodes.shape[0]} bricks, {sbs.values.size * 4 / 1e6:.1f} MB values, built in {time.time() - t0:.1f}s")
     r = L.CreateMultiRenderer(L.DEVICE_GPU)
     r.SetScene(sbs)
@@ -392,11 +398,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded displaced icosphere SBS, seeded orbit cameras)",
-        "config": {"workload": "displaced icosphere SBS 512^3 effective (LOD 128 x 4^3 bricks, 8-bit), 8 views x 2048x2048 primary rays, Newton",
+        "config": {"workload": f"displaced icosphere SBS {4 * args.lod}^3 effective (LOD {args.lod} x 4^3 bricks, 8-bit), "
+                               "8 views x 2048x2048 primary rays, Newton",
                    "bricks": int(sbs.nodes.shape[0]), "rays_per_step": rays, "parallelism": "single GPU"},
         "kernel_ms_per_step": round(avg_kernel_ms, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.lod),
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "mlp_seeded_mrays_s": mlp_mrays,
