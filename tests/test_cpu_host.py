@@ -105,3 +105,12 @@ def test_displaced_torus_builder():
     q = np.sqrt(np.hypot(c[:, 0], c[:, 1]) ** 2) - 0.6
     d = np.abs(np.hypot(q, c[:, 2]) - 0.2)
     assert d.max() < 0.02 + 2 * np.sqrt(3) / 16 + np.sqrt(3) / 16
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asking for more exits with a message (before any GPU work)."""
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
