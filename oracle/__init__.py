@@ -7,6 +7,11 @@ baseline; the product library never calls it.
 The reference itself cannot be compiled here (its HydraCore3 submodule --
 LiteMath, CrossRT -- is empty and the Embree runtime is missing), so there is
 no oracle/_ref build; see lrt_oracle.h for the restated functions.
+
+Parity unpinned (restated as written, no independent check; details in
+lrt_oracle.h): LiteMath's camera/matrix conventions, Embree's BVH topology,
+the ANALYTIC cubic's R formula, the smoothed-normal blend weights of _IN
+layouts, and the DR border sampling's statistics beyond seeded determinism.
 """
 import ctypes as C
 import os

@@ -9,6 +9,20 @@
  * links or calls it.
  *
  * precision: 32 = FP32 twin of the reference, 64 = FP64 error reference.
+ *
+ * Parity unpinned (no check independent of the reference's source text fixes
+ * these; they restate it as written):
+ *  - LiteMath's perspectiveMatrix / lookAt / inverse4x4 conventions (the
+ *    camera helpers use the GL convention and a double-cofactor inverse);
+ *  - Embree's BVH topology: the oracle traverses the BVH it is given, so which
+ *    of two bricks hit along one ray "stop at first hit" returns follows that
+ *    tree (bounded by the brute-force nearest-hit tests);
+ *  - the ANALYTIC cubic's R = (2a^2 - 9ab + 27c) / 54 (BVH2Common.cpp:383, not
+ *    the textbook 2a^3): range-checked and compared with the other root
+ *    finders on smooth scenes only;
+ *  - the smoothed-normal blend weights of ID32F_IRGB32F_IN scenes beyond the
+ *    code path itself;
+ *  - the DR border sampling's statistics beyond seeded determinism.
  */
 #ifndef LRT_ORACLE_H
 #define LRT_ORACLE_H
