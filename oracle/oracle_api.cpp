@@ -166,6 +166,8 @@ extern "C" int orc_brute_nearest(const orc_scene* s, const MultiRenderPreset* pr
 }
 
 // ---- unit-level -----------------------------------------------------------
+// wrappers of eval_dist_trilinear / eval_dist_trilinear_diff
+// (BVH2Common.cpp:62-72, 657-687) for the closed-form tests
 
 template <typename T> static T tri(const double v[8], const double dp[3]) {
   T vv[8];

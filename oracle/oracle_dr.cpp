@@ -165,6 +165,7 @@ template <typename T> struct DRTracer {
     }
   }
 
+  // trilinear colour of the hit brick, BVH2Common.cpp:962-982 / BVH2DR.cpp:1489-1524
   static V3<T> color_lerp(const V3<T> c[8], V3<T> dp) {
     V3<T> r(0, 0, 0);
     for (int i = 0; i < 8; i++) {

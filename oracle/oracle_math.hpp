@@ -56,6 +56,8 @@ template <typename T> static inline V3<T> vabs(V3<T> a) { return {std::abs(a.x),
 // 4x4 column-major: m[c*4 + r]
 template <typename T> struct M4 { T m[16]; };
 
+// LiteMath float4x4 * float4 as the camera code uses it (render_common.h:45-55,
+// eye_ray.cpp:146-160); LiteMath itself is absent, its convention is unpinned
 template <typename T> static inline V4<T> mul(const M4<T>& M, V4<T> v) {
   const T* m = M.m;
   V4<T> r;
@@ -71,7 +73,9 @@ template <typename T> static inline V3<T> mul4x3(const M4<T>& M, V3<T> v) {
 }
 
 // General 4x4 inverse by cofactor expansion, evaluated in double and rounded
-// once to float, so every consumer of the same matrix sees the same bits.
+// once to float, so every consumer of the same matrix sees the same bits. It
+// stands for LiteMath's inverse4x4 in UpdateCamera (eye_ray_host.cpp:59-65),
+// whose own arithmetic is unpinned.
 static inline void inverse4x4_via_double(const float in[16], float out[16]) {
   double m[16], inv[16];
   for (int i = 0; i < 16; i++) m[i] = in[i];

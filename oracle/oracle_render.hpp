@@ -58,6 +58,7 @@ template <typename T> static inline V3<T> SafeInverse(V3<T> d) {
   return res;
 }
 
+// raytrace_common.h:59-82 (slab test with the precomputed inverse direction)
 template <typename T> static inline V2<T> RayBoxIntersection2(V3<T> o, V3<T> inv, V3<T> bmin, V3<T> bmax) {
   const T lo = inv.x * (bmin.x - o.x);
   const T hi = inv.x * (bmax.x - o.x);
@@ -547,6 +548,7 @@ static inline void pcg3d(uint32_t& x, uint32_t& y, uint32_t& z) {
   x ^= x >> 16u; y ^= y >> 16u; z ^= z >> 16u;
   x += y * z; y += z * x; z += x * y;
 }
+// eye_ray.cpp:34-81 (PCG3D keyed by pixel, sample and seed)
 static inline V2<float> rand2(uint32_t x, uint32_t y, uint32_t iter, uint32_t seed) {
   x = x + 1233u * (iter + seed) % 171u;
   y = y + 453u * (iter + seed) % 765u;
