@@ -103,7 +103,7 @@ __device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, S
     float2 d = rand2(x, y, s, A.seed);
     v3 pos, dir;
     eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
-    DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, A.bminmax, Qc, pos, dir, 1e9f, &py, st);
+    DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, A.bminmax, Qc, A.relax_eps, pos, dir, 1e9f, &py, st);
     if (hit.primId == 0xFFFFFFFFu) continue;
     v3 c = dr_color(A.mode, hit);
     res = res + v3(c.x / (float)spp, c.y / (float)spp, c.z / (float)spp);
@@ -132,7 +132,7 @@ __device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, S
     float2 d = rand2(x, y, s, A.seed);
     v3 pos, dir;
     eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
-    DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, A.bminmax, Qg, pos, dir, 1e9f, &py, st);
+    DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, A.bminmax, Qg, A.relax_eps, pos, dir, 1e9f, &py, st);
     if (hit.primId != 0xFFFFFFFFu) scatter_pd<SPEC>(A, py, dLdC);
   }
 }
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(256) void k_dr_border_rays(DRArgs A, const uint32_t
     float2 d = rand2(x, y, s, A.seed);
     v3 pos, dir;
     eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
-    DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, A.bminmax, Qb, pos, dir, 1e9f, &py, st);
+    DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, A.bminmax, Qb, A.relax_eps, pos, dir, 1e9f, &py, st);
     if (!(py.missed_hit.sdf < A.relax_eps)) continue;
     local_hits++;
     float delta[4];

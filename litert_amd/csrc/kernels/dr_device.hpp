@@ -143,8 +143,8 @@ __device__ __forceinline__ float corner_w(int i, v3 q) {
 // BVH2DR.cpp:1130-1610 for ID32F layouts without normal derivatives.
 template <int SPEC>
 __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restrict__ bvals, const float2* __restrict__ bminmax,
-                                         const DRParams& Q, v3 ray_pos, v3 ray_dir, v3 inv, float tNear, uint32_t brick, DRPayload* relax,
-                                         DRHit* pHit) {
+                                         const DRParams& Q, float relax_eps, v3 ray_pos, v3 ray_dir, v3 inv, float tNear, uint32_t brick,
+                                         DRPayload* relax, DRHit* pHit) {
   constexpr bool kColour = SPEC == kDrGeneric;
   float values[8];
   const uint4 N = S.nodes[brick];
@@ -169,7 +169,8 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restr
     const float2 mm = bminmax[brick];
     bool skip = mm.x > 1e-6f && mm.x > 1e-3f * mm.y;
     if (SPEC == kDrDepthBorder) {
-      skip = skip && mm.x > relax->missed_hit.sdf + 1e-3f * mm.y && (!(bnf.x < bnf.y) || bnf.x > tNear + 1e-5f);
+      skip = skip && (mm.x > relax->missed_hit.sdf + 1e-3f * mm.y || mm.x > relax_eps + 1e-3f * mm.y) &&
+             (!(bnf.x < bnf.y) || bnf.x > tNear + 1e-5f);
       if (skip && bnf.x < bnf.y) {
         relax->missed_cand.sdf = 1000.f;
         relax->missed_hit.check_vox_border = 0;
@@ -213,9 +214,17 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restr
       if (SPEC == kDrDepthBorder) {
         // Border rays also track the smallest distance along the ray
         // (missed_hit.sdf). Every update in dr_intersect needs a distance
-        // below it, and none along this segment is (same bound), so the
-        // voxel's only effects are the two state resets done here.
-        skip = skip && vmin > relax->missed_hit.sdf + 1e-3f * vmax;
+        // below it; if none along this segment is (same bound), the voxel's
+        // only effects are the two state resets done here. A border ray also
+        // contributes only if that minimum ends below relax_eps, and the
+        // state that decides it (point, voxel, derivatives) is set by the
+        // update that first goes below relax_eps. A voxel whose distances all
+        // exceed relax_eps (same margin) offers only values above it, so it is
+        // skipped as well; the tracked minimum then differs only above
+        // relax_eps, which can change the outcome only where a distance the
+        // next voxel recomputes at the shared face lands within FP32 rounding
+        // of relax_eps.
+        skip = skip && (vmin > relax->missed_hit.sdf + 1e-3f * vmax || vmin > relax_eps + 1e-3f * vmax);
         if (skip) {
           relax->missed_cand.sdf = 1000.f;
           relax->missed_hit.check_vox_border = 0;
@@ -327,7 +336,7 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restr
 // BVH2DR.cpp:12-167 (stop at the first hit, forced in the reference)
 template <int SPEC>
 __device__ __forceinline__ DRHit dr_nearest_hit(const DevScene& S, const float* __restrict__ bvals, const float2* __restrict__ bminmax,
-                                                const DRParams& Q, v3 pos, v3 dir, float tFar, DRPayload* relax,
+                                                const DRParams& Q, float relax_eps, v3 pos, v3 dir, float tFar, DRPayload* relax,
                                                 Stack& st) {
   if (tFar <= 0.0f) tFar *= -1.0f;
   const v3 inv = SafeInverse(dir);
@@ -378,7 +387,7 @@ __device__ __forceinline__ DRHit dr_nearest_hit(const DevScene& S, const float* 
           left = st.get(top > 0 ? top : 0);
         }
       }
-      if (top >= 0 && left != 0xFFFFFFFFu) dr_brick<SPEC>(S, bvals, bminmax, Q, pos, dir, inv, mx(tNear, 0.1f), left & LRT_START_MASK, relax, &hit);
+      if (top >= 0 && left != 0xFFFFFFFFu) dr_brick<SPEC>(S, bvals, bminmax, Q, relax_eps, pos, dir, inv, mx(tNear, 0.1f), left & LRT_START_MASK, relax, &hit);
       top--;
       left = st.get(top > 0 ? top : 0);
     }
