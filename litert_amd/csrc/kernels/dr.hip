@@ -248,27 +248,96 @@ __global__ __launch_bounds__(256) void k_dr_border_rays(DRArgs A, const uint32_t
 // against values_f when the scene was set). One dependent load per corner in
 // the ray casts instead of two; refreshed before every RenderDR, so it always
 // equals the current parameters.
-__global__ void k_dr_gather(DevScene S, float* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_dr_gather(DevScene S, float* __restrict__ out, float2* __restrict__ minmax) {
+  // one wavefront per brick; its smallest and largest value are folded with
+  // fminf / fmaxf from the brick's first value, as a sequential scan would
+  const uint32_t b = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (b >= S.n_nodes) return;  // uniform per wavefront
   const uint32_t nv = S.v_size * S.v_size * S.v_size;
-  const uint64_t total = (uint64_t)S.n_nodes * nv;
-  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t b = (uint32_t)(g / nv), k = (uint32_t)(g - (uint64_t)b * nv);
-    out[g] = S.values_f[S.values[S.nodes[b].z + k]];
+  const uint32_t* idx = S.values + S.nodes[b].z;
+  float* o = out + (size_t)b * nv;
+  const float v0 = S.values_f[idx[0]];
+  float lo = v0, hi = v0;
+  for (uint32_t k = lane; k < nv; k += 64) {
+    const float v = S.values_f[idx[k]];
+    o[k] = v;
+    lo = fminf(lo, v);
+    hi = fmaxf(hi, v);
+  }
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+    lo = fminf(lo, __shfl_xor(lo, m, 64));
+    hi = fmaxf(hi, __shfl_xor(hi, m, 64));
+  }
+  if (lane == 0) minmax[b] = make_float2(lo, hi);
+}
+
+// Copies of the BVH for the linear-depth passes.
+//
+// The primary pass skips, without any effect, every brick whose gathered
+// corner values all exceed 1e-6 and 1e-3 of their maximum (dr_brick), so a
+// subtree holding only such bricks need not be entered: its slot (one child
+// of a node pair) gets a box at the point 1e30, which no ray reaches within
+// its tFar. The traversal then visits the bricks it does not skip in the same
+// order as over the full tree, and no others.
+//
+// The border pass skips in addition every brick whose values all exceed
+// relax_eps by the same margin, but such a skip clears two parts of the
+// ray's state when the ray crosses the brick. A subtree of such bricks whose
+// bricks tile its box exactly (tiled[slot], computed on the host) keeps its
+// box and gets the marker kDrCulledSlot in its escapeIndex: the traversal
+// treats it as one leaf that applies the clearing if the ray crosses the box
+// (past the bricks' tNear), in the order the subtree would have been visited.
+//
+// k_dr_cull_mark marks, for each pass, every slot on the path from each brick
+// that pass does not skip up to the root; k_dr_cull_write writes both copies.
+__global__ void k_dr_cull_clear(uint32_t* __restrict__ occ_p, uint32_t* __restrict__ occ_b, const uint8_t* __restrict__ tiled,
+                                uint32_t n_slots) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_slots) return;
+  occ_p[s] = 0u;
+  occ_b[s] = tiled[s] ? 0u : 1u;  // a slot whose bricks do not tile its box is never culled
+}
+
+__device__ __forceinline__ void cull_mark_path(uint32_t* occ, const uint32_t* __restrict__ pair_slot, uint32_t s) {
+  // a parent pair precedes its children (checked on the host), so the walk
+  // ends at the root within the tree depth; the bound is a second guard
+  for (int it = 0; it < 256 && s != 0xFFFFFFFFu; it++) {
+    if (atomicOr(&occ[s], 1u) != 0u) break;  // the path above is marked or being marked
+    s = pair_slot[s >> 1];
   }
 }
 
-// Smallest and largest gathered corner value of every brick.
-__global__ void k_dr_brick_minmax(DevScene S, const float* __restrict__ bvals, float2* __restrict__ out) {
+__global__ void k_dr_cull_mark(const float2* __restrict__ bminmax, const uint32_t* __restrict__ leaf_slot,
+                               const uint32_t* __restrict__ pair_slot, uint32_t* occ_p, uint32_t* occ_b, float relax_eps,
+                               uint32_t n_bricks) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= S.n_nodes) return;
-  const uint32_t nv = S.v_size * S.v_size * S.v_size;
-  const float* v = bvals + (size_t)b * nv;
-  float lo = v[0], hi = v[0];
-  for (uint32_t k = 1; k < nv; k++) {
-    lo = fminf(lo, v[k]);
-    hi = fmaxf(hi, v[k]);
+  if (b >= n_bricks) return;
+  const float2 mm = bminmax[b];
+  const bool skip_p = mm.x > 1e-6f && mm.x > 1e-3f * mm.y;
+  const bool skip_b = skip_p && mm.x > relax_eps + 1e-3f * mm.y;
+  const uint32_t s = leaf_slot[b];
+  if (!skip_p) cull_mark_path(occ_p, pair_slot, s);
+  if (!skip_b) cull_mark_path(occ_b, pair_slot, s);
+}
+
+__global__ void k_dr_cull_write(const uint4* __restrict__ pairs, const uint32_t* __restrict__ occ_p, const uint32_t* __restrict__ occ_b,
+                                uint4* __restrict__ out_p, uint4* __restrict__ out_b, uint32_t n_slots) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_slots) return;
+  const uint4 lo = pairs[2 * s], hi = pairs[2 * s + 1];  // {boxMin, leftOffset}, {boxMax, escapeIndex}
+  uint4 plo = lo, phi = hi;
+  if (occ_p[s] == 0u) {
+    const uint32_t far = __float_as_uint(1e30f);
+    plo.x = plo.y = plo.z = far;
+    phi.x = phi.y = phi.z = far;
   }
-  out[b] = make_float2(lo, hi);
+  out_p[2 * s] = plo;
+  out_p[2 * s + 1] = phi;
+  uint4 bhi = hi;
+  bhi.w = occ_b[s] == 0u ? kDrCulledSlot : 0u;
+  out_b[2 * s] = lo;
+  out_b[2 * s + 1] = bhi;
 }
 
 __global__ void k_dr_scale(float* grad, uint32_t n, float mult) {
@@ -301,13 +370,28 @@ namespace lrt {
 // kDrDepth: linear-depth reconstruction (distance gradients only), analytic
 // dt/dv, one sample per pixel, a render mode whose colour is not the diffuse
 // colour. Everything else runs the generic form.
+bool dr_primary_is_depth(const lrtd::DRArgs& A) {
+  return A.Q.ray_flags == LRT_DR_RAY_FLAG_DDIST_DPOS && A.Q.dtdv_fd == 0 && A.spp == 1 && A.mode != LRT_DR_RENDER_MODE_DIFFUSE;
+}
+bool dr_border_is_depth(const lrtd::DRArgs& A) { return A.input_type == LRT_DR_INPUT_TYPE_LINEAR_DEPTH; }
+hipError_t launch_dr_cull(const DRCullTables& T, const float2* d_minmax, float relax_eps, const uint4* d_pairs, uint32_t n_bricks,
+                          uint32_t n_pairs, hipStream_t s) {
+  const uint32_t n_slots = 2 * n_pairs;
+  uint32_t* occ_b = T.occ + n_slots;
+  const dim3 gs((n_slots + 255) / 256), gb((n_bricks + 255) / 256);
+  hipLaunchKernelGGL(lrtd::k_dr_cull_clear, gs, dim3(256), 0, s, T.occ, occ_b, T.tiled, n_slots);
+  hipLaunchKernelGGL(lrtd::k_dr_cull_mark, gb, dim3(256), 0, s, d_minmax, T.leaf_slot, T.pair_slot, T.occ, occ_b, relax_eps, n_bricks);
+  hipLaunchKernelGGL(lrtd::k_dr_cull_write, gs, dim3(256), 0, s, d_pairs, (const uint32_t*)T.occ, (const uint32_t*)occ_b, T.pairs_primary,
+                     T.pairs_border, n_slots);
+  return hipGetLastError();
+}
 hipError_t launch_dr_primary(const lrtd::DRArgs& A, uint32_t* d_counter, uint32_t* d_spill, float* d_loss, uint32_t blocks,
-                             hipStream_t s) {
-  const bool depth = A.Q.ray_flags == LRT_DR_RAY_FLAG_DDIST_DPOS && A.Q.dtdv_fd == 0 && A.spp == 1 &&
-                     A.mode != LRT_DR_RENDER_MODE_DIFFUSE;
-  if (depth)
-    hipLaunchKernelGGL(lrtd::k_dr_primary<lrtd::kDrDepth>, dim3(blocks), dim3(256), 0, s, A, d_counter, d_spill, d_loss);
-  else
+                             hipStream_t s, const uint4* cull_pairs) {
+  if (dr_primary_is_depth(A)) {
+    lrtd::DRArgs Ad = A;
+    if (cull_pairs) Ad.S.pairs = cull_pairs;
+    hipLaunchKernelGGL(lrtd::k_dr_primary<lrtd::kDrDepth>, dim3(blocks), dim3(256), 0, s, Ad, d_counter, d_spill, d_loss);
+  } else
     hipLaunchKernelGGL(lrtd::k_dr_primary<lrtd::kDrGeneric>, dim3(blocks), dim3(256), 0, s, A, d_counter, d_spill, d_loss);
   return hipGetLastError();
 }
@@ -317,19 +401,18 @@ hipError_t launch_dr_border_find(const lrtd::DRArgs& A, uint32_t* d_list, uint32
   return hipGetLastError();
 }
 hipError_t launch_dr_border_rays(const lrtd::DRArgs& A, const uint32_t* d_list, const uint32_t* d_count, uint32_t* d_spill,
-                                 uint32_t* d_hits, uint32_t blocks, hipStream_t s) {
+                                 uint32_t* d_hits, uint32_t blocks, hipStream_t s, const uint4* cull_pairs) {
   // linear-depth input reads only depths and hit flags of border rays
-  if (A.input_type == LRT_DR_INPUT_TYPE_LINEAR_DEPTH)
-    hipLaunchKernelGGL(lrtd::k_dr_border_rays<lrtd::kDrDepthBorder>, dim3(blocks), dim3(256), 0, s, A, d_list, d_count, d_spill, d_hits);
-  else
+  if (dr_border_is_depth(A)) {
+    lrtd::DRArgs Ab = A;
+    if (cull_pairs) Ab.S.pairs = cull_pairs;
+    hipLaunchKernelGGL(lrtd::k_dr_border_rays<lrtd::kDrDepthBorder>, dim3(blocks), dim3(256), 0, s, Ab, d_list, d_count, d_spill, d_hits);
+  } else
     hipLaunchKernelGGL(lrtd::k_dr_border_rays<lrtd::kDrGeneric>, dim3(blocks), dim3(256), 0, s, A, d_list, d_count, d_spill, d_hits);
   return hipGetLastError();
 }
 hipError_t launch_dr_gather(const lrtd::DevScene& S, float* d_out, float2* d_minmax, hipStream_t s) {
-  const uint64_t total = (uint64_t)S.n_nodes * S.v_size * S.v_size * S.v_size;
-  const uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
-  hipLaunchKernelGGL(lrtd::k_dr_gather, dim3(blocks), dim3(256), 0, s, S, d_out);
-  hipLaunchKernelGGL(lrtd::k_dr_brick_minmax, dim3((S.n_nodes + 255) / 256), dim3(256), 0, s, S, (const float*)d_out, d_minmax);
+  hipLaunchKernelGGL(lrtd::k_dr_gather, dim3((S.n_nodes + 3) / 4), dim3(256), 0, s, S, d_out, d_minmax);
   return hipGetLastError();
 }
 hipError_t launch_dr_scale(float* d_grad, uint32_t n, float mult, hipStream_t s) {

@@ -29,10 +29,30 @@ struct DRArgs {
 }  // namespace lrtd
 
 namespace lrt {
-hipError_t launch_dr_primary(const lrtd::DRArgs& A, uint32_t* d_counter, uint32_t* d_spill, float* d_loss, uint32_t blocks, hipStream_t s);
+// true if launch_dr_primary runs the linear-depth form of the primary pass
+bool dr_primary_is_depth(const lrtd::DRArgs& A);
+// true if launch_dr_border_rays runs the linear-depth form of the border pass
+bool dr_border_is_depth(const lrtd::DRArgs& A);
+// Device tables of launch_dr_cull. Slots are node-pair children, 2 * pair + side.
+struct DRCullTables {
+  const uint32_t* leaf_slot;  // per brick: the slot whose leaf references it, 0xFFFFFFFF if none
+  const uint32_t* pair_slot;  // per node pair: the slot that references it, 0xFFFFFFFF for the root
+  const uint8_t* tiled;       // per slot: 1 if the bricks of its subtree tile its box exactly
+  uint32_t* occ;              // scratch, 2 x 2 * n_pairs words
+  uint4* pairs_primary;       // n_pairs x 64 B: copy for the linear-depth primary pass
+  uint4* pairs_border;        // n_pairs x 64 B: copy for the linear-depth border pass
+};
+// Writes both BVH copies for the current corner values (d_minmax from
+// launch_dr_gather) and relax_eps.
+hipError_t launch_dr_cull(const DRCullTables& T, const float2* d_minmax, float relax_eps, const uint4* d_pairs, uint32_t n_bricks,
+                          uint32_t n_pairs, hipStream_t s);
+// cull_pairs: T.pairs_primary / T.pairs_border of launch_dr_cull for the
+// current values, or null for the full tree
+hipError_t launch_dr_primary(const lrtd::DRArgs& A, uint32_t* d_counter, uint32_t* d_spill, float* d_loss, uint32_t blocks, hipStream_t s,
+                             const uint4* cull_pairs = nullptr);
 hipError_t launch_dr_border_find(const lrtd::DRArgs& A, uint32_t* d_list, uint32_t* d_count, hipStream_t s);
 hipError_t launch_dr_border_rays(const lrtd::DRArgs& A, const uint32_t* d_list, const uint32_t* d_count, uint32_t* d_spill,
-                                 uint32_t* d_hits, uint32_t blocks, hipStream_t s);
+                                 uint32_t* d_hits, uint32_t blocks, hipStream_t s, const uint4* cull_pairs = nullptr);
 hipError_t launch_dr_gather(const lrtd::DevScene& S, float* d_out, float2* d_minmax, hipStream_t s);
 hipError_t launch_dr_scale(float* d_grad, uint32_t n, float mult, hipStream_t s);
 hipError_t launch_adam(const float* g, float* X, float* V, float* S, uint32_t n, float lr, float b1, float b2, float eps, float c1,

@@ -45,6 +45,12 @@ static_assert(sizeof(DRParams) == 8, "DRArgs::Q layout");
 // expressions for what they keep and compile the rest of the payload out.
 enum : int { kDrGeneric = 0, kDrDepth = 1, kDrDepthBorder = 2 };
 
+// BVH copy of the linear-depth border pass: escapeIndex of a slot whose
+// subtree holds only bricks that pass skips (k_dr_cull_write), and the leaf
+// value the traversal gives such a slot when the ray crosses it.
+constexpr uint32_t kDrCulledSlot = 0xFFFFFFFEu;
+constexpr uint32_t kDrResetLeaf = 0xFFFFFFFEu;
+
 template <int SPEC>
 __device__ __forceinline__ uint32_t dr_flags(const DRParams& Q) {
   return SPEC == kDrDepth ? LRT_DR_RAY_FLAG_DDIST_DPOS : SPEC == kDrDepthBorder ? LRT_DR_RAY_FLAG_BORDER : Q.ray_flags;
@@ -375,11 +381,23 @@ __device__ __forceinline__ DRHit dr_nearest_hit(const DevScene& S, const float* 
                                   v3(__uint_as_float(e.x), __uint_as_float(e.y), __uint_as_float(e.z)));
         const bool h0 = (tm0.x <= tm0.y) && (tm0.y >= tNear) && (tm0.x <= hit.t);
         const bool h1 = (tm1.x <= tm1.y) && (tm1.y >= tNear) && (tm1.x <= hit.t);
-        left = h0 ? a.w : c.w;
+        uint32_t ch0 = a.w, ch1 = c.w;
+        if (SPEC == kDrDepthBorder) {
+          // A culled slot stands for bricks that are all skipped, each
+          // clearing the ray's state if the ray crosses it past the bricks'
+          // tNear (0.1, 1e-5 margin in dr_brick). As the bricks tile the
+          // box, a ray that crosses the box (positive length, entry past
+          // 0.1 + 1e-3) crosses some brick past that tNear: the slot becomes
+          // a leaf that only clears the state, visited in the subtree's
+          // place. Any other ray descends as over the full tree.
+          if (b.w == kDrCulledSlot && tm0.x < tm0.y && tm0.x > 0.1f + 1e-3f) ch0 = kDrResetLeaf;
+          if (e.w == kDrCulledSlot && tm1.x < tm1.y && tm1.x > 0.1f + 1e-3f) ch1 = kDrResetLeaf;
+        }
+        left = h0 ? ch0 : ch1;
         if (h0 && h1) {
           const bool near0 = tm0.x <= tm1.x;
-          left = near0 ? a.w : c.w;
-          st.put(top, near0 ? c.w : a.w);
+          left = near0 ? ch0 : ch1;
+          st.put(top, near0 ? ch1 : ch0);
           top++;
         }
         if (!h0 && !h1) {
@@ -387,7 +405,12 @@ __device__ __forceinline__ DRHit dr_nearest_hit(const DevScene& S, const float* 
           left = st.get(top > 0 ? top : 0);
         }
       }
-      if (top >= 0 && left != 0xFFFFFFFFu) dr_brick<SPEC>(S, bvals, bminmax, Q, relax_eps, pos, dir, inv, mx(tNear, 0.1f), left & LRT_START_MASK, relax, &hit);
+      if (SPEC == kDrDepthBorder && top >= 0 && left == kDrResetLeaf) {
+        relax->missed_cand.sdf = 1000.f;
+        relax->missed_hit.check_vox_border = 0;
+      } else if (top >= 0 && left != 0xFFFFFFFFu) {
+        dr_brick<SPEC>(S, bvals, bminmax, Q, relax_eps, pos, dir, inv, mx(tNear, 0.1f), left & LRT_START_MASK, relax, &hit);
+      }
       top--;
       left = st.get(top > 0 ? top : 0);
     }

@@ -338,6 +338,7 @@ int lrt_set_scene_sbs(lrt_handle h, const SdfSBSHeader* header, const SdfSBSNode
   if ((e = hipStreamSynchronize(r->stream)) != hipSuccess) return hip_err(e, "scene upload");
   r->has_scene = true;
   r->bfeat_valid = false;
+  r->dr_tree_state = 0;
   return LRT_OK;
 }
 

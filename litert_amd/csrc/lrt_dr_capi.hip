@@ -1,5 +1,6 @@
 // lrt_dr_capi.hip - C ABI of the differentiable renderer (dr::MultiRendererDR).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -45,6 +46,76 @@ static int dr_check(Renderer* r, const MultiRendererDRPreset& p) {
   if (r->dr_n_ref == 0) return set_err(LRT_E_INVALID, "no reference images: call lrt_dr_set_reference");
   if ((p.render_width && p.render_width != r->dr_w) || (p.render_height && p.render_height != r->dr_h))
     return set_err(LRT_E_UNSUPPORTED, "render size must equal the reference image size");
+  return LRT_OK;
+}
+
+// Slot tables of the scene's BVH for launch_dr_cull, built once per scene: the
+// slot (2 * pair + side) that references each brick and each node pair, and
+// whether the bricks under a slot tile its box. They describe a tree only if
+// every pair is referenced at most once and after its parent, and every brick
+// by at most one leaf; otherwise the full tree is used. A slot is tiled if all
+// bricks of the scene share one LOD and no two share a position (so bricks
+// never overlap) and the slot's brick count equals its box volume in bricks.
+static int dr_ensure_tree(Renderer* r) {
+  if (r->dr_tree_state != 0) return LRT_OK;
+  const uint32_t np = (uint32_t)r->pairs.size(), nb = (uint32_t)r->nodes.size();
+  const uint32_t none = 0xFFFFFFFFu;
+  std::vector<uint32_t> pair_slot(np, none), leaf_slot(nb, none);
+  bool ok = np > 0;
+  for (uint32_t p = 0; p < np && ok; p++)
+    for (uint32_t side = 0; side < 2 && ok; side++) {
+      const uint32_t off = side ? r->pairs[p].right.leftOffset : r->pairs[p].left.leftOffset;
+      if (off == none) continue;
+      if (off & LRT_LEAF_BIT) {
+        const uint32_t b = off & LRT_START_MASK;
+        ok = b < nb && leaf_slot[b] == none;
+        if (ok) leaf_slot[b] = 2 * p + side;
+      } else {
+        ok = off > p && off < np && pair_slot[off] == none;
+        if (ok) pair_slot[off] = 2 * p + side;
+      }
+    }
+  r->dr_tree_state = -1;
+  if (!ok) return LRT_OK;
+  // one LOD and distinct positions
+  const uint32_t lod = r->nodes[0].pos_z_lod_size & 0xFFFFu;
+  std::vector<uint64_t> keys(nb);
+  bool one_lod = true;
+  for (uint32_t i = 0; i < nb; i++) {
+    const SdfSBSNode& n = r->nodes[i];
+    one_lod = one_lod && (n.pos_z_lod_size & 0xFFFFu) == lod;
+    keys[i] = ((uint64_t)n.pos_xy << 16) | (n.pos_z_lod_size >> 16);
+  }
+  std::sort(keys.begin(), keys.end());
+  const bool disjoint = one_lod && std::adjacent_find(keys.begin(), keys.end()) == keys.end();
+  // bricks per slot, children before parents (pairs follow their parent)
+  std::vector<uint64_t> count(2 * (size_t)np, 0);
+  for (uint32_t b = 0; b < nb; b++)
+    if (leaf_slot[b] != none) count[leaf_slot[b]] = 1;
+  for (uint32_t p = np; p-- > 0;)
+    if (pair_slot[p] != none) count[pair_slot[p]] = count[2 * (size_t)p] + count[2 * (size_t)p + 1];
+  std::vector<uint8_t> tiled(2 * (size_t)np, 0);
+  for (uint32_t p = 0; p < np && disjoint; p++)
+    for (uint32_t side = 0; side < 2; side++) {
+      const BVHNode& n = side ? r->pairs[p].right : r->pairs[p].left;
+      if (n.leftOffset == none) continue;
+      double vol = 1.0;
+      bool whole = true;
+      for (int k = 0; k < 3; k++) {
+        const double ext = ((double)n.boxMax[k] - (double)n.boxMin[k]) * 0.5 * (double)lod;
+        whole = whole && ext > 0.5 && std::fabs(ext - std::nearbyint(ext)) < 1e-3;
+        vol *= std::nearbyint(ext);
+      }
+      tiled[2 * (size_t)p + side] = whole && vol == (double)count[2 * (size_t)p + side] ? 1 : 0;
+    }
+  hipError_t e;
+  if ((e = r->d_dr_leaf_slot.upload(leaf_slot.data(), (size_t)nb * 4, r->stream)) != hipSuccess) return hip_err(e, "upload");
+  if ((e = r->d_dr_pair_slot.upload(pair_slot.data(), (size_t)np * 4, r->stream)) != hipSuccess) return hip_err(e, "upload");
+  if ((e = r->d_dr_tiled.upload(tiled.data(), tiled.size(), r->stream)) != hipSuccess) return hip_err(e, "upload");
+  if ((e = r->d_dr_occ.ensure((size_t)np * 16)) != hipSuccess) return hip_err(e, "alloc");
+  if ((e = r->d_dr_cull_pairs.ensure((size_t)np * 2 * sizeof(BVHNodePair))) != hipSuccess) return hip_err(e, "alloc");
+  if ((e = hipStreamSynchronize(r->stream)) != hipSuccess) return hip_err(e, "upload");
+  r->dr_tree_state = 1;
   return LRT_OK;
 }
 
@@ -102,14 +173,29 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
   A.grad = (float*)r->d_dr_grad.p;
   A.bvals = (const float*)r->d_dr_bvals.p;
   A.bminmax = (const float2*)r->d_dr_bminmax.p;
+  const bool geometry_border = (p.dr_reconstruction_flags & LRT_DR_RECONSTRUCTION_FLAG_GEOMETRY) && p.border_spp > 0;
+  const bool cull = dr_primary_is_depth(A) || (geometry_border && dr_border_is_depth(A));
+  if (cull && (c = dr_ensure_tree(r)) != LRT_OK) return c;
 
   if ((e = hipEventRecord(r->ev0, r->stream)) != hipSuccess) return hip_err(e, "event");
   if ((e = launch_dr_gather(A.S, (float*)r->d_dr_bvals.p, (float2*)r->d_dr_bminmax.p, r->stream)) != hipSuccess) return hip_err(e, "dr gather");
-  if ((e = launch_dr_primary(A, misc, (uint32_t*)r->d_spill.p, (float*)(misc + 4), blocks, r->stream)) != hipSuccess)
+  const uint4 *cull_primary = nullptr, *cull_border = nullptr;
+  if (cull && r->dr_tree_state == 1) {
+    const uint32_t np_pairs = (uint32_t)r->pairs.size();
+    DRCullTables T{(const uint32_t*)r->d_dr_leaf_slot.p, (const uint32_t*)r->d_dr_pair_slot.p, (const uint8_t*)r->d_dr_tiled.p,
+                   (uint32_t*)r->d_dr_occ.p, (uint4*)r->d_dr_cull_pairs.p, (uint4*)r->d_dr_cull_pairs.p + 4 * (size_t)np_pairs};
+    if ((e = launch_dr_cull(T, (const float2*)r->d_dr_bminmax.p, A.relax_eps, A.S.pairs, (uint32_t)r->nodes.size(), np_pairs,
+                            r->stream)) != hipSuccess)
+      return hip_err(e, "dr cull");
+    cull_primary = T.pairs_primary;
+    cull_border = T.pairs_border;
+  }
+  if ((e = launch_dr_primary(A, misc, (uint32_t*)r->d_spill.p, (float*)(misc + 4), blocks, r->stream, cull_primary)) != hipSuccess)
     return hip_err(e, "dr primary");
   if ((e = launch_dr_border_find(A, (uint32_t*)r->d_dr_list.p, misc + 1, r->stream)) != hipSuccess) return hip_err(e, "dr border");
-  if ((p.dr_reconstruction_flags & LRT_DR_RECONSTRUCTION_FLAG_GEOMETRY) && p.border_spp > 0) {
-    if ((e = launch_dr_border_rays(A, (uint32_t*)r->d_dr_list.p, misc + 1, (uint32_t*)r->d_spill.p, misc + 2, blocks, r->stream)) != hipSuccess)
+  if (geometry_border) {
+    if ((e = launch_dr_border_rays(A, (uint32_t*)r->d_dr_list.p, misc + 1, (uint32_t*)r->d_spill.p, misc + 2, blocks, r->stream,
+                                   cull_border)) != hipSuccess)
       return hip_err(e, "dr border rays");
   }
   // getBaseGradientMult, MultiRendererDR.cpp:140-145

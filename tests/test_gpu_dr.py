@@ -172,6 +172,60 @@ def test_render_dr_on_noisy_distances_matches_fp32_twin(problem, seed):
     assert np.linalg.norm(grad - o32) / np.linalg.norm(o32) <= 1e-5
 
 
+def _sparse(sbs, limit):
+    """The bricks of a dense grid scene whose smallest distance value is below
+    `limit` (indices and values kept as they are), as a sparse brick set."""
+    vs = sbs.header.brick_size + 2 * sbs.header.brick_pad + 1
+    off = sbs.nodes[:, 2].astype(np.int64)
+    vmin = sbs.values_f[sbs.values[(off[:, None] + np.arange(vs ** 3)[None, :])].astype(np.int64)].min(axis=1)
+    keep = vmin < limit
+    assert 0 < keep.sum() < keep.size
+    return L.SdfSBS(sbs.header, sbs.nodes[keep], sbs.values, sbs.values_f.copy())
+
+
+@pytest.mark.parametrize("case", ["sparse", "inside_camera", "sparse_inside_camera"])
+def test_render_dr_tree_forms_match_fp32_twin(case):
+    """Linear-depth RenderDR where the ray casts cannot use every shortcut over
+    the BVH: a sparse brick set (subtrees whose bricks do not fill their box),
+    and cameras inside the brick grid (boxes entered within 0.1 of the eye).
+    Image, border statistics and dLoss/dS against the FP32 twin, which walks
+    the whole tree and every voxel as the reference does."""
+    tgt = L.grid_sbs("torus", 0.6, 0.2, 0.02, 3, seed=1, brick_count=8, brick_size=4)
+    ini = L.grid_sbs("sphere", 0.5, 0.0, 0.0, 3, seed=1, brick_count=8, brick_size=4)
+    if case.startswith("sparse"):
+        tgt, ini = _sparse(tgt, 0.3), _sparse(ini, 0.3)
+    dist = 0.9 if case.endswith("inside_camera") else 3.0
+    cams = [L.orbit_camera(5, i, distance=dist) for i in range(2)]
+    r = L.CreateMultiRenderer(L.DEVICE_GPU)
+    r.SetScene(tgt)
+    p = L.getDefaultPreset()
+    p.sdf_node_intersect = L.SDF_OCTREE_NODE_INTERSECT_NEWTON
+    p.render_mode = L.MULTI_RENDER_MODE_LINEAR_DEPTH
+    refs = np.stack([r.RenderFloat(W, H, wv, pr, p) for wv, pr in cams])
+    r.close()
+    dr = _preset(border_spp=64)
+    g = L.MultiRendererDR()
+    g.SetScene(ini)
+    wvs = np.stack([c[0] for c in cams])
+    prs = np.stack([c[1] for c in cams])
+    g.SetReference(refs, wvs, prs)
+    P = ini.values_f.size
+    pairs, _ = g.GetBVH()
+    sc = oracle.Scene(ini, pairs)
+    for view in range(2):
+        ref = refs[view].copy()
+        ref[..., 3] = (np.linalg.norm(ref[..., :3], axis=-1) >= 0.001)
+        img, grad, loss, ms = g.RenderDR(dr, view, seed=90 + view, n_params=P)
+        o_img, o32, st32 = sc.dr_render_grad(dr, wvs[view], prs[view], W, H, ref, P, seed=90 + view, precision=32)
+        bp, bh = g.LastStats()
+        print(case, view, "loss", loss, st32, "border", bp, bh)
+        assert np.mean(np.all(img == o_img.astype(np.float32), axis=-1)) > 0.999
+        assert bp == st32["border_pixels"] and abs(bh - st32["border_rays_hit"]) <= 2
+        assert abs(loss - st32["loss"]) <= 1e-5 * max(1.0, abs(st32["loss"]))
+        assert np.linalg.norm(o32) > 0
+        assert np.linalg.norm(grad - o32) / np.linalg.norm(o32) <= 1e-5
+
+
 def test_adam_step_matches_oracle():
     rng = np.random.default_rng(3)
     n = 1000
