@@ -257,8 +257,8 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0, help="append a device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); every GPU box here has a single MI355X "
-              "and the multi-GPU sharding is a design note, not a built path.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): the library renders on a single MI355X "
+              "and sharding across GPUs is not built.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
