@@ -131,7 +131,13 @@ int lrt_dr_set_reference(lrt_handle h, const float* images, const float* masks, 
  * reference view with the current scene (indexed ID32F layout): image, loss
  * and dLoss/dS (values_f-sized, already scaled by getBaseGradientMult) of that
  * view. dtdv_mode 0 = analytic dt/dS (implicit function theorem), 1 = the
- * reference's central difference (BVH2DR.cpp:1075-1090). Outputs may be NULL. */
+ * reference's central difference (BVH2DR.cpp:1075-1090). Outputs may be NULL.
+ * Every DR render mode: DIFFUSE, LAMBERT (the renderer's default lights,
+ * eye_ray_host.cpp:40: direct along (1,1,1) of colour 2/3 plus ambient 0.25),
+ * MASK, LINEAR_DEPTH, NORMAL; LAMBERT and NORMAL differentiate the hit normal
+ * (smoothed over neighbouring voxels for the ID32F_IRGB32F_IN layout,
+ * BVH2DR.cpp:1300-1484). Not built: SVM border sampling and the
+ * finite-difference DR mode (LRT_E_UNSUPPORTED). */
 int lrt_dr_render_grad(lrt_handle h, const MultiRendererDRPreset* preset, uint32_t view, uint32_t seed, int dtdv_mode, float* out_image,
                        float* out_grad, float* loss, float* kernel_ms);
 /* MultiRendererDR::OptimizeFixedStructure(preset, sbs), MultiRendererDR.cpp:278-576:

@@ -21,6 +21,13 @@ __device__ __forceinline__ float lin_depth(float z) { return (z - 0.1f) / (10.0f
 __device__ __forceinline__ v3 dr_color(uint32_t mode, const DRHit& h) {  // MultiRendererDR.cpp:855-898
   if (mode == LRT_DR_RENDER_MODE_DIFFUSE) return h.color;
   if (mode == LRT_DR_RENDER_MODE_MASK) return v3(1, 1, 1);
+  if (mode == LRT_DR_RENDER_MODE_NORMAL) return h.normal;
+  if (mode == LRT_DR_RENDER_MODE_LAMBERT) {  // default lights: direct, then ambient
+    const v3 lc(kLightCol, kLightCol, kLightCol), la(kAmbient, kAmbient, kAmbient);
+    const float q = fmaxf(0.0f, dot(h.normal, v3(kLightDir, kLightDir, kLightDir)));
+    v3 c = (lc * h.color) * q;
+    return c + la * h.color;
+  }
   float d = lin_depth(h.t);
   return v3(d, d, d);
 }
@@ -93,12 +100,18 @@ __device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, S
   if (x >= A.W || y >= A.H) return;
   const size_t pix = (size_t)y * A.W + x;
   DRPayload py;
+  py.scatter = 0;
   v3 res(0, 0, 0);
   int hit_count = 0;
   float samples[16];
   const uint32_t spp = SPEC == kDrDepth ? 1u : A.spp;
-  const DRParams Qc = {spp == 1 ? A.Q.ray_flags : 0u, A.Q.dtdv_fd};
+  // kDrNormal always traces twice: the second pass scatters at the hit
+  const bool one_pass = spp == 1 && SPEC != kDrNormal;
+  const DRParams Qc = {one_pass ? A.Q.ray_flags : 0u, A.Q.dtdv_fd};
   const DRParams Qg = {A.Q.ray_flags, A.Q.dtdv_fd};
+  py.scatter = 0;
+  py.mode = A.mode;
+  py.grad = A.grad;
   for (uint32_t s = 0; s < spp; s++) {
     float2 d = rand2(x, y, s, A.seed);
     v3 pos, dir;
@@ -124,16 +137,21 @@ __device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, S
   const v3 dLdC(loss_grad1(A.loss_fn, cm[0], rr[0]) / (float)spp, loss_grad1(A.loss_fn, cm[1], rr[1]) / (float)spp,
                 loss_grad1(A.loss_fn, cm[2], rr[2]) / (float)spp);
   if (A.Q.ray_flags == 0) return;
-  if (spp == 1) {
+  if (one_pass) {
     if (hit_count) scatter_pd<SPEC>(A, py, dLdC);
     return;
+  }
+  if (SPEC == kDrNormal) {
+    if (hit_count == 0) return;
+    py.scatter = 1;
+    py.dldc = dLdC;
   }
   for (uint32_t s = 0; s < spp; s++) {
     float2 d = rand2(x, y, s, A.seed);
     v3 pos, dir;
     eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
     DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, A.bminmax, Qg, A.relax_eps, pos, dir, 1e9f, &py, st);
-    if (hit.primId != 0xFFFFFFFFu) scatter_pd<SPEC>(A, py, dLdC);
+    if (SPEC != kDrNormal && hit.primId != 0xFFFFFFFFu) scatter_pd<SPEC>(A, py, dLdC);
   }
 }
 
@@ -370,8 +388,12 @@ namespace lrt {
 // kDrDepth: linear-depth reconstruction (distance gradients only), analytic
 // dt/dv, one sample per pixel, a render mode whose colour is not the diffuse
 // colour. Everything else runs the generic form.
+static bool dr_normal_mode(const lrtd::DRArgs& A) {
+  return A.mode == LRT_DR_RENDER_MODE_LAMBERT || A.mode == LRT_DR_RENDER_MODE_NORMAL;
+}
 bool dr_primary_is_depth(const lrtd::DRArgs& A) {
-  return A.Q.ray_flags == LRT_DR_RAY_FLAG_DDIST_DPOS && A.Q.dtdv_fd == 0 && A.spp == 1 && A.mode != LRT_DR_RENDER_MODE_DIFFUSE;
+  return A.Q.ray_flags == LRT_DR_RAY_FLAG_DDIST_DPOS && A.Q.dtdv_fd == 0 && A.spp == 1 &&
+         (A.mode == LRT_DR_RENDER_MODE_LINEAR_DEPTH || A.mode == LRT_DR_RENDER_MODE_MASK);
 }
 bool dr_border_is_depth(const lrtd::DRArgs& A) { return A.input_type == LRT_DR_INPUT_TYPE_LINEAR_DEPTH; }
 hipError_t launch_dr_cull(const DRCullTables& T, const float2* d_minmax, float relax_eps, const uint4* d_pairs, uint32_t n_bricks,
@@ -391,7 +413,9 @@ hipError_t launch_dr_primary(const lrtd::DRArgs& A, uint32_t* d_counter, uint32_
     lrtd::DRArgs Ad = A;
     if (cull_pairs) Ad.S.pairs = cull_pairs;
     hipLaunchKernelGGL(lrtd::k_dr_primary<lrtd::kDrDepth>, dim3(blocks), dim3(256), 0, s, Ad, d_counter, d_spill, d_loss);
-  } else
+  } else if (dr_normal_mode(A))
+    hipLaunchKernelGGL(lrtd::k_dr_primary<lrtd::kDrNormal>, dim3(blocks), dim3(256), 0, s, A, d_counter, d_spill, d_loss);
+  else
     hipLaunchKernelGGL(lrtd::k_dr_primary<lrtd::kDrGeneric>, dim3(blocks), dim3(256), 0, s, A, d_counter, d_spill, d_loss);
   return hipGetLastError();
 }
@@ -407,7 +431,9 @@ hipError_t launch_dr_border_rays(const lrtd::DRArgs& A, const uint32_t* d_list, 
     lrtd::DRArgs Ab = A;
     if (cull_pairs) Ab.S.pairs = cull_pairs;
     hipLaunchKernelGGL(lrtd::k_dr_border_rays<lrtd::kDrDepthBorder>, dim3(blocks), dim3(256), 0, s, Ab, d_list, d_count, d_spill, d_hits);
-  } else
+  } else if (dr_normal_mode(A))
+    hipLaunchKernelGGL(lrtd::k_dr_border_rays<lrtd::kDrNormal>, dim3(blocks), dim3(256), 0, s, A, d_list, d_count, d_spill, d_hits);
+  else
     hipLaunchKernelGGL(lrtd::k_dr_border_rays<lrtd::kDrGeneric>, dim3(blocks), dim3(256), 0, s, A, d_list, d_count, d_spill, d_hits);
   return hipGetLastError();
 }

@@ -3,9 +3,10 @@
 // Same steps as BVHDR::RayQuery_NearestHitWithGrad / BVH2TraverseF32WithGrad
 // / OctreeBrickIntersectWithGrad / Intersect (diff_render/BVH2DR.cpp:12-190,
 // 623-1117, 1130-1610) for the Newton root find that MultiRendererDR forces,
-// indexed (ID32F) layouts, and the partial derivatives of depth and diffuse
-// colour. dt/dv is either the implicit-function-theorem derivative of the
-// root (default) or the reference's central difference of the root find.
+// indexed (ID32F) layouts, and the partial derivatives of depth, diffuse
+// colour and the hit normal. dt/dv is either the implicit-function-theorem
+// derivative of the root (default) or the reference's central difference of
+// the root find.
 #pragma once
 #include "traverse_device.hpp"
 
@@ -29,6 +30,11 @@ struct DRPayload {
   v3 pd_ddiffuse[8];     // dDiffuse/dS
   uint32_t col_idx[8];   // colour parameter base indices of the hit brick
   float col_w[8];        // dDiffuse/dColour (trilinear weight)
+  // kDrNormal only: with scatter set, the hit adds its dLoss/dS terms to grad
+  // directly (dLoss/dColour dldc of the pixel is known: second pass)
+  float* grad;
+  v3 dldc;
+  uint32_t mode, scatter;
 };
 
 struct DRParams {
@@ -43,7 +49,10 @@ static_assert(sizeof(DRParams) == 8, "DRArgs::Q layout");
 // dt/dv, colour unused); kDrDepthBorder the border pass of linear-depth input
 // (flags = BORDER, colour unused). The specialised forms evaluate the same
 // expressions for what they keep and compile the rest of the payload out.
-enum : int { kDrGeneric = 0, kDrDepth = 1, kDrDepthBorder = 2 };
+// kDrNormal: the render modes that read the hit normal (LAMBERT, NORMAL):
+// run-time flags, colour, the normal from the hit voxel or smoothed over its
+// neighbours, and the normal's derivatives scattered at the hit.
+enum : int { kDrGeneric = 0, kDrDepth = 1, kDrDepthBorder = 2, kDrNormal = 3 };
 
 // BVH copy of the linear-depth border pass: escapeIndex of a slot whose
 // subtree holds only bricks that pass skips (k_dr_cull_write), and the leaf
@@ -146,12 +155,309 @@ __device__ __forceinline__ float corner_w(int i, v3 q) {
   return ((i & 4) ? q.x : 1 - q.x) * ((i & 2) ? q.y : 1 - q.y) * ((i & 1) ? q.z : 1 - q.z);
 }
 
-// BVH2DR.cpp:1130-1610 for ID32F layouts without normal derivatives.
+// 3x3 matrix as rows; m * v = the row dot products
+struct m3 {
+  v3 r[3];
+};
+__device__ __forceinline__ v3 mul(const m3& m, v3 v) { return v3(dot(m.r[0], v), dot(m.r[1], v), dot(m.r[2], v)); }
+
+// BVH2DR.cpp:339-404: Jacobian of v / |v|
+__device__ __forceinline__ m3 normalize_diff(v3 v) {
+  const float a = dot(v, v);
+  const float b = 1.0f / (a * sqrtf(a));
+  m3 m;
+  m.r[0] = b * v3(v.y * v.y + v.z * v.z, -v.x * v.y, -v.x * v.z);
+  m.r[1] = b * v3(-v.x * v.y, v.x * v.x + v.z * v.z, -v.y * v.z);
+  m.r[2] = b * v3(-v.x * v.z, -v.y * v.z, v.x * v.x + v.y * v.y);
+  return m;
+}
+
+// BVH2DR.cpp:406-530: Hessian of the trilinear distance times a direction
+__device__ __forceinline__ v3 trilinear_hess_dir(const float v[8], v3 dp, v3 dir) {
+  const float xy = (1 - dp.z) * v[0] + (dp.z) * v[1] + -(1 - dp.z) * v[2] + -(dp.z) * v[3] + -(1 - dp.z) * v[4] + -(dp.z) * v[5] +
+                   (1 - dp.z) * v[6] + (dp.z) * v[7];
+  const float xz = (1 - dp.y) * v[0] + -(1 - dp.y) * v[1] + (dp.y) * v[2] + -(dp.y) * v[3] + -(1 - dp.y) * v[4] + (1 - dp.y) * v[5] +
+                   -(dp.y) * v[6] + (dp.y) * v[7];
+  const float yz = (1 - dp.x) * v[0] + -(1 - dp.x) * v[1] + -(1 - dp.x) * v[2] + (1 - dp.x) * v[3] + (dp.x) * v[4] + -(dp.x) * v[5] +
+                   -(dp.x) * v[6] + (dp.x) * v[7];
+  m3 h;
+  h.r[0] = v3(0.0f, xy, xz);
+  h.r[1] = v3(xy, 0.0f, yz);
+  h.r[2] = v3(xz, yz, 0.0f);
+  return mul(h, dir);
+}
+
+// BVH2DR.cpp:531-611: column c of d(grad f)/d(values)
+__device__ __forceinline__ v3 trilinear_ddp_dvalue(int c, v3 dp) {
+  const float sx = (c & 4) ? 1.0f : -1.0f, sy = (c & 2) ? 1.0f : -1.0f, sz = (c & 1) ? 1.0f : -1.0f;
+  const float wx = (c & 4) ? dp.x : 1 - dp.x, wy = (c & 2) ? dp.y : 1 - dp.y, wz = (c & 1) ? dp.z : 1 - dp.z;
+  return v3((sx * wy) * wz, (sy * wx) * wz, (sz * wx) * wy);
+}
+
+// The renderer's default lights (Renderer/eye_ray_host.cpp:40): direct along
+// normalize(1,1,1) (1/sqrt(3) rounded to float) of colour 2/3, ambient 0.25.
+constexpr float kLightDir = 0.57735026918962576f, kLightCol = 2.0f / 3.0f, kAmbient = 0.25f;
+
+// CalculateColorWithGrad (MultiRendererDR.cpp:900-980) for LAMBERT and NORMAL
+__device__ __forceinline__ void dr_color_grad(uint32_t mode, const DRHit& h, m3& dCdD, m3& dCdN) {
+  const v3 z(0, 0, 0);
+  dCdD.r[0] = dCdD.r[1] = dCdD.r[2] = z;
+  dCdN.r[0] = dCdN.r[1] = dCdN.r[2] = z;
+  if (mode == LRT_DR_RENDER_MODE_NORMAL) {
+    dCdN.r[0] = v3(1, 0, 0);
+    dCdN.r[1] = v3(0, 1, 0);
+    dCdN.r[2] = v3(0, 0, 1);
+    return;
+  }
+  const v3 L(kLightDir, kLightDir, kLightDir);
+  const float q = fmaxf(0.0f, dot(h.normal, L));
+  if (q > 0.0f) {
+    dCdD.r[0] = v3(kLightCol * q, 0, 0);
+    dCdD.r[1] = v3(0, kLightCol * q, 0);
+    dCdD.r[2] = v3(0, 0, kLightCol * q);
+    dCdN.r[0] = (kLightCol * h.color.x) * L;
+    dCdN.r[1] = (kLightCol * h.color.y) * L;
+    dCdN.r[2] = (kLightCol * h.color.z) * L;
+  }
+  dCdD.r[0] = dCdD.r[0] + v3(kAmbient, 0, 0);
+  dCdD.r[1] = dCdD.r[1] + v3(0, kAmbient, 0);
+  dCdD.r[2] = dCdD.r[2] + v3(0, 0, kAmbient);
+}
+
+// The hit of a kDrNormal ray cast (BVH2DR.cpp:1277-1524 with the pixel's
+// accumulation, MultiRendererDR.cpp:1148-1265): dt/dv, the diffuse colour
+// and its derivatives, the normal from the hit voxel or, for the _IN layout,
+// smoothed over the 8 voxels around the hit, and its derivatives. With
+// relax->scatter set the dLoss/dS terms are added to relax->grad here, one
+// atomic per payload entry of the reference, instead of being kept in a
+// 64-entry payload. values / vx,vy,vz: the hit voxel; start_q, t, qFar: its
+// segment and root; bmin, sz, d: the brick.
+__device__ __noinline__ void dr_normal_hit(const DevScene& S, uint32_t flags, bool dtdv_fd, v3 ray_pos, v3 ray_dir, const float values[8],
+                                           uint32_t brick, uint32_t vx, uint32_t vy, uint32_t vz, v3 start_q, float t, float qFar,
+                                           v3 bmin, float sz, float d, DRPayload* relax, DRHit* pHit) {
+  const uint4 N = S.nodes[brick];
+  const uint32_t vs = S.v_size;
+  auto vid = [&](uint32_t x, uint32_t y, uint32_t z, int i) {
+    return (x + ((i & 4) >> 2)) * vs * vs + (y + ((i & 2) >> 1)) * vs + (z + (i & 1));
+  };
+  float dtdv[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) dtdv[i] = 0.0f;
+  const float d_inv = 1.0f / d;
+  if (flags & (LRT_DR_RAY_FLAG_DDIFFUSE_DPOS | LRT_DR_RAY_FLAG_DNORM_DPOS | LRT_DR_RAY_FLAG_DDIST_DPOS)) {
+    if (dtdv_fd) {  // BVH2DR.cpp:1075-1090
+      float vv[8];
+      for (int i = 0; i < 8; i++) vv[i] = values[i];
+      const float delta = 0.001f;
+      for (int i = 0; i < 8; i++) {
+        vv[i] += delta;
+        float d1 = dr_intersect(flags, ray_dir, vv, d, d_inv, qFar, start_q, nullptr);
+        vv[i] -= 2 * delta;
+        float d2 = dr_intersect(flags, ray_dir, vv, d, d_inv, qFar, start_q, nullptr);
+        vv[i] += delta;
+        dtdv[i] = (d1 - d2) / (2 * delta);
+      }
+    } else {
+      v3 q = start_q + t * ray_dir;
+      float fd = dot(trilinear_diff(values, q), ray_dir);
+      for (int i = 0; i < 8; i++) dtdv[i] = fd != 0.0f ? -corner_w(i, q) / fd : 0.0f;
+    }
+  }
+  // diffuse colour (BVH2DR.cpp:1489-1524)
+  const v3 pos = ray_pos + pHit->t * ray_dir;
+  const v3 dpc = (pos - bmin) * (0.5f * sz);
+  const uint32_t t_off = N.z + vs * vs * vs;
+  v3 col(0, 0, 0), J0(0, 0, 0), J1(0, 0, 0), J2(0, 0, 0);
+  for (int i = 0; i < 8; i++) {
+    uint32_t ci = S.values[t_off + i];
+    v3 c(S.values_f[ci], S.values_f[ci + 1], S.values_f[ci + 2]);
+    float wx = (i & 4) ? dpc.x : 1 - dpc.x, wy = (i & 2) ? dpc.y : 1 - dpc.y, wz = (i & 1) ? dpc.z : 1 - dpc.z;
+    col = col + (wx * wy * wz) * c;
+    float ax = (i & 4) ? 1.0f : -1.0f, ay = (i & 2) ? 1.0f : -1.0f, az = (i & 1) ? 1.0f : -1.0f;
+    J0 = J0 + (ax * wy * wz) * c;
+    J1 = J1 + (wx * ay * wz) * c;
+    J2 = J2 + (wx * wy * az) * c;
+  }
+  pHit->color = col;
+  const bool smoothed = S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F_IN;
+  const bool dn = relax->scatter && (flags & LRT_DR_RAY_FLAG_DNORM_DPOS);
+  const v3 p0 = start_q + t * ray_dir;
+  if (!smoothed) {
+    const v3 g = trilinear_diff(values, p0);
+    pHit->normal = normalize(g);
+    if (!relax->scatter) return;
+    m3 dCdD, dCdN;
+    dr_color_grad(relax->mode, *pHit, dCdD, dCdN);
+    const v3 dl = relax->dldc;
+    if (flags & LRT_DR_RAY_FLAG_DDIFFUSE_DCOLOR) {
+      for (int i = 0; i < 8; i++) {
+        const uint32_t ci = S.values[t_off + i];
+        const float w = corner_w(i, dpc);
+        const v3 dc = mul(dCdD, v3(w, w, w));
+        atomicAdd(&relax->grad[ci + 0], dl.x * dc.x);
+        atomicAdd(&relax->grad[ci + 1], dl.y * dc.y);
+        atomicAdd(&relax->grad[ci + 2], dl.z * dc.z);
+      }
+    }
+    if (flags & (LRT_DR_RAY_FLAG_DDIFFUSE_DPOS | LRT_DR_RAY_FLAG_DNORM_DPOS)) {
+      m3 nd;
+      v3 t1(0, 0, 0);
+      if (dn) {
+        nd = normalize_diff(g);
+        t1 = trilinear_hess_dir(values, p0, ray_dir);
+      }
+      for (int i = 0; i < 8; i++) {
+        v3 ddiff(0, 0, 0), dnorm(0, 0, 0);
+        if (flags & LRT_DR_RAY_FLAG_DDIFFUSE_DPOS) {
+          v3 v(0.5f * sz * ray_dir.x * dtdv[i], 0.5f * sz * ray_dir.y * dtdv[i], 0.5f * sz * ray_dir.z * dtdv[i]);
+          ddiff = J0 * v.x + J1 * v.y + J2 * v.z;
+        }
+        if (dn) {
+          v3 c = trilinear_ddp_dvalue(i, p0);
+          dnorm = mul(nd, v3(c.x + t1.x * dtdv[i], c.y + t1.y * dtdv[i], c.z + t1.z * dtdv[i]));
+        }
+        atomicAdd(&relax->grad[S.values[N.z + vid(vx, vy, vz, i)]], dot(dl, mul(dCdD, ddiff) + mul(dCdN, dnorm)));
+      }
+    } else if (flags & LRT_DR_RAY_FLAG_DDIST_DPOS) {
+      for (int i = 0; i < 8; i++) {
+        float v = d * dtdv[i] / (10.0f - 0.1f);
+        atomicAdd(&relax->grad[S.values[N.z + vid(vx, vy, vz, i)]], dot(dl, v3(v, v, v)));
+      }
+    }
+    return;
+  }
+  // smoothed normal over the 8 voxels around the hit (BVH2DR.cpp:1305-1437)
+  const float beta = 0.5f;
+  const v3 dp = p0;
+  v3 nmq;
+  nmq.x = dp.x < 0.5f ? step01(-beta, beta, dp.x) : step01(-beta, beta, dp.x - 1.0f);
+  nmq.y = dp.y < 0.5f ? step01(-beta, beta, dp.y) : step01(-beta, beta, dp.y - 1.0f);
+  nmq.z = dp.z < 0.5f ? step01(-beta, beta, dp.z) : step01(-beta, beta, dp.z - 1.0f);
+  const int bs = (int)S.brick_size;
+  const v3 vp((float)vx, (float)vy, (float)vz);
+  const int vp0[3] = {dp.x < 0.5f ? (int)vx - 1 : (int)vx, dp.y < 0.5f ? (int)vy - 1 : (int)vy, dp.z < 0.5f ? (int)vz - 1 : (int)vz};
+  const uint32_t k = (dp.x < 0.5f ? 4u : 0u) + (dp.y < 0.5f ? 2u : 0u) + (dp.z < 0.5f ? 1u : 0u);
+  // neighbour i of the hit: its brick (0xFFFFFFFF if absent) and voxel
+  auto neighbour = [&](int i, uint32_t& nb, uint32_t nv[3], v3& dvp) {
+    int VI[3] = {vp0[0] + ((i & 4) >> 2), vp0[1] + ((i & 2) >> 1), vp0[2] + (i & 1)};
+    int BP[3];
+    for (int a = 0; a < 3; a++) BP[a] = VI[a] >= 0 ? (VI[a] < bs ? 0 : 1) : -1;
+    uint32_t nid = 9 * (BP[0] + 1) + 3 * (BP[1] + 1) + (BP[2] + 1);
+    dvp = v3((float)VI[0], (float)VI[1], (float)VI[2]) - vp;
+    nb = brick;
+    if (nid == 13) {
+      for (int a = 0; a < 3; a++) nv[a] = (uint32_t)VI[a];
+    } else {
+      nb = S.values[N.z + vs * vs * vs + 8 + nid];
+      for (int a = 0; a < 3; a++) nv[a] = (uint32_t)(VI[a] - bs * BP[a]);
+    }
+  };
+  v3 normals[8];
+  for (int i = 0; i < 8; i++) {
+    uint32_t nb, nv[3];
+    v3 dvp;
+    neighbour(i, nb, nv, dvp);
+    normals[i] = v3(0, 0, 0);
+    if (nb == LRT_INVALID_IDX) continue;
+    const uint32_t off = S.nodes[nb].z;
+    float vn[8];
+    for (int c = 0; c < 8; c++) vn[c] = S.values_f[S.values[off + vid(nv[0], nv[1], nv[2], c)]];
+    normals[i] = normalize(trilinear_diff(vn, dp - dvp));
+  }
+  v3 tri = corner_w(0, nmq) * normals[0];
+  for (int i = 1; i < 8; i++) tri = tri + corner_w(i, nmq) * normals[i];
+  pHit->normal = normalize(tri);
+  if (!relax->scatter) return;
+  m3 dCdD, dCdN;
+  dr_color_grad(relax->mode, *pHit, dCdD, dCdN);
+  const v3 dl = relax->dldc;
+  if (flags & LRT_DR_RAY_FLAG_DDIFFUSE_DCOLOR) {
+    for (int i = 0; i < 8; i++) {
+      const uint32_t ci = S.values[t_off + i];
+      const float w = corner_w(i, dpc);
+      const v3 dc = mul(dCdD, v3(w, w, w));
+      atomicAdd(&relax->grad[ci + 0], dl.x * dc.x);
+      atomicAdd(&relax->grad[ci + 1], dl.y * dc.y);
+      atomicAdd(&relax->grad[ci + 2], dl.z * dc.z);
+    }
+  }
+  if (!(flags & (LRT_DR_RAY_FLAG_DDIFFUSE_DPOS | LRT_DR_RAY_FLAG_DNORM_DPOS))) {
+    if (flags & LRT_DR_RAY_FLAG_DDIST_DPOS)
+      for (int i = 0; i < 8; i++) {
+        float v = d * dtdv[i] / (10.0f - 0.1f);
+        atomicAdd(&relax->grad[S.values[N.z + vid(vx, vy, vz, i)]], dot(dl, v3(v, v, v)));
+      }
+    return;
+  }
+  // dNorm of the hit voxel's corners accumulates over the neighbours; the
+  // neighbours' own corners are scattered as they are reached
+  v3 acc[8];
+  for (int c = 0; c < 8; c++) acc[c] = v3(0, 0, 0);
+  const m3 M = normalize_diff(tri);
+  if (dn) {
+    for (int i = 0; i < 8; i++) {
+      uint32_t nb, nv[3];
+      v3 dvp;
+      neighbour(i, nb, nv, dvp);
+      if (nb == LRT_INVALID_IDX) continue;
+      const uint32_t off = S.nodes[nb].z;
+      float vn[8];
+      uint32_t in[8];
+      for (int c = 0; c < 8; c++) {
+        in[c] = S.values[off + vid(nv[0], nv[1], nv[2], c)];
+        vn[c] = S.values_f[in[c]];
+      }
+      const v3 q = dp - dvp;
+      const v3 g = trilinear_diff(vn, q);
+      const m3 nd = normalize_diff(g);
+      const v3 t1 = trilinear_hess_dir(vn, q, ray_dir);
+      const float wi = corner_w(i, nmq);
+      const bool valid = sqrtf(dot(normals[i], normals[i])) > 0.5f;
+      for (int c = 0; c < 8; c++) {  // d(n_i)/d(v_i)
+        const v3 e = wi * mul(nd, trilinear_ddp_dvalue(c, q));
+        if ((uint32_t)i == k) {
+          acc[c] = acc[c] + e;
+        } else {
+          const v3 dnorm = valid ? mul(M, v3(0, 0, 0) + e) : v3(0, 0, 0) + e;
+          atomicAdd(&relax->grad[in[c]], dot(dl, mul(dCdD, v3(0, 0, 0)) + mul(dCdN, dnorm)));
+        }
+      }
+      for (int c = 0; c < 8; c++)  // d(n_i)/d(v_k) through the hit point
+        acc[c] = acc[c] + wi * mul(nd, v3(t1.x * dtdv[c], t1.y * dtdv[c], t1.z * dtdv[c]));
+    }
+    // dtrilinear3/dnmq (by columns) times dp/dv_k
+    v3 cx, cy, cz;
+    for (int i = 0; i < 8; i++) {
+      const float ax = (i & 4) ? 1.0f : -1.0f, ay = (i & 2) ? 1.0f : -1.0f, az = (i & 1) ? 1.0f : -1.0f;
+      const float wx = (i & 4) ? nmq.x : 1 - nmq.x, wy = (i & 2) ? nmq.y : 1 - nmq.y, wz = (i & 1) ? nmq.z : 1 - nmq.z;
+      const v3 ex = (ax * wy * wz) * normals[i], ey = (wx * ay * wz) * normals[i], ez = (wx * wy * az) * normals[i];
+      cx = i == 0 ? ex : cx + ex;
+      cy = i == 0 ? ey : cy + ey;
+      cz = i == 0 ? ez : cz + ez;
+    }
+    for (int c = 0; c < 8; c++) {
+      const v3 ddp(ray_dir.x * dtdv[c], ray_dir.y * dtdv[c], ray_dir.z * dtdv[c]);
+      acc[c] = acc[c] + (cx * ddp.x + cy * ddp.y + cz * ddp.z);
+    }
+    if (sqrtf(dot(normals[k], normals[k])) > 0.5f)
+      for (int c = 0; c < 8; c++) acc[c] = mul(M, acc[c]);
+  }
+  for (int c = 0; c < 8; c++) {  // the hit voxel's corners
+    v3 ddiff(0, 0, 0);
+    if (flags & LRT_DR_RAY_FLAG_DDIFFUSE_DPOS) {
+      v3 v(0.5f * sz * ray_dir.x * dtdv[c], 0.5f * sz * ray_dir.y * dtdv[c], 0.5f * sz * ray_dir.z * dtdv[c]);
+      ddiff = J0 * v.x + J1 * v.y + J2 * v.z;
+    }
+    atomicAdd(&relax->grad[S.values[N.z + vid(vx, vy, vz, c)]], dot(dl, mul(dCdD, ddiff) + mul(dCdN, acc[c])));
+  }
+}
+
+// BVH2DR.cpp:1130-1610 for ID32F layouts.
 template <int SPEC>
 __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restrict__ bvals, const float2* __restrict__ bminmax,
                                          const DRParams& Q, float relax_eps, v3 ray_pos, v3 ray_dir, v3 inv, float tNear, uint32_t brick,
                                          DRPayload* relax, DRHit* pHit) {
-  constexpr bool kColour = SPEC == kDrGeneric;
+  constexpr bool kColour = SPEC == kDrGeneric || SPEC == kDrNormal;
   float values[8];
   const uint4 N = S.nodes[brick];
   const float px = (float)(N.x >> 16), py = (float)(N.x & 0xFFFFu), pz = (float)(N.y >> 16), sz = (float)(N.y & 0xFFFFu);
@@ -277,6 +583,10 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restr
       if (t <= qFar && tReal < pHit->t) {  // BVH2DR.cpp:1277-1600
         pHit->t = tReal;
         pHit->primId = brick;
+        if (SPEC == kDrNormal) {
+          dr_normal_hit(S, flags, Q.dtdv_fd != 0, ray_pos, ray_dir, values, brick, vx, vy, vz, start_q, t, qFar, bmin, sz, d, relax, pHit);
+          break;
+        }
         float dtdv[8];
         if (flags & (LRT_DR_RAY_FLAG_DDIFFUSE_DPOS | LRT_DR_RAY_FLAG_DDIST_DPOS)) {
           if (dtdv_fd) {  // BVH2DR.cpp:1075-1090

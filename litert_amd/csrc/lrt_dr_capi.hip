@@ -24,8 +24,11 @@ static uint32_t dr_ray_flags(const MultiRendererDRPreset& p) {
   if (p.dr_input_type == LRT_DR_INPUT_TYPE_LINEAR_DEPTH) return LRT_DR_RAY_FLAG_DDIST_DPOS;
   uint32_t f = LRT_DR_RAY_FLAG_NO_DIFF;
   if (p.dr_reconstruction_flags & LRT_DR_RECONSTRUCTION_FLAG_COLOR) f |= LRT_DR_RAY_FLAG_DDIFFUSE_DCOLOR;
-  if ((p.dr_reconstruction_flags & LRT_DR_RECONSTRUCTION_FLAG_GEOMETRY) && p.dr_render_mode == LRT_DR_RENDER_MODE_DIFFUSE)
-    f |= LRT_DR_RAY_FLAG_DDIFFUSE_DPOS;
+  if (p.dr_reconstruction_flags & LRT_DR_RECONSTRUCTION_FLAG_GEOMETRY) {
+    if (p.dr_render_mode == LRT_DR_RENDER_MODE_DIFFUSE) f |= LRT_DR_RAY_FLAG_DDIFFUSE_DPOS;
+    else if (p.dr_render_mode == LRT_DR_RENDER_MODE_LAMBERT) f |= LRT_DR_RAY_FLAG_DDIFFUSE_DPOS | LRT_DR_RAY_FLAG_DNORM_DPOS;
+    else if (p.dr_render_mode == LRT_DR_RENDER_MODE_NORMAL) f |= LRT_DR_RAY_FLAG_DNORM_DPOS;
+  }
   return f;
 }
 
@@ -35,9 +38,7 @@ static int dr_check(Renderer* r, const MultiRendererDRPreset& p) {
   if (lay != LRT_SBS_LAYOUT_ID32F_IRGB32F && lay != LRT_SBS_LAYOUT_ID32F_IRGB32F_IN)
     return set_err(LRT_E_UNSUPPORTED, "differentiable rendering needs an indexed ID32F_IRGB32F(_IN) layout (MultiRendererDR.cpp:283-284)");
   if (r->header.brick_pad != 0) return set_err(LRT_E_UNSUPPORTED, "indexed DR layouts have brick_pad 0");
-  if (p.dr_render_mode != LRT_DR_RENDER_MODE_LINEAR_DEPTH && p.dr_render_mode != LRT_DR_RENDER_MODE_MASK &&
-      p.dr_render_mode != LRT_DR_RENDER_MODE_DIFFUSE)
-    return set_err(LRT_E_UNSUPPORTED, "DR render modes LAMBERT and NORMAL (normal derivatives) are not built");
+  if (p.dr_render_mode > LRT_DR_RENDER_MODE_NORMAL) return set_err(LRT_E_INVALID, "bad DR render mode");
   if (p.dr_loss_function > LRT_DR_LOSS_FUNCTION_MAE) return set_err(LRT_E_INVALID, "bad loss function");
   if (p.dr_border_sampling != LRT_DR_BORDER_SAMPLING_RANDOM) return set_err(LRT_E_UNSUPPORTED, "SVM border sampling is not built");
   if (p.dr_diff_mode != LRT_DR_DIFF_MODE_DEFAULT) return set_err(LRT_E_UNSUPPORTED, "finite-difference DR mode is not built");

@@ -11,7 +11,8 @@ no oracle/_ref build; see lrt_oracle.h for the restated functions.
 Parity unpinned (restated as written, no independent check; details in
 lrt_oracle.h): LiteMath's camera/matrix conventions, Embree's BVH topology,
 the ANALYTIC cubic's R formula, the smoothed-normal blend weights of _IN
-layouts, and the DR border sampling's statistics beyond seeded determinism.
+layouts, the DR border sampling's statistics beyond seeded determinism, and
+LAMBERT's light direction to the last bit.
 """
 import ctypes as C
 import os

@@ -22,7 +22,12 @@
  *    finders on smooth scenes only;
  *  - the smoothed-normal blend weights of ID32F_IRGB32F_IN scenes beyond the
  *    code path itself;
- *  - the DR border sampling's statistics beyond seeded determinism.
+ *  - the DR border sampling's statistics beyond seeded determinism;
+ *  - LAMBERT's light direction to the last bit (1/sqrt(3) rounded to float;
+ *    LiteMath's normalize of (1,1,1) may differ by an ulp).
+ * The DR normal-mode derivatives (NORMAL, LAMBERT, both layouts) are pinned
+ * by central differences of the FP64 loss, and the diffuse position term by
+ * the same differences times the reference's sz*bs/2 scale.
  */
 #ifndef LRT_ORACLE_H
 #define LRT_ORACLE_H

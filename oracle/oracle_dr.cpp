@@ -3,8 +3,10 @@
 //
 // TEST INFRASTRUCTURE ONLY (see lrt_oracle.h). Covers what RenderDR does for
 // the Newton root find that OptimizeFixedStructure forces
-// (MultiRendererDR.cpp:324) and the render modes LINEAR_DEPTH, MASK and
-// DIFFUSE; the other modes return an error. Gradients are accumulated in
+// (MultiRendererDR.cpp:324) and every render mode (DIFFUSE, LAMBERT with the
+// renderer's default lights, MASK, LINEAR_DEPTH, NORMAL), with the normal and
+// its derivatives from the hit voxel (ID32F_IRGB32F) or smoothed over the 8
+// voxels around the hit (ID32F_IRGB32F_IN). Gradients are accumulated in
 // double whatever the precision of the per-ray arithmetic (the reference sums
 // per-thread float buffers in a thread-dependent order).
 #include <omp.h>
@@ -46,11 +48,168 @@ template <typename T> struct Payload {  // RayDiffPayload, DR_common.h:59-72
   PDDist<T> dSd[64];
 };
 
+// 3x3 matrix as three rows; m * v takes the row dot products (x, y, z left to right)
+template <typename T> struct M3 {
+  V3<T> r[3];
+};
+template <typename T> static inline V3<T> mul(const M3<T>& m, V3<T> v) { return {dot(m.r[0], v), dot(m.r[1], v), dot(m.r[2], v)}; }
+
+// BVH2DR.cpp:339-404: Jacobian of v / |v|
+template <typename T> static M3<T> normalize_diff(V3<T> v) {
+  T a = dot(v, v);
+  T b = T(1) / (a * std::sqrt(a));  // the reference divides 1.0 (double) and rounds: the same value as the T division
+  return {{b * V3<T>(v.y * v.y + v.z * v.z, -v.x * v.y, -v.x * v.z), b * V3<T>(-v.x * v.y, v.x * v.x + v.z * v.z, -v.y * v.z),
+           b * V3<T>(-v.x * v.z, -v.y * v.z, v.x * v.x + v.y * v.y)}};
+}
+
+// BVH2DR.cpp:406-530: Hessian of the trilinear distance (zero diagonal)
+template <typename T> static M3<T> trilinear_ddp_ddp(const T v[8], V3<T> dp) {
+  T xy = (1 - dp.z) * v[0] + (dp.z) * v[1] + -(1 - dp.z) * v[2] + -(dp.z) * v[3] + -(1 - dp.z) * v[4] + -(dp.z) * v[5] +
+         (1 - dp.z) * v[6] + (dp.z) * v[7];
+  T xz = (1 - dp.y) * v[0] + -(1 - dp.y) * v[1] + (dp.y) * v[2] + -(dp.y) * v[3] + -(1 - dp.y) * v[4] + (1 - dp.y) * v[5] +
+         -(dp.y) * v[6] + (dp.y) * v[7];
+  T yz = (1 - dp.x) * v[0] + -(1 - dp.x) * v[1] + -(1 - dp.x) * v[2] + (1 - dp.x) * v[3] + (dp.x) * v[4] + -(dp.x) * v[5] +
+         -(dp.x) * v[6] + (dp.x) * v[7];
+  return {{V3<T>(0, xy, xz), V3<T>(xy, 0, yz), V3<T>(xz, yz, 0)}};
+}
+
+// BVH2DR.cpp:531-611: d(grad f)/d(values), rows x, y, z of 8
+template <typename T> static void trilinear_ddp_dvalues(V3<T> dp, T m[24]) {
+  m[0] = -(1 - dp.y) * (1 - dp.z); m[1] = -(1 - dp.y) * (dp.z); m[2] = -(dp.y) * (1 - dp.z); m[3] = -(dp.y) * (dp.z);
+  m[4] = (1 - dp.y) * (1 - dp.z);  m[5] = (1 - dp.y) * (dp.z);  m[6] = (dp.y) * (1 - dp.z);  m[7] = (dp.y) * (dp.z);
+  m[8] = -(1 - dp.x) * (1 - dp.z); m[9] = -(1 - dp.x) * (dp.z); m[10] = (1 - dp.x) * (1 - dp.z); m[11] = (1 - dp.x) * (dp.z);
+  m[12] = -(dp.x) * (1 - dp.z);    m[13] = -(dp.x) * (dp.z);    m[14] = (dp.x) * (1 - dp.z);     m[15] = (dp.x) * (dp.z);
+  m[16] = -(1 - dp.x) * (1 - dp.y); m[17] = (1 - dp.x) * (1 - dp.y); m[18] = -(1 - dp.x) * (dp.y); m[19] = (1 - dp.x) * (dp.y);
+  m[20] = -(dp.x) * (1 - dp.y);     m[21] = (dp.x) * (1 - dp.y);     m[22] = -(dp.x) * (dp.y);     m[23] = (dp.x) * (dp.y);
+}
+
+// BVH2DR.cpp:612-617: transition from 0 (x = e0) to 1 (x = e1)
+template <typename T> static T step01(T e0, T e1, T x) { return clampv((x - e0) / (e1 - e0), T(0), T(1)); }
+
+// BVH2DR.cpp:232-243: trilinear weights
+template <typename T> static void trilinear_weights(V3<T> dp, T w[8]) {
+  w[0] = (1 - dp.x) * (1 - dp.y) * (1 - dp.z); w[1] = (1 - dp.x) * (1 - dp.y) * (dp.z);
+  w[2] = (1 - dp.x) * (dp.y) * (1 - dp.z);     w[3] = (1 - dp.x) * (dp.y) * (dp.z);
+  w[4] = (dp.x) * (1 - dp.y) * (1 - dp.z);     w[5] = (dp.x) * (1 - dp.y) * (dp.z);
+  w[6] = (dp.x) * (dp.y) * (1 - dp.z);         w[7] = (dp.x) * (dp.y) * (dp.z);
+}
+
+// The renderer's default lights (Renderer/eye_ray_host.cpp:40): a direct
+// light along normalize(1,1,1) of colour 2/3 and an ambient light of 0.25.
+// The direction is 1/sqrt(3) rounded to float.
+constexpr float kLightDir = 0.57735026918962576f, kLightCol = 2.0f / 3.0f, kAmbient = 0.25f;
+
 template <typename T> struct DRTracer {
   const Scene& S;
   MultiRenderPreset preset;
   int dtdv_mode;  // 0 analytic (implicit function theorem), 1 central differences (BVH2DR.cpp:1075-1090)
+  bool need_normal = false;  // render modes LAMBERT and NORMAL (BVH2Common.cpp:20-27)
   DRTracer(const Scene& s, const MultiRenderPreset& p, int m) : S(s), preset(p), dtdv_mode(m) {}
+
+  // Hit normal and, with DNORM_DPOS, dNormal/dvalues (BVH2DR.cpp:1300-1484).
+  // values / idx: the hit voxel's corners; vp its position in the brick;
+  // start_q, t: the voxel-local ray segment start and root; dvo: first
+  // payload slot of the hit voxel's corners; dtdv: dt/dvalues in voxel units.
+  void NormalWithGrad(uint32_t ray_flags, V3<T> ray_dir, const T values[8], const uint32_t idx[8], V3<T> start_q, T t, V3<T> vp,
+                      uint32_t brick, unsigned dvo, const T dtdv[8], Payload<T>* relax, HitDR<T>* pHit) const {
+    const SdfSBSHeader& h = S.header;
+    const uint32_t v_size = h.brick_size + 2 * h.brick_pad + 1;
+    if ((h.aux_data & LRT_SBS_LAYOUT_MASK) != LRT_SBS_LAYOUT_ID32F_IRGB32F_IN) {
+      V3<T> p0 = start_q + t * ray_dir;
+      V3<T> g = eval_dist_trilinear_diff(values, p0);
+      pHit->normal = normalize(g);
+      if (ray_flags & LRT_DR_RAY_FLAG_DNORM_DPOS) {
+        M3<T> nd = normalize_diff(g);
+        T d1[24];
+        trilinear_ddp_dvalues(p0, d1);
+        V3<T> t1 = mul(trilinear_ddp_ddp(values, p0), ray_dir);
+        for (int i = 0; i < 8; i++) {
+          d1[i] += t1.x * dtdv[i];
+          d1[8 + i] += t1.y * dtdv[i];
+          d1[16 + i] += t1.z * dtdv[i];
+        }
+        for (int i = 0; i < 8; i++) relax->dSd[dvo + i].dNorm = mul(nd, V3<T>(d1[i], d1[8 + i], d1[16 + i]));
+      }
+      return;
+    }
+    // smoothed over the 8 voxels around the hit point (beta 0.5)
+    const T beta = T(0.5f);
+    const bool dn = (ray_flags & LRT_DR_RAY_FLAG_DNORM_DPOS) != 0;
+    V3<T> dp = start_q + t * ray_dir;
+    V3<T> nmq(dp.x < T(0.5f) ? step01(-beta, beta, dp.x) : step01(-beta, beta, dp.x - T(1)),
+              dp.y < T(0.5f) ? step01(-beta, beta, dp.y) : step01(-beta, beta, dp.y - T(1)),
+              dp.z < T(0.5f) ? step01(-beta, beta, dp.z) : step01(-beta, beta, dp.z - T(1)));
+    const unsigned k = dvo / 8;
+    V3<T> normals[8];
+    T wn[8];
+    if (dn) trilinear_weights(nmq, wn);
+    const int bs = (int)h.brick_size;
+    const SdfSBSNode& N = S.nodes[brick];
+    int vp0[3] = {dp.x < T(0.5f) ? (int)vp.x - 1 : (int)vp.x, dp.y < T(0.5f) ? (int)vp.y - 1 : (int)vp.y,
+                  dp.z < T(0.5f) ? (int)vp.z - 1 : (int)vp.z};
+    for (int i = 0; i < 8; i++) {
+      int VI[3] = {vp0[0] + ((i & 4) >> 2), vp0[1] + ((i & 2) >> 1), vp0[2] + (i & 1)};
+      int BP[3];
+      for (int a = 0; a < 3; a++) BP[a] = VI[a] >= 0 ? (VI[a] < bs ? 0 : 1) : -1;
+      uint32_t nid = 9 * (BP[0] + 1) + 3 * (BP[1] + 1) + (BP[2] + 1);
+      V3<T> dvp = V3<T>(T(VI[0]), T(VI[1]), T(VI[2])) - vp;
+      uint32_t nb = brick, nv[3];
+      if (nid == 13) {
+        for (int a = 0; a < 3; a++) nv[a] = (uint32_t)VI[a];
+      } else {
+        nb = S.values[N.data_offset + v_size * v_size * v_size + 8 + nid];
+        for (int a = 0; a < 3; a++) nv[a] = (uint32_t)(VI[a] - bs * BP[a]);
+      }
+      normals[i] = V3<T>(0, 0, 0);
+      if (nb == LRT_INVALID_IDX) continue;
+      T vn[8];
+      uint32_t in[8];
+      const uint32_t off = S.nodes[nb].data_offset;
+      for (int c = 0; c < 8; c++) {
+        uint32_t vId = (nv[0] + ((c & 4) >> 2)) * v_size * v_size + (nv[1] + ((c & 2) >> 1)) * v_size + (nv[2] + (c & 1));
+        in[c] = S.values[off + vId];
+        vn[c] = T(S.values_f[in[c]]);
+      }
+      V3<T> q = dp - dvp;
+      V3<T> g = eval_dist_trilinear_diff(vn, q);
+      normals[i] = normalize(g);
+      if (!dn) continue;
+      for (int c = 0; c < 8; c++) relax->dSd[8 * i + c].index = in[c];
+      M3<T> nd = normalize_diff(g);
+      T d1[24];
+      trilinear_ddp_dvalues(q, d1);
+      V3<T> t1 = mul(trilinear_ddp_ddp(vn, q), ray_dir);
+      for (int c = 0; c < 8; c++)  // d(n_i)/d(v_i)
+        relax->dSd[8 * i + c].dNorm = relax->dSd[8 * i + c].dNorm + wn[i] * mul(nd, V3<T>(d1[c], d1[8 + c], d1[16 + c]));
+      for (int c = 0; c < 8; c++)  // d(n_i)/d(v_k) through the hit point
+        relax->dSd[8 * k + c].dNorm =
+            relax->dSd[8 * k + c].dNorm + wn[i] * mul(nd, V3<T>(t1.x * dtdv[c], t1.y * dtdv[c], t1.z * dtdv[c]));
+    }
+    T w[8];
+    trilinear_weights(nmq, w);
+    V3<T> tri = w[0] * normals[0];
+    for (int i = 1; i < 8; i++) tri = tri + w[i] * normals[i];
+    pHit->normal = normalize(tri);
+    if (!dn) return;
+    M3<T> nd = normalize_diff(tri);
+    // dtrilinear3/dnmq by columns (BVH2DR.cpp:257-337)
+    V3<T> cx, cy, cz;
+    for (int i = 0; i < 8; i++) {
+      T ax = (i & 4) ? T(1) : T(-1), ay = (i & 2) ? T(1) : T(-1), az = (i & 1) ? T(1) : T(-1);
+      T wx = (i & 4) ? nmq.x : 1 - nmq.x, wy = (i & 2) ? nmq.y : 1 - nmq.y, wz = (i & 1) ? nmq.z : 1 - nmq.z;
+      V3<T> ex = (ax * wy * wz) * normals[i], ey = (wx * ay * wz) * normals[i], ez = (wx * wy * az) * normals[i];
+      cx = i == 0 ? ex : cx + ex;
+      cy = i == 0 ? ey : cy + ey;
+      cz = i == 0 ? ez : cz + ez;
+    }
+    for (int c = 0; c < 8; c++) {
+      V3<T> ddp(ray_dir.x * dtdv[c], ray_dir.y * dtdv[c], ray_dir.z * dtdv[c]);  // dnmq/ddp taken as 1
+      relax->dSd[8 * k + c].dNorm = relax->dSd[8 * k + c].dNorm + (cx * ddp.x + cy * ddp.y + cz * ddp.z);
+    }
+    for (int i = 0; i < 8; i++)
+      if (length(normals[i]) > T(0.5f))
+        for (int c = 0; c < 8; c++) relax->dSd[8 * i + c].dNorm = mul(nd, relax->dSd[8 * i + c].dNorm);
+  }
 
   // BVH2DR.cpp:623-1073, Newton branch; returns hit ? t : qFar + EPS
   T Intersect(uint32_t ray_flags, V3<T> ray_dir, const T values[8], T d, T qFar, V3<T> start_q, Payload<T>* relax) const {
@@ -275,6 +434,7 @@ template <typename T> struct DRTracer {
               relax->dSd[dvo + i].dDist = d * dtdv[i];
             }
           }
+          if (need_normal) NormalWithGrad(ray_flags, ray_dir, values, idx, start_q, t, vp, brick, dvo, dtdv, relax, pHit);
           V3<T> pos = ray_pos + pHit->t * ray_dir;
           V3<T> dp = (pos - bmin) * (T(0.5f) * sz);
           uint32_t t_off = N.data_offset + v_size * v_size * v_size;
@@ -376,14 +536,47 @@ template <typename T> struct DRTracer {
 
 template <typename T> static T lin_depth(T z) { return (z - T(kZNear)) / (T(kZFar) - T(kZNear)); }
 
-// CalculateColor (MultiRendererDR.cpp:855-898) for the supported modes
-template <typename T> static V3<T> calc_color(uint32_t mode, const HitDR<T>& h) {
+// CalculateColorWithGrad (MultiRendererDR.cpp:900-980); CalculateColor
+// (:855-898) is its colour. dCdD / dCdN: dColor/dDiffuse and dColor/dNormal.
+template <typename T> static V3<T> calc_color_grad(uint32_t mode, const HitDR<T>& h, M3<T>& dCdD, M3<T>& dCdN) {
+  const V3<T> z(0, 0, 0);
+  dCdD = {{V3<T>(1, 0, 0), V3<T>(0, 1, 0), V3<T>(0, 0, 1)}};
+  dCdN = {{z, z, z}};
   switch (mode) {
     case LRT_DR_RENDER_MODE_DIFFUSE: return h.color;
-    case LRT_DR_RENDER_MODE_MASK: return V3<T>(1, 1, 1);
+    case LRT_DR_RENDER_MODE_LAMBERT: {
+      V3<T> c(0, 0, 0);
+      dCdD = {{z, z, z}};
+      const T ld = T(kLightDir), lcs = T(kLightCol), las = T(kAmbient);
+      const V3<T> L{ld, ld, ld}, lc{lcs, lcs, lcs}, la{las, las, las};
+      T q = mx(T(0), dot(h.normal, L));  // direct light
+      c = c + (lc * h.color) * q;
+      if (q > T(0)) {
+        dCdD.r[0] = dCdD.r[0] + V3<T>(lc.x * q, 0, 0);
+        dCdD.r[1] = dCdD.r[1] + V3<T>(0, lc.y * q, 0);
+        dCdD.r[2] = dCdD.r[2] + V3<T>(0, 0, lc.z * q);
+        dCdN.r[0] = dCdN.r[0] + (lc.x * h.color.x) * L;
+        dCdN.r[1] = dCdN.r[1] + (lc.y * h.color.y) * L;
+        dCdN.r[2] = dCdN.r[2] + (lc.z * h.color.z) * L;
+      }
+      c = c + la * h.color;  // ambient light
+      dCdD.r[0] = dCdD.r[0] + V3<T>(la.x, 0, 0);
+      dCdD.r[1] = dCdD.r[1] + V3<T>(0, la.y, 0);
+      dCdD.r[2] = dCdD.r[2] + V3<T>(0, 0, la.z);
+      return c;
+    }
+    case LRT_DR_RENDER_MODE_MASK: dCdD = {{z, z, z}}; return V3<T>(1, 1, 1);
     case LRT_DR_RENDER_MODE_LINEAR_DEPTH: { T d = lin_depth(h.t); return V3<T>(d, d, d); }
+    case LRT_DR_RENDER_MODE_NORMAL:
+      dCdD = {{z, z, z}};
+      dCdN = {{V3<T>(1, 0, 0), V3<T>(0, 1, 0), V3<T>(0, 0, 1)}};
+      return h.normal;
     default: return V3<T>(1, 0, 1);
   }
+}
+template <typename T> static V3<T> calc_color(uint32_t mode, const HitDR<T>& h) {
+  M3<T> a, b;
+  return calc_color_grad(mode, h, a, b);
 }
 
 // t_samples_to_threshold (MultiRendererDR.cpp:997-1063)
@@ -417,7 +610,7 @@ static int dr_impl(const Scene& S, const MultiRendererDRPreset& dr, const Camera
                    uint32_t H, uint32_t seed, const float* ref, int nthreads, int dtdv_mode, double* out_image, double* out_grad,
                    orc_dr_stats* stats) {
   const uint32_t mode = dr.dr_render_mode;
-  if (mode != LRT_DR_RENDER_MODE_LINEAR_DEPTH && mode != LRT_DR_RENDER_MODE_MASK && mode != LRT_DR_RENDER_MODE_DIFFUSE) return -3;
+  if (mode > LRT_DR_RENDER_MODE_NORMAL) return -3;
   if (dr.dr_loss_function != LRT_DR_LOSS_FUNCTION_MSE && dr.dr_loss_function != LRT_DR_LOSS_FUNCTION_MAE) return -3;
   if (S.type != LRT_TYPE_SDF_SBS_COL) return -4;  // DR needs indexed colour layouts (MultiRendererDR.cpp:283-284)
   MultiRenderPreset p{};
@@ -425,6 +618,7 @@ static int dr_impl(const Scene& S, const MultiRendererDRPreset& dr, const Camera
   p.ray_gen_mode = LRT_RAY_GEN_MODE_RANDOM;
   p.spp = dr.spp;
   DRTracer<T> tr(S, p, dtdv_mode);
+  tr.need_normal = mode == LRT_DR_RENDER_MODE_LAMBERT || mode == LRT_DR_RENDER_MODE_NORMAL;
   const uint32_t spp = dr.spp;
   const size_t np = (size_t)W * H;
   std::vector<uint32_t> packed = PackXY(W, H);
@@ -438,8 +632,11 @@ static int dr_impl(const Scene& S, const MultiRendererDRPreset& dr, const Camera
   else if (dr.dr_input_type == LRT_DR_INPUT_TYPE_LINEAR_DEPTH) ray_flags = LRT_DR_RAY_FLAG_DDIST_DPOS;
   else {
     if (dr.dr_reconstruction_flags & LRT_DR_RECONSTRUCTION_FLAG_COLOR) ray_flags |= LRT_DR_RAY_FLAG_DDIFFUSE_DCOLOR;
-    if ((dr.dr_reconstruction_flags & LRT_DR_RECONSTRUCTION_FLAG_GEOMETRY) && mode == LRT_DR_RENDER_MODE_DIFFUSE)
-      ray_flags |= LRT_DR_RAY_FLAG_DDIFFUSE_DPOS;
+    if (dr.dr_reconstruction_flags & LRT_DR_RECONSTRUCTION_FLAG_GEOMETRY) {
+      if (mode == LRT_DR_RENDER_MODE_DIFFUSE) ray_flags |= LRT_DR_RAY_FLAG_DDIFFUSE_DPOS;
+      else if (mode == LRT_DR_RENDER_MODE_LAMBERT) ray_flags |= LRT_DR_RAY_FLAG_DDIFFUSE_DPOS | LRT_DR_RAY_FLAG_DNORM_DPOS;
+      else if (mode == LRT_DR_RENDER_MODE_NORMAL) ray_flags |= LRT_DR_RAY_FLAG_DNORM_DPOS;
+    }
   }
   const bool mse = dr.dr_loss_function == LRT_DR_LOSS_FUNCTION_MSE;
   auto loss_grad = [&](const float c[4], const float r[4], T g[4]) {
@@ -469,13 +666,24 @@ static int dr_impl(const Scene& S, const MultiRendererDRPreset& dr, const Camera
         Payload<T>& py = pl[s];
         HitDR<T> hit = tr.NearestHitWithGrad(ray_flags, pn, df, &py);
         if (hit.primId == kInvalid) continue;
-        V3<T> color = calc_color<T>(mode, hit);
+        M3<T> dCdD, dCdN;
+        V3<T> color = calc_color_grad<T>(mode, hit, dCdD, dCdN);
         res = res + color / T(spp);
-        // dColor/dDiffuse is I for DIFFUSE (and left at I for LINEAR_DEPTH), 0 for MASK
-        if ((ray_flags & LRT_DR_RAY_FLAG_DDIFFUSE_DCOLOR) && mode != LRT_DR_RENDER_MODE_MASK)
+        if (!tr.need_normal) {
+          // dColor/dDiffuse is I for DIFFUSE (and left at I for LINEAR_DEPTH), 0 for MASK
+          if ((ray_flags & LRT_DR_RAY_FLAG_DDIFFUSE_DCOLOR) && mode != LRT_DR_RENDER_MODE_MASK)
+            for (int k = 0; k < 8; k++)
+              if (py.dSc_index[k] != kInvalid)
+                pds.push_back({py.dSc_index[k] | 0x80000000u, V3<T>(py.dSc_value[k], py.dSc_value[k], py.dSc_value[k])});
+        } else if (ray_flags & LRT_DR_RAY_FLAG_DDIFFUSE_DCOLOR) {
           for (int k = 0; k < 8; k++)
-            if (py.dSc_index[k] != kInvalid) pds.push_back({py.dSc_index[k] | 0x80000000u, V3<T>(py.dSc_value[k], py.dSc_value[k], py.dSc_value[k])});
-        if (ray_flags & LRT_DR_RAY_FLAG_DDIFFUSE_DPOS) {
+            if (py.dSc_index[k] != kInvalid)
+              pds.push_back({py.dSc_index[k] | 0x80000000u, mul(dCdD, V3<T>(py.dSc_value[k], py.dSc_value[k], py.dSc_value[k]))});
+        }
+        if (tr.need_normal && (ray_flags & (LRT_DR_RAY_FLAG_DDIFFUSE_DPOS | LRT_DR_RAY_FLAG_DNORM_DPOS))) {
+          for (int k = 0; k < 64; k++)
+            if (py.dSd[k].index != kInvalid) pds.push_back({py.dSd[k].index, mul(dCdD, py.dSd[k].dDiffuse) + mul(dCdN, py.dSd[k].dNorm)});
+        } else if (ray_flags & LRT_DR_RAY_FLAG_DDIFFUSE_DPOS) {
           for (int k = 0; k < 64; k++)
             if (py.dSd[k].index != kInvalid) pds.push_back({py.dSd[k].index, py.dSd[k].dDiffuse});  // dColor_dDiffuse = I
         } else if (ray_flags & LRT_DR_RAY_FLAG_DDIST_DPOS) {

@@ -362,6 +362,97 @@ def test_dr_gradient_matches_loss_finite_differences_fp64():
     assert np.linalg.norm(g - g_fd) / np.linalg.norm(g) < 0.1
 
 
+def _colour_indices(sbs):
+    """values_f indices of the colour points (3 floats at each index stored
+    after a brick's distance indices) of an indexed scene."""
+    vs = sbs.header.brick_size + 2 * sbs.header.brick_pad + 1
+    off = sbs.nodes[:, 2].astype(np.int64)
+    return np.unique(sbs.values[(off[:, None] + vs ** 3 + np.arange(8)[None, :]).ravel()].astype(np.int64))
+
+
+def _random_colours(sbs, seed):
+    """Every colour point of an indexed scene gets a seeded random colour."""
+    ci = _colour_indices(sbs)
+    vf = sbs.values_f.copy()
+    rng = np.random.default_rng(seed)
+    for k in range(3):
+        vf[ci + k] = rng.uniform(0.1, 0.9, ci.size).astype(np.float32)
+    return L.SdfSBS(sbs.header, sbs.nodes, sbs.values, vf)
+
+
+def _dr_fd_problem(mode, smoothed, flags, colours):
+    tgt = L.grid_sbs("torus", 0.6, 0.2, 0.0, 1, seed=1, brick_count=4, brick_size=4, with_neighbors=smoothed)
+    ini = L.grid_sbs("sphere", 0.5, 0.0, 0.0, 1, seed=1, brick_count=4, brick_size=4, with_neighbors=smoothed)
+    if colours:
+        ini, tgt = _random_colours(ini, 3), _random_colours(tgt, 4)
+    pairs, _ = L.build_bvh(ini)
+    st, si = oracle.Scene(tgt, pairs), oracle.Scene(ini, pairs)
+    wv, pr = L.orbit_camera(3, 0)
+    W = H = 32
+    dr = L.getDefaultPresetDR()
+    dr.dr_render_mode, dr.dr_input_type, dr.dr_reconstruction_flags, dr.border_spp = mode, 0, flags, 0
+    n = ini.values_f.size
+    img, _, _ = st.dr_render_grad(dr, wv, pr, W, H, np.zeros((H, W, 4), np.float32), n, precision=64)
+    ref = img.astype(np.float32).reshape(H, W, 4)
+    _, g, _ = si.dr_render_grad(dr, wv, pr, W, H, ref, n, precision=64, dtdv_mode=0)
+    g_mult = float(np.float32((n + 1.0) ** (2.0 / 3.0) / (W * H)))
+
+    def fd(i, h=1e-4):
+        vf = ini.values_f.astype(np.float32)
+        losses = []
+        for sgn in (1, -1):
+            v = vf.copy()
+            v[i] += sgn * h
+            si.set_values_f(v)
+            losses.append(si.dr_render_grad(dr, wv, pr, W, H, ref, n, precision=64, dtdv_mode=0)[2]["loss"])
+        si.set_values_f(vf)
+        return (losses[0] - losses[1]) / (2 * h) * W * H * g_mult
+
+    return ini, g, fd
+
+
+@pytest.mark.parametrize("mode,smoothed", [(4, False), (4, True), (1, False), (1, True)])
+def test_dr_normal_modes_gradient_matches_loss_finite_differences_fp64(mode, smoothed):
+    """NORMAL (4) and LAMBERT (1, the renderer's default lights) render modes
+    with geometry reconstruction: dLoss/dS through the hit normal (from the
+    hit voxel, or smoothed over the 8 voxels around the hit with the _IN
+    layout's neighbour table) and the hit point, against central differences
+    of the oracle's own FP64 loss; border term off as above. The scene has one
+    colour, so LAMBERT's diffuse term does not move with the hit point (that
+    term is checked on its own below)."""
+    _, g, fd = _dr_fd_problem(mode, smoothed, 2, colours=False)
+    idx = np.argsort(-np.abs(g))[:25]
+    ok = sum(abs(fd(i) - g[i]) <= 5e-3 * abs(g[i]) + 1e-9 for i in idx)
+    assert ok >= 22, ok
+
+
+def test_dr_diffuse_position_term_is_reference_scaled():
+    """The diffuse colour's derivative through the hit point: the reference
+    scales the ray direction by 0.5*sz*dt/dvalues with dt in voxel units
+    (BVH2DR.cpp:1517-1522), which is sz*brick_size/2 times the derivative of
+    the rendered colour (the world-space hit moves by d*dt, d = 2/(sz*bs)).
+    The oracle keeps that scale: its gradient is exactly 8x (sz 4, bs 4) the
+    central differences of its own FP64 loss."""
+    _, g, fd = _dr_fd_problem(0, False, 2, colours=True)
+    idx = np.argsort(-np.abs(g))[:25]
+    ok = sum(abs(8.0 * fd(i) - g[i]) <= 5e-3 * abs(g[i]) + 1e-9 for i in idx)
+    assert ok >= 22, ok
+
+
+@pytest.mark.parametrize("smoothed", [False, True])
+def test_dr_lambert_colour_gradient_matches_finite_differences(smoothed):
+    """LAMBERT with colour reconstruction: dLoss/d(colour) = dColor/dDiffuse
+    (direct light q = max(0, n.l) plus ambient) x the trilinear weights, against
+    central differences of the FP64 loss over the colour parameters."""
+    ini, g, fd = _dr_fd_problem(1, smoothed, 3, colours=True)
+    ci = _colour_indices(ini)
+    cols = np.concatenate([ci, ci + 1, ci + 2])
+    idx = cols[np.argsort(-np.abs(g[cols]))[:20]]
+    assert np.abs(g[idx]).min() > 0
+    ok = sum(abs(fd(i) - g[i]) <= 1e-3 * abs(g[i]) + 1e-9 for i in idx)
+    assert ok >= 19, ok
+
+
 def test_adam_step_closed_form():
     rng = np.random.default_rng(9)
     n = 64

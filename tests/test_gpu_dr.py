@@ -226,6 +226,52 @@ def test_render_dr_tree_forms_match_fp32_twin(case):
         assert np.linalg.norm(grad - o32) / np.linalg.norm(o32) <= 1e-5
 
 
+@pytest.mark.parametrize("mode,smoothed,flags,spp", [(4, False, 2, 1), (4, True, 2, 1), (1, False, 3, 1), (1, True, 3, 1),
+                                                 (4, True, 2, 4), (1, False, 2, 1)])
+def test_render_dr_normal_modes_match_oracle(mode, smoothed, flags, spp):
+    """NORMAL (4) and LAMBERT (1, default lights) render modes: the hit normal
+    from the hit voxel or smoothed over its 8 neighbours (_IN layout), its
+    derivatives, dColor/dDiffuse and dColor/dNormal, colour reconstruction
+    (flags 3) and the colour-input border term. Image, border statistics and
+    dLoss/dS within 1e-5 of the FP32 twin; no further from FP64 than the twin."""
+    tgt = L.grid_sbs("torus", 0.6, 0.2, 0.02, 3, seed=1, brick_count=8, brick_size=4, with_neighbors=smoothed)
+    ini = L.grid_sbs("sphere", 0.5, 0.0, 0.0, 3, seed=1, brick_count=8, brick_size=4, with_neighbors=smoothed)
+    tgt, ini = _random_colours(tgt, 21), _random_colours(ini, 22)
+    cams = [L.orbit_camera(3, i) for i in range(2)]
+    wvs = np.stack([c[0] for c in cams])
+    prs = np.stack([c[1] for c in cams])
+    dr = _preset(border_spp=64)
+    dr.dr_render_mode, dr.dr_input_type, dr.dr_reconstruction_flags, dr.spp = mode, 0, flags, spp
+    pairs_t, _ = L.build_bvh(tgt)
+    st = oracle.Scene(tgt, pairs_t)
+    zero = np.zeros((H, W, 4), np.float32)
+    refs = np.stack([st.dr_render_grad(dr, wvs[v], prs[v], W, H, zero, tgt.values_f.size, precision=64)[0].astype(np.float32)
+                     for v in range(2)])
+    g = L.MultiRendererDR()
+    g.SetScene(ini)
+    g.SetReference(refs, wvs, prs)
+    P = ini.values_f.size
+    pairs, _ = g.GetBVH()
+    sc = oracle.Scene(ini, pairs)
+    view = 1
+    ref = refs[view].copy()
+    ref[..., 3] = (np.linalg.norm(ref[..., :3], axis=-1) >= 0.001)
+    img, grad, loss, ms = g.RenderDR(dr, view, seed=55, n_params=P)
+    o_img, o32, st32 = sc.dr_render_grad(dr, wvs[view], prs[view], W, H, ref, P, seed=55, precision=32)
+    _, o64, st64 = sc.dr_render_grad(dr, wvs[view], prs[view], W, H, ref, P, seed=55, precision=64)
+    bp, bh = g.LastStats()
+    print(mode, smoothed, flags, spp, "loss", loss, st32, "border", bp, bh, "ms", ms)
+    assert np.mean(np.all(img == o_img.astype(np.float32), axis=-1)) > 0.999
+    assert bp == st32["border_pixels"] and abs(bh - st32["border_rays_hit"]) <= 2
+    assert abs(loss - st64["loss"]) <= 1e-5 * max(1.0, abs(st64["loss"]))
+    nz = np.linalg.norm(o64)
+    assert nz > 0
+    assert np.linalg.norm(grad - o32) / np.linalg.norm(o32) <= 1e-5
+    fp32_err = np.linalg.norm(o32 - o64) / nz
+    print("fp64 distance: gpu", np.linalg.norm(grad - o64) / nz, "fp32 oracle", fp32_err)
+    assert np.linalg.norm(grad - o64) / nz <= max(1e-4, 1.2 * fp32_err)
+
+
 def test_adam_step_matches_oracle():
     rng = np.random.default_rng(3)
     n = 1000
