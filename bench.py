@@ -15,7 +15,8 @@ by the kernel's average duration (HIP events on the launch stream).
 FP32, OpenMP over the host cores) on the same workload. The DR leg
 (`dr_fwd_bwd_mrays_s`) times RenderDR + Adam on the torus-from-sphere problem (torus target,
 sphere start, 128^3 indexed SBS, 16 views at 512x512, border_spp 256, 200
-Adam steps at lr 1e-3).
+Adam steps at lr 1e-3). The `phong_shadow` leg renders the plain views with
+Phong shading, one shadow ray toward the direct light per hit.
 """
 import argparse
 import json
@@ -249,6 +250,7 @@ def main():
     ap.add_argument("--lod", type=int, default=128,
                     help="brick LOD of the plain workload's scene (128: 512^3 effective, the headline workload)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched many-scene leg")
+    ap.add_argument("--no-shade", action="store_true", help="skip the Phong-with-shadow-rays leg")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--mlp-scenes", type=int, default=32)
     ap.add_argument("--mlp-events", type=int, default=16_000_000)
@@ -299,6 +301,27 @@ def main():
     achieved = alg_bytes / (avg_kernel_ms * 1e-3) / 1e9
     log(f"work/ray: pairs {counts[0] / rays:.3f} bricks {counts[1] / rays:.4f} voxels {counts[2] / rays:.4f}; "
         f"algorithmic {alg_bytes / 1e9:.3f} GB/launch -> {achieved:.0f} GB/s")
+
+    shade_info = None
+    if not args.no_shade:
+        # Phong shading of the same views: a shadow ray toward the direct light
+        # from every hit, RGBA out (eye_ray.cpp:426-478)
+        hits = 0
+        for v in range(N_VIEWS):
+            r.RenderViewsDevice(wvs[v:v + 1], prs[v:v + 1], W, H, 0, timed=True)
+            hits += int(np.count_nonzero(r.DownloadView(0, W, H)[2] != 0xFFFFFFFF))
+        ps = L.getDefaultPreset()
+        ps.sdf_node_intersect = L.SDF_OCTREE_NODE_INTERSECT_NEWTON
+        ps.render_mode = L.MULTI_RENDER_MODE_PHONG_NO_TEX
+        r.SetPreset(ps)
+        r.RenderViewsDevice(wvs, prs, W, H, 1, timed=True)
+        sms = [r.RenderViewsDevice(wvs, prs, W, H, 1, timed=True) for _ in range(5)]
+        r.SetPreset(p)
+        sms = float(np.mean(sms))
+        shade_info = {"render_mode": "PHONG_NO_TEX", "kernel_ms": round(sms, 3), "primary_rays": rays, "shadow_rays": hits,
+                      "mrays_s_primary_plus_shadow": round((rays + hits) / sms / 1e3, 1)}
+        log(f"phong + shadow rays: {sms:.3f} ms/launch, {rays} primary + {hits} shadow rays -> "
+            f"{(rays + hits) / sms / 1e3:.1f} Mrays/s")
 
     mlp_mrays, mlp_info, mlp_params = None, None, None
     if not args.no_mlp:
@@ -409,6 +432,7 @@ def main():
         "mlp_seeded_mrays_s": mlp_mrays,
         "mlp": mlp_info,
         "batch": batch_info,
+        "phong_shadow": shade_info,
         "dr_fwd_bwd_mrays_s": None if dr_mrays is None else round(dr_mrays, 2),
         "dr": dr_info,
     }

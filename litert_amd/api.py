@@ -28,8 +28,13 @@ MULTI_RENDER_MODE_PRIMITIVE = 5
 MULTI_RENDER_MODE_TYPE = 6
 MULTI_RENDER_MODE_GEOM = 7
 MULTI_RENDER_MODE_NORMAL = 8
+MULTI_RENDER_MODE_BARYCENTRIC = 9
 MULTI_RENDER_MODE_ST_ITERATIONS = 10
+MULTI_RENDER_MODE_PHONG_NO_TEX = 12
+MULTI_RENDER_MODE_TEX_COORDS = 15
 MULTI_RENDER_MODE_DIFFUSE = 16
+MULTI_RENDER_MODE_LAMBERT = 17
+MULTI_RENDER_MODE_PHONG = 18
 NORMAL_MODE_GEOMETRY, NORMAL_MODE_VERTEX, NORMAL_MODE_SDF_SMOOTHED = 0, 1, 2
 RAY_GEN_MODE_REGULAR, RAY_GEN_MODE_RANDOM = 0, 1
 REPRESENTATION_MODE_SURFACE, REPRESENTATION_MODE_VOLUME = 0, 1

@@ -5,10 +5,25 @@
 
 namespace lrtd {
 
+// Phong's specular power: x^32 by five squarings (the reference calls pow)
+__device__ __forceinline__ float pow32(float x) {
+  x = x * x;
+  x = x * x;
+  x = x * x;
+  x = x * x;
+  return x * x;
+}
+
+// True for the modes whose colour needs a shadow ray toward the direct light.
+__device__ __forceinline__ bool needs_shadow(uint32_t mode) {
+  return mode == LRT_RENDER_MODE_PHONG || mode == LRT_RENDER_MODE_PHONG_NO_TEX;
+}
+
 // Render modes that apply to SBS geometry (eye_ray.cpp:162-489) with the
 // renderer's default lights: direct normalize(1,1,1) colour 2/3 and ambient
-// 0.25 (eye_ray_host.cpp:40).
-__device__ __forceinline__ float4 shade(uint32_t mode, const Hit& hit, v3 dir) {
+// 0.25 (eye_ray_host.cpp:40). shadow: 1 if the Phong shadow ray toward the
+// direct light found nothing, else 0.
+__device__ __forceinline__ float4 shade(uint32_t mode, const Hit& hit, v3 dir, float shadow = 1.0f) {
   if (hit.primId == 0xFFFFFFFFu) return make_float4(0, 0, 0, 1);
   const float z = hit.t, z_near = 0.1f, z_far = 10.0f;
   const uint32_t type = hit.geomId >> LRT_SH_TYPE;
@@ -51,6 +66,32 @@ __device__ __forceinline__ float4 shade(uint32_t mode, const Hit& hit, v3 dir) {
       float q = mx(0.0f, dot(norm, ld));
       float c = (2.0f / 3.0f) * q + 0.25f;
       return make_float4(c, c, c, 1);
+    }
+    case LRT_RENDER_MODE_BARYCENTRIC: return make_float4(hit.coords[0], hit.coords[1], 1 - hit.coords[0] - hit.coords[1], 1);
+    case LRT_RENDER_MODE_TEX_COORDS: return make_float4(hit.coords[0], hit.coords[1], 0, 1);
+    case LRT_RENDER_MODE_LAMBERT:
+    case LRT_RENDER_MODE_PHONG:
+    case LRT_RENDER_MODE_PHONG_NO_TEX: {
+      // colour: white for PHONG_NO_TEX, the brick colour for colour layouts,
+      // else the default material's (0,1,1) texture (eye_ray.cpp:376-392, 426-441)
+      v3 col(1, 1, 1);
+      if (mode != LRT_RENDER_MODE_PHONG_NO_TEX) {
+        col = type == LRT_TYPE_SDF_SBS_COL
+                  ? v3(roundf(hit.coords[0]) / 255.0f, hit.coords[0] - floorf(hit.coords[0]), roundf(hit.coords[1]) / 255.0f)
+                  : v3(0, 1, 1);
+      }
+      const v3 ld = normalize(v3(1, 1, 1));
+      const v3 lc(2.0f / 3.0f, 2.0f / 3.0f, 2.0f / 3.0f), la(0.25f, 0.25f, 0.25f);
+      const float q = mx(0.0f, dot(norm, ld));
+      v3 c = (lc * col) * q;  // direct light, diffuse part
+      if (mode != LRT_RENDER_MODE_LAMBERT) {  // Phong: shadowed diffuse + specular (eye_ray.cpp:443-475)
+        const float Kd = 0.25f, Ks = 0.25f;
+        const v3 refl = -1.0f * ld + (2.0f * dot(norm, ld)) * norm;
+        const float e = Kd * mx(0.0f, dot(norm, ld)) + Ks * pow32(mx(0.0f, dot(norm, refl)));
+        c = c + ((shadow * lc) * e) * col;
+      }
+      c = c + la * col;  // ambient light
+      return make_float4(c.x, c.y, c.z, 1);
     }
     default: return make_float4(1, 0, 1, 1);
   }

@@ -68,7 +68,15 @@ static void render_impl(const Scene& S, const Camera& cam, const MultiRenderPres
           if (out_n) { out_n[2 * pi] = (double)hit.coords[2]; out_n[2 * pi + 1] = (double)hit.coords[3]; }
         }
         if (out_rgba) {
-          V4<T> c = shade<T>(p, hit, dfar);
+          T shadow = T(1);
+          if (needs_shadow(p.render_mode) && hit.primId != 0xFFFFFFFFu) {
+            // shadow ray toward the direct light (eye_ray.cpp:448-468)
+            V3<float> ldf = normalize(V3<float>(1, 1, 1));
+            V3<T> surf = V3<T>(pn.x, pn.y, pn.z) + (hit.t - T(1e-6f)) * V3<T>(dfar.x, dfar.y, dfar.z);
+            Hit<T> sh = tr.RayQuery_NearestHit(V4<T>{surf.x, surf.y, surf.z, pn.w}, V4<T>{T(ldf.x), T(ldf.y), T(ldf.z), dfar.w});
+            shadow = sh.primId == 0xFFFFFFFFu ? T(1) : T(0);
+          }
+          V4<T> c = shade<T>(p, hit, dfar, shadow);
           acc = V4<T>{acc.x + c.x, acc.y + c.y, acc.z + c.z, acc.w + c.w};
         }
       }

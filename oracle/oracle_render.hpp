@@ -601,9 +601,19 @@ static const uint32_t kPalette[20] = {0xffe6194b, 0xff3cb44b, 0xffffe119, 0xff00
                                       0xfff032e6, 0xffd2f53c, 0xfffabebe, 0xff008080, 0xffe6beff, 0xffaa6e28, 0xfffffac8,
                                       0xff800000, 0xffaaffc3, 0xff808000, 0xffffd8b1, 0xff000080, 0xff808080};
 
+// Phong's specular power x^32 as five squarings (the reference calls pow;
+// which overload LiteMath resolves it to is not known here)
+template <typename T> static inline T pow32(T x) {
+  for (int i = 0; i < 5; i++) x = x * x;
+  return x;
+}
+
+static inline bool needs_shadow(uint32_t mode) { return mode == LRT_RENDER_MODE_PHONG || mode == LRT_RENDER_MODE_PHONG_NO_TEX; }
+
 // eye_ray.cpp:162-489, the render modes that apply to SBS geometry with the
-// renderer's default lights (eye_ray_host.cpp:40).
-template <typename T> static inline V4<T> shade(const MultiRenderPreset& p, const Hit<T>& hit, V4<T> rayDir) {
+// renderer's default lights (eye_ray_host.cpp:40). shadow: 1 if the Phong
+// shadow ray toward the direct light (eye_ray.cpp:448-468) hit nothing, else 0.
+template <typename T> static inline V4<T> shade(const MultiRenderPreset& p, const Hit<T>& hit, V4<T> rayDir, T shadow = T(1)) {
   if (hit.primId == 0xFFFFFFFFu) return {0, 0, 0, 1};
   T z = hit.t;
   const T z_near = T(0.1f), z_far = T(10);
@@ -645,6 +655,34 @@ template <typename T> static inline V4<T> shade(const MultiRenderPreset& p, cons
       T q = mx(T(0), dot(norm, V3<T>(T(ld.x), T(ld.y), T(ld.z))));
       T c = T(2.0f / 3.0f) * q + T(0.25);
       return {c, c, c, 1};
+    }
+    case LRT_RENDER_MODE_BARYCENTRIC: return {hit.coords[0], hit.coords[1], 1 - hit.coords[0] - hit.coords[1], 1};
+    case LRT_RENDER_MODE_TEX_COORDS: return {hit.coords[0], hit.coords[1], 0, 1};  // tc = coords[0..1] for SBS
+    case LRT_RENDER_MODE_LAMBERT:
+    case LRT_RENDER_MODE_PHONG:
+    case LRT_RENDER_MODE_PHONG_NO_TEX: {  // eye_ray.cpp:376-483
+      V3<T> col(1, 1, 1);
+      if (p.render_mode != LRT_RENDER_MODE_PHONG_NO_TEX)
+        col = type == LRT_TYPE_SDF_SBS_COL
+                  ? V3<T>(std::round(hit.coords[0]) / T(255), hit.coords[0] - std::floor(hit.coords[0]), std::round(hit.coords[1]) / T(255))
+                  : V3<T>(0, 1, 1);  // default material: the (0,1,1,1) texture
+      V3<float> ldf = normalize(V3<float>(1, 1, 1));
+      const V3<T> ld(T(ldf.x), T(ldf.y), T(ldf.z));
+      const T lcs = T(2.0f / 3.0f), las = T(0.25f);
+      const V3<T> lc(lcs, lcs, lcs), la(las, las, las);
+      V3<T> c(0, 0, 0);
+      // light 0: direct
+      T q = mx(T(0), dot(norm, ld));
+      c = c + (lc * col) * q;
+      if (p.render_mode != LRT_RENDER_MODE_LAMBERT) {
+        const T Kd = T(0.25f), Ks = T(0.25f);
+        V3<T> refl = T(-1) * ld + (T(2) * dot(norm, ld)) * norm;
+        T e = Kd * mx(T(0), dot(norm, ld)) + Ks * pow32(mx(T(0), dot(norm, refl)));
+        c = c + ((shadow * lc) * e) * col;
+      }
+      // light 1: ambient
+      c = c + la * col;
+      return {c.x, c.y, c.z, 1};
     }
     default: return {1, 0, 1, 1};
   }

@@ -13,7 +13,7 @@ import pytest
 
 import litert_amd.api as L
 import oracle
-from helpers import MISS, angle, ico128_scene, ico512_scene, compare_gbuffer as _compare, oracle_scene, preset
+from helpers import MISS, angle, decode_oct, ico128_scene, ico512_scene, compare_gbuffer as _compare, oracle_scene, preset
 
 pytestmark = pytest.mark.gpu
 
@@ -101,7 +101,9 @@ def test_full_size_views_sampled_against_oracle():
 
 @pytest.mark.parametrize("rmode", [L.MULTI_RENDER_MODE_MASK, L.MULTI_RENDER_MODE_LINEAR_DEPTH, L.MULTI_RENDER_MODE_DEPTH,
                                    L.MULTI_RENDER_MODE_NORMAL, L.MULTI_RENDER_MODE_LAMBERT_NO_TEX,
-                                   L.MULTI_RENDER_MODE_PRIMITIVE, L.MULTI_RENDER_MODE_ST_ITERATIONS])
+                                   L.MULTI_RENDER_MODE_PRIMITIVE, L.MULTI_RENDER_MODE_ST_ITERATIONS,
+                                   L.MULTI_RENDER_MODE_LAMBERT, L.MULTI_RENDER_MODE_PHONG, L.MULTI_RENDER_MODE_PHONG_NO_TEX,
+                                   L.MULTI_RENDER_MODE_BARYCENTRIC, L.MULTI_RENDER_MODE_TEX_COORDS])
 def test_render_float_modes(r_ico128, rmode):
     """RenderFloat colour per render mode vs the FP32 oracle (eye_ray.cpp:162-489)."""
     sc = oracle_scene("ico128")
@@ -111,6 +113,69 @@ def test_render_float_modes(r_ico128, rmode):
     ref = sc.render(wv, pr, p, 128, 128, precision=32, force_normal=False, rgba=True)["rgba"]
     diff = np.abs(img - ref).max(axis=-1)
     assert np.mean(diff <= 1e-5) > 0.9995, (rmode, np.sort(diff.ravel())[-10:])
+
+
+@pytest.mark.parametrize("rmode", [L.MULTI_RENDER_MODE_LAMBERT, L.MULTI_RENDER_MODE_PHONG, L.MULTI_RENDER_MODE_PHONG_NO_TEX,
+                                   L.MULTI_RENDER_MODE_DIFFUSE])
+def test_shading_with_brick_colours_and_shadow_rays(rmode):
+    """Lambert and Phong shading (eye_ray.cpp:376-483) of a coloured brick set:
+    brick colours through the colour layout, and for Phong a shadow ray per
+    hit toward the direct light (a secondary RayQuery_NearestHit from just
+    before the hit). Colour vs the FP32 oracle."""
+    sbs = L.grid_sbs("torus", 0.6, 0.2, 0.02, 3, seed=3, brick_count=16, brick_size=4)
+    vs = sbs.header.brick_size + 2 * sbs.header.brick_pad + 1
+    off = sbs.nodes[:, 2].astype(np.int64)
+    ci = np.unique(sbs.values[(off[:, None] + vs ** 3 + np.arange(8)[None, :]).ravel()].astype(np.int64))
+    vf = sbs.values_f.copy()
+    rng = np.random.default_rng(5)
+    for k in range(3):
+        vf[ci + k] = rng.uniform(0.1, 0.9, ci.size).astype(np.float32)
+    sbs = L.SdfSBS(sbs.header, sbs.nodes, sbs.values, vf)
+    r = L.CreateMultiRenderer(L.DEVICE_GPU)
+    r.SetScene(sbs)
+    pairs, _ = r.GetBVH()
+    sc = oracle.Scene(sbs, pairs)
+    for view in range(2):
+        wv, pr = L.orbit_camera(7, view)
+        p = preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON, render_mode=rmode)
+        img = r.RenderFloat(160, 160, wv, pr, p)
+        ref = sc.render(wv, pr, p, 160, 160, precision=32, force_normal=False, rgba=True)["rgba"]
+        diff = np.abs(img - ref).max(axis=-1)
+        assert np.mean(diff <= 1e-5) > 0.9995, (rmode, view, np.sort(diff.ravel())[-10:])
+        assert np.count_nonzero(img[..., :3].sum(-1) > 0) > 2000
+    r.close()
+
+
+def test_phong_shadow_rays_find_occluders():
+    """A sphere (r 0.25) inside a torus ring (R 0.6, r 0.15), both sparse
+    8-bit brick sets merged into one scene: the sphere shadows the ring's far
+    inner side. Phong colour vs the FP32 oracle, and pixels that face the
+    light but equal their Lambert colour (no diffuse/specular Phong term:
+    the shadow ray hit) must exist on the GPU image."""
+    a = L.displaced_torus(R=0.6, r=0.15, amplitude=0.0, lod=32, brick_size=4, brick_pad=0, bytes_per_value=1)
+    b = L.displaced_icosphere(subdiv=4, radius=0.25, amplitude=0.0, lod=32, brick_size=4, brick_pad=0, bytes_per_value=1)
+    assert not set(map(tuple, a.nodes[:, :2].tolist())) & set(map(tuple, b.nodes[:, :2].tolist()))
+    nb = b.nodes.copy()
+    nb[:, 2] += a.values.size
+    sbs = L.SdfSBS(a.header, np.concatenate([a.nodes, nb]), np.concatenate([a.values, b.values]))
+    r = L.CreateMultiRenderer(L.DEVICE_GPU)
+    r.SetScene(sbs)
+    pairs, _ = r.GetBVH()
+    sc = oracle.Scene(sbs, pairs)
+    wv, pr = L.orbit_camera(7, 0)
+    ph_p = preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON, render_mode=L.MULTI_RENDER_MODE_PHONG)
+    ph = r.RenderFloat(160, 160, wv, pr, ph_p)
+    ref = sc.render(wv, pr, ph_p, 160, 160, precision=32, force_normal=False, rgba=True)["rgba"]
+    diff = np.abs(ph - ref).max(axis=-1)
+    assert np.mean(diff <= 1e-5) > 0.9995, np.sort(diff.ravel())[-10:]
+    lam = r.RenderFloat(160, 160, wv, pr, preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON, render_mode=L.MULTI_RENDER_MODE_LAMBERT))
+    t, n, prim = r.RenderGBuffer(160, 160, wv, pr)
+    hit = prim != MISS
+    lit = (decode_oct(n) @ (np.ones(3) / np.sqrt(3.0))) > 0.2  # outward normals, surfaces seen from outside
+    same = np.all(np.abs(ph - lam) < 1e-7, axis=-1)
+    assert np.count_nonzero(hit & same & lit) > 50  # in shadow
+    assert np.count_nonzero(hit & ~same) > 500  # lit: Phong adds diffuse + specular
+    r.close()
 
 
 def test_render_rgba8_and_timer(r_ico128):
