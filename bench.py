@@ -251,6 +251,7 @@ def main():
                     help="brick LOD of the plain workload's scene (128: 512^3 effective, the headline workload)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched many-scene leg")
     ap.add_argument("--no-shade", action="store_true", help="skip the Phong-with-shadow-rays leg")
+    ap.add_argument("--no-build", action="store_true", help="skip the SBS-builder leg")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--mlp-scenes", type=int, default=32)
     ap.add_argument("--mlp-events", type=int, default=16_000_000)
@@ -322,6 +323,25 @@ def main():
                       "mrays_s_primary_plus_shadow": round((rays + hits) / sms / 1e3, 1)}
         log(f"phong + shadow rays: {sms:.3f} ms/launch, {rays} primary + {hits} shadow rays -> "
             f"{(rays + hits) / sms / 1e3:.1f} Mrays/s")
+
+    build_info = None
+    if not args.no_build:
+        # the plain scene built from its mesh on the GPU and on the host
+        mv, mt = L.mesh_displaced_icosphere(5 if args.lod <= 128 else 6, 0.8, 0.05, 4, SCENE_SEED)
+        kw = dict(lod=args.lod, brick_size=4, brick_pad=0, bytes_per_value=1, voxel_diag_scale=True)
+        L.sbs_from_mesh(mv, mt, renderer=r, **kw)  # warm-up (module load)
+        dms = []
+        for _ in range(3):
+            dsbs, ms = L.sbs_from_mesh(mv, mt, renderer=r, **kw)
+            dms.append(ms)
+        hsbs, hms = L.sbs_from_mesh(mv, mt, **kw)
+        same = bool(np.array_equal(dsbs.nodes, hsbs.nodes) and np.array_equal(dsbs.values, hsbs.values))
+        build_info = {"mesh_triangles": int(mt.shape[0]), "bricks": int(dsbs.nodes.shape[0]), "device_ms": round(min(dms), 1),
+                      "host_ms": round(hms, 1), "host_threads": min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "0")) or 1 << 30),
+                      "identical": same}
+        log(f"sbs build from mesh ({mt.shape[0]} triangles, {dsbs.nodes.shape[0]} bricks): device {min(dms):.1f} ms, "
+            f"host {hms:.1f} ms, identical {same}")
+        r.SetScene(sbs)
 
     mlp_mrays, mlp_info, mlp_params = None, None, None
     if not args.no_mlp:
@@ -433,6 +453,7 @@ def main():
         "mlp": mlp_info,
         "batch": batch_info,
         "phong_shadow": shade_info,
+        "sbs_build": build_info,
         "dr_fwd_bwd_mrays_s": None if dr_mrays is None else round(dr_mrays, 2),
         "dr": dr_info,
     }

@@ -227,6 +227,25 @@ int lrt_sbs_displaced_torus(float R, float r, float amplitude, uint32_t octaves,
  * with noise displacement amp (shape = 1) or a sphere of radius R (shape = 0). */
 int lrt_sbs_grid(uint32_t shape, float R, float r, float amplitude, uint32_t octaves, uint32_t seed, uint32_t brick_count,
                  uint32_t brick_size, int with_neighbors, lrt_sbs_handle* out);
+/* SBS of a closed triangle mesh (vertices: 3 floats each; indices: 3 per
+ * triangle), as sdf_converter::create_sdf_SBS(settings, header, mesh) builds
+ * one (utils/sdf_converter.cpp:183-228): exact signed distance to the mesh,
+ * bricks of brick_size voxels at brick LOD lod (a power of two) kept by the
+ * reference's border rule and quantiser (utils/sparse_octree_builder.cpp:
+ * 49-53, 405-428), DX layout; the candidate search is this library's
+ * coarse-to-fine cell test. lrt_sbs_from_mesh builds on the host (OpenMP,
+ * threads <= 0: all cores); lrt_sbs_from_mesh_device on the GPU of h, with the
+ * same result bit for bit. build_ms (may be NULL): wall time from mesh to scene. */
+int lrt_sbs_from_mesh(const float* vertices, uint32_t n_vertices, const uint32_t* indices, uint32_t n_triangles, uint32_t lod,
+                      uint32_t brick_size, uint32_t brick_pad, uint32_t bytes_per_value, int voxel_diag_scale, int threads,
+                      lrt_sbs_handle* out, float* build_ms);
+int lrt_sbs_from_mesh_device(lrt_handle h, const float* vertices, uint32_t n_vertices, const uint32_t* indices, uint32_t n_triangles,
+                             uint32_t lod, uint32_t brick_size, uint32_t brick_pad, uint32_t bytes_per_value, int voxel_diag_scale,
+                             lrt_sbs_handle* out, float* build_ms);
+/* The displaced icosphere mesh that lrt_sbs_displaced_icosphere builds from
+ * (no reference counterpart). NULL vertices / indices: only the counts. */
+int lrt_mesh_displaced_icosphere(uint32_t subdiv, float radius, float amplitude, uint32_t octaves, uint32_t seed, float* vertices,
+                                 uint32_t* n_vertices, uint32_t* indices, uint32_t* n_triangles);
 int lrt_sbs_info(lrt_sbs_handle s, SdfSBSHeader* header, uint32_t* n_nodes, uint64_t* n_values, uint64_t* n_values_f);
 int lrt_sbs_copy(lrt_sbs_handle s, SdfSBSNode* nodes, uint32_t* values, float* values_f);
 int lrt_sbs_free(lrt_sbs_handle s);

@@ -145,6 +145,9 @@ def _load():
         "lrt_set_launch_blocks": (i32, [P, u32]),
         "lrt_sbs_displaced_icosphere": (i32, [u32, f32, f32, u32, u32, u32, u32, u32, u32, i32, i32, C.POINTER(P)]),
         "lrt_sbs_grid": (i32, [u32, f32, f32, f32, u32, u32, u32, u32, i32, C.POINTER(P)]),
+        "lrt_sbs_from_mesh": (i32, [P, u32, P, u32, u32, u32, u32, u32, i32, i32, C.POINTER(P), C.POINTER(f32)]),
+        "lrt_sbs_from_mesh_device": (i32, [P, P, u32, P, u32, u32, u32, u32, u32, i32, C.POINTER(P), C.POINTER(f32)]),
+        "lrt_mesh_displaced_icosphere": (i32, [u32, f32, f32, u32, u32, P, C.POINTER(u32), P, C.POINTER(u32)]),
         "lrt_sbs_info": (i32, [P, C.POINTER(SdfSBSHeader), C.POINTER(u32), C.POINTER(u64), C.POINTER(u64)]),
         "lrt_sbs_copy": (i32, [P, P, P, P]),
         "lrt_sbs_free": (i32, [P]),
@@ -252,6 +255,36 @@ def displaced_torus(R=0.6, r=0.2, amplitude=0.02, octaves=3, seed=1, lod=32, bri
     _check(lib.lrt_sbs_displaced_torus(R, r, amplitude, octaves, seed, lod, brick_size, brick_pad, bytes_per_value,
                                        int(voxel_diag_scale), threads, C.byref(h)), "displaced_torus")
     return SdfSBS._from_handle(h)
+
+
+def mesh_displaced_icosphere(subdiv=5, radius=0.8, amplitude=0.05, octaves=3, seed=1):
+    """The displaced icosphere mesh behind displaced_icosphere(): (vertices (n, 3) float32, triangles (m, 3) uint32)."""
+    nv, nt = C.c_uint32(0), C.c_uint32(0)
+    _check(lib.lrt_mesh_displaced_icosphere(subdiv, radius, amplitude, octaves, seed, None, C.byref(nv), None, C.byref(nt)),
+           "lrt_mesh_displaced_icosphere")
+    v = np.zeros((nv.value, 3), np.float32)
+    t = np.zeros((nt.value, 3), np.uint32)
+    _check(lib.lrt_mesh_displaced_icosphere(subdiv, radius, amplitude, octaves, seed, _ptr(v), C.byref(nv), _ptr(t), C.byref(nt)),
+           "lrt_mesh_displaced_icosphere")
+    return v, t
+
+
+def sbs_from_mesh(vertices, triangles, lod=32, brick_size=4, brick_pad=0, bytes_per_value=1, voxel_diag_scale=False, threads=0,
+                  renderer=None):
+    """SBS of a closed triangle mesh (sdf_converter::create_sdf_SBS, utils/sdf_converter.cpp:183-228); on the
+    host (OpenMP) or, with `renderer`, on its GPU. Returns (SdfSBS, build_ms)."""
+    v = _f32(vertices).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
+    h = C.c_void_p()
+    ms = C.c_float(0.0)
+    if renderer is None:
+        _check(lib.lrt_sbs_from_mesh(_ptr(v), v.shape[0], _ptr(t), t.shape[0], lod, brick_size, brick_pad, bytes_per_value,
+                                     int(voxel_diag_scale), threads, C.byref(h), C.byref(ms)), "lrt_sbs_from_mesh")
+    else:
+        _check(lib.lrt_sbs_from_mesh_device(renderer._h, _ptr(v), v.shape[0], _ptr(t), t.shape[0], lod, brick_size, brick_pad,
+                                            bytes_per_value, int(voxel_diag_scale), C.byref(h), C.byref(ms)),
+               "lrt_sbs_from_mesh_device")
+    return SdfSBS._from_handle(h), ms.value
 
 
 def grid_sbs(shape="torus", R=0.6, r=0.2, amplitude=0.0, octaves=3, seed=1, brick_count=32, brick_size=4,

@@ -93,13 +93,7 @@ TriMesh displaced_icosphere(int subdiv, float r, float amp, int octaves, uint32_
 // mesh signed distance
 // ---------------------------------------------------------------------------
 struct MeshSdf::Impl {
-  TriMesh mesh;
-  BuildTree tree;
-  std::vector<f3> fnorm;            // face normals (unit)
-  std::vector<f3> vnorm;            // angle-weighted vertex pseudo-normals
-  std::map<uint64_t, f3> enorm;     // edge pseudo-normals (sum of adjacent face normals)
-
-  static uint64_t ekey(uint32_t a, uint32_t b) { return a < b ? ((uint64_t)a << 32 | b) : ((uint64_t)b << 32 | a); }
+  MeshSdfData D;
 
   // closest point on triangle (Ericson, Real-Time Collision Detection 5.1.5);
   // region: 0 face, 1..3 vertex a/b/c, 4 edge ab, 5 edge bc, 6 edge ca
@@ -130,36 +124,46 @@ struct MeshSdf::Impl {
   }
 };
 
-MeshSdf::MeshSdf(const TriMesh& m) : impl_(new Impl) {
-  Impl& I = *impl_;
-  I.mesh = m;
+MeshSdfData mesh_sdf_data(const TriMesh& m) {
+  MeshSdfData D;
+  D.mesh = m;
   size_t nt = m.idx.size() / 3;
   std::vector<f3> tmin(nt), tmax(nt);
-  I.fnorm.resize(nt);
-  I.vnorm.assign(m.v.size(), f3());
+  std::map<uint64_t, f3> enorm;  // edge -> sum of adjacent face normals
+  auto ekey = [](uint32_t a, uint32_t b) { return a < b ? ((uint64_t)a << 32 | b) : ((uint64_t)b << 32 | a); };
+  D.fnorm.resize(nt);
+  D.vnorm.assign(m.v.size(), f3());
   for (size_t t = 0; t < nt; t++) {
     f3 a = m.v[m.idx[3 * t]], b = m.v[m.idx[3 * t + 1]], c = m.v[m.idx[3 * t + 2]];
     tmin[t] = fmin3(a, fmin3(b, c));
     tmax[t] = fmax3(a, fmax3(b, c));
     f3 n = normalize(cross(b - a, c - a));
-    I.fnorm[t] = n;
+    D.fnorm[t] = n;
     f3 pts[3] = {a, b, c};
     for (int k = 0; k < 3; k++) {
       f3 e1 = normalize(pts[(k + 1) % 3] - pts[k]), e2 = normalize(pts[(k + 2) % 3] - pts[k]);
       float ang = std::acos(std::max(-1.f, std::min(1.f, dot(e1, e2))));
       uint32_t vi = m.idx[3 * t + k];
-      I.vnorm[vi] = I.vnorm[vi] + ang * n;
-      uint64_t ek = Impl::ekey(m.idx[3 * t + k], m.idx[3 * t + (k + 1) % 3]);
-      I.enorm[ek] = I.enorm[ek] + n;
+      D.vnorm[vi] = D.vnorm[vi] + ang * n;
+      uint64_t ek = ekey(m.idx[3 * t + k], m.idx[3 * t + (k + 1) % 3]);
+      enorm[ek] = enorm[ek] + n;
     }
   }
-  I.tree = build_sah(tmin, tmax, 4);
+  D.enorm3.resize(3 * nt);
+  for (size_t t = 0; t < nt; t++)
+    for (int k = 0; k < 3; k++) D.enorm3[3 * t + k] = enorm.at(ekey(m.idx[3 * t + k], m.idx[3 * t + (k + 1) % 3]));
+  D.tree = build_sah(tmin, tmax, 4);
+  return D;
 }
+
+MeshSdf::MeshSdf(const TriMesh& m) : impl_(new Impl) { impl_->D = mesh_sdf_data(m); }
 
 MeshSdf::~MeshSdf() { delete impl_; }
 
+const MeshSdfData& MeshSdf::data() const { return impl_->D; }
+
 float MeshSdf::operator()(f3 p) const {
-  const Impl& I = *impl_;
+  const MeshSdfData& I = impl_->D;
   const TriMesh& m = I.mesh;
   float best = FLT_MAX;
   uint32_t best_t = 0;
@@ -201,9 +205,9 @@ float MeshSdf::operator()(f3 p) const {
     case 1: pn = I.vnorm[tri[0]]; break;
     case 2: pn = I.vnorm[tri[1]]; break;
     case 3: pn = I.vnorm[tri[2]]; break;
-    case 4: pn = I.enorm.at(Impl::ekey(tri[0], tri[1])); break;
-    case 5: pn = I.enorm.at(Impl::ekey(tri[1], tri[2])); break;
-    default: pn = I.enorm.at(Impl::ekey(tri[2], tri[0])); break;
+    case 4: pn = I.enorm3[3 * best_t + 0]; break;
+    case 5: pn = I.enorm3[3 * best_t + 1]; break;
+    default: pn = I.enorm3[3 * best_t + 2]; break;
   }
   float d = std::sqrt(best);
   return dot(p - best_pt, pn) < 0.f ? -d : d;

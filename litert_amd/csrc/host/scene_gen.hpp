@@ -10,6 +10,7 @@
 #include <functional>
 #include <vector>
 #include "hmath.hpp"
+#include "bvh_build.hpp"
 #include "../../../include/lrt_types.h"
 
 namespace lrt {
@@ -34,6 +35,17 @@ float value_noise_fbm(f3 p, uint32_t seed, int octaves, float base_freq);
 // radius r, each vertex moved along its normal by amp * fbm(vertex).
 TriMesh displaced_icosphere(int subdiv, float r, float amp, int octaves, uint32_t seed);
 
+// What the mesh distance query reads: the triangle BVH (binned SAH, at most 4
+// triangles per leaf), unit face normals, angle-weighted vertex
+// pseudo-normals and per-triangle edge pseudo-normals (sum of the adjacent
+// face normals) in the order ab, bc, ca.
+struct MeshSdfData {
+  TriMesh mesh;
+  BuildTree tree;
+  std::vector<f3> fnorm, vnorm, enorm3;
+};
+MeshSdfData mesh_sdf_data(const TriMesh& m);
+
 // Signed distance to a closed triangle mesh: exact Euclidean distance through
 // a BVH over triangles; sign from the angle-weighted pseudo-normal of the
 // closest feature (Baerentzen & Aanaes 2005).
@@ -42,6 +54,7 @@ class MeshSdf {
   explicit MeshSdf(const TriMesh& m);
   ~MeshSdf();
   float operator()(f3 p) const;
+  const MeshSdfData& data() const;
 
  private:
   struct Impl;

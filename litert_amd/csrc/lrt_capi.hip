@@ -232,10 +232,6 @@ int Renderer::finish_timer() {
 
 using namespace lrt;
 
-struct lrt_sbs {
-  SbsScene s;
-};
-
 extern "C" {
 
 const char* lrt_last_error(void) { return g_err.c_str(); }

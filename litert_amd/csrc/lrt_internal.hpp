@@ -7,6 +7,7 @@
 #include "../../include/lrt.h"
 #include "kernels/mlp.hpp"
 #include "kernels/render.hpp"
+#include "host/scene_gen.hpp"
 
 namespace lrt {
 
@@ -114,3 +115,8 @@ struct Renderer {
 }  // namespace lrt
 
 struct lrt_renderer : lrt::Renderer {};
+
+// A scene produced by one of the builders (lrt_sbs_* entry points).
+struct lrt_sbs {
+  lrt::SbsScene s;
+};

@@ -6,6 +6,7 @@ import re
 import subprocess
 
 import numpy as np
+import pytest
 
 import litert_amd.api as L
 
@@ -105,6 +106,23 @@ def test_displaced_torus_builder():
     q = np.sqrt(np.hypot(c[:, 0], c[:, 1]) ** 2) - 0.6
     d = np.abs(np.hypot(q, c[:, 2]) - 0.2)
     assert d.max() < 0.02 + 2 * np.sqrt(3) / 16 + np.sqrt(3) / 16
+
+
+def test_mesh_builder_matches_icosphere_builder():
+    """lrt_sbs_from_mesh on the displaced icosphere's own mesh gives the scene
+    lrt_sbs_displaced_icosphere builds (same distance function and rule)."""
+    v, t = L.mesh_displaced_icosphere(4, 0.8, 0.05, 3, 11)
+    assert v.shape == (2562, 3) and t.shape == (5120, 3) and t.max() < v.shape[0]
+    a, ms = L.sbs_from_mesh(v, t, lod=32, brick_size=4, brick_pad=1, bytes_per_value=4)
+    b = L.displaced_icosphere(4, 0.8, 0.05, 3, seed=11, lod=32, brick_size=4, brick_pad=1, bytes_per_value=4)
+    assert ms > 0 and a.nodes.shape[0] > 500
+    assert np.array_equal(a.nodes, b.nodes) and np.array_equal(a.values, b.values)
+    with pytest.raises(L.LrtError):
+        L.sbs_from_mesh(v, t, lod=48)  # not a power of two
+    bad = t.copy()
+    bad[0, 0] = v.shape[0]
+    with pytest.raises(L.LrtError):
+        L.sbs_from_mesh(v, bad, lod=32)
 
 
 def test_bench_refuses_more_than_one_gpu():
