@@ -233,7 +233,8 @@ int lrt_sbs_grid(uint32_t shape, float R, float r, float amplitude, uint32_t oct
  * bricks of brick_size voxels at brick LOD lod (a power of two) kept by the
  * reference's border rule and quantiser (utils/sparse_octree_builder.cpp:
  * 49-53, 405-428), DX layout; the candidate search is this library's
- * coarse-to-fine cell test. lrt_sbs_from_mesh builds on the host (OpenMP,
+ * coarse-to-fine cell test, and values are quantised in double (the reference
+ * computes the same expression in float). lrt_sbs_from_mesh builds on the host (OpenMP,
  * threads <= 0: all cores); lrt_sbs_from_mesh_device on the GPU of h, with the
  * same result bit for bit. build_ms (may be NULL): wall time from mesh to scene. */
 int lrt_sbs_from_mesh(const float* vertices, uint32_t n_vertices, const uint32_t* indices, uint32_t n_triangles, uint32_t lod,

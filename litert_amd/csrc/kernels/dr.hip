@@ -358,6 +358,35 @@ __global__ void k_dr_cull_write(const uint4* __restrict__ pairs, const uint32_t*
   out_b[2 * s + 1] = bhi;
 }
 
+// MultiRendererDR::Regularization (MultiRendererDR.cpp:1933-2057): every
+// distance value of a brick is pulled toward its in-brick neighbours along
+// the three axes, lambda * power * sign(d) * |d|^(power-1) per neighbour with
+// d = own value - neighbour value. One lane per (brick, value); the six terms
+// are summed in the reference's neighbour order, then added with one atomic.
+__global__ void k_dr_regularize(DevScene S, uint32_t n_nodes, float lambda, float power, float* __restrict__ grad) {
+  const uint32_t vs = S.brick_size + 1u, nv = vs * vs * vs;
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (uint64_t)n_nodes * nv) return;
+  const uint32_t b = (uint32_t)(g / nv), v = (uint32_t)(g - (uint64_t)b * nv);
+  const uint32_t i0 = v / (vs * vs), j0 = (v / vs) % vs, k0 = v % vs;
+  const uint32_t* idx = S.values + S.nodes[b].z;
+  const float own = S.values_f[idx[v]];
+  float acc = 0.0f;
+  auto term = [&](uint32_t nbr) {
+    float d = own - S.values_f[idx[nbr]];
+    const float sgn_d = d > 0.0f ? 1.0f : -1.0f;
+    d = fabsf(d);
+    acc += lambda * power * sgn_d * powf(d, power - 1);
+  };
+  if (i0 > 0) term(v - vs * vs);
+  if (i0 < vs - 1) term(v + vs * vs);
+  if (j0 > 0) term(v - vs);
+  if (j0 < vs - 1) term(v + vs);
+  if (k0 > 0) term(v - 1);
+  if (k0 < vs - 1) term(v + 1);
+  atomicAdd(&grad[idx[v]], acc);
+}
+
 __global__ void k_dr_scale(float* grad, uint32_t n, float mult) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) grad[i] *= mult;
@@ -439,6 +468,13 @@ hipError_t launch_dr_border_rays(const lrtd::DRArgs& A, const uint32_t* d_list, 
 }
 hipError_t launch_dr_gather(const lrtd::DevScene& S, float* d_out, float2* d_minmax, hipStream_t s) {
   hipLaunchKernelGGL(lrtd::k_dr_gather, dim3((S.n_nodes + 3) / 4), dim3(256), 0, s, S, d_out, d_minmax);
+  return hipGetLastError();
+}
+hipError_t launch_dr_regularize(const lrtd::DevScene& S, uint32_t n_nodes, float lambda, float power, float* d_grad, hipStream_t s) {
+  const uint32_t vs = S.brick_size + 1u;
+  const uint64_t threads = (uint64_t)n_nodes * vs * vs * vs;
+  if (threads == 0) return hipSuccess;
+  hipLaunchKernelGGL(lrtd::k_dr_regularize, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, s, S, n_nodes, lambda, power, d_grad);
   return hipGetLastError();
 }
 hipError_t launch_dr_scale(float* d_grad, uint32_t n, float mult, hipStream_t s) {

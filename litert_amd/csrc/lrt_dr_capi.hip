@@ -42,6 +42,8 @@ static int dr_check(Renderer* r, const MultiRendererDRPreset& p) {
   if (p.dr_loss_function > LRT_DR_LOSS_FUNCTION_MAE) return set_err(LRT_E_INVALID, "bad loss function");
   if (p.dr_border_sampling != LRT_DR_BORDER_SAMPLING_RANDOM) return set_err(LRT_E_UNSUPPORTED, "SVM border sampling is not built");
   if (p.dr_diff_mode != LRT_DR_DIFF_MODE_DEFAULT) return set_err(LRT_E_UNSUPPORTED, "finite-difference DR mode is not built");
+  if (p.reg_function > LRT_DR_REG_FUNCTION_LK_DENOISING) return set_err(LRT_E_INVALID, "bad regularization function");
+  if (p.redistancing_enable) return set_err(LRT_E_UNSUPPORTED, "redistancing is not built");
   if (p.spp < 1 || p.spp > 16) return set_err(LRT_E_INVALID, "DR spp must be 1..16");
   if (p.border_relax_eps <= 0.f) return set_err(LRT_E_INVALID, "border_relax_eps must be > 0");
   if (r->dr_n_ref == 0) return set_err(LRT_E_INVALID, "no reference images: call lrt_dr_set_reference");
@@ -294,6 +296,15 @@ int lrt_dr_optimize(lrt_handle h, const MultiRendererDRPreset* p, const SdfSBSHe
       lsum += h->dr_last_loss;
     }
     if (losses) losses[it] = (float)(lsum / p->image_batch_size);
+    if (p->reg_function != LRT_DR_REG_FUNCTION_NONE) {
+      // Regularization, MultiRendererDR.cpp:453-455: after the step's renders,
+      // added to the gradient before the batch division. The reference walks
+      // brick ids below min(brick count, W*H) (its per-thread range is bounded
+      // by the image size, :1962-1963); the same bricks are taken here.
+      const uint32_t limit = (uint32_t)std::min<uint64_t>(h->nodes.size(), (uint64_t)h->dr_w * h->dr_h);
+      if ((e = launch_dr_regularize(h->dev_scene(), limit, p->reg_lambda, p->reg_power, (float*)h->d_dr_grad.p, h->stream)) != hipSuccess)
+        return hip_err(e, "regularization");
+    }
     const float c1 = (float)(1.0 - std::pow((double)p->opt_beta_1, (double)(it + 1)));
     const float c2 = (float)(1.0 - std::pow((double)p->opt_beta_2, (double)(it + 1)));
     e = launch_adam((const float*)h->d_dr_grad.p, (float*)h->d_values_f.p, (float*)h->d_adam_v.p, (float*)h->d_adam_s.p, P, p->opt_lr,

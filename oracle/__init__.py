@@ -74,6 +74,7 @@ def lib():
         optional = {
             "orc_dr_render_grad": (i32, [P, P, P, P, u32, u32, u32, P, i32, i32, i32, P, P, P]),
             "orc_adam_step": (None, [u32, P, P, P, P, u32, f32, f32, f32, f32]),
+            "orc_dr_regularize": (i32, [P, u32, f32, f32, P]),
             "orc_mlp_forward": (None, [P, P, u32, i32, P]),
             "orc_mlp_loss_grad": (f64, [P, P, P, u32, i32, P]),
             "orc_mlp_brick_features": (i32, [P, u32, P]),
@@ -169,6 +170,13 @@ class Scene:
         self._keep = self._keep[:2] + (vf,) + self._keep[3:]
         if lib().orc_scene_set_values_f(self.h, _p(vf), vf.size) != 0:
             raise ValueError("size mismatch")
+
+    def dr_regularize(self, n_nodes, lam, power, n_params):
+        """Regularization gradient (MultiRendererDR.cpp:1933-2057) of bricks [0, n_nodes)."""
+        g = np.zeros(n_params, np.float64)
+        if lib().orc_dr_regularize(self.h, n_nodes, lam, power, _p(g)) != 0:
+            raise ValueError("orc_dr_regularize failed")
+        return g
 
     def mlp_features(self, rays, bricks):
         """Seeder features for rays (n x 6: pos, dir) entering bricks (n): (n x 24 float32, t0 float32)."""

@@ -101,6 +101,9 @@ int orc_dr_render_grad(const orc_scene* s, const MultiRendererDRPreset* dr, cons
                        uint32_t W, uint32_t H, uint32_t seed, const float* ref, int precision, int nthreads, int dtdv_mode,
                        double* out_image, double* out_grad, orc_dr_stats* stats);
 
+/* Regularization (MultiRendererDR.cpp:1933-2057) of bricks [0, n_nodes), added to grad. */
+int orc_dr_regularize(const orc_scene* s, uint32_t n_nodes, float lambda, float power, double* grad);
+
 /* OptimizeStepAdam (MultiRendererDR.cpp:828-853): updates X, V, S in place. */
 void orc_adam_step(uint32_t iter, const float* dX, float* X, float* V, float* S, uint32_t n, float lr, float beta1,
                    float beta2, float eps);

@@ -840,6 +840,38 @@ extern "C" int orc_dr_render_grad(const orc_scene* s, const MultiRendererDRPrese
   return dr_impl<float>(s->s, *dr, cam, worldView, proj, W, H, seed, ref, nthreads, dtdv_mode, out_image, out_grad, stats);
 }
 
+// MultiRendererDR::Regularization, MultiRendererDR.cpp:1933-2057: for bricks
+// [0, n_nodes) and every distance value, the six in-brick neighbour terms
+// lambda * power * sign(d) * |d|^(power-1), d = value - neighbour, added to
+// grad (double) in the reference's order (float terms, as its buffers hold).
+extern "C" int orc_dr_regularize(const orc_scene* s, uint32_t n_nodes, float lambda, float power, double* grad) {
+  if (!s || !grad) return -1;
+  const Scene& S = s->s;
+  if (S.type != LRT_TYPE_SDF_SBS_COL || n_nodes > S.nodes.size()) return -4;
+  const uint32_t vs = S.header.brick_size + 1;
+  for (uint32_t b = 0; b < n_nodes; b++) {
+    const uint32_t off = S.nodes[b].data_offset;
+    for (uint32_t i0 = 0; i0 < vs; i0++)
+      for (uint32_t j0 = 0; j0 < vs; j0++)
+        for (uint32_t k0 = 0; k0 < vs; k0++) {
+          const uint32_t idx = off + i0 * vs * vs + j0 * vs + k0;
+          auto term = [&](uint32_t nbr) {
+            float d = S.values_f[S.values[idx]] - S.values_f[S.values[nbr]];
+            float sgn_d = d > 0.0f ? 1.0f : -1.0f;
+            d = std::abs(d);
+            grad[S.values[idx]] += (double)(lambda * power * sgn_d * std::pow(d, power - 1));
+          };
+          if (i0 > 0) term(idx - vs * vs);
+          if (i0 < vs - 1) term(idx + vs * vs);
+          if (j0 > 0) term(idx - vs);
+          if (j0 < vs - 1) term(idx + vs);
+          if (k0 > 0) term(idx - 1);
+          if (k0 < vs - 1) term(idx + 1);
+        }
+  }
+  return 0;
+}
+
 // MultiRendererDR.cpp:828-853
 extern "C" void orc_adam_step(uint32_t iter, const float* dX, float* X, float* V, float* S, uint32_t n, float lr, float beta1,
                               float beta2, float eps) {

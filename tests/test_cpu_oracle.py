@@ -453,6 +453,42 @@ def test_dr_lambert_colour_gradient_matches_finite_differences(smoothed):
     assert ok >= 19, ok
 
 
+@pytest.mark.parametrize("power", [2.0, 1.5])
+def test_dr_regularization_is_gradient_of_neighbour_energy(power):
+    """The oracle's Regularization term equals the gradient of
+    R(v) = lambda/2 * sum over bricks and ordered in-brick neighbour pairs
+    (i, j) of |v_i - v_j|^power, computed here independently with numpy and
+    differentiated by central differences."""
+    sbs = L.grid_sbs("sphere", 0.5, 0.0, 0.0, 1, seed=1, brick_count=3, brick_size=4)
+    rng = np.random.default_rng(1)
+    vf = sbs.values_f.copy()
+    vs = sbs.header.brick_size + 1
+    off = sbs.nodes[:, 2].astype(np.int64)
+    idx = sbs.values[off[:, None] + np.arange(vs ** 3)[None, :]].astype(np.int64).reshape(-1, vs, vs, vs)
+    vf[np.unique(idx)] += rng.normal(0, 0.01, np.unique(idx).size).astype(np.float32)
+    lam = 0.25
+    pairs, _ = L.build_bvh(sbs)
+    sc = oracle.Scene(L.SdfSBS(sbs.header, sbs.nodes, sbs.values, vf), pairs)
+    g = sc.dr_regularize(sbs.nodes.shape[0], lam, power, vf.size)
+
+    def energy(v):
+        e = 0.0
+        x = v.astype(np.float64)[idx]
+        for ax in (1, 2, 3):
+            d = np.abs(np.diff(x, axis=ax))
+            e += 2 * np.sum(d ** power)  # each unordered pair twice (both orders)
+        return lam / 2 * e
+
+    h = 1e-4
+    check = np.unique(idx)[:: max(1, np.unique(idx).size // 40)]
+    for k in check:
+        vp, vm = vf.astype(np.float64).copy(), vf.astype(np.float64).copy()
+        vp[k] += h
+        vm[k] -= h
+        fd = (energy(vp) - energy(vm)) / (2 * h)
+        assert abs(fd - g[k]) <= 2e-3 * max(1.0, abs(g[k])), (k, fd, g[k])
+
+
 def test_adam_step_closed_form():
     rng = np.random.default_rng(9)
     n = 64
