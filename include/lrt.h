@@ -136,10 +136,11 @@ int lrt_dr_set_reference(lrt_handle h, const float* images, const float* masks, 
  * eye_ray_host.cpp:40: direct along (1,1,1) of colour 2/3 plus ambient 0.25),
  * MASK, LINEAR_DEPTH, NORMAL; LAMBERT and NORMAL differentiate the hit normal
  * (smoothed over neighbouring voxels for the ID32F_IRGB32F_IN layout,
- * BVH2DR.cpp:1300-1484). Not built: SVM border sampling, redistancing and the
+ * BVH2DR.cpp:1300-1484). Not built: SVM border sampling and the
  * finite-difference DR mode (LRT_E_UNSUPPORTED). lrt_dr_optimize applies the
- * LK_DENOISING regularization (MultiRendererDR.cpp:1933-2057) when
- * reg_function asks for it. */
+ * LK_DENOISING regularization (MultiRendererDR.cpp:1933-2057) and
+ * redistancing (:2058-2382, dense brick grids whose first values are the
+ * lattice) when the preset asks for them. */
 int lrt_dr_render_grad(lrt_handle h, const MultiRendererDRPreset* preset, uint32_t view, uint32_t seed, int dtdv_mode, float* out_image,
                        float* out_grad, float* loss, float* kernel_ms);
 /* MultiRendererDR::OptimizeFixedStructure(preset, sbs), MultiRendererDR.cpp:278-576:

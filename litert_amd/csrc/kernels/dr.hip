@@ -387,6 +387,90 @@ __global__ void k_dr_regularize(DevScene S, uint32_t n_nodes, float lambda, floa
   atomicAdd(&grad[idx[v]], acc);
 }
 
+// MultiRendererDR::Redistance (MultiRendererDR.cpp:2122-2382) over the dense
+// p^3 lattice of distance values (x-major), grid spacing h.
+//
+// The reference sweeps the lattice 8 times per iteration along anti-diagonal
+// planes, but every update reads only the previous iteration's buffer and the
+// point's own previous value, so each iteration is one Jacobi step and the
+// sweep order does not change any value: here one lane updates one point per
+// iteration. Points with a zero value or a sign change to a 6-neighbour are
+// frozen at their value; the others start at 99999.
+__device__ __forceinline__ float solve_eikonal(float m0, float m1, float m2, float h) {  // SolveEikonal, :2058-2120
+  float t;
+  if (m0 > m1) { t = m0; m0 = m1; m1 = t; }
+  if (m0 > m2) { t = m0; m0 = m2; m2 = t; }
+  if (m1 > m2) { t = m1; m1 = m2; m2 = t; }
+  float dist_new = m0 + h;
+  if (dist_new > m1) {
+    const float h_2 = h * h;
+    float m_sum = m0 + m1, c_2 = (m0 - m1) * (m0 - m1);
+    dist_new = (m_sum + sqrtf(2 * h_2 - c_2)) * 0.5f;
+    if (dist_new > m2) {
+      const float one_third = 1.f / 3;
+      m_sum += m2;
+      c_2 = 3.f * (m0 * m0 + m1 * m1 + m2 * m2 - h_2);
+      dist_new = (m_sum + sqrtf(m_sum * m_sum - c_2)) * one_third;
+    }
+  }
+  return dist_new;
+}
+
+__global__ void k_redist_init(const float* __restrict__ d, uint32_t p, uint8_t* __restrict__ frozen, float* __restrict__ b0,
+                              float* __restrict__ b1) {
+  const uint32_t n = p * p * p;
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const uint32_t c[3] = {g / (p * p), (g / p) % p, g % p};
+  const uint32_t off[3] = {p * p, p, 1};
+  const float v = d[g];
+  bool fr = v == 0.f;
+  float init = fr ? 0.f : 99999.f;
+  if (!fr)
+    for (int dim = 0; dim < 3; dim++) {
+      if (c[dim] > 0 && d[g - off[dim]] * v < 0) fr = true;
+      if (c[dim] + 1 < p && d[g + off[dim]] * v < 0) fr = true;
+    }
+  if (fr && v != 0.f) init = v;
+  frozen[g] = fr ? 1 : 0;
+  b0[g] = init;
+  b1[g] = init;
+}
+
+__global__ void k_redist_step(const float* __restrict__ old, float* __restrict__ nw, const uint8_t* __restrict__ frozen, uint32_t p,
+                              float h) {
+  const uint32_t n = p * p * p;
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const float big = 99999.f;
+  const float own = old[g];
+  nw[g] = own;
+  if (frozen[g]) return;
+  const uint32_t c[3] = {g / (p * p), (g / p) % p, g % p};
+  const uint32_t off[3] = {p * p, p, 1};
+  float am[3] = {big, big, big};
+  for (int dim = 0; dim < 3; dim++) {
+    if (c[dim] > 0 && c[dim] < p - 1) {
+      am[dim] = old[g - off[dim]];
+      const float t = old[g + off[dim]];
+      if (fabsf(t) < fabsf(am[dim])) am[dim] = t;
+    } else if (c[dim] > 0 && c[dim] == p - 1) {
+      am[dim] = old[g - off[dim]];
+    } else if (c[dim] == 0 && c[dim] < p - 1) {
+      am[dim] = old[g + off[dim]];
+    }
+  }
+  if (am[0] < big || am[1] < big || am[2] < big) {
+    if (am[0] >= 0.f && am[1] >= 0.f && am[2] >= 0.f) {
+      nw[g] = mn(solve_eikonal(am[0], am[1], am[2], h), own);  // std::min(a, b)
+    } else {
+      for (int dim = 0; dim < 3; dim++)
+        if (am[dim] < 0.f) am[dim] = -am[dim];
+      nw[g] = mx(-solve_eikonal(am[0], am[1], am[2], h), -fabsf(own));  // std::max(a, b)
+    }
+  }
+}
+
 __global__ void k_dr_scale(float* grad, uint32_t n, float mult) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) grad[i] *= mult;
@@ -475,6 +559,21 @@ hipError_t launch_dr_regularize(const lrtd::DevScene& S, uint32_t n_nodes, float
   const uint64_t threads = (uint64_t)n_nodes * vs * vs * vs;
   if (threads == 0) return hipSuccess;
   hipLaunchKernelGGL(lrtd::k_dr_regularize, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, s, S, n_nodes, lambda, power, d_grad);
+  return hipGetLastError();
+}
+hipError_t launch_dr_redistance(float* d_values, uint32_t p, float h, uint32_t iters, uint8_t* d_frozen, float* d_b0, float* d_b1,
+                                hipStream_t s) {
+  const uint32_t n = p * p * p;
+  const dim3 grid((n + 255) / 256);
+  hipLaunchKernelGGL(lrtd::k_redist_init, grid, dim3(256), 0, s, (const float*)d_values, p, d_frozen, d_b0, d_b1);
+  float* b[2] = {d_b0, d_b1};
+  int cur = 0;  // the buffer the next step writes
+  for (uint32_t it = 0; it < iters; it++) {
+    hipLaunchKernelGGL(lrtd::k_redist_step, grid, dim3(256), 0, s, (const float*)b[1 - cur], b[cur], (const uint8_t*)d_frozen, p, h);
+    cur = 1 - cur;
+  }
+  hipError_t e = hipMemcpyAsync(d_values, b[1 - cur], (size_t)n * 4, hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) return e;
   return hipGetLastError();
 }
 hipError_t launch_dr_scale(float* d_grad, uint32_t n, float mult, hipStream_t s) {

@@ -55,6 +55,10 @@ hipError_t launch_dr_border_rays(const lrtd::DRArgs& A, const uint32_t* d_list, 
                                  uint32_t* d_hits, uint32_t blocks, hipStream_t s, const uint4* cull_pairs = nullptr);
 hipError_t launch_dr_gather(const lrtd::DevScene& S, float* d_out, float2* d_minmax, hipStream_t s);
 hipError_t launch_dr_scale(float* d_grad, uint32_t n, float mult, hipStream_t s);
+// Redistance (MultiRendererDR.cpp:2122-2382) of the first p^3 values (a dense
+// x-major lattice), spacing h, iters Jacobi steps; scratch: p^3 bytes + 2 x p^3 floats
+hipError_t launch_dr_redistance(float* d_values, uint32_t p, float h, uint32_t iters, uint8_t* d_frozen, float* d_b0, float* d_b1,
+                                hipStream_t s);
 // Regularization (MultiRendererDR.cpp:1933-2057) of bricks [0, n_nodes) added to d_grad
 hipError_t launch_dr_regularize(const lrtd::DevScene& S, uint32_t n_nodes, float lambda, float power, float* d_grad, hipStream_t s);
 hipError_t launch_adam(const float* g, float* X, float* V, float* S, uint32_t n, float lr, float b1, float b2, float eps, float c1,

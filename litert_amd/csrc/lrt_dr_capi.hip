@@ -43,7 +43,7 @@ static int dr_check(Renderer* r, const MultiRendererDRPreset& p) {
   if (p.dr_border_sampling != LRT_DR_BORDER_SAMPLING_RANDOM) return set_err(LRT_E_UNSUPPORTED, "SVM border sampling is not built");
   if (p.dr_diff_mode != LRT_DR_DIFF_MODE_DEFAULT) return set_err(LRT_E_UNSUPPORTED, "finite-difference DR mode is not built");
   if (p.reg_function > LRT_DR_REG_FUNCTION_LK_DENOISING) return set_err(LRT_E_INVALID, "bad regularization function");
-  if (p.redistancing_enable) return set_err(LRT_E_UNSUPPORTED, "redistancing is not built");
+  if (p.redistancing_enable && p.redistancing_interval == 0) return set_err(LRT_E_INVALID, "redistancing_interval must be > 0");
   if (p.spp < 1 || p.spp > 16) return set_err(LRT_E_INVALID, "DR spp must be 1..16");
   if (p.border_relax_eps <= 0.f) return set_err(LRT_E_INVALID, "border_relax_eps must be > 0");
   if (r->dr_n_ref == 0) return set_err(LRT_E_INVALID, "no reference images: call lrt_dr_set_reference");
@@ -285,6 +285,19 @@ int lrt_dr_optimize(lrt_handle h, const MultiRendererDRPreset* p, const SdfSBSHe
   if ((e = h->d_adam_s.ensure((size_t)P * 4)) != hipSuccess) return hip_err(e, "alloc adam");
   if ((e = zero_u32((uint32_t*)h->d_adam_v.p, P, h->stream)) != hipSuccess) return hip_err(e, "clear");
   if ((e = zero_u32((uint32_t*)h->d_adam_s.p, P, h->stream)) != hipSuccess) return hip_err(e, "clear");
+  // Redistance treats the first p^3 values as the dense lattice of a
+  // brick_count^3 grid of bricks, p = brick_count * brick_size + 1
+  // (MultiRendererDR.cpp:460-466, the layout the grid builders write)
+  uint32_t rp = 0;
+  if (p->redistancing_enable) {
+    uint32_t bc = (uint32_t)std::llround(std::cbrt((double)n_nodes));
+    while ((uint64_t)bc * bc * bc > n_nodes) bc--;
+    while ((uint64_t)(bc + 1) * (bc + 1) * (bc + 1) <= n_nodes) bc++;
+    rp = bc * header->brick_size + 1;
+    if ((uint64_t)bc * bc * bc != n_nodes || (uint64_t)rp * rp * rp > n_values_f)
+      return set_err(LRT_E_UNSUPPORTED, "redistancing needs a dense brick_count^3 grid whose first values are its lattice");
+    if ((e = h->d_dr_redist.ensure((size_t)rp * rp * rp * 9)) != hipSuccess) return hip_err(e, "alloc redistancing");
+  }
   auto t0 = std::chrono::steady_clock::now();
   h->dr_last_batch = p->image_batch_size;
   for (uint32_t it = 0; it < p->opt_iterations; it++) {
@@ -304,6 +317,14 @@ int lrt_dr_optimize(lrt_handle h, const MultiRendererDRPreset* p, const SdfSBSHe
       const uint32_t limit = (uint32_t)std::min<uint64_t>(h->nodes.size(), (uint64_t)h->dr_w * h->dr_h);
       if ((e = launch_dr_regularize(h->dev_scene(), limit, p->reg_lambda, p->reg_power, (float*)h->d_dr_grad.p, h->stream)) != hipSuccess)
         return hip_err(e, "regularization");
+    }
+    if (p->redistancing_enable && it % p->redistancing_interval == 0) {
+      // MultiRendererDR.cpp:459-466: on the parameters, before the Adam step
+      const size_t n3 = (size_t)rp * rp * rp;
+      float* b0 = (float*)h->d_dr_redist.p;
+      uint8_t* frozen = (uint8_t*)(b0 + 2 * n3);
+      if ((e = launch_dr_redistance((float*)h->d_values_f.p, rp, 2.f / (float)rp, rp, frozen, b0, b0 + n3, h->stream)) != hipSuccess)
+        return hip_err(e, "redistancing");
     }
     const float c1 = (float)(1.0 - std::pow((double)p->opt_beta_1, (double)(it + 1)));
     const float c2 = (float)(1.0 - std::pow((double)p->opt_beta_2, (double)(it + 1)));

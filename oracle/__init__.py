@@ -75,6 +75,7 @@ def lib():
             "orc_dr_render_grad": (i32, [P, P, P, P, u32, u32, u32, P, i32, i32, i32, P, P, P]),
             "orc_adam_step": (None, [u32, P, P, P, P, u32, f32, f32, f32, f32]),
             "orc_dr_regularize": (i32, [P, u32, f32, f32, P]),
+            "orc_dr_redistance": (i32, [P, u32, f32, u32]),
             "orc_mlp_forward": (None, [P, P, u32, i32, P]),
             "orc_mlp_loss_grad": (f64, [P, P, P, u32, i32, P]),
             "orc_mlp_brick_features": (i32, [P, u32, P]),
@@ -203,6 +204,15 @@ class Scene:
         if rc != 0:
             raise ValueError(f"orc_dr_render_grad failed {rc}")
         return img, grad, {k: getattr(st, k) for k, _ in DRStats._fields_}
+
+
+def dr_redistance(lattice, h, iters):
+    """Redistance (MultiRendererDR.cpp:2122-2382) of a p x p x p lattice; returns the new lattice."""
+    d = np.ascontiguousarray(lattice, np.float32).copy()
+    p = d.shape[0]
+    if lib().orc_dr_redistance(_p(d), p, h, iters) != 0:
+        raise ValueError("orc_dr_redistance failed")
+    return d
 
 
 def eye_ray(world_view, proj, x, y, dx, dy, W, H, precision=32):

@@ -104,6 +104,9 @@ int orc_dr_render_grad(const orc_scene* s, const MultiRendererDRPreset* dr, cons
 /* Regularization (MultiRendererDR.cpp:1933-2057) of bricks [0, n_nodes), added to grad. */
 int orc_dr_regularize(const orc_scene* s, uint32_t n_nodes, float lambda, float power, double* grad);
 
+/* Redistance (MultiRendererDR.cpp:2122-2382) of a dense p^3 lattice (x-major) in place. */
+int orc_dr_redistance(float* values, uint32_t p, float h, uint32_t iters);
+
 /* OptimizeStepAdam (MultiRendererDR.cpp:828-853): updates X, V, S in place. */
 void orc_adam_step(uint32_t iter, const float* dX, float* X, float* V, float* S, uint32_t n, float lr, float beta1,
                    float beta2, float eps);

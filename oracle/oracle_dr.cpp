@@ -872,6 +872,107 @@ extern "C" int orc_dr_regularize(const orc_scene* s, uint32_t n_nodes, float lam
   return 0;
 }
 
+// SolveEikonal, MultiRendererDR.cpp:2058-2120 (the unit-speed branch)
+static float solve_eikonal(float m[3], float h) {
+  if (m[0] > m[1]) std::swap(m[0], m[1]);
+  if (m[0] > m[2]) std::swap(m[0], m[2]);
+  if (m[1] > m[2]) std::swap(m[1], m[2]);
+  float dist_new = m[0] + h;
+  if (dist_new > m[1]) {
+    float h_2 = h * h;
+    float m_sum = m[0] + m[1], c_2 = (m[0] - m[1]) * (m[0] - m[1]);
+    dist_new = (m_sum + std::sqrt(2 * h_2 - c_2)) * 0.5f;
+    if (dist_new > m[2]) {
+      const float one_third = 1.f / 3;
+      m_sum += m[2];
+      c_2 = 3.f * (m[0] * m[0] + m[1] * m[1] + m[2] * m[2] - h_2);
+      dist_new = (m_sum + std::sqrt(m_sum * m_sum - c_2)) * one_third;
+    }
+  }
+  return dist_new;
+}
+
+// MultiRendererDR::Redistance, MultiRendererDR.cpp:2122-2382, as written:
+// frozen interface points, then per iteration 8 sweeps over the anti-diagonal
+// planes i + j + k = level (ascending, or descending for orderings 1, 4, 6, 7),
+// each point updated from the other buffer.
+extern "C" int orc_dr_redistance(float* d, uint32_t p, float h, uint32_t iters) {
+  if (!d || p == 0) return -1;
+  const size_t n = (size_t)p * p * p;
+  const int off[3] = {(int)(p * p), (int)p, 1};
+  const int sz[3] = {(int)p, (int)p, (int)p};
+  const float big = 99999.f;
+  std::vector<float> b[2] = {std::vector<float>(n, big), std::vector<float>(n, big)};
+  std::vector<uint8_t> frozen(n, 0);
+  for (int i = 0; i < sz[0]; i++)
+    for (int j = 0; j < sz[1]; j++)
+      for (int k = 0; k < sz[2]; k++) {
+        const int c[3] = {i, j, k};
+        const size_t lin = (size_t)i * off[0] + (size_t)j * off[1] + k;
+        const float v = d[lin];
+        if (v == 0.f) {
+          frozen[lin] = 1;
+          b[0][lin] = b[1][lin] = 0.f;
+          continue;
+        }
+        for (int dim = 0; dim < 3; dim++)
+          for (int sgn = -1; sgn < 2; sgn += 2) {
+            int q[3] = {c[0], c[1], c[2]};
+            q[dim] += sgn;
+            if (q[0] < 0 || q[0] >= sz[0] || q[1] < 0 || q[1] >= sz[1] || q[2] < 0 || q[2] >= sz[2]) continue;
+            if (d[(size_t)q[0] * off[0] + (size_t)q[1] * off[1] + q[2]] * v < 0) {
+              b[0][lin] = b[1][lin] = v;
+              frozen[lin] = 1;
+            }
+          }
+      }
+  int cur = 0;
+  const int max_level = sz[0] + sz[1] + sz[2] - 3;
+  for (uint32_t it = 0; it < iters; it++) {
+    for (int ordering = 0; ordering < 8; ordering++) {
+      const bool down = ordering == 1 || ordering == 4 || ordering == 6 || ordering == 7;
+      for (int level = down ? max_level : 0; down ? level >= 0 : level <= max_level; level += down ? -1 : 1) {
+        const int xs = std::max(0, level - (sz[1] + sz[2] - 2)), ys = std::max(0, level - (sz[0] + sz[2] - 2));
+        const int xe = std::min(sz[0] - 1, level), ye = std::min(sz[1] - 1, level);
+        for (int i = xs; i <= xe; i++)
+          for (int j = ys; j <= ye; j++) {
+            const int k = level - i - j;
+            if (k < 0 || k >= sz[2]) continue;
+            const int c[3] = {i, j, k};
+            const size_t lin = (size_t)i * off[0] + (size_t)j * off[1] + k;
+            const std::vector<float>& o = b[1 - cur];
+            b[cur][lin] = o[lin];
+            if (frozen[lin]) continue;
+            float am[3] = {big, big, big};
+            for (int dim = 0; dim < 3; dim++) {
+              if (c[dim] > 0 && c[dim] < sz[dim] - 1) {
+                am[dim] = o[lin - off[dim]];
+                float t = o[lin + off[dim]];
+                if (std::abs(t) < std::abs(am[dim])) am[dim] = t;
+              } else if (c[dim] > 0 && c[dim] == sz[dim] - 1) {
+                am[dim] = o[lin - off[dim]];
+              } else if (c[dim] == 0 && c[dim] < sz[dim] - 1) {
+                am[dim] = o[lin + off[dim]];
+              }
+            }
+            if (am[0] < big || am[1] < big || am[2] < big) {
+              if (am[0] >= 0.f && am[1] >= 0.f && am[2] >= 0.f) {
+                b[cur][lin] = std::min(solve_eikonal(am, h), o[lin]);
+              } else {
+                for (int dim = 0; dim < 3; dim++)
+                  if (am[dim] < 0.f) am[dim] = -am[dim];
+                b[cur][lin] = std::max(-solve_eikonal(am, h), -std::abs(o[lin]));
+              }
+            }
+          }
+      }
+    }
+    cur = 1 - cur;
+  }
+  for (size_t i = 0; i < n; i++) d[i] = b[1 - cur][i];
+  return 0;
+}
+
 // MultiRendererDR.cpp:828-853
 extern "C" void orc_adam_step(uint32_t iter, const float* dX, float* X, float* V, float* S, uint32_t n, float lr, float beta1,
                               float beta2, float eps) {

@@ -489,6 +489,29 @@ def test_dr_regularization_is_gradient_of_neighbour_energy(power):
         assert abs(fd - g[k]) <= 2e-3 * max(1.0, abs(g[k])), (k, fd, g[k])
 
 
+def test_dr_redistance_plane_closed_form():
+    """Redistance of a plane field with slope 2 along x: the two lattice
+    planes around the interface keep their values (frozen), every other point
+    ends at its frozen neighbour's value plus h per lattice step (the one-axis
+    Eikonal update m0 + h), accumulated in float; the sign is kept."""
+    p = 13
+    h = np.float32(2.0 / p)
+    x = (np.arange(p, dtype=np.float32) * np.float32(2.0 / (p - 1)) - 1).astype(np.float32)
+    field = np.broadcast_to((np.float32(2) * (x - np.float32(0.13)))[:, None, None], (p, p, p)).astype(np.float32)
+    out = oracle.dr_redistance(field, float(h), p)
+    col = field[:, 0, 0]
+    b = int(np.argmax(col > 0))  # first positive plane; b - 1 is the last negative one
+    exp = col.copy()
+    for i in range(b + 1, p):
+        exp[i] = np.float32(exp[i - 1] + h)
+    for i in range(b - 2, -1, -1):
+        exp[i] = -np.float32(abs(exp[i + 1]) + h)
+    assert np.array_equal(out, np.broadcast_to(exp[:, None, None], (p, p, p)))
+    # fewer iterations than the farthest plane needs leave the far planes at the start value
+    part = oracle.dr_redistance(field, float(h), 2)
+    assert part[p - 1, 0, 0] == np.float32(99999.0) and part[b + 1, 0, 0] == exp[b + 1]
+
+
 def test_adam_step_closed_form():
     rng = np.random.default_rng(9)
     n = 64

@@ -353,6 +353,46 @@ def test_adam_steps_with_regularization_match_oracle(power):
     assert np.abs(plain.values_f - sbs.values_f).max() > 1e-4
 
 
+def test_adam_steps_with_redistancing_match_oracle():
+    """OptimizeFixedStructure with redistancing every other step
+    (MultiRendererDR.cpp:459-466, 2058-2382): the dense lattice of the grid is
+    rebuilt by the Eikonal sweeps before each such Adam step. Values after
+    three steps against the oracle (render gradient, the reference's sweep
+    loops, Adam)."""
+    tgt, ini, cams, refs = _problem(brick_count=4, brick_size=4)
+    dr = _preset(border_spp=16)
+    dr.opt_iterations = 3
+    dr.opt_lr = 1e-3
+    dr.image_batch_size = 1
+    dr.redistancing_enable = True
+    dr.redistancing_interval = 2
+    wvs = np.stack([c[0] for c in cams])[:1]
+    prs = np.stack([c[1] for c in cams])[:1]
+    g = L.MultiRendererDR()
+    g.SetScene(ini)
+    g.SetReference(refs[:1], wvs, prs)
+    sbs = L.SdfSBS(ini.header, ini.nodes, ini.values, ini.values_f.copy())
+    g.OptimizeFixedStructure(dr, sbs, seed=5)
+    pairs, _ = g.GetBVH()
+    sc = oracle.Scene(ini, pairs)
+    ref = refs[0].copy()
+    ref[..., 3] = (np.linalg.norm(ref[..., :3], axis=-1) >= 0.001)
+    p = 4 * 4 + 1
+    X = ini.values_f.copy()
+    V = np.zeros_like(X)
+    S = np.zeros_like(X)
+    for it in range(3):
+        sc.set_values_f(X)
+        _, gr, _ = sc.dr_render_grad(dr, wvs[0], prs[0], W, H, ref, X.size, seed=5, precision=32)
+        gr = gr.astype(np.float32)
+        if it % 2 == 0:
+            X[:p ** 3] = oracle.dr_redistance(X[:p ** 3].reshape(p, p, p), 2.0 / p, p).ravel()
+        oracle.lib().orc_adam_step(it, oracle._p(gr), oracle._p(X), oracle._p(V), oracle._p(S), X.size, dr.opt_lr, dr.opt_beta_1,
+                                   dr.opt_beta_2, dr.opt_eps)
+    assert np.abs(sbs.values_f - X).max() <= 1e-6
+    assert np.abs(sbs.values_f[:p ** 3] - ini.values_f[:p ** 3]).max() > 1e-3  # the lattice was rebuilt
+
+
 def test_dr_optimisation_reduces_loss(problem):
     tgt, ini, cams, refs = problem
     dr = _preset(border_spp=32)
