@@ -151,6 +151,15 @@ int lrt_dr_render_grad(lrt_handle h, const MultiRendererDRPreset* preset, uint32
 int lrt_dr_optimize(lrt_handle h, const MultiRendererDRPreset* preset, const SdfSBSHeader* header, const SdfSBSNode* nodes,
                     uint32_t n_nodes, const uint32_t* values, uint64_t n_values, float* values_f, uint64_t n_values_f, uint32_t seed,
                     int dtdv_mode, float* losses, float* total_ms);
+/* MultiRendererDR::OptimizeGrid(start_grid_size, no_last_step_resize, presets),
+ * MultiRendererDR.cpp:228-276: coarse-to-fine reconstruction on dense grids
+ * of start_grid_size^3 bricks, starting from the reference's sphere (centre
+ * (0,-0.15,0), radius 0.7, brick size 1): per preset one OptimizeFixedStructure
+ * against the reference images set with lrt_dr_set_reference, then the
+ * lattice resampled into bricks of twice the size. out (may be NULL) receives
+ * the final grid, the scene the reference leaves in its local variable. */
+int lrt_dr_optimize_grid(lrt_handle h, uint32_t start_grid_size, int no_last_step_resize, const MultiRendererDRPreset* presets,
+                         uint32_t n_presets, uint32_t seed, int dtdv_mode, lrt_sbs_handle* out, float* total_ms);
 /* MultiRendererDR::getLastImage(view), MultiRendererDR.h:86: the image the last
  * RenderDR of that reference view produced (W*H*4 floats: colour + coverage). */
 int lrt_dr_get_last_image(lrt_handle h, uint32_t view, float* out);

@@ -145,6 +145,7 @@ def _load():
         "lrt_set_launch_blocks": (i32, [P, u32]),
         "lrt_sbs_displaced_icosphere": (i32, [u32, f32, f32, u32, u32, u32, u32, u32, u32, i32, i32, C.POINTER(P)]),
         "lrt_sbs_grid": (i32, [u32, f32, f32, f32, u32, u32, u32, u32, i32, C.POINTER(P)]),
+        "lrt_dr_optimize_grid": (i32, [P, u32, i32, P, u32, u32, i32, C.POINTER(P), C.POINTER(f32)]),
         "lrt_sbs_from_mesh": (i32, [P, u32, P, u32, u32, u32, u32, u32, i32, i32, C.POINTER(P), C.POINTER(f32)]),
         "lrt_sbs_from_mesh_device": (i32, [P, P, u32, P, u32, u32, u32, u32, u32, i32, C.POINTER(P), C.POINTER(f32)]),
         "lrt_mesh_displaced_icosphere": (i32, [u32, f32, f32, u32, u32, P, C.POINTER(u32), P, C.POINTER(u32)]),
@@ -590,6 +591,15 @@ class MultiRendererDR(MultiRenderer):
                                    _ptr(sbs.values), sbs.values.size, _ptr(sbs.values_f), sbs.values_f.size, seed, dtdv_mode,
                                    _ptr(losses), C.byref(ms)), "OptimizeFixedStructure")
         return losses, ms.value
+
+    def OptimizeGrid(self, start_grid_size, no_last_step_resize, presets, seed=0, dtdv_mode=0):
+        """MultiRendererDR::OptimizeGrid (MultiRendererDR.cpp:228-276); returns (final grid SdfSBS, total_ms)."""
+        arr = (MultiRendererDRPreset * len(presets))(*presets)
+        h = C.c_void_p()
+        ms = C.c_float()
+        _check(lib.lrt_dr_optimize_grid(self._h, start_grid_size, int(no_last_step_resize), arr, len(presets), seed, dtdv_mode,
+                                        C.byref(h), C.byref(ms)), "OptimizeGrid")
+        return SdfSBS._from_handle(h), ms.value
 
     def getLastdLoss_dS(self, n):
         out = np.zeros(n, np.float32)

@@ -338,12 +338,14 @@ SbsScene build_grid_sbs(uint32_t brick_count, uint32_t brick_size, const std::fu
   const uint32_t nb = brick_count * brick_count * brick_count;
   s.values.resize((size_t)nb * per_node);
   s.nodes.resize(nb);
-  const float inv = 1.f / (float)(brick_count * brick_size);
+  const float n = (float)(brick_count * brick_size);
+  // 2 * (xyz / n) - 1, DR_common.cpp:68-79
 #pragma omp parallel for schedule(static)
   for (int64_t x = 0; x < (int64_t)p_count; x++)
     for (uint32_t y = 0; y < p_count; y++)
       for (uint32_t z = 0; z < p_count; z++)
-        s.values_f[((size_t)x * p_count + y) * p_count + z] = sdf(f3(2.f * (x * inv) - 1.f, 2.f * (y * inv) - 1.f, 2.f * (z * inv) - 1.f));
+        s.values_f[((size_t)x * p_count + y) * p_count + z] =
+            sdf(f3(2.f * ((float)x / n) - 1.f, 2.f * ((float)y / n) - 1.f, 2.f * ((float)z / n) - 1.f));
   for (uint32_t x = 0; x < c_count; x++)
     for (uint32_t y = 0; y < c_count; y++)
       for (uint32_t z = 0; z < c_count; z++) {

@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "../../include/lrt.h"
@@ -337,6 +338,66 @@ int lrt_dr_optimize(lrt_handle h, const MultiRendererDRPreset* p, const SdfSBSHe
   if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_err(e, "optimize");
   h->values_f.assign(values_f, values_f + P);
   h->bfeat_valid = false;  // brick features follow the optimised distances
+  if (total_ms) *total_ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return LRT_OK;
+}
+
+// MultiRendererDR::OptimizeGrid, MultiRendererDR.cpp:228-276: start from a
+// sphere grid (centre (0,-0.15,0), radius 0.7, brick size 1, gradient colours,
+// with neighbour indices as create_grid_sbs returns it), and per preset run
+// OptimizeFixedStructure, then resample the optimised lattice trilinearly into
+// a grid with twice the brick size (the same size after the last preset when
+// no_last_step_resize).
+int lrt_dr_optimize_grid(lrt_handle h, uint32_t start_grid_size, int no_last_step_resize, const MultiRendererDRPreset* presets,
+                         uint32_t n_presets, uint32_t seed, int dtdv_mode, lrt_sbs_handle* out, float* total_ms) {
+  if (!h || !presets || n_presets == 0 || start_grid_size == 0 || start_grid_size > 256)
+    return set_err(LRT_E_INVALID, "bad arguments");
+  auto t0 = std::chrono::steady_clock::now();
+  auto circle = [](f3 p) { return length(p - f3(0.f, -0.15f, 0.f)) - 0.7f; };  // circle_sdf, DR_common.cpp:16-19
+  auto gradient_color = [](f3 p) {                                              // DR_common.cpp:25-35
+    const f3 r(1, 0, 0), g(0, 1, 0);
+    return (1 - p.x) * (1 - p.y) * (1 - p.z) * r + (1 - p.x) * (1 - p.y) * (p.z) * r + (1 - p.x) * (p.y) * (1 - p.z) * g +
+           (1 - p.x) * (p.y) * (p.z) * g + (p.x) * (1 - p.y) * (1 - p.z) * r + (p.x) * (1 - p.y) * (p.z) * r +
+           (p.x) * (p.y) * (1 - p.z) * g + (p.x) * (p.y) * (p.z) * g;
+  };
+  const uint32_t grid_size = start_grid_size;
+  uint32_t brick_size = 1;
+  SbsScene grid;
+  try {
+    grid = build_grid_sbs(grid_size, brick_size, circle, gradient_color, true);
+  } catch (const std::exception& ex) {
+    return set_err(LRT_E_INVALID, ex.what());
+  }
+  for (uint32_t i = 0; i < n_presets; i++) {
+    int c = lrt_dr_optimize(h, &presets[i], &grid.header, grid.nodes.data(), (uint32_t)grid.nodes.size(), grid.values.data(),
+                            grid.values.size(), grid.values_f.data(), grid.values_f.size(), seed, dtdv_mode, nullptr, nullptr);
+    if (c) return c;
+    const uint32_t p_count = grid_size * brick_size + 1u;
+    const std::vector<float> vf = grid.values_f;
+    auto sampler = [&](f3 p) {  // grid_sampler, MultiRendererDR.cpp:240-262
+      const float gs = (float)grid_size, bs = (float)brick_size;
+      auto cl = [](float v) { return std::min(std::max(v, 0.0f), 1.0f + 1e-6f); };
+      const f3 idx_f(cl(0.5f * (p.x + 1.0f)) * gs * bs, cl(0.5f * (p.y + 1.0f)) * gs * bs, cl(0.5f * (p.z + 1.0f)) * gs * bs);
+      const uint32_t ix = (uint32_t)(idx_f.x + 1e-6f), iy = (uint32_t)(idx_f.y + 1e-6f), iz = (uint32_t)(idx_f.z + 1e-6f);
+      const f3 dp(idx_f.x - (float)ix, idx_f.y - (float)iy, idx_f.z - (float)iz);
+      auto at = [&](uint32_t x, uint32_t y, uint32_t z) { return vf[(size_t)x * p_count * p_count + (size_t)y * p_count + z]; };
+      if (ix < p_count - 1 && iy < p_count - 1 && iz < p_count - 1)
+        return (1 - dp.x) * (1 - dp.y) * (1 - dp.z) * at(ix, iy, iz) + (1 - dp.x) * (1 - dp.y) * dp.z * at(ix, iy, iz + 1) +
+               (1 - dp.x) * dp.y * (1 - dp.z) * at(ix, iy + 1, iz) + (1 - dp.x) * dp.y * dp.z * at(ix, iy + 1, iz + 1) +
+               dp.x * (1 - dp.y) * (1 - dp.z) * at(ix + 1, iy, iz) + dp.x * (1 - dp.y) * dp.z * at(ix + 1, iy, iz + 1) +
+               dp.x * dp.y * (1 - dp.z) * at(ix + 1, iy + 1, iz) + dp.x * dp.y * dp.z * at(ix + 1, iy + 1, iz + 1);
+      return at(ix, iy, iz);
+    };
+    const uint32_t new_brick_size = (i == n_presets - 1 && no_last_step_resize) ? brick_size : 2 * brick_size;
+    if (grid_size * new_brick_size > 4096) return set_err(LRT_E_INVALID, "grid too fine");
+    grid = build_grid_sbs(grid_size, new_brick_size, sampler, gradient_color, true);
+    brick_size = new_brick_size;
+  }
+  if (out) {
+    std::unique_ptr<lrt_sbs> s(new lrt_sbs);
+    s->s = std::move(grid);
+    *out = s.release();
+  }
   if (total_ms) *total_ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return LRT_OK;
 }

@@ -393,6 +393,70 @@ def test_adam_steps_with_redistancing_match_oracle():
     assert np.abs(sbs.values_f[:p ** 3] - ini.values_f[:p ** 3]).max() > 1e-3  # the lattice was rebuilt
 
 
+def _grid_resample_reference(vf, grid_size, brick_size, new_brick_size):
+    """numpy restatement of OptimizeGrid's trilinear grid_sampler
+    (MultiRendererDR.cpp:240-262) evaluated at create_grid_sbs's lattice points
+    (DR_common.cpp:68-79), in float32 with the same operation order."""
+    f = np.float32
+    p_old = grid_size * brick_size + 1
+    lat = vf[:p_old ** 3].reshape(p_old, p_old, p_old)
+    p_new = grid_size * new_brick_size + 1
+    x = f(2) * (np.arange(p_new, dtype=f) / f(grid_size * new_brick_size)) - f(1)
+    idx_f = np.minimum(np.maximum(f(0.5) * (x + f(1)), f(0)), f(1) + f(1e-6)) * f(grid_size) * f(brick_size)
+    idx = (idx_f + f(1e-6)).astype(np.uint32)
+    dp = (idx_f - idx.astype(f)).astype(f)
+    I, J, K = np.meshgrid(idx, idx, idx, indexing="ij")
+    DX, DY, DZ = np.meshgrid(dp, dp, dp, indexing="ij")
+    inner = (I < p_old - 1) & (J < p_old - 1) & (K < p_old - 1)
+    i1, j1, k1 = np.minimum(I + 1, p_old - 1), np.minimum(J + 1, p_old - 1), np.minimum(K + 1, p_old - 1)
+    terms = [((1 - DX) * (1 - DY) * (1 - DZ), lat[I, J, K]), ((1 - DX) * (1 - DY) * DZ, lat[I, J, k1]),
+             ((1 - DX) * DY * (1 - DZ), lat[I, j1, K]), ((1 - DX) * DY * DZ, lat[I, j1, k1]),
+             (DX * (1 - DY) * (1 - DZ), lat[i1, J, K]), (DX * (1 - DY) * DZ, lat[i1, J, k1]),
+             (DX * DY * (1 - DZ), lat[i1, j1, K]), (DX * DY * DZ, lat[i1, j1, k1])]
+    acc = (terms[0][0] * terms[0][1]).astype(f)
+    for w, v in terms[1:]:
+        acc = (acc + (w * v).astype(f)).astype(f)
+    return np.where(inner, acc, lat[I, J, K]).astype(f)
+
+
+def test_optimize_grid_resamples_like_the_reference():
+    """OptimizeGrid (MultiRendererDR.cpp:228-276): with a zero learning rate
+    every OptimizeFixedStructure leaves the values unchanged, so the final grid
+    is the start sphere (create_grid_sbs of circle_sdf((0,-0.15,0), 0.7))
+    resampled twice by grid_sampler; compared bit for bit with a numpy
+    restatement of both. A second run with a learning rate then changes it."""
+    tgt, ini, cams, refs = _problem(brick_count=4, brick_size=4)
+    wvs = np.stack([c[0] for c in cams])[:2]
+    prs = np.stack([c[1] for c in cams])[:2]
+    g = L.MultiRendererDR()
+    g.SetReference(refs[:2], wvs, prs)
+    presets = []
+    for _ in range(2):
+        dr = _preset(border_spp=8)
+        dr.opt_iterations = 2
+        dr.opt_lr = 0.0
+        dr.image_batch_size = 1
+        presets.append(dr)
+    gs = 4
+    final, ms = g.OptimizeGrid(gs, False, presets, seed=3)
+    assert final.header.brick_size == 4 and final.nodes.shape[0] == gs ** 3
+    f = np.float32
+    p0 = gs * 1 + 1
+    x = f(2) * (np.arange(p0, dtype=f) / f(gs)) - f(1)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    # circle_sdf: length(p - c) - r, dot summed x, y, z
+    lat0 = (np.sqrt((X * X + (Y - f(-0.15)) * (Y - f(-0.15))) + Z * Z).astype(f) - f(0.7)).astype(f)
+    lat1 = _grid_resample_reference(lat0.ravel(), gs, 1, 2)
+    lat2 = _grid_resample_reference(lat1.ravel(), gs, 2, 4)
+    p2 = gs * 4 + 1
+    assert np.array_equal(final.values_f[:p2 ** 3].reshape(p2, p2, p2), lat2)
+    for dr in presets:
+        dr.opt_lr = 5e-3
+    moved, _ = g.OptimizeGrid(gs, True, presets, seed=3)
+    assert moved.header.brick_size == 2
+    assert np.abs(moved.values_f[:(gs * 2 + 1) ** 3] - lat1.ravel()).max() > 1e-3
+
+
 def test_dr_optimisation_reduces_loss(problem):
     tgt, ini, cams, refs = problem
     dr = _preset(border_spp=32)
