@@ -54,6 +54,12 @@ int lrt_get_bvh(lrt_handle h, BVHNodePair* out, uint32_t capacity, uint32_t* cou
  * SAH builder in place of Embree). out = NULL queries *count and *depth. */
 int lrt_build_bvh(const SdfSBSNode* nodes, uint32_t n_nodes, BVHNodePair* out, uint32_t capacity, uint32_t* count, uint32_t* depth);
 
+/* MultiRenderer::SetLights, Renderer/eye_ray.h:158, eye_ray_host.cpp:673-676:
+ * the lights of the LAMBERT / PHONG render modes and of DR LAMBERT, taken as
+ * given (directions already unit, as create_direct_light leaves them); at
+ * most LRT_MAX_LIGHTS. Default: a direct light along normalize(1,1,1) of
+ * colour 2/3 and an ambient light of 0.25 (eye_ray_host.cpp:40). */
+int lrt_set_lights(lrt_handle h, const LrtLight* lights, uint32_t n);
 /* MultiRenderer::SetPreset / GetPreset, eye_ray_host.cpp:549-561. */
 int lrt_set_preset(lrt_handle h, const MultiRenderPreset* preset);
 int lrt_get_preset(lrt_handle h, MultiRenderPreset* preset);

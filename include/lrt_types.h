@@ -128,6 +128,18 @@ typedef struct BVHNodePair {
   BVHNode right;
 } BVHNodePair;
 
+/* ---- lights (Renderer/eye_ray.h:44-74) ----------------------------------- */
+#define LRT_LIGHT_TYPE_DIRECT  0u
+#define LRT_LIGHT_TYPE_POINT   1u
+#define LRT_LIGHT_TYPE_AMBIENT 2u
+#define LRT_MAX_LIGHTS         8u
+typedef struct LrtLight {
+  float    space[3];  /* direction (unit, direct) or position (point) */
+  uint32_t type;
+  float    color[3];  /* intensity included */
+  uint32_t _pad;
+} LrtLight;
+
 /* ---- ray hit (CrossRT CRT_Hit, fields per BVH2Common.cpp:3497-3505) ----- */
 typedef struct CRT_Hit {
   float    t;

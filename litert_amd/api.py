@@ -58,6 +58,18 @@ class CRT_Hit(C.Structure):
                 ("coords", C.c_float * 4)]
 
 
+LIGHT_TYPE_DIRECT, LIGHT_TYPE_POINT, LIGHT_TYPE_AMBIENT = 0, 1, 2
+
+
+class Light(C.Structure):
+    """Renderer/eye_ray.h:48-54: direction (direct) or position (point), type, colour."""
+    _fields_ = [("space", C.c_float * 3), ("type", C.c_uint32), ("color", C.c_float * 3), ("_pad", C.c_uint32)]
+
+
+def make_light(kind, space, color):
+    return Light((C.c_float * 3)(*space), kind, (C.c_float * 3)(*color), 0)
+
+
 class MultiRendererDRPreset(C.Structure):
     _fields_ = [
         ("dr_loss_function", C.c_uint32), ("dr_render_mode", C.c_uint32), ("dr_diff_mode", C.c_uint32),
@@ -146,6 +158,7 @@ def _load():
         "lrt_sbs_displaced_icosphere": (i32, [u32, f32, f32, u32, u32, u32, u32, u32, u32, i32, i32, C.POINTER(P)]),
         "lrt_sbs_grid": (i32, [u32, f32, f32, f32, u32, u32, u32, u32, i32, C.POINTER(P)]),
         "lrt_dr_optimize_grid": (i32, [P, u32, i32, P, u32, u32, i32, C.POINTER(P), C.POINTER(f32)]),
+        "lrt_set_lights": (i32, [P, P, u32]),
         "lrt_sbs_from_mesh": (i32, [P, u32, P, u32, u32, u32, u32, u32, i32, i32, C.POINTER(P), C.POINTER(f32)]),
         "lrt_sbs_from_mesh_device": (i32, [P, P, u32, P, u32, u32, u32, u32, u32, i32, C.POINTER(P), C.POINTER(f32)]),
         "lrt_mesh_displaced_icosphere": (i32, [u32, f32, f32, u32, u32, P, C.POINTER(u32), P, C.POINTER(u32)]),
@@ -364,6 +377,11 @@ class MultiRenderer:
         pairs = np.zeros((n.value, 16), np.uint32)  # 2 x (boxMin[3], leftOffset, boxMax[3], escapeIndex)
         _check(lib.lrt_get_bvh(self._h, _ptr(pairs), n.value, None, None), "GetBVH")
         return pairs, depth.value
+
+    def SetLights(self, lights):
+        """MultiRenderer::SetLights (eye_ray.h:158): a list of Light."""
+        arr = (Light * max(1, len(lights)))(*lights)
+        _check(lib.lrt_set_lights(self._h, arr, len(lights)), "SetLights")
 
     def SetPreset(self, preset):
         _check(lib.lrt_set_preset(self._h, C.byref(preset)), "SetPreset")

@@ -18,15 +18,22 @@ namespace lrtd {
 
 __device__ __forceinline__ float lin_depth(float z) { return (z - 0.1f) / (10.0f - 0.1f); }
 
-__device__ __forceinline__ v3 dr_color(uint32_t mode, const DRHit& h) {  // MultiRendererDR.cpp:855-898
+__device__ __forceinline__ v3 dr_color(uint32_t mode, const DRHit& h, const LightSet& L) {  // MultiRendererDR.cpp:855-898
   if (mode == LRT_DR_RENDER_MODE_DIFFUSE) return h.color;
   if (mode == LRT_DR_RENDER_MODE_MASK) return v3(1, 1, 1);
   if (mode == LRT_DR_RENDER_MODE_NORMAL) return h.normal;
-  if (mode == LRT_DR_RENDER_MODE_LAMBERT) {  // default lights: direct, then ambient
-    const v3 lc(kLightCol, kLightCol, kLightCol), la(kAmbient, kAmbient, kAmbient);
-    const float q = fmaxf(0.0f, dot(h.normal, v3(kLightDir, kLightDir, kLightDir)));
-    v3 c = (lc * h.color) * q;
-    return c + la * h.color;
+  if (mode == LRT_DR_RENDER_MODE_LAMBERT) {  // direct and ambient lights, in the renderer's order
+    v3 c(0, 0, 0);
+    for (uint32_t i = 0; i < L.n; i++) {
+      const v3 lc(L.col[i].x, L.col[i].y, L.col[i].z);
+      if (__float_as_uint(L.sp[i].w) == LRT_LIGHT_TYPE_DIRECT) {
+        const float q = fmaxf(0.0f, dot(h.normal, v3(L.sp[i].x, L.sp[i].y, L.sp[i].z)));
+        c = c + (lc * h.color) * q;
+      } else {
+        c = c + lc * h.color;
+      }
+    }
+    return c;
   }
   float d = lin_depth(h.t);
   return v3(d, d, d);
@@ -112,13 +119,14 @@ __device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, S
   py.scatter = 0;
   py.mode = A.mode;
   py.grad = A.grad;
+  if (SPEC == kDrNormal) py.lights = &A.lights;
   for (uint32_t s = 0; s < spp; s++) {
     float2 d = rand2(x, y, s, A.seed);
     v3 pos, dir;
     eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
     DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, A.bminmax, Qc, A.relax_eps, pos, dir, 1e9f, &py, st);
     if (hit.primId == 0xFFFFFFFFu) continue;
-    v3 c = dr_color(A.mode, hit);
+    v3 c = dr_color(A.mode, hit, A.lights);
     res = res + v3(c.x / (float)spp, c.y / (float)spp, c.z / (float)spp);
     if (hit_count < 16) samples[hit_count] = hit.t;
     hit_count++;
@@ -233,8 +241,8 @@ __global__ __launch_bounds__(256) void k_dr_border_rays(DRArgs A, const uint32_t
       float dout = hit.primId == 0xFFFFFFFFu ? 0.0f : lin_depth(hit.t);
       delta[0] = delta[1] = delta[2] = din - dout;
     } else {
-      v3 fin = dr_color(A.mode, py.missed_hit);
-      v3 fout = hit.primId == 0xFFFFFFFFu ? v3(0, 0, 0) : dr_color(A.mode, hit);
+      v3 fin = dr_color(A.mode, py.missed_hit, A.lights);
+      v3 fout = hit.primId == 0xFFFFFFFFu ? v3(0, 0, 0) : dr_color(A.mode, hit, A.lights);
       delta[0] = fin.x - fout.x; delta[1] = fin.y - fout.y; delta[2] = fin.z - fout.z;
     }
     delta[3] = hit.primId == 0xFFFFFFFFu ? 1.0f : 0.0f;

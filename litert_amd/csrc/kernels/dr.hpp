@@ -24,6 +24,7 @@ struct DRArgs {
   float* grad;         // dLoss/dS, one float per values_f entry
   const float* bvals;     // per-brick corner values gathered from values_f (k_dr_gather)
   const float2* bminmax;  // smallest / largest of each brick's gathered values
+  LightSet lights;        // LAMBERT: direct and ambient lights
 };
 
 }  // namespace lrtd

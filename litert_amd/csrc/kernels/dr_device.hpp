@@ -35,6 +35,7 @@ struct DRPayload {
   float* grad;
   v3 dldc;
   uint32_t mode, scatter;
+  const LightSet* lights;
 };
 
 struct DRParams {
@@ -194,12 +195,10 @@ __device__ __forceinline__ v3 trilinear_ddp_dvalue(int c, v3 dp) {
   return v3((sx * wy) * wz, (sy * wx) * wz, (sz * wx) * wy);
 }
 
-// The renderer's default lights (Renderer/eye_ray_host.cpp:40): direct along
-// normalize(1,1,1) (1/sqrt(3) rounded to float) of colour 2/3, ambient 0.25.
-constexpr float kLightDir = 0.57735026918962576f, kLightCol = 2.0f / 3.0f, kAmbient = 0.25f;
-
-// CalculateColorWithGrad (MultiRendererDR.cpp:900-980) for LAMBERT and NORMAL
-__device__ __forceinline__ void dr_color_grad(uint32_t mode, const DRHit& h, m3& dCdD, m3& dCdN) {
+// CalculateColorWithGrad (MultiRendererDR.cpp:900-980) for LAMBERT and NORMAL;
+// LAMBERT sums over the renderer's direct and ambient lights (point lights are
+// refused on the host, as the reference asserts).
+__device__ __forceinline__ void dr_color_grad(uint32_t mode, const DRHit& h, const LightSet& L, m3& dCdD, m3& dCdN) {
   const v3 z(0, 0, 0);
   dCdD.r[0] = dCdD.r[1] = dCdD.r[2] = z;
   dCdN.r[0] = dCdN.r[1] = dCdN.r[2] = z;
@@ -209,19 +208,25 @@ __device__ __forceinline__ void dr_color_grad(uint32_t mode, const DRHit& h, m3&
     dCdN.r[2] = v3(0, 0, 1);
     return;
   }
-  const v3 L(kLightDir, kLightDir, kLightDir);
-  const float q = fmaxf(0.0f, dot(h.normal, L));
-  if (q > 0.0f) {
-    dCdD.r[0] = v3(kLightCol * q, 0, 0);
-    dCdD.r[1] = v3(0, kLightCol * q, 0);
-    dCdD.r[2] = v3(0, 0, kLightCol * q);
-    dCdN.r[0] = (kLightCol * h.color.x) * L;
-    dCdN.r[1] = (kLightCol * h.color.y) * L;
-    dCdN.r[2] = (kLightCol * h.color.z) * L;
+  for (uint32_t i = 0; i < L.n; i++) {
+    const v3 lc(L.col[i].x, L.col[i].y, L.col[i].z);
+    if (__float_as_uint(L.sp[i].w) == LRT_LIGHT_TYPE_DIRECT) {
+      const v3 ld(L.sp[i].x, L.sp[i].y, L.sp[i].z);
+      const float q = fmaxf(0.0f, dot(h.normal, ld));
+      if (q > 0.0f) {
+        dCdD.r[0] = dCdD.r[0] + v3(lc.x * q, 0, 0);
+        dCdD.r[1] = dCdD.r[1] + v3(0, lc.y * q, 0);
+        dCdD.r[2] = dCdD.r[2] + v3(0, 0, lc.z * q);
+        dCdN.r[0] = dCdN.r[0] + (lc.x * h.color.x) * ld;
+        dCdN.r[1] = dCdN.r[1] + (lc.y * h.color.y) * ld;
+        dCdN.r[2] = dCdN.r[2] + (lc.z * h.color.z) * ld;
+      }
+    } else {  // ambient
+      dCdD.r[0] = dCdD.r[0] + v3(lc.x, 0, 0);
+      dCdD.r[1] = dCdD.r[1] + v3(0, lc.y, 0);
+      dCdD.r[2] = dCdD.r[2] + v3(0, 0, lc.z);
+    }
   }
-  dCdD.r[0] = dCdD.r[0] + v3(kAmbient, 0, 0);
-  dCdD.r[1] = dCdD.r[1] + v3(0, kAmbient, 0);
-  dCdD.r[2] = dCdD.r[2] + v3(0, 0, kAmbient);
 }
 
 // The hit of a kDrNormal ray cast (BVH2DR.cpp:1277-1524 with the pixel's
@@ -287,7 +292,7 @@ __device__ __noinline__ void dr_normal_hit(const DevScene& S, uint32_t flags, bo
     pHit->normal = normalize(g);
     if (!relax->scatter) return;
     m3 dCdD, dCdN;
-    dr_color_grad(relax->mode, *pHit, dCdD, dCdN);
+    dr_color_grad(relax->mode, *pHit, *relax->lights, dCdD, dCdN);
     const v3 dl = relax->dldc;
     if (flags & LRT_DR_RAY_FLAG_DDIFFUSE_DCOLOR) {
       for (int i = 0; i < 8; i++) {
@@ -369,7 +374,7 @@ __device__ __noinline__ void dr_normal_hit(const DevScene& S, uint32_t flags, bo
   pHit->normal = normalize(tri);
   if (!relax->scatter) return;
   m3 dCdD, dCdN;
-  dr_color_grad(relax->mode, *pHit, dCdD, dCdN);
+  dr_color_grad(relax->mode, *pHit, *relax->lights, dCdD, dCdN);
   const v3 dl = relax->dldc;
   if (flags & LRT_DR_RAY_FLAG_DDIFFUSE_DCOLOR) {
     for (int i = 0; i < 8; i++) {

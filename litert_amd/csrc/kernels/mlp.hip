@@ -413,7 +413,16 @@ __device__ __forceinline__ void seeded_pixel(const DevScene& S, const TraceParam
         if (O.n) O.n[pix] = make_float2(hit.coords[2], hit.coords[3]);
       }
       if (RGBA) {
-        const float4 c = shade(P.render_mode, hit, dir);
+        uint32_t lit = 0xFFFFFFFFu;
+        if (needs_shadow(P.render_mode) && hit.primId != 0xFFFFFFFFu) {  // shadow rays: plain exact traversal
+          for (uint32_t li = 0; li < A.lights.n; li++) {
+            v3 o, ld;
+            if (!shadow_ray(A.lights, li, pos, dir, hit.t, o, ld)) continue;
+            const Hit sh = RayQuery_NearestHit<COUNT, RF, IDX>(S, P, o, 0.0f, ld, 1e9f, st, &wc);
+            if (sh.primId != 0xFFFFFFFFu) lit &= ~(1u << li);
+          }
+        }
+        const float4 c = shade(P.render_mode, hit, pos, dir, A.lights, lit);
         acc = make_float4(acc.x + c.x, acc.y + c.y, acc.z + c.z, acc.w + c.w);
       }
     }

@@ -52,15 +52,16 @@ __device__ __forceinline__ void render_pixel(const DevScene& S, const TraceParam
       if (O.n) O.n[pix] = make_float2(hit.coords[2], hit.coords[3]);
     }
     if (RGBA) {
-      float shadow = 1.0f;
+      uint32_t lit = 0xFFFFFFFFu;
       if (needs_shadow(P.render_mode) && hit.primId != 0xFFFFFFFFu) {
-        // shadow ray toward the direct light from just before the hit
-        // (eye_ray.cpp:448-468: BIAS 1e-6, the eye ray's tNear and tFar)
-        const v3 surf = pos + (hit.t - 1e-6f) * dir;
-        const Hit sh = RayQuery_NearestHit<COUNT, RF, IDX>(S, P, surf, 0.0f, normalize(v3(1, 1, 1)), 1e9f, st, &wc);
-        shadow = sh.primId == 0xFFFFFFFFu ? 1.0f : 0.0f;
+        for (uint32_t li = 0; li < A.lights.n; li++) {
+          v3 o, ld;
+          if (!shadow_ray(A.lights, li, pos, dir, hit.t, o, ld)) continue;
+          const Hit sh = RayQuery_NearestHit<COUNT, RF, IDX>(S, P, o, 0.0f, ld, 1e9f, st, &wc);
+          if (sh.primId != 0xFFFFFFFFu) lit &= ~(1u << li);
+        }
       }
-      float4 c = shade(P.render_mode, hit, dir, shadow);
+      float4 c = shade(P.render_mode, hit, pos, dir, A.lights, lit);
       acc = make_float4(acc.x + c.x, acc.y + c.y, acc.z + c.z, acc.w + c.w);
     }
   }

@@ -20,6 +20,7 @@ struct RenderArgs {
   uint32_t packed_w, n_packed;  // packed dims are rounded up to multiples of 8
   uint32_t n_views;
   uint32_t spp, ray_gen_mode, seed;
+  LightSet lights;  // shading modes LAMBERT / PHONG
 };
 
 struct RenderOut {

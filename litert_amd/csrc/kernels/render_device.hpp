@@ -14,16 +14,36 @@ __device__ __forceinline__ float pow32(float x) {
   return x * x;
 }
 
-// True for the modes whose colour needs a shadow ray toward the direct light.
+// True for the modes whose colour needs shadow rays toward the lights.
 __device__ __forceinline__ bool needs_shadow(uint32_t mode) {
   return mode == LRT_RENDER_MODE_PHONG || mode == LRT_RENDER_MODE_PHONG_NO_TEX;
 }
 
+__device__ __forceinline__ v3 light_v(const float4 f) { return v3(f.x, f.y, f.z); }
+__device__ __forceinline__ uint32_t light_type(const LightSet& L, uint32_t i) { return __float_as_uint(L.sp[i].w); }
+
+// Phong's shadow ray toward light i from just before the hit (eye_ray.cpp:
+// 443-468: BIAS 1e-6; direct lights along their direction, point lights
+// toward their position; the eye ray's tNear / tFar, so the ray is not
+// stopped at the light). Returns false for ambient lights.
+__device__ __forceinline__ bool shadow_ray(const LightSet& L, uint32_t i, v3 ray_pos, v3 ray_dir, float t, v3& origin, v3& dir) {
+  const uint32_t ty = light_type(L, i);
+  if (ty == LRT_LIGHT_TYPE_AMBIENT) return false;
+  origin = ray_pos + (t - 1e-6f) * ray_dir;
+  if (ty == LRT_LIGHT_TYPE_DIRECT) {
+    dir = light_v(L.sp[i]);
+  } else {
+    const v3 d = light_v(L.sp[i]) - origin;
+    const float l = sqrtf(dot(d, d));
+    dir = v3(d.x / l, d.y / l, d.z / l);
+  }
+  return true;
+}
+
 // Render modes that apply to SBS geometry (eye_ray.cpp:162-489) with the
-// renderer's default lights: direct normalize(1,1,1) colour 2/3 and ambient
-// 0.25 (eye_ray_host.cpp:40). shadow: 1 if the Phong shadow ray toward the
-// direct light found nothing, else 0.
-__device__ __forceinline__ float4 shade(uint32_t mode, const Hit& hit, v3 dir, float shadow = 1.0f) {
+// renderer's lights. lit: bit i set if Phong's shadow ray toward light i found
+// nothing. ray_pos: the eye ray's origin.
+__device__ __forceinline__ float4 shade(uint32_t mode, const Hit& hit, v3 ray_pos, v3 dir, const LightSet& L, uint32_t lit) {
   if (hit.primId == 0xFFFFFFFFu) return make_float4(0, 0, 0, 1);
   const float z = hit.t, z_near = 0.1f, z_far = 10.0f;
   const uint32_t type = hit.geomId >> LRT_SH_TYPE;
@@ -61,36 +81,55 @@ __device__ __forceinline__ float4 shade(uint32_t mode, const Hit& hit, v3 dir, f
         return make_float4(roundf(hit.coords[0]) / 255.0f, hit.coords[0] - floorf(hit.coords[0]), roundf(hit.coords[1]) / 255.0f, 1);
       return make_float4(0, 1, 1, 1);
     }
-    case LRT_RENDER_MODE_LAMBERT_NO_TEX: {
-      v3 ld = normalize(v3(1, 1, 1));
-      float q = mx(0.0f, dot(norm, ld));
-      float c = (2.0f / 3.0f) * q + 0.25f;
-      return make_float4(c, c, c, 1);
-    }
     case LRT_RENDER_MODE_BARYCENTRIC: return make_float4(hit.coords[0], hit.coords[1], 1 - hit.coords[0] - hit.coords[1], 1);
     case LRT_RENDER_MODE_TEX_COORDS: return make_float4(hit.coords[0], hit.coords[1], 0, 1);
+    case LRT_RENDER_MODE_LAMBERT_NO_TEX:
     case LRT_RENDER_MODE_LAMBERT:
     case LRT_RENDER_MODE_PHONG:
     case LRT_RENDER_MODE_PHONG_NO_TEX: {
-      // colour: white for PHONG_NO_TEX, the brick colour for colour layouts,
-      // else the default material's (0,1,1) texture (eye_ray.cpp:376-392, 426-441)
+      // colour: white for the NO_TEX modes, the brick colour for colour
+      // layouts, else the default material's (0,1,1) texture (eye_ray.cpp:376-392, 426-441)
       v3 col(1, 1, 1);
-      if (mode != LRT_RENDER_MODE_PHONG_NO_TEX) {
+      if (mode == LRT_RENDER_MODE_LAMBERT || mode == LRT_RENDER_MODE_PHONG) {
         col = type == LRT_TYPE_SDF_SBS_COL
                   ? v3(roundf(hit.coords[0]) / 255.0f, hit.coords[0] - floorf(hit.coords[0]), roundf(hit.coords[1]) / 255.0f)
                   : v3(0, 1, 1);
       }
-      const v3 ld = normalize(v3(1, 1, 1));
-      const v3 lc(2.0f / 3.0f, 2.0f / 3.0f, 2.0f / 3.0f), la(0.25f, 0.25f, 0.25f);
-      const float q = mx(0.0f, dot(norm, ld));
-      v3 c = (lc * col) * q;  // direct light, diffuse part
-      if (mode != LRT_RENDER_MODE_LAMBERT) {  // Phong: shadowed diffuse + specular (eye_ray.cpp:443-475)
+      const bool phong = mode == LRT_RENDER_MODE_PHONG || mode == LRT_RENDER_MODE_PHONG_NO_TEX;
+      v3 c(0, 0, 0);
+      for (uint32_t i = 0; i < L.n; i++) {
+        const uint32_t ty = light_type(L, i);
+        const v3 lc = light_v(L.col[i]);
+        if (ty == LRT_LIGHT_TYPE_AMBIENT) {
+          c = c + lc * col;
+          continue;
+        }
+        if (!phong) {  // Lambert, eye_ray.cpp:394-418
+          if (ty == LRT_LIGHT_TYPE_DIRECT) {
+            const float q = mx(0.0f, dot(norm, light_v(L.sp[i])));
+            c = c + (lc * col) * q;
+          } else {
+            const v3 surf = ray_pos + hit.t * dir;
+            v3 ld = light_v(L.sp[i]) - surf;
+            const float l = sqrtf(dot(ld, ld));
+            ld = v3(ld.x / l, ld.y / l, ld.z / l);
+            const float q = mx(0.0f, dot(norm, ld));
+            const v3 e = (lc * col) * q;
+            c = c + v3(e.x / (l * l), e.y / (l * l), e.z / (l * l));
+          }
+          continue;
+        }
+        // Phong, eye_ray.cpp:443-475: the direct light's Lambert term, then
+        // the shadowed diffuse + specular term of every non-ambient light
+        v3 o, ld;
+        shadow_ray(L, i, ray_pos, dir, hit.t, o, ld);
+        if (ty == LRT_LIGHT_TYPE_DIRECT) c = c + (lc * col) * mx(0.0f, dot(norm, ld));
         const float Kd = 0.25f, Ks = 0.25f;
         const v3 refl = -1.0f * ld + (2.0f * dot(norm, ld)) * norm;
         const float e = Kd * mx(0.0f, dot(norm, ld)) + Ks * pow32(mx(0.0f, dot(norm, refl)));
+        const float shadow = (lit >> i) & 1u ? 1.0f : 0.0f;
         c = c + ((shadow * lc) * e) * col;
       }
-      c = c + la * col;  // ambient light
       return make_float4(c.x, c.y, c.z, 1);
     }
     default: return make_float4(1, 0, 1, 1);

@@ -181,6 +181,14 @@ __device__ __forceinline__ float load_voxel(const DevScene& S, uint32_t v_off, u
   return load_voxel_q<IDX>(S, v_off, vx, vy, vz, d_max, mult, values);
 }
 
+// The renderer's lights (MultiRenderer::m_lights, eye_ray.h:230), passed by
+// value: sp = direction or position with the type's bits in w, col = colour.
+struct LightSet {
+  float4 sp[LRT_MAX_LIGHTS];
+  float4 col[LRT_MAX_LIGHTS];
+  uint32_t n = 0;
+};
+
 struct TraceParams {
   uint32_t sdf_node_intersect;
   uint32_t representation_mode;

@@ -196,6 +196,7 @@ int Renderer::render_device(const lrtd::Camera* cams, uint32_t n_views, uint32_t
   A.spp = preset.spp;
   A.ray_gen_mode = preset.ray_gen_mode;
   A.seed = seed;
+  A.lights = light_set();
   const uint32_t blocks = grid_blocks();
   const bool seeded = seeding && mlp_ready;
   if ((e = ensure_spill(blocks, seeded ? lrtd::kStackLdsSeeded : lrtd::kStackLds)) != hipSuccess) return hip_err(e, "alloc spill");
@@ -255,6 +256,15 @@ int lrt_create(uint32_t device, uint32_t max_primitives, lrt_handle* out) {
   r->preset = MultiRenderPreset{LRT_RENDER_MODE_LAMBERT_NO_TEX, LRT_SDF_NODE_INTERSECT_ST, LRT_NORMAL_MODE_GEOMETRY,
                                 LRT_RAY_GEN_MODE_REGULAR, LRT_INTERPOLATION_MODE_TRILINEAR, LRT_REPRESENTATION_MODE_VOLUME, 1};
   *out = r.release();
+  return LRT_OK;
+}
+
+int lrt_set_lights(lrt_handle h, const LrtLight* lights, uint32_t n) {
+  if (!h || (n && !lights)) return set_err(LRT_E_INVALID, "null argument");
+  if (n > LRT_MAX_LIGHTS) return set_err(LRT_E_INVALID, "at most LRT_MAX_LIGHTS lights");
+  for (uint32_t i = 0; i < n; i++)
+    if (lights[i].type > LRT_LIGHT_TYPE_AMBIENT) return set_err(LRT_E_INVALID, "bad light type");
+  h->lights.assign(lights, lights + n);
   return LRT_OK;
 }
 
@@ -559,6 +569,7 @@ int lrt_render_batch(lrt_handle owner, uint32_t n_items, const lrt_handle* scene
     it.A.spp = 1;
     it.A.ray_gen_mode = r->preset.ray_gen_mode;
     it.A.seed = r->seed;
+    it.A.lights = r->light_set();
     it.bfeat = nullptr;
     if (seeded) {
       if ((e = hipSetDevice(s->device)) != hipSuccess) return hip_err(e, "hipSetDevice");

@@ -45,6 +45,10 @@ static int dr_check(Renderer* r, const MultiRendererDRPreset& p) {
   if (p.dr_diff_mode != LRT_DR_DIFF_MODE_DEFAULT) return set_err(LRT_E_UNSUPPORTED, "finite-difference DR mode is not built");
   if (p.reg_function > LRT_DR_REG_FUNCTION_LK_DENOISING) return set_err(LRT_E_INVALID, "bad regularization function");
   if (p.redistancing_enable && p.redistancing_interval == 0) return set_err(LRT_E_INVALID, "redistancing_interval must be > 0");
+  if (p.dr_render_mode == LRT_DR_RENDER_MODE_LAMBERT)
+    for (const LrtLight& l : r->lights)
+      if (l.type == LRT_LIGHT_TYPE_POINT)
+        return set_err(LRT_E_UNSUPPORTED, "DR LAMBERT supports direct and ambient lights (MultiRendererDR.cpp:925-928 asserts)");
   if (p.spp < 1 || p.spp > 16) return set_err(LRT_E_INVALID, "DR spp must be 1..16");
   if (p.border_relax_eps <= 0.f) return set_err(LRT_E_INVALID, "border_relax_eps must be > 0");
   if (r->dr_n_ref == 0) return set_err(LRT_E_INVALID, "no reference images: call lrt_dr_set_reference");
@@ -171,6 +175,7 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
   A.border_spp = p.border_spp;
   A.forced_border = p.debug_forced_border ? 1u : 0u;
   A.relax_eps = p.border_relax_eps;
+  A.lights = r->light_set();
   A.ref = (const float4*)r->d_dr_ref.p + np * view;
   A.out_image = (float4*)r->d_dr_img.p + np * view;
   A.depth_thr = (float*)r->d_dr_thr.p;

@@ -1,6 +1,8 @@
 // lrt_internal.hpp - state behind an lrt_handle.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstring>
 #include <map>
 #include <string>
 #include <vector>
@@ -33,6 +35,25 @@ struct Renderer {
 
   MultiRenderPreset preset{};
   uint32_t width = 0, height = 0, seed = 0;
+  // MultiRenderer::m_lights; the default set of eye_ray_host.cpp:40: a direct
+  // light along normalize(1,1,1) of colour 2/3 and an ambient light of 0.25
+  std::vector<LrtLight> lights = default_lights();
+  static std::vector<LrtLight> default_lights() {
+    const float d = 1.0f / std::sqrt(3.0f), c = 2.0f / 3.0f;
+    return {LrtLight{{d, d, d}, LRT_LIGHT_TYPE_DIRECT, {c, c, c}, 0}, LrtLight{{0, 0, 0}, LRT_LIGHT_TYPE_AMBIENT, {0.25f, 0.25f, 0.25f}, 0}};
+  }
+  lrtd::LightSet light_set() const {
+    lrtd::LightSet L{};
+    L.n = (uint32_t)lights.size();
+    for (uint32_t i = 0; i < L.n; i++) {
+      uint32_t tb = lights[i].type;
+      float tf;
+      std::memcpy(&tf, &tb, 4);
+      L.sp[i] = make_float4(lights[i].space[0], lights[i].space[1], lights[i].space[2], tf);
+      L.col[i] = make_float4(lights[i].color[0], lights[i].color[1], lights[i].color[2], 0.f);
+    }
+    return L;
+  }
   float world_view[16] = {}, proj[16] = {};
   lrtd::Camera cam{};
   bool has_camera = false;

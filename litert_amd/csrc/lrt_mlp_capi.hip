@@ -183,6 +183,7 @@ int lrt_mlp_collect(lrt_handle h, const float* wvs, const float* prs, uint32_t n
   A.spp = 1;
   A.ray_gen_mode = r->preset.ray_gen_mode;
   A.seed = r->seed;
+  A.lights = r->light_set();
   const uint32_t blocks = r->grid_blocks();
   if ((e = r->ensure_spill(blocks)) != hipSuccess) return hip_err(e, "alloc spill");
   e = launch_mlp_collect(r->dev_scene(), r->trace_params(false), (const float4*)r->d_bfeat.p, (const lrtd::Camera*)r->d_cams.p, A,

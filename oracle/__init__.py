@@ -55,6 +55,7 @@ def lib():
             "orc_scene_create": (P, [P, P, u32, P, u64, P, u64, P, u32]),
             "orc_scene_destroy": (None, [P]),
             "orc_scene_set_values_f": (i32, [P, P, u64]),
+            "orc_scene_set_lights": (i32, [P, P, u32]),
             "orc_render": (i32, [P, P, P, P, u32, u32, u32, i32, i32, i32, P, P, P, P, P]),
             "orc_cast_rays": (i32, [P, P, P, u32, i32, i32, i32, P]),
             "orc_brute_nearest": (i32, [P, P, P, u32, i32, i32, P, P]),
@@ -165,6 +166,13 @@ class Scene:
         out = np.zeros(8, np.float64)
         vmin = lib().orc_load_voxel(self.h, node, vx, vy, vz, precision, _p(out))
         return out, vmin
+
+    def set_lights(self, lights):
+        """lights: a list of litert_amd.api.Light (same layout as LrtLight)."""
+        import ctypes
+        arr = (type(lights[0]) * len(lights))(*lights) if lights else None
+        if lib().orc_scene_set_lights(self.h, arr, len(lights)) != 0:
+            raise ValueError("bad lights")
 
     def set_values_f(self, vf):
         vf = np.ascontiguousarray(vf, np.float32)

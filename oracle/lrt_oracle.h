@@ -113,6 +113,8 @@ void orc_adam_step(uint32_t iter, const float* dX, float* X, float* V, float* S,
 
 /* Replaces values_f of the scene (indexed layouts). */
 int orc_scene_set_values_f(orc_scene* s, const float* values_f, uint64_t n);
+/* MultiRenderer::SetLights for the scene's shading and DR LAMBERT. */
+int orc_scene_set_lights(orc_scene* s, const LrtLight* lights, uint32_t n);
 
 /* ---- MLP (learned seeding, no reference counterpart) -------------------- */
 /* FP64 forward of the 24-64-64-64-3 SiLU MLP: weights laid out as

@@ -29,6 +29,12 @@ extern "C" orc_scene* orc_scene_create(const SdfSBSHeader* h, const SdfSBSNode* 
 
 extern "C" void orc_scene_destroy(orc_scene* s) { delete s; }
 
+extern "C" int orc_scene_set_lights(orc_scene* s, const LrtLight* lights, uint32_t n) {
+  if (!s || (n && !lights) || n > LRT_MAX_LIGHTS) return -1;
+  s->s.lights.assign(lights, lights + n);
+  return 0;
+}
+
 extern "C" int orc_scene_set_values_f(orc_scene* s, const float* values_f, uint64_t n) {
   if (!s || n != s->s.values_f.size()) return -1;
   s->s.values_f.assign(values_f, values_f + n);
@@ -68,15 +74,17 @@ static void render_impl(const Scene& S, const Camera& cam, const MultiRenderPres
           if (out_n) { out_n[2 * pi] = (double)hit.coords[2]; out_n[2 * pi + 1] = (double)hit.coords[3]; }
         }
         if (out_rgba) {
-          T shadow = T(1);
+          uint32_t lit = 0xFFFFFFFFu;
           if (needs_shadow(p.render_mode) && hit.primId != 0xFFFFFFFFu) {
-            // shadow ray toward the direct light (eye_ray.cpp:448-468)
-            V3<float> ldf = normalize(V3<float>(1, 1, 1));
-            V3<T> surf = V3<T>(pn.x, pn.y, pn.z) + (hit.t - T(1e-6f)) * V3<T>(dfar.x, dfar.y, dfar.z);
-            Hit<T> sh = tr.RayQuery_NearestHit(V4<T>{surf.x, surf.y, surf.z, pn.w}, V4<T>{T(ldf.x), T(ldf.y), T(ldf.z), dfar.w});
-            shadow = sh.primId == 0xFFFFFFFFu ? T(1) : T(0);
+            // shadow rays toward the lights (eye_ray.cpp:443-468)
+            for (size_t li = 0; li < S.lights.size(); li++) {
+              V3<T> o, ld;
+              if (!shadow_ray<T>(S.lights[li], V3<T>(pn.x, pn.y, pn.z), V3<T>(dfar.x, dfar.y, dfar.z), hit.t, o, ld)) continue;
+              Hit<T> sh = tr.RayQuery_NearestHit(V4<T>{o.x, o.y, o.z, pn.w}, V4<T>{ld.x, ld.y, ld.z, dfar.w});
+              if (sh.primId != 0xFFFFFFFFu) lit &= ~(1u << li);
+            }
           }
-          V4<T> c = shade<T>(p, hit, dfar, shadow);
+          V4<T> c = shade<T>(p, hit, pn, dfar, S.lights, lit);
           acc = V4<T>{acc.x + c.x, acc.y + c.y, acc.z + c.z, acc.w + c.w};
         }
       }

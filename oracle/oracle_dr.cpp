@@ -94,10 +94,6 @@ template <typename T> static void trilinear_weights(V3<T> dp, T w[8]) {
   w[6] = (dp.x) * (dp.y) * (1 - dp.z);         w[7] = (dp.x) * (dp.y) * (dp.z);
 }
 
-// The renderer's default lights (Renderer/eye_ray_host.cpp:40): a direct
-// light along normalize(1,1,1) of colour 2/3 and an ambient light of 0.25.
-// The direction is 1/sqrt(3) rounded to float.
-constexpr float kLightDir = 0.57735026918962576f, kLightCol = 2.0f / 3.0f, kAmbient = 0.25f;
 
 template <typename T> struct DRTracer {
   const Scene& S;
@@ -538,31 +534,37 @@ template <typename T> static T lin_depth(T z) { return (z - T(kZNear)) / (T(kZFa
 
 // CalculateColorWithGrad (MultiRendererDR.cpp:900-980); CalculateColor
 // (:855-898) is its colour. dCdD / dCdN: dColor/dDiffuse and dColor/dNormal.
-template <typename T> static V3<T> calc_color_grad(uint32_t mode, const HitDR<T>& h, M3<T>& dCdD, M3<T>& dCdN) {
+template <typename T>
+static V3<T> calc_color_grad(uint32_t mode, const HitDR<T>& h, const std::vector<LrtLight>& lights, M3<T>& dCdD, M3<T>& dCdN) {
   const V3<T> z(0, 0, 0);
   dCdD = {{V3<T>(1, 0, 0), V3<T>(0, 1, 0), V3<T>(0, 0, 1)}};
   dCdN = {{z, z, z}};
   switch (mode) {
     case LRT_DR_RENDER_MODE_DIFFUSE: return h.color;
-    case LRT_DR_RENDER_MODE_LAMBERT: {
+    case LRT_DR_RENDER_MODE_LAMBERT: {  // direct and ambient lights (point lights assert in the reference)
       V3<T> c(0, 0, 0);
       dCdD = {{z, z, z}};
-      const T ld = T(kLightDir), lcs = T(kLightCol), las = T(kAmbient);
-      const V3<T> L{ld, ld, ld}, lc{lcs, lcs, lcs}, la{las, las, las};
-      T q = mx(T(0), dot(h.normal, L));  // direct light
-      c = c + (lc * h.color) * q;
-      if (q > T(0)) {
-        dCdD.r[0] = dCdD.r[0] + V3<T>(lc.x * q, 0, 0);
-        dCdD.r[1] = dCdD.r[1] + V3<T>(0, lc.y * q, 0);
-        dCdD.r[2] = dCdD.r[2] + V3<T>(0, 0, lc.z * q);
-        dCdN.r[0] = dCdN.r[0] + (lc.x * h.color.x) * L;
-        dCdN.r[1] = dCdN.r[1] + (lc.y * h.color.y) * L;
-        dCdN.r[2] = dCdN.r[2] + (lc.z * h.color.z) * L;
+      for (const LrtLight& l : lights) {
+        const V3<T> lc(T(l.color[0]), T(l.color[1]), T(l.color[2]));
+        if (l.type == LRT_LIGHT_TYPE_DIRECT) {
+          const V3<T> L(T(l.space[0]), T(l.space[1]), T(l.space[2]));
+          T q = mx(T(0), dot(h.normal, L));
+          c = c + (lc * h.color) * q;
+          if (q > T(0)) {
+            dCdD.r[0] = dCdD.r[0] + V3<T>(lc.x * q, 0, 0);
+            dCdD.r[1] = dCdD.r[1] + V3<T>(0, lc.y * q, 0);
+            dCdD.r[2] = dCdD.r[2] + V3<T>(0, 0, lc.z * q);
+            dCdN.r[0] = dCdN.r[0] + (lc.x * h.color.x) * L;
+            dCdN.r[1] = dCdN.r[1] + (lc.y * h.color.y) * L;
+            dCdN.r[2] = dCdN.r[2] + (lc.z * h.color.z) * L;
+          }
+        } else {
+          c = c + lc * h.color;
+          dCdD.r[0] = dCdD.r[0] + V3<T>(lc.x, 0, 0);
+          dCdD.r[1] = dCdD.r[1] + V3<T>(0, lc.y, 0);
+          dCdD.r[2] = dCdD.r[2] + V3<T>(0, 0, lc.z);
+        }
       }
-      c = c + la * h.color;  // ambient light
-      dCdD.r[0] = dCdD.r[0] + V3<T>(la.x, 0, 0);
-      dCdD.r[1] = dCdD.r[1] + V3<T>(0, la.y, 0);
-      dCdD.r[2] = dCdD.r[2] + V3<T>(0, 0, la.z);
       return c;
     }
     case LRT_DR_RENDER_MODE_MASK: dCdD = {{z, z, z}}; return V3<T>(1, 1, 1);
@@ -574,9 +576,9 @@ template <typename T> static V3<T> calc_color_grad(uint32_t mode, const HitDR<T>
     default: return V3<T>(1, 0, 1);
   }
 }
-template <typename T> static V3<T> calc_color(uint32_t mode, const HitDR<T>& h) {
+template <typename T> static V3<T> calc_color(uint32_t mode, const HitDR<T>& h, const std::vector<LrtLight>& lights) {
   M3<T> a, b;
-  return calc_color_grad(mode, h, a, b);
+  return calc_color_grad(mode, h, lights, a, b);
 }
 
 // t_samples_to_threshold (MultiRendererDR.cpp:997-1063)
@@ -613,6 +615,9 @@ static int dr_impl(const Scene& S, const MultiRendererDRPreset& dr, const Camera
   if (mode > LRT_DR_RENDER_MODE_NORMAL) return -3;
   if (dr.dr_loss_function != LRT_DR_LOSS_FUNCTION_MSE && dr.dr_loss_function != LRT_DR_LOSS_FUNCTION_MAE) return -3;
   if (S.type != LRT_TYPE_SDF_SBS_COL) return -4;  // DR needs indexed colour layouts (MultiRendererDR.cpp:283-284)
+  if (mode == LRT_DR_RENDER_MODE_LAMBERT)
+    for (const LrtLight& l : S.lights)
+      if (l.type == LRT_LIGHT_TYPE_POINT) return -3;  // MultiRendererDR.cpp:925-928 asserts
   MultiRenderPreset p{};
   p.sdf_node_intersect = LRT_SDF_NODE_INTERSECT_NEWTON;
   p.ray_gen_mode = LRT_RAY_GEN_MODE_RANDOM;
@@ -667,7 +672,7 @@ static int dr_impl(const Scene& S, const MultiRendererDRPreset& dr, const Camera
         HitDR<T> hit = tr.NearestHitWithGrad(ray_flags, pn, df, &py);
         if (hit.primId == kInvalid) continue;
         M3<T> dCdD, dCdN;
-        V3<T> color = calc_color_grad<T>(mode, hit, dCdD, dCdN);
+        V3<T> color = calc_color_grad<T>(mode, hit, S.lights, dCdD, dCdN);
         res = res + color / T(spp);
         if (!tr.need_normal) {
           // dColor/dDiffuse is I for DIFFUSE (and left at I for LINEAR_DEPTH), 0 for MASK
@@ -777,8 +782,8 @@ static int dr_impl(const Scene& S, const MultiRendererDRPreset& dr, const Camera
             T dout = hit.primId == kInvalid ? T(0) : lin_depth(hit.t);
             delta[0] = delta[1] = delta[2] = din - dout;
           } else {
-            V3<T> fin = calc_color<T>(mode, py.missed_hit);
-            V3<T> fout = hit.primId == kInvalid ? V3<T>(0, 0, 0) : calc_color<T>(mode, hit);
+            V3<T> fin = calc_color<T>(mode, py.missed_hit, S.lights);
+            V3<T> fout = hit.primId == kInvalid ? V3<T>(0, 0, 0) : calc_color<T>(mode, hit, S.lights);
             delta[0] = fin.x - fout.x; delta[1] = fin.y - fout.y; delta[2] = fin.z - fout.z;
           }
           delta[3] = hit.primId == kInvalid ? T(1) : T(0);

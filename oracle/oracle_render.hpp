@@ -35,6 +35,11 @@ struct Scene {
   std::vector<float> values_f;
   std::vector<BVHNodePair> pairs;  // fat BVH2 (cbvh.h:30-33), root pair at index 0
   uint32_t type = LRT_TYPE_SDF_SBS;
+  // MultiRenderer::m_lights; default: direct along normalize(1,1,1) of colour
+  // 2/3, ambient 0.25 (eye_ray_host.cpp:40), the direction as v / |v| in float
+  std::vector<LrtLight> lights = {LrtLight{{1.0f / std::sqrt(3.0f), 1.0f / std::sqrt(3.0f), 1.0f / std::sqrt(3.0f)}, LRT_LIGHT_TYPE_DIRECT,
+                                           {2.0f / 3.0f, 2.0f / 3.0f, 2.0f / 3.0f}, 0},
+                                  LrtLight{{0, 0, 0}, LRT_LIGHT_TYPE_AMBIENT, {0.25f, 0.25f, 0.25f}, 0}};
 };
 
 template <typename T> struct Hit {
@@ -610,10 +615,30 @@ template <typename T> static inline T pow32(T x) {
 
 static inline bool needs_shadow(uint32_t mode) { return mode == LRT_RENDER_MODE_PHONG || mode == LRT_RENDER_MODE_PHONG_NO_TEX; }
 
+template <typename T> static inline V3<T> light_v(const float v[3]) { return V3<T>(T(v[0]), T(v[1]), T(v[2])); }
+
+// Phong's shadow ray toward light l (eye_ray.cpp:443-468): from just before
+// the hit, along a direct light's direction or toward a point light.
+template <typename T>
+static inline bool shadow_ray(const LrtLight& l, V3<T> ray_pos, V3<T> ray_dir, T t, V3<T>& origin, V3<T>& dir) {
+  if (l.type == LRT_LIGHT_TYPE_AMBIENT) return false;
+  origin = ray_pos + (t - T(1e-6f)) * ray_dir;
+  if (l.type == LRT_LIGHT_TYPE_DIRECT) {
+    dir = light_v<T>(l.space);
+  } else {
+    V3<T> d = light_v<T>(l.space) - origin;
+    T len = length(d);
+    dir = V3<T>(d.x / len, d.y / len, d.z / len);
+  }
+  return true;
+}
+
 // eye_ray.cpp:162-489, the render modes that apply to SBS geometry with the
-// renderer's default lights (eye_ray_host.cpp:40). shadow: 1 if the Phong
-// shadow ray toward the direct light (eye_ray.cpp:448-468) hit nothing, else 0.
-template <typename T> static inline V4<T> shade(const MultiRenderPreset& p, const Hit<T>& hit, V4<T> rayDir, T shadow = T(1)) {
+// scene's lights. lit: bit i set if Phong's shadow ray toward light i hit
+// nothing. rayPos: the eye ray's origin.
+template <typename T>
+static inline V4<T> shade(const MultiRenderPreset& p, const Hit<T>& hit, V4<T> rayPos, V4<T> rayDir, const std::vector<LrtLight>& lights,
+                          uint32_t lit) {
   if (hit.primId == 0xFFFFFFFFu) return {0, 0, 0, 1};
   T z = hit.t;
   const T z_near = T(0.1f), z_far = T(10);
@@ -649,39 +674,50 @@ template <typename T> static inline V4<T> shade(const MultiRenderPreset& p, cons
         return {std::round(hit.coords[0]) / T(255), hit.coords[0] - std::floor(hit.coords[0]), std::round(hit.coords[1]) / T(255), 1};
       return {0, 1, 1, 1};  // default textured material samples the default (0,1,1,1) texture
     }
-    case LRT_RENDER_MODE_LAMBERT_NO_TEX: {
-      // lights: direct normalize(1,1,1) colour 2/3, ambient 0.25 (eye_ray_host.cpp:40)
-      V3<float> ld = normalize(V3<float>(1, 1, 1));
-      T q = mx(T(0), dot(norm, V3<T>(T(ld.x), T(ld.y), T(ld.z))));
-      T c = T(2.0f / 3.0f) * q + T(0.25);
-      return {c, c, c, 1};
-    }
     case LRT_RENDER_MODE_BARYCENTRIC: return {hit.coords[0], hit.coords[1], 1 - hit.coords[0] - hit.coords[1], 1};
     case LRT_RENDER_MODE_TEX_COORDS: return {hit.coords[0], hit.coords[1], 0, 1};  // tc = coords[0..1] for SBS
+    case LRT_RENDER_MODE_LAMBERT_NO_TEX:
     case LRT_RENDER_MODE_LAMBERT:
     case LRT_RENDER_MODE_PHONG:
     case LRT_RENDER_MODE_PHONG_NO_TEX: {  // eye_ray.cpp:376-483
+      const uint32_t m = p.render_mode;
       V3<T> col(1, 1, 1);
-      if (p.render_mode != LRT_RENDER_MODE_PHONG_NO_TEX)
+      if (m == LRT_RENDER_MODE_LAMBERT || m == LRT_RENDER_MODE_PHONG)
         col = type == LRT_TYPE_SDF_SBS_COL
                   ? V3<T>(std::round(hit.coords[0]) / T(255), hit.coords[0] - std::floor(hit.coords[0]), std::round(hit.coords[1]) / T(255))
                   : V3<T>(0, 1, 1);  // default material: the (0,1,1,1) texture
-      V3<float> ldf = normalize(V3<float>(1, 1, 1));
-      const V3<T> ld(T(ldf.x), T(ldf.y), T(ldf.z));
-      const T lcs = T(2.0f / 3.0f), las = T(0.25f);
-      const V3<T> lc(lcs, lcs, lcs), la(las, las, las);
+      const bool phong = m == LRT_RENDER_MODE_PHONG || m == LRT_RENDER_MODE_PHONG_NO_TEX;
+      const V3<T> rp(rayPos.x, rayPos.y, rayPos.z), rd(rayDir.x, rayDir.y, rayDir.z);
       V3<T> c(0, 0, 0);
-      // light 0: direct
-      T q = mx(T(0), dot(norm, ld));
-      c = c + (lc * col) * q;
-      if (p.render_mode != LRT_RENDER_MODE_LAMBERT) {
+      for (size_t i = 0; i < lights.size(); i++) {
+        const LrtLight& l = lights[i];
+        const V3<T> lc = light_v<T>(l.color);
+        if (l.type == LRT_LIGHT_TYPE_AMBIENT) {
+          c = c + lc * col;
+          continue;
+        }
+        if (!phong) {
+          if (l.type == LRT_LIGHT_TYPE_DIRECT) {
+            c = c + (lc * col) * mx(T(0), dot(norm, light_v<T>(l.space)));
+          } else {
+            V3<T> surf = rp + hit.t * rd;
+            V3<T> ld = light_v<T>(l.space) - surf;
+            T len = length(ld);
+            ld = V3<T>(ld.x / len, ld.y / len, ld.z / len);
+            V3<T> e = (lc * col) * mx(T(0), dot(norm, ld));
+            c = c + V3<T>(e.x / (len * len), e.y / (len * len), e.z / (len * len));
+          }
+          continue;
+        }
+        V3<T> o, ld;
+        shadow_ray<T>(l, rp, rd, hit.t, o, ld);
+        if (l.type == LRT_LIGHT_TYPE_DIRECT) c = c + (lc * col) * mx(T(0), dot(norm, ld));
         const T Kd = T(0.25f), Ks = T(0.25f);
         V3<T> refl = T(-1) * ld + (T(2) * dot(norm, ld)) * norm;
         T e = Kd * mx(T(0), dot(norm, ld)) + Ks * pow32(mx(T(0), dot(norm, refl)));
+        T shadow = ((lit >> i) & 1u) ? T(1) : T(0);
         c = c + ((shadow * lc) * e) * col;
       }
-      // light 1: ambient
-      c = c + la * col;
       return {c.x, c.y, c.z, 1};
     }
     default: return {1, 0, 1, 1};

@@ -178,6 +178,44 @@ def test_phong_shadow_rays_find_occluders():
     r.close()
 
 
+def _custom_lights():
+    return [L.make_light(L.LIGHT_TYPE_DIRECT, (0.0, 1.0, 0.0), (0.5, 0.4, 0.3)),
+            L.make_light(L.LIGHT_TYPE_POINT, (0.0, 0.9, 0.9), (1.5, 1.2, 1.0)),
+            L.make_light(L.LIGHT_TYPE_DIRECT, (0.6, 0.0, 0.8), (0.3, 0.3, 0.6)),
+            L.make_light(L.LIGHT_TYPE_AMBIENT, (0, 0, 0), (0.1, 0.12, 0.15))]
+
+
+@pytest.mark.parametrize("rmode", [L.MULTI_RENDER_MODE_LAMBERT, L.MULTI_RENDER_MODE_LAMBERT_NO_TEX, L.MULTI_RENDER_MODE_PHONG,
+                                   L.MULTI_RENDER_MODE_PHONG_NO_TEX])
+def test_shading_with_set_lights(rmode):
+    """SetLights (eye_ray.h:158): two direct lights, a point light (Lambert's
+    1/l^2 falloff; Phong's shadow ray toward its position) and an ambient
+    light, on the sphere-in-torus scene; colour vs the FP32 oracle with the
+    same lights, and different from the default lights."""
+    a = L.displaced_torus(R=0.6, r=0.15, amplitude=0.0, lod=32, brick_size=4, brick_pad=0, bytes_per_value=1)
+    b = L.displaced_icosphere(subdiv=4, radius=0.25, amplitude=0.0, lod=32, brick_size=4, brick_pad=0, bytes_per_value=1)
+    nb = b.nodes.copy()
+    nb[:, 2] += a.values.size
+    sbs = L.SdfSBS(a.header, np.concatenate([a.nodes, nb]), np.concatenate([a.values, b.values]))
+    r = L.CreateMultiRenderer(L.DEVICE_GPU)
+    r.SetScene(sbs)
+    pairs, _ = r.GetBVH()
+    sc = oracle.Scene(sbs, pairs)
+    wv, pr = L.orbit_camera(7, 0)
+    p = preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON, render_mode=rmode)
+    default = r.RenderFloat(128, 128, wv, pr, p)
+    r.SetLights(_custom_lights())
+    sc.set_lights(_custom_lights())
+    img = r.RenderFloat(128, 128, wv, pr, p)
+    ref = sc.render(wv, pr, p, 128, 128, precision=32, force_normal=False, rgba=True)["rgba"]
+    diff = np.abs(img - ref).max(axis=-1)
+    assert np.mean(diff <= 1e-5) > 0.9995, np.sort(diff.ravel())[-10:]
+    assert np.abs(img - default).max() > 0.1
+    with pytest.raises(L.LrtError):
+        r.SetLights(_custom_lights() * 3)  # more than LRT_MAX_LIGHTS
+    r.close()
+
+
 def test_render_rgba8_and_timer(r_ico128):
     wv, pr = L.orbit_camera(3, 2)
     p = preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON, render_mode=L.MULTI_RENDER_MODE_LAMBERT_NO_TEX)
