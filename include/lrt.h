@@ -142,7 +142,9 @@ int lrt_dr_set_reference(lrt_handle h, const float* images, const float* masks, 
  * eye_ray_host.cpp:40: direct along (1,1,1) of colour 2/3 plus ambient 0.25),
  * MASK, LINEAR_DEPTH, NORMAL; LAMBERT and NORMAL differentiate the hit normal
  * (smoothed over neighbouring voxels for the ID32F_IRGB32F_IN layout,
- * BVH2DR.cpp:1300-1484). Not built: SVM border sampling and the
+ * BVH2DR.cpp:1300-1484). Both border samplers: RANDOM and SVM
+ * (CastBorderRaySVM, MultiRendererDR.cpp:1651-1931; border_spp < 1156, i.e. at
+ * most 256 SVM samples per pixel, else LRT_E_UNSUPPORTED). Not built: the
  * finite-difference DR mode (LRT_E_UNSUPPORTED). lrt_dr_optimize applies the
  * LK_DENOISING regularization (MultiRendererDR.cpp:1933-2057) and
  * redistancing (:2058-2382, dense brick grids whose first values are the

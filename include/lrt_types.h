@@ -164,6 +164,7 @@ typedef struct CRT_Hit {
 #define LRT_DR_INPUT_TYPE_COLOR        0u
 #define LRT_DR_INPUT_TYPE_LINEAR_DEPTH 1u
 #define LRT_DR_BORDER_SAMPLING_RANDOM 0u
+#define LRT_DR_BORDER_SAMPLING_SVM 1u
 #define LRT_DR_REG_FUNCTION_NONE         0u
 #define LRT_DR_REG_FUNCTION_LK_DENOISING 1u
 #define LRT_DR_RAY_FLAG_NO_DIFF         0u

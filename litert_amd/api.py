@@ -35,6 +35,8 @@ MULTI_RENDER_MODE_TEX_COORDS = 15
 MULTI_RENDER_MODE_DIFFUSE = 16
 MULTI_RENDER_MODE_LAMBERT = 17
 MULTI_RENDER_MODE_PHONG = 18
+DR_BORDER_SAMPLING_RANDOM = 0  # DR_common.h:123-124
+DR_BORDER_SAMPLING_SVM = 1
 NORMAL_MODE_GEOMETRY, NORMAL_MODE_VERTEX, NORMAL_MODE_SDF_SMOOTHED = 0, 1, 2
 RAY_GEN_MODE_REGULAR, RAY_GEN_MODE_RANDOM = 0, 1
 REPRESENTATION_MODE_SURFACE, REPRESENTATION_MODE_VOLUME = 0, 1

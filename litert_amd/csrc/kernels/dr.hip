@@ -210,7 +210,43 @@ __global__ void k_dr_border_find(DRArgs A, uint32_t* list, uint32_t* count) {
   if (border || A.forced_border) list[atomicAdd(count, 1u)] = i;
 }
 
-// Phase III (MultiRendererDR.cpp:1319-1499): one lane per (border pixel, sample)
+// CalculateBorderRayDerivatives (MultiRendererDR.cpp:1319-1375) of one border
+// ray that found a relaxation point: pdf * (1/eps) * <dL/dC, C_in - C_out> *
+// dSDF/dtheta added to the 8 values of the relaxation voxel.
+__device__ __forceinline__ void border_contribution(const DRArgs& A, const DRPayload& py, const DRHit& hit, v3 pos, v3 dir, float pdf) {
+  float delta[4];
+  if (A.input_type == LRT_DR_INPUT_TYPE_LINEAR_DEPTH) {
+    float din = lin_depth(py.missed_hit.t);
+    float dout = hit.primId == 0xFFFFFFFFu ? 0.0f : lin_depth(hit.t);
+    delta[0] = delta[1] = delta[2] = din - dout;
+  } else {
+    v3 fin = dr_color(A.mode, py.missed_hit, A.lights);
+    v3 fout = hit.primId == 0xFFFFFFFFu ? v3(0, 0, 0) : dr_color(A.mode, hit, A.lights);
+    delta[0] = fin.x - fout.x; delta[1] = fin.y - fout.y; delta[2] = fin.z - fout.z;
+  }
+  delta[3] = hit.primId == 0xFFFFFFFFu ? 1.0f : 0.0f;
+  for (int j = 0; j < 8; j++) {
+    if (py.missed_idx[j] == 0xFFFFFFFFu) continue;
+    float bd = mx(0.0f, py.missed_hit.sdf);
+    v3 ystar = pos + py.missed_hit.t * dir;
+    v3 xstar = ystar - bd * py.missed_hit.normal;
+    float ps[4];
+    mul4(A.proj_view, xstar.x, xstar.y, xstar.z, 1.0f, ps);
+    float sx = 0.5f * (ps[0] / ps[3] + 1.0f), sy = 1.0f - 0.5f * (ps[1] / ps[3] + 1.0f);
+    // the reference converts to unsigned without a range check; clamped here
+    float fx = floorf(sx * (float)A.W), fy = floorf(sy * (float)A.H);
+    uint32_t xp = (uint32_t)fminf(fmaxf(fx, 0.0f), (float)(A.W - 1)), yp = (uint32_t)fminf(fmaxf(fy, 0.0f), (float)(A.H - 1));
+    size_t q = (size_t)yp * A.W + xp;
+    float4 c = A.out_image[q], r = A.ref[q];
+    float dd = loss_grad1(A.loss_fn, c.x, r.x) * delta[0] + loss_grad1(A.loss_fn, c.y, r.y) * delta[1] +
+               loss_grad1(A.loss_fn, c.z, r.z) * delta[2] + loss_grad1(A.loss_fn, c.w, r.w) * delta[3];
+    float diff = pdf * (1.0f / A.relax_eps) * dd * py.missed_dsdf[j];
+    atomicAdd(&A.grad[py.missed_idx[j]], diff);
+  }
+}
+
+// Phase III (MultiRendererDR.cpp:1377-1499), random border sampling: one lane
+// per (border pixel, sample)
 template <int SPEC>
 __global__ __launch_bounds__(256) void k_dr_border_rays(DRArgs A, const uint32_t* list, const uint32_t* count, uint32_t* spill,
                                                         uint32_t* hits) {
@@ -225,6 +261,7 @@ __global__ __launch_bounds__(256) void k_dr_border_rays(DRArgs A, const uint32_t
   const DRParams Qb = {LRT_DR_RAY_FLAG_BORDER, A.Q.dtdv_fd};
   uint32_t local_hits = 0;
   DRPayload py;
+  py.scatter = 0;
   for (uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x; g < n; g += (uint64_t)gridDim.x * 256) {
     const uint32_t item = (uint32_t)(g / A.border_spp), s = (uint32_t)(g % A.border_spp);
     uint32_t x, y;
@@ -235,36 +272,243 @@ __global__ __launch_bounds__(256) void k_dr_border_rays(DRArgs A, const uint32_t
     DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, A.bminmax, Qb, A.relax_eps, pos, dir, 1e9f, &py, st);
     if (!(py.missed_hit.sdf < A.relax_eps)) continue;
     local_hits++;
-    float delta[4];
-    if (A.input_type == LRT_DR_INPUT_TYPE_LINEAR_DEPTH) {
-      float din = lin_depth(py.missed_hit.t);
-      float dout = hit.primId == 0xFFFFFFFFu ? 0.0f : lin_depth(hit.t);
-      delta[0] = delta[1] = delta[2] = din - dout;
+    border_contribution(A, py, hit, pos, dir, 1.0f / (float)A.border_spp);
+  }
+  if (local_hits) atomicAdd(hits, local_hits);
+}
+
+// ---- SVM border sampling (MultiRendererDR.cpp:1501-1932) -------------------
+
+// SVM_fit (:1518-1546): a perceptron with weight decay on the labelled
+// samples X (already multiplied by their label), etha 0.01, alpha 0.1, at most
+// 200 epochs; returns W and the last epoch's error rate.
+__device__ float4 svm_fit(const float* xs, const float* ys, const float* lab, uint32_t N) {
+  const float etha = 0.01f, alpha = 0.1f;
+  const int epochs = 200;
+  v3 W(-1, 1, 0);
+  const float weightMult = 1.0f - etha * alpha / epochs;
+  uint32_t errors = N;
+  for (int i = 0; i < epochs && errors > 0; i++) {
+    errors = 0;
+    for (uint32_t j = 0; j < N; j++) {
+      const v3 X = v3(xs[j] * lab[j], ys[j] * lab[j], 1.0f * lab[j]) * etha;
+      const float margin = dot(W, X);
+      W = W * weightMult;
+      if (margin < 0) {
+        W = W + X;
+        errors++;
+      }
+    }
+  }
+  return make_float4(W.x, W.y, W.z, float(errors) / float(N));
+}
+
+__device__ __forceinline__ uint32_t out_code(float x, float y) {  // ComputeOutCode, :1550-1571, window [0,1]^2
+  uint32_t code = 0;
+  if (x < 0) code |= 1u;
+  else if (x > 1) code |= 2u;
+  if (y < 0) code |= 4u;
+  else if (y > 1) code |= 8u;
+  return code;
+}
+
+// CohenSutherlandLineClip, :1576-1645
+__device__ bool clip_line(float& x0, float& y0, float& x1, float& y1) {
+  uint32_t c0 = out_code(x0, y0), c1 = out_code(x1, y1);
+  for (int guard = 0; guard < 16; guard++) {  // each pass moves one end onto the window: at most 4 needed
+    if (!(c0 | c1)) return true;
+    if (c0 & c1) return false;
+    float x = -1, y = -1;
+    const uint32_t co = c1 > c0 ? c1 : c0;
+    if (co & 8u) {
+      x = x0 + (x1 - x0) * (1 - y0) / (y1 - y0);
+      y = 1;
+    } else if (co & 4u) {
+      x = x0 + (x1 - x0) * (0 - y0) / (y1 - y0);
+      y = 0;
+    } else if (co & 2u) {
+      y = y0 + (y1 - y0) * (1 - x0) / (x1 - x0);
+      x = 1;
+    } else if (co & 1u) {
+      y = y0 + (y1 - y0) * (0 - x0) / (x1 - x0);
+      x = 0;
+    }
+    if (co == c0) {
+      x0 = x; y0 = y; c0 = out_code(x0, y0);
     } else {
-      v3 fin = dr_color(A.mode, py.missed_hit, A.lights);
-      v3 fout = hit.primId == 0xFFFFFFFFu ? v3(0, 0, 0) : dr_color(A.mode, hit, A.lights);
-      delta[0] = fin.x - fout.x; delta[1] = fin.y - fout.y; delta[2] = fin.z - fout.z;
+      x1 = x; y1 = y; c1 = out_code(x1, y1);
     }
-    delta[3] = hit.primId == 0xFFFFFFFFu ? 1.0f : 0.0f;
-    const float pdf = 1.0f / (float)A.border_spp;
-    for (int j = 0; j < 8; j++) {
-      if (py.missed_idx[j] == 0xFFFFFFFFu) continue;
-      float bd = mx(0.0f, py.missed_hit.sdf);
-      v3 ystar = pos + py.missed_hit.t * dir;
-      v3 xstar = ystar - bd * py.missed_hit.normal;
-      float ps[4];
-      mul4(A.proj_view, xstar.x, xstar.y, xstar.z, 1.0f, ps);
-      float sx = 0.5f * (ps[0] / ps[3] + 1.0f), sy = 1.0f - 0.5f * (ps[1] / ps[3] + 1.0f);
-      // the reference converts to unsigned without a range check; clamped here
-      float fx = floorf(sx * (float)A.W), fy = floorf(sy * (float)A.H);
-      uint32_t xp = (uint32_t)fminf(fmaxf(fx, 0.0f), (float)(A.W - 1)), yp = (uint32_t)fminf(fmaxf(fy, 0.0f), (float)(A.H - 1));
-      size_t q = (size_t)yp * A.W + xp;
-      float4 c = A.out_image[q], r = A.ref[q];
-      float dd = loss_grad1(A.loss_fn, c.x, r.x) * delta[0] + loss_grad1(A.loss_fn, c.y, r.y) * delta[1] +
-                 loss_grad1(A.loss_fn, c.z, r.z) * delta[2] + loss_grad1(A.loss_fn, c.w, r.w) * delta[3];
-      float diff = pdf * (1.0f / A.relax_eps) * dd * py.missed_dsdf[j];
-      atomicAdd(&A.grad[py.missed_idx[j]], diff);
+  }
+  return false;
+}
+
+__device__ __forceinline__ float2 world_to_screen(const DRArgs& A, v3 p) {  // TransformWorldToScreenSpace, :1296-1301
+  float ps[4];
+  mul4(A.proj_view, p.x, p.y, p.z, 1.0f, ps);
+  return make_float2(0.5f * (ps[0] / ps[3] + 1.0f), 1.0f - 0.5f * (ps[1] / ps[3] + 1.0f));
+}
+
+constexpr uint32_t kSvmMaxSamples = 256;  // svm_spp = 0.25 * border_spp, rounded down to a square
+
+// Orders one wave's LDS stores before its later LDS loads (other lanes).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// CastBorderRaySVM (:1651-1931): one wavefront per border pixel. The lanes
+// trace the SVM samples (a jittered sqrt(n) x sqrt(n) grid over the pixel) in
+// parallel; lane 0 labels them by the depth threshold, fits the separating
+// line, clips it to the pixel and builds the sampling stripe; then the lanes
+// trace the stripe's border rays. The reference draws the stripe positions
+// with rand(); here they come from rand2 of the sample index + 0x10000.
+template <int SPEC>
+__global__ __launch_bounds__(256) void k_dr_border_svm(DRArgs A, const uint32_t* list, const uint32_t* count, uint32_t* spill,
+                                                       uint32_t* hits) {
+  __shared__ uint32_t lds_stack[kStackLds * 256];
+  __shared__ float s_x[4][kSvmMaxSamples], s_y[4][kSvmMaxSamples], s_t[4][kSvmMaxSamples], s_l[4][kSvmMaxSamples];
+  __shared__ float s_geo[4][12];  // p0, r, n, radius, pdf, force, n_rays, skip
+  // Each wave owns row wv of these arrays, so waves run through their pixels
+  // independently and synchronise only with themselves.
+  Stack st;
+  st.lds = (lds_u32*)(lds_stack + threadIdx.x);
+  st.lds_stride = 256;
+  st.lds_entries = kStackLds;
+  st.spill = spill + blockIdx.x * 256 + threadIdx.x;
+  st.spill_stride = gridDim.x * 256;
+  const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const DRParams Q0 = {LRT_DR_RAY_FLAG_NO_DIFF, 0u}, Qb = {LRT_DR_RAY_FLAG_BORDER, A.Q.dtdv_fd};
+  const uint32_t n_items = *count;
+  const uint32_t sq = (uint32_t)sqrt((double)max(1u, (uint32_t)(0.25f * (float)A.border_spp)));
+  const uint32_t svm_spp = sq * sq;
+  const float i_sq = 1.0f / (float)sq;
+  uint32_t local_hits = 0;
+  DRPayload py;
+  py.scatter = 0;
+  for (uint32_t item = blockIdx.x * 4 + wv; item < n_items; item += gridDim.x * 4) {  // uniform within the wave
+    uint32_t x, y;
+    tid_to_xy(list[item], A.packed_w, x, y);
+    // step 1: SVM samples
+    for (uint32_t s = lane; s < svm_spp; s += 64) {
+      const float2 rnd = rand2(x, y, s, A.seed);
+      const float dx = i_sq * ((float)(s / sq) + rnd.x), dy = i_sq * ((float)(s % sq) + rnd.y);
+      v3 pos, dir;
+      eye_ray(A.cam, x, y, dx, dy, A.W, A.H, pos, dir);
+      const DRHit hit = dr_nearest_hit<SPEC == kDrDepthBorder ? kDrGeneric : SPEC>(A.S, A.bvals, A.bminmax, Q0, A.relax_eps, pos, dir,
+                                                                                  1e9f, &py, st);
+      s_x[wv][s] = dx;
+      s_y[wv][s] = dy;
+      s_t[wv][s] = hit.t;
     }
+    wave_sync();
+    if (lane == 0) {
+      // step 2: threshold, labels, fit (:1716-1745)
+      const float thr = samples_to_threshold(s_t[wv], svm_spp, 0.0f, 0u);
+      uint32_t type0 = 0, closest = 0;
+      float closest_t = 1e6f;
+      for (uint32_t s = 0; s < svm_spp; s++) {
+        const float t = s_t[wv][s];
+        if (t < closest_t) { closest_t = t; closest = s; }
+        s_l[wv][s] = t <= thr ? -1.0f : 1.0f;
+        type0 += t <= thr ? 1u : 0u;
+      }
+      float skip = 0.0f, force = 0.0f, stripe_area = 1.0f, sample_radius = 0.0f;
+      float2 p0 = make_float2(0, 0), p1 = make_float2(0, 0), r = make_float2(0, 0), n = make_float2(0, 0);
+      if (type0 == svm_spp || type0 == 0) {
+        skip = 1.0f;
+      } else {
+        const float4 W = svm_fit(s_x[wv], s_y[wv], s_l[wv], svm_spp);
+        const float error_rate = W.w;
+        const float f0 = (float)type0 / (float)svm_spp;
+        const float rare = fminf(f0, 1.0f - f0);
+        if (error_rate > 0.05f || 2 * error_rate > rare) {
+          force = 1.0f;
+          stripe_area = 1.0f;
+        } else {
+          if (fabsf(W.y) > 1e-6f && fabsf(W.x) > 1e-6f) {
+            p0 = make_float2(0, -W.z / W.y);
+            p1 = make_float2(1, (-W.z - W.x) / W.y);
+            const float ry = p1.y - p0.y;
+            if (p0.y < -1e-6f || p0.y > 1 + 1e-6f) {
+              const float k1 = -p0.y / ry, k2 = (1 - p0.y) / ry, k = (fabsf(k1) < fabsf(k2)) ? k1 : k2;
+              p0 = make_float2(p0.x + k * 1.0f, p0.y + k * ry);
+            }
+            if (p1.y < -1e-6f || p1.y > 1 + 1e-6f) {
+              const float k1 = -p1.y / ry, k2 = (1 - p1.y) / ry, k = (fabsf(k1) < fabsf(k2)) ? k1 : k2;
+              p1 = make_float2(p1.x + k * 1.0f, p1.y + k * ry);
+            }
+          } else if (fabsf(W.x) > 1e-6f) {
+            p0 = make_float2(-W.z / W.x, 0);
+            p1 = make_float2(-W.z / W.x, 1);
+          } else {
+            force = 1.0f;  // the reference leaves p0, p1 unset here; they are (0,0)
+          }
+          if (!clip_line(p0.x, p0.y, p1.x, p1.y)) {
+            skip = 1.0f;
+          } else {
+            // border thickness from the closest hit (:1781-1788)
+            float2 q0, q1;
+            {
+              const float rnd_x = rand2(x, y, closest, A.seed).x, rnd_y = rand2(x, y, closest, A.seed).y;
+              v3 pos, dir;
+              eye_ray(A.cam, x, y, i_sq * ((float)(closest / sq) + rnd_x), i_sq * ((float)(closest % sq) + rnd_y), A.W, A.H, pos, dir);
+              const v3 ch = closest_t < 1e6f ? pos + dir * closest_t : v3(1e6f, 1e6f, 1e6f);
+              q0 = world_to_screen(A, ch);
+              q1 = world_to_screen(A, ch + v3(A.up[0], A.up[1], A.up[2]) * A.relax_eps);
+            }
+            const float ddx = q1.x - q0.x, ddy = q1.y - q0.y;
+            const float thick = sqrtf(ddx * ddx + ddy * ddy) * (float)max(A.W, A.H);
+            sample_radius = fmaxf(0.15f, thick) + 2.0f * error_rate;
+            r = make_float2(p1.x - p0.x, p1.y - p0.y);
+            float lr = sqrtf(r.x * r.x + r.y * r.y);
+            const float2 rn = make_float2(r.x / lr, r.y / lr);
+            p0 = make_float2(p0.x - sample_radius * rn.x, p0.y - sample_radius * rn.y);
+            p1 = make_float2(p1.x + sample_radius * rn.x, p1.y + sample_radius * rn.y);
+            r = make_float2(p1.x - p0.x, p1.y - p0.y);
+            lr = sqrtf(r.x * r.x + r.y * r.y);
+            const float ln = sqrtf(r.y * r.y + r.x * r.x);
+            n = make_float2(-r.y / ln, r.x / ln);
+            stripe_area = 2 * sample_radius * lr;
+            if (!force && stripe_area < 0.005f) skip = 1.0f;
+          }
+        }
+      }
+      const float area_mult = fminf(1.0f, 0.05f + 2.0f * stripe_area);
+      const uint32_t nb = max(1u, (uint32_t)(area_mult * (float)(A.border_spp - svm_spp)));
+      float* g = s_geo[wv];
+      g[0] = p0.x; g[1] = p0.y; g[2] = r.x; g[3] = r.y; g[4] = n.x; g[5] = n.y;
+      g[6] = sample_radius;
+      g[7] = force != 0.0f ? 1.0f / (float)nb : stripe_area / (float)nb;
+      g[8] = force;
+      g[9] = __uint_as_float(nb);
+      g[10] = skip;
+    }
+    wave_sync();
+    const float* g = s_geo[wv];
+    if (g[10] == 0.0f) {
+      // step 3: border rays in the stripe (or the whole pixel)
+      const uint32_t nb = __float_as_uint(g[9]);
+      for (uint32_t k = lane; k < nb; k += 64) {
+        const uint32_t sid = svm_spp + k;
+        float2 d;
+        if (g[8] != 0.0f) {
+          d = rand2(x, y, sid, A.seed);
+        } else {
+          const float2 u = rand2(x, y, sid + 0x10000u, A.seed);
+          const float u1 = u.x, u2 = 2.0f * u.y - 1.0f;
+          d = make_float2(g[0] + u1 * g[2] + u2 * g[6] * g[4], g[1] + u1 * g[3] + u2 * g[6] * g[5]);
+        }
+        if (!(d.x > 0 && d.x < 1 && d.y > 0 && d.y < 1)) continue;
+        v3 pos, dir;
+        eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
+        const DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, A.bminmax, Qb, A.relax_eps, pos, dir, 1e9f, &py, st);
+        if (!(py.missed_hit.sdf < A.relax_eps)) continue;
+        local_hits++;
+        border_contribution(A, py, hit, pos, dir, g[7]);
+      }
+    }
+    wave_sync();  // the next pixel reuses this wave's LDS rows
   }
   if (local_hits) atomicAdd(hits, local_hits);
 }
@@ -548,6 +792,18 @@ hipError_t launch_dr_border_find(const lrtd::DRArgs& A, uint32_t* d_list, uint32
 hipError_t launch_dr_border_rays(const lrtd::DRArgs& A, const uint32_t* d_list, const uint32_t* d_count, uint32_t* d_spill,
                                  uint32_t* d_hits, uint32_t blocks, hipStream_t s, const uint4* cull_pairs) {
   // linear-depth input reads only depths and hit flags of border rays
+  if (A.border_sampling == LRT_DR_BORDER_SAMPLING_SVM) {
+    // one wave per border pixel, four per block; the spill rows of `blocks` blocks bound the grid
+    if (dr_border_is_depth(A)) {
+      lrtd::DRArgs Ab = A;
+      if (cull_pairs) Ab.S.pairs = cull_pairs;
+      hipLaunchKernelGGL(lrtd::k_dr_border_svm<lrtd::kDrDepthBorder>, dim3(blocks), dim3(256), 0, s, Ab, d_list, d_count, d_spill, d_hits);
+    } else if (dr_normal_mode(A))
+      hipLaunchKernelGGL(lrtd::k_dr_border_svm<lrtd::kDrNormal>, dim3(blocks), dim3(256), 0, s, A, d_list, d_count, d_spill, d_hits);
+    else
+      hipLaunchKernelGGL(lrtd::k_dr_border_svm<lrtd::kDrGeneric>, dim3(blocks), dim3(256), 0, s, A, d_list, d_count, d_spill, d_hits);
+    return hipGetLastError();
+  }
   if (dr_border_is_depth(A)) {
     lrtd::DRArgs Ab = A;
     if (cull_pairs) Ab.S.pairs = cull_pairs;

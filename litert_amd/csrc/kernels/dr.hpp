@@ -18,6 +18,8 @@ struct DRArgs {
   uint32_t W, H, packed_w, spp, seed;
   uint32_t mode, input_type, loss_fn, border_spp, forced_border;
   float relax_eps;
+  uint32_t border_sampling;  // LRT_DR_BORDER_SAMPLING_RANDOM / _SVM
+  float up[3];               // row 1 of worldView (CastBorderRaySVM, MultiRendererDR.cpp:1784)
   const float4* ref;   // reference image, alpha = mask
   float4* out_image;   // rendered colour + coverage
   float* depth_thr;    // per-pixel depth-discontinuity threshold

@@ -41,7 +41,9 @@ static int dr_check(Renderer* r, const MultiRendererDRPreset& p) {
   if (r->header.brick_pad != 0) return set_err(LRT_E_UNSUPPORTED, "indexed DR layouts have brick_pad 0");
   if (p.dr_render_mode > LRT_DR_RENDER_MODE_NORMAL) return set_err(LRT_E_INVALID, "bad DR render mode");
   if (p.dr_loss_function > LRT_DR_LOSS_FUNCTION_MAE) return set_err(LRT_E_INVALID, "bad loss function");
-  if (p.dr_border_sampling != LRT_DR_BORDER_SAMPLING_RANDOM) return set_err(LRT_E_UNSUPPORTED, "SVM border sampling is not built");
+  if (p.dr_border_sampling > LRT_DR_BORDER_SAMPLING_SVM) return set_err(LRT_E_INVALID, "bad border sampling");
+  if (p.dr_border_sampling == LRT_DR_BORDER_SAMPLING_SVM && p.border_spp >= 1156)
+    return set_err(LRT_E_UNSUPPORTED, "SVM border sampling takes border_spp < 1156 (at most 256 SVM samples per pixel)");
   if (p.dr_diff_mode != LRT_DR_DIFF_MODE_DEFAULT) return set_err(LRT_E_UNSUPPORTED, "finite-difference DR mode is not built");
   if (p.reg_function > LRT_DR_REG_FUNCTION_LK_DENOISING) return set_err(LRT_E_INVALID, "bad regularization function");
   if (p.redistancing_enable && p.redistancing_interval == 0) return set_err(LRT_E_INVALID, "redistancing_interval must be > 0");
@@ -175,6 +177,8 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
   A.border_spp = p.border_spp;
   A.forced_border = p.debug_forced_border ? 1u : 0u;
   A.relax_eps = p.border_relax_eps;
+  A.border_sampling = p.dr_border_sampling;
+  for (int k = 0; k < 3; k++) A.up[k] = wv[4 * k + 1];  // worldView row 1, column-major storage
   A.lights = r->light_set();
   A.ref = (const float4*)r->d_dr_ref.p + np * view;
   A.out_image = (float4*)r->d_dr_img.p + np * view;

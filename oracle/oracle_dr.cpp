@@ -607,6 +607,175 @@ static float samples_to_threshold(const float* s, unsigned count, float dv, unsi
   return 0.0f;
 }
 
+// ---- SVM border sampling (MultiRendererDR.cpp:1501-1932) ----------------
+// The fit, the clip and the stripe are computed in float, as the reference
+// does, for both precisions of the tracer; only the traced distances follow T.
+
+// SVM_fit (:1518-1546): perceptron with weight decay over X_mod = (x, y, 1) *
+// label, etha 0.01, alpha 0.1, 200 epochs with early exit on zero errors.
+static void svm_fit(const std::vector<float>& xs, const std::vector<float>& ys, const std::vector<float>& lab, float W[4]) {
+  const float etha = 0.01f, alpha = 0.1f;
+  const int epochs = 200;
+  const unsigned N = (unsigned)xs.size();
+  float w0 = -1.f, w1 = 1.f, w2 = 0.f;
+  const float mult = 1.0f - etha * alpha / epochs;
+  unsigned errors = N;
+  for (int i = 0; i < epochs && errors > 0; i++) {
+    errors = 0;
+    for (unsigned j = 0; j < N; j++) {
+      const float X0 = xs[j] * lab[j] * etha, X1 = ys[j] * lab[j] * etha, X2 = 1.0f * lab[j] * etha;
+      const float margin = w0 * X0 + w1 * X1 + w2 * X2;
+      w0 = w0 * mult; w1 = w1 * mult; w2 = w2 * mult;
+      if (margin < 0) {
+        w0 = w0 + X0; w1 = w1 + X1; w2 = w2 + X2;
+        errors++;
+      }
+    }
+  }
+  W[0] = w0; W[1] = w1; W[2] = w2; W[3] = float(errors) / float(N);
+}
+
+static unsigned outcode(float x, float y) {  // ComputeOutCode (:1550-1571) on [0,1]^2
+  unsigned c = 0;
+  if (x < 0) c |= 1u;
+  else if (x > 1) c |= 2u;
+  if (y < 0) c |= 4u;
+  else if (y > 1) c |= 8u;
+  return c;
+}
+
+// Cohen-Sutherland line clipping (:1576-1645)
+static bool clip_line(float& x0, float& y0, float& x1, float& y1) {
+  unsigned c0 = outcode(x0, y0), c1 = outcode(x1, y1);
+  for (int guard = 0; guard < 16; guard++) {
+    if (!(c0 | c1)) return true;
+    if (c0 & c1) return false;
+    float x = -1, y = -1;
+    const unsigned co = c1 > c0 ? c1 : c0;
+    if (co & 8u) { x = x0 + (x1 - x0) * (1 - y0) / (y1 - y0); y = 1; }
+    else if (co & 4u) { x = x0 + (x1 - x0) * (0 - y0) / (y1 - y0); y = 0; }
+    else if (co & 2u) { y = y0 + (y1 - y0) * (1 - x0) / (x1 - x0); x = 1; }
+    else if (co & 1u) { y = y0 + (y1 - y0) * (0 - x0) / (x1 - x0); x = 0; }
+    if (co == c0) { x0 = x; y0 = y; c0 = outcode(x0, y0); }
+    else { x1 = x; y1 = y; c1 = outcode(x1, y1); }
+  }
+  return false;
+}
+
+static void to_screen(const float PV[16], float px, float py, float pz, float& sx, float& sy) {  // :1296-1301
+  float ps[4];
+  for (int r = 0; r < 4; r++) ps[r] = PV[0 * 4 + r] * px + PV[1 * 4 + r] * py + PV[2 * 4 + r] * pz + PV[3 * 4 + r] * 1.0f;
+  sx = 0.5f * (ps[0] / ps[3] + 1.0f);
+  sy = 1.0f - 0.5f * (ps[1] / ps[3] + 1.0f);
+}
+
+// CastBorderRaySVM (:1651-1931) for one border pixel; returns the number of
+// border rays that found a relaxation point. The reference draws the stripe
+// positions with rand(); here they are rand2 of (sample index + 0x10000).
+template <typename T, typename Tracer, typename Contribute, typename PayloadT>
+static uint32_t svm_border_pixel(Tracer& tr, const Camera& cam, uint32_t x, uint32_t y, uint32_t W, uint32_t H, uint32_t seed, uint32_t bspp,
+                                 float eps, const float* wv, const float PV[16], Contribute& contribute, PayloadT& py) {
+  unsigned svm_spp = std::max(1u, (unsigned)(0.25f * (float)bspp));
+  const uint32_t sq = (uint32_t)std::sqrt((double)svm_spp);
+  const float i_sq = 1.0f / (float)sq;
+  svm_spp = sq * sq;
+  std::vector<float> xs(svm_spp), ys(svm_spp), ts(svm_spp), lab(svm_spp);
+  float chx = 1e6f, chy = 1e6f, chz = 1e6f, closest_t = 1e6f;
+  for (unsigned s = 0; s < svm_spp; s++) {  // step 1
+    V2<float> rnd = rand2(x, y, s, seed);
+    V2<float> d{i_sq * ((float)(s / sq) + rnd.x), i_sq * ((float)(s % sq) + rnd.y)};
+    V4<T> pn, df;
+    InitEyeRay<T>(cam, x, y, d, W, H, &pn, &df);
+    HitDR<T> hit = tr.NearestHitWithGrad(LRT_DR_RAY_FLAG_NO_DIFF, pn, df, &py);
+    xs[s] = d.x;
+    ys[s] = d.y;
+    ts[s] = (float)hit.t;
+    if (ts[s] < closest_t) {
+      closest_t = ts[s];
+      chx = (float)pn.x + (float)df.x * closest_t;
+      chy = (float)pn.y + (float)df.y * closest_t;
+      chz = (float)pn.z + (float)df.z * closest_t;
+    }
+  }
+  const float thr = samples_to_threshold(ts.data(), svm_spp, 0.0f, 0u);
+  unsigned type0 = 0;
+  for (unsigned s = 0; s < svm_spp; s++) {
+    lab[s] = ts[s] <= thr ? -1.0f : 1.0f;
+    type0 += ts[s] <= thr ? 1u : 0u;
+  }
+  if (type0 == svm_spp || type0 == 0) return 0;
+  float Wt[4];  // step 2
+  svm_fit(xs, ys, lab, Wt);
+  const float error_rate = Wt[3];
+  const float f0 = (float)type0 / (float)svm_spp;
+  const float rare = std::min(f0, 1.0f - f0);
+  float p0x = 0, p0y = 0, p1x = 0, p1y = 0, rx = 0, ry = 0, nx = 0, ny = 0, radius = 0, area = 1.0f;
+  bool force = false;
+  if (error_rate > 0.05f || 2 * error_rate > rare) {
+    force = true;
+  } else {
+    if (std::abs(Wt[1]) > 1e-6f && std::abs(Wt[0]) > 1e-6f) {
+      p0x = 0; p0y = -Wt[2] / Wt[1];
+      p1x = 1; p1y = (-Wt[2] - Wt[0]) / Wt[1];
+      const float dy = p1y - p0y;
+      if (p0y < -1e-6f || p0y > 1 + 1e-6f) {
+        const float k1 = -p0y / dy, k2 = (1 - p0y) / dy, k = std::abs(k1) < std::abs(k2) ? k1 : k2;
+        p0x = p0x + k * 1.0f; p0y = p0y + k * dy;
+      }
+      if (p1y < -1e-6f || p1y > 1 + 1e-6f) {
+        const float k1 = -p1y / dy, k2 = (1 - p1y) / dy, k = std::abs(k1) < std::abs(k2) ? k1 : k2;
+        p1x = p1x + k * 1.0f; p1y = p1y + k * dy;
+      }
+    } else if (std::abs(Wt[0]) > 1e-6f) {
+      p0x = -Wt[2] / Wt[0]; p0y = 0;
+      p1x = -Wt[2] / Wt[0]; p1y = 1;
+    } else {
+      force = true;  // the reference leaves p0, p1 unset here; (0,0) is used
+    }
+    if (!clip_line(p0x, p0y, p1x, p1y)) return 0;
+    float q0x, q0y, q1x, q1y;  // border thickness at the closest hit, up = worldView row 1
+    to_screen(PV, chx, chy, chz, q0x, q0y);
+    to_screen(PV, chx + wv[1] * eps, chy + wv[5] * eps, chz + wv[9] * eps, q1x, q1y);
+    const float ddx = q1x - q0x, ddy = q1y - q0y;
+    const float thick = std::sqrt(ddx * ddx + ddy * ddy) * (float)std::max(W, H);
+    radius = std::max(0.15f, thick) + 2.0f * error_rate;
+    rx = p1x - p0x; ry = p1y - p0y;
+    float lr = std::sqrt(rx * rx + ry * ry);
+    const float ux = rx / lr, uy = ry / lr;
+    p0x = p0x - radius * ux; p0y = p0y - radius * uy;
+    p1x = p1x + radius * ux; p1y = p1y + radius * uy;
+    rx = p1x - p0x; ry = p1y - p0y;
+    lr = std::sqrt(rx * rx + ry * ry);
+    const float ln = std::sqrt(ry * ry + rx * rx);
+    nx = -ry / ln; ny = rx / ln;
+    area = 2 * radius * lr;
+    if (!force && area < 0.005f) return 0;
+  }
+  const float area_mult = std::min(1.0f, 0.05f + 2.0f * area);
+  const unsigned nb = std::max(1u, (unsigned)(area_mult * (float)(bspp - svm_spp)));
+  const float pdf = force ? 1.0f / (float)nb : area / (float)nb;
+  uint32_t hits = 0;
+  for (unsigned k = 0; k < nb; k++) {  // step 3
+    const unsigned sid = svm_spp + k;
+    V2<float> d;
+    if (force) {
+      d = rand2(x, y, sid, seed);
+    } else {
+      V2<float> u = rand2(x, y, sid + 0x10000u, seed);
+      const float u1 = u.x, u2 = 2.0f * u.y - 1.0f;
+      d = V2<float>{p0x + u1 * rx + u2 * radius * nx, p0y + u1 * ry + u2 * radius * ny};
+    }
+    if (!(d.x > 0 && d.x < 1 && d.y > 0 && d.y < 1)) continue;
+    V4<T> pn, df;
+    InitEyeRay<T>(cam, x, y, d, W, H, &pn, &df);
+    HitDR<T> hit = tr.NearestHitWithGrad(LRT_DR_RAY_FLAG_BORDER, pn, df, &py);
+    if (!(py.missed_hit.sdf < T(eps))) continue;
+    hits++;
+    contribute(pn, df, hit, T(pdf));
+  }
+  return hits;
+}
+
 template <typename T>
 static int dr_impl(const Scene& S, const MultiRendererDRPreset& dr, const Camera& cam, const float* wv, const float* pr, uint32_t W,
                    uint32_t H, uint32_t seed, const float* ref, int nthreads, int dtdv_mode, double* out_image, double* out_grad,
@@ -765,49 +934,57 @@ static int dr_impl(const Scene& S, const MultiRendererDRPreset& dr, const Camera
     {
       std::vector<double>& G = grads[omp_get_thread_num()];
       Payload<T> py;
+      // CalculateBorderRayDerivatives (MultiRendererDR.cpp:1319-1375) of a
+      // border ray that found a relaxation point
+      auto contribute = [&](const V4<T>& pn, const V4<T>& df, const HitDR<T>& hit, const T pdf) {
+        T delta[4];
+        if (dr.dr_input_type == LRT_DR_INPUT_TYPE_LINEAR_DEPTH) {
+          T din = lin_depth(py.missed_hit.t);
+          T dout = hit.primId == kInvalid ? T(0) : lin_depth(hit.t);
+          delta[0] = delta[1] = delta[2] = din - dout;
+        } else {
+          V3<T> fin = calc_color<T>(mode, py.missed_hit, S.lights);
+          V3<T> fout = hit.primId == kInvalid ? V3<T>(0, 0, 0) : calc_color<T>(mode, hit, S.lights);
+          delta[0] = fin.x - fout.x; delta[1] = fin.y - fout.y; delta[2] = fin.z - fout.z;
+        }
+        delta[3] = hit.primId == kInvalid ? T(1) : T(0);
+        for (int j = 0; j < 8; j++) {
+          if (py.missed_indices[j] == kInvalid) continue;
+          T bd = mx(T(0), py.missed_hit.sdf);
+          V3<T> ystar = V3<T>(pn.x, pn.y, pn.z) + py.missed_hit.t * V3<T>(df.x, df.y, df.z);
+          V3<T> xstar = ystar - bd * py.missed_hit.normal;
+          V4<T> ps = mul(M4<T>{{T(PV[0]), T(PV[1]), T(PV[2]), T(PV[3]), T(PV[4]), T(PV[5]), T(PV[6]), T(PV[7]), T(PV[8]), T(PV[9]),
+                                T(PV[10]), T(PV[11]), T(PV[12]), T(PV[13]), T(PV[14]), T(PV[15])}},
+                         V4<T>{xstar.x, xstar.y, xstar.z, T(1)});
+          T sx = T(0.5f) * (ps.x / ps.w + T(1)), sy = T(1) - T(0.5f) * (ps.y / ps.w + T(1));
+          // the reference converts to unsigned without a range check; the
+          // pixel is clamped into the image here
+          double fx = std::floor((double)(sx * T(W))), fy = std::floor((double)(sy * T(H)));
+          uint32_t xp = (uint32_t)std::min(std::max(fx, 0.0), (double)W - 1), yp = (uint32_t)std::min(std::max(fy, 0.0), (double)H - 1);
+          size_t q = (size_t)yp * W + xp;
+          T g4[4];
+          loss_grad(&img[4 * q], ref + 4 * q, g4);
+          T dd = g4[0] * delta[0] + g4[1] * delta[1] + g4[2] * delta[2] + g4[3] * delta[3];
+          T diff = pdf * (T(1) / eps) * dd * py.missed_dSDF_dtheta[j];
+          G[py.missed_indices[j]] += (double)diff;
+        }
+      };
 #pragma omp for schedule(dynamic, 4)
       for (int64_t bi = 0; bi < (int64_t)border.size(); bi++) {
         const uint32_t XY = packed[border[bi]];
         const uint32_t x = XY & 0xFFFF, y = XY >> 16;
-        for (uint32_t s = 0; s < bspp; s++) {
+        if (dr.dr_border_sampling == LRT_DR_BORDER_SAMPLING_SVM) {
+          border_hits += svm_border_pixel<T>(tr, cam, x, y, W, H, seed, bspp, float(eps), wv, PV, contribute, py);
+          continue;
+        }
+        for (uint32_t s = 0; s < bspp; s++) {  // CastBorderRay, MultiRendererDR.cpp:1377-1499
           V2<float> d = rand2(x, y, s, seed);
           V4<T> pn, df;
           InitEyeRay<T>(cam, x, y, d, W, H, &pn, &df);
           HitDR<T> hit = tr.NearestHitWithGrad(LRT_DR_RAY_FLAG_BORDER, pn, df, &py);
           if (!(py.missed_hit.sdf < eps)) continue;
           border_hits++;
-          T delta[4];
-          if (dr.dr_input_type == LRT_DR_INPUT_TYPE_LINEAR_DEPTH) {
-            T din = lin_depth(py.missed_hit.t);
-            T dout = hit.primId == kInvalid ? T(0) : lin_depth(hit.t);
-            delta[0] = delta[1] = delta[2] = din - dout;
-          } else {
-            V3<T> fin = calc_color<T>(mode, py.missed_hit, S.lights);
-            V3<T> fout = hit.primId == kInvalid ? V3<T>(0, 0, 0) : calc_color<T>(mode, hit, S.lights);
-            delta[0] = fin.x - fout.x; delta[1] = fin.y - fout.y; delta[2] = fin.z - fout.z;
-          }
-          delta[3] = hit.primId == kInvalid ? T(1) : T(0);
-          const T pdf = T(1.0f) / T(bspp);
-          for (int j = 0; j < 8; j++) {
-            if (py.missed_indices[j] == kInvalid) continue;
-            T bd = mx(T(0), py.missed_hit.sdf);
-            V3<T> ystar = V3<T>(pn.x, pn.y, pn.z) + py.missed_hit.t * V3<T>(df.x, df.y, df.z);
-            V3<T> xstar = ystar - bd * py.missed_hit.normal;
-            V4<T> ps = mul(M4<T>{{T(PV[0]), T(PV[1]), T(PV[2]), T(PV[3]), T(PV[4]), T(PV[5]), T(PV[6]), T(PV[7]), T(PV[8]), T(PV[9]),
-                                  T(PV[10]), T(PV[11]), T(PV[12]), T(PV[13]), T(PV[14]), T(PV[15])}},
-                           V4<T>{xstar.x, xstar.y, xstar.z, T(1)});
-            T sx = T(0.5f) * (ps.x / ps.w + T(1)), sy = T(1) - T(0.5f) * (ps.y / ps.w + T(1));
-            // the reference converts to unsigned without a range check; the
-            // pixel is clamped into the image here
-            double fx = std::floor((double)(sx * T(W))), fy = std::floor((double)(sy * T(H)));
-            uint32_t xp = (uint32_t)std::min(std::max(fx, 0.0), (double)W - 1), yp = (uint32_t)std::min(std::max(fy, 0.0), (double)H - 1);
-            size_t q = (size_t)yp * W + xp;
-            T g4[4];
-            loss_grad(&img[4 * q], ref + 4 * q, g4);
-            T dd = g4[0] * delta[0] + g4[1] * delta[1] + g4[2] * delta[2] + g4[3] * delta[3];
-            T diff = pdf * (T(1) / eps) * dd * py.missed_dSDF_dtheta[j];
-            G[py.missed_indices[j]] += (double)diff;
-          }
+          contribute(pn, df, hit, T(1.0f) / T(bspp));
         }
       }
     }

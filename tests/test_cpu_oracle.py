@@ -512,6 +512,39 @@ def test_dr_redistance_plane_closed_form():
     assert part[p - 1, 0, 0] == np.float32(99999.0) and part[b + 1, 0, 0] == exp[b + 1]
 
 
+@pytest.mark.parametrize("view", [0, 1])
+def test_dr_svm_border_term_estimates_the_random_sampling_integral(view):
+    """SVM border sampling (MultiRendererDR.cpp:1651-1931) and random border
+    sampling (:1377-1499) are two Monte Carlo estimators of the same
+    silhouette integral per border pixel: the SVM draws its rays from a
+    stripe around the fitted depth edge and weights them by the stripe's
+    area. Their border-only terms (gradient with border_spp 1024 minus the
+    gradient with border rays off) must agree in direction and size; a lost
+    area factor, a stripe across the wrong axis or a wrong label split
+    moves the ratio or the angle far outside these bounds."""
+    W = H = 64
+    tgt = L.grid_sbs("torus", 0.6, 0.2, 0.0, 1, seed=1, brick_count=4, brick_size=4)
+    ini = L.grid_sbs("sphere", 0.5, 0.0, 0.0, 1, seed=1, brick_count=4, brick_size=4)
+    pairs, _ = L.build_bvh(ini)
+    st, si = oracle.Scene(tgt, pairs), oracle.Scene(ini, pairs)
+    wv, pr = L.orbit_camera(3, view)
+    p = preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON, render_mode=L.MULTI_RENDER_MODE_LINEAR_DEPTH)
+    ref = st.render(wv, pr, p, W, H, precision=64, rgba=True)["rgba"].astype(np.float32)
+    ref[..., 3] = np.linalg.norm(ref[..., :3], axis=-1) >= 0.001
+    n = ini.values_f.size
+    g = {}
+    for sampling, spp in ((0, 0), (0, 1024), (1, 1024)):
+        dr = L.getDefaultPresetDR()
+        dr.dr_render_mode, dr.dr_input_type, dr.dr_reconstruction_flags, dr.border_spp = 3, 1, 2, spp
+        dr.dr_border_sampling = sampling
+        _, g[(sampling, spp)], stats = si.dr_render_grad(dr, wv, pr, W, H, ref, n, seed=3, precision=64)
+    rnd, svm = g[(0, 1024)] - g[(0, 0)], g[(1, 1024)] - g[(0, 0)]
+    cos = rnd @ svm / np.linalg.norm(rnd) / np.linalg.norm(svm)
+    ratio = np.linalg.norm(svm) / np.linalg.norm(rnd)
+    assert stats["border_rays_hit"] > 0
+    assert cos > 0.93 and 0.85 < ratio < 1.18, (cos, ratio)
+
+
 def test_adam_step_closed_form():
     rng = np.random.default_rng(9)
     n = 64

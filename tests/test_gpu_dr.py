@@ -172,6 +172,57 @@ def test_render_dr_on_noisy_distances_matches_fp32_twin(problem, seed):
     assert np.linalg.norm(grad - o32) / np.linalg.norm(o32) <= 1e-5
 
 
+@pytest.mark.parametrize("input_type,mode,view,seed", [(1, 3, 0, 5), (1, 3, 2, 6), (0, 2, 1, 7), (0, 2, 3, 8)])
+def test_render_dr_svm_border_sampling_matches_fp32_twin(problem, input_type, mode, view, seed):
+    """SVM border sampling (CastBorderRaySVM, MultiRendererDR.cpp:1651-1931):
+    per border pixel a jittered 16 x 16 grid of rays, a perceptron fit of the
+    depth split, the stripe around the fitted line and its border rays. The
+    fit and the stripe are sequential float arithmetic; the GPU must take the
+    same decisions as the FP32 twin, so border statistics match and dLoss/dS
+    within 1e-5. Linear-depth input and mask input, border_spp 1024 as the
+    reference's SVM tests use."""
+    tgt, ini, cams, refs = problem
+    dr = _preset(border_spp=1024)
+    dr.dr_input_type = input_type
+    dr.dr_render_mode = mode
+    dr.dr_border_sampling = L.DR_BORDER_SAMPLING_SVM
+    refs_in = refs.copy()
+    if input_type == 0:  # mask input: the reference colour is the silhouette
+        refs_in[..., :3] = (np.linalg.norm(refs[..., :3], axis=-1) >= 0.001)[..., None]
+    g = L.MultiRendererDR()
+    g.SetScene(ini)
+    wvs = np.stack([c[0] for c in cams])
+    prs = np.stack([c[1] for c in cams])
+    g.SetReference(refs_in, wvs, prs)
+    P = ini.values_f.size
+    pairs, _ = g.GetBVH()
+    sc = oracle.Scene(ini, pairs)
+    ref = refs_in[view].copy()
+    ref[..., 3] = (np.linalg.norm(ref[..., :3], axis=-1) >= 0.001)
+    img, grad, loss, ms = g.RenderDR(dr, view, seed=seed, n_params=P)
+    o_img, o32, st32 = sc.dr_render_grad(dr, wvs[view], prs[view], W, H, ref, P, seed=seed, precision=32)
+    bp, bh = g.LastStats()
+    print("svm", input_type, mode, view, "loss", loss, st32, "border", bp, bh, "ms", ms)
+    assert abs(loss - st32["loss"]) <= 1e-5 * max(1.0, abs(st32["loss"]))
+    assert np.mean(np.all(img == o_img.astype(np.float32), axis=-1)) > 0.999
+    assert bp == st32["border_pixels"] and st32["border_rays_hit"] > 0
+    assert abs(bh - st32["border_rays_hit"]) <= 2
+    assert np.linalg.norm(grad - o32) / np.linalg.norm(o32) <= 1e-5
+
+
+def test_svm_border_sampling_rejects_more_than_256_svm_samples(problem):
+    tgt, ini, cams, refs = problem
+    dr = _preset(border_spp=1156)
+    dr.dr_border_sampling = L.DR_BORDER_SAMPLING_SVM
+    g = L.MultiRendererDR()
+    g.SetScene(ini)
+    wvs = np.stack([c[0] for c in cams])
+    prs = np.stack([c[1] for c in cams])
+    g.SetReference(refs, wvs, prs)
+    with pytest.raises(L.LrtError, match="SVM"):
+        g.RenderDR(dr, 0, seed=1, n_params=ini.values_f.size)
+
+
 def _sparse(sbs, limit):
     """The bricks of a dense grid scene whose smallest distance value is below
     `limit` (indices and values kept as they are), as a sparse brick set."""
