@@ -144,8 +144,10 @@ int lrt_dr_set_reference(lrt_handle h, const float* images, const float* masks, 
  * (smoothed over neighbouring voxels for the ID32F_IRGB32F_IN layout,
  * BVH2DR.cpp:1300-1484). Both border samplers: RANDOM and SVM
  * (CastBorderRaySVM, MultiRendererDR.cpp:1651-1931; border_spp < 1156, i.e. at
- * most 256 SVM samples per pixel, else LRT_E_UNSUPPORTED). Not built: the
- * finite-difference DR mode (LRT_E_UNSUPPORTED). lrt_dr_optimize applies the
+ * most 256 SVM samples per pixel, else LRT_E_UNSUPPORTED). Both diff modes:
+ * DEFAULT and FINITE_DIFF (RenderDRFiniteDiff, MultiRendererDR.cpp:751-815:
+ * central differences of the summed image loss, one parameter at a time, over
+ * the distance values with the geometry flag, else the colour values). lrt_dr_optimize applies the
  * LK_DENOISING regularization (MultiRendererDR.cpp:1933-2057) and
  * redistancing (:2058-2382, dense brick grids whose first values are the
  * lattice) when the preset asks for them. */

@@ -513,6 +513,57 @@ __global__ __launch_bounds__(256) void k_dr_border_svm(DRArgs A, const uint32_t*
   if (local_hits) atomicAdd(hits, local_hits);
 }
 
+// ---- finite-difference DR (RenderDRFiniteDiff, MultiRendererDR.cpp:751-815) ----
+
+// Sets parameter j to p0 + delta (mode 0), p0 - delta (1) or back to p0 (2)
+// in values_f and at its positions in the per-brick corner copy, and clears
+// the primary pass's work counter for the render that follows.
+__global__ __launch_bounds__(256) void k_dr_fd_set(float* values_f, float* bvals, const float* p0s, const uint32_t* pos, uint32_t n_pos,
+                                                   uint32_t j, uint32_t mode, float delta, uint32_t* counter) {
+  const float p0 = p0s[j];
+  const float v = mode == 0 ? p0 + delta : (mode == 1 ? p0 - delta : p0);
+  for (uint32_t k = threadIdx.x; k < n_pos; k += 256) bvals[pos[k]] = v;
+  if (threadIdx.x == 0) {
+    values_f[j] = v;
+    *counter = 0u;
+  }
+}
+
+// Sum over the image of Loss(colour, reference) (MultiRendererDR.cpp:51-73:
+// float per pixel, summed in double), in a fixed order (one block: strided
+// partial sums, then a tree in LDS), so that equal images give equal sums.
+__global__ __launch_bounds__(1024) void k_dr_fd_loss(const float4* img, const float4* ref, uint32_t n, uint32_t loss_fn, double* out) {
+  __shared__ double part[1024];
+  double acc = 0.0;
+  for (uint32_t i = threadIdx.x; i < n; i += 1024) {
+    const float4 c = img[i], r = ref[i];
+    float l;
+    if (loss_fn == LRT_DR_LOSS_FUNCTION_MSE) {
+      const float dx = c.x - r.x, dy = c.y - r.y, dz = c.z - r.z, dw = c.w - r.w;
+      l = dx * dx + dy * dy + dz * dz + dw * dw;
+    } else {
+      l = fabsf(c.x - r.x) + fabsf(c.y - r.y) + fabsf(c.z - r.z) + fabsf(c.w - r.w);
+    }
+    acc += (double)l;
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (uint32_t h = 512; h > 0; h >>= 1) {
+    if (threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = part[0];
+}
+
+// dLoss/dS[j] = (g * L(p0 + delta) - g * L(p0 - delta)) / (2 delta), as float
+__global__ __launch_bounds__(256) void k_dr_fd_grad(const double* acc, const uint32_t* ids, uint32_t n, float g_mult, float delta,
+                                                    float* grad) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double lp = acc[2 * i] * g_mult, lm = acc[2 * i + 1] * g_mult;
+  grad[ids[i]] = (float)((lp - lm) / (2 * delta));
+}
+
 // Per-brick copy of the corner values the voxel walk reads:
 // out[b * v^3 + k] = values_f[values[data_offset(b) + k]] (indices checked
 // against values_f when the scene was set). One dependent load per corner in
@@ -782,6 +833,20 @@ hipError_t launch_dr_primary(const lrtd::DRArgs& A, uint32_t* d_counter, uint32_
     hipLaunchKernelGGL(lrtd::k_dr_primary<lrtd::kDrNormal>, dim3(blocks), dim3(256), 0, s, A, d_counter, d_spill, d_loss);
   else
     hipLaunchKernelGGL(lrtd::k_dr_primary<lrtd::kDrGeneric>, dim3(blocks), dim3(256), 0, s, A, d_counter, d_spill, d_loss);
+  return hipGetLastError();
+}
+hipError_t launch_dr_fd_set(float* d_values_f, float* d_bvals, const float* d_p0, const uint32_t* d_pos, uint32_t n_pos, uint32_t j,
+                            uint32_t mode, float delta, uint32_t* d_counter, hipStream_t s) {
+  hipLaunchKernelGGL(lrtd::k_dr_fd_set, dim3(1), dim3(256), 0, s, d_values_f, d_bvals, d_p0, d_pos, n_pos, j, mode, delta, d_counter);
+  return hipGetLastError();
+}
+hipError_t launch_dr_fd_loss(const float4* d_img, const float4* d_ref, uint32_t n, uint32_t loss_fn, double* d_out, hipStream_t s) {
+  hipLaunchKernelGGL(lrtd::k_dr_fd_loss, dim3(1), dim3(1024), 0, s, d_img, d_ref, n, loss_fn, d_out);
+  return hipGetLastError();
+}
+hipError_t launch_dr_fd_grad(const double* d_acc, const uint32_t* d_ids, uint32_t n, float g_mult, float delta, float* d_grad, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(lrtd::k_dr_fd_grad, dim3((n + 255) / 256), dim3(256), 0, s, d_acc, d_ids, n, g_mult, delta, d_grad);
   return hipGetLastError();
 }
 hipError_t launch_dr_border_find(const lrtd::DRArgs& A, uint32_t* d_list, uint32_t* d_count, hipStream_t s) {

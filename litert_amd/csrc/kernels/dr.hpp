@@ -47,6 +47,15 @@ struct DRCullTables {
 };
 // Writes both BVH copies for the current corner values (d_minmax from
 // launch_dr_gather) and relax_eps.
+// Finite-difference DR (RenderDRFiniteDiff, MultiRendererDR.cpp:751-815):
+// set one parameter (mode 0: p0 + delta, 1: p0 - delta, 2: p0) in values_f
+// and at its positions in the per-brick corner copy, clearing the primary's
+// work counter; sum the image's per-pixel loss in double (fixed order); turn
+// the loss pairs into dLoss/dS for the listed parameters.
+hipError_t launch_dr_fd_set(float* d_values_f, float* d_bvals, const float* d_p0, const uint32_t* d_pos, uint32_t n_pos, uint32_t j,
+                            uint32_t mode, float delta, uint32_t* d_counter, hipStream_t s);
+hipError_t launch_dr_fd_loss(const float4* d_img, const float4* d_ref, uint32_t n, uint32_t loss_fn, double* d_out, hipStream_t s);
+hipError_t launch_dr_fd_grad(const double* d_acc, const uint32_t* d_ids, uint32_t n, float g_mult, float delta, float* d_grad, hipStream_t s);
 hipError_t launch_dr_cull(const DRCullTables& T, const float2* d_minmax, float relax_eps, const uint4* d_pairs, uint32_t n_bricks,
                           uint32_t n_pairs, hipStream_t s);
 // cull_pairs: T.pairs_primary / T.pairs_border of launch_dr_cull for the

@@ -44,7 +44,7 @@ static int dr_check(Renderer* r, const MultiRendererDRPreset& p) {
   if (p.dr_border_sampling > LRT_DR_BORDER_SAMPLING_SVM) return set_err(LRT_E_INVALID, "bad border sampling");
   if (p.dr_border_sampling == LRT_DR_BORDER_SAMPLING_SVM && p.border_spp >= 1156)
     return set_err(LRT_E_UNSUPPORTED, "SVM border sampling takes border_spp < 1156 (at most 256 SVM samples per pixel)");
-  if (p.dr_diff_mode != LRT_DR_DIFF_MODE_DEFAULT) return set_err(LRT_E_UNSUPPORTED, "finite-difference DR mode is not built");
+  if (p.dr_diff_mode > LRT_DR_DIFF_MODE_FINITE_DIFF) return set_err(LRT_E_INVALID, "bad diff mode");
   if (p.reg_function > LRT_DR_REG_FUNCTION_LK_DENOISING) return set_err(LRT_E_INVALID, "bad regularization function");
   if (p.redistancing_enable && p.redistancing_interval == 0) return set_err(LRT_E_INVALID, "redistancing_interval must be > 0");
   if (p.dr_render_mode == LRT_DR_RENDER_MODE_LAMBERT)
@@ -129,6 +129,93 @@ static int dr_ensure_tree(Renderer* r) {
   return LRT_OK;
 }
 
+// RenderDRFiniteDiff (MultiRendererDR.cpp:751-815) over the active range of
+// :428-436 (geometry: every parameter before the 24 colour values per brick;
+// otherwise those colour values): for each parameter, the image rendered with
+// it at p0 + delta and at p0 - delta (primary rays only, no border rays), the
+// summed per-pixel loss of each, and their scaled central difference. A
+// parameter that no brick references leaves every image unchanged, so its
+// difference is 0 and it is not rendered. Like the reference, the image and
+// loss left behind are those of the last render (last parameter at p0 - delta).
+static int dr_finite_diff(Renderer* r, const MultiRendererDRPreset& p, const lrtd::DRArgs& A, uint32_t* misc, uint32_t blocks, size_t P,
+                          float g_mult, double* h_loss_sum) {
+  const size_t n_col = 24 * r->nodes.size();
+  if (P < n_col) return set_err(LRT_E_INVALID, "finite differences: fewer parameters than colour values");
+  const bool geo = (p.dr_reconstruction_flags & LRT_DR_RECONSTRUCTION_FLAG_GEOMETRY) != 0;
+  const size_t start = geo ? 0 : P - n_col, end = geo ? P - n_col : P;
+  if (end <= start) return set_err(LRT_E_INVALID, "finite differences need a non-empty parameter range (MultiRendererDR.cpp:754)");
+  const uint32_t nv = (uint32_t)A.S.v_size * A.S.v_size * A.S.v_size;
+  std::vector<uint32_t> cnt(end - start + 1, 0);
+  std::vector<uint8_t> used(end - start, 0);
+  for (size_t b = 0; b < r->nodes.size(); b++) {
+    const size_t off = r->nodes[b].data_offset;
+    for (uint32_t k = 0; k < nv; k++) {
+      const size_t j = r->values[off + k];
+      if (j >= start && j < end) cnt[j - start + 1]++, used[j - start] = 1;
+    }
+    for (uint32_t c = 0; c < 8; c++)
+      for (uint32_t e = 0; e < 3; e++) {
+        const size_t j = (size_t)r->values[off + nv + c] + e;
+        if (j >= start && j < end) used[j - start] = 1;
+      }
+  }
+  for (size_t i = 1; i < cnt.size(); i++) cnt[i] += cnt[i - 1];
+  std::vector<uint32_t> pos(cnt.back()), fill(cnt.begin(), cnt.end() - 1);
+  for (size_t b = 0; b < r->nodes.size(); b++) {
+    const size_t off = r->nodes[b].data_offset;
+    for (uint32_t k = 0; k < nv; k++) {
+      const size_t j = r->values[off + k];
+      if (j >= start && j < end) pos[fill[j - start]++] = (uint32_t)(b * nv + k);
+    }
+  }
+  std::vector<uint32_t> ids;
+  for (size_t j = start; j < end; j++)
+    if (used[j - start]) ids.push_back((uint32_t)j);
+  const uint32_t n = (uint32_t)ids.size();
+  std::vector<uint32_t> ids_up(ids), pos_up(pos);  // never empty: at least one word is uploaded
+  if (ids_up.empty()) ids_up.push_back(0);
+  if (pos_up.empty()) pos_up.push_back(0);
+  hipError_t e;
+  if ((e = r->d_dr_fd_p0.ensure(P * 4)) != hipSuccess) return hip_err(e, "alloc");
+  if ((e = hipMemcpyAsync(r->d_dr_fd_p0.p, r->d_values_f.p, P * 4, hipMemcpyDeviceToDevice, r->stream)) != hipSuccess) return hip_err(e, "copy");
+  if ((e = r->d_dr_fd_ids.upload(ids_up.data(), ids_up.size() * 4, r->stream)) != hipSuccess) return hip_err(e, "upload");
+  if ((e = r->d_dr_fd_pos.upload(pos_up.data(), pos_up.size() * 4, r->stream)) != hipSuccess) return hip_err(e, "upload");
+  if ((e = r->d_dr_fd_acc.ensure((2 * (size_t)n + 1) * 8)) != hipSuccess) return hip_err(e, "alloc");
+  if ((e = hipStreamSynchronize(r->stream)) != hipSuccess) return hip_err(e, "upload");  // the host tables are locals
+  float* vf = (float*)r->d_values_f.p;
+  float* bv = (float*)r->d_dr_bvals.p;
+  const float* p0 = (const float*)r->d_dr_fd_p0.p;
+  const uint32_t* dpos = (const uint32_t*)r->d_dr_fd_pos.p;
+  double* acc = (double*)r->d_dr_fd_acc.p;
+  const uint32_t npx = A.W * A.H;
+  const float delta = p.finite_diff_delta;
+  auto render = [&](double* out) -> int {
+    if ((e = launch_dr_primary(A, misc, (uint32_t*)r->d_spill.p, (float*)(misc + 4), blocks, r->stream, nullptr)) != hipSuccess)
+      return hip_err(e, "dr primary");
+    if ((e = launch_dr_fd_loss(A.out_image, A.ref, npx, A.loss_fn, out, r->stream)) != hipSuccess) return hip_err(e, "dr fd loss");
+    return LRT_OK;
+  };
+  int c;
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t j = ids[i], b0 = cnt[j - start], np_j = cnt[j - start + 1] - b0;
+    for (uint32_t mode = 0; mode < 3; mode++) {
+      if ((e = launch_dr_fd_set(vf, bv, p0, dpos + b0, np_j, j, mode, delta, misc, r->stream)) != hipSuccess) return hip_err(e, "dr fd set");
+      if (mode < 2 && (c = render(acc + 2 * i + mode)) != LRT_OK) return c;
+    }
+  }
+  size_t last = 2 * (size_t)n - 1;
+  if (n == 0 || ids.back() != end - 1) {  // the last parameter is not referenced: its renders equal the unperturbed one
+    if ((e = zero_u32(misc, 1, r->stream)) != hipSuccess) return hip_err(e, "clear");
+    if ((c = render(acc + 2 * (size_t)n)) != LRT_OK) return c;
+    last = 2 * (size_t)n;
+  }
+  if ((e = zero_u32((uint32_t*)r->d_dr_grad.p, (uint32_t)P, r->stream)) != hipSuccess) return hip_err(e, "clear grad");
+  if ((e = launch_dr_fd_grad(acc, (const uint32_t*)r->d_dr_fd_ids.p, n, g_mult, delta, (float*)r->d_dr_grad.p, r->stream)) != hipSuccess)
+    return hip_err(e, "dr fd grad");
+  if ((e = hipMemcpyAsync(h_loss_sum, acc + last, 8, hipMemcpyDeviceToHost, r->stream)) != hipSuccess) return hip_err(e, "copy");
+  return LRT_OK;
+}
+
 // one RenderDR call; gradient left in d_dr_grad (scaled)
 static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t view, uint32_t seed, int dtdv_mode, float* kernel_ms) {
   int c = dr_check(r, p);
@@ -192,6 +279,22 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
 
   if ((e = hipEventRecord(r->ev0, r->stream)) != hipSuccess) return hip_err(e, "event");
   if ((e = launch_dr_gather(A.S, (float*)r->d_dr_bvals.p, (float2*)r->d_dr_bminmax.p, r->stream)) != hipSuccess) return hip_err(e, "dr gather");
+  if (p.dr_diff_mode == LRT_DR_DIFF_MODE_FINITE_DIFF) {
+    const float g_mult_fd = (float)(std::pow((double)P + 1.0, 2.0 / 3.0) / (double)(W * H));  // getBaseGradientMult
+    double loss_sum = 0.0;
+    if ((c = dr_finite_diff(r, p, A, misc, blocks, P, g_mult_fd, &loss_sum)) != LRT_OK) return c;
+    if ((e = hipEventRecord(r->ev1, r->stream)) != hipSuccess) return hip_err(e, "event");
+    if ((e = hipStreamSynchronize(r->stream)) != hipSuccess) return hip_err(e, "dr finite differences");
+    r->dr_last_loss = (float)(loss_sum / (double)np);
+    r->dr_last_border = 0;  // border pixels are searched only in the default mode (MultiRendererDR.cpp:613)
+    r->dr_last_border_hits = 0;
+    r->dr_img_valid[view] = 1;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, r->ev0, r->ev1);
+    r->times["RenderDR"] = ms;
+    if (kernel_ms) *kernel_ms = ms;
+    return LRT_OK;
+  }
   const uint4 *cull_primary = nullptr, *cull_border = nullptr;
   if (cull && r->dr_tree_state == 1) {
     const uint32_t np_pairs = (uint32_t)r->pairs.size();

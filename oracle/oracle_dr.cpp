@@ -1012,11 +1012,71 @@ static int dr_impl(const Scene& S, const MultiRendererDRPreset& dr, const Camera
 
 }  // namespace
 
+// RenderDRFiniteDiff, MultiRendererDR.cpp:751-815, over the active range of
+// :428-436: every parameter of the range is set to p0 + delta and p0 - delta
+// in turn, the image re-rendered (primary rays only: border rays run in the
+// default mode alone, :703), the per-pixel Loss (:51-73, float) summed in
+// double and scaled by getBaseGradientMult; the central difference is stored
+// as float. The image and loss returned are those of the last render.
+template <typename T>
+static int dr_fd_impl(const Scene& S0, const MultiRendererDRPreset& dr, const Camera& cam, const float* wv, const float* pr, uint32_t W,
+                      uint32_t H, uint32_t seed, const float* ref, int nthreads, double* out_image, double* out_grad, orc_dr_stats* stats) {
+  Scene S = S0;
+  const size_t P = S.values_f.size(), n_col = 24 * S.nodes.size();
+  if (P < n_col) return -3;
+  const bool geo = (dr.dr_reconstruction_flags & LRT_DR_RECONSTRUCTION_FLAG_GEOMETRY) != 0;
+  const size_t start = geo ? 0 : P - n_col, end = geo ? P - n_col : P;
+  if (end <= start) return -3;  // the reference asserts
+  const size_t np = (size_t)W * H;
+  std::vector<double> img(np * 4), scratch(P);
+  const float g_mult = (float)(std::pow((double)P + 1.0, 2.0 / 3.0) / (double)(W * H));
+  const float delta = dr.finite_diff_delta;
+  auto loss_sum = [&]() {
+    double acc = 0.0;
+    for (size_t j = 0; j < np; j++) {
+      float d[4], l;
+      for (int k = 0; k < 4; k++) d[k] = (float)img[4 * j + k] - ref[4 * j + k];
+      if (dr.dr_loss_function == LRT_DR_LOSS_FUNCTION_MSE) l = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + d[3] * d[3];
+      else l = std::abs(d[0]) + std::abs(d[1]) + std::abs(d[2]) + std::abs(d[3]);
+      acc += l;
+    }
+    return acc;
+  };
+  double loss_minus = 0.0;
+  for (size_t i = start; i < end; i++) {
+    const float p0 = S.values_f[i];
+    S.values_f[i] = p0 + delta;
+    int rc = dr_impl<T>(S, dr, cam, wv, pr, W, H, seed, ref, nthreads, 0, img.data(), scratch.data(), nullptr);
+    if (rc) return rc;
+    double loss_plus = loss_sum();
+    S.values_f[i] = p0 - delta;
+    rc = dr_impl<T>(S, dr, cam, wv, pr, W, H, seed, ref, nthreads, 0, img.data(), scratch.data(), nullptr);
+    if (rc) return rc;
+    loss_minus = loss_sum();
+    S.values_f[i] = p0;
+    const double lp = loss_plus * g_mult, lm = loss_minus * g_mult;
+    out_grad[i] += (double)(float)((lp - lm) / (2 * delta));
+  }
+  if (out_image)
+    for (size_t k = 0; k < np * 4; k++) out_image[k] = img[k];
+  if (stats) {
+    stats->loss = loss_minus / (double)np;
+    stats->border_pixels = 0;
+    stats->border_rays_hit = 0;
+    stats->rays = (uint64_t)np * dr.spp * 2 * (end - start);
+  }
+  return 0;
+}
+
 extern "C" int orc_dr_render_grad(const orc_scene* s, const MultiRendererDRPreset* dr, const float worldView[16], const float proj[16],
                                   uint32_t W, uint32_t H, uint32_t seed, const float* ref, int precision, int nthreads, int dtdv_mode,
                                   double* out_image, double* out_grad, orc_dr_stats* stats) {
   if (!s || !dr || !ref || !out_grad || W < 1 || H < 1 || dr->spp == 0) return -1;
   Camera cam = Camera::from(worldView, proj);
+  if (dr->dr_diff_mode == LRT_DR_DIFF_MODE_FINITE_DIFF) {
+    if (precision == 64) return dr_fd_impl<double>(s->s, *dr, cam, worldView, proj, W, H, seed, ref, nthreads, out_image, out_grad, stats);
+    return dr_fd_impl<float>(s->s, *dr, cam, worldView, proj, W, H, seed, ref, nthreads, out_image, out_grad, stats);
+  }
   if (precision == 64)
     return dr_impl<double>(s->s, *dr, cam, worldView, proj, W, H, seed, ref, nthreads, dtdv_mode, out_image, out_grad, stats);
   return dr_impl<float>(s->s, *dr, cam, worldView, proj, W, H, seed, ref, nthreads, dtdv_mode, out_image, out_grad, stats);

@@ -545,6 +545,52 @@ def test_dr_svm_border_term_estimates_the_random_sampling_integral(view):
     assert cos > 0.93 and 0.85 < ratio < 1.18, (cos, ratio)
 
 
+def _fd_problem():
+    W = H = 32
+    tgt = L.grid_sbs("torus", 0.6, 0.2, 0.0, 1, seed=1, brick_count=2, brick_size=4)
+    ini = L.grid_sbs("sphere", 0.5, 0.0, 0.0, 1, seed=1, brick_count=2, brick_size=4)
+    vs = ini.header.brick_size + 1
+    off = ini.nodes[:, 2].astype(np.int64)
+    ci = np.unique(ini.values[(off[:, None] + vs ** 3 + np.arange(8)[None, :]).ravel()].astype(np.int64))
+    vf = ini.values_f.copy()
+    rng = np.random.default_rng(3)
+    for k in range(3):
+        vf[ci + k] = rng.uniform(0.1, 0.9, ci.size).astype(np.float32)
+    ini = L.SdfSBS(ini.header, ini.nodes, ini.values, vf)
+    st, si = oracle.Scene(tgt, L.build_bvh(tgt)[0]), oracle.Scene(ini, L.build_bvh(ini)[0])
+    wv, pr = L.orbit_camera(3, 0)
+    p = preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON, render_mode=L.MULTI_RENDER_MODE_LINEAR_DEPTH)
+    ref = st.render(wv, pr, p, W, H, precision=64, rgba=True)["rgba"].astype(np.float32)
+    ref[..., 3] = np.linalg.norm(ref[..., :3], axis=-1) >= 0.001
+    return ini, si, wv, pr, W, H, ref
+
+
+def test_dr_finite_difference_mode_matches_analytic_colour_gradient():
+    """DR_DIFF_MODE_FINITE_DIFF (RenderDRFiniteDiff, MultiRendererDR.cpp:
+    751-815) against the analytic colour gradient of the default mode, as the
+    reference's own test compares them (tests_diff_render.cpp:240-300: random
+    colours, DIFFUSE, spp 4): the rendered colour is linear in the colour
+    parameters and the MSE loss quadratic, so the central difference equals
+    the derivative up to float rounding of the per-pixel losses. Parameters
+    outside the active range keep a zero gradient; with the geometry flag
+    only distance parameters are differentiated."""
+    ini, si, wv, pr, W, H, ref = _fd_problem()
+    n = ini.values_f.size
+    c0 = n - 24 * ini.nodes.shape[0]
+    g = {}
+    for diff_mode in (0, 1):
+        dr = L.getDefaultPresetDR()
+        dr.dr_render_mode, dr.dr_input_type, dr.dr_reconstruction_flags, dr.border_spp, dr.spp = 0, 0, 1, 0, 4
+        dr.dr_diff_mode, dr.finite_diff_delta = diff_mode, 0.005
+        _, g[diff_mode], stats = si.dr_render_grad(dr, wv, pr, W, H, ref, n, seed=3, precision=32)
+    assert np.linalg.norm(g[1][c0:] - g[0][c0:]) / np.linalg.norm(g[0][c0:]) < 1e-4
+    assert not np.any(g[1][:c0])
+    assert stats["border_pixels"] == 0
+    dr.dr_render_mode, dr.dr_input_type, dr.dr_reconstruction_flags, dr.spp = 3, 1, 2, 1
+    _, gg, _ = si.dr_render_grad(dr, wv, pr, W, H, ref, n, seed=3, precision=32)
+    assert not np.any(gg[c0:]) and np.count_nonzero(gg[:c0]) > 10
+
+
 def test_adam_step_closed_form():
     rng = np.random.default_rng(9)
     n = 64

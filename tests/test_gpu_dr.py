@@ -223,6 +223,49 @@ def test_svm_border_sampling_rejects_more_than_256_svm_samples(problem):
         g.RenderDR(dr, 0, seed=1, n_params=ini.values_f.size)
 
 
+@pytest.mark.parametrize("case", ["colour", "geometry"])
+def test_render_dr_finite_difference_mode_matches_fp32_twin(case):
+    """DR_DIFF_MODE_FINITE_DIFF (RenderDRFiniteDiff, MultiRendererDR.cpp:
+    751-815): every referenced parameter of the active range re-rendered at
+    p0 +- delta on the GPU, per-pixel losses summed in double. Against the
+    FP32 twin's restatement (which renders every parameter of the range):
+    gradients within 1e-5, and the image and loss of the last render."""
+    W = H = 32
+    tgt = L.grid_sbs("torus", 0.6, 0.2, 0.0, 1, seed=1, brick_count=2, brick_size=4)
+    ini = _random_colours(L.grid_sbs("sphere", 0.5, 0.0, 0.0, 1, seed=1, brick_count=2, brick_size=4), 3)
+    wv, pr = L.orbit_camera(3, 0)
+    r = L.CreateMultiRenderer(L.DEVICE_GPU)
+    r.SetScene(tgt)
+    p = L.getDefaultPreset()
+    p.sdf_node_intersect = L.SDF_OCTREE_NODE_INTERSECT_NEWTON
+    p.render_mode = L.MULTI_RENDER_MODE_LINEAR_DEPTH
+    refs = r.RenderFloat(W, H, wv, pr, p)[None]
+    r.close()
+    dr = L.getDefaultPresetDR()
+    if case == "colour":
+        dr.dr_render_mode, dr.dr_input_type, dr.dr_reconstruction_flags, dr.spp = 0, 0, 1, 4
+    else:
+        dr.dr_render_mode, dr.dr_input_type, dr.dr_reconstruction_flags, dr.spp = 3, 1, 2, 1
+    dr.dr_diff_mode, dr.finite_diff_delta, dr.border_spp = 1, 0.005, 64
+    g = L.MultiRendererDR()
+    g.SetScene(ini)
+    g.SetReference(refs, wv[None], pr[None])
+    P = ini.values_f.size
+    pairs, _ = g.GetBVH()
+    sc = oracle.Scene(ini, pairs)
+    ref = refs[0].copy()
+    ref[..., 3] = (np.linalg.norm(ref[..., :3], axis=-1) >= 0.001)
+    img, grad, loss, ms = g.RenderDR(dr, 0, seed=11, n_params=P)
+    o_img, o32, st32 = sc.dr_render_grad(dr, wv, pr, W, H, ref, P, seed=11, precision=32)
+    bp, bh = g.LastStats()
+    print(case, "loss", loss, st32, "border", bp, bh, "ms", ms)
+    assert np.mean(np.all(img == o_img.astype(np.float32), axis=-1)) > 0.999
+    assert bp == 0 and bh == 0
+    assert abs(loss - st32["loss"]) <= 1e-5 * max(1.0, abs(st32["loss"]))
+    assert np.count_nonzero(o32) > 10
+    assert np.linalg.norm(grad - o32) / np.linalg.norm(o32) <= 1e-5
+
+
 def _sparse(sbs, limit):
     """The bricks of a dense grid scene whose smallest distance value is below
     `limit` (indices and values kept as they are), as a sparse brick set."""
