@@ -43,6 +43,12 @@ __device__ __forceinline__ bool shadow_ray(const LightSet& L, uint32_t i, v3 ray
 // Render modes that apply to SBS geometry (eye_ray.cpp:162-489) with the
 // renderer's lights. lit: bit i set if Phong's shadow ray toward light i found
 // nothing. ray_pos: the eye ray's origin.
+// Colours of the PRIMITIVE / TYPE / GEOM modes (eye_ray.cpp palette), read
+// with a run-time index: a constant table, not a per-lane array in scratch.
+static __constant__ uint32_t kPalette[20] = {0xffe6194b, 0xff3cb44b, 0xffffe119, 0xff0082c8, 0xfff58231, 0xff911eb4, 0xff46f0f0,
+                                             0xfff032e6, 0xffd2f53c, 0xfffabebe, 0xff008080, 0xffe6beff, 0xffaa6e28, 0xfffffac8,
+                                             0xff800000, 0xffaaffc3, 0xff808000, 0xffffd8b1, 0xff000080, 0xff808080};
+
 __device__ __forceinline__ float4 shade(uint32_t mode, const Hit& hit, v3 ray_pos, v3 dir, const LightSet& L, uint32_t lit) {
   if (hit.primId == 0xFFFFFFFFu) return make_float4(0, 0, 0, 1);
   const float z = hit.t, z_near = 0.1f, z_far = 10.0f;
@@ -51,9 +57,7 @@ __device__ __forceinline__ float4 shade(uint32_t mode, const Hit& hit, v3 ray_po
   v3 norm = decode_normal(hit.coords[2], hit.coords[3]);
   const float ns = sgn(dot(-1.0f * dir, norm));
   norm = norm * ns;
-  const uint32_t pal[20] = {0xffe6194b, 0xff3cb44b, 0xffffe119, 0xff0082c8, 0xfff58231, 0xff911eb4, 0xff46f0f0,
-                            0xfff032e6, 0xffd2f53c, 0xfffabebe, 0xff008080, 0xffe6beff, 0xffaa6e28, 0xfffffac8,
-                            0xff800000, 0xffaaffc3, 0xff808000, 0xffffd8b1, 0xff000080, 0xff808080};
+  const uint32_t* pal = kPalette;
   auto palc = [](uint32_t c) {
     return make_float4((float)(c & 0xFF) * (1.0f / 255.0f), (float)((c >> 8) & 0xFF) * (1.0f / 255.0f),
                        (float)((c >> 16) & 0xFF) * (1.0f / 255.0f), (float)((c >> 24) & 0xFF) * (1.0f / 255.0f));
