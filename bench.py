@@ -376,6 +376,19 @@ def main():
                    "loss_mean_first_last_16_steps": [float(np.mean(losses[:16])), float(np.mean(losses[-16:]))]}
         log(f"dr: {dr_mrays:.1f} Mrays/s (fwd+bwd+Adam, {total_ms / dr.opt_iterations:.2f} ms/iter), RenderDR kernels "
             f"{np.mean(view_ms):.2f} ms/view, border pixels {bp}")
+        # SVM border sampling (MultiRendererDR.cpp:1651-1931) on the same views
+        # and parameters: per border pixel 64 SVM samples, the perceptron fit
+        # and the stripe's border rays, against random sampling above
+        dr.dr_border_sampling = L.DR_BORDER_SAMPLING_SVM
+        g.RenderDR(dr, 0, seed=1)  # warm-up
+        svm_ms = [g.RenderDR(dr, v, seed=1)[3] for v in range(4)]
+        sbp, sbh = g.LastStats()
+        dr.dr_border_sampling = L.DR_BORDER_SAMPLING_RANDOM
+        dr_info["svm_renderdr_kernel_ms"] = round(float(np.mean(svm_ms)), 3)
+        dr_info["svm_last_view_border_hits"] = int(sbh)
+        dr_info["random_last_view_border_hits"] = int(bh)
+        log(f"dr svm: RenderDR kernels {np.mean(svm_ms):.2f} ms/view (random sampling {np.mean(view_ms):.2f}), border hits "
+            f"{sbh} vs {bh} on the last view")
 
     cpu = None
     if not args.no_cpu:

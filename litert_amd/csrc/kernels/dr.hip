@@ -279,28 +279,52 @@ __global__ __launch_bounds__(256) void k_dr_border_rays(DRArgs A, const uint32_t
 
 // ---- SVM border sampling (MultiRendererDR.cpp:1501-1932) -------------------
 
+
 // SVM_fit (:1518-1546): a perceptron with weight decay on the labelled
-// samples X (already multiplied by their label), etha 0.01, alpha 0.1, at most
-// 200 epochs; returns W and the last epoch's error rate.
-__device__ float4 svm_fit(const float* xs, const float* ys, const float* lab, uint32_t N) {
+// samples X = (x, y, 1) * label * etha (etha 0.01, alpha 0.1), at most 200
+// epochs, stopping after an epoch without errors; returns W and the last
+// epoch's error rate. The samples are spread over the wave (sample s in lane
+// s % 64, slot s / 64): every lane runs the same sequential epochs on a
+// wave-uniform W and reads sample j's terms from its lane with readlane, in
+// the reference's order and with its float operations.
+__device__ float4 svm_fit_wave(const float* xs, const float* ys, const float* lab, uint32_t N, uint32_t lane) {
   const float etha = 0.01f, alpha = 0.1f;
   const int epochs = 200;
-  v3 W(-1, 1, 0);
+  float X0[4], X1[4], X2[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t s = lane + 64u * k;
+    const float l = s < N ? lab[s] : 0.0f;
+    X0[k] = s < N ? xs[s] * l * etha : 0.0f;
+    X1[k] = s < N ? ys[s] * l * etha : 0.0f;
+    X2[k] = s < N ? 1.0f * l * etha : 0.0f;
+  }
+  float w0 = -1.0f, w1 = 1.0f, w2 = 0.0f;
   const float weightMult = 1.0f - etha * alpha / epochs;
   uint32_t errors = N;
   for (int i = 0; i < epochs && errors > 0; i++) {
     errors = 0;
-    for (uint32_t j = 0; j < N; j++) {
-      const v3 X = v3(xs[j] * lab[j], ys[j] * lab[j], 1.0f * lab[j]) * etha;
-      const float margin = dot(W, X);
-      W = W * weightMult;
-      if (margin < 0) {
-        W = W + X;
-        errors++;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint32_t n_k = N > 64u * k ? min(64u, N - 64u * k) : 0u;
+      for (uint32_t src = 0; src < n_k; src++) {
+        const float a0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(X0[k]), src));
+        const float a1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(X1[k]), src));
+        const float a2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(X2[k]), src));
+        const float margin = w0 * a0 + w1 * a1 + w2 * a2;
+        w0 = w0 * weightMult;
+        w1 = w1 * weightMult;
+        w2 = w2 * weightMult;
+        if (margin < 0) {
+          w0 = w0 + a0;
+          w1 = w1 + a1;
+          w2 = w2 + a2;
+          errors++;
+        }
       }
     }
   }
-  return make_float4(W.x, W.y, W.z, float(errors) / float(N));
+  return make_float4(w0, w1, w2, float(errors) / float(N));
 }
 
 __device__ __forceinline__ uint32_t out_code(float x, float y) {  // ComputeOutCode, :1550-1571, window [0,1]^2
@@ -368,7 +392,7 @@ __global__ __launch_bounds__(256) void k_dr_border_svm(DRArgs A, const uint32_t*
                                                        uint32_t* hits) {
   __shared__ uint32_t lds_stack[kStackLds * 256];
   __shared__ float s_x[4][kSvmMaxSamples], s_y[4][kSvmMaxSamples], s_t[4][kSvmMaxSamples], s_l[4][kSvmMaxSamples];
-  __shared__ float s_geo[4][12];  // p0, r, n, radius, pdf, force, n_rays, skip
+  __shared__ float s_geo[4][16];  // p0, r, n, radius, pdf, force, n_rays, skip; labels' skip, type0, closest, closest t
   // Each wave owns row wv of these arrays, so waves run through their pixels
   // independently and synchronise only with themselves.
   Stack st;
@@ -395,15 +419,19 @@ __global__ __launch_bounds__(256) void k_dr_border_svm(DRArgs A, const uint32_t*
       const float dx = i_sq * ((float)(s / sq) + rnd.x), dy = i_sq * ((float)(s % sq) + rnd.y);
       v3 pos, dir;
       eye_ray(A.cam, x, y, dx, dy, A.W, A.H, pos, dir);
-      const DRHit hit = dr_nearest_hit<SPEC == kDrDepthBorder ? kDrGeneric : SPEC>(A.S, A.bvals, A.bminmax, Q0, A.relax_eps, pos, dir,
-                                                                                  1e9f, &py, st);
+      // Only t is used. The linear-depth primary form finds the same t (its
+      // voxel and brick skips drop only voxels that cannot hold a root) and,
+      // with scatter off, writes nothing; on the border pass's BVH copy it
+      // descends as over the full tree.
+      const DRHit hit = dr_nearest_hit<SPEC == kDrDepthBorder ? kDrDepth : SPEC>(A.S, A.bvals, A.bminmax, Q0, A.relax_eps, pos, dir, 1e9f,
+                                                                                &py, st);
       s_x[wv][s] = dx;
       s_y[wv][s] = dy;
       s_t[wv][s] = hit.t;
     }
     wave_sync();
     if (lane == 0) {
-      // step 2: threshold, labels, fit (:1716-1745)
+      // step 2: threshold and labels (:1716-1736)
       const float thr = samples_to_threshold(s_t[wv], svm_spp, 0.0f, 0u);
       uint32_t type0 = 0, closest = 0;
       float closest_t = 1e6f;
@@ -413,12 +441,23 @@ __global__ __launch_bounds__(256) void k_dr_border_svm(DRArgs A, const uint32_t*
         s_l[wv][s] = t <= thr ? -1.0f : 1.0f;
         type0 += t <= thr ? 1u : 0u;
       }
+      s_geo[wv][12] = (type0 == svm_spp || type0 == 0) ? 1.0f : 0.0f;
+      s_geo[wv][13] = __uint_as_float(type0);
+      s_geo[wv][14] = __uint_as_float(closest);
+      s_geo[wv][15] = closest_t;
+    }
+    wave_sync();
+    const bool one_class = s_geo[wv][12] != 0.0f;  // uniform within the wave
+    // the fit (:1740), wave-wide
+    const float4 W = one_class ? make_float4(0, 0, 0, 0) : svm_fit_wave(s_x[wv], s_y[wv], s_l[wv], svm_spp, lane);
+    if (lane == 0) {
+      const uint32_t type0 = __float_as_uint(s_geo[wv][13]), closest = __float_as_uint(s_geo[wv][14]);
+      const float closest_t = s_geo[wv][15];
       float skip = 0.0f, force = 0.0f, stripe_area = 1.0f, sample_radius = 0.0f;
       float2 p0 = make_float2(0, 0), p1 = make_float2(0, 0), r = make_float2(0, 0), n = make_float2(0, 0);
-      if (type0 == svm_spp || type0 == 0) {
+      if (one_class) {
         skip = 1.0f;
       } else {
-        const float4 W = svm_fit(s_x[wv], s_y[wv], s_l[wv], svm_spp);
         const float error_rate = W.w;
         const float f0 = (float)type0 / (float)svm_spp;
         const float rare = fminf(f0, 1.0f - f0);
