@@ -62,23 +62,15 @@ def plain_scene(L, lod=128):
 def dr_problem(L, np, views=16, res=512, brick_count=32, brick_size=4):
     """DR problem: noisy torus (R 0.6, r 0.2) target, sphere r 0.5 start, dense
     indexed SBS of 128^3 voxels, `views` seeded views at res x res."""
-    tgt = L.grid_sbs("torus", 0.6, 0.2, 0.02, 3, seed=4, brick_count=brick_count, brick_size=brick_size)
-    ini = L.grid_sbs("sphere", 0.5, 0.0, 0.0, 3, seed=4, brick_count=brick_count, brick_size=brick_size)
-    cams = [L.orbit_camera(4, i) for i in range(views)]
+    from litert_amd import workloads as WL
+    tgt, ini = WL.dr_scenes(brick_count, brick_size)
+    cams = WL.dr_cameras(views)
     r = L.CreateMultiRenderer(L.DEVICE_GPU)
     r.SetScene(tgt)
-    p = L.getDefaultPreset()
-    p.sdf_node_intersect = L.SDF_OCTREE_NODE_INTERSECT_NEWTON
-    p.render_mode = L.MULTI_RENDER_MODE_LINEAR_DEPTH
+    p = WL.dr_reference_preset()
     refs = np.stack([r.RenderFloat(res, res, wv, pr, p) for wv, pr in cams])
     r.close()
-    dr = L.getDefaultPresetDR()
-    dr.dr_render_mode = 3  # DR_RENDER_MODE_LINEAR_DEPTH
-    dr.dr_input_type = 1   # DR_INPUT_TYPE_LINEAR_DEPTH
-    dr.dr_reconstruction_flags = 2  # geometry
-    dr.border_spp = 256
-    dr.opt_lr = 1e-3
-    return tgt, ini, cams, refs, dr
+    return tgt, ini, cams, refs, WL.dr_preset()
 
 
 def time_views(r, wvs, prs, steps, warmup):
@@ -169,27 +161,21 @@ def batch_leg(L, np, args, mlp_params):
     (the same batched kernel given one item at a time, so the plain
     single-scene kernel's statistics cover the 512^3 workload only). With a
     trained network the seeded form of the same batch is timed too."""
-    rng = np.random.default_rng(BATCH_SEED)
+    from litert_amd import workloads as WL
     p = L.getDefaultPreset()
     p.sdf_node_intersect = L.SDF_OCTREE_NODE_INTERSECT_NEWTON
     p.render_mode = L.MULTI_RENDER_MODE_NORMAL
     rs, wvs, prs, ws, hs, bricks = [], [], [], [], [], []
     t0 = time.perf_counter()
-    for k in range(args.batch_scenes):
-        depth = int(rng.integers(5, 10))
-        bpv = int(rng.choice([1, 4]))
-        torus = bool(rng.integers(0, 2))
-        w, h = int(rng.integers(128, 2049)), int(rng.integers(128, 2049))
-        kw = dict(seed=BATCH_SEED * 1000 + k, lod=1 << (depth - 2), brick_size=4, brick_pad=0, bytes_per_value=bpv,
-                  voxel_diag_scale=bpv == 1)
-        s = L.displaced_torus(0.6, 0.2, 0.02, 3, **kw) if torus else L.displaced_icosphere(5, 0.8, 0.05, 3, **kw)
+    for spec in WL.batch_specs(args.batch_scenes, BATCH_SEED):
+        s = WL.batch_scene(spec)
         r = L.CreateMultiRenderer(L.DEVICE_GPU)
         r.SetScene(s)
         r.SetPreset(p)
         bricks.append(int(s.nodes.shape[0]))
         del s
-        wv, pr = L.orbit_camera(BATCH_SEED * 1000 + k, 0, aspect=w / h)
-        rs.append(r), wvs.append(wv), prs.append(pr), ws.append(w), hs.append(h)
+        wv, pr = WL.batch_camera(spec)
+        rs.append(r), wvs.append(wv), prs.append(pr), ws.append(spec["w"]), hs.append(spec["h"])
     build_s = time.perf_counter() - t0
     wvs, prs = np.stack(wvs), np.stack(prs)
     rays = int(sum(w * h for w, h in zip(ws, hs)))

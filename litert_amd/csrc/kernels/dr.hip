@@ -18,6 +18,10 @@ namespace lrtd {
 
 __device__ __forceinline__ float lin_depth(float z) { return (z - 0.1f) / (10.0f - 0.1f); }
 
+// Hit depths a pixel keeps in registers for its depth-discontinuity
+// threshold; pixels with more hits take the streamed form below.
+constexpr uint32_t kDrKeptSamples = 16;
+
 __device__ __forceinline__ v3 dr_color(uint32_t mode, const DRHit& h, const LightSet& L) {  // MultiRendererDR.cpp:855-898
   if (mode == LRT_DR_RENDER_MODE_DIFFUSE) return h.color;
   if (mode == LRT_DR_RENDER_MODE_MASK) return v3(1, 1, 1);
@@ -43,15 +47,14 @@ __device__ __forceinline__ float loss_grad1(uint32_t fn, float c, float r) {
   return fn == LRT_DR_LOSS_FUNCTION_MSE ? 2.0f * (c - r) : sgn(c - r);
 }
 
-// t_samples_to_threshold (MultiRendererDR.cpp:997-1063)
-__device__ float samples_to_threshold(const float* s, unsigned count, float dv, unsigned dc) {
-  float mnv = s[0], mxv = s[0];
-  for (unsigned i = 1; i < count; i++) { mnv = fminf(mnv, s[i]); mxv = fmaxf(mxv, s[i]); }
-  if (dc > 0) { mnv = fminf(mnv, dv); mxv = fmaxf(mxv, dv); }
-  float diff = fmaxf(1e-9f, mxv - mnv);
-  unsigned g[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (unsigned i = 0; i < count; i++) g[min(max(int((s[i] - mnv) / diff * 10), 0), 9)]++;
-  g[min(max(int((dv - mnv) / diff * 10), 0), 9)] += dc;
+// t_samples_to_threshold (MultiRendererDR.cpp:997-1063) in three steps: the
+// depth range, the 10-bin histogram over it, the longest empty run of bins.
+// A pixel with more samples than it can keep in registers takes the range
+// from one pass over its rays and bins the depths while it traces them again
+// (same rays, same hits), with the same operations as the stored form.
+__device__ __forceinline__ int depth_bin(float t, float mnv, float diff) { return min(max(int((t - mnv) / diff * 10), 0), 9); }
+
+__device__ __forceinline__ float depth_gap_threshold(const unsigned g[10], float mnv, float diff) {
   int best_len = 0, best_start = -1, start = -1;
   for (int i = 1; i < 10; i++) {
     if (start != -1) {
@@ -66,6 +69,25 @@ __device__ float samples_to_threshold(const float* s, unsigned count, float dv, 
   }
   if (best_len > 4) return mnv + diff * (best_start + 0.5f * best_len) / 10;
   return 0.0f;
+}
+
+// depth range of `count` hits (running min / max mnv, mxv) and dc default
+// values dv; returns the bin width base diff and seeds g with the defaults
+__device__ __forceinline__ float depth_range(float& mnv, float& mxv, float dv, unsigned dc, unsigned g[10]) {
+  if (dc > 0) { mnv = fminf(mnv, dv); mxv = fmaxf(mxv, dv); }
+  const float diff = fmaxf(1e-9f, mxv - mnv);
+  for (int i = 0; i < 10; i++) g[i] = 0u;
+  g[depth_bin(dv, mnv, diff)] += dc;
+  return diff;
+}
+
+__device__ float samples_to_threshold(const float* s, unsigned count, float dv, unsigned dc) {
+  float mnv = s[0], mxv = s[0];
+  for (unsigned i = 1; i < count; i++) { mnv = fminf(mnv, s[i]); mxv = fmaxf(mxv, s[i]); }
+  unsigned g[10];
+  const float diff = depth_range(mnv, mxv, dv, dc, g);
+  for (unsigned i = 0; i < count; i++) g[depth_bin(s[i], mnv, diff)]++;
+  return depth_gap_threshold(g, mnv, diff);
 }
 
 template <int SPEC>
@@ -110,7 +132,8 @@ __device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, S
   py.scatter = 0;
   v3 res(0, 0, 0);
   int hit_count = 0;
-  float samples[16];
+  float samples[kDrKeptSamples];
+  float t_mn = 0.0f, t_mx = 0.0f;  // range of the hit depths, for pixels with more hits than samples[] holds
   const uint32_t spp = SPEC == kDrDepth ? 1u : A.spp;
   // kDrNormal always traces twice: the second pass scatters at the hit
   const bool one_pass = spp == 1 && SPEC != kDrNormal;
@@ -128,12 +151,34 @@ __device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, S
     if (hit.primId == 0xFFFFFFFFu) continue;
     v3 c = dr_color(A.mode, hit, A.lights);
     res = res + v3(c.x / (float)spp, c.y / (float)spp, c.z / (float)spp);
-    if (hit_count < 16) samples[hit_count] = hit.t;
+    if (hit_count < (int)kDrKeptSamples) samples[hit_count] = hit.t;
+    t_mn = hit_count ? fminf(t_mn, hit.t) : hit.t;
+    t_mx = hit_count ? fmaxf(t_mx, hit.t) : hit.t;
     hit_count++;
   }
   const float cm[4] = {res.x, res.y, res.z, (float)hit_count / (float)spp};
   A.out_image[pix] = make_float4(cm[0], cm[1], cm[2], cm[3]);
-  A.depth_thr[pix] = (hit_count == 0 || hit_count > 16) ? 0.0f : samples_to_threshold(samples, hit_count, 1e9f, spp - hit_count);
+  // depth-discontinuity threshold (written only where phase II reads it)
+  const bool streamed = hit_count > (int)kDrKeptSamples;
+  unsigned hist[10];
+  float h_diff = 0.0f;
+  if (A.depth_thr) {
+    if (!streamed) {
+      A.depth_thr[pix] = hit_count == 0 ? 0.0f : samples_to_threshold(samples, hit_count, 1e9f, spp - hit_count);
+    } else {
+      h_diff = depth_range(t_mn, t_mx, 1e9f, spp - hit_count, hist);
+      if (A.Q.ray_flags == 0) {  // no gradient pass follows: trace again for the bins
+        for (uint32_t s = 0; s < spp; s++) {
+          float2 d = rand2(x, y, s, A.seed);
+          v3 pos, dir;
+          eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
+          DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, A.bminmax, Qc, A.relax_eps, pos, dir, 1e9f, &py, st);
+          if (hit.primId != 0xFFFFFFFFu) hist[depth_bin(hit.t, t_mn, h_diff)]++;
+        }
+        A.depth_thr[pix] = depth_gap_threshold(hist, t_mn, h_diff);
+      }
+    }
+  }
   const float4 r = A.ref[pix];
   const float rr[4] = {r.x, r.y, r.z, r.w};
   float l = 0.0f;
@@ -159,8 +204,10 @@ __device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, S
     v3 pos, dir;
     eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
     DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, A.bminmax, Qg, A.relax_eps, pos, dir, 1e9f, &py, st);
+    if (streamed && hit.primId != 0xFFFFFFFFu) hist[depth_bin(hit.t, t_mn, h_diff)]++;
     if (SPEC != kDrNormal && hit.primId != 0xFFFFFFFFu) scatter_pd<SPEC>(A, py, dLdC);
   }
+  if (streamed && A.depth_thr) A.depth_thr[pix] = depth_gap_threshold(hist, t_mn, h_diff);
 }
 
 template <int SPEC>
@@ -404,7 +451,7 @@ __global__ __launch_bounds__(256) void k_dr_border_svm(DRArgs A, const uint32_t*
   const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const DRParams Q0 = {LRT_DR_RAY_FLAG_NO_DIFF, 0u}, Qb = {LRT_DR_RAY_FLAG_BORDER, A.Q.dtdv_fd};
   const uint32_t n_items = *count;
-  const uint32_t sq = (uint32_t)sqrt((double)max(1u, (uint32_t)(0.25f * (float)A.border_spp)));
+  const uint32_t sq = A.svm_side;
   const uint32_t svm_spp = sq * sq;
   const float i_sq = 1.0f / (float)sq;
   uint32_t local_hits = 0;

@@ -19,6 +19,7 @@ struct DRArgs {
   uint32_t mode, input_type, loss_fn, border_spp, forced_border;
   float relax_eps;
   uint32_t border_sampling;  // LRT_DR_BORDER_SAMPLING_RANDOM / _SVM
+  uint32_t svm_side;         // SVM samples per axis: uint(sqrt(max(1, uint(0.25 * border_spp)))), computed on the host
   float up[3];               // row 1 of worldView (CastBorderRaySVM, MultiRendererDR.cpp:1784)
   const float4* ref;   // reference image, alpha = mask
   float4* out_image;   // rendered colour + coverage

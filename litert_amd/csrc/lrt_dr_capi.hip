@@ -51,7 +51,7 @@ static int dr_check(Renderer* r, const MultiRendererDRPreset& p) {
     for (const LrtLight& l : r->lights)
       if (l.type == LRT_LIGHT_TYPE_POINT)
         return set_err(LRT_E_UNSUPPORTED, "DR LAMBERT supports direct and ambient lights (MultiRendererDR.cpp:925-928 asserts)");
-  if (p.spp < 1 || p.spp > 16) return set_err(LRT_E_INVALID, "DR spp must be 1..16");
+  if (p.spp < 1 || p.spp > 4096) return set_err(LRT_E_INVALID, "DR spp must be 1..4096");
   if (p.border_relax_eps <= 0.f) return set_err(LRT_E_INVALID, "border_relax_eps must be > 0");
   if (r->dr_n_ref == 0) return set_err(LRT_E_INVALID, "no reference images: call lrt_dr_set_reference");
   if ((p.render_width && p.render_width != r->dr_w) || (p.render_height && p.render_height != r->dr_h))
@@ -217,7 +217,12 @@ static int dr_finite_diff(Renderer* r, const MultiRendererDRPreset& p, const lrt
 }
 
 // one RenderDR call; gradient left in d_dr_grad (scaled)
-static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t view, uint32_t seed, int dtdv_mode, float* kernel_ms) {
+// d_loss (optional, default mode only): the view's loss sum over its pixels is
+// added to this device float and the call returns without waiting for the
+// GPU (no timing, no statistics, dr_last_loss unchanged), so a loop of steps
+// never stalls on the host.
+static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t view, uint32_t seed, int dtdv_mode, float* kernel_ms,
+                          float* d_loss = nullptr) {
   int c = dr_check(r, p);
   if (c) return c;
   if (view >= r->dr_n_ref) return set_err(LRT_E_INVALID, "view out of range");
@@ -265,6 +270,14 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
   A.forced_border = p.debug_forced_border ? 1u : 0u;
   A.relax_eps = p.border_relax_eps;
   A.border_sampling = p.dr_border_sampling;
+  // CastBorderRaySVM's sample grid side, uint(sqrt(svm_spp)) of the reference
+  // (MultiRendererDR.cpp:1664-1666): the exact integer square root
+  {
+    const uint32_t n = std::max(1u, (uint32_t)(0.25f * (float)p.border_spp));
+    uint32_t q = 0;
+    while ((uint64_t)(q + 1) * (q + 1) <= n) q++;
+    A.svm_side = q;
+  }
   for (int k = 0; k < 3; k++) A.up[k] = wv[4 * k + 1];  // worldView row 1, column-major storage
   A.lights = r->light_set();
   A.ref = (const float4*)r->d_dr_ref.p + np * view;
@@ -282,7 +295,9 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
   if (p.dr_diff_mode == LRT_DR_DIFF_MODE_FINITE_DIFF) {
     const float g_mult_fd = (float)(std::pow((double)P + 1.0, 2.0 / 3.0) / (double)(W * H));  // getBaseGradientMult
     double loss_sum = 0.0;
-    if ((c = dr_finite_diff(r, p, A, misc, blocks, P, g_mult_fd, &loss_sum)) != LRT_OK) return c;
+    lrtd::DRArgs Afd = A;
+    Afd.depth_thr = nullptr;  // no border search in this mode (MultiRendererDR.cpp:613), so no depth threshold
+    if ((c = dr_finite_diff(r, p, Afd, misc, blocks, P, g_mult_fd, &loss_sum)) != LRT_OK) return c;
     if ((e = hipEventRecord(r->ev1, r->stream)) != hipSuccess) return hip_err(e, "event");
     if ((e = hipStreamSynchronize(r->stream)) != hipSuccess) return hip_err(e, "dr finite differences");
     r->dr_last_loss = (float)(loss_sum / (double)np);
@@ -306,7 +321,8 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
     cull_primary = T.pairs_primary;
     cull_border = T.pairs_border;
   }
-  if ((e = launch_dr_primary(A, misc, (uint32_t*)r->d_spill.p, (float*)(misc + 4), blocks, r->stream, cull_primary)) != hipSuccess)
+  float* loss_out = d_loss ? d_loss : (float*)(misc + 4);
+  if ((e = launch_dr_primary(A, misc, (uint32_t*)r->d_spill.p, loss_out, blocks, r->stream, cull_primary)) != hipSuccess)
     return hip_err(e, "dr primary");
   if ((e = launch_dr_border_find(A, (uint32_t*)r->d_dr_list.p, misc + 1, r->stream)) != hipSuccess) return hip_err(e, "dr border");
   if (geometry_border) {
@@ -317,6 +333,8 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
   // getBaseGradientMult, MultiRendererDR.cpp:140-145
   const float g_mult = (float)(std::pow((double)P + 1.0, 2.0 / 3.0) / (double)(W * H));
   if ((e = launch_dr_scale((float*)r->d_dr_grad.p, P, g_mult, r->stream)) != hipSuccess) return hip_err(e, "dr scale");
+  r->dr_img_valid[view] = 1;
+  if (d_loss) return LRT_OK;
   if ((e = hipEventRecord(r->ev1, r->stream)) != hipSuccess) return hip_err(e, "event");
   uint32_t h_misc[6];
   if ((e = hipMemcpyAsync(h_misc, misc, sizeof(h_misc), hipMemcpyDeviceToHost, r->stream)) != hipSuccess) return hip_err(e, "stats");
@@ -325,7 +343,6 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
   std::memcpy(&loss, &h_misc[4], sizeof(float));
   r->dr_last_loss = (float)((double)loss / (double)np);
   r->dr_last_border = h_misc[1];
-  r->dr_img_valid[view] = 1;
   r->dr_last_border_hits = h_misc[2];
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, r->ev0, r->ev1);
@@ -411,6 +428,16 @@ int lrt_dr_optimize(lrt_handle h, const MultiRendererDRPreset* p, const SdfSBSHe
       return set_err(LRT_E_UNSUPPORTED, "redistancing needs a dense brick_count^3 grid whose first values are its lattice");
     if ((e = h->d_dr_redist.ensure((size_t)rp * rp * rp * 9)) != hipSuccess) return hip_err(e, "alloc redistancing");
   }
+  // Default mode: each step's loss is summed on the device into its own slot
+  // and the slots are read once after the loop, so the host never waits on a
+  // step. The finite-difference mode synchronises inside every render anyway.
+  const bool deferred = p->dr_diff_mode == LRT_DR_DIFF_MODE_DEFAULT;
+  float* d_losses = nullptr;
+  if (deferred) {
+    if ((e = h->d_dr_losses.ensure((size_t)p->opt_iterations * 4)) != hipSuccess) return hip_err(e, "alloc losses");
+    d_losses = (float*)h->d_dr_losses.p;
+    if ((e = zero_u32((uint32_t*)d_losses, p->opt_iterations, h->stream)) != hipSuccess) return hip_err(e, "clear");
+  }
   auto t0 = std::chrono::steady_clock::now();
   h->dr_last_batch = p->image_batch_size;
   for (uint32_t it = 0; it < p->opt_iterations; it++) {
@@ -418,10 +445,10 @@ int lrt_dr_optimize(lrt_handle h, const MultiRendererDRPreset* p, const SdfSBSHe
     for (uint32_t b = 0; b < p->image_batch_size; b++) {
       // MultiRendererDR.cpp:395-406; the gradient buffer is cleared per image as there
       uint32_t view = p->image_batch_size == h->dr_n_ref ? b : pcg_hash(seed ^ pcg_hash(it * 131u + b)) % h->dr_n_ref;
-      if ((c = dr_render_view(h, *p, view, seed, dtdv_mode, nullptr))) return c;
+      if ((c = dr_render_view(h, *p, view, seed, dtdv_mode, nullptr, deferred ? d_losses + it : nullptr))) return c;
       lsum += h->dr_last_loss;
     }
-    if (losses) losses[it] = (float)(lsum / p->image_batch_size);
+    if (losses && !deferred) losses[it] = (float)(lsum / p->image_batch_size);
     if (p->reg_function != LRT_DR_REG_FUNCTION_NONE) {
       // Regularization, MultiRendererDR.cpp:453-455: after the step's renders,
       // added to the gradient before the batch division. The reference walks
@@ -447,7 +474,22 @@ int lrt_dr_optimize(lrt_handle h, const MultiRendererDRPreset* p, const SdfSBSHe
   }
   if ((e = hipMemcpyAsync(values_f, h->d_values_f.p, (size_t)P * 4, hipMemcpyDeviceToHost, h->stream)) != hipSuccess)
     return hip_err(e, "download values_f");
+  std::vector<float> step_loss(deferred ? p->opt_iterations : 0);
+  uint32_t h_misc[3] = {0, 0, 0};  // the last view's counters: [1] border pixels, [2] border rays that found a point
+  if (deferred && (e = hipMemcpyAsync(step_loss.data(), d_losses, step_loss.size() * 4, hipMemcpyDeviceToHost, h->stream)) != hipSuccess)
+    return hip_err(e, "download losses");
+  if (deferred && (e = hipMemcpyAsync(h_misc, h->d_dr_misc.p, sizeof(h_misc), hipMemcpyDeviceToHost, h->stream)) != hipSuccess)
+    return hip_err(e, "download stats");
   if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_err(e, "optimize");
+  if (deferred) {
+    const double per = (double)h->dr_w * h->dr_h * p->image_batch_size;  // RenderDR's loss / (W*H), averaged over the batch
+    for (uint32_t it = 0; it < p->opt_iterations; it++) {
+      if (losses) losses[it] = (float)((double)step_loss[it] / per);
+    }
+    h->dr_last_loss = (float)((double)step_loss[p->opt_iterations - 1] / per);
+    h->dr_last_border = h_misc[1];
+    h->dr_last_border_hits = h_misc[2];
+  }
   h->values_f.assign(values_f, values_f + P);
   h->bfeat_valid = false;  // brick features follow the optimised distances
   if (total_ms) *total_ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -87,6 +87,7 @@ struct Renderer {
   // tree does not have the form the tables assume (the full tree is used)
   DevBuf d_dr_leaf_slot, d_dr_pair_slot, d_dr_tiled, d_dr_occ, d_dr_cull_pairs;
   DevBuf d_dr_redist;  // redistancing scratch: frozen flags + two lattice buffers
+  DevBuf d_dr_losses;  // per-step loss sums of lrt_dr_optimize
   // finite-difference DR: saved parameters, the referenced parameters, their
   // positions in the corner copy (CSR), and the loss pair of each
   DevBuf d_dr_fd_p0, d_dr_fd_ids, d_dr_fd_pos, d_dr_fd_acc;

@@ -8,11 +8,10 @@ The reference itself cannot be compiled here (its HydraCore3 submodule --
 LiteMath, CrossRT -- is empty and the Embree runtime is missing), so there is
 no oracle/_ref build; see lrt_oracle.h for the restated functions.
 
-Parity unpinned (restated as written, no independent check; details in
-lrt_oracle.h): LiteMath's camera/matrix conventions, Embree's BVH topology,
-the ANALYTIC cubic's R formula, the smoothed-normal blend weights of _IN
-layouts, the DR border sampling's statistics beyond seeded determinism, and
-LAMBERT's light direction to the last bit.
+Parity unpinned (only an import of the reference's third-party code could
+pin these; details in lrt_oracle.h): LiteMath's camera/matrix conventions
+(the GL convention used here is pinned against the closed-form pinhole),
+Embree's BVH topology, and LAMBERT's light direction to the last bit.
 """
 import ctypes as C
 import os

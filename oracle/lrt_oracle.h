@@ -10,21 +10,23 @@
  *
  * precision: 32 = FP32 twin of the reference, 64 = FP64 error reference.
  *
- * Parity unpinned (no check independent of the reference's source text fixes
- * these; they restate it as written):
- *  - LiteMath's perspectiveMatrix / lookAt / inverse4x4 conventions (the
- *    camera helpers use the GL convention and a double-cofactor inverse);
+ * Parity unpinned (only an import of the reference's third-party code could
+ * pin these; they restate the reference as written):
+ *  - LiteMath's perspectiveMatrix / lookAt / inverse4x4 conventions: the
+ *    camera helpers use the GL convention (pinned against the closed-form
+ *    pinhole, horizontal and vertical field of view, in
+ *    tests/test_cpu_oracle_pins.py), but whether LiteMath's differ is not;
  *  - Embree's BVH topology: the oracle traverses the BVH it is given, so which
  *    of two bricks hit along one ray "stop at first hit" returns follows that
  *    tree (bounded by the brute-force nearest-hit tests);
- *  - the ANALYTIC cubic's R = (2a^2 - 9ab + 27c) / 54 (BVH2Common.cpp:383, not
- *    the textbook 2a^3): range-checked and compared with the other root
- *    finders on smooth scenes only;
- *  - the smoothed-normal blend weights of ID32F_IRGB32F_IN scenes beyond the
- *    code path itself;
- *  - the DR border sampling's statistics beyond seeded determinism;
  *  - LAMBERT's light direction to the last bit (1/sqrt(3) rounded to float;
  *    LiteMath's normalize of (1,1,1) may differ by an ulp).
+ * Pinned by checks that do not restate the formulas: the ANALYTIC cubic with
+ * the reference's R = (2a^2 - 9ab + 27c) / 54 against FP64 bracketing where it
+ * equals Vieta's (a in {0, 1}); the smoothed-normal blend of
+ * ID32F_IRGB32F_IN scenes (plane normal on linear fields, continuity across
+ * voxel and brick faces); the DR border term against finite differences (the
+ * reference's test_position_derivatives, tests/test_cpu_border_fd.py).
  * The DR normal-mode derivatives (NORMAL, LAMBERT, both layouts) are pinned
  * by central differences of the FP64 loss, and the diffuse position term by
  * the same differences times the reference's sz*bs/2 scale.

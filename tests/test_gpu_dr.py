@@ -277,13 +277,16 @@ def _sparse(sbs, limit):
     return L.SdfSBS(sbs.header, sbs.nodes[keep], sbs.values, sbs.values_f.copy())
 
 
+@pytest.mark.parametrize("sampling", [0, 1], ids=["random", "svm"])
 @pytest.mark.parametrize("case", ["sparse", "inside_camera", "sparse_inside_camera"])
-def test_render_dr_tree_forms_match_fp32_twin(case):
+def test_render_dr_tree_forms_match_fp32_twin(case, sampling):
     """Linear-depth RenderDR where the ray casts cannot use every shortcut over
     the BVH: a sparse brick set (subtrees whose bricks do not fill their box),
-    and cameras inside the brick grid (boxes entered within 0.1 of the eye).
-    Image, border statistics and dLoss/dS against the FP32 twin, which walks
-    the whole tree and every voxel as the reference does."""
+    and cameras inside the brick grid (boxes entered within 0.1 of the eye),
+    with random and with SVM border sampling (whose sample rays take the
+    skipping linear-depth form on the border pass's BVH copy). Image, border
+    statistics and dLoss/dS against the FP32 twin, which walks the whole tree
+    and every voxel as the reference does."""
     tgt = L.grid_sbs("torus", 0.6, 0.2, 0.02, 3, seed=1, brick_count=8, brick_size=4)
     ini = L.grid_sbs("sphere", 0.5, 0.0, 0.0, 3, seed=1, brick_count=8, brick_size=4)
     if case.startswith("sparse"):
@@ -297,7 +300,8 @@ def test_render_dr_tree_forms_match_fp32_twin(case):
     p.render_mode = L.MULTI_RENDER_MODE_LINEAR_DEPTH
     refs = np.stack([r.RenderFloat(W, H, wv, pr, p) for wv, pr in cams])
     r.close()
-    dr = _preset(border_spp=64)
+    dr = _preset(border_spp=64 if sampling == 0 else 256)
+    dr.dr_border_sampling = sampling
     g = L.MultiRendererDR()
     g.SetScene(ini)
     wvs = np.stack([c[0] for c in cams])
@@ -320,22 +324,26 @@ def test_render_dr_tree_forms_match_fp32_twin(case):
         assert np.linalg.norm(grad - o32) / np.linalg.norm(o32) <= 1e-5
 
 
-@pytest.mark.parametrize("mode,smoothed,flags,spp", [(4, False, 2, 1), (4, True, 2, 1), (1, False, 3, 1), (1, True, 3, 1),
-                                                 (4, True, 2, 4), (1, False, 2, 1)])
-def test_render_dr_normal_modes_match_oracle(mode, smoothed, flags, spp):
+@pytest.mark.parametrize("mode,smoothed,flags,spp,sampling", [(4, False, 2, 1, 0), (4, True, 2, 1, 0), (1, False, 3, 1, 0),
+                                                          (1, True, 3, 1, 0), (4, True, 2, 4, 0), (1, False, 2, 1, 0),
+                                                          (1, False, 3, 1, 1), (4, True, 2, 1, 1)])
+def test_render_dr_normal_modes_match_oracle(mode, smoothed, flags, spp, sampling):
     """NORMAL (4) and LAMBERT (1, default lights) render modes: the hit normal
     from the hit voxel or smoothed over its 8 neighbours (_IN layout), its
     derivatives, dColor/dDiffuse and dColor/dNormal, colour reconstruction
-    (flags 3) and the colour-input border term. Image, border statistics and
-    dLoss/dS within 1e-5 of the FP32 twin; no further from FP64 than the twin."""
+    (flags 3) and the colour-input border term, with random (0) or SVM (1,
+    border_spp 256: 64 SVM samples per border pixel) border sampling. Image,
+    border statistics and dLoss/dS within 1e-5 of the FP32 twin; no further
+    from FP64 than the twin."""
     tgt = L.grid_sbs("torus", 0.6, 0.2, 0.02, 3, seed=1, brick_count=8, brick_size=4, with_neighbors=smoothed)
     ini = L.grid_sbs("sphere", 0.5, 0.0, 0.0, 3, seed=1, brick_count=8, brick_size=4, with_neighbors=smoothed)
     tgt, ini = _random_colours(tgt, 21), _random_colours(ini, 22)
     cams = [L.orbit_camera(3, i) for i in range(2)]
     wvs = np.stack([c[0] for c in cams])
     prs = np.stack([c[1] for c in cams])
-    dr = _preset(border_spp=64)
+    dr = _preset(border_spp=64 if sampling == 0 else 256)
     dr.dr_render_mode, dr.dr_input_type, dr.dr_reconstruction_flags, dr.spp = mode, 0, flags, spp
+    dr.dr_border_sampling = sampling
     pairs_t, _ = L.build_bvh(tgt)
     st = oracle.Scene(tgt, pairs_t)
     zero = np.zeros((H, W, 4), np.float32)
@@ -354,7 +362,7 @@ def test_render_dr_normal_modes_match_oracle(mode, smoothed, flags, spp):
     o_img, o32, st32 = sc.dr_render_grad(dr, wvs[view], prs[view], W, H, ref, P, seed=55, precision=32)
     _, o64, st64 = sc.dr_render_grad(dr, wvs[view], prs[view], W, H, ref, P, seed=55, precision=64)
     bp, bh = g.LastStats()
-    print(mode, smoothed, flags, spp, "loss", loss, st32, "border", bp, bh, "ms", ms)
+    print(mode, smoothed, flags, spp, sampling, "loss", loss, st32, "border", bp, bh, "ms", ms)
     assert np.mean(np.all(img == o_img.astype(np.float32), axis=-1)) > 0.999
     assert bp == st32["border_pixels"] and abs(bh - st32["border_rays_hit"]) <= 2
     assert abs(loss - st64["loss"]) <= 1e-5 * max(1.0, abs(st64["loss"]))
@@ -433,14 +441,20 @@ def test_adam_step_matches_oracle():
     X = ini.values_f.copy()
     V = np.zeros_like(X)
     S = np.zeros_like(X)
+    o_losses = []
     for it in range(3):
         sc.set_values_f(X)
-        _, gr, _ = sc.dr_render_grad(dr, wvs[0], prs[0], W, H, ref, X.size, seed=5, precision=32)
+        _, gr, st = sc.dr_render_grad(dr, wvs[0], prs[0], W, H, ref, X.size, seed=5, precision=32)
+        o_losses.append(st["loss"])
         gr = gr.astype(np.float32)
         oracle.lib().orc_adam_step(it, oracle._p(gr), oracle._p(X), oracle._p(V), oracle._p(S), X.size, dr.opt_lr, dr.opt_beta_1,
                                    dr.opt_beta_2, dr.opt_eps)
     assert np.abs(sbs.values_f - X).max() <= 1e-6
+    # per-step losses, summed on the device and read once after the loop
+    assert np.allclose(losses, o_losses, rtol=1e-5, atol=0), (losses, o_losses)
     assert losses[-1] <= losses[0]
+    bp, bh = g.LastStats()
+    assert bp > 0 and bh > 0
     del x0
 
 
