@@ -98,6 +98,21 @@ int lrt_get_execution_time(lrt_handle h, const char* name, float out[4]);
 int lrt_render_views_device(lrt_handle h, const float* world_views, const float* projs, uint32_t n_views, uint32_t width,
                             uint32_t height, int out_mode, float* kernel_ms);
 int lrt_download_view(lrt_handle h, uint32_t view, float* t, float* normal_oct, uint32_t* prim, float* rgba);
+
+/* No counterpart in the reference: lrt_render_views_device for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). It allocates nothing: its scratch
+ * (chunk counters, camera records, traversal-stack spill) is a 32 MiB
+ * __device__ array per GPU, held until the process ends, and the caller
+ * passes device output buffers of n_views * width * height pixels
+ * ([view][y][x]): t (float), normal_oct (2 floats), prim (uint32) and, for
+ * the shaded render modes, rgba (4 floats); any may be NULL, not all. At most
+ * LRT_NOALLOC_MAX_VIEWS views per call; the learned seeder must be off.
+ * Asynchronous on the handle's stream (lrt_sync waits). Every call on one
+ * GPU uses that GPU's one scratch array: calls that overlap on a GPU race. */
+#define LRT_NOALLOC_MAX_VIEWS 16
+int lrt_render_views_noalloc(lrt_handle h, const float* world_views, const float* projs, uint32_t n_views, uint32_t width,
+                             uint32_t height, float* d_t, float* d_normal_oct, uint32_t* d_prim, float* d_rgba);
 /* Same launch with per-ray work counters: out[0] node pairs, out[1] bricks, out[2] voxels. */
 int lrt_count_work(lrt_handle h, const float* world_views, const float* projs, uint32_t n_views, uint32_t width, uint32_t height,
                    uint64_t out[3]);

@@ -154,6 +154,7 @@ def _load():
         "lrt_get_execution_time": (i32, [P, C.c_char_p, P]),
         "lrt_render_views_device": (i32, [P, P, P, u32, u32, u32, i32, C.POINTER(f32)]),
         "lrt_download_view": (i32, [P, u32, P, P, P, P]),
+        "lrt_render_views_noalloc": (i32, [P, P, P, u32, u32, u32, P, P, P, P]),
         "lrt_count_work": (i32, [P, P, P, u32, u32, u32, P]),
         "lrt_sync": (i32, [P]),
         "lrt_set_launch_blocks": (i32, [P, u32]),
@@ -453,6 +454,15 @@ class MultiRenderer:
         _check(lib.lrt_render_views_device(self._h, _ptr(wv), _ptr(pr), wv.shape[0], width, height, out_mode,
                                            C.byref(ms) if timed else None), "RenderViewsDevice")
         return ms.value
+
+    def RenderViewsNoAlloc(self, world_views, projs, width, height, d_t=None, d_normal=None, d_prim=None, d_rgba=None):
+        """lrt_render_views_noalloc: renders into caller-owned device buffers
+        (integer device addresses, or None) without allocating; asynchronous
+        (Sync() waits)."""
+        wv = _f32(world_views).reshape(-1, 16)
+        pr = _f32(projs).reshape(-1, 16)
+        _check(lib.lrt_render_views_noalloc(self._h, _ptr(wv), _ptr(pr), wv.shape[0], width, height, d_t, d_normal, d_prim, d_rgba),
+               "RenderViewsNoAlloc")
 
     def DownloadView(self, view, width, height, rgba=False):
         t = np.zeros((height, width), np.float32)

@@ -17,8 +17,8 @@ LIB = os.path.join(HERE, "liblrt_hip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HOST_SOURCES = ["host/scene_gen.cpp", "host/bvh_build.cpp"]
-HIP_SOURCES = ["lrt_capi.hip", "lrt_dr_capi.hip", "lrt_mlp_capi.hip", "lrt_build_capi.hip", "kernels/render.hip", "kernels/dr.hip",
-               "kernels/mlp.hip", "kernels/build.hip"]
+HIP_SOURCES = ["lrt_capi.hip", "lrt_dr_capi.hip", "lrt_mlp_capi.hip", "lrt_build_capi.hip", "lrt_noalloc.hip", "kernels/render.hip",
+               "kernels/dr.hip", "kernels/mlp.hip", "kernels/build.hip"]
 
 # -ffp-contract=off: kernels evaluate every expression in the reference's
 # operation order, so the FP32 results can be compared bit for bit with a CPU
