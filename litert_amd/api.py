@@ -13,7 +13,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))  # litert_amd/
-LIB_PATH = os.path.join(_HERE, "liblrt_hip.so")
+# LRT_LIBRARY names another build of the same library (same C ABI), e.g. one
+# compiled with different occupancy settings; default: the in-tree build.
+LIB_PATH = os.environ.get("LRT_LIBRARY") or os.path.join(_HERE, "liblrt_hip.so")
 
 # ---- constants (include/lrt_types.h) ---------------------------------------
 DEVICE_CPU, DEVICE_GPU, DEVICE_GPU_RTX = 0, 1, 2

@@ -50,6 +50,11 @@ struct Hit {
   float t;
   uint32_t primId, instId, geomId;
   float coords[4];
+  // Deferred normal (DX layouts): the hit brick, the hit voxel packed as
+  // x | y << 8 | z << 16 (bit 31: negative start sign) and the hit point in
+  // voxel units, from which FinishNormal evaluates the normal once per ray.
+  uint32_t n_brick, n_voxel;
+  float n_p0[3];
 };
 
 // ---- raytrace_common.h:59-82 ----------------------------------------------
@@ -247,6 +252,16 @@ __device__ __forceinline__ bool cubic_above_on(float c0, float c1, float c2, flo
   return true;
 }
 
+// BVH2Common.cpp:574-602: central difference (h = 1e-3) of the trilinear at
+// p0 (voxel units), normalised, times the start sign.
+__device__ __forceinline__ v3 fd_normal(const float values[8], v3 p0, float start_sign) {
+  const float h = 0.001f;
+  float ddx = (trilinear(values, p0 + v3(h, 0, 0)) - trilinear(values, p0 + v3(-h, 0, 0))) / (2 * h);
+  float ddy = (trilinear(values, p0 + v3(0, h, 0)) - trilinear(values, p0 + v3(0, -h, 0))) / (2 * h);
+  float ddz = (trilinear(values, p0 + v3(0, 0, h)) - trilinear(values, p0 + v3(0, 0, -h))) / (2 * h);
+  return start_sign * normalize(v3(ddx, ddy, ddz));
+}
+
 // ---- BVH2Common.cpp:274-631 ------------------------------------------------
 // Returns true when *pHit was updated. d_inv = 1/d is passed in (same value
 // the reference computes per call).
@@ -258,7 +273,10 @@ __device__ __forceinline__ bool cubic_above_on(float c0, float c1, float c2, flo
 // cubic along the ray is shown to stay above the stopping threshold up to the
 // start, so the root it converges to is the first one either way. Any other mode, or a seed that fails the checks,
 // runs unseeded. n_iter (optional) accumulates root-finder iterations.
-template <bool SEEDED = false, int RF = -1>
+// DEFER: on a hit, record the hit point (voxel units) and the start sign in
+// pHit instead of evaluating the normal here (FinishNormal does it later
+// with the same operations); the caller records the voxel.
+template <bool SEEDED = false, int RF = -1, bool DEFER = false>
 __device__ __forceinline__ bool LocalSurfaceIntersection(const TraceParams& P, v3 ray_dir, const float values[8], uint32_t primId,
                                                          uint32_t geomType, float d, float d_inv, float qFar, float fNearX,
                                                          v3 start_q, Hit* pHit, float seed_q = -1.0f, uint32_t* n_iter = nullptr) {
@@ -458,24 +476,23 @@ __device__ __forceinline__ bool LocalSurfaceIntersection(const TraceParams& P, v
   if (n_iter) *n_iter += iter;
   float tReal = fNearX + d * t;
   if (t <= qFar && hit && tReal < pHit->t) {
-    v3 norm(0, 0, 1);
-    if (P.need_normal) {
-      v3 p0 = start_q + t * ray_dir;
-      const float h = 0.001f;
-      float ddx = (trilinear(values, p0 + v3(h, 0, 0)) - trilinear(values, p0 + v3(-h, 0, 0))) / (2 * h);
-      float ddy = (trilinear(values, p0 + v3(0, h, 0)) - trilinear(values, p0 + v3(0, -h, 0))) / (2 * h);
-      float ddz = (trilinear(values, p0 + v3(0, 0, h)) - trilinear(values, p0 + v3(0, 0, -h))) / (2 * h);
-      norm = start_sign * normalize(v3(ddx, ddy, ddz));
+    if (DEFER) {
+      const v3 p0 = start_q + t * ray_dir;
+      pHit->n_p0[0] = p0.x;
+      pHit->n_p0[1] = p0.y;
+      pHit->n_p0[2] = p0.z;
+      pHit->n_voxel = start_sign < 0.0f ? 0x80000000u : 0u;
+    } else {
+      const float2 en = P.need_normal ? encode_normal(fd_normal(values, start_q + t * ray_dir, start_sign)) : encode_normal(v3(0, 0, 1));
+      pHit->coords[2] = en.x;
+      pHit->coords[3] = en.y;
     }
-    float2 en = encode_normal(norm);
     pHit->t = tReal;
     pHit->primId = (P.render_mode == LRT_RENDER_MODE_ST_ITERATIONS) ? iter : primId;
     pHit->instId = 0;
     pHit->geomId = 0u | (geomType << LRT_SH_TYPE);
     pHit->coords[0] = 0;
     pHit->coords[1] = 0;
-    pHit->coords[2] = en.x;
-    pHit->coords[3] = en.y;
     return true;
   }
   return false;
@@ -491,10 +508,12 @@ __device__ __forceinline__ float step01(float e0, float e1, float x) { return cl
 // each candidate voxel hands it to the root finder in its own units.
 // RF >= 0 fixes the root finder at compile time; IDX = false compiles out the
 // indexed-layout paths (colour, smoothed normals) for DX-layout scenes.
-template <bool SEEDED = false, int RF = -1, bool IDX = true>
+// DEFER (DX layouts only): the hit's normal is left to FinishNormal.
+template <bool SEEDED = false, int RF = -1, bool IDX = true, bool DEFER = false>
 __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TraceParams& P, v3 ray_pos, v3 ray_dir, v3 rayDirInv,
                                                float tNear, uint32_t brick, Hit* pHit, uint32_t* n_vox, float seed_world = -1.0f,
                                                uint32_t* n_iter = nullptr) {
+  static_assert(!(DEFER && IDX), "the deferred normal covers DX layouts (geometry normals) only");
   float values[8];
   // Per-brick constants precomputed on the host with the same float
   // operations (brick_records): bmin = -1 + sz_inv * pos, sz_inv = 2/sz,
@@ -528,8 +547,12 @@ __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TracePar
       v3 start_q = (start_pos - min_pos) * qscale;
       float qFar = (fnf.y - fnf.x) * qscale;
       const float seed_q = SEEDED ? (seed_world - fnf.x) * d_inv : -1.0f;
-      bool updated =
-          LocalSurfaceIntersection<SEEDED, RF>(P, ray_dir, values, brick, S.type, d, d_inv, qFar, fnf.x, start_q, pHit, seed_q, n_iter);
+      bool updated = LocalSurfaceIntersection<SEEDED, RF, DEFER>(P, ray_dir, values, brick, S.type, d, d_inv, qFar, fnf.x, start_q, pHit,
+                                                                 seed_q, n_iter);
+      if (DEFER && updated) {
+        pHit->n_brick = brick;
+        pHit->n_voxel |= (uint32_t)vp.x | ((uint32_t)vp.y << 8) | ((uint32_t)vp.z << 16);
+      }
       if (IDX && updated && S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F_IN && P.normal_mode == LRT_NORMAL_MODE_SDF_SMOOTHED && P.need_normal) {
         // smoothed normal, BVH2Common.cpp:838-906
         const float beta = 0.5f;
@@ -588,6 +611,25 @@ __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TracePar
     pHit->coords[0] = col.x + col.y / 256.0f;
     pHit->coords[1] = col.z;
   }
+}
+
+// The normal of a hit recorded by BrickIntersect<..., DEFER = true>: the hit
+// voxel's 8 values are loaded again and the central difference is evaluated
+// at the recorded point, the same operations the voxel loop would have done.
+// Brick sizes up to 255 (the voxel coordinates are packed in bytes).
+__device__ __forceinline__ void FinishNormal(const DevScene& S, const TraceParams& P, Hit* pHit) {
+  if (pHit->primId == 0xFFFFFFFFu) return;
+  v3 norm(0, 0, 1);
+  if (P.need_normal) {
+    const uint32_t b = pHit->n_brick, vox = pHit->n_voxel;
+    const float4 r0 = S.brecs[3 * (size_t)b], r2 = S.brecs[3 * (size_t)b + 2];
+    float values[8];
+    load_voxel_q<false>(S, __float_as_uint(r0.w), vox & 0xFFu, (vox >> 8) & 0xFFu, (vox >> 16) & 0xFFu, r2.x, r2.y, values);
+    norm = fd_normal(values, v3(pHit->n_p0[0], pHit->n_p0[1], pHit->n_p0[2]), (vox & 0x80000000u) ? -1.0f : 1.0f);
+  }
+  const float2 en = encode_normal(norm);
+  pHit->coords[2] = en.x;
+  pHit->coords[3] = en.y;
 }
 
 // ---- camera, eye_ray.cpp:146-160, render_common.h:25-55 -------------------

@@ -70,7 +70,7 @@ __device__ __forceinline__ void BVH2Traverse(const DevScene& S, const TraceParam
       const float tNearSdf = mx(tNear, 0.1f);  // SDF_BIAS, BVH2Common.cpp:3472-3473
       const float tPrev = hit->t;
       if (COUNT) wc->bricks++;
-      BrickIntersect<false, RF, IDX>(S, P, ray_pos, ray_dir, inv, tNearSdf, left & LRT_START_MASK, hit, COUNT ? &wc->voxels : nullptr, -1.0f,
+      BrickIntersect<false, RF, IDX, !IDX>(S, P, ray_pos, ray_dir, inv, tNearSdf, left & LRT_START_MASK, hit, COUNT ? &wc->voxels : nullptr, -1.0f,
                      COUNT ? &wc->iters : nullptr);
       hitFound = hit->t < tPrev;
     }
@@ -92,6 +92,9 @@ __device__ __forceinline__ Hit RayQuery_NearestHit(const DevScene& S, const Trac
   hit.coords[0] = 1.0f; hit.coords[1] = 0.0f; hit.coords[2] = 0.0f; hit.coords[3] = 0.0f;
   const float2 bh = RayBox(pos, inv, v3(-1, -1, -1), v3(1, 1, 1));
   if ((bh.x <= bh.y) && (bh.y > tNear) && (bh.x < hit.t)) BVH2Traverse<COUNT, RF, IDX>(S, P, pos, dir, inv, tNear, stopOnFirstHit, &hit, st, wc);
+  // DX layouts: the hit normal, once per ray after the traversal (all hit
+  // lanes of the wave together rather than in every voxel step that hits)
+  if (!IDX) FinishNormal(S, P, &hit);
   if (hit.primId != 0xFFFFFFFFu) hit.instId = 0;
   return hit;
 }
