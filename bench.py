@@ -39,6 +39,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def measured_traffic(lod):
     """HBM bytes per plain-render launch from the FETCH_SIZE / WRITE_SIZE counter
     passes (scaled by a 1 GiB copy), recorded per scene size in
@@ -49,6 +60,58 @@ def measured_traffic(lod):
             return json.load(f)["traffic_bytes"]
     except (OSError, KeyError, ValueError):
         return None
+
+
+def reduce_counters(dirs, kernel, rays, calib_bytes, out_path):
+    """Reduces rocprofv3 --pmc passes (csv output, one pass per directory or
+    several in one tree) to the per-launch figures bench.py reports: the
+    average over the dispatches of `kernel` of every counter, FETCH_SIZE /
+    WRITE_SIZE scaled to bytes by what the same counters read for the HIP
+    runtime's device-to-device copy of `calib_bytes` (bench.py --calib-copy-mib),
+    the L2 hit rate, VALU lane utilisation, VALU instructions per 64-ray wave
+    and resident waves per CU. Writes the JSON bench.py reads as `traffic`."""
+    import csv
+    import glob
+    per = {}    # counter -> list of per-dispatch values of the kernel
+    calib = {}  # counter -> list of per-dispatch values of the copy kernel
+    names = set()
+    for d in dirs:
+        for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            acc = {}
+            with open(path, newline="") as f:
+                for row in csv.DictReader(f):
+                    name = row.get("Kernel_Name", "")
+                    key = (name, row.get("Dispatch_Id", ""), row["Counter_Name"])
+                    acc[key] = acc.get(key, 0.0) + float(row["Counter_Value"])
+            for (name, _, counter), v in acc.items():
+                if kernel in name:
+                    per.setdefault(counter, []).append(v)
+                    names.add(name)
+                elif "copyBuffer" in name:
+                    calib.setdefault(counter, []).append(v)
+    if not per:
+        raise SystemExit(f"no dispatch of a kernel matching {kernel!r} in {dirs}")
+    mean = {k: sum(v) / len(v) for k, v in per.items()}
+    out = {"kernel": sorted(names), "dispatches": {k: len(v) for k, v in per.items()}, "counters_per_dispatch": mean}
+    if "FETCH_SIZE" in mean and "WRITE_SIZE" in mean and "FETCH_SIZE" in calib and "WRITE_SIZE" in calib:
+        fs = calib_bytes / 1024.0 / max(calib["FETCH_SIZE"])   # KiB read by the copy vs KiB counted
+        ws = calib_bytes / 1024.0 / max(calib["WRITE_SIZE"])
+        rd, wr = mean["FETCH_SIZE"] * fs * 1024, mean["WRITE_SIZE"] * ws * 1024
+        out.update({"calibration": {"bytes": calib_bytes, "fetch_size_kib": max(calib["FETCH_SIZE"]),
+                                    "write_size_kib": max(calib["WRITE_SIZE"]), "fetch_scale": round(fs, 4),
+                                    "write_scale": round(ws, 4)},
+                    "hbm_read_bytes": int(rd), "hbm_write_bytes": int(wr), "traffic_bytes": int(rd + wr)})
+    if "TCC_HIT_sum" in mean and "TCC_MISS_sum" in mean:
+        out["l2_hit_rate"] = round(mean["TCC_HIT_sum"] / (mean["TCC_HIT_sum"] + mean["TCC_MISS_sum"]), 4)
+    if "SQ_THREAD_CYCLES_VALU" in mean and "SQ_ACTIVE_INST_VALU" in mean:
+        out["valu_lane_utilisation"] = round(mean["SQ_THREAD_CYCLES_VALU"] / (64.0 * mean["SQ_ACTIVE_INST_VALU"]), 4)
+    if "SQ_INSTS_VALU" in mean and rays:
+        out["valu_insts_per_wave_tile"] = round(mean["SQ_INSTS_VALU"] / (rays / 64.0), 1)
+    if "SQ_WAVE_CYCLES" in mean and "GRBM_GUI_ACTIVE" in mean:
+        out["waves_per_cu"] = round(4.0 * mean["SQ_WAVE_CYCLES"] / mean["GRBM_GUI_ACTIVE"] / 256.0, 2)
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out, indent=1))
 
 
 def plain_scene(L, lod=128):
@@ -242,9 +305,16 @@ def main():
     ap.add_argument("--mlp-scenes", type=int, default=32)
     ap.add_argument("--mlp-events", type=int, default=16_000_000)
     ap.add_argument("--mlp-epochs", type=int, default=3)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU work for the baseline sample")
     ap.add_argument("--calib-copy-mib", type=int, default=0, help="append a device copy of this size (HBM counter calibration)")
+    ap.add_argument("--reduce-counters", nargs="+", metavar="DIR",
+                    help="reduce rocprofv3 --pmc csv passes found under DIR(s) instead of running (see reduce_counters)")
+    ap.add_argument("--kernel", default="k_render<false, false, 2, false>", help="kernel-name substring for --reduce-counters")
+    ap.add_argument("--out", default=None, help="output JSON of --reduce-counters")
     args = ap.parse_args()
+    if args.reduce_counters:
+        reduce_counters(args.reduce_counters, args.kernel, N_VIEWS * W * H, args.calib_copy_mib << 20,
+                        args.out or os.path.join(ROOT, "profiles", f"k_render_ico{4 * args.lod}_hbm.json"))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): the library renders on a single MI355X "
               "and sharding across GPUs is not built.", file=sys.stderr)
@@ -379,37 +449,51 @@ def main():
     cpu = None
     if not args.no_cpu:
         import oracle
-        sc = oracle.Scene(sbs, pairs)
+        # the oracle's CPU-baseline build (-O3, AVX2/FMA, no FP contraction);
+        # protocol: one warm-up pass, then the median of 5 timed passes
+        sc = oracle.Scene(sbs, pairs, baseline=True)
         nthreads = min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "0")) or 1 << 30)
-        sc.render(wvs[0], prs[0], p, 256, 256, precision=32, threads=nthreads)  # warm-up
-        reps, tcpu = 0, 0.0
-        tc = time.perf_counter()
-        while tcpu < args.cpu_seconds and reps < 64:
+        model = cpu_model()
+
+        def full_pass(threads):
+            t = time.perf_counter()
             for v in range(N_VIEWS):
-                sc.render(wvs[v], prs[v], p, W, H, precision=32, threads=nthreads)
-            reps += 1
-            tcpu = time.perf_counter() - tc
-        cpu_mrays = reps * rays / tcpu / 1e6
-        t1 = time.perf_counter()
-        sc.render(wvs[0], prs[0], p, 1024, 1024, precision=32, threads=1)
-        one_thread = 1024 * 1024 / (time.perf_counter() - t1) / 1e6
+                sc.render(wvs[v], prs[v], p, W, H, precision=32, threads=threads)
+            return time.perf_counter() - t
+
+        full_pass(nthreads)  # warm-up
+        times = sorted(full_pass(nthreads) for _ in range(5))
+        cpu_mrays = rays / times[2] / 1e6
+        one = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            sc.render(wvs[0], prs[0], p, 1024, 1024, precision=32, threads=1)
+            one.append(time.perf_counter() - t1)
+        one_thread = 1024 * 1024 / sorted(one)[1] / 1e6
         cpu = {"value": round(cpu_mrays, 2), "unit": "Mrays/s", "cores": nthreads, "kind": "port",
-               "sample": f"the full workload (8 views x {W}x{H}, same scene, cameras, Newton preset) repeated {reps}x "
-                         f"({tcpu:.1f}s), FP32 oracle, OpenMP {nthreads} threads; 1 thread: {one_thread:.2f} Mrays/s on one "
-                         f"1024x1024 view"}
-        log(f"cpu baseline: {cpu_mrays:.2f} Mrays/s on {nthreads} threads, {one_thread:.2f} on 1 thread")
+               "sample": f"the full workload (8 views x {W}x{H}, same scene, cameras, Newton preset): median of 5 passes after one "
+                         f"warm-up ({', '.join(f'{t:.2f}' for t in times)} s), FP32 oracle built -O3 -march=x86-64-v3, OpenMP "
+                         f"{nthreads} threads on {model}; 1 thread: {one_thread:.2f} Mrays/s (median of 3 on one 1024x1024 view)"}
+        log(f"cpu baseline: {cpu_mrays:.2f} Mrays/s on {nthreads} threads ({model}), {one_thread:.2f} on 1 thread")
         if dr_info is not None:
             tgt, ini, dcams, refs, dr = dr_problem(L, np, views=1)
             ip, _ = L.build_bvh(ini)
-            si = oracle.Scene(ini, ip)
+            si = oracle.Scene(ini, ip, baseline=True)
             ref = refs[0].copy()
             ref[..., 3] = np.linalg.norm(ref[..., :3], axis=-1) >= 0.001
-            t2 = time.perf_counter()
-            si.dr_render_grad(dr, dcams[0][0], dcams[0][1], refs.shape[2], refs.shape[1], ref, ini.values_f.size, seed=1,
-                              precision=32, threads=nthreads)
-            dt = time.perf_counter() - t2
+
+            def dr_pass():
+                t2 = time.perf_counter()
+                si.dr_render_grad(dr, dcams[0][0], dcams[0][1], refs.shape[2], refs.shape[1], ref, ini.values_f.size, seed=1,
+                                  precision=32, threads=nthreads)
+                return time.perf_counter() - t2
+
+            dr_pass()  # warm-up
+            dts = sorted(dr_pass() for _ in range(5))
+            dt = dts[2]
             dr_info["cpu_baseline_mrays_s"] = round(refs.shape[1] * refs.shape[2] / dt / 1e6, 3)
-            dr_info["cpu_baseline_sample"] = f"one RenderDR view (fwd+bwd incl. border rays), FP32 oracle, {nthreads} threads, {dt:.2f}s"
+            dr_info["cpu_baseline_sample"] = (f"one RenderDR view (fwd+bwd incl. border rays), FP32 oracle baseline build, {nthreads} "
+                                              f"threads, median of 5 after a warm-up: {dt:.3f}s")
             log(f"dr cpu baseline: {dr_info['cpu_baseline_mrays_s']} Mrays/s")
 
     if args.calib_copy_mib:

@@ -57,7 +57,7 @@ __device__ __forceinline__ void render_pixel(const DevScene& S, const TraceParam
         for (uint32_t li = 0; li < A.lights.n; li++) {
           v3 o, ld;
           if (!shadow_ray(A.lights, li, pos, dir, hit.t, o, ld)) continue;
-          const Hit sh = RayQuery_NearestHit<COUNT, RF, IDX>(S, P, o, 0.0f, ld, 1e9f, st, &wc);
+          const Hit sh = RayQuery_NearestHit<COUNT, RF, IDX, false>(S, P, o, 0.0f, ld, 1e9f, st, &wc);
           if (sh.primId != 0xFFFFFFFFu) lit &= ~(1u << li);
         }
       }

@@ -79,8 +79,9 @@ __device__ __forceinline__ void BVH2Traverse(const DevScene& S, const TraceParam
   }
 }
 
-// BVH2Common.cpp:3490-3559 for the single identity instance.
-template <bool COUNT, int RF = -1, bool IDX = true>
+// BVH2Common.cpp:3490-3559 for the single identity instance. NORMAL = false
+// leaves the hit normal out (shadow rays use only hit / miss).
+template <bool COUNT, int RF = -1, bool IDX = true, bool NORMAL = true>
 __device__ __forceinline__ Hit RayQuery_NearestHit(const DevScene& S, const TraceParams& P, v3 pos, float tNear, v3 dir, float tFar,
                                                    Stack& st, WorkCount* wc) {
   bool stopOnFirstHit = (tFar <= 0.0f);
@@ -94,7 +95,7 @@ __device__ __forceinline__ Hit RayQuery_NearestHit(const DevScene& S, const Trac
   if ((bh.x <= bh.y) && (bh.y > tNear) && (bh.x < hit.t)) BVH2Traverse<COUNT, RF, IDX>(S, P, pos, dir, inv, tNear, stopOnFirstHit, &hit, st, wc);
   // DX layouts: the hit normal, once per ray after the traversal (all hit
   // lanes of the wave together rather than in every voxel step that hits)
-  if (!IDX) FinishNormal(S, P, &hit);
+  if (!IDX && NORMAL) FinishNormal(S, P, &hit);
   if (hit.primId != 0xFFFFFFFFu) hit.instId = 0;
   return hit;
 }

@@ -22,72 +22,89 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD_DIR = os.path.join(_HERE, "_build")
 LIB_PATH = os.path.join(BUILD_DIR, "liblrt_oracle.so")
+BASELINE_LIB_PATH = os.path.join(BUILD_DIR, "liblrt_oracle_baseline.so")
 SOURCES = ["oracle_api.cpp", "oracle_dr.cpp", "oracle_mlp.cpp"]
 # -ffp-contract=off matches the reference's ISO C++17 build (no FP contraction).
 FLAGS = ["-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC", "-shared",
          "-Wall", "-Wno-unused-function"]
+# The CPU-baseline build that bench.py times: optimised for speed with the
+# AVX2 / FMA instruction set every x86-64 host of this pipeline has, still
+# without FP contraction or fast-math, so it computes the same FP32 results.
+BASELINE_FLAGS = ["-O3", "-march=x86-64-v3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC", "-shared",
+                  "-Wall", "-Wno-unused-function"]
+
+
+def _compile(flags, out):
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    srcs = [os.path.join(_HERE, s) for s in SOURCES if os.path.exists(os.path.join(_HERE, s))]
+    tmp = out + ".tmp"
+    r = subprocess.run(["g++", *flags, *srcs, "-o", tmp], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout)
+    os.replace(tmp, out)
+    return out
 
 
 def build():
-    os.makedirs(BUILD_DIR, exist_ok=True)
-    srcs = [os.path.join(_HERE, s) for s in SOURCES if os.path.exists(os.path.join(_HERE, s))]
-    tmp = LIB_PATH + ".tmp"
-    r = subprocess.run(["g++", *FLAGS, *srcs, "-o", tmp], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("oracle build failed:\n" + r.stdout)
-    os.replace(tmp, LIB_PATH)
-    return LIB_PATH
+    """Builds the checker (liblrt_oracle.so) and the CPU-baseline build; returns the checker's path."""
+    _compile(BASELINE_FLAGS, BASELINE_LIB_PATH)
+    return _compile(FLAGS, LIB_PATH)
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def _load(path):
+    L = C.CDLL(path)
+    P = C.c_void_p
+    u32, u64, i32, f32, f64 = C.c_uint32, C.c_uint64, C.c_int, C.c_float, C.c_double
+    sig = {
+        "orc_scene_create": (P, [P, P, u32, P, u64, P, u64, P, u32]),
+        "orc_scene_destroy": (None, [P]),
+        "orc_scene_set_values_f": (i32, [P, P, u64]),
+        "orc_scene_set_lights": (i32, [P, P, u32]),
+        "orc_render": (i32, [P, P, P, P, u32, u32, u32, i32, i32, i32, P, P, P, P, P]),
+        "orc_cast_rays": (i32, [P, P, P, u32, i32, i32, i32, P]),
+        "orc_brute_nearest": (i32, [P, P, P, u32, i32, i32, P, P]),
+        "orc_eval_trilinear": (f64, [P, P, i32]),
+        "orc_eval_trilinear_diff": (None, [P, P, i32, P]),
+        "orc_voxel_root": (i32, [u32, P, P, P, f64, f64, i32, P, P]),
+        "orc_encode_normal": (None, [P, i32, P]),
+        "orc_decode_normal": (None, [P, i32, P]),
+        "orc_pcg3d": (None, [P]),
+        "orc_rand2": (None, [u32, u32, u32, u32, P]),
+        "orc_pack_xy": (None, [u32, u32, P]),
+        "orc_safe_inverse": (None, [P, i32, P]),
+        "orc_ray_box": (None, [P, P, P, P, i32, P]),
+        "orc_load_voxel": (f64, [P, u32, u32, u32, u32, i32, P]),
+        "orc_eye_ray": (None, [P, P, u32, u32, f32, f32, u32, u32, i32, P]),
+    }
+    optional = {
+        "orc_dr_render_grad": (i32, [P, P, P, P, u32, u32, u32, P, i32, i32, i32, P, P, P]),
+        "orc_adam_step": (None, [u32, P, P, P, P, u32, f32, f32, f32, f32]),
+        "orc_dr_regularize": (i32, [P, u32, f32, f32, P]),
+        "orc_dr_redistance": (i32, [P, u32, f32, u32]),
+        "orc_mlp_forward": (None, [P, P, u32, i32, P]),
+        "orc_mlp_loss_grad": (f64, [P, P, P, u32, i32, P]),
+        "orc_mlp_brick_features": (i32, [P, u32, P]),
+        "orc_mlp_features": (i32, [P, P, u32, P, P]),
+    }
+    for name, (res, args) in list(sig.items()) + list(optional.items()):
+        if name in optional and not hasattr(L, name):
+            continue
+        f = getattr(L, name)
+        f.restype, f.argtypes = res, args
+    return L
+
+
+def lib(baseline=False):
+    """The checker build, or (baseline=True) the CPU-baseline build bench.py times."""
+    path = BASELINE_LIB_PATH if baseline else LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB_PATH)
-        P = C.c_void_p
-        u32, u64, i32, f32, f64 = C.c_uint32, C.c_uint64, C.c_int, C.c_float, C.c_double
-        sig = {
-            "orc_scene_create": (P, [P, P, u32, P, u64, P, u64, P, u32]),
-            "orc_scene_destroy": (None, [P]),
-            "orc_scene_set_values_f": (i32, [P, P, u64]),
-            "orc_scene_set_lights": (i32, [P, P, u32]),
-            "orc_render": (i32, [P, P, P, P, u32, u32, u32, i32, i32, i32, P, P, P, P, P]),
-            "orc_cast_rays": (i32, [P, P, P, u32, i32, i32, i32, P]),
-            "orc_brute_nearest": (i32, [P, P, P, u32, i32, i32, P, P]),
-            "orc_eval_trilinear": (f64, [P, P, i32]),
-            "orc_eval_trilinear_diff": (None, [P, P, i32, P]),
-            "orc_voxel_root": (i32, [u32, P, P, P, f64, f64, i32, P, P]),
-            "orc_encode_normal": (None, [P, i32, P]),
-            "orc_decode_normal": (None, [P, i32, P]),
-            "orc_pcg3d": (None, [P]),
-            "orc_rand2": (None, [u32, u32, u32, u32, P]),
-            "orc_pack_xy": (None, [u32, u32, P]),
-            "orc_safe_inverse": (None, [P, i32, P]),
-            "orc_ray_box": (None, [P, P, P, P, i32, P]),
-            "orc_load_voxel": (f64, [P, u32, u32, u32, u32, i32, P]),
-            "orc_eye_ray": (None, [P, P, u32, u32, f32, f32, u32, u32, i32, P]),
-        }
-        optional = {
-            "orc_dr_render_grad": (i32, [P, P, P, P, u32, u32, u32, P, i32, i32, i32, P, P, P]),
-            "orc_adam_step": (None, [u32, P, P, P, P, u32, f32, f32, f32, f32]),
-            "orc_dr_regularize": (i32, [P, u32, f32, f32, P]),
-            "orc_dr_redistance": (i32, [P, u32, f32, u32]),
-            "orc_mlp_forward": (None, [P, P, u32, i32, P]),
-            "orc_mlp_loss_grad": (f64, [P, P, P, u32, i32, P]),
-            "orc_mlp_brick_features": (i32, [P, u32, P]),
-            "orc_mlp_features": (i32, [P, P, u32, P, P]),
-        }
-        for name, (res, args) in list(sig.items()) + list(optional.items()):
-            if name in optional and not hasattr(L, name):
-                continue
-            f = getattr(L, name)
-            f.restype, f.argtypes = res, args
-        _lib = L
-    return _lib
+        _libs[path] = _load(path)
+    return _libs[path]
 
 
 class Counters(C.Structure):
@@ -107,19 +124,20 @@ def _p(a):
 class Scene:
     """Oracle copy of an SBS scene plus the fat BVH2 pairs (uint32 (n,16))."""
 
-    def __init__(self, sbs, pairs):
+    def __init__(self, sbs, pairs, baseline=False):
+        self._L = lib(baseline)
         self._keep = (np.ascontiguousarray(sbs.nodes, np.uint32), np.ascontiguousarray(sbs.values, np.uint32),
                       np.ascontiguousarray(sbs.values_f, np.float32), np.ascontiguousarray(pairs, np.uint32))
         n, v, vf, pr = self._keep
-        self.h = lib().orc_scene_create(C.byref(sbs.header), _p(n), n.shape[0], _p(v), v.size,
-                                        _p(vf) if vf.size else None, vf.size, _p(pr), pr.shape[0])
+        self.h = self._L.orc_scene_create(C.byref(sbs.header), _p(n), n.shape[0], _p(v), v.size,
+                                          _p(vf) if vf.size else None, vf.size, _p(pr), pr.shape[0])
         if not self.h:
             raise ValueError("orc_scene_create rejected the scene")
 
     def __del__(self):
         try:
             if self.h:
-                lib().orc_scene_destroy(self.h)
+                self._L.orc_scene_destroy(self.h)
         except Exception:
             pass
 
@@ -132,7 +150,7 @@ class Scene:
         cnt = Counters() if counters else None
         wv = np.ascontiguousarray(world_view, np.float32)
         pr = np.ascontiguousarray(proj, np.float32)
-        rc = lib().orc_render(self.h, _p(wv), _p(pr), C.byref(preset), W, H, seed, precision, threads, int(force_normal),
+        rc = self._L.orc_render(self.h, _p(wv), _p(pr), C.byref(preset), W, H, seed, precision, threads, int(force_normal),
                               _p(col), _p(t), _p(prim), _p(n), C.byref(cnt) if cnt is not None else None)
         if rc != 0:
             raise ValueError(f"orc_render failed {rc}")
@@ -146,7 +164,7 @@ class Scene:
     def cast_rays(self, preset, rays, precision=32, threads=0, force_normal=True):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
         out = np.zeros((rays.shape[0], 6), np.float64)
-        rc = lib().orc_cast_rays(self.h, C.byref(preset), _p(rays), rays.shape[0], precision, threads, int(force_normal),
+        rc = self._L.orc_cast_rays(self.h, C.byref(preset), _p(rays), rays.shape[0], precision, threads, int(force_normal),
                                  _p(out))
         if rc != 0:
             raise ValueError("orc_cast_rays failed")
@@ -156,33 +174,33 @@ class Scene:
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
         t = np.zeros(rays.shape[0], np.float64)
         prim = np.zeros(rays.shape[0], np.uint32)
-        rc = lib().orc_brute_nearest(self.h, C.byref(preset), _p(rays), rays.shape[0], precision, threads, _p(t), _p(prim))
+        rc = self._L.orc_brute_nearest(self.h, C.byref(preset), _p(rays), rays.shape[0], precision, threads, _p(t), _p(prim))
         if rc != 0:
             raise ValueError("orc_brute_nearest failed")
         return t, prim
 
     def load_voxel(self, node, vx, vy, vz, precision=32):
         out = np.zeros(8, np.float64)
-        vmin = lib().orc_load_voxel(self.h, node, vx, vy, vz, precision, _p(out))
+        vmin = self._L.orc_load_voxel(self.h, node, vx, vy, vz, precision, _p(out))
         return out, vmin
 
     def set_lights(self, lights):
         """lights: a list of litert_amd.api.Light (same layout as LrtLight)."""
         import ctypes
         arr = (type(lights[0]) * len(lights))(*lights) if lights else None
-        if lib().orc_scene_set_lights(self.h, arr, len(lights)) != 0:
+        if self._L.orc_scene_set_lights(self.h, arr, len(lights)) != 0:
             raise ValueError("bad lights")
 
     def set_values_f(self, vf):
         vf = np.ascontiguousarray(vf, np.float32)
         self._keep = self._keep[:2] + (vf,) + self._keep[3:]
-        if lib().orc_scene_set_values_f(self.h, _p(vf), vf.size) != 0:
+        if self._L.orc_scene_set_values_f(self.h, _p(vf), vf.size) != 0:
             raise ValueError("size mismatch")
 
     def dr_regularize(self, n_nodes, lam, power, n_params):
         """Regularization gradient (MultiRendererDR.cpp:1933-2057) of bricks [0, n_nodes)."""
         g = np.zeros(n_params, np.float64)
-        if lib().orc_dr_regularize(self.h, n_nodes, lam, power, _p(g)) != 0:
+        if self._L.orc_dr_regularize(self.h, n_nodes, lam, power, _p(g)) != 0:
             raise ValueError("orc_dr_regularize failed")
         return g
 
@@ -193,7 +211,7 @@ class Scene:
         t0 = np.zeros(rays.shape[0], np.float32)
         for i in range(rays.shape[0]):
             t = C.c_float()
-            if lib().orc_mlp_features(self.h, _p(rays[i]), int(bricks[i]), _p(out[i]), C.byref(t)) != 0:
+            if self._L.orc_mlp_features(self.h, _p(rays[i]), int(bricks[i]), _p(out[i]), C.byref(t)) != 0:
                 raise ValueError("orc_mlp_features failed")
             t0[i] = t.value
         return out, t0
@@ -206,7 +224,7 @@ class Scene:
         wv = np.ascontiguousarray(world_view, np.float32)
         pr = np.ascontiguousarray(proj, np.float32)
         ref = np.ascontiguousarray(ref, np.float32)
-        rc = lib().orc_dr_render_grad(self.h, C.byref(dr_preset), _p(wv), _p(pr), W, H, seed, _p(ref), precision, threads,
+        rc = self._L.orc_dr_render_grad(self.h, C.byref(dr_preset), _p(wv), _p(pr), W, H, seed, _p(ref), precision, threads,
                                       dtdv_mode, _p(img), _p(grad), C.byref(st))
         if rc != 0:
             raise ValueError(f"orc_dr_render_grad failed {rc}")

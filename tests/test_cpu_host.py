@@ -4,6 +4,7 @@ import ctypes as C
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -132,3 +133,54 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_oracle_baseline_build_computes_the_same_fp32_results():
+    """bench.py times the oracle's -O3 / AVX2 build; with FP contraction off it
+    must render exactly what the checker build renders."""
+    import oracle
+    from helpers import preset
+    sbs = L.displaced_icosphere(3, 0.8, 0.05, 3, seed=4, lod=16, brick_size=4, brick_pad=0, bytes_per_value=1)
+    pairs, _ = L.build_bvh(sbs)
+    wv, pr = L.orbit_camera(3, 1)
+    p = preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON)
+    a = oracle.Scene(sbs, pairs).render(wv, pr, p, 64, 64, precision=32)
+    b = oracle.Scene(sbs, pairs, baseline=True).render(wv, pr, p, 64, 64, precision=32)
+    assert (a["prim"] != 0xFFFFFFFF).sum() > 200
+    for k in ("t", "n", "prim"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_counter_reduction_averages_dispatches_and_calibrates(tmp_path):
+    """bench.py --reduce-counters: per-dispatch averages of the named kernel's
+    counters, FETCH/WRITE_SIZE scaled by the copy kernel's reading of a known
+    byte count, and the derived L2 hit rate and lane utilisation."""
+    import csv
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cols = ["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"]
+    rows = [(1, "__amd_rocclr_copyBuffer", "FETCH_SIZE", 512.0), (1, "__amd_rocclr_copyBuffer", "WRITE_SIZE", 1024.0),
+            (2, "void lrtd::k_render<false, false, 2, false>(x)", "FETCH_SIZE", 10.0),
+            (2, "void lrtd::k_render<false, false, 2, false>(x)", "WRITE_SIZE", 30.0),
+            (3, "void lrtd::k_render<false, false, 2, false>(x)", "FETCH_SIZE", 20.0),
+            (3, "void lrtd::k_render<false, false, 2, false>(x)", "WRITE_SIZE", 50.0),
+            (2, "void lrtd::k_render<false, false, 2, false>(x)", "TCC_HIT_sum", 3.0),
+            (2, "void lrtd::k_render<false, false, 2, false>(x)", "TCC_MISS_sum", 1.0),
+            (2, "void lrtd::k_render<false, false, 2, false>(x)", "SQ_THREAD_CYCLES_VALU", 32.0),
+            (2, "void lrtd::k_render<false, false, 2, false>(x)", "SQ_ACTIVE_INST_VALU", 1.0),
+            (4, "lrtd::k_zero_u32(unsigned int*, unsigned int)", "FETCH_SIZE", 99.0)]
+    d = tmp_path / "pass"
+    d.mkdir()
+    with open(d / "pmc_counter_collection.csv", "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(cols)
+        w.writerows(rows)
+    out = tmp_path / "out.json"
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--reduce-counters", str(d), "--calib-copy-mib", "1",
+                        "--out", str(out)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    import json
+    j = json.load(open(out))
+    # the 1 MiB copy read 1024 KiB while FETCH_SIZE counted 512: scale 2; writes scale 1
+    assert j["calibration"]["fetch_scale"] == 2.0 and j["calibration"]["write_scale"] == 1.0
+    assert j["hbm_read_bytes"] == int(15.0 * 2 * 1024) and j["hbm_write_bytes"] == int(40.0 * 1024)
+    assert j["l2_hit_rate"] == 0.75 and j["valu_lane_utilisation"] == 0.5
