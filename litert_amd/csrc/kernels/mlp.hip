@@ -4,7 +4,7 @@
 //   k_brick_features  per-brick corner / octant-minimum table
 //   k_mlp_collect     training events from the plain traversal
 //   k_mlp_infer       BF16 MFMA inference over a feature batch
-//   k_mlp_grad        FP32 forward + backward, per-block partial gradients
+//   k_mlp_grad        FP32 forward + backward on the matrix cores, per-block partial gradients
 //   k_mlp_reduce_adam deterministic gradient reduction + Adam
 //   k_render_seeded   primary rays with the network consulted at every brick
 #include "mlp.hpp"
@@ -157,174 +157,307 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const void* __restrict__ pack
 }
 
 // --------------------------------------------------------------- training --
-// One 64-lane block = 64 samples in FP32. Activations live in LDS as
-// [unit][sample] rows of 65 floats (the odd stride keeps both the per-lane
-// column walks and the per-parameter row sums free of bank conflicts).
-// Weights are read with wave-uniform addresses.
-constexpr int kRS = 65;
+// FP32 forward + backward of one 64-sample minibatch slice per 256-lane block
+// on the FP32 matrix cores (v_mfma_f32_16x16x4_f32, FP32 operands and
+// accumulation). Every layer is a GEMM over the block's 64 samples: wave w
+// owns units 16w .. 16w + 15 of each hidden layer, so the forward output
+// rows, the weight-gradient rows and the back-propagated rows of a wave are
+// all its own 16 units, while the other operand (activations, dLoss/dz) is
+// read from LDS, where every layer's tiles are [unit][sample] rows of kGS
+// floats. The block writes its partial gradient (summed over its samples)
+// and loss, and k_mlp_reduce_adam sums the partials in block order.
+//
+// 16x16x4 operand layout: lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16], and holds D[4 (l / 16) + r][l % 16], r = 0..3.
+constexpr int kGS = 65;  // LDS row stride: the MFMA operand walks touch 16 distinct banks
 
-__device__ __forceinline__ float silu_acc(float z) { return z / (1.0f + expf(-z)); }
+// SiLU and its derivative with the hardware exp2 and reciprocal (about 1 ulp
+// each; the gradient is held to 1e-4 of the FP64 oracle, not bit for bit).
+__device__ __forceinline__ float sigmoid_hw(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504f * z));
+}
+__device__ __forceinline__ float silu_acc(float z) { return z * sigmoid_hw(z); }
 __device__ __forceinline__ float dsilu_acc(float z) {
-  const float sg = 1.0f / (1.0f + expf(-z));
+  const float sg = sigmoid_hw(z);
   return sg * (1.0f + z * (1.0f - sg));
 }
 
-// partial[o*cols + i] = sum_s D[o][s] * X[i][s]; bias[o] = sum_s D[o][s]
-__device__ __forceinline__ void block_outer(const float* D, const float* X, int rows, int cols, float* __restrict__ gW,
-                                            float* __restrict__ gB, int lane) {
-  for (int p = lane; p < rows * cols; p += 64) {
-    const int o = p / cols, i = p - o * cols;
-    float acc = 0.0f;
-    for (int s = 0; s < 64; s++) acc += D[o * kRS + s] * X[i * kRS + s];
-    gW[p] = acc;
-  }
-  for (int o = lane; o < rows; o += 64) {
-    float acc = 0.0f;
-    for (int s = 0; s < 64; s++) acc += D[o * kRS + s];
-    gB[o] = acc;
-  }
+__device__ __forceinline__ f32x4 mfma_f32(float a, float b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+// sample slot s of minibatch slice `blk`: its index into the dataset, or -1
+__device__ __forceinline__ int64_t train_sample(const MlpTrainArgs& T, uint32_t blk, int s) {
+  const uint64_t local = (uint64_t)blk * 64 + s;
+  return local < T.batch ? (int64_t)((T.perm_a * (T.first + local) + T.perm_b) % T.n) : -1;
 }
 
-__global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ prm, MlpTrainArgs T, float* __restrict__ partial,
-                                                 float* __restrict__ partial_loss) {
-  __shared__ float A0[kMlpIn * kRS], Z1[kMlpH * kRS], Z2[kMlpH * kRS], Z3[kMlpH * kRS], DZ[kMlpOut * kRS], ACT[kMlpH * kRS];
-  __shared__ float red[64];
-  const int lane = threadIdx.x;
-  const uint64_t local = (uint64_t)blockIdx.x * 64 + lane;
-  const bool valid = local < T.batch;
-  const uint64_t idx = valid ? (T.perm_a * (T.first + local) + T.perm_b) % T.n : 0;
-  float a[kMlpH];
-  float x[kMlpIn];
+// One hidden layer's forward GEMM for this wave's 16 units: z = b + W x over
+// the 64 samples, K = 4 * ksteps inputs; w[ks] = W[16w + l%16][4 ks + l/16].
+// Writes z and silu(z) to the [unit][sample] tiles.
+template <int KSTEPS>
+__device__ __forceinline__ void fwd_layer(const float* __restrict__ w, const float bias[4], const float* x_lds, float* z_lds, float* a_lds,
+                                          int o0, int lr, int lg) {
+  f32x4 acc[4];
 #pragma unroll
-  for (int k = 0; k < kMlpIn; k++) {
-    x[k] = valid ? T.feats[idx * kMlpIn + k] : 0.0f;
-    A0[k * kRS + lane] = x[k];
-  }
-  // forward
-  for (int o = 0; o < kMlpH; o++) {
-    float z = prm[kOffB1 + o];
+  for (int nt = 0; nt < 4; nt++)
 #pragma unroll
-    for (int i = 0; i < kMlpIn; i++) z += prm[kOffW1 + o * kMlpIn + i] * x[i];
-    Z1[o * kRS + lane] = z;
+    for (int r = 0; r < 4; r++) acc[nt][r] = bias[r];
+#pragma unroll
+  for (int ks = 0; ks < KSTEPS; ks++) {
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) acc[nt] = mfma_f32(w[ks], x_lds[(4 * ks + lg) * kGS + 16 * nt + lr], acc[nt]);
   }
 #pragma unroll
-  for (int i = 0; i < kMlpH; i++) a[i] = silu_acc(Z1[i * kRS + lane]);
-  for (int o = 0; o < kMlpH; o++) {
-    float z = prm[kOffB2 + o];
+  for (int nt = 0; nt < 4; nt++)
 #pragma unroll
-    for (int i = 0; i < kMlpH; i++) z += prm[kOffW2 + o * kMlpH + i] * a[i];
-    Z2[o * kRS + lane] = z;
+    for (int r = 0; r < 4; r++) {
+      const int u = o0 + 4 * lg + r, sl = 16 * nt + lr;
+      z_lds[u * kGS + sl] = acc[nt][r];
+      a_lds[u * kGS + sl] = silu_acc(acc[nt][r]);
+    }
+}
+
+// Weight gradient rows of this wave's 16 output units, summed over the 64
+// samples: gW[o][i] = sum_s dz[o][s] a[i][s] for i < n_in (n_in <= 16 * NT),
+// and the bias gradient gB[o] = sum_s dz[o][s] from the same A operands
+// (lane l sums samples 4 ks + l/16; the 4 lane groups are added in order).
+template <int NT>
+__device__ __forceinline__ void grad_rows(const float* dz_lds, const float* a_lds, int n_in, float* __restrict__ gW,
+                                          float* __restrict__ gB, int o0, int lr, int lg) {
+  f32x4 acc[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; nt++) acc[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  float bsum = 0.0f;
+#pragma unroll
+  for (int ks = 0; ks < 16; ks++) {
+    const float a = dz_lds[(o0 + lr) * kGS + 4 * ks + lg];
+    bsum += a;
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) {
+      const int i = 16 * nt + lr;
+      const float b = i < n_in ? a_lds[i * kGS + 4 * ks + lg] : 0.0f;
+      acc[nt] = mfma_f32(a, b, acc[nt]);
+    }
   }
 #pragma unroll
-  for (int i = 0; i < kMlpH; i++) a[i] = silu_acc(Z2[i * kRS + lane]);
-  for (int o = 0; o < kMlpH; o++) {
-    float z = prm[kOffB3 + o];
+  for (int nt = 0; nt < NT; nt++) {
+    const int i = 16 * nt + lr;
+    if (i < n_in)
 #pragma unroll
-    for (int i = 0; i < kMlpH; i++) z += prm[kOffW3 + o * kMlpH + i] * a[i];
-    Z3[o * kRS + lane] = z;
+      for (int r = 0; r < 4; r++) gW[(o0 + 4 * lg + r) * n_in + i] = acc[nt][r];
+  }
+  const float b1 = __shfl(bsum, lr + 16, 64), b2 = __shfl(bsum, lr + 32, 64), b3 = __shfl(bsum, lr + 48, 64);
+  if (lg == 0) gB[o0 + lr] = ((bsum + b1) + b2) + b3;
+}
+
+// dLoss/dz of the previous layer for this wave's 16 input units,
+// dz_prev[i][s] = (sum_o W[o][i] dz[o][s]) * silu'(z_prev[i][s]) with
+// wt[ks] = W[4 ks + l/16][16w + l%16], together with the weight-gradient rows
+// (gW = dz a^T, gB = dz 1) of this wave's 16 output units: both GEMMs read the
+// same dz tile and are independent, so one loop keeps 8 accumulators in flight.
+__device__ __forceinline__ void grad_back_rows(const float* dz_lds, const float* a_lds, float* __restrict__ gW, float* __restrict__ gB,
+                                               const float* __restrict__ wt, const float* zprev_lds, float* out_lds, int o0, int lr,
+                                               int lg) {
+  f32x4 ag[4], ab[4];
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) ag[nt] = ab[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  float bsum = 0.0f;
+#pragma unroll
+  for (int ks = 0; ks < 16; ks++) {
+    const float a = dz_lds[(o0 + lr) * kGS + 4 * ks + lg];
+    bsum += a;
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+      ag[nt] = mfma_f32(a, a_lds[(16 * nt + lr) * kGS + 4 * ks + lg], ag[nt]);
+      ab[nt] = mfma_f32(wt[ks], dz_lds[(4 * ks + lg) * kGS + 16 * nt + lr], ab[nt]);
+    }
   }
 #pragma unroll
-  for (int i = 0; i < kMlpH; i++) a[i] = silu_acc(Z3[i * kRS + lane]);
-  float z4[kMlpOut];
-  for (int o = 0; o < kMlpOut; o++) {
-    float z = prm[kOffB4 + o];
+  for (int nt = 0; nt < 4; nt++)
 #pragma unroll
-    for (int i = 0; i < kMlpH; i++) z += prm[kOffW4 + o * kMlpH + i] * a[i];
-    z4[o] = z;
-  }
-  // loss: BCE(sigmoid(z2), hit) + hit * ((z0 - (t - m))^2 + (z1 - (t + m))^2)
-  float dz[kMlpOut] = {0.0f, 0.0f, 0.0f};
-  float loss = 0.0f;
-  if (valid) {
-    const float th = T.tgts[idx * 3 + 0], h = T.tgts[idx * 3 + 1] > 0.5f ? 1.0f : 0.0f, m = T.tgts[idx * 3 + 2];
-    const float z2 = z4[2];
-    loss = fmaxf(z2, 0.0f) - z2 * h + log1pf(expf(-fabsf(z2)));
-    dz[2] = 1.0f / (1.0f + expf(-z2)) - h;
-    const float e0 = z4[0] - (th - m), e1 = z4[1] - (th + m);
-    loss += h * (e0 * e0 + e1 * e1);
-    dz[0] = h * 2.0f * e0;
-    dz[1] = h * 2.0f * e1;
-  }
-  for (int o = 0; o < kMlpOut; o++) DZ[o * kRS + lane] = dz[o];
-  red[lane] = loss;
-  // layer 4 gradient: ACT = silu(Z3)
+    for (int r = 0; r < 4; r++) {
+      const int u = o0 + 4 * lg + r, sl = 16 * nt + lr;
+      gW[u * kMlpH + 16 * nt + lr] = ag[nt][r];
+      out_lds[u * kGS + sl] = ab[nt][r] * dsilu_acc(zprev_lds[u * kGS + sl]);
+    }
+  const float b1 = __shfl(bsum, lr + 16, 64), b2 = __shfl(bsum, lr + 32, 64), b3 = __shfl(bsum, lr + 48, 64);
+  if (lg == 0) gB[o0 + lr] = ((bsum + b1) + b2) + b3;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ prm, MlpTrainArgs T, float* __restrict__ partial,
+                                                  float* __restrict__ partial_loss) {
+  __shared__ float sX[kMlpIn * kGS];
+  __shared__ float sZ[3][kMlpH * kGS];  // z1, z2, z3
+  __shared__ float sA[3][kMlpH * kGS];  // silu(z1), silu(z2), silu(z3)
+  __shared__ float sD[2][kMlpH * kGS];  // dLoss/dz of the layer being back-propagated (two, alternating)
+  __shared__ float sD4[kMlpOut * 64], sLoss[64], sT[3 * 64];
+  __shared__ int64_t sIdx[64];  // dataset index of each sample slot, -1 past the minibatch
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int lr = lane & 15, lg = lane >> 4, o0 = 16 * wv;
+  float* gp = partial + (size_t)blockIdx.x * kMlpPartialStride;
+
+  // everything this wave reads from global memory, loaded up front: weight
+  // rows (forward) and columns (backward), biases, and the samples' inputs
+  // and targets
+  float w1[6], w2[16], w3[16], w2t[16], w3t[16], w4[16], bias[3][4], b4[4];
 #pragma unroll
-  for (int i = 0; i < kMlpH; i++) ACT[i * kRS + lane] = a[i];
+  for (int ks = 0; ks < 16; ks++) w4[ks] = lr < kMlpOut ? prm[kOffW4 + lr * kMlpH + 4 * ks + lg] : 0.0f;
+  const float w4t = lg < kMlpOut ? prm[kOffW4 + lg * kMlpH + o0 + lr] : 0.0f;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    bias[0][r] = prm[kOffB1 + o0 + 4 * lg + r];
+    bias[1][r] = prm[kOffB2 + o0 + 4 * lg + r];
+    bias[2][r] = prm[kOffB3 + o0 + 4 * lg + r];
+    b4[r] = 4 * lg + r < kMlpOut ? prm[kOffB4 + 4 * lg + r] : 0.0f;
+  }
+#pragma unroll
+  for (int ks = 0; ks < 6; ks++) w1[ks] = prm[kOffW1 + (o0 + lr) * kMlpIn + 4 * ks + lg];
+#pragma unroll
+  for (int ks = 0; ks < 16; ks++) {
+    w2[ks] = prm[kOffW2 + (o0 + lr) * kMlpH + 4 * ks + lg];
+    w3[ks] = prm[kOffW3 + (o0 + lr) * kMlpH + 4 * ks + lg];
+    w2t[ks] = prm[kOffW2 + (4 * ks + lg) * kMlpH + o0 + lr];
+    w3t[ks] = prm[kOffW3 + (4 * ks + lg) * kMlpH + o0 + lr];
+  }
+  if (tid < 64) sIdx[tid] = train_sample(T, blockIdx.x, tid);
   __syncthreads();
-  float* gp = partial + (size_t)blockIdx.x * kMlpParams;
-  block_outer(DZ, ACT, kMlpOut, kMlpH, gp + kOffW4, gp + kOffB4, lane);
-  if (lane == 0) {
+  for (int e = tid; e < 64 * kMlpIn; e += 256) {
+    const int sl = e / kMlpIn, k = e - sl * kMlpIn;
+    const int64_t idx = sIdx[sl];
+    sX[k * kGS + sl] = idx >= 0 ? T.feats[idx * kMlpIn + k] : 0.0f;
+  }
+  if (tid < 64 * 3) {
+    const int sl = tid / 3, c = tid - 3 * sl;
+    const int64_t idx = sIdx[sl];
+    sT[c * 64 + sl] = idx >= 0 ? T.tgts[idx * 3 + c] : 0.0f;
+  }
+  __syncthreads();
+  // forward, hidden layers
+  fwd_layer<6>(w1, bias[0], sX, sZ[0], sA[0], o0, lr, lg);
+  __syncthreads();
+  fwd_layer<16>(w2, bias[1], sA[0], sZ[1], sA[1], o0, lr, lg);
+  __syncthreads();
+  fwd_layer<16>(w3, bias[2], sA[1], sZ[2], sA[2], o0, lr, lg);
+  __syncthreads();
+  // Output layer (3 units, padded to a 16-row tile) for this wave's 16
+  // samples 16w .. 16w + 15: A = W4 rows 0..2, B = a3 of those samples. Lanes
+  // of group 0 end up holding z4[0..2] of sample 16w + l%16 and evaluate the
+  // loss there: BCE(sigmoid(z2), hit) + hit ((z0 - (t - m))^2 + (z1 - (t + m))^2).
+  {
+    f32x4 z = {b4[0], b4[1], b4[2], b4[3]};
+#pragma unroll
+    for (int ks = 0; ks < 16; ks++) z = mfma_f32(w4[ks], sA[2][(4 * ks + lg) * kGS + o0 + lr], z);
+    if (lg == 0) {
+      const int sl = o0 + lr;
+      const int64_t idx = sIdx[sl];
+      float dz0 = 0.0f, dz1 = 0.0f, dz2 = 0.0f, loss = 0.0f;
+      if (idx >= 0) {
+        const float th = sT[sl], h = sT[64 + sl] > 0.5f ? 1.0f : 0.0f, m = sT[128 + sl];
+        const float z2 = z[2];
+        loss = fmaxf(z2, 0.0f) - z2 * h + log1pf(expf(-fabsf(z2)));
+        dz2 = 1.0f / (1.0f + expf(-z2)) - h;
+        const float e0 = z[0] - (th - m), e1 = z[1] - (th + m);
+        loss += h * (e0 * e0 + e1 * e1);
+        dz0 = h * 2.0f * e0;
+        dz1 = h * 2.0f * e1;
+      }
+      sD4[sl] = dz0;
+      sD4[64 + sl] = dz1;
+      sD4[128 + sl] = dz2;
+      sLoss[sl] = loss;
+    }
+  }
+  __syncthreads();
+  if (tid == 255) {
     float acc = 0.0f;
-    for (int s = 0; s < 64; s++) acc += red[s];
+    for (int sl = 0; sl < 64; sl++) acc += sLoss[sl];
     partial_loss[blockIdx.x] = acc;
   }
-  // d3 = (W4^T dz) * silu'(z3), into Z3 (own column); every register array
-  // below is indexed with compile-time indices, the cross-unit sums read the
-  // lane's own LDS column.
-  float dv[kMlpH];
+  // output-layer gradient, columns 16w .. 16w + 15 of W4 (rows 0..2), summed
+  // over the 64 samples; wave 0 also sums the bias gradient
+  {
+    f32x4 g = {0.0f, 0.0f, 0.0f, 0.0f};
+    float bsum = 0.0f;
 #pragma unroll
-  for (int i = 0; i < kMlpH; i++) {
-    float sacc = 0.0f;
+    for (int ks = 0; ks < 16; ks++) {
+      const float a = lr < kMlpOut ? sD4[lr * 64 + 4 * ks + lg] : 0.0f;
+      bsum += a;
+      g = mfma_f32(a, sA[2][(o0 + lr) * kGS + 4 * ks + lg], g);
+    }
 #pragma unroll
-    for (int o = 0; o < kMlpOut; o++) sacc += prm[kOffW4 + o * kMlpH + i] * dz[o];
-    dv[i] = sacc * dsilu_acc(Z3[i * kRS + lane]);
+    for (int r = 0; r < 4; r++)
+      if (4 * lg + r < kMlpOut) gp[kOffW4 + (4 * lg + r) * kMlpH + o0 + lr] = g[r];
+    const float b1 = __shfl(bsum, lr + 16, 64), b2 = __shfl(bsum, lr + 32, 64), b3 = __shfl(bsum, lr + 48, 64);
+    if (wv == 0 && lg == 0 && lr < kMlpOut) gp[kOffB4 + lr] = ((bsum + b1) + b2) + b3;
   }
-  __syncthreads();  // layer-4 sums are done with ACT
+  // dLoss/dz3 for this wave's 16 units: (W4^T dz4) * silu'(z3), K = 3 padded to 4
+  {
 #pragma unroll
-  for (int i = 0; i < kMlpH; i++) {
-    Z3[i * kRS + lane] = dv[i];
-    ACT[i * kRS + lane] = silu_acc(Z2[i * kRS + lane]);
-  }
-  __syncthreads();
-  block_outer(Z3, ACT, kMlpH, kMlpH, gp + kOffW3, gp + kOffB3, lane);
-  // d2 = (W3^T d3) * silu'(z2)
+    for (int nt = 0; nt < 4; nt++) {
+      const float b = lg < kMlpOut ? sD4[lg * 64 + 16 * nt + lr] : 0.0f;
+      const f32x4 d = mfma_f32(w4t, b, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
 #pragma unroll
-  for (int i = 0; i < kMlpH; i++) {
-    float sacc = 0.0f;
-    for (int o = 0; o < kMlpH; o++) sacc += prm[kOffW3 + o * kMlpH + i] * Z3[o * kRS + lane];
-    dv[i] = sacc * dsilu_acc(Z2[i * kRS + lane]);
-  }
-  __syncthreads();  // layer-3 sums are done with ACT
-#pragma unroll
-  for (int i = 0; i < kMlpH; i++) {
-    Z2[i * kRS + lane] = dv[i];
-    ACT[i * kRS + lane] = silu_acc(Z1[i * kRS + lane]);
+      for (int r = 0; r < 4; r++) {
+        const int u = o0 + 4 * lg + r, sl = 16 * nt + lr;
+        sD[0][u * kGS + sl] = d[r] * dsilu_acc(sZ[2][u * kGS + sl]);
+      }
+    }
   }
   __syncthreads();
-  block_outer(Z2, ACT, kMlpH, kMlpH, gp + kOffW2, gp + kOffB2, lane);
-  // d1 = (W2^T d2) * silu'(z1)
-#pragma unroll
-  for (int i = 0; i < kMlpH; i++) {
-    float sacc = 0.0f;
-    for (int o = 0; o < kMlpH; o++) sacc += prm[kOffW2 + o * kMlpH + i] * Z2[o * kRS + lane];
-    dv[i] = sacc * dsilu_acc(Z1[i * kRS + lane]);
-  }
+  // layer 3: gradient from dz3 (sD[0]) and a2, dz2 into sD[1]
+  grad_back_rows(sD[0], sA[1], gp + kOffW3, gp + kOffB3, w3t, sZ[1], sD[1], o0, lr, lg);
   __syncthreads();
-#pragma unroll
-  for (int i = 0; i < kMlpH; i++) Z1[i * kRS + lane] = dv[i];
+  // layer 2: gradient from dz2 (sD[1]) and a1, dz1 into sD[0]
+  grad_back_rows(sD[1], sA[0], gp + kOffW2, gp + kOffB2, w2t, sZ[0], sD[0], o0, lr, lg);
   __syncthreads();
-  block_outer(Z1, A0, kMlpH, kMlpIn, gp + kOffW1, gp + kOffB1, lane);
+  // layer 1: gradient from dz1 (sD[0]) and the inputs
+  grad_rows<2>(sD[0], sX, kMlpIn, gp + kOffW1, gp + kOffB1, o0, lr, lg);
 }
 
+// Gradient reduction over the blocks' partials + Adam. A block takes 64
+// parameters: lane (q, j) sums parameters 4q .. 4q + 3 of the block over the
+// partials b = j mod 16 with 16-B loads (rows are kMlpPartialStride floats,
+// 16-B aligned), then each parameter's 16 sums are added in j order -- a
+// fixed order, so the result is deterministic.
+constexpr int kRedParams = 64, kRedLanes = 16;
 __global__ __launch_bounds__(256) void k_mlp_reduce_adam(float* __restrict__ prm, float* __restrict__ m, float* __restrict__ v,
                                                          const float* __restrict__ partial, const float* __restrict__ partial_loss,
                                                          uint32_t parts, uint32_t batch, AdamArgs adam, int apply,
                                                          float* __restrict__ grad_out, float* __restrict__ loss_out) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ float red[kRedLanes][kRedParams];
+  __shared__ float lred[256];
+  const int q = threadIdx.x % (kRedParams / 4), j = threadIdx.x / (kRedParams / 4);
+  const uint32_t p0 = blockIdx.x * kRedParams + 4 * q;
   const float inv_b = 1.0f / (float)batch;
-  if (p == 0 && loss_out) {
+  if (blockIdx.x == 0 && loss_out) {  // loss: lane t sums partials t mod 256, then lane 0 adds the 256 sums in order
     float acc = 0.0f;
-    for (uint32_t b = 0; b < parts; b++) acc += partial_loss[b];
+    for (uint32_t b = threadIdx.x; b < parts; b += 256) acc += partial_loss[b];
+    lred[threadIdx.x] = acc;
+  }
+  float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (p0 < (uint32_t)kMlpParams)  // p0 + 3 < kMlpPartialStride: the padding is never summed into a parameter
+    for (uint32_t b = j; b < parts; b += kRedLanes) {
+      const float4 v4 = *(const float4*)(partial + (size_t)b * kMlpPartialStride + p0);
+      g.x += v4.x; g.y += v4.y; g.z += v4.z; g.w += v4.w;
+    }
+  red[j][4 * q + 0] = g.x;
+  red[j][4 * q + 1] = g.y;
+  red[j][4 * q + 2] = g.z;
+  red[j][4 * q + 3] = g.w;
+  __syncthreads();
+  if (blockIdx.x == 0 && loss_out && threadIdx.x == 0) {
+    float acc = 0.0f;
+    for (int t = 0; t < 256; t++) acc += lred[t];
     *loss_out = acc * inv_b;
   }
-  if (p >= (uint32_t)kMlpParams) return;
-  float g = 0.0f;
-  for (uint32_t b = 0; b < parts; b++) g += partial[(size_t)b * kMlpParams + p];
-  g *= inv_b;
-  if (grad_out) grad_out[p] = g;
+  const uint32_t p = blockIdx.x * kRedParams + threadIdx.x;
+  if (threadIdx.x >= kRedParams || p >= (uint32_t)kMlpParams) return;
+  float gs = 0.0f;
+  for (int k = 0; k < kRedLanes; k++) gs += red[k][threadIdx.x];
+  gs *= inv_b;
+  if (grad_out) grad_out[p] = gs;
   if (apply) {
-    const float mm = adam.b1 * m[p] + (1.0f - adam.b1) * g;
-    const float vv = adam.b2 * v[p] + (1.0f - adam.b2) * g * g;
+    const float mm = adam.b1 * m[p] + (1.0f - adam.b1) * gs;
+    const float vv = adam.b2 * v[p] + (1.0f - adam.b2) * gs * gs;
     m[p] = mm;
     v[p] = vv;
     prm[p] -= adam.lr * (mm * adam.c1) / (sqrtf(vv * adam.c2) + adam.eps);
@@ -567,14 +700,15 @@ hipError_t launch_mlp_infer(const void* d_packed, const float* d_feat, uint64_t 
 
 hipError_t launch_mlp_grad(const float* d_params, const lrtd::MlpTrainArgs& T, float* d_partial, float* d_partial_loss, uint32_t blocks,
                            hipStream_t stream) {
-  hipLaunchKernelGGL(lrtd::k_mlp_grad, dim3(blocks), dim3(64), 0, stream, d_params, T, d_partial, d_partial_loss);
+  hipLaunchKernelGGL(lrtd::k_mlp_grad, dim3(blocks), dim3(256), 0, stream, d_params, T, d_partial, d_partial_loss);
   return hipGetLastError();
 }
 
 hipError_t launch_mlp_reduce_adam(float* d_params, float* d_m, float* d_v, const float* d_partial, const float* d_partial_loss,
                                   uint32_t parts, uint32_t batch, const lrtd::AdamArgs& adam, bool apply, float* d_grad_out,
                                   float* d_loss_out, hipStream_t stream) {
-  hipLaunchKernelGGL(lrtd::k_mlp_reduce_adam, dim3((lrtd::kMlpParams + 255) / 256), dim3(256), 0, stream, d_params, d_m, d_v, d_partial,
+  hipLaunchKernelGGL(lrtd::k_mlp_reduce_adam, dim3((lrtd::kMlpParams + lrtd::kRedParams - 1) / lrtd::kRedParams), dim3(256), 0, stream, d_params,
+                     d_m, d_v, d_partial,
                      d_partial_loss, parts, batch, adam, apply ? 1 : 0, d_grad_out, d_loss_out);
   return hipGetLastError();
 }

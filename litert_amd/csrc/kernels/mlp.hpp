@@ -18,6 +18,9 @@ constexpr int kMlpIn = 24, kMlpH = 64, kMlpOut = 3;
 constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kMlpH * kMlpIn, kOffW2 = kOffB1 + kMlpH, kOffB2 = kOffW2 + kMlpH * kMlpH,
               kOffW3 = kOffB2 + kMlpH, kOffB3 = kOffW3 + kMlpH * kMlpH, kOffW4 = kOffB3 + kMlpH, kOffB4 = kOffW4 + kMlpOut * kMlpH,
               kMlpParams = kOffB4 + kMlpOut;
+// Row stride (floats) of the training kernel's per-block partial gradients:
+// kMlpParams rounded up to 64, so every row starts 256-B aligned.
+constexpr int kMlpPartialStride = (kMlpParams + 63) & ~63;
 
 // Packed inference weights. Each 16x32 A tile of v_mfma_f32_16x16x32_bf16 is
 // stored as 64 lanes x 8 bf16 (16 B per lane, 1 KiB per tile) in the order

@@ -240,7 +240,7 @@ int lrt_mlp_train(lrt_handle h, uint32_t epochs, uint32_t batch, float lr, uint3
   const uint64_t n = r->ds_n;
   const uint32_t max_blocks = batch / 64;
   const uint64_t steps = (n + batch - 1) / batch;
-  if ((e = r->d_mlp_partial.ensure((size_t)max_blocks * kMlpParams * 4)) != hipSuccess) return hip_err(e, "alloc partials");
+  if ((e = r->d_mlp_partial.ensure((size_t)max_blocks * kMlpPartialStride * 4)) != hipSuccess) return hip_err(e, "alloc partials");
   if ((e = r->d_mlp_ploss.ensure((size_t)max_blocks * 4)) != hipSuccess) return hip_err(e, "alloc partials");
   if ((e = r->d_mlp_loss.ensure(steps * 4)) != hipSuccess) return hip_err(e, "alloc losses");
   std::vector<float> step_loss(steps);
@@ -305,7 +305,7 @@ int lrt_mlp_loss_grad(lrt_handle h, const float* feats, const float* tgts, uint3
   do {
     if ((e = df.upload(feats, (size_t)n * 24 * 4, r->stream)) != hipSuccess) { rc = hip_err(e, "upload features"); break; }
     if ((e = dt.upload(tgts, (size_t)n * 3 * 4, r->stream)) != hipSuccess) { rc = hip_err(e, "upload targets"); break; }
-    if ((e = r->d_mlp_partial.ensure((size_t)blocks * kMlpParams * 4)) != hipSuccess) { rc = hip_err(e, "alloc partials"); break; }
+    if ((e = r->d_mlp_partial.ensure((size_t)blocks * kMlpPartialStride * 4)) != hipSuccess) { rc = hip_err(e, "alloc partials"); break; }
     if ((e = r->d_mlp_ploss.ensure((size_t)blocks * 4)) != hipSuccess) { rc = hip_err(e, "alloc partials"); break; }
     if ((e = r->d_mlp_grad.ensure(kMlpParams * 4)) != hipSuccess) { rc = hip_err(e, "alloc grad"); break; }
     if ((e = r->d_mlp_loss.ensure(16)) != hipSuccess) { rc = hip_err(e, "alloc loss"); break; }
