@@ -50,16 +50,17 @@ def cpu_model():
     return "unknown CPU"
 
 
-def measured_traffic(lod):
-    """HBM bytes per plain-render launch from the FETCH_SIZE / WRITE_SIZE counter
-    passes (scaled by a 1 GiB copy), recorded per scene size in
-    profiles/k_render_ico<4*lod>_hbm.json (None when no pass was recorded)."""
+def measured_counters(lod):
+    """The plain-render kernel's counter figures recorded per scene size in
+    profiles/k_render_ico<4*lod>_hbm.json by --reduce-counters: HBM bytes per
+    launch (FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy), VALU lane
+    utilisation and VALU issue fraction ({} when no pass was recorded)."""
     path = os.path.join(ROOT, "profiles", f"k_render_ico{4 * lod}_hbm.json")
     try:
         with open(path) as f:
-            return json.load(f)["traffic_bytes"]
-    except (OSError, KeyError, ValueError):
-        return None
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
 
 
 def reduce_counters(dirs, kernel, rays, calib_bytes, out_path):
@@ -107,8 +108,16 @@ def reduce_counters(dirs, kernel, rays, calib_bytes, out_path):
         out["valu_lane_utilisation"] = round(mean["SQ_THREAD_CYCLES_VALU"] / (64.0 * mean["SQ_ACTIVE_INST_VALU"]), 4)
     if "SQ_INSTS_VALU" in mean and rays:
         out["valu_insts_per_wave_tile"] = round(mean["SQ_INSTS_VALU"] / (rays / 64.0), 1)
-    if "SQ_WAVE_CYCLES" in mean and "GRBM_GUI_ACTIVE" in mean:
-        out["waves_per_cu"] = round(4.0 * mean["SQ_WAVE_CYCLES"] / mean["GRBM_GUI_ACTIVE"] / 256.0, 2)
+    if "GRBM_GUI_ACTIVE" in mean:
+        # GRBM_GUI_ACTIVE sums the busy cycles of the 8 XCDs; the SQ cycle
+        # counters count in units of 4 cycles, summed over waves (SQ_WAVE_CYCLES)
+        # or over the SIMDs' VALU issue (SQ_ACTIVE_INST_VALU)
+        cycles = mean["GRBM_GUI_ACTIVE"] / 8.0
+        out["kernel_cycles"] = round(cycles)
+        if "SQ_WAVE_CYCLES" in mean:
+            out["waves_per_cu"] = round(4.0 * mean["SQ_WAVE_CYCLES"] / cycles / 256.0, 2)
+        if "SQ_ACTIVE_INST_VALU" in mean:
+            out["valu_issue_frac"] = round(4.0 * mean["SQ_ACTIVE_INST_VALU"] / (cycles * 1024.0), 4)
     with open(out_path, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out, indent=1))
@@ -511,6 +520,7 @@ def main():
                 raise RuntimeError(f"HIP call failed with {rc} in the calibration copy")
         log(f"calibration copy: {n.value} bytes read + written")
 
+    counters = measured_counters(args.lod)
     out = {
         "metric": "primary Mrays/s on SBS (plain vs MLP-seeded); DR fwd+bwd Mrays/s",
         "value": round(mrays, 2),
@@ -529,8 +539,13 @@ def main():
                    "bricks": int(sbs.nodes.shape[0]), "rays_per_step": rays, "parallelism": "single GPU"},
         "kernel_ms_per_step": round(avg_kernel_ms, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.lod),
-                     "algorithmic_bytes_per_launch": alg_bytes},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": counters.get("traffic_bytes"),
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     # what limits the kernel instead: the VALU pipes issue on every
+                     # cycle (valu_issue_frac ~1) with part of each wave's lanes active
+                     "limiter": "valu_issue" if counters.get("valu_issue_frac", 0) >= 0.9 else None,
+                     "valu_issue_frac": counters.get("valu_issue_frac"),
+                     "valu_lane_util": counters.get("valu_lane_utilisation")},
         "cpu_baseline": cpu,
         "mlp_seeded_mrays_s": mlp_mrays,
         "mlp": mlp_info,
