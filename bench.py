@@ -441,6 +441,16 @@ def main():
                    "loss_mean_first_last_16_steps": [float(np.mean(losses[:16])), float(np.mean(losses[-16:]))]}
         log(f"dr: {dr_mrays:.1f} Mrays/s (fwd+bwd+Adam, {total_ms / dr.opt_iterations:.2f} ms/iter), RenderDR kernels "
             f"{np.mean(view_ms):.2f} ms/view, border pixels {bp}")
+        # image_batch_size 4 (MultiRendererDR.cpp:395-476): four images per Adam
+        # step, the gradient of the last; the other three ride in its primary launch
+        dr.opt_iterations = 50
+        dr.image_batch_size = 4
+        work = L.SdfSBS(ini.header, ini.nodes, ini.values, ini.values_f.copy())
+        _, b4_ms = g.OptimizeFixedStructure(dr, work, seed=1)
+        dr.image_batch_size = 1
+        dr_info["batch4_ms_per_iteration"] = round(b4_ms / 50, 3)
+        dr_info["batch4_mrays_s"] = round(dres * dres * 4 * 50 / (b4_ms * 1e-3) / 1e6, 1)
+        log(f"dr batch 4: {b4_ms / 50:.3f} ms/iter ({dr_info['batch4_mrays_s']} Mrays/s primary rays of all four images)")
         # SVM border sampling (MultiRendererDR.cpp:1651-1931) on the same views
         # and parameters: per border pixel 64 SVM samples, the perceptron fit
         # and the stripe's border rays, against random sampling above

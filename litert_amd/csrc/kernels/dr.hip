@@ -91,28 +91,28 @@ __device__ float samples_to_threshold(const float* s, unsigned count, float dv, 
 }
 
 template <int SPEC>
-__device__ __forceinline__ void scatter_pd(const DRArgs& A, const DRPayload& py, v3 dLdC) {
+__device__ __forceinline__ void scatter_pd(const DRArgs& A, float* __restrict__ grad, const DRPayload& py, v3 dLdC) {
   const uint32_t f = dr_flags<SPEC>(DRParams{A.Q.ray_flags, A.Q.dtdv_fd});
   if ((f & LRT_DR_RAY_FLAG_DDIFFUSE_DCOLOR) && A.mode != LRT_DR_RENDER_MODE_MASK) {
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       if (py.col_idx[k] == 0xFFFFFFFFu) continue;
       float w = py.col_w[k];
-      atomicAdd(&A.grad[py.col_idx[k] + 0], dLdC.x * w);
-      atomicAdd(&A.grad[py.col_idx[k] + 1], dLdC.y * w);
-      atomicAdd(&A.grad[py.col_idx[k] + 2], dLdC.z * w);
+      atomicAdd(&grad[py.col_idx[k] + 0], dLdC.x * w);
+      atomicAdd(&grad[py.col_idx[k] + 1], dLdC.y * w);
+      atomicAdd(&grad[py.col_idx[k] + 2], dLdC.z * w);
     }
   }
   if (f & LRT_DR_RAY_FLAG_DDIFFUSE_DPOS) {
 #pragma unroll
     for (int k = 0; k < 8; k++)
-      if (py.pd_idx[k] != 0xFFFFFFFFu) atomicAdd(&A.grad[py.pd_idx[k]], dot(dLdC, py.pd_ddiffuse[k]));
+      if (py.pd_idx[k] != 0xFFFFFFFFu) atomicAdd(&grad[py.pd_idx[k]], dot(dLdC, py.pd_ddiffuse[k]));
   } else if (f & LRT_DR_RAY_FLAG_DDIST_DPOS) {
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       if (py.pd_idx[k] == 0xFFFFFFFFu) continue;
       float v = py.pd_ddist[k] / (10.0f - 0.1f);
-      atomicAdd(&A.grad[py.pd_idx[k]], dot(dLdC, v3(v, v, v)));
+      atomicAdd(&grad[py.pd_idx[k]], dot(dLdC, v3(v, v, v)));
     }
   }
 }
@@ -123,7 +123,7 @@ __device__ __forceinline__ void scatter_pd(const DRArgs& A, const DRPayload& py,
 // second pass recomputes identical rays and hits. SPEC kDrDepth is used with
 // spp 1 only (one pass, gradient flags on).
 template <int SPEC>
-__device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, Stack& st, float& loss_acc) {
+__device__ __forceinline__ void dr_pixel(const DRArgs& A, const DRView& V, uint32_t packed_idx, Stack& st, float& loss_acc) {
   uint32_t x, y;
   tid_to_xy(packed_idx, A.packed_w, x, y);
   if (x >= A.W || y >= A.H) return;
@@ -141,12 +141,12 @@ __device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, S
   const DRParams Qg = {A.Q.ray_flags, A.Q.dtdv_fd};
   py.scatter = 0;
   py.mode = A.mode;
-  py.grad = A.grad;
+  py.grad = V.grad;
   if (SPEC == kDrNormal) py.lights = &A.lights;
   for (uint32_t s = 0; s < spp; s++) {
     float2 d = rand2(x, y, s, A.seed);
     v3 pos, dir;
-    eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
+    eye_ray(V.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
     DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, A.bminmax, Qc, A.relax_eps, pos, dir, 1e9f, &py, st);
     if (hit.primId == 0xFFFFFFFFu) continue;
     v3 c = dr_color(A.mode, hit, A.lights);
@@ -157,29 +157,32 @@ __device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, S
     hit_count++;
   }
   const float cm[4] = {res.x, res.y, res.z, (float)hit_count / (float)spp};
-  A.out_image[pix] = make_float4(cm[0], cm[1], cm[2], cm[3]);
+  V.out_image[pix] = make_float4(cm[0], cm[1], cm[2], cm[3]);
   // depth-discontinuity threshold (written only where phase II reads it)
   const bool streamed = hit_count > (int)kDrKeptSamples;
   unsigned hist[10];
   float h_diff = 0.0f;
-  if (A.depth_thr) {
+  // a view rendered for its image and loss only (grad null: an earlier image
+  // of an Adam step's batch) takes no gradient pass
+  const bool grad_pass = A.Q.ray_flags != 0 && V.grad != nullptr;
+  if (V.depth_thr) {
     if (!streamed) {
-      A.depth_thr[pix] = hit_count == 0 ? 0.0f : samples_to_threshold(samples, hit_count, 1e9f, spp - hit_count);
+      V.depth_thr[pix] = hit_count == 0 ? 0.0f : samples_to_threshold(samples, hit_count, 1e9f, spp - hit_count);
     } else {
       h_diff = depth_range(t_mn, t_mx, 1e9f, spp - hit_count, hist);
-      if (A.Q.ray_flags == 0) {  // no gradient pass follows: trace again for the bins
+      if (!grad_pass) {  // no gradient pass follows: trace again for the bins
         for (uint32_t s = 0; s < spp; s++) {
           float2 d = rand2(x, y, s, A.seed);
           v3 pos, dir;
-          eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
+          eye_ray(V.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
           DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, A.bminmax, Qc, A.relax_eps, pos, dir, 1e9f, &py, st);
           if (hit.primId != 0xFFFFFFFFu) hist[depth_bin(hit.t, t_mn, h_diff)]++;
         }
-        A.depth_thr[pix] = depth_gap_threshold(hist, t_mn, h_diff);
+        V.depth_thr[pix] = depth_gap_threshold(hist, t_mn, h_diff);
       }
     }
   }
-  const float4 r = A.ref[pix];
+  const float4 r = V.ref[pix];
   const float rr[4] = {r.x, r.y, r.z, r.w};
   float l = 0.0f;
   for (int k = 0; k < 4; k++) {
@@ -189,9 +192,9 @@ __device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, S
   loss_acc += l;
   const v3 dLdC(loss_grad1(A.loss_fn, cm[0], rr[0]) / (float)spp, loss_grad1(A.loss_fn, cm[1], rr[1]) / (float)spp,
                 loss_grad1(A.loss_fn, cm[2], rr[2]) / (float)spp);
-  if (A.Q.ray_flags == 0) return;
+  if (!grad_pass) return;
   if (one_pass) {
-    if (hit_count) scatter_pd<SPEC>(A, py, dLdC);
+    if (hit_count) scatter_pd<SPEC>(A, V.grad, py, dLdC);
     return;
   }
   if (SPEC == kDrNormal) {
@@ -202,15 +205,18 @@ __device__ __forceinline__ void dr_pixel(const DRArgs& A, uint32_t packed_idx, S
   for (uint32_t s = 0; s < spp; s++) {
     float2 d = rand2(x, y, s, A.seed);
     v3 pos, dir;
-    eye_ray(A.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
+    eye_ray(V.cam, x, y, d.x, d.y, A.W, A.H, pos, dir);
     DRHit hit = dr_nearest_hit<SPEC>(A.S, A.bvals, A.bminmax, Qg, A.relax_eps, pos, dir, 1e9f, &py, st);
     if (streamed && hit.primId != 0xFFFFFFFFu) hist[depth_bin(hit.t, t_mn, h_diff)]++;
-    if (SPEC != kDrNormal && hit.primId != 0xFFFFFFFFu) scatter_pd<SPEC>(A, py, dLdC);
+    if (SPEC != kDrNormal && hit.primId != 0xFFFFFFFFu) scatter_pd<SPEC>(A, V.grad, py, dLdC);
   }
-  if (streamed && A.depth_thr) A.depth_thr[pix] = depth_gap_threshold(hist, t_mn, h_diff);
+  if (streamed && V.depth_thr) V.depth_thr[pix] = depth_gap_threshold(hist, t_mn, h_diff);
 }
 
-template <int SPEC>
+// Phase I over one view (MV false: the view of A) or over the A.n_views
+// records of A.views (MV true: the images of one Adam step's batch in one
+// launch, each view's pixels in its own 256-pixel chunks).
+template <int SPEC, bool MV>
 __global__ __launch_bounds__(256) void k_dr_primary(DRArgs A, uint32_t* work_counter, uint32_t* spill, float* loss) {
   __shared__ uint32_t lds_stack[kStackLds * 256];
   __shared__ uint32_t chunk_base;
@@ -222,14 +228,23 @@ __global__ __launch_bounds__(256) void k_dr_primary(DRArgs A, uint32_t* work_cou
   st.spill_stride = gridDim.x * 256;
   float acc = 0.0f;
   const uint32_t total = A.W * A.H;  // the reference walks the first W*H packed indices
+  const uint32_t per_view = (total + 255) / 256;
+  const uint32_t n_chunks = MV ? per_view * A.n_views : per_view;
   for (;;) {
     if (threadIdx.x == 0) chunk_base = atomicAdd(work_counter, 1u);
     __syncthreads();
-    const uint64_t base = (uint64_t)chunk_base * 256;
+    const uint32_t c = __builtin_amdgcn_readfirstlane(chunk_base);
     __syncthreads();
-    if (base >= total) break;
-    const uint64_t i = base + threadIdx.x;
-    if (i < total) dr_pixel<SPEC>(A, (uint32_t)i, st, acc);
+    if (c >= n_chunks) break;
+    if (MV) {
+      const uint32_t v = c / per_view;  // block-uniform
+      const uint32_t i = (c - v * per_view) * 256 + threadIdx.x;
+      if (i < total) dr_pixel<SPEC>(A, A.views[v], i, st, acc);
+    } else {
+      const DRView V = {A.cam, A.ref, A.out_image, A.depth_thr, A.grad};
+      const uint32_t i = c * 256 + threadIdx.x;
+      if (i < total) dr_pixel<SPEC>(A, V, i, st, acc);
+    }
   }
   // loss: per-lane partial sums, a wave sum, one atomic per wave
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
@@ -911,14 +926,19 @@ hipError_t launch_dr_cull(const DRCullTables& T, const float2* d_minmax, float r
 }
 hipError_t launch_dr_primary(const lrtd::DRArgs& A, uint32_t* d_counter, uint32_t* d_spill, float* d_loss, uint32_t blocks,
                              hipStream_t s, const uint4* cull_pairs) {
+  const bool mv = A.views != nullptr;
   if (dr_primary_is_depth(A)) {
     lrtd::DRArgs Ad = A;
     if (cull_pairs) Ad.S.pairs = cull_pairs;
-    hipLaunchKernelGGL(lrtd::k_dr_primary<lrtd::kDrDepth>, dim3(blocks), dim3(256), 0, s, Ad, d_counter, d_spill, d_loss);
-  } else if (dr_normal_mode(A))
-    hipLaunchKernelGGL(lrtd::k_dr_primary<lrtd::kDrNormal>, dim3(blocks), dim3(256), 0, s, A, d_counter, d_spill, d_loss);
-  else
-    hipLaunchKernelGGL(lrtd::k_dr_primary<lrtd::kDrGeneric>, dim3(blocks), dim3(256), 0, s, A, d_counter, d_spill, d_loss);
+    if (mv) hipLaunchKernelGGL((lrtd::k_dr_primary<lrtd::kDrDepth, true>), dim3(blocks), dim3(256), 0, s, Ad, d_counter, d_spill, d_loss);
+    else hipLaunchKernelGGL((lrtd::k_dr_primary<lrtd::kDrDepth, false>), dim3(blocks), dim3(256), 0, s, Ad, d_counter, d_spill, d_loss);
+  } else if (dr_normal_mode(A)) {
+    if (mv) hipLaunchKernelGGL((lrtd::k_dr_primary<lrtd::kDrNormal, true>), dim3(blocks), dim3(256), 0, s, A, d_counter, d_spill, d_loss);
+    else hipLaunchKernelGGL((lrtd::k_dr_primary<lrtd::kDrNormal, false>), dim3(blocks), dim3(256), 0, s, A, d_counter, d_spill, d_loss);
+  } else {
+    if (mv) hipLaunchKernelGGL((lrtd::k_dr_primary<lrtd::kDrGeneric, true>), dim3(blocks), dim3(256), 0, s, A, d_counter, d_spill, d_loss);
+    else hipLaunchKernelGGL((lrtd::k_dr_primary<lrtd::kDrGeneric, false>), dim3(blocks), dim3(256), 0, s, A, d_counter, d_spill, d_loss);
+  }
   return hipGetLastError();
 }
 hipError_t launch_dr_fd_set(float* d_values_f, float* d_bvals, const float* d_p0, const uint32_t* d_pos, uint32_t n_pos, uint32_t j,

@@ -7,6 +7,17 @@ namespace lrtd {
 
 struct DRParams;
 
+// One image of an Adam step's batch for the multi-view primary pass: its
+// camera, reference and image slot; depth_thr and grad are null for every
+// image but the one whose gradient the step keeps.
+struct DRView {
+  Camera cam;
+  const float4* ref;
+  float4* out_image;
+  float* depth_thr;
+  float* grad;
+};
+
 struct DRArgs {
   DevScene S;
   Camera cam;
@@ -28,6 +39,8 @@ struct DRArgs {
   const float* bvals;     // per-brick corner values gathered from values_f (k_dr_gather)
   const float2* bminmax;  // smallest / largest of each brick's gathered values
   LightSet lights;        // LAMBERT: direct and ambient lights
+  const DRView* views;    // null: the primary pass renders the view above; else these n_views records
+  uint32_t n_views;
 };
 
 }  // namespace lrtd

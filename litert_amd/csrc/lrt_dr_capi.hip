@@ -221,8 +221,12 @@ static int dr_finite_diff(Renderer* r, const MultiRendererDRPreset& p, const lrt
 // added to this device float and the call returns without waiting for the
 // GPU (no timing, no statistics, dr_last_loss unchanged), so a loop of steps
 // never stalls on the host.
+// d_batch (optional, with d_loss): n_batch device records of the step's other
+// images, rendered for their image and loss only in the same primary launch
+// as `view` (MultiRendererDR.cpp:395-437 clears the gradient before every
+// image of a batch, so only the last image's gradient reaches Adam).
 static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t view, uint32_t seed, int dtdv_mode, float* kernel_ms,
-                          float* d_loss = nullptr) {
+                          float* d_loss = nullptr, const lrtd::DRView* d_batch = nullptr, uint32_t n_batch = 0) {
   int c = dr_check(r, p);
   if (c) return c;
   if (view >= r->dr_n_ref) return set_err(LRT_E_INVALID, "view out of range");
@@ -322,7 +326,12 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
     cull_border = T.pairs_border;
   }
   float* loss_out = d_loss ? d_loss : (float*)(misc + 4);
-  if ((e = launch_dr_primary(A, misc, (uint32_t*)r->d_spill.p, loss_out, blocks, r->stream, cull_primary)) != hipSuccess)
+  lrtd::DRArgs Ap = A;
+  if (d_batch && n_batch > 1) {
+    Ap.views = d_batch;
+    Ap.n_views = n_batch;
+  }
+  if ((e = launch_dr_primary(Ap, misc, (uint32_t*)r->d_spill.p, loss_out, blocks, r->stream, cull_primary)) != hipSuccess)
     return hip_err(e, "dr primary");
   if ((e = launch_dr_border_find(A, (uint32_t*)r->d_dr_list.p, misc + 1, r->stream)) != hipSuccess) return hip_err(e, "dr border");
   if (geometry_border) {
@@ -438,17 +447,54 @@ int lrt_dr_optimize(lrt_handle h, const MultiRendererDRPreset* p, const SdfSBSHe
     d_losses = (float*)h->d_dr_losses.p;
     if ((e = zero_u32((uint32_t*)d_losses, p->opt_iterations, h->stream)) != hipSuccess) return hip_err(e, "clear");
   }
+  // MultiRendererDR.cpp:395-406: image b of step `it` (all views in order
+  // when the batch is the whole set, otherwise a hashed choice)
+  const uint32_t B = p->image_batch_size;
+  auto batch_view = [&](uint32_t it, uint32_t b) {
+    return B == h->dr_n_ref ? b : pcg_hash(seed ^ pcg_hash(it * 131u + b)) % h->dr_n_ref;
+  };
+  // Batches of several images in the default mode: the reference clears the
+  // gradient before every image, so an image other than the last contributes
+  // only its rendered image and its loss. Those images are rendered in the
+  // last image's primary launch from per-image records, written for every
+  // step before the loop; only the last image takes the gradient and border
+  // passes.
+  const bool batched = deferred && B > 1;
+  if (batched) {
+    const size_t np = (size_t)h->dr_w * h->dr_h;
+    if ((e = h->d_dr_img.ensure(np * 16 * h->dr_n_ref)) != hipSuccess) return hip_err(e, "alloc");
+    if ((e = h->d_dr_thr.ensure(np * 4)) != hipSuccess) return hip_err(e, "alloc");
+    if ((e = h->d_dr_grad.ensure((size_t)P * 4)) != hipSuccess) return hip_err(e, "alloc");
+    std::vector<lrtd::DRView> recs((size_t)p->opt_iterations * B);
+    for (uint32_t it = 0; it < p->opt_iterations; it++)
+      for (uint32_t b = 0; b < B; b++) {
+        const uint32_t v = batch_view(it, b);
+        lrtd::DRView& R = recs[(size_t)it * B + b];
+        R.cam = make_camera(&h->dr_world_views[16 * v], &h->dr_projs[16 * v]);
+        R.ref = (const float4*)h->d_dr_ref.p + np * v;
+        R.out_image = (float4*)h->d_dr_img.p + np * v;
+        R.depth_thr = b + 1 == B ? (float*)h->d_dr_thr.p : nullptr;
+        R.grad = b + 1 == B ? (float*)h->d_dr_grad.p : nullptr;
+      }
+    if ((e = h->d_dr_views.upload(recs.data(), recs.size() * sizeof(lrtd::DRView), h->stream)) != hipSuccess) return hip_err(e, "upload views");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_err(e, "upload views");
+  }
   auto t0 = std::chrono::steady_clock::now();
-  h->dr_last_batch = p->image_batch_size;
+  h->dr_last_batch = B;
   for (uint32_t it = 0; it < p->opt_iterations; it++) {
     double lsum = 0;
-    for (uint32_t b = 0; b < p->image_batch_size; b++) {
-      // MultiRendererDR.cpp:395-406; the gradient buffer is cleared per image as there
-      uint32_t view = p->image_batch_size == h->dr_n_ref ? b : pcg_hash(seed ^ pcg_hash(it * 131u + b)) % h->dr_n_ref;
-      if ((c = dr_render_view(h, *p, view, seed, dtdv_mode, nullptr, deferred ? d_losses + it : nullptr))) return c;
-      lsum += h->dr_last_loss;
+    if (batched) {
+      const lrtd::DRView* recs = (const lrtd::DRView*)h->d_dr_views.p + (size_t)it * B;
+      if ((c = dr_render_view(h, *p, batch_view(it, B - 1), seed, dtdv_mode, nullptr, d_losses + it, recs, B))) return c;
+      for (uint32_t b = 0; b < B; b++) h->dr_img_valid[batch_view(it, b)] = 1;
+    } else {
+      for (uint32_t b = 0; b < B; b++) {
+        // the gradient buffer is cleared per image, as in the reference
+        if ((c = dr_render_view(h, *p, batch_view(it, b), seed, dtdv_mode, nullptr, deferred ? d_losses + it : nullptr))) return c;
+        lsum += h->dr_last_loss;
+      }
     }
-    if (losses && !deferred) losses[it] = (float)(lsum / p->image_batch_size);
+    if (losses && !deferred) losses[it] = (float)(lsum / B);
     if (p->reg_function != LRT_DR_REG_FUNCTION_NONE) {
       // Regularization, MultiRendererDR.cpp:453-455: after the step's renders,
       // added to the gradient before the batch division. The reference walks

@@ -88,6 +88,7 @@ struct Renderer {
   DevBuf d_dr_leaf_slot, d_dr_pair_slot, d_dr_tiled, d_dr_occ, d_dr_cull_pairs;
   DevBuf d_dr_redist;  // redistancing scratch: frozen flags + two lattice buffers
   DevBuf d_dr_losses;  // per-step loss sums of lrt_dr_optimize
+  DevBuf d_dr_views;   // lrt_dr_optimize with image_batch_size > 1: one lrtd::DRView per image of every step
   // finite-difference DR: saved parameters, the referenced parameters, their
   // positions in the corner copy (CSR), and the loss pair of each
   DevBuf d_dr_fd_p0, d_dr_fd_ids, d_dr_fd_pos, d_dr_fd_acc;
@@ -130,7 +131,7 @@ struct Renderer {
     DevBuf* all[] = {&d_pairs, &d_nodes, &d_values, &d_values_f, &d_brecs, &d_cams, &d_counter, &d_spill, &d_counts,
                      &d_rays, &d_hits, &d_out_t, &d_out_n, &d_out_prim, &d_out_rgba, &d_dr_ref, &d_dr_img,
                      &d_dr_thr, &d_dr_grad, &d_dr_list, &d_dr_misc, &d_adam_v, &d_adam_s, &d_dr_bvals, &d_dr_bminmax, &d_dr_leaf_slot,
-                     &d_dr_pair_slot, &d_dr_tiled, &d_dr_occ, &d_dr_redist, &d_dr_losses, &d_dr_cull_pairs, &d_dr_fd_p0, &d_dr_fd_ids, &d_dr_fd_pos, &d_dr_fd_acc, &d_mlp_params,
+                     &d_dr_pair_slot, &d_dr_tiled, &d_dr_occ, &d_dr_redist, &d_dr_losses, &d_dr_views, &d_dr_cull_pairs, &d_dr_fd_p0, &d_dr_fd_ids, &d_dr_fd_pos, &d_dr_fd_acc, &d_mlp_params,
                      &d_mlp_m, &d_mlp_v, &d_mlp_packed, &d_mlp_partial, &d_mlp_ploss, &d_mlp_grad, &d_mlp_loss,
                      &d_bfeat, &d_ds_feat, &d_ds_tgt, &d_ds_meta, &d_ds_out, &d_b_items, &d_b_offsets, &d_b_t,
                      &d_b_n, &d_b_prim};

@@ -458,6 +458,62 @@ def test_adam_step_matches_oracle():
     del x0
 
 
+def _pcg_hash(v):
+    state = (v * 747796405 + 2891336453) & 0xFFFFFFFF
+    word = (((state >> ((state >> 28) + 4)) ^ state) * 277803737) & 0xFFFFFFFF
+    return (word >> 22) ^ word
+
+
+@pytest.mark.parametrize("batch", [3, 4])
+def test_adam_steps_with_image_batch_match_oracle(batch):
+    """OptimizeFixedStructure with image_batch_size > 1
+    (MultiRendererDR.cpp:395-476): every image of a step is rendered (its
+    image and loss count), the gradient is cleared before each image, so only
+    the last image's gradient, divided by the batch size, reaches Adam. The
+    GPU renders the step's other images in the last image's primary launch.
+    Batch 4 of 4 views takes the views in order, batch 3 a hashed choice."""
+    tgt, ini, cams, refs = _problem(brick_count=4, brick_size=4)
+    n_views = len(cams)
+    dr = _preset(border_spp=16)
+    dr.opt_iterations = 3
+    dr.opt_lr = 1e-3
+    dr.image_batch_size = batch
+    wvs = np.stack([c[0] for c in cams])
+    prs = np.stack([c[1] for c in cams])
+    g = L.MultiRendererDR()
+    g.SetScene(ini)
+    g.SetReference(refs, wvs, prs)
+    sbs = L.SdfSBS(ini.header, ini.nodes, ini.values, ini.values_f.copy())
+    seed = 5
+    losses, ms = g.OptimizeFixedStructure(dr, sbs, seed=seed)
+    pairs, _ = g.GetBVH()
+    sc = oracle.Scene(ini, pairs)
+    masked = refs.copy()
+    masked[..., 3] = (np.linalg.norm(refs[..., :3], axis=-1) >= 0.001)
+    X = ini.values_f.copy()
+    V = np.zeros_like(X)
+    S = np.zeros_like(X)
+    o_losses, last_images = [], {}
+    for it in range(dr.opt_iterations):
+        sc.set_values_f(X)
+        views = [b if batch == n_views else _pcg_hash(seed ^ _pcg_hash(it * 131 + b)) % n_views for b in range(batch)]
+        step = []
+        for v in views:
+            img, gr, st = sc.dr_render_grad(dr, wvs[v], prs[v], W, H, masked[v], X.size, seed=seed, precision=32)
+            step.append(st["loss"])
+            last_images[v] = img.astype(np.float32)
+        o_losses.append(np.mean(step))
+        gr = gr.astype(np.float32) / np.float32(batch)  # the last image's gradient
+        oracle.lib().orc_adam_step(it, oracle._p(gr), oracle._p(X), oracle._p(V), oracle._p(S), X.size, dr.opt_lr, dr.opt_beta_1,
+                                   dr.opt_beta_2, dr.opt_eps)
+    print("batch", batch, "views", sorted(last_images), "losses", losses, o_losses)
+    assert batch == n_views or len(last_images) > 1
+    assert np.abs(sbs.values_f - X).max() <= 1e-6
+    assert np.allclose(losses, o_losses, rtol=1e-5, atol=0), (losses, o_losses)
+    for v, o_img in last_images.items():
+        assert np.mean(np.all(g.getLastImage(v) == o_img, axis=-1)) > 0.999, v
+
+
 @pytest.mark.parametrize("power", [2.0, 1.5])
 def test_adam_steps_with_regularization_match_oracle(power):
     """OptimizeFixedStructure with the LK_DENOISING regularization
