@@ -203,7 +203,9 @@ class MultiRendererDR : public MultiRenderer {
   const std::vector<float>& getLosses() const { return losses_; }
   // getLastImage(view), MultiRendererDR.h:86: W * H float4 (colour + coverage)
   const std::vector<float>& getLastImage(uint32_t view) {
-    last_img_.assign((size_t)w_ * h_img_ * 4, 0.0f);
+    uint32_t w = w_, hh = h_img_;
+    lrt_dr_render_size(h_, &w, &hh);  // the render size, resampled references included
+    last_img_.assign((size_t)w * hh * 4, 0.0f);
     if (!ok(lrt_dr_get_last_image(h_, view, last_img_.data()))) last_img_.clear();
     return last_img_;
   }

@@ -144,8 +144,11 @@ int lrt_batch_config(lrt_handle owner, uint32_t* chunk, uint32_t* blocks);
  * diff_render/MultiRendererDR.h:76-82, MultiRendererDR.cpp:147-185: n_views
  * RGBA images of W x H (W*H*4 floats each). masks may be NULL, then the mask is
  * 1 where |rgb| >= 0.001 (CreateRefImageMasks); the alpha used for the loss is
- * the mask. Images must already have the render size (the reference's
- * bilinear resampling, PreprocessRefImages :187-226, is not done). */
+ * the mask. The render size is the preset's render_width x render_height
+ * when both are set, else W x H (MultiRendererDR.cpp:332-349); at another size
+ * than W x H the references are resampled on the device as PreprocessRefImages
+ * does (:187-226: per render pixel the mean of ceil(W / width) x
+ * ceil(H / height) bilinear samples, WRAP addressing). */
 int lrt_dr_set_reference(lrt_handle h, const float* images, const float* masks, uint32_t n_views, uint32_t width, uint32_t height,
                          const float* world_views, const float* projs);
 /* MultiRendererDR::RenderDR (diff_render/MultiRendererDR.cpp:578-750) for one
@@ -192,6 +195,10 @@ int lrt_dr_get_last_image(lrt_handle h, uint32_t view, float* out);
  * last image rendered, divided by the step's image_batch_size as the reference
  * leaves it (MultiRendererDR.cpp:405, 475-476). */
 int lrt_dr_get_last_dloss_ds(lrt_handle h, float* out, uint64_t n);
+/* The render size of the last RenderDR / OptimizeFixedStructure (the size of
+ * MultiRendererDR::getLastImage's images, m_width x m_height of
+ * MultiRendererDR.cpp:332-349); the reference images' size before any call. */
+int lrt_dr_render_size(lrt_handle h, uint32_t* width, uint32_t* height);
 /* Border statistics of the last RenderDR: border pixels and border rays that found a relaxation point. */
 int lrt_dr_last_stats(lrt_handle h, uint32_t* border_pixels, uint32_t* border_rays_hit);
 

@@ -184,6 +184,7 @@ def _load():
         "lrt_dr_get_last_dloss_ds": (i32, [P, P, u64]),
         "lrt_dr_get_last_image": (i32, [P, u32, P]),
         "lrt_dr_last_stats": (i32, [P, C.POINTER(u32), C.POINTER(u32)]),
+        "lrt_dr_render_size": (i32, [P, C.POINTER(u32), C.POINTER(u32)]),
         "lrt_mlp_init": (i32, [P, u32]),
         "lrt_mlp_set_params": (i32, [P, P, u32]),
         "lrt_mlp_get_params": (i32, [P, P, u32]),
@@ -600,9 +601,22 @@ class MultiRendererDR(MultiRenderer):
                "SetReference")
         self.ref_shape = (n, H, W)
 
+    def RenderSize(self, preset=None):
+        """(W, H) a preset renders at (MultiRendererDR.cpp:332-349: render_width x
+        render_height when both are set, else the reference images' size); with
+        no preset, the size of the last render."""
+        if preset is not None:
+            n, H, W = self.ref_shape
+            if preset.render_width and preset.render_height:
+                return int(preset.render_width), int(preset.render_height)
+            return W, H
+        w, h = C.c_uint32(), C.c_uint32()
+        _check(lib.lrt_dr_render_size(self._h, C.byref(w), C.byref(h)), "RenderSize")
+        return w.value, h.value
+
     def RenderDR(self, preset, view, seed=0, dtdv_mode=0, n_params=None):
         """One RenderDR (MultiRendererDR.cpp:578-750): returns (image, dLoss/dS, loss, kernel_ms)."""
-        n, H, W = self.ref_shape
+        W, H = self.RenderSize(preset)
         img = np.zeros((H, W, 4), np.float32)
         g = None if n_params is None else np.zeros(n_params, np.float32)
         loss, ms = C.c_float(), C.c_float()
@@ -639,8 +653,8 @@ class MultiRendererDR(MultiRenderer):
         return out
 
     def getLastImage(self, view):
-        """MultiRendererDR::getLastImage (MultiRendererDR.h:86): (H, W, 4)."""
-        n, H, W = self.ref_shape
+        """MultiRendererDR::getLastImage (MultiRendererDR.h:86): (H, W, 4) at the last render size."""
+        W, H = self.RenderSize()
         out = np.zeros((H, W, 4), np.float32)
         _check(lib.lrt_dr_get_last_image(self._h, view, _ptr(out)), "getLastImage")
         return out

@@ -875,6 +875,53 @@ __global__ void k_redist_step(const float* __restrict__ old, float* __restrict__
   }
 }
 
+// PreprocessRefImages (MultiRendererDR.cpp:187-226): every render pixel is the
+// mean of spp_x * spp_y bilinear samples of the original image (colour, with
+// the mask's coverage in w), spp = ceil(original / render size) per axis, at
+// uv = ((spp * x + d + 0.5) / (spp * size)). The bilinear sample restates
+// LiteImage's Sampler with filter LINEAR and the default WRAP addressing (a
+// third-party header the reference includes): the linear texel filter of the
+// Vulkan specification -- texel centres at (i + 0.5) / size, the four texels
+// around u * size - 0.5 and v * size - 0.5, weights (1-a)(1-b), a(1-b),
+// (1-a)b, ab. One thread per render pixel of every image.
+__device__ __forceinline__ float4 sample_bilinear_wrap(const float4* __restrict__ img, uint32_t W0, uint32_t H0, float u, float v) {
+  const float fx = u * (float)W0 - 0.5f, fy = v * (float)H0 - 0.5f;
+  const float x0f = floorf(fx), y0f = floorf(fy);
+  const float a = fx - x0f, b = fy - y0f;
+  const int iw = (int)W0, ih = (int)H0;
+  const int x0 = (((int)x0f % iw) + iw) % iw, y0 = (((int)y0f % ih) + ih) % ih;
+  const int x1 = (x0 + 1) % iw, y1 = (y0 + 1) % ih;
+  const float4 t00 = img[(size_t)y0 * W0 + x0], t10 = img[(size_t)y0 * W0 + x1];
+  const float4 t01 = img[(size_t)y1 * W0 + x0], t11 = img[(size_t)y1 * W0 + x1];
+  const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+  return make_float4(w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x, w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y,
+                     w00 * t00.z + w10 * t10.z + w01 * t01.z + w11 * t11.z, w00 * t00.w + w10 * t10.w + w01 * t01.w + w11 * t11.w);
+}
+
+__global__ void k_dr_resample(const float4* __restrict__ src, uint32_t W0, uint32_t H0, float4* __restrict__ dst, uint32_t W, uint32_t H,
+                              uint32_t n_images) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t np = (uint64_t)W * H;
+  if (i >= np * n_images) return;
+  const uint32_t img = (uint32_t)(i / np), pix = (uint32_t)(i - (uint64_t)img * np);
+  const uint32_t x = pix % W, y = pix / W;
+  const uint32_t spp_x = (uint32_t)ceilf((float)W0 / (float)W), spp_y = (uint32_t)ceilf((float)H0 / (float)H);
+  const float n_s = (float)(spp_x * spp_y);
+  const float4* im = src + (uint64_t)img * W0 * H0;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (uint32_t dx = 0; dx < spp_x; dx++)
+    for (uint32_t dy = 0; dy < spp_y; dy++) {
+      const float u = ((float)(spp_x * x + dx) + 0.5f) / (float)(spp_x * W);
+      const float v = ((float)(spp_y * y + dy) + 0.5f) / (float)(spp_y * H);
+      const float4 c = sample_bilinear_wrap(im, W0, H0, u, v);
+      acc.x += c.x / n_s;
+      acc.y += c.y / n_s;
+      acc.z += c.z / n_s;
+      acc.w += c.w / n_s;
+    }
+  dst[i] = acc;
+}
+
 __global__ void k_dr_scale(float* grad, uint32_t n, float mult) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) grad[i] *= mult;
@@ -1009,6 +1056,13 @@ hipError_t launch_dr_redistance(float* d_values, uint32_t p, float h, uint32_t i
   }
   hipError_t e = hipMemcpyAsync(d_values, b[1 - cur], (size_t)n * 4, hipMemcpyDeviceToDevice, s);
   if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+hipError_t launch_dr_resample(const float4* d_src, uint32_t W0, uint32_t H0, float4* d_dst, uint32_t W, uint32_t H, uint32_t n_images,
+                              hipStream_t s) {
+  const uint64_t n = (uint64_t)W * H * n_images;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(lrtd::k_dr_resample, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, d_src, W0, H0, d_dst, W, H, n_images);
   return hipGetLastError();
 }
 hipError_t launch_dr_scale(float* d_grad, uint32_t n, float mult, hipStream_t s) {

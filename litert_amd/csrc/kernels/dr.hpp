@@ -81,6 +81,10 @@ hipError_t launch_dr_border_rays(const lrtd::DRArgs& A, const uint32_t* d_list, 
                                  uint32_t* d_hits, uint32_t blocks, hipStream_t s, const uint4* cull_pairs = nullptr);
 hipError_t launch_dr_gather(const lrtd::DevScene& S, float* d_out, float2* d_minmax, hipStream_t s);
 hipError_t launch_dr_scale(float* d_grad, uint32_t n, float mult, hipStream_t s);
+// PreprocessRefImages (MultiRendererDR.cpp:187-226): n_images W0 x H0 images
+// resampled to W x H (supersampled bilinear, WRAP addressing)
+hipError_t launch_dr_resample(const float4* d_src, uint32_t W0, uint32_t H0, float4* d_dst, uint32_t W, uint32_t H, uint32_t n_images,
+                              hipStream_t s);
 // Redistance (MultiRendererDR.cpp:2122-2382) of the first p^3 values (a dense
 // x-major lattice), spacing h, iters Jacobi steps; scratch: p^3 bytes + 2 x p^3 floats
 hipError_t launch_dr_redistance(float* d_values, uint32_t p, float h, uint32_t iters, uint8_t* d_frozen, float* d_b0, float* d_b1,

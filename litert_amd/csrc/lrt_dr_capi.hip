@@ -54,8 +54,33 @@ static int dr_check(Renderer* r, const MultiRendererDRPreset& p) {
   if (p.spp < 1 || p.spp > 4096) return set_err(LRT_E_INVALID, "DR spp must be 1..4096");
   if (p.border_relax_eps <= 0.f) return set_err(LRT_E_INVALID, "border_relax_eps must be > 0");
   if (r->dr_n_ref == 0) return set_err(LRT_E_INVALID, "no reference images: call lrt_dr_set_reference");
-  if ((p.render_width && p.render_width != r->dr_w) || (p.render_height && p.render_height != r->dr_h))
-    return set_err(LRT_E_UNSUPPORTED, "render size must equal the reference image size");
+  if (p.render_width > 65535 || p.render_height > 65535) return set_err(LRT_E_INVALID, "render size above 65535");
+  return LRT_OK;
+}
+
+// The render size of a preset and the references at that size
+// (MultiRendererDR.cpp:332-349): the size of the reference images when
+// render_width or render_height is 0, else the preset's; at another size than
+// the originals the references are resampled (PreprocessRefImages, :187-226,
+// launch_dr_resample). At the originals' size they are used as given: the
+// reference's bilinear sample at a pixel centre returns that pixel up to the
+// rounding of u * size - 0.5. Changing the size invalidates the last images.
+static int dr_prepare_size(Renderer* r, const MultiRendererDRPreset& p) {
+  const bool orig = p.render_width == 0 || p.render_height == 0;
+  const uint32_t W = orig ? r->dr_ref_w : p.render_width, H = orig ? r->dr_ref_h : p.render_height;
+  if (W == r->dr_w && H == r->dr_h) return LRT_OK;
+  hipError_t e;
+  if ((e = hipSetDevice(r->device)) != hipSuccess) return hip_err(e, "hipSetDevice");
+  const size_t np = (size_t)W * H;
+  if ((e = r->d_dr_ref.ensure(np * 16 * r->dr_n_ref)) != hipSuccess) return hip_err(e, "alloc reference");
+  if (W == r->dr_ref_w && H == r->dr_ref_h)
+    e = hipMemcpyAsync(r->d_dr_ref.p, r->d_dr_ref_orig.p, np * 16 * r->dr_n_ref, hipMemcpyDeviceToDevice, r->stream);
+  else
+    e = launch_dr_resample((const float4*)r->d_dr_ref_orig.p, r->dr_ref_w, r->dr_ref_h, (float4*)r->d_dr_ref.p, W, H, r->dr_n_ref, r->stream);
+  if (e != hipSuccess) return hip_err(e, "resample reference images");
+  r->dr_w = W;
+  r->dr_h = H;
+  r->dr_img_valid.assign(r->dr_n_ref, 0);
   return LRT_OK;
 }
 
@@ -229,6 +254,7 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
                           float* d_loss = nullptr, const lrtd::DRView* d_batch = nullptr, uint32_t n_batch = 0) {
   int c = dr_check(r, p);
   if (c) return c;
+  if ((c = dr_prepare_size(r, p))) return c;
   if (view >= r->dr_n_ref) return set_err(LRT_E_INVALID, "view out of range");
   hipError_t e;
   if ((e = hipSetDevice(r->device)) != hipSuccess) return hip_err(e, "hipSetDevice");
@@ -383,11 +409,14 @@ int lrt_dr_set_reference(lrt_handle h, const float* images, const float* masks, 
   }
   hipError_t e;
   if ((e = hipSetDevice(h->device)) != hipSuccess) return hip_err(e, "hipSetDevice");
-  if ((e = h->d_dr_ref.upload(ref.data(), ref.size() * 4, h->stream)) != hipSuccess) return hip_err(e, "upload reference");
+  if ((e = h->d_dr_ref_orig.upload(ref.data(), ref.size() * 4, h->stream)) != hipSuccess) return hip_err(e, "upload reference");
+  if ((e = h->d_dr_ref.ensure(ref.size() * 4)) != hipSuccess) return hip_err(e, "alloc reference");
+  if ((e = hipMemcpyAsync(h->d_dr_ref.p, h->d_dr_ref_orig.p, ref.size() * 4, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess)
+    return hip_err(e, "copy reference");
   if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_err(e, "upload reference");
   h->dr_n_ref = n;
-  h->dr_w = W;
-  h->dr_h = H;
+  h->dr_ref_w = h->dr_w = W;
+  h->dr_ref_h = h->dr_h = H;
   h->dr_img_valid.assign(n, 0);
   h->dr_world_views.assign(wvs, wvs + 16 * n);
   h->dr_projs.assign(prs, prs + 16 * n);
@@ -418,6 +447,7 @@ int lrt_dr_optimize(lrt_handle h, const MultiRendererDRPreset* p, const SdfSBSHe
   int c = lrt_set_scene_sbs(h, header, nodes, n_nodes, values, n_values, values_f, n_values_f);
   if (c) return c;
   if ((c = dr_check(h, *p))) return c;
+  if ((c = dr_prepare_size(h, *p))) return c;
   const uint32_t P = (uint32_t)n_values_f;
   hipError_t e;
   if ((e = h->d_adam_v.ensure((size_t)P * 4)) != hipSuccess) return hip_err(e, "alloc adam");
@@ -620,6 +650,13 @@ int lrt_dr_get_last_image(lrt_handle h, uint32_t view, float* out) {
   const size_t np = (size_t)h->dr_w * h->dr_h;
   hipError_t e = hipMemcpy(out, (const float4*)h->d_dr_img.p + np * view, np * 16, hipMemcpyDeviceToHost);
   return e == hipSuccess ? LRT_OK : hip_err(e, "download image");
+}
+
+int lrt_dr_render_size(lrt_handle h, uint32_t* width, uint32_t* height) {
+  if (!h) return set_err(LRT_E_INVALID, "null handle");
+  if (width) *width = h->dr_w;
+  if (height) *height = h->dr_h;
+  return LRT_OK;
 }
 
 int lrt_dr_last_stats(lrt_handle h, uint32_t* border_pixels, uint32_t* border_rays_hit) {

@@ -76,10 +76,11 @@ struct Renderer {
   uint32_t last_w = 0, last_h = 0, last_views = 0;
 
   // differentiable rendering state (MultiRendererDR)
-  uint32_t dr_n_ref = 0, dr_w = 0, dr_h = 0;
+  uint32_t dr_n_ref = 0, dr_w = 0, dr_h = 0;  // reference views; render size (d_dr_ref holds the references at it)
+  uint32_t dr_ref_w = 0, dr_ref_h = 0;       // size of the images passed to lrt_dr_set_reference (d_dr_ref_orig)
   std::vector<uint8_t> dr_img_valid;  // per reference view: its slot of d_dr_img holds the last RenderDR image
   std::vector<float> dr_world_views, dr_projs;  // 16 floats per reference view
-  DevBuf d_dr_ref, d_dr_img, d_dr_thr, d_dr_grad, d_dr_list, d_dr_misc, d_adam_v, d_adam_s;
+  DevBuf d_dr_ref_orig, d_dr_ref, d_dr_img, d_dr_thr, d_dr_grad, d_dr_list, d_dr_misc, d_adam_v, d_adam_s;
   DevBuf d_dr_bvals;    // per-brick corner values gathered from values_f before every RenderDR
   DevBuf d_dr_bminmax;  // per-brick smallest / largest gathered value
   // BVH copies for the linear-depth passes (launch_dr_cull): slot tables
@@ -129,7 +130,7 @@ struct Renderer {
   int finish_timer();
   void release_all() {
     DevBuf* all[] = {&d_pairs, &d_nodes, &d_values, &d_values_f, &d_brecs, &d_cams, &d_counter, &d_spill, &d_counts,
-                     &d_rays, &d_hits, &d_out_t, &d_out_n, &d_out_prim, &d_out_rgba, &d_dr_ref, &d_dr_img,
+                     &d_rays, &d_hits, &d_out_t, &d_out_n, &d_out_prim, &d_out_rgba, &d_dr_ref_orig, &d_dr_ref, &d_dr_img,
                      &d_dr_thr, &d_dr_grad, &d_dr_list, &d_dr_misc, &d_adam_v, &d_adam_s, &d_dr_bvals, &d_dr_bminmax, &d_dr_leaf_slot,
                      &d_dr_pair_slot, &d_dr_tiled, &d_dr_occ, &d_dr_redist, &d_dr_losses, &d_dr_views, &d_dr_cull_pairs, &d_dr_fd_p0, &d_dr_fd_ids, &d_dr_fd_pos, &d_dr_fd_acc, &d_mlp_params,
                      &d_mlp_m, &d_mlp_v, &d_mlp_packed, &d_mlp_partial, &d_mlp_ploss, &d_mlp_grad, &d_mlp_loss,

@@ -11,7 +11,9 @@ no oracle/_ref build; see lrt_oracle.h for the restated functions.
 Parity unpinned (only an import of the reference's third-party code could
 pin these; details in lrt_oracle.h): LiteMath's camera/matrix conventions
 (the GL convention used here is pinned against the closed-form pinhole),
-Embree's BVH topology, and LAMBERT's light direction to the last bit.
+Embree's BVH topology, LAMBERT's light direction to the last bit, and
+whether LiteImage's bilinear sampler wraps or clamps at the image border
+(resample_reference).
 """
 import ctypes as C
 import os
@@ -238,6 +240,47 @@ def dr_redistance(lattice, h, iters):
     if lib().orc_dr_redistance(_p(d), p, h, iters) != 0:
         raise ValueError("orc_dr_redistance failed")
     return d
+
+
+def resample_reference(images, W, H):
+    """PreprocessRefImages (diff_render/MultiRendererDR.cpp:187-226) in FP32:
+    images (n, H0, W0, 4) -- colour with the mask in w -- to (n, H, W, 4). Each
+    render pixel is the mean of ceil(W0 / W) x ceil(H0 / H) bilinear samples at
+    uv = ((spp * x + d + 0.5) / (spp * size)), accumulated in the reference's
+    order (dx outer, dy inner), each divided by the sample count. The bilinear
+    sample restates LiteImage's Sampler with filter LINEAR and its default
+    WRAP addressing (LiteImage is part of LiteMath in the absent HydraCore3
+    submodule): the Vulkan specification's linear texel filter, texel centres at
+    (i + 0.5) / size, weights (1-a)(1-b), a(1-b), (1-a)b, ab, summed in that
+    order. Whether LiteImage wraps or clamps at the image border is parity
+    unpinned; the rest is pinned in tests/test_cpu_oracle_pins.py."""
+    imgs = np.ascontiguousarray(images, np.float32)
+    n, H0, W0, _ = imgs.shape
+    f = np.float32
+    spp_x = int(np.ceil(f(W0) / f(W)))
+    spp_y = int(np.ceil(f(H0) / f(H)))
+    n_s = f(spp_x * spp_y)
+    xs, ys = np.arange(W, dtype=np.int64), np.arange(H, dtype=np.int64)
+    acc = np.zeros((n, H, W, 4), np.float32)
+    for dx in range(spp_x):
+        for dy in range(spp_y):
+            u = ((spp_x * xs + dx).astype(f) + f(0.5)) / f(spp_x * W)
+            v = ((spp_y * ys + dy).astype(f) + f(0.5)) / f(spp_y * H)
+            fx, fy = u * f(W0) - f(0.5), v * f(H0) - f(0.5)
+            x0f, y0f = np.floor(fx), np.floor(fy)
+            a, b = fx - x0f, fy - y0f
+            x0, y0 = x0f.astype(np.int64) % W0, y0f.astype(np.int64) % H0
+            x1, y1 = (x0 + 1) % W0, (y0 + 1) % H0
+            oa, ob = f(1.0) - a, f(1.0) - b
+            w00 = (oa[None, :] * ob[:, None])[None, :, :, None]
+            w10 = (a[None, :] * ob[:, None])[None, :, :, None]
+            w01 = (oa[None, :] * b[:, None])[None, :, :, None]
+            w11 = (a[None, :] * b[:, None])[None, :, :, None]
+            t00, t10 = imgs[:, y0][:, :, x0], imgs[:, y0][:, :, x1]
+            t01, t11 = imgs[:, y1][:, :, x0], imgs[:, y1][:, :, x1]
+            c = w00 * t00 + w10 * t10 + w01 * t01 + w11 * t11
+            acc += c / n_s
+    return acc
 
 
 def eye_ray(world_view, proj, x, y, dx, dy, W, H, precision=32):

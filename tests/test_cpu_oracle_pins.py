@@ -14,6 +14,12 @@
   perspective / lookAt conventions): every pixel's ray of a non-square image
   against the closed-form pinhole direction, which fixes the horizontal field
   of view and the aspect ratio.
+- Reference-image resampling (PreprocessRefImages, MultiRendererDR.cpp:187-226)
+  by what a supersampled bilinear filter fixes: the same size returns the
+  image, an integer downsampling factor k returns the exact k x k block mean
+  (sample count, positions and weights), a constant image stays constant at
+  any size, and a linear ramp is reproduced in the interior (texel centres at
+  (i + 0.5) / size and the bilinear weights).
 """
 import numpy as np
 import pytest
@@ -221,3 +227,51 @@ def test_eye_rays_match_closed_form_pinhole_non_square():
             assert np.allclose(ray[:3], eye, atol=1e-5)
             worst = max(worst, np.linalg.norm(ray[4:7] - d))
     assert worst < 1e-6, worst
+
+
+# ---- reference-image resampling --------------------------------------------
+def _rand_images(n, H, W, seed):
+    return np.random.default_rng(seed).uniform(0, 1, (n, H, W, 4)).astype(np.float32)
+
+
+def test_resample_same_size_returns_the_image():
+    """Up to the rounding of u * size - 0.5 (weights of a few 1e-6 on the
+    neighbouring texel), which the library skips by passing references of the
+    render size through unchanged."""
+    img = _rand_images(2, 37, 53, 1)
+    out = oracle.resample_reference(img, 53, 37)
+    assert np.max(np.abs(out - img)) <= 1e-5
+
+
+@pytest.mark.parametrize("k", [2, 3])
+def test_resample_integer_downsampling_is_the_block_mean(k):
+    W, H = 24, 16
+    img = _rand_images(2, H * k, W * k, 2 + k)
+    out = oracle.resample_reference(img, W, H)
+    block = img.astype(np.float64).reshape(2, H, k, W, k, 4).mean(axis=(2, 4))
+    assert np.max(np.abs(out - block)) <= 2e-6
+
+
+@pytest.mark.parametrize("size", [(40, 30), (64, 48), (100, 75)])
+def test_resample_keeps_a_constant_image(size):
+    img = np.full((1, 48, 64, 4), 0.3, np.float32)
+    out = oracle.resample_reference(img, *size)
+    assert out.shape == (1, size[1], size[0], 4)
+    assert np.max(np.abs(out - 0.3)) <= 1e-6
+
+
+def test_resample_reproduces_a_linear_ramp_inside():
+    """Upsampling 40 -> 60 (one sample per pixel): at render pixel x the sample
+    sits at texel coordinate (x + 0.5) * 40 / 60 - 0.5, and a bilinear filter
+    returns the ramp's value there wherever both texels are inside."""
+    W0, H0, W, H = 40, 20, 60, 30
+    ramp = np.zeros((1, H0, W0, 4), np.float32)
+    ramp[0, :, :, 0] = np.arange(W0, dtype=np.float32)[None, :] * 0.25
+    ramp[0, :, :, 1] = np.arange(H0, dtype=np.float32)[:, None] * 0.5
+    out = oracle.resample_reference(ramp, W, H)
+    tx = (np.arange(W) + 0.5) * W0 / W - 0.5
+    ty = (np.arange(H) + 0.5) * H0 / H - 0.5
+    ix = (tx >= 0) & (tx <= W0 - 1)
+    iy = (ty >= 0) & (ty <= H0 - 1)
+    assert np.max(np.abs(out[0][:, ix, 0] - 0.25 * tx[ix][None, :])) <= 1e-5
+    assert np.max(np.abs(out[0][iy, :, 1] - 0.5 * ty[iy][:, None])) <= 1e-5

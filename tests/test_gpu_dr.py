@@ -71,6 +71,47 @@ def test_render_dr_matches_oracle(problem, dtdv_mode):
     assert np.linalg.norm(grad - o64) / nz <= 1e-4
 
 
+@pytest.mark.parametrize("ref_size", [192, 64, 120])
+def test_render_dr_resamples_reference_images(ref_size):
+    """References of another size than the render (preset render_width x
+    render_height, MultiRendererDR.cpp:332-349) are resampled on the device as
+    PreprocessRefImages does (:187-226): downsampling by 2 (block means),
+    upsampling (one bilinear sample per pixel) and a non-integer ratio
+    (2 samples per axis). The RenderDR that follows must match the oracle fed
+    with the oracle's resampled references."""
+    tgt, ini, cams, _ = _problem(brick_count=4, brick_size=4)
+    r = L.CreateMultiRenderer(L.DEVICE_GPU)
+    r.SetScene(tgt)
+    rp = L.getDefaultPreset()
+    rp.sdf_node_intersect = L.SDF_OCTREE_NODE_INTERSECT_NEWTON
+    rp.render_mode = L.MULTI_RENDER_MODE_LINEAR_DEPTH
+    big = np.stack([r.RenderFloat(ref_size, ref_size, wv, pr, rp) for wv, pr in cams])
+    r.close()
+    wvs = np.stack([c[0] for c in cams])
+    prs = np.stack([c[1] for c in cams])
+    g = L.MultiRendererDR()
+    g.SetScene(ini)
+    g.SetReference(big, wvs, prs)
+    dr = _preset()
+    dr.render_width = dr.render_height = W
+    P = ini.values_f.size
+    img, grad, loss, ms = g.RenderDR(dr, 1, seed=77, n_params=P)
+    assert img.shape == (H, W, 4) and g.RenderSize() == (W, H)
+    masked = big.copy()
+    masked[..., 3] = (np.linalg.norm(big[..., :3], axis=-1) >= 0.001)
+    ref = oracle.resample_reference(masked, W, H)[1]
+    sc = oracle.Scene(ini, g.GetBVH()[0])
+    o_img, o32, st32 = sc.dr_render_grad(dr, wvs[1], prs[1], W, H, ref, P, seed=77, precision=32)
+    print(ref_size, "loss", loss, st32["loss"], "mask levels", np.unique(ref[..., 3]).size)
+    assert np.mean(np.all(img == o_img.astype(np.float32), axis=-1)) > 0.999
+    assert abs(loss - st32["loss"]) <= 1e-5 * max(1.0, abs(st32["loss"]))
+    assert np.linalg.norm(grad - o32) / np.linalg.norm(o32) <= 1e-5
+    # back to the references' own size
+    dr.render_width = dr.render_height = 0
+    img2, _, _, _ = g.RenderDR(dr, 1, seed=77, n_params=P)
+    assert img2.shape == (ref_size, ref_size, 4) and g.RenderSize() == (ref_size, ref_size)
+
+
 def _random_colours(sbs, seed):
     """The grid scenes carry one constant colour; this gives every colour
     point of an ID32F_IRGB32F scene (3 floats at each index stored after a
