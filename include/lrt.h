@@ -287,6 +287,29 @@ int lrt_sbs_from_mesh(const float* vertices, uint32_t n_vertices, const uint32_t
 int lrt_sbs_from_mesh_device(lrt_handle h, const float* vertices, uint32_t n_vertices, const uint32_t* indices, uint32_t n_triangles,
                              uint32_t lod, uint32_t brick_size, uint32_t brick_pad, uint32_t bytes_per_value, int voxel_diag_scale,
                              lrt_sbs_handle* out, float* build_ms);
+/* sdf_converter::create_sdf_SBS(settings, header, sdf) (utils/sdf_converter.cpp:
+ * 169-181 for a distance function, :183-193 for a mesh) on the GPU of h: the
+ * reference's adaptive frame octree (construct_sdf_frame_octree,
+ * utils/sparse_octree_builder.cpp:125-258: every cell of level min(depth, 4),
+ * then cells split while their centre lies within their half-diagonal of the
+ * surface or their corners change sign, down to `depth`) and
+ * frame_octree_to_SBS (:291-444): every leaf whose lattice reaches the
+ * surface band becomes a brick at its own LOD (mixed LODs), values quantised
+ * over the brick diagonal with the reference's float expression (aux_data
+ * LRT_SBS_LAYOUT_DX, no voxel-diagonal flag). Bricks are ordered by LOD, then Morton order (the reference's
+ * order depends on its threads). Frames above 2^24 nodes, where the
+ * reference's frame_octree_limit_nodes would merge nodes, are refused.
+ * The mesh form takes the exact distance to the mesh with the sign of the
+ * closest feature's pseudo-normal (the reference signs with the closest
+ * triangle's face normal that Embree's point query returns). The primitive
+ * form takes LRT_PRIMITIVE_SPHERE (params: centre xyz, radius) or
+ * LRT_PRIMITIVE_TORUS (around the z axis; params: R, r), evaluated in float
+ * as length(p - c) - radius and length((length(p.xy) - R, p.z)) - r. */
+int lrt_sbs_octree_from_mesh_device(lrt_handle h, const float* vertices, uint32_t n_vertices, const uint32_t* indices, uint32_t n_triangles,
+                                    uint32_t depth, uint32_t brick_size, uint32_t brick_pad, uint32_t bytes_per_value, lrt_sbs_handle* out,
+                                    float* build_ms);
+int lrt_sbs_octree_from_primitive_device(lrt_handle h, uint32_t shape, const float* params, uint32_t depth, uint32_t brick_size,
+                                         uint32_t brick_pad, uint32_t bytes_per_value, lrt_sbs_handle* out, float* build_ms);
 /* The displaced icosphere mesh that lrt_sbs_displaced_icosphere builds from
  * (no reference counterpart). NULL vertices / indices: only the counts. */
 int lrt_mesh_displaced_icosphere(uint32_t subdiv, float radius, float amplitude, uint32_t octaves, uint32_t seed, float* vertices,

@@ -80,6 +80,10 @@ extern "C" {
 #define LRT_SBS_LAYOUT_ID32F_IRGB32F    (4u << 24)
 #define LRT_SBS_LAYOUT_ID32F_IRGB32F_IN (5u << 24)
 #define LRT_SBS_LAYOUT_MASK             0xFF000000u
+
+/* distance fields of lrt_sbs_octree_from_primitive_device (no reference counterpart) */
+#define LRT_PRIMITIVE_SPHERE 0u
+#define LRT_PRIMITIVE_TORUS  1u
 #define LRT_INVALID_IDX                 (1u << 31)
 
 /* ---- BVH leaf encoding (raytrace_common.h:36-57, cbvh.h:17-19) ---------- */

@@ -49,6 +49,183 @@ TriMesh to_mesh(const float* vertices, uint32_t n_vertices, const uint32_t* indi
   return m;
 }
 
+// The mesh and its distance-query data (triangle BVH, face / vertex / edge
+// pseudo-normals) on the device of r; buffers freed with the struct.
+struct MeshUpload {
+  DevBuf v, tri, fn, vn, en, nodes, prims;
+  lrtd::DevMesh M{};
+  ~MeshUpload() {
+    DevBuf* all[] = {&v, &tri, &fn, &vn, &en, &nodes, &prims};
+    for (DevBuf* b : all) b->release();
+  }
+};
+
+int upload_mesh(Renderer* r, const float* vertices, uint32_t n_vertices, const uint32_t* indices, uint32_t n_triangles, MeshUpload& U) {
+  MeshSdfData D;
+  try {
+    D = mesh_sdf_data(to_mesh(vertices, n_vertices, indices, n_triangles));
+  } catch (const std::exception& ex) {
+    return set_err(LRT_E_INVALID, ex.what());
+  }
+  if (D.tree.max_depth + 2 > lrtd::kBuildStack) return set_err(LRT_E_UNSUPPORTED, "triangle BVH too deep for the device builder");
+  const uint32_t nt = n_triangles, nn = (uint32_t)D.tree.nodes.size();
+  std::vector<float4> v4(n_vertices), fn(nt), vn(n_vertices), en(3ull * nt);
+  std::vector<uint4> tri(nt), nodes(3ull * nn);
+  for (uint32_t i = 0; i < n_vertices; i++) {
+    v4[i] = make_float4(D.mesh.v[i].x, D.mesh.v[i].y, D.mesh.v[i].z, 0.f);
+    vn[i] = make_float4(D.vnorm[i].x, D.vnorm[i].y, D.vnorm[i].z, 0.f);
+  }
+  for (uint32_t t = 0; t < nt; t++) {
+    tri[t] = make_uint4(D.mesh.idx[3 * t], D.mesh.idx[3 * t + 1], D.mesh.idx[3 * t + 2], 0);
+    fn[t] = make_float4(D.fnorm[t].x, D.fnorm[t].y, D.fnorm[t].z, 0.f);
+    for (int k = 0; k < 3; k++) en[3 * t + k] = make_float4(D.enorm3[3 * t + k].x, D.enorm3[3 * t + k].y, D.enorm3[3 * t + k].z, 0.f);
+  }
+  auto fbits = [](float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+  };
+  for (uint32_t i = 0; i < nn; i++) {
+    const BuildNode& n = D.tree.nodes[i];
+    nodes[3 * i] = make_uint4(fbits(n.bmin.x), fbits(n.bmin.y), fbits(n.bmin.z), (uint32_t)n.left);
+    nodes[3 * i + 1] = make_uint4(fbits(n.bmax.x), fbits(n.bmax.y), fbits(n.bmax.z), (uint32_t)n.right);
+    nodes[3 * i + 2] = make_uint4(n.start, n.count, 0, 0);
+  }
+  hipStream_t s = r->stream;
+  hipError_t e;
+  if ((e = U.v.upload(v4.data(), v4.size() * 16, s)) != hipSuccess || (e = U.tri.upload(tri.data(), tri.size() * 16, s)) != hipSuccess ||
+      (e = U.fn.upload(fn.data(), fn.size() * 16, s)) != hipSuccess || (e = U.vn.upload(vn.data(), vn.size() * 16, s)) != hipSuccess ||
+      (e = U.en.upload(en.data(), en.size() * 16, s)) != hipSuccess ||
+      (e = U.nodes.upload(nodes.data(), nodes.size() * 16, s)) != hipSuccess ||
+      (e = U.prims.upload(D.tree.prims.data(), D.tree.prims.size() * 4, s)) != hipSuccess)
+    return hip_err(e, "upload mesh");
+  // the staging vectors die with this call
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_err(e, "upload mesh");
+  U.M = lrtd::DevMesh{(const float4*)U.v.p, (const uint4*)U.tri.p, (const float4*)U.fn.p, (const float4*)U.vn.p, (const float4*)U.en.p,
+                      (const uint4*)U.nodes.p, (const uint32_t*)U.prims.p, D.tree.root};
+  return LRT_OK;
+}
+
+// create_sdf_SBS(settings, header, sdf) (utils/sdf_converter.cpp:169-181) on
+// the device: the frame octree of construct_sdf_frame_octree
+// (sparse_octree_builder.cpp:125-258) level by level -- every cell of the last
+// uniform level min(depth, 4), then the children of every split cell -- and
+// frame_octree_to_SBS (:291-444) over its leaves. Bricks are ordered by LOD,
+// then Morton order (the reference appends them from parallel threads).
+int build_octree(Renderer* r, const lrtd::DevField& F, uint32_t depth, uint32_t brick_size, uint32_t brick_pad, uint32_t bytes_per_value,
+                 lrt_sbs_handle* out) {
+  if (depth < 2 || depth > 12) return set_err(LRT_E_INVALID, "octree depth must be 2..12");
+  if (brick_size < 1 || brick_size > 16 || brick_pad > 1) return set_err(LRT_E_INVALID, "bad brick size/pad");
+  if (bytes_per_value != 1 && bytes_per_value != 2 && bytes_per_value != 4) return set_err(LRT_E_INVALID, "bad bytes_per_value");
+  lrtd::OctreeLevels T{};
+  for (int l = 0; l < 16; l++) {
+    const double b = std::sqrt(3.0) * std::pow(2.0, -l);  // is_border, :38-41, in double
+    float bf = (float)b;
+    if ((double)bf < b) bf = std::nextafter(bf, INFINITY);  // |v| < b  <=>  |v| < bf for float v
+    T.border[l] = bf;
+    T.near[l] = std::sqrt(3.0f) * std::ldexp(1.0f, -l);                  // add_node_rec, :84
+    T.keep[l] = (float)(2 * std::sqrt(3.0) / (double)(1u << l));        // is_border_node :51, quantiser :401
+  }
+  hipStream_t s = r->stream;
+  hipError_t e;
+  DevBuf b_cells, b_vals, b_kids, b_leaves, b_lvals, b_count, b_fill, b_keep, b_words;
+  struct Release {
+    std::vector<DevBuf*> bufs;
+    ~Release() {
+      for (DevBuf* b : bufs) b->release();
+    }
+  } rel{{&b_cells, &b_vals, &b_kids, &b_leaves, &b_lvals, &b_count, &b_fill, &b_keep, &b_words}};
+  const uint32_t top = depth < 4 ? depth : 4;
+  const uint32_t side = 1u << top;
+  std::vector<uint4> cells;
+  cells.reserve((size_t)side * side * side);
+  for (uint32_t x = 0; x < side; x++)
+    for (uint32_t y = 0; y < side; y++)
+      for (uint32_t z = 0; z < side; z++) cells.push_back(make_uint4(x, y, z, top));
+  uint64_t frame_nodes = 0;
+  for (uint32_t l = 0; l <= top; l++) frame_nodes += 1ull << (3 * l);
+  if ((e = b_cells.upload(cells.data(), cells.size() * 16, s)) != hipSuccess) return hip_err(e, "upload cells");
+  if ((e = b_count.ensure(8)) != hipSuccess) return hip_err(e, "alloc");
+  std::vector<uint4> leaves;
+  std::vector<float> lvals;
+  uint32_t n = (uint32_t)cells.size();
+  for (uint32_t level = top; n > 0; level++) {
+    if ((e = b_vals.ensure(9ull * n * 4)) != hipSuccess || (e = b_kids.ensure(8ull * n * 16)) != hipSuccess ||
+        (e = b_leaves.ensure((size_t)n * 16)) != hipSuccess || (e = b_lvals.ensure(8ull * n * 4)) != hipSuccess)
+      return hip_err(e, "alloc octree level");
+    if ((e = zero_u32((uint32_t*)b_count.p, 2, s)) != hipSuccess) return hip_err(e, "clear");
+    if ((e = launch_octree_eval(F, (const uint4*)b_cells.p, n, (float*)b_vals.p, s)) != hipSuccess) return hip_err(e, "octree eval");
+    if ((e = launch_octree_classify((const uint4*)b_cells.p, (const float*)b_vals.p, n, level, level == top ? 1 : 0, depth, T,
+                                    (uint4*)b_kids.p, (uint32_t*)b_count.p, (uint4*)b_leaves.p, (float*)b_lvals.p,
+                                    (uint32_t*)b_count.p + 1, s)) != hipSuccess)
+      return hip_err(e, "octree classify");
+    uint32_t cnt[2];
+    if ((e = hipMemcpyAsync(cnt, b_count.p, 8, hipMemcpyDeviceToHost, s)) != hipSuccess || (e = hipStreamSynchronize(s)) != hipSuccess)
+      return hip_err(e, "octree counts");
+    const size_t l0 = leaves.size();
+    leaves.resize(l0 + cnt[1]);
+    lvals.resize(8 * (l0 + cnt[1]));
+    if (cnt[1] && ((e = hipMemcpyAsync(leaves.data() + l0, b_leaves.p, (size_t)cnt[1] * 16, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+                   (e = hipMemcpyAsync(lvals.data() + 8 * l0, b_lvals.p, (size_t)cnt[1] * 32, hipMemcpyDeviceToHost, s)) != hipSuccess))
+      return hip_err(e, "download leaves");
+    frame_nodes += cnt[0];
+    // frame_octree_limit_nodes (:2229-2330) leaves frames of up to 2^24 nodes alone
+    if (frame_nodes > (1ull << 24)) return set_err(LRT_E_UNSUPPORTED, "frame octree above 2^24 nodes (the reference would merge nodes)");
+    std::swap(b_cells, b_kids);
+    n = cnt[0];
+  }
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_err(e, "download leaves");
+  const uint32_t nl = (uint32_t)leaves.size();
+  std::vector<uint32_t> order(nl);
+  for (uint32_t i = 0; i < nl; i++) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    if (leaves[a].w != leaves[b].w) return leaves[a].w < leaves[b].w;
+    return morton3(leaves[a].x, leaves[a].y, leaves[a].z) < morton3(leaves[b].x, leaves[b].y, leaves[b].z);
+  });
+  std::vector<uint4> sl(nl);
+  std::vector<float> sv(8ull * nl);
+  for (uint32_t i = 0; i < nl; i++) {
+    sl[i] = leaves[order[i]];
+    std::copy(lvals.begin() + 8ull * order[i], lvals.begin() + 8ull * order[i] + 8, sv.begin() + 8ull * i);
+  }
+  const uint32_t v_size = brick_size + 2 * brick_pad + 1, nv = v_size * v_size * v_size;
+  const uint32_t vals_per_int = 4 / bytes_per_value, n_words = (nv + vals_per_int - 1) / vals_per_int;
+  if ((e = b_leaves.upload(sl.data(), (size_t)nl * 16, s)) != hipSuccess || (e = b_lvals.upload(sv.data(), sv.size() * 4, s)) != hipSuccess ||
+      (e = b_fill.ensure((size_t)nl * nv * 4)) != hipSuccess || (e = b_keep.ensure((size_t)nl * 4)) != hipSuccess ||
+      (e = b_words.ensure((size_t)nl * n_words * 4)) != hipSuccess)
+    return hip_err(e, "alloc bricks");
+  if ((e = launch_octree_fill(F, (const uint4*)b_leaves.p, (const float*)b_lvals.p, nl, brick_size, brick_pad, (float*)b_fill.p, s)) !=
+      hipSuccess)
+    return hip_err(e, "octree fill");
+  if ((e = launch_octree_pack((const uint4*)b_leaves.p, (const float*)b_fill.p, nl, nv, bytes_per_value, T, (uint32_t*)b_keep.p,
+                              (uint32_t*)b_words.p, s)) != hipSuccess)
+    return hip_err(e, "octree pack");
+  std::vector<uint32_t> keep(nl), words((size_t)nl * n_words);
+  if ((e = hipMemcpyAsync(keep.data(), b_keep.p, (size_t)nl * 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+      (e = hipMemcpyAsync(words.data(), b_words.p, words.size() * 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+      (e = hipStreamSynchronize(s)) != hipSuccess)
+    return hip_err(e, "download bricks");
+  std::unique_ptr<lrt_sbs> sb(new lrt_sbs);
+  SbsScene& S = sb->s;
+  S.header.brick_size = brick_size;
+  S.header.brick_pad = brick_pad;
+  S.header.bytes_per_value = bytes_per_value;
+  S.header.aux_data = LRT_SBS_LAYOUT_DX;  // values over the brick diagonal, as the reference quantises
+  for (uint32_t b = 0; b < nl; b++) {
+    if (!keep[b]) continue;
+    SdfSBSNode nd;
+    nd.pos_xy = (sl[b].x << 16) | sl[b].y;
+    nd.pos_z_lod_size = (sl[b].z << 16) | (1u << sl[b].w);
+    nd.data_offset = (uint32_t)S.values.size();
+    nd._pad = 0;
+    S.nodes.push_back(nd);
+    S.values.insert(S.values.end(), words.begin() + (size_t)b * n_words, words.begin() + (size_t)(b + 1) * n_words);
+  }
+  if (S.nodes.empty()) return set_err(LRT_E_INVALID, "scene produced no bricks");
+  *out = sb.release();
+  return LRT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -100,53 +277,17 @@ int lrt_sbs_from_mesh_device(lrt_handle h, const float* vertices, uint32_t n_ver
   hipError_t e;
   if ((e = hipSetDevice(r->device)) != hipSuccess) return hip_err(e, "hipSetDevice");
   const auto t0 = std::chrono::steady_clock::now();
-  MeshSdfData D;
-  try {
-    D = mesh_sdf_data(to_mesh(vertices, n_vertices, indices, n_triangles));
-  } catch (const std::exception& ex) {
-    return set_err(LRT_E_INVALID, ex.what());
-  }
-  if (D.tree.max_depth + 2 > lrtd::kBuildStack) return set_err(LRT_E_UNSUPPORTED, "triangle BVH too deep for the device builder");
-  // device copies of the mesh and its distance-query data
-  const uint32_t nt = n_triangles, nn = (uint32_t)D.tree.nodes.size();
-  std::vector<float4> v4(n_vertices), fn(nt), vn(n_vertices), en(3ull * nt);
-  std::vector<uint4> tri(nt), nodes(3ull * nn);
-  for (uint32_t i = 0; i < n_vertices; i++) {
-    v4[i] = make_float4(D.mesh.v[i].x, D.mesh.v[i].y, D.mesh.v[i].z, 0.f);
-    vn[i] = make_float4(D.vnorm[i].x, D.vnorm[i].y, D.vnorm[i].z, 0.f);
-  }
-  for (uint32_t t = 0; t < nt; t++) {
-    tri[t] = make_uint4(D.mesh.idx[3 * t], D.mesh.idx[3 * t + 1], D.mesh.idx[3 * t + 2], 0);
-    fn[t] = make_float4(D.fnorm[t].x, D.fnorm[t].y, D.fnorm[t].z, 0.f);
-    for (int k = 0; k < 3; k++) en[3 * t + k] = make_float4(D.enorm3[3 * t + k].x, D.enorm3[3 * t + k].y, D.enorm3[3 * t + k].z, 0.f);
-  }
-  auto fbits = [](float f) {
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    return u;
-  };
-  for (uint32_t i = 0; i < nn; i++) {
-    const BuildNode& n = D.tree.nodes[i];
-    nodes[3 * i] = make_uint4(fbits(n.bmin.x), fbits(n.bmin.y), fbits(n.bmin.z), (uint32_t)n.left);
-    nodes[3 * i + 1] = make_uint4(fbits(n.bmax.x), fbits(n.bmax.y), fbits(n.bmax.z), (uint32_t)n.right);
-    nodes[3 * i + 2] = make_uint4(n.start, n.count, 0, 0);
-  }
-  DevBuf b_v, b_tri, b_fn, b_vn, b_en, b_nodes, b_prims, b_cur, b_next, b_count, b_vals, b_keep, b_words;
+  MeshUpload U;
+  if ((c = upload_mesh(r, vertices, n_vertices, indices, n_triangles, U)) != LRT_OK) return c;
+  const lrtd::DevMesh& M = U.M;
+  DevBuf b_cur, b_next, b_count, b_vals, b_keep, b_words;
   struct Release {
     std::vector<DevBuf*> bufs;
     ~Release() {
       for (DevBuf* b : bufs) b->release();
     }
-  } rel{{&b_v, &b_tri, &b_fn, &b_vn, &b_en, &b_nodes, &b_prims, &b_cur, &b_next, &b_count, &b_vals, &b_keep, &b_words}};
+  } rel{{&b_cur, &b_next, &b_count, &b_vals, &b_keep, &b_words}};
   hipStream_t s = r->stream;
-  if ((e = b_v.upload(v4.data(), v4.size() * 16, s)) != hipSuccess || (e = b_tri.upload(tri.data(), tri.size() * 16, s)) != hipSuccess ||
-      (e = b_fn.upload(fn.data(), fn.size() * 16, s)) != hipSuccess || (e = b_vn.upload(vn.data(), vn.size() * 16, s)) != hipSuccess ||
-      (e = b_en.upload(en.data(), en.size() * 16, s)) != hipSuccess ||
-      (e = b_nodes.upload(nodes.data(), nodes.size() * 16, s)) != hipSuccess ||
-      (e = b_prims.upload(D.tree.prims.data(), D.tree.prims.size() * 4, s)) != hipSuccess)
-    return hip_err(e, "upload mesh");
-  lrtd::DevMesh M{(const float4*)b_v.p, (const uint4*)b_tri.p, (const float4*)b_fn.p, (const float4*)b_vn.p, (const float4*)b_en.p,
-                  (const uint4*)b_nodes.p, (const uint32_t*)b_prims.p, D.tree.root};
 
   // coarse-to-fine candidate cells, level by level (the host builder's rule)
   const float kSqrt3 = 1.7320508075688772f;
@@ -212,6 +353,44 @@ int lrt_sbs_from_mesh_device(lrt_handle h, const float* vertices, uint32_t n_ver
   if (build_ms) *build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (S.nodes.empty()) return set_err(LRT_E_INVALID, "scene produced no bricks");
   *out = sb.release();
+  return LRT_OK;
+}
+
+
+int lrt_sbs_octree_from_mesh_device(lrt_handle h, const float* vertices, uint32_t n_vertices, const uint32_t* indices, uint32_t n_triangles,
+                                    uint32_t depth, uint32_t brick_size, uint32_t brick_pad, uint32_t bytes_per_value, lrt_sbs_handle* out,
+                                    float* build_ms) {
+  if (!h) return set_err(LRT_E_INVALID, "null handle");
+  int c = check_mesh_args(vertices, n_vertices, indices, n_triangles, 1, brick_size, brick_pad, bytes_per_value, out);
+  if (c) return c;
+  Renderer* r = h;
+  hipError_t e;
+  if ((e = hipSetDevice(r->device)) != hipSuccess) return hip_err(e, "hipSetDevice");
+  const auto t0 = std::chrono::steady_clock::now();
+  MeshUpload U;
+  if ((c = upload_mesh(r, vertices, n_vertices, indices, n_triangles, U)) != LRT_OK) return c;
+  lrtd::DevField F{};
+  F.kind = 0;
+  F.mesh = U.M;
+  if ((c = build_octree(r, F, depth, brick_size, brick_pad, bytes_per_value, out)) != LRT_OK) return c;
+  if (build_ms) *build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return LRT_OK;
+}
+
+int lrt_sbs_octree_from_primitive_device(lrt_handle h, uint32_t shape, const float* params, uint32_t depth, uint32_t brick_size,
+                                         uint32_t brick_pad, uint32_t bytes_per_value, lrt_sbs_handle* out, float* build_ms) {
+  if (!h || !params || !out) return set_err(LRT_E_INVALID, "null argument");
+  if (shape != LRT_PRIMITIVE_SPHERE && shape != LRT_PRIMITIVE_TORUS) return set_err(LRT_E_INVALID, "unknown primitive");
+  Renderer* r = h;
+  hipError_t e;
+  if ((e = hipSetDevice(r->device)) != hipSuccess) return hip_err(e, "hipSetDevice");
+  const auto t0 = std::chrono::steady_clock::now();
+  lrtd::DevField F{};
+  F.kind = shape == LRT_PRIMITIVE_SPHERE ? 1u : 2u;
+  for (int k = 0; k < (shape == LRT_PRIMITIVE_SPHERE ? 4 : 2); k++) F.a[k] = params[k];
+  int c = build_octree(r, F, depth, brick_size, brick_pad, bytes_per_value, out);
+  if (c) return c;
+  if (build_ms) *build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return LRT_OK;
 }
 

@@ -25,7 +25,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD_DIR = os.path.join(_HERE, "_build")
 LIB_PATH = os.path.join(BUILD_DIR, "liblrt_oracle.so")
 BASELINE_LIB_PATH = os.path.join(BUILD_DIR, "liblrt_oracle_baseline.so")
-SOURCES = ["oracle_api.cpp", "oracle_dr.cpp", "oracle_mlp.cpp"]
+SOURCES = ["oracle_api.cpp", "oracle_dr.cpp", "oracle_mlp.cpp", "oracle_octree.cpp"]
 # -ffp-contract=off matches the reference's ISO C++17 build (no FP contraction).
 FLAGS = ["-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC", "-shared",
          "-Wall", "-Wno-unused-function"]
@@ -90,6 +90,7 @@ def _load(path):
         "orc_mlp_loss_grad": (f64, [P, P, P, u32, i32, P]),
         "orc_mlp_brick_features": (i32, [P, u32, P]),
         "orc_mlp_features": (i32, [P, P, u32, P, P]),
+        "orc_sbs_octree": (i32, [i32, P, u32, u32, u32, u32, P, P, P, P]),
     }
     for name, (res, args) in list(sig.items()) + list(optional.items()):
         if name in optional and not hasattr(L, name):
@@ -240,6 +241,26 @@ def dr_redistance(lattice, h, iters):
     if lib().orc_dr_redistance(_p(d), p, h, iters) != 0:
         raise ValueError("orc_dr_redistance failed")
     return d
+
+
+def sbs_octree(shape, params, depth, brick_size, brick_pad, bytes_per_value):
+    """create_sdf_SBS(settings, header, sdf) (utils/sdf_converter.cpp:169-181)
+    of a sphere (shape 0, params centre xyz + radius) or a torus around the z
+    axis (shape 1, params R, r): (nodes (n, 4) uint32, values uint32) in the
+    frame octree's order."""
+    prm = np.zeros(4, np.float32)
+    prm[:len(params)] = params
+    nn, nv = C.c_uint32(), C.c_uint64()
+    L = lib()
+    rc = L.orc_sbs_octree(shape, _p(prm), depth, brick_size, brick_pad, bytes_per_value, C.byref(nn), None, C.byref(nv), None)
+    if rc != 0:
+        raise ValueError(f"orc_sbs_octree failed {rc}")
+    nodes = np.zeros((nn.value, 4), np.uint32)
+    values = np.zeros(nv.value, np.uint32)
+    rc = L.orc_sbs_octree(shape, _p(prm), depth, brick_size, brick_pad, bytes_per_value, C.byref(nn), _p(nodes), C.byref(nv), _p(values))
+    if rc != 0:
+        raise ValueError(f"orc_sbs_octree failed {rc}")
+    return nodes, values
 
 
 def resample_reference(images, W, H):
