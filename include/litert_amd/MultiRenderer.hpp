@@ -101,6 +101,8 @@ class MultiRenderer {
            ok(lrt_set_scene_sbs(h_, &s.header, s.nodes, s.size, s.values, s.values_count, s.values_f, s.values_f_count));
   }
   bool SetScene(const SdfSBS& s) { return SetScene(s.view()); }
+  // LoadScene(path) / LoadSceneHydra, eye_ray_host.cpp:52-121 (one identity sdf_sbs instance)
+  bool LoadScene(const char* path) { return ok(lrt_load_scene_hydra(h_, path)); }
   bool SetPreset(const MultiRenderPreset& p) {
     preset_ = p;
     return ok(lrt_set_preset(h_, &p));

@@ -326,6 +326,15 @@ int lrt_sbs_save_file(const char* path, const SdfSBSHeader* header, const SdfSBS
                       uint32_t n_values, const float* values_f, uint32_t n_values_f);
 /* LiteMath lookAt(eye, target, up) and perspectiveMatrix(fov, aspect, near, far)
  * as the reference tests build cameras (tests/tests.cpp:29-41). */
+/* MultiRenderer::LoadScene / LoadSceneHydra (Renderer/eye_ray_host.cpp:52-121):
+ * reads a Hydra XML scene -- the first camera of cam_lib (fov, near and far
+ * clip planes, position, look_at, up; aspect from the viewport) and the
+ * geometry instances of the first scene -- and sets the scene and camera.
+ * This path renders one SBS: the file must hold exactly one instance, with
+ * the identity matrix, of an sdf_sbs geometry node, whose loc (relative to
+ * the scene's folder) is an SBS file (load_sdf_SBS). Other node types and
+ * transformed or several instances return LRT_E_UNSUPPORTED. */
+int lrt_load_scene_hydra(lrt_handle h, const char* path);
 int lrt_camera_lookat(const float eye[3], const float target[3], const float up[3], float fov_deg, float aspect, float znear,
                       float zfar, float world_view[16], float proj[16]);
 /* Seeded orbit camera looking at the origin: azimuth/elevation drawn from

@@ -176,6 +176,7 @@ def _load():
         "lrt_sbs_load_file": (i32, [C.c_char_p, C.POINTER(P)]),
         "lrt_sbs_save_file": (i32, [C.c_char_p, C.POINTER(SdfSBSHeader), P, u32, P, u32, P, u32]),
         "lrt_camera_lookat": (i32, [P, P, P, f32, f32, f32, f32, P, P]),
+        "lrt_load_scene_hydra": (i32, [P, C.c_char_p]),
     }
     optional = {}
     sig.update({
@@ -407,6 +408,11 @@ class MultiRenderer:
         nf = sbs.values_f.size
         _check(lib.lrt_set_scene_sbs(self._h, C.byref(sbs.header), _ptr(sbs.nodes), sbs.nodes.shape[0], _ptr(sbs.values),
                                      nv, _ptr(sbs.values_f) if nf else None, nf), "SetScene")
+
+    def LoadScene(self, path):
+        """MultiRenderer::LoadScene / LoadSceneHydra (Renderer/eye_ray_host.cpp:52-121): one identity
+        instance of an sdf_sbs geometry node and the first camera of a Hydra XML scene."""
+        _check(lib.lrt_load_scene_hydra(self._h, os.fsencode(path)), "LoadScene")
 
     def GetBVH(self):
         n, depth = C.c_uint32(), C.c_uint32()

@@ -16,7 +16,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblrt_hip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HOST_SOURCES = ["host/scene_gen.cpp", "host/bvh_build.cpp"]
+HOST_SOURCES = ["host/scene_gen.cpp", "host/bvh_build.cpp", "host/hydra_scene.cpp"]
 HIP_SOURCES = ["lrt_capi.hip", "lrt_dr_capi.hip", "lrt_mlp_capi.hip", "lrt_build_capi.hip", "lrt_noalloc.hip", "kernels/render.hip",
                "kernels/dr.hip", "kernels/mlp.hip", "kernels/build.hip"]
 

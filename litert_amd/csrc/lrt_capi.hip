@@ -9,6 +9,7 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include "host/hydra_scene.hpp"
 
 #include "../../include/lrt.h"
 #include "host/bvh_build.hpp"
@@ -791,6 +792,47 @@ int lrt_sbs_save_file(const char* path, const SdfSBSHeader* header, const SdfSBS
     ok = std::fwrite(&n_values_f, 4, 1, f) == 1 && std::fwrite(values_f, 4, n_values_f, f) == n_values_f;
   ok = (std::fclose(f) == 0) && ok;
   return ok ? LRT_OK : set_err(LRT_E_INVALID, std::string("write failed ") + path);
+}
+
+// MultiRenderer::LoadScene / LoadSceneHydra (Renderer/eye_ray_host.cpp:52-121)
+// for the scenes this path renders: the first camera of the file (aspect from
+// the viewport) and one instance, with the identity matrix, of one sdf_sbs
+// geometry node (AddCustomGeom_FromFile "sdf_sbs", BVH2Common_host.cpp:142-148).
+int lrt_load_scene_hydra(lrt_handle h, const char* path) {
+  if (!h || !path) return set_err(LRT_E_INVALID, "null argument");
+  HydraScene S;
+  try {
+    S = load_hydra_scene(path);
+  } catch (const std::exception& ex) {
+    return set_err(LRT_E_INVALID, ex.what());
+  }
+  if (S.instances.size() != 1)
+    return set_err(LRT_E_UNSUPPORTED, "the scene must hold exactly one geometry instance (found " + std::to_string(S.instances.size()) + ")");
+  const HydraInstance& I = S.instances[0];
+  const HydraGeom* G = nullptr;
+  for (const HydraGeom& g : S.geoms)
+    if (g.id == I.geom_id) G = &g;
+  if (!G && I.geom_id >= 0 && (size_t)I.geom_id < S.geoms.size() && S.geoms[I.geom_id].id < 0) G = &S.geoms[I.geom_id];
+  if (!G) return set_err(LRT_E_INVALID, "the instance names a geometry node the scene does not have");
+  if (G->type != "sdf_sbs") return set_err(LRT_E_UNSUPPORTED, "geometry node type " + G->type + " (only sdf_sbs is on this path)");
+  for (int k = 0; k < 16; k++)
+    if (I.matrix[k] != ((k % 5) == 0 ? 1.0f : 0.0f)) return set_err(LRT_E_UNSUPPORTED, "instance transforms other than the identity");
+  lrt_sbs_handle sb = nullptr;
+  int c = lrt_sbs_load_file(G->path.c_str(), &sb);
+  if (c) return c;
+  std::unique_ptr<lrt_sbs> keep(sb);
+  const SbsScene& sc = sb->s;
+  c = lrt_set_scene_sbs(h, &sc.header, sc.nodes.data(), (uint32_t)sc.nodes.size(), sc.values.data(), sc.values.size(),
+                        sc.values_f.empty() ? nullptr : sc.values_f.data(), sc.values_f.size());
+  if (c) return c;
+  if (S.has_camera) {
+    const HydraCamera& C = S.camera;
+    const float aspect = (h->width && h->height) ? (float)h->width / (float)h->height : 1.0f;
+    m4 v = look_at(f3(C.pos[0], C.pos[1], C.pos[2]), f3(C.look_at[0], C.look_at[1], C.look_at[2]), f3(C.up[0], C.up[1], C.up[2]));
+    m4 p = perspective(C.fov, aspect, C.znear, C.zfar);
+    if ((c = lrt_update_camera(h, v.m, p.m))) return c;
+  }
+  return LRT_OK;
 }
 
 int lrt_camera_lookat(const float eye[3], const float target[3], const float up[3], float fov_deg, float aspect, float znear, float zfar,
