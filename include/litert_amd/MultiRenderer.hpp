@@ -133,6 +133,21 @@ class MultiRenderer {
     return SetViewport(0, 0, (int)w, (int)h) && UpdateCamera(worldView, proj) && SetPreset(preset) &&
            ok(lrt_render_float(h_, reinterpret_cast<float*>(image), w, h, passNum));
   }
+  // CastRaySingleBlock / CastRayFloatSingleBlock (eye_ray.h:185-188), the
+  // seam the reference's GPU subclasses override: every packed pixel of the
+  // viewport (tidX = the packed pixel count, as Render passes it). A block of
+  // fewer pixels is refused; the per-pixel CastRaySingle stays on the
+  // reference's CPU path and has no form here.
+  bool CastRaySingleBlock(uint32_t tidX, uint32_t* out_color, uint32_t a_numPasses = 1) {
+    return tidX >= packed_count() && ok(lrt_render(h_, out_color, width_, height_, (int)a_numPasses));
+  }
+  template <class F4>
+  bool CastRayFloatSingleBlock(uint32_t tidX, F4* out_color, uint32_t a_numPasses = 1) {
+    static_assert(sizeof(F4) == 4 * sizeof(float), "expects float4 pixels");
+    return tidX >= packed_count() && ok(lrt_render_float(h_, reinterpret_cast<float*>(out_color), width_, height_, (int)a_numPasses));
+  }
+  // the packed pixel count of the viewport: both sides rounded up to multiples of 8 (SetViewport, eye_ray_host.cpp:43-50)
+  uint32_t packed_count() const { return ((width_ + 7) / 8 * 8) * ((height_ + 7) / 8 * 8); }
   void GetExecutionTime(const char* name, float out[4]) { lrt_get_execution_time(h_, name, out); }
   std::shared_ptr<AccelStruct> GetAccelStruct() { return std::make_shared<AccelStruct>(h_); }
   lrt_handle handle() const { return h_; }

@@ -4,6 +4,7 @@
 // Exit code 0 = passed.
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <vector>
 
 #include "../../include/litert_amd/MultiRenderer.hpp"
@@ -68,5 +69,15 @@ int main() {
   std::printf("facade RenderFloat %ux%u: %.1f ms, %zu/%u pixels identical, %zu beyond 1e-5\n", W, H, t[0], same, W * H, bad);
   bool passed = bad <= W * H / 10000;
   std::printf(" 1.1. %-64s%s\n", "SBS render through the C++ facade matches the oracle", passed ? "passed" : "FAILED");
-  return passed ? 0 : 1;
+
+  // the GPU subclasses' seam: CastRayFloatSingleBlock over every packed pixel
+  // gives the same image as RenderFloat; a partial block is refused
+  std::vector<float4> block(W * H);
+  const bool blk = pRender->CastRayFloatSingleBlock(pRender->packed_count(), block.data()) &&
+                   !pRender->CastRayFloatSingleBlock(pRender->packed_count() - 1, block.data());
+  size_t diff = 0;
+  for (size_t i = 0; i < W * H; i++) diff += std::memcmp(&block[i], &image[i], sizeof(float4)) != 0;
+  const bool passed2 = blk && diff == 0;
+  std::printf(" 1.2. %-64s%s\n", "CastRayFloatSingleBlock equals RenderFloat", passed2 ? "passed" : "FAILED");
+  return passed && passed2 ? 0 : 1;
 }
