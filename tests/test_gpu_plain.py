@@ -259,6 +259,30 @@ def test_cast_rays_random(r_ico128):
     assert np.mean(g[both, 0].astype(np.float32) == ref[both, 0].astype(np.float32)) > 0.999
 
 
+@pytest.mark.parametrize("tfar", [1e9, 2.0])
+def test_cast_rays_any_hit(r_ico128, tfar):
+    """RayQuery_NearestHit with dirAndFar.w <= 0 (BVH2Common.cpp:3492-3494):
+    any-hit rays up to |w| -- boxes beyond the current hit are culled -- as
+    shadow rays use them; the hit / miss decision and the hit equal the
+    oracle's on the same rays."""
+    sc = oracle_scene("ico128")
+    rng = np.random.default_rng(5)
+    n = 4096
+    o = rng.uniform(-1.5, 1.5, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3], rays[:, 4:7], rays[:, 7] = o, d, -tfar
+    hits = r_ico128.RayQuery_NearestHit(rays)
+    g = np.array([(h.t, h.primId) for h in hits])
+    ref = sc.cast_rays(preset(), rays, precision=32)
+    gh, rh = g[:, 1] != MISS, ref[:, 1] != MISS
+    assert gh.sum() > 500 and (gh != rh).sum() <= 2
+    both = gh & rh
+    assert np.mean(g[both, 0].astype(np.float32) == ref[both, 0].astype(np.float32)) > 0.999
+    assert np.all(g[gh, 0] <= tfar)
+
+
 def test_work_counters_match_oracle(r_ico128):
     """GPU per-ray work counters equal the FP32 oracle's (same traversal order)."""
     sc = oracle_scene("ico128")
