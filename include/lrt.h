@@ -296,9 +296,13 @@ int lrt_sbs_from_mesh_device(lrt_handle h, const float* vertices, uint32_t n_ver
  * frame_octree_to_SBS (:291-444): every leaf whose lattice reaches the
  * surface band becomes a brick at its own LOD (mixed LODs), values quantised
  * over the brick diagonal with the reference's float expression (aux_data
- * LRT_SBS_LAYOUT_DX, no voxel-diagonal flag). Bricks are ordered by LOD, then Morton order (the reference's
- * order depends on its threads). Frames above 2^24 nodes, where the
- * reference's frame_octree_limit_nodes would merge nodes, are refused.
+ * LRT_SBS_LAYOUT_DX, no voxel-diagonal flag). Bricks are ordered by LOD, then
+ * Morton order (the reference's order depends on its threads). nodes_limit is
+ * SparseOctreeSettings::nodes_limit (0: its default 2^24): when the frame
+ * holds more nodes, frame_octree_limit_nodes (:2229-2330) merges the children
+ * of the parents that approximate them best until at most nodes_limit border
+ * leaves remain (equal merge keys keep depth-first order; the reference's
+ * std::sort leaves them unordered). Frames above 2^26 nodes are refused.
  * The mesh form takes the exact distance to the mesh with the sign of the
  * closest feature's pseudo-normal (the reference signs with the closest
  * triangle's face normal that Embree's point query returns). The primitive
@@ -306,10 +310,11 @@ int lrt_sbs_from_mesh_device(lrt_handle h, const float* vertices, uint32_t n_ver
  * LRT_PRIMITIVE_TORUS (around the z axis; params: R, r), evaluated in float
  * as length(p - c) - radius and length((length(p.xy) - R, p.z)) - r. */
 int lrt_sbs_octree_from_mesh_device(lrt_handle h, const float* vertices, uint32_t n_vertices, const uint32_t* indices, uint32_t n_triangles,
-                                    uint32_t depth, uint32_t brick_size, uint32_t brick_pad, uint32_t bytes_per_value, lrt_sbs_handle* out,
-                                    float* build_ms);
+                                    uint32_t depth, uint32_t brick_size, uint32_t brick_pad, uint32_t bytes_per_value, uint32_t nodes_limit,
+                                    lrt_sbs_handle* out, float* build_ms);
 int lrt_sbs_octree_from_primitive_device(lrt_handle h, uint32_t shape, const float* params, uint32_t depth, uint32_t brick_size,
-                                         uint32_t brick_pad, uint32_t bytes_per_value, lrt_sbs_handle* out, float* build_ms);
+                                         uint32_t brick_pad, uint32_t bytes_per_value, uint32_t nodes_limit, lrt_sbs_handle* out,
+                                         float* build_ms);
 /* The displaced icosphere mesh that lrt_sbs_displaced_icosphere builds from
  * (no reference counterpart). NULL vertices / indices: only the counts. */
 int lrt_mesh_displaced_icosphere(uint32_t subdiv, float radius, float amplitude, uint32_t octaves, uint32_t seed, float* vertices,

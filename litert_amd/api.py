@@ -166,8 +166,8 @@ def _load():
         "lrt_set_lights": (i32, [P, P, u32]),
         "lrt_sbs_from_mesh": (i32, [P, u32, P, u32, u32, u32, u32, u32, i32, i32, C.POINTER(P), C.POINTER(f32)]),
         "lrt_sbs_from_mesh_device": (i32, [P, P, u32, P, u32, u32, u32, u32, u32, i32, C.POINTER(P), C.POINTER(f32)]),
-        "lrt_sbs_octree_from_mesh_device": (i32, [P, P, u32, P, u32, u32, u32, u32, u32, C.POINTER(P), C.POINTER(f32)]),
-        "lrt_sbs_octree_from_primitive_device": (i32, [P, u32, P, u32, u32, u32, u32, C.POINTER(P), C.POINTER(f32)]),
+        "lrt_sbs_octree_from_mesh_device": (i32, [P, P, u32, P, u32, u32, u32, u32, u32, u32, C.POINTER(P), C.POINTER(f32)]),
+        "lrt_sbs_octree_from_primitive_device": (i32, [P, u32, P, u32, u32, u32, u32, u32, C.POINTER(P), C.POINTER(f32)]),
         "lrt_mesh_displaced_icosphere": (i32, [u32, f32, f32, u32, u32, P, C.POINTER(u32), P, C.POINTER(u32)]),
         "lrt_sbs_info": (i32, [P, C.POINTER(SdfSBSHeader), C.POINTER(u32), C.POINTER(u64), C.POINTER(u64)]),
         "lrt_sbs_copy": (i32, [P, P, P, P]),
@@ -313,29 +313,30 @@ def sbs_from_mesh(vertices, triangles, lod=32, brick_size=4, brick_pad=0, bytes_
 PRIMITIVE_SPHERE, PRIMITIVE_TORUS = 0, 1
 
 
-def sbs_octree_from_mesh(renderer, vertices, triangles, depth, brick_size=4, brick_pad=0, bytes_per_value=1):
+def sbs_octree_from_mesh(renderer, vertices, triangles, depth, brick_size=4, brick_pad=0, bytes_per_value=1, nodes_limit=0):
     """create_sdf_SBS(settings, header, mesh) with the reference's adaptive frame octree
     (utils/sparse_octree_builder.cpp:125-444) on the renderer's GPU: mixed-LOD bricks down to
-    octree depth `depth`. Returns (SdfSBS, build_ms)."""
+    octree depth `depth`, frame_octree_limit_nodes at nodes_limit (0: 2^24). Returns (SdfSBS, build_ms)."""
     v = _f32(vertices).reshape(-1, 3)
     t = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
     h = C.c_void_p()
     ms = C.c_float(0.0)
     _check(lib.lrt_sbs_octree_from_mesh_device(renderer._h, _ptr(v), v.shape[0], _ptr(t), t.shape[0], depth, brick_size, brick_pad,
-                                               bytes_per_value, C.byref(h), C.byref(ms)), "lrt_sbs_octree_from_mesh_device")
+                                               bytes_per_value, nodes_limit, C.byref(h), C.byref(ms)), "lrt_sbs_octree_from_mesh_device")
     return SdfSBS._from_handle(h), ms.value
 
 
-def sbs_octree_from_primitive(renderer, shape, params, depth, brick_size=4, brick_pad=0, bytes_per_value=1):
+def sbs_octree_from_primitive(renderer, shape, params, depth, brick_size=4, brick_pad=0, bytes_per_value=1, nodes_limit=0):
     """create_sdf_SBS(settings, header, sdf) with the reference's adaptive frame octree for a sphere
     (PRIMITIVE_SPHERE, params centre xyz + radius) or a torus around the z axis (PRIMITIVE_TORUS,
-    params R, r) on the renderer's GPU. Returns (SdfSBS, build_ms)."""
+    params R, r) on the renderer's GPU, frame_octree_limit_nodes at nodes_limit (0: 2^24).
+    Returns (SdfSBS, build_ms)."""
     prm = np.zeros(4, np.float32)
     prm[:len(params)] = params
     h = C.c_void_p()
     ms = C.c_float(0.0)
     _check(lib.lrt_sbs_octree_from_primitive_device(renderer._h, shape, _ptr(prm), depth, brick_size, brick_pad, bytes_per_value,
-                                                    C.byref(h), C.byref(ms)), "lrt_sbs_octree_from_primitive_device")
+                                                    nodes_limit, C.byref(h), C.byref(ms)), "lrt_sbs_octree_from_primitive_device")
     return SdfSBS._from_handle(h), ms.value
 
 

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <unordered_map>
 #include <vector>
 #include "../../include/lrt.h"
 #include "kernels/build.hpp"
@@ -106,6 +107,121 @@ int upload_mesh(Renderer* r, const float* vertices, uint32_t n_vertices, const u
   return LRT_OK;
 }
 
+// The frame octree on the host, for frame_octree_limit_nodes: node k has its
+// cell (x, y, z, level), its 8 corner values and the index of its first child
+// (0: leaf; the 8 children are consecutive).
+struct FrameNodeH {
+  uint4 cell;
+  float v[8];
+  uint32_t first;
+};
+constexpr uint32_t kMerged = 1u << 31;  // a child of a merged node (INVALID_IDX, sdf_scene.h:42)
+
+bool frame_is_leaf(uint32_t first) { return first == 0 || (first & kMerged); }
+
+// is_border_node (sparse_octree_builder.cpp:49-53) of a frame node at `level`
+bool frame_border(const float* v, uint32_t level) {
+  float mn = v[0], mx = v[0];
+  for (int i = 0; i < 8; i++) {
+    mn = v[i] < mn ? v[i] : mn;
+    mx = mx < v[i] ? v[i] : mx;
+  }
+  const float d_max = (float)(2 * std::sqrt(3.0) / (double)(1u << level));
+  return mn <= 0 && mx > -d_max;
+}
+
+struct LayerNode {
+  uint32_t idx;
+  bool leaf, border;
+  uint32_t border_children;
+};
+
+// fill_layers_rec (:1270-1299) without recursion: per level, the reachable
+// nodes in depth-first order (children 0..7), each with its border flag and
+// its count of border children.
+void frame_layers(const std::vector<FrameNodeH>& f, std::vector<std::vector<LayerNode>>& layers) {
+  layers.clear();
+  struct Item {
+    uint32_t idx, level, next_child, bc;
+  };
+  std::vector<Item> stack{{0, 0, 0, 0}};
+  while (!stack.empty()) {
+    Item& it = stack.back();
+    const FrameNodeH& n = f[it.idx];
+    const bool leaf = frame_is_leaf(n.first);
+    if (!leaf && it.next_child < 8) {
+      const uint32_t c = n.first + it.next_child++;
+      stack.push_back({c, it.level + 1, 0, 0});
+      continue;
+    }
+    const bool border = frame_border(n.v, it.level);
+    if (layers.size() <= it.level) layers.resize(it.level + 1);
+    layers[it.level].push_back({it.idx, leaf, border, it.bc});
+    stack.pop_back();
+    if (!stack.empty()) stack.back().bc += border ? 1u : 0u;
+  }
+}
+
+// frame_octree_limit_nodes (:2229-2330), count_only_border_nodes = true:
+// merges the children of the nodes one level above the layer being reduced,
+// in order of their mean |child corner - parent trilinear| over (border
+// children - 1), until at most nodes_limit border leaves remain. Equal keys
+// keep their depth-first order (the reference's std::sort leaves them in no
+// specified order). Returns the leaves.
+std::vector<uint32_t> frame_limit(std::vector<FrameNodeH>& f, uint32_t nodes_limit) {
+  std::vector<std::vector<LayerNode>> layers;
+  frame_layers(f, layers);
+  auto count = [&]() {
+    uint32_t c = 0;
+    for (const auto& layer : layers)
+      for (const LayerNode& n : layer) c += n.leaf && n.border;
+    return c;
+  };
+  auto tri = [](const float* v, float x, float y, float z) {  // trilinear_interpolation, :1301-1311
+    return (1 - x) * (1 - y) * (1 - z) * v[0] + (1 - x) * (1 - y) * (z) * v[1] + (1 - x) * (y) * (1 - z) * v[2] + (1 - x) * (y) * (z) * v[3] +
+           (x) * (1 - y) * (1 - z) * v[4] + (x) * (1 - y) * (z) * v[5] + (x) * (y) * (1 - z) * v[6] + (x) * (y) * (z) * v[7];
+  };
+  uint32_t cnt = count();
+  uint32_t active = (uint32_t)layers.size() - 1;
+  struct Cand {
+    uint32_t idx, active_children;
+    float key;
+  };
+  while (cnt > nodes_limit && active > 0) {
+    std::vector<Cand> mc;
+    for (const LayerNode& p : layers[active - 1]) {
+      const uint32_t first = f[p.idx].first;
+      if (frame_is_leaf(first)) continue;
+      float diff = 0;
+      for (uint32_t i = 0; i < 8; i++)
+        for (uint32_t j = 0; j < 8; j++) {
+          const float qx = 0.5f * ((float)((i & 4) >> 2) + (float)((j & 4) >> 2));
+          const float qy = 0.5f * ((float)((i & 2) >> 1) + (float)((j & 2) >> 1));
+          const float qz = 0.5f * ((float)(i & 1) + (float)(j & 1));
+          diff += std::abs(f[first + i].v[j] - tri(f[p.idx].v, qx, qy, qz));
+        }
+      mc.push_back({p.idx, p.border_children, p.border_children > 1 ? diff / (float)(p.border_children - 1) : 1000.0f});
+    }
+    uint32_t potential = 0;
+    for (const Cand& c : mc) potential += c.active_children - 1;  // unsigned, as the reference sums them
+    if (cnt - potential < nodes_limit)
+      std::stable_sort(mc.begin(), mc.end(), [](const Cand& a, const Cand& b) { return a.key < b.key; });
+    for (size_t c = 0; cnt > nodes_limit && c < mc.size(); c++) {
+      for (uint32_t i = 0; i < 8; i++) f[f[mc[c].idx].first + i].first = kMerged;
+      f[mc[c].idx].first = 0;
+      cnt -= mc[c].active_children - 1;
+    }
+    frame_layers(f, layers);
+    cnt = count();
+    active--;
+  }
+  std::vector<uint32_t> leaves;
+  for (const auto& layer : layers)
+    for (const LayerNode& n : layer)
+      if (n.leaf) leaves.push_back(n.idx);
+  return leaves;
+}
+
 // create_sdf_SBS(settings, header, sdf) (utils/sdf_converter.cpp:169-181) on
 // the device: the frame octree of construct_sdf_frame_octree
 // (sparse_octree_builder.cpp:125-258) level by level -- every cell of the last
@@ -113,7 +229,8 @@ int upload_mesh(Renderer* r, const float* vertices, uint32_t n_vertices, const u
 // frame_octree_to_SBS (:291-444) over its leaves. Bricks are ordered by LOD,
 // then Morton order (the reference appends them from parallel threads).
 int build_octree(Renderer* r, const lrtd::DevField& F, uint32_t depth, uint32_t brick_size, uint32_t brick_pad, uint32_t bytes_per_value,
-                 lrt_sbs_handle* out) {
+                 uint32_t nodes_limit, lrt_sbs_handle* out) {
+  if (nodes_limit == 0) nodes_limit = 1u << 24;  // SparseOctreeSettings::nodes_limit default, sdf_converter.h:32
   if (depth < 2 || depth > 12) return set_err(LRT_E_INVALID, "octree depth must be 2..12");
   if (brick_size < 1 || brick_size > 16 || brick_pad > 1) return set_err(LRT_E_INVALID, "bad brick size/pad");
   if (bytes_per_value != 1 && bytes_per_value != 2 && bytes_per_value != 4) return set_err(LRT_E_INVALID, "bad bytes_per_value");
@@ -148,16 +265,35 @@ int build_octree(Renderer* r, const lrtd::DevField& F, uint32_t depth, uint32_t 
   if ((e = b_count.ensure(8)) != hipSuccess) return hip_err(e, "alloc");
   std::vector<uint4> leaves;
   std::vector<float> lvals;
+  // Every level's cells and field values stay on the device until the frame's
+  // size is known: frame_octree_limit_nodes needs the whole frame.
+  std::vector<DevBuf> lvl_cells, lvl_vals;
+  std::vector<uint32_t> lvl_n;
+  struct ReleaseLevels {
+    std::vector<DevBuf>* a;
+    std::vector<DevBuf>* b;
+    ~ReleaseLevels() {
+      for (auto* v : {a, b})
+        for (DevBuf& d : *v) d.release();
+    }
+  } rel_levels{&lvl_cells, &lvl_vals};
   uint32_t n = (uint32_t)cells.size();
+  DevBuf cur = b_cells;  // the level's cells; ownership moves into lvl_cells
+  b_cells = DevBuf{};
   for (uint32_t level = top; n > 0; level++) {
-    if ((e = b_vals.ensure(9ull * n * 4)) != hipSuccess || (e = b_kids.ensure(8ull * n * 16)) != hipSuccess ||
+    lvl_cells.push_back(cur);
+    cur = DevBuf{};
+    lvl_vals.push_back(DevBuf{});
+    lvl_n.push_back(n);
+    DevBuf& vals = lvl_vals.back();
+    const DevBuf& cl = lvl_cells.back();
+    if ((e = vals.ensure(9ull * n * 4)) != hipSuccess || (e = b_kids.ensure(8ull * n * 16)) != hipSuccess ||
         (e = b_leaves.ensure((size_t)n * 16)) != hipSuccess || (e = b_lvals.ensure(8ull * n * 4)) != hipSuccess)
       return hip_err(e, "alloc octree level");
     if ((e = zero_u32((uint32_t*)b_count.p, 2, s)) != hipSuccess) return hip_err(e, "clear");
-    if ((e = launch_octree_eval(F, (const uint4*)b_cells.p, n, (float*)b_vals.p, s)) != hipSuccess) return hip_err(e, "octree eval");
-    if ((e = launch_octree_classify((const uint4*)b_cells.p, (const float*)b_vals.p, n, level, level == top ? 1 : 0, depth, T,
-                                    (uint4*)b_kids.p, (uint32_t*)b_count.p, (uint4*)b_leaves.p, (float*)b_lvals.p,
-                                    (uint32_t*)b_count.p + 1, s)) != hipSuccess)
+    if ((e = launch_octree_eval(F, (const uint4*)cl.p, n, (float*)vals.p, s)) != hipSuccess) return hip_err(e, "octree eval");
+    if ((e = launch_octree_classify((const uint4*)cl.p, (const float*)vals.p, n, level, level == top ? 1 : 0, depth, T, (uint4*)b_kids.p,
+                                    (uint32_t*)b_count.p, (uint4*)b_leaves.p, (float*)b_lvals.p, (uint32_t*)b_count.p + 1, s)) != hipSuccess)
       return hip_err(e, "octree classify");
     uint32_t cnt[2];
     if ((e = hipMemcpyAsync(cnt, b_count.p, 8, hipMemcpyDeviceToHost, s)) != hipSuccess || (e = hipStreamSynchronize(s)) != hipSuccess)
@@ -169,12 +305,86 @@ int build_octree(Renderer* r, const lrtd::DevField& F, uint32_t depth, uint32_t 
                    (e = hipMemcpyAsync(lvals.data() + 8 * l0, b_lvals.p, (size_t)cnt[1] * 32, hipMemcpyDeviceToHost, s)) != hipSuccess))
       return hip_err(e, "download leaves");
     frame_nodes += cnt[0];
-    // frame_octree_limit_nodes (:2229-2330) leaves frames of up to 2^24 nodes alone
-    if (frame_nodes > (1ull << 24)) return set_err(LRT_E_UNSUPPORTED, "frame octree above 2^24 nodes (the reference would merge nodes)");
-    std::swap(b_cells, b_kids);
+    if (frame_nodes > (1ull << 26)) return set_err(LRT_E_UNSUPPORTED, "frame octree above 2^26 nodes");
+    cur = b_kids;  // the children are the next level's cells
+    b_kids = DevBuf{};
     n = cnt[0];
   }
+  cur.release();
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_err(e, "download leaves");
+  if (frame_nodes > nodes_limit) {
+    // frame_octree_limit_nodes (:2229-2330) over the whole frame, on the host:
+    // the uniform top levels are evaluated now, every other level comes from
+    // the device, and the tree is rebuilt from cell positions
+    std::vector<std::vector<uint4>> hc(depth + 1);
+    std::vector<std::vector<float>> hv(depth + 1);
+    std::vector<uint4> upper;
+    for (uint32_t l = 0; l < top; l++)
+      for (uint32_t x = 0; x < (1u << l); x++)
+        for (uint32_t y = 0; y < (1u << l); y++)
+          for (uint32_t z = 0; z < (1u << l); z++) upper.push_back(make_uint4(x, y, z, l));
+    std::vector<float> upper_v(9 * upper.size());
+    DevBuf b_uc, b_uv;
+    struct ReleaseUpper {
+      DevBuf* a;
+      DevBuf* b;
+      ~ReleaseUpper() {
+        a->release();
+        b->release();
+      }
+    } rel_upper{&b_uc, &b_uv};
+    if ((e = b_uc.upload(upper.data(), upper.size() * 16, s)) != hipSuccess || (e = b_uv.ensure(upper_v.size() * 4)) != hipSuccess ||
+        (e = launch_octree_eval(F, (const uint4*)b_uc.p, (uint32_t)upper.size(), (float*)b_uv.p, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(upper_v.data(), b_uv.p, upper_v.size() * 4, hipMemcpyDeviceToHost, s)) != hipSuccess)
+      return hip_err(e, "octree top levels");
+    for (size_t i = 0; i < upper.size(); i++) {
+      hc[upper[i].w].push_back(upper[i]);
+      hv[upper[i].w].insert(hv[upper[i].w].end(), upper_v.begin() + 9 * i, upper_v.begin() + 9 * i + 9);
+    }
+    for (size_t k = 0; k < lvl_cells.size(); k++) {
+      const uint32_t l = top + (uint32_t)k;
+      hc[l].resize(lvl_n[k]);
+      hv[l].resize(9ull * lvl_n[k]);
+      if ((e = hipMemcpyAsync(hc[l].data(), lvl_cells[k].p, (size_t)lvl_n[k] * 16, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+          (e = hipMemcpyAsync(hv[l].data(), lvl_vals[k].p, 9ull * lvl_n[k] * 4, hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return hip_err(e, "download frame");
+    }
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_err(e, "download frame");
+    auto key = [](uint32_t x, uint32_t y, uint32_t z) { return (uint64_t)x | ((uint64_t)y << 21) | ((uint64_t)z << 42); };
+    std::vector<std::unordered_map<uint64_t, uint32_t>> at(depth + 1);
+    for (uint32_t l = 0; l <= depth; l++) {
+      at[l].reserve(hc[l].size());
+      for (uint32_t i = 0; i < hc[l].size(); i++) at[l][key(hc[l][i].x, hc[l][i].y, hc[l][i].z)] = i;
+    }
+    std::vector<FrameNodeH> fr;
+    fr.reserve(frame_nodes);
+    auto node = [&](uint32_t l, uint32_t i) {
+      FrameNodeH nd;
+      nd.cell = hc[l][i];
+      std::copy(hv[l].begin() + 9ull * i, hv[l].begin() + 9ull * i + 8, nd.v);
+      nd.first = 0;
+      return nd;
+    };
+    fr.push_back(node(0, 0));
+    for (size_t k = 0; k < fr.size(); k++) {  // children appended as a block of 8, in creation order
+      const uint4 c = fr[k].cell;
+      if (c.w >= depth) continue;
+      auto it = at[c.w + 1].find(key(2 * c.x, 2 * c.y, 2 * c.z));
+      if (it == at[c.w + 1].end()) continue;  // not split
+      fr[k].first = (uint32_t)fr.size();
+      for (uint32_t q = 0; q < 8; q++) {
+        auto jt = at[c.w + 1].find(key(2 * c.x + ((q >> 2) & 1u), 2 * c.y + ((q >> 1) & 1u), 2 * c.z + (q & 1u)));
+        if (jt == at[c.w + 1].end()) return set_err(LRT_E_HIP, "octree frame: a split cell lacks a child");
+        fr.push_back(node(c.w + 1, jt->second));
+      }
+    }
+    leaves.clear();
+    lvals.clear();
+    for (uint32_t i : frame_limit(fr, nodes_limit)) {
+      leaves.push_back(fr[i].cell);
+      lvals.insert(lvals.end(), fr[i].v, fr[i].v + 8);
+    }
+  }
   const uint32_t nl = (uint32_t)leaves.size();
   std::vector<uint32_t> order(nl);
   for (uint32_t i = 0; i < nl; i++) order[i] = i;
@@ -358,8 +568,8 @@ int lrt_sbs_from_mesh_device(lrt_handle h, const float* vertices, uint32_t n_ver
 
 
 int lrt_sbs_octree_from_mesh_device(lrt_handle h, const float* vertices, uint32_t n_vertices, const uint32_t* indices, uint32_t n_triangles,
-                                    uint32_t depth, uint32_t brick_size, uint32_t brick_pad, uint32_t bytes_per_value, lrt_sbs_handle* out,
-                                    float* build_ms) {
+                                    uint32_t depth, uint32_t brick_size, uint32_t brick_pad, uint32_t bytes_per_value, uint32_t nodes_limit,
+                                    lrt_sbs_handle* out, float* build_ms) {
   if (!h) return set_err(LRT_E_INVALID, "null handle");
   int c = check_mesh_args(vertices, n_vertices, indices, n_triangles, 1, brick_size, brick_pad, bytes_per_value, out);
   if (c) return c;
@@ -372,13 +582,14 @@ int lrt_sbs_octree_from_mesh_device(lrt_handle h, const float* vertices, uint32_
   lrtd::DevField F{};
   F.kind = 0;
   F.mesh = U.M;
-  if ((c = build_octree(r, F, depth, brick_size, brick_pad, bytes_per_value, out)) != LRT_OK) return c;
+  if ((c = build_octree(r, F, depth, brick_size, brick_pad, bytes_per_value, nodes_limit, out)) != LRT_OK) return c;
   if (build_ms) *build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return LRT_OK;
 }
 
 int lrt_sbs_octree_from_primitive_device(lrt_handle h, uint32_t shape, const float* params, uint32_t depth, uint32_t brick_size,
-                                         uint32_t brick_pad, uint32_t bytes_per_value, lrt_sbs_handle* out, float* build_ms) {
+                                         uint32_t brick_pad, uint32_t bytes_per_value, uint32_t nodes_limit, lrt_sbs_handle* out,
+                                         float* build_ms) {
   if (!h || !params || !out) return set_err(LRT_E_INVALID, "null argument");
   if (shape != LRT_PRIMITIVE_SPHERE && shape != LRT_PRIMITIVE_TORUS) return set_err(LRT_E_INVALID, "unknown primitive");
   Renderer* r = h;
@@ -388,7 +599,7 @@ int lrt_sbs_octree_from_primitive_device(lrt_handle h, uint32_t shape, const flo
   lrtd::DevField F{};
   F.kind = shape == LRT_PRIMITIVE_SPHERE ? 1u : 2u;
   for (int k = 0; k < (shape == LRT_PRIMITIVE_SPHERE ? 4 : 2); k++) F.a[k] = params[k];
-  int c = build_octree(r, F, depth, brick_size, brick_pad, bytes_per_value, out);
+  int c = build_octree(r, F, depth, brick_size, brick_pad, bytes_per_value, nodes_limit, out);
   if (c) return c;
   if (build_ms) *build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return LRT_OK;

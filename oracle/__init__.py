@@ -90,7 +90,7 @@ def _load(path):
         "orc_mlp_loss_grad": (f64, [P, P, P, u32, i32, P]),
         "orc_mlp_brick_features": (i32, [P, u32, P]),
         "orc_mlp_features": (i32, [P, P, u32, P, P]),
-        "orc_sbs_octree": (i32, [i32, P, u32, u32, u32, u32, P, P, P, P]),
+        "orc_sbs_octree": (i32, [i32, P, u32, u32, u32, u32, u32, P, P, P, P]),
     }
     for name, (res, args) in list(sig.items()) + list(optional.items()):
         if name in optional and not hasattr(L, name):
@@ -243,21 +243,24 @@ def dr_redistance(lattice, h, iters):
     return d
 
 
-def sbs_octree(shape, params, depth, brick_size, brick_pad, bytes_per_value):
+def sbs_octree(shape, params, depth, brick_size, brick_pad, bytes_per_value, nodes_limit=0):
     """create_sdf_SBS(settings, header, sdf) (utils/sdf_converter.cpp:169-181)
     of a sphere (shape 0, params centre xyz + radius) or a torus around the z
-    axis (shape 1, params R, r): (nodes (n, 4) uint32, values uint32) in the
-    frame octree's order."""
+    axis (shape 1, params R, r), with SparseOctreeSettings::nodes_limit (0: its
+    default 2^24): (nodes (n, 4) uint32, values uint32) in the frame octree's
+    order."""
     prm = np.zeros(4, np.float32)
     prm[:len(params)] = params
     nn, nv = C.c_uint32(), C.c_uint64()
     L = lib()
-    rc = L.orc_sbs_octree(shape, _p(prm), depth, brick_size, brick_pad, bytes_per_value, C.byref(nn), None, C.byref(nv), None)
+    rc = L.orc_sbs_octree(shape, _p(prm), depth, brick_size, brick_pad, bytes_per_value, nodes_limit, C.byref(nn), None, C.byref(nv),
+                          None)
     if rc != 0:
         raise ValueError(f"orc_sbs_octree failed {rc}")
     nodes = np.zeros((nn.value, 4), np.uint32)
     values = np.zeros(nv.value, np.uint32)
-    rc = L.orc_sbs_octree(shape, _p(prm), depth, brick_size, brick_pad, bytes_per_value, C.byref(nn), _p(nodes), C.byref(nv), _p(values))
+    rc = L.orc_sbs_octree(shape, _p(prm), depth, brick_size, brick_pad, bytes_per_value, nodes_limit, C.byref(nn), _p(nodes),
+                          C.byref(nv), _p(values))
     if rc != 0:
         raise ValueError(f"orc_sbs_octree failed {rc}")
     return nodes, values

@@ -135,11 +135,12 @@ int orc_mlp_features(const orc_scene* s, const float ray[6], uint32_t brick, flo
 /* create_sdf_SBS(settings, header, sdf) (utils/sdf_converter.cpp:169-181) for
  * a distance field -- shape 0: sphere, params = centre xyz, radius; shape 1:
  * torus around the z axis, params = R, r -- at octree depth `depth`: the
- * frame octree of construct_sdf_frame_octree and the bricks of
- * frame_octree_to_SBS, in the frame octree's order. NULL outputs: counts only.
- * Returns 0, -1 (bad arguments) or -2 (more than 2^24 frame nodes). */
+ * frame octree of construct_sdf_frame_octree, frame_octree_limit_nodes with
+ * nodes_limit (0: the default 2^24) and the bricks of frame_octree_to_SBS, in
+ * the frame octree's order. NULL outputs: counts only. Returns 0 or -1 (bad
+ * arguments). */
 int orc_sbs_octree(int shape, const float* params, uint32_t depth, uint32_t brick_size, uint32_t brick_pad, uint32_t bytes_per_value,
-                   uint32_t* n_nodes, uint32_t* nodes_out, uint64_t* n_values, uint32_t* values_out);
+                   uint32_t nodes_limit, uint32_t* n_nodes, uint32_t* nodes_out, uint64_t* n_values, uint32_t* values_out);
 
 #ifdef __cplusplus
 }

@@ -7,8 +7,9 @@
 // The reference builds in parallel (OpenMP groups of subtrees, bricks appended
 // under a critical section, so its brick order varies from run to run); this
 // restatement runs the same steps on one thread, so its order is the frame
-// octree's. frame_octree_limit_nodes (:2229-2330) is a no-op below its
-// default limit of 2^24 leaves and is refused above it.
+// octree's. frame_octree_limit_nodes (:2229-2330) runs on the frame before
+// the bricks are made, as in create_sdf_SBS.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -153,19 +154,127 @@ std::vector<FrameNode> frame_octree(const Field& sdf, unsigned depth, std::vecto
   return res;
 }
 
+constexpr uint32_t kInvalidIdx = 1u << 31;  // INVALID_IDX, sdf_scene.h:42
+
+bool frame_leaf(uint32_t offset) { return offset == 0 || (offset & kInvalidIdx); }  // is_leaf, :43-46
+
+struct LayerInfo {  // LayerFrameNodeInfo, :1255-1261
+  uint32_t idx;
+  bool leaf, border;
+  uint32_t border_children;
+};
+
+// fill_layers_rec (:1270-1299): every reachable node, per level in depth-first
+// order, with its border flag (is_border_node over its 8 values) and the
+// number of border children.
+bool fill_layers(const std::vector<FrameNode>& f, std::vector<std::vector<LayerInfo>>& layers, unsigned level, uint32_t idx) {
+  if (level >= layers.size()) layers.resize(level + 1);
+  const uint32_t ofs = f[idx].offset;
+  const bool leaf = frame_leaf(ofs);
+  float mn = f[idx].values[0], mx = f[idx].values[0];
+  for (int i = 0; i < 8; i++) {
+    mn = f[idx].values[i] < mn ? f[idx].values[i] : mn;
+    mx = mx < f[idx].values[i] ? f[idx].values[i] : mx;
+  }
+  const bool border = is_border_node(mn, mx, 1u << level);
+  uint32_t bc = 0;
+  if (!leaf)
+    for (uint32_t i = 0; i < 8; i++) bc += fill_layers(f, layers, level + 1, ofs + i);
+  layers[level].push_back({idx, leaf, border, bc});
+  return border;
+}
+
+float trilinear8(const float v[8], F3 dp) {  // trilinear_interpolation, :1301-1311
+  return (1 - dp.x) * (1 - dp.y) * (1 - dp.z) * v[0] + (1 - dp.x) * (1 - dp.y) * (dp.z) * v[1] + (1 - dp.x) * (dp.y) * (1 - dp.z) * v[2] +
+         (1 - dp.x) * (dp.y) * (dp.z) * v[3] + (dp.x) * (1 - dp.y) * (1 - dp.z) * v[4] + (dp.x) * (1 - dp.y) * (dp.z) * v[5] +
+         (dp.x) * (dp.y) * (1 - dp.z) * v[6] + (dp.x) * (dp.y) * (dp.z) * v[7];
+}
+
+// frame_octree_limit_nodes (:2229-2330) with count_only_border_nodes: while
+// more border leaves than nodes_limit remain, merge the children of the
+// parents one level above the deepest layer still being merged, in order of
+// their mean difference from the parent's trilinear interpolation (weights
+// and unsigned counts as the reference computes them). The reference sorts
+// with std::sort, which leaves equal differences in no specified order; here
+// they keep their depth-first order (stable sort). pos / size follow the
+// nodes through the final compaction.
+void limit_nodes(std::vector<FrameNode>& f, std::vector<F3>& pos, std::vector<float>& size, uint32_t nodes_limit) {
+  if (nodes_limit >= f.size()) return;
+  struct Cand {
+    uint32_t idx, active_children;
+    float weighted_diff;
+  };
+  auto count = [&](const std::vector<std::vector<LayerInfo>>& layers) {
+    uint32_t c = 0;
+    for (const auto& layer : layers)
+      for (const LayerInfo& n : layer) c += n.leaf && n.border;
+    return c;
+  };
+  std::vector<std::vector<LayerInfo>> layers;
+  fill_layers(f, layers, 0, 0);
+  uint32_t cnt = count(layers);
+  uint32_t active = (uint32_t)layers.size() - 1;
+  while (cnt > nodes_limit && active > 0) {
+    std::vector<Cand> mc;
+    for (const LayerInfo& p : layers[active - 1]) {
+      const uint32_t ofs = f[p.idx].offset;
+      if (frame_leaf(ofs)) continue;
+      float diff = 0;
+      for (unsigned i = 0; i < 8; i++)
+        for (unsigned j = 0; j < 8; j++) {
+          const float child_val = f[ofs + i].values[j];
+          const F3 q = 0.5f * (f3((float)((i & 4) >> 2), (float)((i & 2) >> 1), (float)(i & 1)) +
+                               f3((float)((j & 4) >> 2), (float)((j & 2) >> 1), (float)(j & 1)));
+          diff += std::abs(child_val - trilinear8(f[p.idx].values, q));
+        }
+      mc.push_back({p.idx, p.border_children, p.border_children > 1 ? diff / (float)(p.border_children - 1) : 1000.0f});
+    }
+    uint32_t potential = 0;
+    for (const Cand& c : mc) potential += c.active_children - 1;  // unsigned, as the reference adds them
+    if (cnt - potential < nodes_limit)
+      std::stable_sort(mc.begin(), mc.end(), [](const Cand& a, const Cand& b) { return a.weighted_diff < b.weighted_diff; });
+    for (size_t c = 0; cnt > nodes_limit && c < mc.size(); c++) {
+      const uint32_t idx = mc[c].idx;
+      for (uint32_t i = 0; i < 8; i++) f[f[idx].offset + i].offset = kInvalidIdx;
+      f[idx].offset = 0;
+      cnt -= mc[c].active_children - 1;
+    }
+    layers.clear();
+    fill_layers(f, layers, 0, 0);
+    cnt = count(layers);
+    active--;
+  }
+  std::vector<uint32_t> remap(f.size(), 0);
+  std::vector<FrameNode> nf;
+  std::vector<F3> npos;
+  std::vector<float> nsize;
+  for (uint32_t i = 0; i < f.size(); i++)
+    if (f[i].offset != kInvalidIdx) {
+      remap[i] = (uint32_t)nf.size();
+      nf.push_back(f[i]);
+      npos.push_back(pos[i]);
+      nsize.push_back(size[i]);
+    }
+  for (FrameNode& n : nf)
+    if (n.offset > 0) n.offset = remap[n.offset];
+  f.swap(nf);
+  pos.swap(npos);
+  size.swap(nsize);
+}
+
 }  // namespace
 
 // frame_octree_to_SBS (:291-444) of the field's frame octree; see lrt_oracle.h.
 extern "C" int orc_sbs_octree(int shape, const float* params, uint32_t depth, uint32_t brick_size, uint32_t brick_pad, uint32_t bytes_per_value,
-                              uint32_t* n_nodes, uint32_t* nodes_out, uint64_t* n_values, uint32_t* values_out) {
+                              uint32_t nodes_limit, uint32_t* n_nodes, uint32_t* nodes_out, uint64_t* n_values, uint32_t* values_out) {
   if ((shape != 0 && shape != 1) || !params || depth < 2 || depth > 12 || brick_size < 1 || brick_size > 16 || brick_pad > 1 ||
       (bytes_per_value != 1 && bytes_per_value != 2 && bytes_per_value != 4) || !n_nodes || !n_values)
     return -1;
   Field sdf{shape, {params[0], params[1], params[2], params[3]}};
   std::vector<F3> pos;
   std::vector<float> size;
-  const std::vector<FrameNode> nodes = frame_octree(sdf, depth, pos, size);
-  if (nodes.size() > (1u << 24)) return -2;  // frame_octree_limit_nodes would merge nodes
+  std::vector<FrameNode> nodes = frame_octree(sdf, depth, pos, size);
+  limit_nodes(nodes, pos, size, nodes_limit ? nodes_limit : (1u << 24));  // SparseOctreeSettings::nodes_limit, default 2^24
   const uint32_t v_size = brick_size + 2 * brick_pad + 1;
   const unsigned bits = 8 * bytes_per_value;
   const unsigned q_max = bytes_per_value == 4 ? 0xFFFFFFFFu : ((1u << bits) - 1);  // the reference's inner max_val

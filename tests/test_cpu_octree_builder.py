@@ -150,3 +150,67 @@ def test_octree_builder_torus_has_mixed_lods_and_covers_it():
     pts = np.stack([(0.6 + 0.2 * np.cos(b)) * np.cos(a), (0.6 + 0.2 * np.cos(b)) * np.sin(a), 0.2 * np.sin(b)], -1)
     idx = np.floor((pts + 1.0) * 0.5 * (1 << depth)).astype(int)
     assert all((int(x), int(y), int(z)) in cells for x, y, z in idx)
+
+
+# ---- frame_octree_limit_nodes (:2229-2330) ----------------------------------
+def _key_set(nodes):
+    return {(int(n[0] >> 16), int(n[0] & 0xFFFF), int(n[1] >> 16), int(n[1] & 0xFFFF)) for n in nodes}
+
+
+def test_node_limit_above_the_frame_size_changes_nothing():
+    a, va = oracle.sbs_octree(0, [0, 0, 0, R], 6, 4, 0, 1)
+    b, vb = oracle.sbs_octree(0, [0, 0, 0, R], 6, 4, 0, 1, 1 << 22)
+    assert np.array_equal(a, b) and np.array_equal(va, vb)
+
+
+def test_node_limit_is_monotone_and_bounds_the_bricks():
+    counts = []
+    for lim in (14000, 8000, 3000, 1000):
+        nodes, _ = oracle.sbs_octree(0, [0, 0, 0, R], 6, 4, 0, 1, lim)
+        counts.append(nodes.shape[0])
+        # the limit counts leaves whose 8 corners pass the border test; the
+        # kept bricks test their whole lattice, which can only keep fewer
+        assert nodes.shape[0] <= lim
+    assert counts == sorted(counts, reverse=True) and counts[-1] < counts[0]
+
+
+def _weighted_diff(parent):
+    """Mean |child corner - parent trilinear| over the 8 children x 8 corners,
+    divided by (border children - 1), in FP64 (the merge order's key)."""
+    p = np.asarray(parent, np.float64)
+    pv = _sphere(_corners(p[None, :3], 2.0 ** -p[3]))[0]
+    bits = np.array([[(c >> 2) & 1, (c >> 1) & 1, c & 1] for c in range(8)], np.float64)
+    kids = 2 * p[None, :3] + bits
+    kv = _sphere(_corners(kids, 2.0 ** -(p[3] + 1)))
+    diff = 0.0
+    for i in range(8):
+        for j in range(8):
+            q = 0.5 * (bits[i] + bits[j])
+            w = np.array([(1 - q[0]) if not (c & 4) else q[0] for c in range(8)]) * \
+                np.array([(1 - q[1]) if not (c & 2) else q[1] for c in range(8)]) * \
+                np.array([(1 - q[2]) if not (c & 1) else q[2] for c in range(8)])
+            diff += abs(kv[i, j] - float(w @ pv))
+    d_max = 2 * SQ3 * 2.0 ** -(p[3] + 1)
+    border = int(np.sum((kv.min(axis=1) <= 0) & (kv.max(axis=1) > -d_max)))
+    return diff / (border - 1) if border > 1 else 1000.0
+
+
+def test_node_limit_merges_the_best_approximated_parents_first():
+    """In the layer it stops in, the merged parents (their children gone,
+    which the unlimited build kept) approximate their children no worse than
+    any parent left split: the merge order is the reference's mean difference
+    between the children's corner values and the parent's trilinear
+    interpolation."""
+    depth, lim = 6, 8000
+    full, _ = oracle.sbs_octree(0, [0, 0, 0, R], depth, 4, 0, 1)
+    part, _ = oracle.sbs_octree(0, [0, 0, 0, R], depth, 4, 0, 1, lim)
+    fine_full = {k for k in _key_set(full) if k[3] == 1 << depth}
+    fine_part = {k for k in _key_set(part) if k[3] == 1 << depth}
+    parents_full = {(x // 2, y // 2, z // 2) for x, y, z, _ in fine_full}
+    parents_part = {(x // 2, y // 2, z // 2) for x, y, z, _ in fine_part}
+    merged = parents_full - parents_part
+    kept = parents_part
+    assert len(merged) > 50 and len(kept) > 50
+    wm = np.array([_weighted_diff((*c, depth - 1)) for c in merged])
+    wk = np.array([_weighted_diff((*c, depth - 1)) for c in kept])
+    assert wm.max() <= wk.min() + 1e-5, (wm.max(), wk.min())

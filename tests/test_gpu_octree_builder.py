@@ -63,6 +63,32 @@ def test_octree_builder_matches_oracle(gpu, case):
         assert len(lods) > 1
 
 
+LIMIT_CASES = [  # shape, params, depth, brick_size, brick_pad, nodes_limit
+    (L.PRIMITIVE_SPHERE, (0.0, 0.0, 0.0, 0.7), 6, 4, 0, 12000),
+    (L.PRIMITIVE_SPHERE, (0.0, 0.0, 0.0, 0.7), 6, 4, 1, 3000),
+    (L.PRIMITIVE_TORUS, (0.6, 0.2), 7, 2, 0, 20000),
+]
+
+
+@pytest.mark.parametrize("case", LIMIT_CASES, ids=[f"shape{c[0]}-d{c[2]}-limit{c[5]}" for c in LIMIT_CASES])
+def test_octree_node_limit_matches_oracle(gpu, case):
+    """frame_octree_limit_nodes (:2229-2330): the device builder's frame,
+    merged on the host, gives the oracle's bricks bit for bit."""
+    shape, params, depth, bs, pad, lim = case
+    dev, ms = L.sbs_octree_from_primitive(gpu, shape, params, depth, bs, pad, 1, nodes_limit=lim)
+    full, _ = L.sbs_octree_from_primitive(gpu, shape, params, depth, bs, pad, 1)
+    o_nodes, o_values = oracle.sbs_octree(shape, params, depth, bs, pad, 1, lim)
+    v = bs + 2 * pad + 1
+    n_words = (v ** 3 + 3) // 4
+    print(case, dev.nodes.shape[0], "bricks of", full.nodes.shape[0], round(ms, 1), "ms")
+    assert dev.nodes.shape[0] < full.nodes.shape[0]
+    if pad == 0:  # the limit counts leaves by their 8 corners; a padded lattice can keep more
+        assert dev.nodes.shape[0] <= lim
+    got, want = _bricks(dev.nodes, dev.values, n_words), _bricks(o_nodes, o_values, n_words)
+    assert got.keys() == want.keys()
+    assert all(got[k] == want[k] for k in want)
+
+
 @pytest.fixture(scope="module")
 def ico_octree(gpu):
     v, t = L.mesh_displaced_icosphere(5, 0.8, 0.05, 4, 7)
