@@ -245,13 +245,13 @@ int build_octree(Renderer* r, const lrtd::DevField& F, uint32_t depth, uint32_t 
   }
   hipStream_t s = r->stream;
   hipError_t e;
-  DevBuf b_cells, b_vals, b_kids, b_leaves, b_lvals, b_count, b_fill, b_keep, b_words;
+  DevBuf b_cells, b_kids, b_leaves, b_lvals, b_count, b_fill, b_keep, b_words;
   struct Release {
     std::vector<DevBuf*> bufs;
     ~Release() {
       for (DevBuf* b : bufs) b->release();
     }
-  } rel{{&b_cells, &b_vals, &b_kids, &b_leaves, &b_lvals, &b_count, &b_fill, &b_keep, &b_words}};
+  } rel{{&b_cells, &b_kids, &b_leaves, &b_lvals, &b_count, &b_fill, &b_keep, &b_words}};
   const uint32_t top = depth < 4 ? depth : 4;
   const uint32_t side = 1u << top;
   std::vector<uint4> cells;
