@@ -27,14 +27,13 @@ __device__ __forceinline__ void render_pixel(const DevScene& S, const TraceParam
   // view spans a multiple of 64 packed indices and chunks start on multiples
   // of 256), so the view, the super-block and the camera are wave-uniform:
   // computed on the scalar unit and read with scalar loads into SGPRs.
-  const uint64_t g0 = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(gtid >> 32)) << 32) |
-                      (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)gtid);
-  const uint32_t view = (uint32_t)(g0 / A.n_packed);
-  const uint32_t tid = (uint32_t)(gtid - (uint64_t)view * A.n_packed);
-  const uint32_t sb = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const uint32_t wbh = A.packed_w >> 3;
-  const uint32_t bhx = sb % wbh, bhy = sb / wbh;
-  const uint32_t r = tid & 63u, lh = r >> 4, l = r & 15u;
+  // The wave's super-block index in the launch fits 32 bits (launches hold
+  // fewer than 2^38 packed pixels, checked on the host).
+  const uint32_t wsb = __builtin_amdgcn_readfirstlane((uint32_t)(gtid >> 6));
+  const uint32_t view = A.sb_view.div(wsb);
+  const uint32_t sb = wsb - view * A.sb_view.d;
+  const uint32_t bhy = A.sb_row.div(sb), bhx = sb - bhy * A.sb_row.d;
+  const uint32_t r = (uint32_t)gtid & 63u, lh = r >> 4, l = r & 15u;
   const uint32_t x = ((bhx * 2 + (lh & 1u)) << 2) + (l & 3u);  // tid_to_xy with the uniform part hoisted
   const uint32_t y = ((bhy * 2 + (lh >> 1)) << 2) + (l >> 2);
   if (x >= A.W || y >= A.H) return;

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "trace_device.hpp"
+#include "fastdiv.hpp"
 
 namespace lrtd {
 
@@ -21,7 +22,20 @@ struct RenderArgs {
   uint32_t n_views;
   uint32_t spp, ray_gen_mode, seed;
   LightSet lights;  // shading modes LAMBERT / PHONG
+  // 8x8 super-blocks per view (n_packed / 64) and per row (packed_w / 8), as
+  // divisors of a wave's super-block index (set_image)
+  FastDiv sb_view, sb_row;
 };
+
+// Image size of a launch: W x H pixels, packed to multiples of 8.
+inline void set_image(RenderArgs& A, uint32_t W, uint32_t H) {
+  A.W = W;
+  A.H = H;
+  A.packed_w = 8 * ((W + 7) / 8);
+  A.n_packed = A.packed_w * (8 * ((H + 7) / 8));
+  A.sb_view = make_fastdiv(A.n_packed / 64);
+  A.sb_row = make_fastdiv(A.packed_w / 8);
+}
 
 struct RenderOut {
   float* t;        // [view][y][x] hit distance (1e9 on miss)

@@ -44,7 +44,17 @@ struct DevScene {
   const float4* brecs;      // per brick: {bmin.xyz, data_offset}, {sz_inv, d, d_inv, qscale}, {d_max, mult, 0.5*sz, -}
   uint32_t brick_size, brick_pad, bytes_per_value, layout;
   uint32_t v_size, quant_voxel_diag, type, n_nodes;
+  uint32_t pairs_bytes, values_bytes;  // sizes of pairs / values (< 4 GiB, checked when the scene is set)
 };
+
+// A raw buffer resource over [p, p + bytes): loads through it take a 32-bit
+// byte offset plus a scalar offset, so a brick's or a node pair's fields are
+// addressed without per-lane 64-bit address arithmetic, and an offset past
+// the end reads 0 instead of faulting. 0x00020000: the buffer's dword 3
+// (32-bit data format) for the gfx9 family.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
 
 struct Hit {
   float t;
@@ -125,6 +135,65 @@ __device__ __forceinline__ bool is_indexed(const DevScene& S) {
   return S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F || S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F_IN;
 }
 
+// The 8 quantised corner values of voxel (vx, vy, vz) of a DX-layout brick
+// (BVH2Common.cpp:725-762): q = (values[v_off + vId / (4/bpv)] >>
+// (8*bpv * (vId % (4/bpv)))) & max_val is the little-endian bpv-byte element
+// vId of the brick's words, read here directly with a byte / short / dword
+// load. Returns the smallest of the 8.
+__device__ __forceinline__ uint32_t load_corners_q(const DevScene& S, uint32_t v_off, uint32_t vx, uint32_t vy, uint32_t vz, uint32_t q[8]) {
+  const uint32_t vs = S.v_size, pad = S.brick_pad;
+  const uint32_t bpv = S.bytes_per_value;
+  const uint32_t b0 = (vx + pad) * vs * vs + (vy + pad) * vs + (vz + pad);
+  // corner i at element b0 + (i&4 ? vs*vs : 0) + (i&2 ? vs : 0) + (i&1): the
+  // x / y steps are scene constants (scalar offsets), the z step an immediate
+  const __amdgpu_buffer_rsrc_t rs = buf_rsrc(S.values, S.values_bytes);
+  const uint32_t sy = vs * bpv, sx = vs * vs * bpv, sxy = sx + sy;
+  const uint32_t base = v_off * 4u + b0 * bpv;
+  if (bpv == 1) {
+    q[0] = __builtin_amdgcn_raw_buffer_load_b8(rs, base, 0, 0);
+    q[1] = __builtin_amdgcn_raw_buffer_load_b8(rs, base + 1, 0, 0);
+    q[2] = __builtin_amdgcn_raw_buffer_load_b8(rs, base, sy, 0);
+    q[3] = __builtin_amdgcn_raw_buffer_load_b8(rs, base + 1, sy, 0);
+    q[4] = __builtin_amdgcn_raw_buffer_load_b8(rs, base, sx, 0);
+    q[5] = __builtin_amdgcn_raw_buffer_load_b8(rs, base + 1, sx, 0);
+    q[6] = __builtin_amdgcn_raw_buffer_load_b8(rs, base, sxy, 0);
+    q[7] = __builtin_amdgcn_raw_buffer_load_b8(rs, base + 1, sxy, 0);
+  } else if (bpv == 2) {
+    q[0] = __builtin_amdgcn_raw_buffer_load_b16(rs, base, 0, 0);
+    q[1] = __builtin_amdgcn_raw_buffer_load_b16(rs, base + 2, 0, 0);
+    q[2] = __builtin_amdgcn_raw_buffer_load_b16(rs, base, sy, 0);
+    q[3] = __builtin_amdgcn_raw_buffer_load_b16(rs, base + 2, sy, 0);
+    q[4] = __builtin_amdgcn_raw_buffer_load_b16(rs, base, sx, 0);
+    q[5] = __builtin_amdgcn_raw_buffer_load_b16(rs, base + 2, sx, 0);
+    q[6] = __builtin_amdgcn_raw_buffer_load_b16(rs, base, sxy, 0);
+    q[7] = __builtin_amdgcn_raw_buffer_load_b16(rs, base + 2, sxy, 0);
+  } else {
+    q[0] = __builtin_amdgcn_raw_buffer_load_b32(rs, base, 0, 0);
+    q[1] = __builtin_amdgcn_raw_buffer_load_b32(rs, base + 4, 0, 0);
+    q[2] = __builtin_amdgcn_raw_buffer_load_b32(rs, base, sy, 0);
+    q[3] = __builtin_amdgcn_raw_buffer_load_b32(rs, base + 4, sy, 0);
+    q[4] = __builtin_amdgcn_raw_buffer_load_b32(rs, base, sx, 0);
+    q[5] = __builtin_amdgcn_raw_buffer_load_b32(rs, base + 4, sx, 0);
+    q[6] = __builtin_amdgcn_raw_buffer_load_b32(rs, base, sxy, 0);
+    q[7] = __builtin_amdgcn_raw_buffer_load_b32(rs, base + 4, sxy, 0);
+  }
+  return min(min(min(q[0], q[1]), min(q[2], q[3])), min(min(q[4], q[5]), min(q[6], q[7])));
+}
+
+// Dequantised corner values -d_max + mult * q (BVH2Common.cpp:749-755) and
+// their minimum. The map is monotone (mult > 0, rounding preserves order), so
+// the minimum is also -d_max + mult * min(q): callers that only test the
+// minimum can defer the other conversions.
+__device__ __forceinline__ float dequant_corners(const uint32_t q[8], float d_max, float mult, float values[8]) {
+  float vmin = 1e6f;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    values[i] = -d_max + mult * (float)q[i];
+    vmin = fminf(vmin, values[i]);  // finite values: equal to the reference's min, maps to v_min3
+  }
+  return vmin;
+}
+
 // ---- BVH2Common.cpp:725-762, trilinear branch -----------------------------
 // d_max / mult: the brick's dequantisation constants (ignored for indexed
 // layouts), the same values load_voxel below derives from sz_inv.
@@ -141,31 +210,9 @@ __device__ __forceinline__ float load_voxel_q(const DevScene& S, uint32_t v_off,
       vmin = mn(vmin, values[i]);
     }
   } else {
-    // q = (values[v_off + vId / (4/bpv)] >> (8*bpv * (vId % (4/bpv)))) & max_val
-    // is the little-endian bpv-byte element vId of the brick's words, read
-    // here directly with a byte / short / dword load.
-    const uint32_t bpv = S.bytes_per_value;
-    const uint32_t b0 = (vx + pad) * vs * vs + (vy + pad) * vs + (vz + pad);
-    const uint32_t dx = vs * vs, dy = vs;
     uint32_t q[8];
-    if (bpv == 1) {
-      const uint8_t* p = (const uint8_t*)(S.values + v_off) + b0;
-#pragma unroll
-      for (int i = 0; i < 8; i++) q[i] = p[((i & 4) ? dx : 0u) + ((i & 2) ? dy : 0u) + (i & 1)];
-    } else if (bpv == 2) {
-      const uint16_t* p = (const uint16_t*)(S.values + v_off) + b0;
-#pragma unroll
-      for (int i = 0; i < 8; i++) q[i] = p[((i & 4) ? dx : 0u) + ((i & 2) ? dy : 0u) + (i & 1)];
-    } else {
-      const uint32_t* p = S.values + v_off + b0;
-#pragma unroll
-      for (int i = 0; i < 8; i++) q[i] = p[((i & 4) ? dx : 0u) + ((i & 2) ? dy : 0u) + (i & 1)];
-    }
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-      values[i] = -d_max + mult * (float)q[i];
-      vmin = fminf(vmin, values[i]);  // finite values: equal to the reference's min, maps to v_min3
-    }
+    load_corners_q(S, v_off, vx, vy, vz, q);
+    vmin = dequant_corners(q, d_max, mult, values);
   }
   return vmin;
 }
@@ -286,7 +333,10 @@ __device__ __forceinline__ bool LocalSurfaceIntersection(const TraceParams& P, v
   bool hit = false;
   unsigned iter = 0;
   float start_sign = 1;
-  float start_dist = trilinear(values, start_q + t * ray_dir);
+  // trilinear(values, start_q + 0 * ray_dir) in the reference: the zero step
+  // can change only the sign of a zero coordinate, which changes no output
+  // (it can flip only the sign of an all-zero sum, and sgn(+-0) = 0 either way).
+  float start_dist = trilinear(values, start_q);
   if (P.representation_mode == LRT_REPRESENTATION_MODE_SURFACE) start_sign = sgn(start_dist);
   start_dist *= start_sign;
 
@@ -500,6 +550,19 @@ __device__ __forceinline__ bool LocalSurfaceIntersection(const TraceParams& P, v
 
 __device__ __forceinline__ float step01(float e0, float e1, float x) { return clampf_((x - e0) / (e1 - e0), 0.0f, 1.0f); }
 
+// Per-brick constants (3 x float4, brick_records on the host) through a
+// buffer resource: 48 B per brick, fewer than 2^24 bricks.
+__device__ __forceinline__ void load_brick_rec(const DevScene& S, uint32_t brick, float4& r0, float4& r1, float4& r2) {
+  typedef __attribute__((ext_vector_type(4))) unsigned int u4v;
+  const __amdgpu_buffer_rsrc_t rb = buf_rsrc(S.brecs, S.n_nodes * 48u);
+  const uint32_t off = brick * 48u;
+  const u4v x = __builtin_amdgcn_raw_buffer_load_b128(rb, off, 0, 0), y = __builtin_amdgcn_raw_buffer_load_b128(rb, off + 16, 0, 0),
+            z = __builtin_amdgcn_raw_buffer_load_b128(rb, off + 32, 0, 0);
+  r0 = make_float4(__uint_as_float(x.x), __uint_as_float(x.y), __uint_as_float(x.z), __uint_as_float(x.w));
+  r1 = make_float4(__uint_as_float(y.x), __uint_as_float(y.y), __uint_as_float(y.z), __uint_as_float(y.w));
+  r2 = make_float4(__uint_as_float(z.x), __uint_as_float(z.y), __uint_as_float(z.z), __uint_as_float(z.w));
+}
+
 // ---- BVH2Common.cpp:764-984 ------------------------------------------------
 // rayDirInv is SafeInverse(ray_dir), evaluated once per ray by the caller
 // (the reference recomputes the identical value for every box test).
@@ -518,7 +581,8 @@ __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TracePar
   // Per-brick constants precomputed on the host with the same float
   // operations (brick_records): bmin = -1 + sz_inv * pos, sz_inv = 2/sz,
   // d = 2/(sz*bs), d_inv = 1/d, qscale = 0.5*sz*bs, d_max, mult, 0.5*sz.
-  const float4 r0 = S.brecs[3 * (size_t)brick], r1 = S.brecs[3 * (size_t)brick + 1], r2 = S.brecs[3 * (size_t)brick + 2];
+  float4 r0, r1, r2;
+  load_brick_rec(S, brick, r0, r1, r2);
   const uint32_t v_off = __float_as_uint(r0.w);
   const float bsf = (float)S.brick_size;
   const float sz_inv = r1.x, d = r1.y, d_inv = r1.z, qscale = r1.w;
@@ -540,9 +604,14 @@ __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TracePar
     v3 min_pos = bmin + d * vp;
     v3 max_pos = min_pos + d * v3(1, 1, 1);
     if (n_vox) (*n_vox)++;
-    float vmin = load_voxel_q<IDX>(S, v_off, (uint32_t)vp.x, (uint32_t)vp.y, (uint32_t)vp.z, r2.x, r2.y, values);
+    // DX layouts: the voxel's minimum from the smallest quantised value, the
+    // 8 values converted only at a candidate voxel (dequant_corners)
+    uint32_t q[8];
+    const float vmin = IDX ? load_voxel_q<IDX>(S, v_off, (uint32_t)vp.x, (uint32_t)vp.y, (uint32_t)vp.z, r2.x, r2.y, values)
+                           : -r2.x + r2.y * (float)load_corners_q(S, v_off, (uint32_t)vp.x, (uint32_t)vp.y, (uint32_t)vp.z, q);
     float2 fnf = RayBox(ray_pos, rayDirInv, min_pos, max_pos);
     if (tNear < fnf.x && vmin <= 0.0f) {
+      if (!IDX) dequant_corners(q, r2.x, r2.y, values);
       v3 start_pos = ray_pos + fnf.x * ray_dir;
       v3 start_q = (start_pos - min_pos) * qscale;
       float qFar = (fnf.y - fnf.x) * qscale;
@@ -622,7 +691,8 @@ __device__ __forceinline__ void FinishNormal(const DevScene& S, const TraceParam
   v3 norm(0, 0, 1);
   if (P.need_normal) {
     const uint32_t b = pHit->n_brick, vox = pHit->n_voxel;
-    const float4 r0 = S.brecs[3 * (size_t)b], r2 = S.brecs[3 * (size_t)b + 2];
+    float4 r0, r1, r2;
+    load_brick_rec(S, b, r0, r1, r2);
     float values[8];
     load_voxel_q<false>(S, __float_as_uint(r0.w), vox & 0xFFu, (vox >> 8) & 0xFFu, (vox >> 16) & 0xFFu, r2.x, r2.y, values);
     norm = fd_normal(values, v3(pHit->n_p0[0], pHit->n_p0[1], pHit->n_p0[2]), (vox & 0x80000000u) ? -1.0f : 1.0f);
@@ -643,6 +713,9 @@ __device__ __forceinline__ void mul4(const float* m, float x, float y, float z, 
 struct Camera {
   float projInv[16];
   float worldViewInv[16];
+  // worldViewInv * (0, 0, 0, 1), the eye position: the same float operations as
+  // mul4 on it, evaluated once per camera on the host (make_camera)
+  float origin[4];
 };
 
 __device__ __forceinline__ void eye_ray(const Camera& C, uint32_t x, uint32_t y, float dx, float dy, uint32_t W, uint32_t H, v3& pos,
@@ -652,11 +725,10 @@ __device__ __forceinline__ void eye_ray(const Camera& C, uint32_t x, uint32_t y,
   float p[4];
   mul4(C.projInv, 2.0f * fx - 1.0f, -2.0f * fy + 1.0f, 1.0f, 1.0f, p);
   v3 rd = normalize(v3(p[0] / p[3], p[1] / p[3], p[2] / p[3]));
-  float a[4], b[4];
-  mul4(C.worldViewInv, 0.0f, 0.0f, 0.0f, 1.0f, a);
+  float b[4];
   v3 q = v3(0, 0, 0) + 100.0f * rd;
   mul4(C.worldViewInv, q.x, q.y, q.z, 1.0f, b);
-  pos = v3(a[0], a[1], a[2]);
+  pos = v3(C.origin[0], C.origin[1], C.origin[2]);
   dir = normalize(v3(b[0], b[1], b[2]) - pos);
 }
 

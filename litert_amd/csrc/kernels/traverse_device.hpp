@@ -34,6 +34,93 @@ struct WorkCount {
   uint32_t seeds = 0;   // brick entries that passed the confidence gate
 };
 
+// Node pair `left` (BVHNodePair, 64 B) as four 16-B loads through a buffer
+// resource (fewer than 2^24 pairs: the scene's brick limit).
+__device__ __forceinline__ void load_pair(const DevScene& S, uint32_t left, uint4& a, uint4& b, uint4& c, uint4& e) {
+  typedef __attribute__((ext_vector_type(4))) unsigned int u4v;
+  const __amdgpu_buffer_rsrc_t rp = buf_rsrc(S.pairs, S.pairs_bytes);
+  const uint32_t off = left * 64u;
+  const u4v x = __builtin_amdgcn_raw_buffer_load_b128(rp, off, 0, 0), y = __builtin_amdgcn_raw_buffer_load_b128(rp, off + 16, 0, 0);
+  const u4v z = __builtin_amdgcn_raw_buffer_load_b128(rp, off + 32, 0, 0), w = __builtin_amdgcn_raw_buffer_load_b128(rp, off + 48, 0, 0);
+  a = make_uint4(x.x, x.y, x.z, x.w);
+  b = make_uint4(y.x, y.y, y.z, y.w);
+  c = make_uint4(z.x, z.y, z.z, z.w);
+  e = make_uint4(w.x, w.y, w.z, w.w);
+}
+
+// Slab test of one box for a ray whose direction signs are known: bit a of
+// OCT set when component a of the inverse direction is negative. The near and
+// far plane of each axis are then known, so min(lo, hi) / max(lo, hi) of
+// RayBox are the near / far products themselves (rounding is monotonic, so
+// the order of the two products follows the order of the planes): the same
+// two values with 6 fewer instructions. OCT < 0: the general form.
+template <int OCT>
+__device__ __forceinline__ float2 RayBoxOct(v3 o, v3 inv, v3 bmin, v3 bmax) {
+  if (OCT < 0) return RayBox(o, inv, bmin, bmax);
+  const float nx = inv.x * (((OCT & 1) ? bmax.x : bmin.x) - o.x), fx = inv.x * (((OCT & 1) ? bmin.x : bmax.x) - o.x);
+  const float ny = inv.y * (((OCT & 2) ? bmax.y : bmin.y) - o.y), fy = inv.y * (((OCT & 2) ? bmin.y : bmax.y) - o.y);
+  const float nz = inv.z * (((OCT & 4) ? bmax.z : bmin.z) - o.z), fz = inv.z * (((OCT & 4) ? bmin.z : bmax.z) - o.z);
+  return make_float2(fmaxf(fmaxf(nx, ny), nz), fminf(fminf(fx, fy), fz));
+}
+
+// Node-pair part of BVH2TraverseF32 (BVH2Common.cpp:3431-3466): descends
+// until the lane sits at a leaf or its stack is empty.
+template <bool COUNT, int OCT>
+__device__ __forceinline__ void DescendPairs(const DevScene& S, v3 ray_pos, v3 inv, float tNear, bool stopOnFirstHit, float hit_t, int& top,
+                                             uint32_t& left, Stack& st, WorkCount* wc) {
+  while (top >= 0 && ((left & LRT_LEAF_BIT) == 0)) {
+    uint4 a, b, c, e;
+    load_pair(S, left, a, b, c, e);
+    if (COUNT) wc->pairs++;
+    const float2 tm0 = RayBoxOct<OCT>(ray_pos, inv, v3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)),
+                                      v3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z)));
+    const float2 tm1 = RayBoxOct<OCT>(ray_pos, inv, v3(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z)),
+                                      v3(__uint_as_float(e.x), __uint_as_float(e.y), __uint_as_float(e.z)));
+    const uint32_t n0 = a.w, n1 = c.w;
+    const bool h0 = (tm0.x <= tm0.y) && (tm0.y >= tNear) && (tm0.x <= hit_t || !stopOnFirstHit);
+    const bool h1 = (tm1.x <= tm1.y) && (tm1.y >= tNear) && (tm1.x <= hit_t || !stopOnFirstHit);
+    left = h0 ? n0 : n1;
+    if (h0 && h1) {
+      const bool near0 = tm0.x <= tm1.x;
+      left = near0 ? n0 : n1;
+      st.put(top, near0 ? n1 : n0);
+      top++;
+    }
+    if (!h0 && !h1) {
+      top--;
+      left = st.get(top > 0 ? top : 0);
+    }
+  }
+}
+
+// The wave's common direction octant (bit a: inverse direction component a
+// negative), or -1 when the active lanes differ or a component is infinite
+// (a zero direction component: there the general slab test's min / max also
+// decide what an infinite product against a plane through the origin gives).
+__device__ __forceinline__ int wave_octant(v3 inv) {
+  const bool fin = fabsf(inv.x) < INFINITY && fabsf(inv.y) < INFINITY && fabsf(inv.z) < INFINITY;
+  const uint32_t o = (inv.x < 0.0f ? 1u : 0u) | (inv.y < 0.0f ? 2u : 0u) | (inv.z < 0.0f ? 4u : 0u) | (fin ? 0u : 8u);
+  const uint32_t o0 = __builtin_amdgcn_readfirstlane(o);
+  const bool same = __builtin_amdgcn_ballot_w64(o != o0) == 0;
+  return (same && o0 < 8u) ? (int)o0 : -1;
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void DescendPairsAny(int oct, const DevScene& S, v3 ray_pos, v3 inv, float tNear, bool stopOnFirstHit, float hit_t,
+                                                int& top, uint32_t& left, Stack& st, WorkCount* wc) {
+  switch (oct) {  // wave-uniform
+    case 0: DescendPairs<COUNT, 0>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc); break;
+    case 1: DescendPairs<COUNT, 1>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc); break;
+    case 2: DescendPairs<COUNT, 2>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc); break;
+    case 3: DescendPairs<COUNT, 3>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc); break;
+    case 4: DescendPairs<COUNT, 4>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc); break;
+    case 5: DescendPairs<COUNT, 5>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc); break;
+    case 6: DescendPairs<COUNT, 6>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc); break;
+    case 7: DescendPairs<COUNT, 7>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc); break;
+    default: DescendPairs<COUNT, -1>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc); break;
+  }
+}
+
 // BVH2Common.cpp:3407-3488. Returns through *hit; stack capacity is
 // lds_entries + spill entries (>= the tree's pair depth, checked on the host).
 template <bool COUNT, int RF = -1, bool IDX = true>
@@ -42,30 +129,9 @@ __device__ __forceinline__ void BVH2Traverse(const DevScene& S, const TraceParam
   int top = 0;
   uint32_t left = 0;
   bool hitFound = false;
+  const int oct = wave_octant(inv);
   while (top >= 0 && !hitFound) {
-    while (top >= 0 && ((left & LRT_LEAF_BIT) == 0)) {
-      const uint4* pp = S.pairs + 4 * (size_t)left;
-      const uint4 a = pp[0], b = pp[1], c = pp[2], e = pp[3];
-      if (COUNT) wc->pairs++;
-      const float2 tm0 = RayBox(ray_pos, inv, v3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)),
-                                v3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z)));
-      const float2 tm1 = RayBox(ray_pos, inv, v3(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z)),
-                                v3(__uint_as_float(e.x), __uint_as_float(e.y), __uint_as_float(e.z)));
-      const uint32_t n0 = a.w, n1 = c.w;
-      const bool h0 = (tm0.x <= tm0.y) && (tm0.y >= tNear) && (tm0.x <= hit->t || !stopOnFirstHit);
-      const bool h1 = (tm1.x <= tm1.y) && (tm1.y >= tNear) && (tm1.x <= hit->t || !stopOnFirstHit);
-      left = h0 ? n0 : n1;
-      if (h0 && h1) {
-        const bool near0 = tm0.x <= tm1.x;
-        left = near0 ? n0 : n1;
-        st.put(top, near0 ? n1 : n0);
-        top++;
-      }
-      if (!h0 && !h1) {
-        top--;
-        left = st.get(top > 0 ? top : 0);
-      }
-    }
+    DescendPairsAny<COUNT>(oct, S, ray_pos, inv, tNear, stopOnFirstHit, hit->t, top, left, st, wc);
     if (top >= 0 && left != 0xFFFFFFFFu) {
       const float tNearSdf = mx(tNear, 0.1f);  // SDF_BIAS, BVH2Common.cpp:3472-3473
       const float tPrev = hit->t;
@@ -110,8 +176,8 @@ __device__ __forceinline__ bool NextLeaf(const DevScene& S, v3 ray_pos, v3 inv, 
                                          uint32_t& left, Stack& st, WorkCount* wc) {
   for (;;) {
     while (top >= 0 && ((left & LRT_LEAF_BIT) == 0)) {
-      const uint4* pp = S.pairs + 4 * (size_t)left;
-      const uint4 a = pp[0], b = pp[1], c = pp[2], e = pp[3];
+      uint4 a, b, c, e;
+      load_pair(S, left, a, b, c, e);
       if (COUNT) wc->pairs++;
       const float2 tm0 = RayBox(ray_pos, inv, v3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)),
                                 v3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z)));

@@ -74,6 +74,9 @@ lrtd::Camera make_camera(const float wv[16], const float pr[16]) {
   lrtd::Camera c;
   inverse4x4(pr, c.projInv);
   inverse4x4(wv, c.worldViewInv);
+  // eye_ray's mul4(worldViewInv, 0, 0, 0, 1), operation for operation
+  const float* m = c.worldViewInv;
+  for (int i = 0; i < 4; i++) c.origin[i] = m[i] * 0.0f + m[4 + i] * 0.0f + m[8 + i] * 0.0f + m[12 + i] * 1.0f;
   return c;
 }
 
@@ -92,6 +95,8 @@ lrtd::DevScene Renderer::dev_scene() const {
   S.quant_voxel_diag = quant_voxel_diag;
   S.type = geom_type;
   S.n_nodes = (uint32_t)nodes.size();
+  S.pairs_bytes = (uint32_t)(pairs.size() * sizeof(BVHNodePair));
+  S.values_bytes = (uint32_t)(values.size() * 4);
   return S;
 }
 
@@ -189,10 +194,8 @@ int Renderer::render_device(const lrtd::Camera* cams, uint32_t n_views, uint32_t
     O.rgba = (float4*)d_out_rgba.p;
   }
   lrtd::RenderArgs A;
-  A.W = W;
-  A.H = H;
-  A.packed_w = 8 * ((W + 7) / 8);
-  A.n_packed = A.packed_w * (8 * ((H + 7) / 8));
+  lrtd::set_image(A, W, H);
+  if ((uint64_t)n_views * A.n_packed >= (1ull << 38)) return set_err(LRT_E_INVALID, "too many pixels in one launch (views x packed pixels >= 2^38)");
   A.n_views = n_views;
   A.spp = preset.spp;
   A.ray_gen_mode = preset.ray_gen_mode;
@@ -320,6 +323,9 @@ int lrt_set_scene_sbs(lrt_handle h, const SdfSBSHeader* header, const SdfSBSNode
           if (w[nv + 8 + k] != LRT_INVALID_IDX && w[nv + 8 + k] >= n_nodes) return set_err(LRT_E_INVALID, "neighbour index out of range");
     }
   }
+  // the kernels address the values through 32-bit byte offsets (the BVH's
+  // < 2^24 pairs of 64 B fit them by the leaf-index limit above)
+  if ((uint64_t)n_values * 4 > 0xFFFFFFFFull) return set_err(LRT_E_INVALID, "brick values exceed 4 GiB");
   Renderer* r = h;
   hipError_t e;
   if ((e = hipSetDevice(r->device)) != hipSuccess) return hip_err(e, "hipSetDevice");
@@ -562,10 +568,7 @@ int lrt_render_batch(lrt_handle owner, uint32_t n_items, const lrt_handle* scene
     lrtd::BatchItem& it = items[i];
     it.S = s->dev_scene();
     it.cam = make_camera(wvs + 16 * (size_t)i, prs + 16 * (size_t)i);
-    it.A.W = W;
-    it.A.H = H;
-    it.A.packed_w = 8 * ((W + 7) / 8);
-    it.A.n_packed = it.A.packed_w * (8 * ((H + 7) / 8));
+    lrtd::set_image(it.A, W, H);
     it.A.n_views = 1;
     it.A.spp = 1;
     it.A.ray_gen_mode = r->preset.ray_gen_mode;

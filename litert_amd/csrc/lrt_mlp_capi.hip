@@ -160,6 +160,8 @@ int lrt_mlp_collect(lrt_handle h, const float* wvs, const float* prs, uint32_t n
   if (!h->has_scene) return set_err(LRT_E_NOSCENE, "no scene: call lrt_set_scene_sbs first");
   if (!wvs || !prs || n_views == 0 || W == 0 || H == 0 || W > 65535 || H > 65535 || max_events == 0)
     return set_err(LRT_E_INVALID, "bad arguments");
+  if ((uint64_t)(8 * ((W + 7) / 8)) * (uint64_t)(8 * ((H + 7) / 8)) > 0xFFFFFFFFull)
+    return set_err(LRT_E_INVALID, "image too large: the 8-aligned pixel count must fit 32 bits");
   if ((uint64_t)n_views * W * H >= (1ull << 32)) return set_err(LRT_E_INVALID, "too many rays for 32-bit event ids");
   Renderer* r = h;
   hipError_t e;
@@ -175,10 +177,7 @@ int lrt_mlp_collect(lrt_handle h, const float* wvs, const float* prs, uint32_t n
   if ((e = r->d_counts.ensure(64)) != hipSuccess) return hip_err(e, "alloc counts");
   if ((e = zero_u32((uint32_t*)r->d_counts.p, 16, r->stream)) != hipSuccess) return hip_err(e, "clear counts");
   lrtd::RenderArgs A;
-  A.W = W;
-  A.H = H;
-  A.packed_w = 8 * ((W + 7) / 8);
-  A.n_packed = A.packed_w * (8 * ((H + 7) / 8));
+  lrtd::set_image(A, W, H);
   A.n_views = n_views;
   A.spp = 1;
   A.ray_gen_mode = r->preset.ray_gen_mode;

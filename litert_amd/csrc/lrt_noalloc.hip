@@ -118,10 +118,7 @@ extern "C" int lrt_render_views_noalloc(lrt_handle h, const float* world_views, 
   O.prim = d_prim;
   O.rgba = (float4*)d_rgba;
   lrtd::RenderArgs A;
-  A.W = width;
-  A.H = height;
-  A.packed_w = 8 * ((width + 7) / 8);
-  A.n_packed = A.packed_w * (8 * ((height + 7) / 8));
+  lrtd::set_image(A, width, height);
   A.n_views = n_views;
   A.spp = h->preset.spp;
   A.ray_gen_mode = h->preset.ray_gen_mode;

@@ -184,3 +184,13 @@ def test_counter_reduction_averages_dispatches_and_calibrates(tmp_path):
     assert j["calibration"]["fetch_scale"] == 2.0 and j["calibration"]["write_scale"] == 1.0
     assert j["hbm_read_bytes"] == int(15.0 * 2 * 1024) and j["hbm_write_bytes"] == int(40.0 * 1024)
     assert j["l2_hit_rate"] == 0.75 and j["valu_lane_utilisation"] == 0.5
+
+
+def test_fastdiv_matches_integer_division(tmp_path):
+    """The render kernels' divisions by launch constants (FastDiv,
+    litert_amd/csrc/kernels/fastdiv.hpp) against the host compiler's integer
+    division, compiled from the same header with g++."""
+    exe = str(tmp_path / "test_fastdiv")
+    subprocess.run(["g++", "-O2", "-std=c++17", os.path.join(ROOT, "tests", "cpp", "test_fastdiv.cpp"), "-o", exe], check=True)
+    out = subprocess.run([exe], stdout=subprocess.PIPE, text=True)
+    assert out.returncode == 0 and out.stdout.startswith("ok"), out.stdout
