@@ -132,11 +132,12 @@ std::vector<BVHNodePair> to_fat_pairs(const BuildTree& t, const std::vector<f3>&
   if (t.root < 0) { if (max_pair_depth) *max_pair_depth = 0; return out; }
   const BuildNode& root = t.nodes[t.root];
   auto leaf_code = [&](const BuildNode& n) { return LRT_LEAF_BIT | ((1u << 24) & LRT_SIZE_MASK) | (t.prims[n.start] & LRT_START_MASK); };
-  if (root.count > 0) {  // one primitive: pair it with an unreachable empty box
+  if (root.count > 0) {  // one primitive: the pair's second slot repeats its leaf and box
+    // (a second test of the same brick can only repeat the first one's result)
     uint32_t p = t.prims[root.start];
     BVHNodePair pr;
     pr.left = to_node(pmin[p], pmax[p], leaf_code(root));
-    pr.right = to_node(f3(1, 1, 1), f3(-1, -1, -1), LRT_LEAF_BIT | (1u << 24) | (p & LRT_START_MASK));
+    pr.right = to_node(pmin[p], pmax[p], leaf_code(root));
     out.push_back(pr);
     if (max_pair_depth) *max_pair_depth = 1;
     return out;

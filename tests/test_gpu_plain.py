@@ -411,7 +411,7 @@ def test_indexed_neighbour_layout_smoothed_normals():
 
 
 def test_single_brick_scene():
-    """A one-brick scene (its BVH is one pair with an empty second child)."""
+    """A one-brick scene (its BVH is one pair whose second slot repeats the brick)."""
     sbs = L.grid_sbs("sphere", 0.5, 0.0, 0.0, 1, seed=1, brick_count=1, brick_size=4)
     r = L.CreateMultiRenderer(L.DEVICE_GPU)
     r.SetScene(sbs)
