@@ -474,7 +474,8 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restr
   const v3 bmax = bmin + sz_inv * v3(1, 1, 1);
   const float qscale = 0.5f * sz * bsf;
   const uint32_t vs = S.v_size;
-  const float* bv = bvals + (size_t)brick * vs * vs * vs;  // this brick's corner values, gathered from values_f
+  // the bricks' corner values gathered from values_f (k_dr_gather), vs^3 floats per brick
+  const __amdgpu_buffer_rsrc_t rv = buf_rsrc(bvals, S.n_nodes * vs * vs * vs * 4u);
   float2 bnf = RayBox(ray_pos, inv, bmin, bmax);
   if (SPEC == kDrDepth || SPEC == kDrDepthBorder) {
     // Whole-brick form of the voxel skip below: the brick's smallest and
@@ -508,10 +509,21 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restr
     const uint32_t vx = (uint32_t)vp.x, vy = (uint32_t)vp.y, vz = (uint32_t)vp.z;
     v3 min_pos = bmin + d * vp;
     v3 max_pos = min_pos + d * v3(1, 1, 1);
-    // corner i of the voxel: values[i] = values_f[values[data_offset + vid(i)]]
+    // corner i of the voxel: values[i] = values_f[values[data_offset + vid(i)]],
+    // read from the brick's gathered copy: one vector offset, the y / x
+    // strides as scalar offsets, the z step as an immediate
     auto vid = [&](int i) { return (vx + ((i & 4) >> 2)) * vs * vs + (vy + ((i & 2) >> 1)) * vs + (vz + (i & 1)); };
-#pragma unroll
-    for (int i = 0; i < 8; i++) values[i] = bv[vid(i)];
+    {
+      const uint32_t o = (brick * vs * vs * vs + (vx * vs + vy) * vs + vz) * 4u, sy = vs * 4u, sx = vs * vs * 4u, sxy = sx + sy;
+      values[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rv, o, 0, 0));
+      values[1] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rv, o + 4, 0, 0));
+      values[2] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rv, o, sy, 0));
+      values[3] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rv, o + 4, sy, 0));
+      values[4] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rv, o, sx, 0));
+      values[5] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rv, o + 4, sx, 0));
+      values[6] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rv, o, sxy, 0));
+      values[7] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rv, o + 4, sxy, 0));
+    }
     float2 fnf = RayBox(ray_pos, inv, min_pos, max_pos);
     bool candidate = tNear < fnf.x;
     if ((SPEC == kDrDepth || SPEC == kDrDepthBorder) && candidate) {
@@ -655,6 +667,62 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restr
 }
 
 // BVH2DR.cpp:12-167 (stop at the first hit, forced in the reference)
+// Node-pair descent of dr_nearest_hit (BVH2Common.cpp:3431-3466 with the
+// border pass's culled slots), for rays of one direction octant (OCT >= 0,
+// RayBoxOct) or any (OCT = -1).
+template <int SPEC, int OCT>
+__device__ __forceinline__ void dr_descend(const DevScene& S, v3 pos, v3 inv, float tNear, float hit_t, int& top, uint32_t& left, Stack& st) {
+  while (top >= 0 && ((left & LRT_LEAF_BIT) == 0)) {
+    uint4 a, b, c, e;
+    load_pair(S, left, a, b, c, e);
+    const float2 tm0 = RayBoxOct<OCT>(pos, inv, v3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)),
+                              v3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z)));
+    const float2 tm1 = RayBoxOct<OCT>(pos, inv, v3(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z)),
+                              v3(__uint_as_float(e.x), __uint_as_float(e.y), __uint_as_float(e.z)));
+    const bool h0 = (tm0.x <= tm0.y) && (tm0.y >= tNear) && (tm0.x <= hit_t);
+    const bool h1 = (tm1.x <= tm1.y) && (tm1.y >= tNear) && (tm1.x <= hit_t);
+    uint32_t ch0 = a.w, ch1 = c.w;
+    if (SPEC == kDrDepthBorder) {
+      // A culled slot stands for bricks that are all skipped, each
+      // clearing the ray's state if the ray crosses it past the bricks'
+      // tNear (0.1, 1e-5 margin in dr_brick). As the bricks tile the
+      // box, a ray that crosses the box (positive length, entry past
+      // 0.1 + 1e-3) crosses some brick past that tNear: the slot becomes
+      // a leaf that only clears the state, visited in the subtree's
+      // place. Any other ray descends as over the full tree.
+      if (b.w == kDrCulledSlot && tm0.x < tm0.y && tm0.x > 0.1f + 1e-3f) ch0 = kDrResetLeaf;
+      if (e.w == kDrCulledSlot && tm1.x < tm1.y && tm1.x > 0.1f + 1e-3f) ch1 = kDrResetLeaf;
+    }
+    left = h0 ? ch0 : ch1;
+    if (h0 && h1) {
+      const bool near0 = tm0.x <= tm1.x;
+      left = near0 ? ch0 : ch1;
+      st.put(top, near0 ? ch1 : ch0);
+      top++;
+    }
+    if (!h0 && !h1) {
+      top--;
+      left = st.get(top > 0 ? top : 0);
+    }
+  }
+}
+
+template <int SPEC>
+__device__ __forceinline__ void dr_descend_any(int oct, const DevScene& S, v3 pos, v3 inv, float tNear, float hit_t, int& top, uint32_t& left,
+                                               Stack& st) {
+  switch (oct) {  // wave-uniform
+    case 0: dr_descend<SPEC, 0>(S, pos, inv, tNear, hit_t, top, left, st); break;
+    case 1: dr_descend<SPEC, 1>(S, pos, inv, tNear, hit_t, top, left, st); break;
+    case 2: dr_descend<SPEC, 2>(S, pos, inv, tNear, hit_t, top, left, st); break;
+    case 3: dr_descend<SPEC, 3>(S, pos, inv, tNear, hit_t, top, left, st); break;
+    case 4: dr_descend<SPEC, 4>(S, pos, inv, tNear, hit_t, top, left, st); break;
+    case 5: dr_descend<SPEC, 5>(S, pos, inv, tNear, hit_t, top, left, st); break;
+    case 6: dr_descend<SPEC, 6>(S, pos, inv, tNear, hit_t, top, left, st); break;
+    case 7: dr_descend<SPEC, 7>(S, pos, inv, tNear, hit_t, top, left, st); break;
+    default: dr_descend<SPEC, -1>(S, pos, inv, tNear, hit_t, top, left, st); break;
+  }
+}
+
 template <int SPEC>
 __device__ __forceinline__ DRHit dr_nearest_hit(const DevScene& S, const float* __restrict__ bvals, const float2* __restrict__ bminmax,
                                                 const DRParams& Q, float relax_eps, v3 pos, v3 dir, float tFar, DRPayload* relax,
@@ -686,40 +754,9 @@ __device__ __forceinline__ DRHit dr_nearest_hit(const DevScene& S, const float* 
     const float tNear = 0.0f;
     int top = 0;
     uint32_t left = 0;
+    const int oct = wave_octant(inv);
     while (top >= 0 && hit.primId == 0xFFFFFFFFu) {
-      while (top >= 0 && ((left & LRT_LEAF_BIT) == 0)) {
-        const uint4* pp = S.pairs + 4 * (size_t)left;
-        const uint4 a = pp[0], b = pp[1], c = pp[2], e = pp[3];
-        const float2 tm0 = RayBox(pos, inv, v3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)),
-                                  v3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z)));
-        const float2 tm1 = RayBox(pos, inv, v3(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z)),
-                                  v3(__uint_as_float(e.x), __uint_as_float(e.y), __uint_as_float(e.z)));
-        const bool h0 = (tm0.x <= tm0.y) && (tm0.y >= tNear) && (tm0.x <= hit.t);
-        const bool h1 = (tm1.x <= tm1.y) && (tm1.y >= tNear) && (tm1.x <= hit.t);
-        uint32_t ch0 = a.w, ch1 = c.w;
-        if (SPEC == kDrDepthBorder) {
-          // A culled slot stands for bricks that are all skipped, each
-          // clearing the ray's state if the ray crosses it past the bricks'
-          // tNear (0.1, 1e-5 margin in dr_brick). As the bricks tile the
-          // box, a ray that crosses the box (positive length, entry past
-          // 0.1 + 1e-3) crosses some brick past that tNear: the slot becomes
-          // a leaf that only clears the state, visited in the subtree's
-          // place. Any other ray descends as over the full tree.
-          if (b.w == kDrCulledSlot && tm0.x < tm0.y && tm0.x > 0.1f + 1e-3f) ch0 = kDrResetLeaf;
-          if (e.w == kDrCulledSlot && tm1.x < tm1.y && tm1.x > 0.1f + 1e-3f) ch1 = kDrResetLeaf;
-        }
-        left = h0 ? ch0 : ch1;
-        if (h0 && h1) {
-          const bool near0 = tm0.x <= tm1.x;
-          left = near0 ? ch0 : ch1;
-          st.put(top, near0 ? ch1 : ch0);
-          top++;
-        }
-        if (!h0 && !h1) {
-          top--;
-          left = st.get(top > 0 ? top : 0);
-        }
-      }
+      dr_descend_any<SPEC>(oct, S, pos, inv, tNear, hit.t, top, left, st);
       if (SPEC == kDrDepthBorder && top >= 0 && left == kDrResetLeaf) {
         relax->missed_cand.sdf = 1000.f;
         relax->missed_hit.check_vox_border = 0;
