@@ -476,7 +476,8 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restr
   const uint32_t vs = S.v_size;
   // the bricks' corner values gathered from values_f (k_dr_gather), vs^3 floats per brick
   const __amdgpu_buffer_rsrc_t rv = buf_rsrc(bvals, S.n_nodes * vs * vs * vs * 4u);
-  float2 bnf = RayBox(ray_pos, inv, bmin, bmax);
+  const bool ref = wave_ref_slabs(inv);
+  float2 bnf = RayBoxW(ref, ray_pos, inv, bmin, bmax);
   if (SPEC == kDrDepth || SPEC == kDrDepthBorder) {
     // Whole-brick form of the voxel skip below: the brick's smallest and
     // largest corner values bound every voxel's, so if they pass the test
@@ -524,7 +525,7 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restr
       values[6] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rv, o, sxy, 0));
       values[7] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rv, o + 4, sxy, 0));
     }
-    float2 fnf = RayBox(ray_pos, inv, min_pos, max_pos);
+    float2 fnf = RayBoxW(ref, ray_pos, inv, min_pos, max_pos);
     bool candidate = tNear < fnf.x;
     if ((SPEC == kDrDepth || SPEC == kDrDepthBorder) && candidate) {
       // Along the segment the trilinear distance is a convex combination of
@@ -671,14 +672,15 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restr
 // border pass's culled slots), for rays of one direction octant (OCT >= 0,
 // RayBoxOct) or any (OCT = -1).
 template <int SPEC, int OCT>
-__device__ __forceinline__ void dr_descend(const DevScene& S, v3 pos, v3 inv, float tNear, float hit_t, int& top, uint32_t& left, Stack& st) {
+__device__ __forceinline__ void dr_descend(const DevScene& S, v3 pos, v3 inv, float tNear, float hit_t, int& top, uint32_t& left, Stack& st,
+                                           bool ref) {
   while (top >= 0 && ((left & LRT_LEAF_BIT) == 0)) {
     uint4 a, b, c, e;
     load_pair(S, left, a, b, c, e);
     const float2 tm0 = RayBoxOct<OCT>(pos, inv, v3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)),
-                              v3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z)));
+                                      v3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z)), ref);
     const float2 tm1 = RayBoxOct<OCT>(pos, inv, v3(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z)),
-                              v3(__uint_as_float(e.x), __uint_as_float(e.y), __uint_as_float(e.z)));
+                                      v3(__uint_as_float(e.x), __uint_as_float(e.y), __uint_as_float(e.z)), ref);
     const bool h0 = (tm0.x <= tm0.y) && (tm0.y >= tNear) && (tm0.x <= hit_t);
     const bool h1 = (tm1.x <= tm1.y) && (tm1.y >= tNear) && (tm1.x <= hit_t);
     uint32_t ch0 = a.w, ch1 = c.w;
@@ -709,17 +711,17 @@ __device__ __forceinline__ void dr_descend(const DevScene& S, v3 pos, v3 inv, fl
 
 template <int SPEC>
 __device__ __forceinline__ void dr_descend_any(int oct, const DevScene& S, v3 pos, v3 inv, float tNear, float hit_t, int& top, uint32_t& left,
-                                               Stack& st) {
+                                               Stack& st, bool ref) {
   switch (oct) {  // wave-uniform
-    case 0: dr_descend<SPEC, 0>(S, pos, inv, tNear, hit_t, top, left, st); break;
-    case 1: dr_descend<SPEC, 1>(S, pos, inv, tNear, hit_t, top, left, st); break;
-    case 2: dr_descend<SPEC, 2>(S, pos, inv, tNear, hit_t, top, left, st); break;
-    case 3: dr_descend<SPEC, 3>(S, pos, inv, tNear, hit_t, top, left, st); break;
-    case 4: dr_descend<SPEC, 4>(S, pos, inv, tNear, hit_t, top, left, st); break;
-    case 5: dr_descend<SPEC, 5>(S, pos, inv, tNear, hit_t, top, left, st); break;
-    case 6: dr_descend<SPEC, 6>(S, pos, inv, tNear, hit_t, top, left, st); break;
-    case 7: dr_descend<SPEC, 7>(S, pos, inv, tNear, hit_t, top, left, st); break;
-    default: dr_descend<SPEC, -1>(S, pos, inv, tNear, hit_t, top, left, st); break;
+    case 0: dr_descend<SPEC, 0>(S, pos, inv, tNear, hit_t, top, left, st, false); break;
+    case 1: dr_descend<SPEC, 1>(S, pos, inv, tNear, hit_t, top, left, st, false); break;
+    case 2: dr_descend<SPEC, 2>(S, pos, inv, tNear, hit_t, top, left, st, false); break;
+    case 3: dr_descend<SPEC, 3>(S, pos, inv, tNear, hit_t, top, left, st, false); break;
+    case 4: dr_descend<SPEC, 4>(S, pos, inv, tNear, hit_t, top, left, st, false); break;
+    case 5: dr_descend<SPEC, 5>(S, pos, inv, tNear, hit_t, top, left, st, false); break;
+    case 6: dr_descend<SPEC, 6>(S, pos, inv, tNear, hit_t, top, left, st, false); break;
+    case 7: dr_descend<SPEC, 7>(S, pos, inv, tNear, hit_t, top, left, st, false); break;
+    default: dr_descend<SPEC, -1>(S, pos, inv, tNear, hit_t, top, left, st, ref); break;
   }
 }
 
@@ -749,14 +751,15 @@ __device__ __forceinline__ DRHit dr_nearest_hit(const DevScene& S, const float* 
     relax->col_idx[i] = 0xFFFFFFFFu;
     relax->col_w[i] = 0.0f;
   }
-  const float2 bh = RayBox(pos, inv, v3(-1, -1, -1), v3(1, 1, 1));
+  const bool ref = wave_ref_slabs(inv);  // see RayBoxRef
+  const float2 bh = RayBoxW(ref, pos, inv, v3(-1, -1, -1), v3(1, 1, 1));
   if ((bh.x <= bh.y) && (bh.y > 0.0f) && (bh.x < hit.t)) {
     const float tNear = 0.0f;
     int top = 0;
     uint32_t left = 0;
     const int oct = wave_octant(inv);
     while (top >= 0 && hit.primId == 0xFFFFFFFFu) {
-      dr_descend_any<SPEC>(oct, S, pos, inv, tNear, hit.t, top, left, st);
+      dr_descend_any<SPEC>(oct, S, pos, inv, tNear, hit.t, top, left, st, ref);
       if (SPEC == kDrDepthBorder && top >= 0 && left == kDrResetLeaf) {
         relax->missed_cand.sdf = 1000.f;
         relax->missed_hit.check_vox_border = 0;

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void k_mlp_collect(DevScene S, TraceParams P, 
     hit.t = 1e9f;
     hit.primId = hit.instId = hit.geomId = 0xFFFFFFFFu;
     hit.coords[0] = 1.0f; hit.coords[1] = 0.0f; hit.coords[2] = 0.0f; hit.coords[3] = 0.0f;
-    const float2 bh = RayBox(pos, inv, v3(-1, -1, -1), v3(1, 1, 1));
+    const float2 bh = RayBoxW(wave_ref_slabs(inv), pos, inv, v3(-1, -1, -1), v3(1, 1, 1));
     if (!((bh.x <= bh.y) && (bh.y > 0.0f) && (bh.x < hit.t))) continue;
     int top = 0;
     uint32_t left = 0;
@@ -492,7 +492,7 @@ __device__ __forceinline__ void seeded_pixel(const DevScene& S, const TraceParam
     hit.coords[0] = 1.0f; hit.coords[1] = 0.0f; hit.coords[2] = 0.0f; hit.coords[3] = 0.0f;
     bool active = false;
     if (valid) {
-      const float2 bh = RayBox(pos, inv, v3(-1, -1, -1), v3(1, 1, 1));
+      const float2 bh = RayBoxW(wave_ref_slabs(inv), pos, inv, v3(-1, -1, -1), v3(1, 1, 1));
       active = (bh.x <= bh.y) && (bh.y > 0.0f) && (bh.x < hit.t);
     }
     int top = 0;

@@ -84,12 +84,43 @@ __device__ __forceinline__ float2 RayBox(v3 o, v3 inv, v3 bmin, v3 bmax) {
   const float hi1 = inv.y * (bmax.y - o.y);
   const float lo2 = inv.z * (bmin.z - o.z);
   const float hi2 = inv.z * (bmax.z - o.z);
-  // fminf/fmaxf equal the reference's min/max here: the operands are finite
-  // (SafeInverse keeps |inv| <= 1e36) and a signed-zero difference cannot
-  // change any later comparison. They map to single v_min/v_max instructions.
+  // fminf/fmaxf equal the reference's min/max for finite operands (a
+  // signed-zero difference cannot change any later comparison) and map to
+  // single v_min/v_max instructions. The operands are finite unless a
+  // direction component is zero (SafeInverse then returns an infinite
+  // component); waves holding such a ray take RayBoxRef instead (RayBoxW).
   const float tmin = fmaxf(fminf(lo, hi), fminf(lo1, hi1));
   const float tmax = fminf(fmaxf(lo, hi), fmaxf(lo1, hi1));
   return make_float2(fmaxf(tmin, fminf(lo2, hi2)), fminf(tmax, fmaxf(lo2, hi2)));
+}
+
+// The same slab test with the reference's std::min / std::max ((b < a) ? b : a
+// and (a < b) ? b : a), which differ from fminf / fmaxf only for NaN operands:
+// a zero direction component (infinite inverse) and an origin on a slab plane
+// give inf * 0. The reference then keeps or drops the NaN by argument position
+// (a ray lying in a brick face still walks the brick), and this form does too.
+__device__ __forceinline__ float2 RayBoxRef(v3 o, v3 inv, v3 bmin, v3 bmax) {
+  const float lo = inv.x * (bmin.x - o.x);
+  const float hi = inv.x * (bmax.x - o.x);
+  const float lo1 = inv.y * (bmin.y - o.y);
+  const float hi1 = inv.y * (bmax.y - o.y);
+  const float lo2 = inv.z * (bmin.z - o.z);
+  const float hi2 = inv.z * (bmax.z - o.z);
+  const float tmin = mx(mn(lo, hi), mn(lo1, hi1));
+  const float tmax = mn(mx(lo, hi), mx(lo1, hi1));
+  return make_float2(mx(tmin, mn(lo2, hi2)), mn(tmax, mx(lo2, hi2)));
+}
+
+// True when an active lane of the wave has an infinite inverse-direction
+// component: that wave's slab tests use RayBoxRef (a wave-uniform choice; with
+// finite operands both forms give the same values).
+__device__ __forceinline__ bool wave_ref_slabs(v3 inv) {
+  const bool fin = fabsf(inv.x) < INFINITY && fabsf(inv.y) < INFINITY && fabsf(inv.z) < INFINITY;
+  return __builtin_amdgcn_ballot_w64(!fin) != 0;
+}
+
+__device__ __forceinline__ float2 RayBoxW(bool ref, v3 o, v3 inv, v3 bmin, v3 bmax) {
+  return ref ? RayBoxRef(o, inv, bmin, bmax) : RayBox(o, inv, bmin, bmax);
 }
 
 // ---- BVH2Common.cpp:62-72 --------------------------------------------------
@@ -589,7 +620,8 @@ __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TracePar
   const v3 bmin(r0.x, r0.y, r0.z);
   const v3 bmax = bmin + sz_inv * v3(1, 1, 1);
   const float hi_clamp = bsf - 1e-6f;
-  float2 bnf = RayBox(ray_pos, rayDirInv, bmin, bmax);
+  const bool ref = wave_ref_slabs(rayDirInv);
+  float2 bnf = RayBoxW(ref, ray_pos, rayDirInv, bmin, bmax);
   const float old_t = pHit->t;
   // A ray crosses at most 3*brick_size+1 voxels of a brick; the cap only
   // guarantees termination where the 1e-6 step could vanish in rounding.
@@ -609,7 +641,7 @@ __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TracePar
     uint32_t q[8];
     const float vmin = IDX ? load_voxel_q<IDX>(S, v_off, (uint32_t)vp.x, (uint32_t)vp.y, (uint32_t)vp.z, r2.x, r2.y, values)
                            : -r2.x + r2.y * (float)load_corners_q(S, v_off, (uint32_t)vp.x, (uint32_t)vp.y, (uint32_t)vp.z, q);
-    float2 fnf = RayBox(ray_pos, rayDirInv, min_pos, max_pos);
+    float2 fnf = RayBoxW(ref, ray_pos, rayDirInv, min_pos, max_pos);
     if (tNear < fnf.x && vmin <= 0.0f) {
       if (!IDX) dequant_corners(q, r2.x, r2.y, values);
       v3 start_pos = ray_pos + fnf.x * ray_dir;

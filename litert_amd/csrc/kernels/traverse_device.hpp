@@ -55,8 +55,8 @@ __device__ __forceinline__ void load_pair(const DevScene& S, uint32_t left, uint
 // the order of the two products follows the order of the planes): the same
 // two values with 6 fewer instructions. OCT < 0: the general form.
 template <int OCT>
-__device__ __forceinline__ float2 RayBoxOct(v3 o, v3 inv, v3 bmin, v3 bmax) {
-  if (OCT < 0) return RayBox(o, inv, bmin, bmax);
+__device__ __forceinline__ float2 RayBoxOct(v3 o, v3 inv, v3 bmin, v3 bmax, bool ref = false) {
+  if (OCT < 0) return RayBoxW(ref, o, inv, bmin, bmax);
   const float nx = inv.x * (((OCT & 1) ? bmax.x : bmin.x) - o.x), fx = inv.x * (((OCT & 1) ? bmin.x : bmax.x) - o.x);
   const float ny = inv.y * (((OCT & 2) ? bmax.y : bmin.y) - o.y), fy = inv.y * (((OCT & 2) ? bmin.y : bmax.y) - o.y);
   const float nz = inv.z * (((OCT & 4) ? bmax.z : bmin.z) - o.z), fz = inv.z * (((OCT & 4) ? bmin.z : bmax.z) - o.z);
@@ -67,15 +67,15 @@ __device__ __forceinline__ float2 RayBoxOct(v3 o, v3 inv, v3 bmin, v3 bmax) {
 // until the lane sits at a leaf or its stack is empty.
 template <bool COUNT, int OCT>
 __device__ __forceinline__ void DescendPairs(const DevScene& S, v3 ray_pos, v3 inv, float tNear, bool stopOnFirstHit, float hit_t, int& top,
-                                             uint32_t& left, Stack& st, WorkCount* wc) {
+                                             uint32_t& left, Stack& st, WorkCount* wc, bool ref) {
   while (top >= 0 && ((left & LRT_LEAF_BIT) == 0)) {
     uint4 a, b, c, e;
     load_pair(S, left, a, b, c, e);
     if (COUNT) wc->pairs++;
     const float2 tm0 = RayBoxOct<OCT>(ray_pos, inv, v3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)),
-                                      v3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z)));
+                                      v3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z)), ref);
     const float2 tm1 = RayBoxOct<OCT>(ray_pos, inv, v3(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z)),
-                                      v3(__uint_as_float(e.x), __uint_as_float(e.y), __uint_as_float(e.z)));
+                                      v3(__uint_as_float(e.x), __uint_as_float(e.y), __uint_as_float(e.z)), ref);
     const uint32_t n0 = a.w, n1 = c.w;
     const bool h0 = (tm0.x <= tm0.y) && (tm0.y >= tNear) && (tm0.x <= hit_t || !stopOnFirstHit);
     const bool h1 = (tm1.x <= tm1.y) && (tm1.y >= tNear) && (tm1.x <= hit_t || !stopOnFirstHit);
@@ -107,17 +107,17 @@ __device__ __forceinline__ int wave_octant(v3 inv) {
 
 template <bool COUNT>
 __device__ __forceinline__ void DescendPairsAny(int oct, const DevScene& S, v3 ray_pos, v3 inv, float tNear, bool stopOnFirstHit, float hit_t,
-                                                int& top, uint32_t& left, Stack& st, WorkCount* wc) {
+                                                int& top, uint32_t& left, Stack& st, WorkCount* wc, bool ref) {
   switch (oct) {  // wave-uniform
-    case 0: DescendPairs<COUNT, 0>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc); break;
-    case 1: DescendPairs<COUNT, 1>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc); break;
-    case 2: DescendPairs<COUNT, 2>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc); break;
-    case 3: DescendPairs<COUNT, 3>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc); break;
-    case 4: DescendPairs<COUNT, 4>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc); break;
-    case 5: DescendPairs<COUNT, 5>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc); break;
-    case 6: DescendPairs<COUNT, 6>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc); break;
-    case 7: DescendPairs<COUNT, 7>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc); break;
-    default: DescendPairs<COUNT, -1>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc); break;
+    case 0: DescendPairs<COUNT, 0>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc, false); break;
+    case 1: DescendPairs<COUNT, 1>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc, false); break;
+    case 2: DescendPairs<COUNT, 2>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc, false); break;
+    case 3: DescendPairs<COUNT, 3>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc, false); break;
+    case 4: DescendPairs<COUNT, 4>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc, false); break;
+    case 5: DescendPairs<COUNT, 5>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc, false); break;
+    case 6: DescendPairs<COUNT, 6>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc, false); break;
+    case 7: DescendPairs<COUNT, 7>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc, false); break;
+    default: DescendPairs<COUNT, -1>(S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc, ref); break;
   }
 }
 
@@ -129,9 +129,10 @@ __device__ __forceinline__ void BVH2Traverse(const DevScene& S, const TraceParam
   int top = 0;
   uint32_t left = 0;
   bool hitFound = false;
-  const int oct = wave_octant(inv);
+  const int oct = wave_octant(inv);  // -1 when a lane's inverse direction has an infinite component
+  const bool ref = wave_ref_slabs(inv);
   while (top >= 0 && !hitFound) {
-    DescendPairsAny<COUNT>(oct, S, ray_pos, inv, tNear, stopOnFirstHit, hit->t, top, left, st, wc);
+    DescendPairsAny<COUNT>(oct, S, ray_pos, inv, tNear, stopOnFirstHit, hit->t, top, left, st, wc, ref);
     if (top >= 0 && left != 0xFFFFFFFFu) {
       const float tNearSdf = mx(tNear, 0.1f);  // SDF_BIAS, BVH2Common.cpp:3472-3473
       const float tPrev = hit->t;
@@ -157,7 +158,7 @@ __device__ __forceinline__ Hit RayQuery_NearestHit(const DevScene& S, const Trac
   hit.t = tFar;
   hit.primId = hit.instId = hit.geomId = 0xFFFFFFFFu;
   hit.coords[0] = 1.0f; hit.coords[1] = 0.0f; hit.coords[2] = 0.0f; hit.coords[3] = 0.0f;
-  const float2 bh = RayBox(pos, inv, v3(-1, -1, -1), v3(1, 1, 1));
+  const float2 bh = RayBoxW(wave_ref_slabs(inv), pos, inv, v3(-1, -1, -1), v3(1, 1, 1));
   if ((bh.x <= bh.y) && (bh.y > tNear) && (bh.x < hit.t)) BVH2Traverse<COUNT, RF, IDX>(S, P, pos, dir, inv, tNear, stopOnFirstHit, &hit, st, wc);
   // DX layouts: the hit normal, once per ray after the traversal (all hit
   // lanes of the wave together rather than in every voxel step that hits)
@@ -174,15 +175,16 @@ __device__ __forceinline__ Hit RayQuery_NearestHit(const DevScene& S, const Trac
 template <bool COUNT>
 __device__ __forceinline__ bool NextLeaf(const DevScene& S, v3 ray_pos, v3 inv, float tNear, bool stopOnFirstHit, float hit_t, int& top,
                                          uint32_t& left, Stack& st, WorkCount* wc) {
+  const bool ref = wave_ref_slabs(inv);
   for (;;) {
     while (top >= 0 && ((left & LRT_LEAF_BIT) == 0)) {
       uint4 a, b, c, e;
       load_pair(S, left, a, b, c, e);
       if (COUNT) wc->pairs++;
-      const float2 tm0 = RayBox(ray_pos, inv, v3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)),
-                                v3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z)));
-      const float2 tm1 = RayBox(ray_pos, inv, v3(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z)),
-                                v3(__uint_as_float(e.x), __uint_as_float(e.y), __uint_as_float(e.z)));
+      const float2 tm0 = RayBoxW(ref, ray_pos, inv, v3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)),
+                                 v3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z)));
+      const float2 tm1 = RayBoxW(ref, ray_pos, inv, v3(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z)),
+                                 v3(__uint_as_float(e.x), __uint_as_float(e.y), __uint_as_float(e.z)));
       const uint32_t n0 = a.w, n1 = c.w;
       const bool h0 = (tm0.x <= tm0.y) && (tm0.y >= tNear) && (tm0.x <= hit_t || !stopOnFirstHit);
       const bool h1 = (tm1.x <= tm1.y) && (tm1.y >= tNear) && (tm1.x <= hit_t || !stopOnFirstHit);

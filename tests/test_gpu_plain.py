@@ -250,6 +250,7 @@ def test_cast_rays_random(r_ico128):
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     rays = np.zeros((n, 8), np.float32)
     rays[:, :3], rays[:, 4:7], rays[:, 7] = o, d, 1e9
+    r_ico128.SetPreset(preset())  # the oracle's preset (the fixture's default is another root finder)
     hits = r_ico128.RayQuery_NearestHit(rays)
     g = np.array([(h.t, h.primId, h.coords[2], h.coords[3]) for h in hits])
     ref = sc.cast_rays(preset(), rays, precision=32)
@@ -257,6 +258,37 @@ def test_cast_rays_random(r_ico128):
     assert (gh != rh).sum() <= 2
     both = gh & rh
     assert np.mean(g[both, 0].astype(np.float32) == ref[both, 0].astype(np.float32)) > 0.999
+
+
+def test_cast_rays_on_grid_planes(r_ico128):
+    """Axis-aligned rays whose origins lie exactly on brick and voxel planes:
+    a zero direction component makes the inverse infinite, and a slab whose
+    plane holds the origin gives inf * 0 = NaN. The slab test keeps the
+    reference's std::min / std::max (raytrace_common.h:68-81), whose NaN
+    handling decides whether such a ray, lying in a brick face, meets the
+    brick; hit mask and depth must equal the oracle's."""
+    sc = oracle_scene("ico128")
+    rng = np.random.default_rng(11)
+    n = 2048
+    step = 2.0 / 128  # voxel planes of the 128^3 grid (brick planes every 4th)
+    o = -1.0 + step * rng.integers(0, 129, (n, 3)).astype(np.float64)
+    d = np.zeros((n, 3))
+    axis = rng.integers(0, 3, n)
+    sign = np.where(rng.random(n) < 0.5, -1.0, 1.0)
+    d[np.arange(n), axis] = sign
+    o[np.arange(n), axis] = -1.5 * sign  # start outside, travel through the grid
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3], rays[:, 4:7], rays[:, 7] = o, d, 1e9
+    r_ico128.SetPreset(preset())
+    hits = r_ico128.RayQuery_NearestHit(rays)
+    g = np.array([(h.t, h.primId) for h in hits])
+    ref = sc.cast_rays(preset(), rays, precision=32)
+    gh, rh = g[:, 1] != MISS, ref[:, 1] != MISS
+    print("hits", gh.sum(), "oracle", rh.sum(), "mask mismatches", (gh != rh).sum())
+    assert gh.sum() > 200
+    assert (gh != rh).sum() == 0
+    both = gh & rh
+    assert np.array_equal(g[both, 0].astype(np.float32), ref[both, 0].astype(np.float32))
 
 
 @pytest.mark.parametrize("tfar", [1e9, 2.0])
@@ -273,6 +305,7 @@ def test_cast_rays_any_hit(r_ico128, tfar):
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     rays = np.zeros((n, 8), np.float32)
     rays[:, :3], rays[:, 4:7], rays[:, 7] = o, d, -tfar
+    r_ico128.SetPreset(preset())
     hits = r_ico128.RayQuery_NearestHit(rays)
     g = np.array([(h.t, h.primId) for h in hits])
     ref = sc.cast_rays(preset(), rays, precision=32)
