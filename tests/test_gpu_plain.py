@@ -375,10 +375,11 @@ def test_reference_fixture_gbuffer_parity(mode):
     r.close()
 
 
-@pytest.mark.parametrize("W,H", [(1, 1), (7, 13), (129, 65), (1000, 3), (3, 700)])
+@pytest.mark.parametrize("W,H", [(1, 1), (7, 13), (129, 65), (1000, 3), (3, 700), (65535, 5), (5, 65535)])
 def test_ragged_image_sizes(r_ico128, W, H):
     """Image sides that are not multiples of the 8x8 tile (partial tiles,
-    single rows / columns) against the FP32 and FP64 oracle."""
+    single rows / columns, the largest side the API takes) against the FP32
+    and FP64 oracle."""
     sc = oracle_scene("ico128")
     wv, pr = L.orbit_camera(31, 0, aspect=W / H)
     res = _compare(r_ico128, sc, preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON), wv, pr, W, H, voxels_per_unit=32 * 4 / 2.0)
@@ -387,6 +388,24 @@ def test_ragged_image_sizes(r_ico128, W, H):
     assert res["mask_mismatch_fp64"] <= max(1, 1e-4 * res["n_pix"]) and res["mismatch_all_grazing"]
     assert res["depth_rel_max"] <= 1e-5
     assert res["normal_bad"] == 0
+
+
+def test_many_views_in_one_launch(r_ico128):
+    """37 views of an odd-sized image in one launch (a wave finds its view and
+    8x8 super-block from its packed index): every view equals the same view
+    rendered alone, bit for bit."""
+    W, H = 43, 29
+    cams = [L.orbit_camera(17, i, aspect=W / H) for i in range(37)]
+    wvs, prs = np.stack([c[0] for c in cams]), np.stack([c[1] for c in cams])
+    r_ico128.SetPreset(preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON))
+    r_ico128.RenderViewsDevice(wvs, prs, W, H, 0)
+    multi = [r_ico128.DownloadView(v, W, H) for v in range(37)]
+    for v in (0, 1, 17, 36):
+        t, n, prim = r_ico128.RenderGBuffer(W, H, wvs[v], prs[v])
+        tm, nm, pm = multi[v]
+        assert (prim != MISS).sum() > 20, v
+        assert np.array_equal(prim, pm) and np.array_equal(t.view(np.uint32), tm.view(np.uint32)), v
+        assert np.array_equal(n.view(np.uint32), nm.view(np.uint32)), v
 
 
 @pytest.mark.parametrize("eye", [[0.95, 0.1, 0.05], [0.0, 0.0, 0.0], [0.3, -0.85, 0.2]])
