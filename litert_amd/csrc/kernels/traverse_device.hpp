@@ -175,31 +175,10 @@ __device__ __forceinline__ Hit RayQuery_NearestHit(const DevScene& S, const Trac
 template <bool COUNT>
 __device__ __forceinline__ bool NextLeaf(const DevScene& S, v3 ray_pos, v3 inv, float tNear, bool stopOnFirstHit, float hit_t, int& top,
                                          uint32_t& left, Stack& st, WorkCount* wc) {
+  const int oct = wave_octant(inv);
   const bool ref = wave_ref_slabs(inv);
   for (;;) {
-    while (top >= 0 && ((left & LRT_LEAF_BIT) == 0)) {
-      uint4 a, b, c, e;
-      load_pair(S, left, a, b, c, e);
-      if (COUNT) wc->pairs++;
-      const float2 tm0 = RayBoxW(ref, ray_pos, inv, v3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)),
-                                 v3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z)));
-      const float2 tm1 = RayBoxW(ref, ray_pos, inv, v3(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z)),
-                                 v3(__uint_as_float(e.x), __uint_as_float(e.y), __uint_as_float(e.z)));
-      const uint32_t n0 = a.w, n1 = c.w;
-      const bool h0 = (tm0.x <= tm0.y) && (tm0.y >= tNear) && (tm0.x <= hit_t || !stopOnFirstHit);
-      const bool h1 = (tm1.x <= tm1.y) && (tm1.y >= tNear) && (tm1.x <= hit_t || !stopOnFirstHit);
-      left = h0 ? n0 : n1;
-      if (h0 && h1) {
-        const bool near0 = tm0.x <= tm1.x;
-        left = near0 ? n0 : n1;
-        st.put(top, near0 ? n1 : n0);
-        top++;
-      }
-      if (!h0 && !h1) {
-        top--;
-        left = st.get(top > 0 ? top : 0);
-      }
-    }
+    DescendPairsAny<COUNT>(oct, S, ray_pos, inv, tNear, stopOnFirstHit, hit_t, top, left, st, wc, ref);
     if (top < 0) return false;
     if (left != 0xFFFFFFFFu) return true;
     top--;
