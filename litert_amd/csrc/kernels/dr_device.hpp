@@ -506,7 +506,9 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restr
     v3 hit_pos = ray_pos + bnf.x * ray_dir;
     v3 local = (hit_pos - bmin) * qscale;
     const float hi = bsf - 1e-6f;
-    v3 vp(floorf(clampf_(local.x, 1e-6f, hi)), floorf(clampf_(local.y, 1e-6f, hi)), floorf(clampf_(local.z, 1e-6f, hi)));
+    // clampf_ of finite values: the median of value and bounds (v_med3)
+    v3 vp(floorf(__builtin_amdgcn_fmed3f(local.x, 1e-6f, hi)), floorf(__builtin_amdgcn_fmed3f(local.y, 1e-6f, hi)),
+          floorf(__builtin_amdgcn_fmed3f(local.z, 1e-6f, hi)));
     const uint32_t vx = (uint32_t)vp.x, vy = (uint32_t)vp.y, vz = (uint32_t)vp.z;
     v3 min_pos = bmin + d * vp;
     v3 max_pos = min_pos + d * v3(1, 1, 1);

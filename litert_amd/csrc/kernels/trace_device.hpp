@@ -120,7 +120,9 @@ __device__ __forceinline__ bool wave_ref_slabs(v3 inv) {
 }
 
 __device__ __forceinline__ float2 RayBoxW(bool ref, v3 o, v3 inv, v3 bmin, v3 bmax) {
-  return ref ? RayBoxRef(o, inv, bmin, bmax) : RayBox(o, inv, bmin, bmax);
+  float2 r = RayBox(o, inv, bmin, bmax);
+  if (ref) r = RayBoxRef(o, inv, bmin, bmax);
+  return r;
 }
 
 // ---- BVH2Common.cpp:62-72 --------------------------------------------------
@@ -630,9 +632,10 @@ __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TracePar
   while (bnf.x < bnf.y && pHit->t == old_t && steps++ < max_steps) {
     v3 hit_pos = ray_pos + bnf.x * ray_dir;
     v3 local = (hit_pos - bmin) * qscale;
-    // clamp of finite values with fminf/fmaxf: same result, one v_med3 per axis
-    v3 vp(floorf(fminf(fmaxf(local.x, 1e-6f), hi_clamp)), floorf(fminf(fmaxf(local.y, 1e-6f), hi_clamp)),
-          floorf(fminf(fmaxf(local.z, 1e-6f), hi_clamp)));
+    // the reference's clamp to [1e-6, bs - 1e-6] of finite values (the
+    // median of value and bounds, one v_med3 per axis)
+    v3 vp(floorf(__builtin_amdgcn_fmed3f(local.x, 1e-6f, hi_clamp)), floorf(__builtin_amdgcn_fmed3f(local.y, 1e-6f, hi_clamp)),
+          floorf(__builtin_amdgcn_fmed3f(local.z, 1e-6f, hi_clamp)));
     v3 min_pos = bmin + d * vp;
     v3 max_pos = min_pos + d * v3(1, 1, 1);
     if (n_vox) (*n_vox)++;
