@@ -476,6 +476,7 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restr
   const uint32_t vs = S.v_size;
   // the bricks' corner values gathered from values_f (k_dr_gather), vs^3 floats per brick
   const __amdgpu_buffer_rsrc_t rv = buf_rsrc(bvals, S.n_nodes * vs * vs * vs * 4u);
+  const uint32_t bbase = brick * vs * vs * vs * 4u;  // byte offset of the brick's copy
   const bool ref = wave_ref_slabs(inv);
   float2 bnf = RayBoxW(ref, ray_pos, inv, bmin, bmax);
   if (SPEC == kDrDepth || SPEC == kDrDepthBorder) {
@@ -517,7 +518,10 @@ __device__ __forceinline__ void dr_brick(const DevScene& S, const float* __restr
     // strides as scalar offsets, the z step as an immediate
     auto vid = [&](int i) { return (vx + ((i & 4) >> 2)) * vs * vs + (vy + ((i & 2) >> 1)) * vs + (vz + (i & 1)); };
     {
-      const uint32_t o = (brick * vs * vs * vs + (vx * vs + vy) * vs + vz) * 4u, sy = vs * 4u, sx = vs * vs * 4u, sxy = sx + sy;
+      // element (vx * vs + vy) * vs + vz from the float voxel coordinates:
+      // small integers (brick_size <= 16), exact in float, one conversion
+      const uint32_t o = bbase + ((uint32_t)fmaf(fmaf(vp.x, S.v_size_f, vp.y), S.v_size_f, vp.z) << 2), sy = vs * 4u, sx = vs * vs * 4u,
+                     sxy = sx + sy;
       values[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rv, o, 0, 0));
       values[1] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rv, o + 4, 0, 0));
       values[2] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rv, o, sy, 0));

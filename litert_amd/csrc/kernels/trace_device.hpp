@@ -45,6 +45,7 @@ struct DevScene {
   uint32_t brick_size, brick_pad, bytes_per_value, layout;
   uint32_t v_size, quant_voxel_diag, type, n_nodes;
   uint32_t pairs_bytes, values_bytes;  // sizes of pairs / values (< 4 GiB, checked when the scene is set)
+  float v_size_f;                      // (float)v_size, a scalar operand for the voxel walk's index arithmetic
 };
 
 // A raw buffer resource over [p, p + bytes): loads through it take a 32-bit
@@ -168,20 +169,19 @@ __device__ __forceinline__ bool is_indexed(const DevScene& S) {
   return S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F || S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F_IN;
 }
 
-// The 8 quantised corner values of voxel (vx, vy, vz) of a DX-layout brick
+// The 8 quantised corner values of a voxel of a DX-layout brick
 // (BVH2Common.cpp:725-762): q = (values[v_off + vId / (4/bpv)] >>
 // (8*bpv * (vId % (4/bpv)))) & max_val is the little-endian bpv-byte element
 // vId of the brick's words, read here directly with a byte / short / dword
-// load. Returns the smallest of the 8.
-__device__ __forceinline__ uint32_t load_corners_q(const DevScene& S, uint32_t v_off, uint32_t vx, uint32_t vy, uint32_t vz, uint32_t q[8]) {
-  const uint32_t vs = S.v_size, pad = S.brick_pad;
+// load. base: byte offset of the voxel's corner 0 in the values array
+// (corner_base). Returns the smallest of the 8.
+__device__ __forceinline__ uint32_t load_corners_q_at(const DevScene& S, uint32_t base, uint32_t q[8]) {
+  const uint32_t vs = S.v_size;
   const uint32_t bpv = S.bytes_per_value;
-  const uint32_t b0 = (vx + pad) * vs * vs + (vy + pad) * vs + (vz + pad);
   // corner i at element b0 + (i&4 ? vs*vs : 0) + (i&2 ? vs : 0) + (i&1): the
   // x / y steps are scene constants (scalar offsets), the z step an immediate
   const __amdgpu_buffer_rsrc_t rs = buf_rsrc(S.values, S.values_bytes);
   const uint32_t sy = vs * bpv, sx = vs * vs * bpv, sxy = sx + sy;
-  const uint32_t base = v_off * 4u + b0 * bpv;
   if (bpv == 1) {
     q[0] = __builtin_amdgcn_raw_buffer_load_b8(rs, base, 0, 0);
     q[1] = __builtin_amdgcn_raw_buffer_load_b8(rs, base + 1, 0, 0);
@@ -211,6 +211,31 @@ __device__ __forceinline__ uint32_t load_corners_q(const DevScene& S, uint32_t v
     q[7] = __builtin_amdgcn_raw_buffer_load_b32(rs, base + 4, sxy, 0);
   }
   return min(min(min(q[0], q[1]), min(q[2], q[3])), min(min(q[4], q[5]), min(q[6], q[7])));
+}
+
+// Byte offset of corner 0 of voxel (vx, vy, vz) of the brick at word v_off:
+// element ((vx + pad) * vs + vy + pad) * vs + vz + pad of bpv bytes, written
+// as (vx * vs + vy) * vs + vz plus the brick's constant pad_base.
+__device__ __forceinline__ uint32_t pad_base(const DevScene& S, uint32_t v_off) {
+  const uint32_t vs = S.v_size;
+  return v_off * 4u + ((S.brick_pad * (vs * vs + vs + 1u)) << (S.bytes_per_value >> 1));  // x bpv for bpv 1, 2, 4
+}
+__device__ __forceinline__ uint32_t corner_base(const DevScene& S, uint32_t pbase, uint32_t vx, uint32_t vy, uint32_t vz) {
+  const uint32_t vs = S.v_size;
+  return pbase + (((vx * vs + vy) * vs + vz) << (S.bytes_per_value >> 1));
+}
+
+// The same offset from the voxel walk's float coordinates (floored, in
+// [0, brick_size)): scenes are accepted with brick_size <= 16 and pad <= 1,
+// so every term is a small integer, the float operations are exact, and one
+// conversion replaces three conversions and two 32-bit integer multiplies.
+__device__ __forceinline__ uint32_t corner_base_f(const DevScene& S, uint32_t pbase, v3 vp) {
+  const float vs = S.v_size_f;
+  return pbase + ((uint32_t)fmaf(fmaf(vp.x, vs, vp.y), vs, vp.z) << (S.bytes_per_value >> 1));
+}
+
+__device__ __forceinline__ uint32_t load_corners_q(const DevScene& S, uint32_t v_off, uint32_t vx, uint32_t vy, uint32_t vz, uint32_t q[8]) {
+  return load_corners_q_at(S, corner_base(S, pad_base(S, v_off), vx, vy, vz), q);
 }
 
 // Dequantised corner values -d_max + mult * q (BVH2Common.cpp:749-755) and
@@ -624,6 +649,7 @@ __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TracePar
   const float hi_clamp = bsf - 1e-6f;
   const bool ref = wave_ref_slabs(rayDirInv);
   float2 bnf = RayBoxW(ref, ray_pos, rayDirInv, bmin, bmax);
+  const uint32_t pbase = IDX ? 0u : pad_base(S, v_off);
   const float old_t = pHit->t;
   // A ray crosses at most 3*brick_size+1 voxels of a brick; the cap only
   // guarantees termination where the 1e-6 step could vanish in rounding.
@@ -643,7 +669,7 @@ __device__ __forceinline__ void BrickIntersect(const DevScene& S, const TracePar
     // 8 values converted only at a candidate voxel (dequant_corners)
     uint32_t q[8];
     const float vmin = IDX ? load_voxel_q<IDX>(S, v_off, (uint32_t)vp.x, (uint32_t)vp.y, (uint32_t)vp.z, r2.x, r2.y, values)
-                           : -r2.x + r2.y * (float)load_corners_q(S, v_off, (uint32_t)vp.x, (uint32_t)vp.y, (uint32_t)vp.z, q);
+                           : -r2.x + r2.y * (float)load_corners_q_at(S, corner_base_f(S, pbase, vp), q);
     float2 fnf = RayBoxW(ref, ray_pos, rayDirInv, min_pos, max_pos);
     if (tNear < fnf.x && vmin <= 0.0f) {
       if (!IDX) dequant_corners(q, r2.x, r2.y, values);

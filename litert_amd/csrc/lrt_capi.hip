@@ -97,6 +97,7 @@ lrtd::DevScene Renderer::dev_scene() const {
   S.n_nodes = (uint32_t)nodes.size();
   S.pairs_bytes = (uint32_t)(pairs.size() * sizeof(BVHNodePair));
   S.values_bytes = (uint32_t)(values.size() * 4);
+  S.v_size_f = (float)S.v_size;
   return S;
 }
 
