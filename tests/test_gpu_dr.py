@@ -43,9 +43,15 @@ def problem():
     return _problem()
 
 
-@pytest.mark.parametrize("dtdv_mode", [0, 1])
-def test_render_dr_matches_oracle(problem, dtdv_mode):
-    tgt, ini, cams, refs = problem
+@pytest.fixture(scope="module")
+def problem_bs16():
+    # the largest accepted bricks: 2^3 bricks of 16^3 voxels
+    return _problem(brick_count=2, brick_size=16)
+
+
+@pytest.mark.parametrize("dtdv_mode,bricks", [(0, 4), (1, 4), (0, 16)])
+def test_render_dr_matches_oracle(problem, problem_bs16, dtdv_mode, bricks):
+    tgt, ini, cams, refs = problem if bricks == 4 else problem_bs16
     dr = _preset()
     g = L.MultiRendererDR()
     g.SetScene(ini)

@@ -40,18 +40,21 @@ def random_sbs(seed, bs, pad, bpv, lods, n_bricks):
     for i, (x, y, z, lod) in enumerate(nodes):
         offset = rng.uniform(-0.3, 0.8)
         q = np.clip(0.5 + 0.5 * (offset + rng.uniform(-0.5, 0.5, v ** 3)), 0.0, 1.0)
-        qi = np.round(q * max_q).astype(np.uint64)
-        packed = np.zeros(words, np.uint64)
-        for k in range(v ** 3):
-            packed[k // per_word] |= qi[k] << np.uint64(8 * bpv * (k % per_word))
+        qi = np.zeros(words * per_word, np.uint64)
+        qi[:v ** 3] = np.round(q * max_q).astype(np.uint64)
+        # element k in bits 8 * bpv * (k % per_word) of word k // per_word
+        shifts = (8 * bpv * np.arange(per_word)).astype(np.uint64)
+        packed = np.bitwise_or.reduce(qi.reshape(words, per_word) << shifts, axis=1)
         values[i * words:(i + 1) * words] = packed.astype(np.uint32)
         arr.append(((x << 16) | y, (z << 16) | lod, i * words, 0))
     hdr = L.SdfSBSHeader(bs, pad, bpv, L.SDF_SBS_NODE_LAYOUT_DX)
     return L.SdfSBS(hdr, np.array(arr, np.uint32), values)
 
 
+# (brick_size, pad, bytes per value, LODs); the last four reach the largest
+# accepted bricks (brick_size 16, 19^3 values with padding)
 CASES = [(1, 0, 1, (16,)), (2, 0, 2, (8,)), (4, 0, 1, (8,)), (4, 1, 4, (4,)), (3, 1, 2, (6,)), (2, 1, 1, (8,)), (4, 0, 4, (6,)),
-         (2, 0, 1, (4, 8))]
+         (2, 0, 1, (4, 8)), (8, 1, 2, (4,)), (16, 1, 1, (2,)), (16, 0, 4, (3,)), (15, 1, 4, (2, 3))]
 
 
 @pytest.mark.parametrize("case", range(len(CASES)))
