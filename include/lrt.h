@@ -41,8 +41,10 @@ int lrt_destroy(lrt_handle h);
  * (:1416-1487). Copies the arrays, builds the BVH2 over brick boxes and uploads
  * the repacked scene. values_f may be NULL for DX layouts. aux_data bit 0 (an
  * extension; zero in reference data) selects the voxel diagonal as the DX
- * quantisation range. Limits: fewer than 2^24 bricks (the leaf index) and at
- * most 4 GiB of values (the kernels address them with 32-bit offsets). */
+ * quantisation range. Limits: brick_size 1..16 and brick_pad 0..1 (the voxel
+ * walk's index arithmetic relies on them), fewer than 2^24 bricks (the leaf
+ * index) and at most 4 GiB of values (the kernels address them with 32-bit
+ * offsets); other scenes return LRT_E_INVALID. */
 int lrt_set_scene_sbs(lrt_handle h, const SdfSBSHeader* header, const SdfSBSNode* nodes, uint32_t n_nodes, const uint32_t* values,
                       uint64_t n_values, const float* values_f, uint64_t n_values_f);
 
