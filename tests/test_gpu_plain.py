@@ -486,6 +486,9 @@ def test_bad_inputs_raise():
     bad.nodes[0, 2] = sbs.values.size  # data_offset out of range
     with pytest.raises(L.LrtError):
         r.SetScene(bad)
+    for bs, pad in ((17, 0), (16, 2), (0, 0)):  # brick sizes outside 1..16, padding outside 0..1
+        with pytest.raises(L.LrtError):
+            r.SetScene(L.SdfSBS(L.SdfSBSHeader(bs, pad, sbs.header.bytes_per_value, sbs.header.aux_data), sbs.nodes, sbs.values))
     r.SetScene(sbs)
     # sizes checked before any work; outputs NULL so no host buffers are needed
     for w, h in ((65535, 65535), (0, 16), (65536, 1)):  # 65535^2 rounds up to 65536^2 packed pixels, past 32 bits
