@@ -99,6 +99,46 @@ def test_full_size_views_sampled_against_oracle():
     r.close()
 
 
+def test_scene_beyond_the_caches_sampled_against_oracle():
+    """The 2048^3 displaced icosphere (brick LOD 512 x 4^3, 8-bit over the
+    voxel diagonal: 1.7 M bricks, a 24-deep BVH, about 440 MB of scene arrays,
+    past the 256 MB Infinity Cache): two orbit views at 1024x1024, 1,024
+    seeded pixels per view against the FP32 oracle's RayQuery_NearestHit on
+    the same eye rays, with the bars of the benchmark-shape test above."""
+    sbs = L.displaced_icosphere(6, 0.8, 0.05, 4, seed=2, lod=512, brick_size=4, brick_pad=0, bytes_per_value=1, voxel_diag_scale=True)
+    r = L.CreateMultiRenderer(L.DEVICE_GPU)
+    r.SetScene(sbs)
+    pairs, depth = r.GetBVH()
+    assert sbs.nodes.shape[0] > 1_000_000 and depth > 20
+    r.SetPreset(preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON, render_mode=L.MULTI_RENDER_MODE_NORMAL))
+    sc = oracle.Scene(sbs, pairs)
+    W = H = 1024
+    cams = [L.orbit_camera(4, i) for i in range(2)]
+    wvs, prs = np.stack([c[0] for c in cams]), np.stack([c[1] for c in cams])
+    r.RenderViewsDevice(wvs, prs, W, H, 0)
+    rng = np.random.default_rng(9)
+    n_hits = n_exact = n_mask = 0
+    for v in range(2):
+        t, n, prim = r.DownloadView(v, W, H)
+        xs = np.concatenate([[0, W - 1, 0, W - 1], rng.integers(0, W, 1020)])
+        ys = np.concatenate([[0, 0, H - 1, H - 1], rng.integers(0, H, 1020)])
+        rays = np.array([oracle.eye_ray(wvs[v], prs[v], int(x), int(y), 0.5, 0.5, W, H) for x, y in zip(xs, ys)], np.float32)
+        ref = sc.cast_rays(preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON), rays, precision=32)
+        g_prim, r_prim = prim[ys, xs], ref[:, 1].astype(np.uint32)
+        n_mask += int(((g_prim != MISS) != (r_prim != MISS)).sum())
+        hit = (g_prim != MISS) & (g_prim == r_prim)
+        assert hit.sum() > 100, v
+        gt, rt = t[ys, xs][hit], ref[hit, 0].astype(np.float32)
+        n_hits += int(hit.sum())
+        n_exact += int((gt == rt).sum())
+        assert np.max(np.abs(gt.astype(np.float64) - rt) / rt) <= 1e-6, v
+        assert np.max(angle(n[ys, xs][hit], ref[hit][:, 3:5].astype(np.float32))) <= 1e-5, v
+    print("hits", n_hits, "bit-exact", n_exact, "mask mismatches", n_mask)
+    assert n_mask <= 1
+    assert n_exact >= 0.999 * n_hits
+    r.close()
+
+
 @pytest.mark.parametrize("rmode", [L.MULTI_RENDER_MODE_MASK, L.MULTI_RENDER_MODE_LINEAR_DEPTH, L.MULTI_RENDER_MODE_DEPTH,
                                    L.MULTI_RENDER_MODE_NORMAL, L.MULTI_RENDER_MODE_LAMBERT_NO_TEX,
                                    L.MULTI_RENDER_MODE_PRIMITIVE, L.MULTI_RENDER_MODE_ST_ITERATIONS,
