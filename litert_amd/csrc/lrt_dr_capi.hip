@@ -267,6 +267,9 @@ static int dr_render_view(Renderer* r, const MultiRendererDRPreset& p, uint32_t 
   if ((e = r->d_dr_list.ensure(np * 4)) != hipSuccess) return hip_err(e, "alloc");
   if ((e = r->d_dr_misc.ensure(64)) != hipSuccess) return hip_err(e, "alloc");
   const uint64_t vs = r->header.brick_size + 2 * r->header.brick_pad + 1;
+  // the kernels address the gathered copy with 32-bit byte offsets (it is no
+  // larger than the index words SetScene already holds under 4 GiB)
+  if ((uint64_t)r->nodes.size() * vs * vs * vs * 4 > 0xFFFFFFFFull) return set_err(LRT_E_INVALID, "gathered brick values exceed 4 GiB");
   if ((e = r->d_dr_bvals.ensure((size_t)r->nodes.size() * vs * vs * vs * 4)) != hipSuccess) return hip_err(e, "alloc");
   if ((e = r->d_dr_bminmax.ensure((size_t)r->nodes.size() * 8)) != hipSuccess) return hip_err(e, "alloc");
   const uint32_t blocks = r->grid_blocks();
