@@ -123,6 +123,13 @@ int lrt_sync(lrt_handle h);
 
 /* Persistent-grid size of the render kernels (0 = device default). */
 int lrt_set_launch_blocks(lrt_handle h, uint32_t blocks);
+/* Tile culling of the unseeded G-buffer renders (no reference counterpart: a
+ * performance switch, on by default). Each launch first bounds the rays of
+ * every 8x8 pixel tile by a pyramid from the eye (k_tile_planes); a tile whose
+ * pyramid reaches no box of the BVH's top levels is written as misses without
+ * tracing. The images are identical either way; lrt_count_work(_ex) counts
+ * the work of the form that runs (out[6]: culled tiles). */
+int lrt_set_tile_culling(lrt_handle h, int on);
 
 /* ---- batched uneven workload (no reference counterpart: the reference
  * renders one scene per call, eye_ray_host.cpp:246-282) ------------------
@@ -254,7 +261,8 @@ int lrt_mlp_infer_dataset(lrt_handle h, uint32_t repeats, float* kernel_ms, floa
 int lrt_set_seeding(lrt_handle h, int enabled, float threshold);
 /* lrt_count_work with more counters: out[0] node pairs, [1] bricks, [2]
  * voxels, [3] root-finder iterations, [4] 16-sample MLP tiles, [5] brick
- * entries whose prediction cleared the gate. n_out <= 6. */
+ * entries whose prediction cleared the gate, [6] 8x8 tiles written as misses
+ * by the tile cull. n_out <= 7. */
 int lrt_count_work_ex(lrt_handle h, const float* world_views, const float* projs, uint32_t n_views, uint32_t width, uint32_t height,
                       uint64_t* out, uint32_t n_out);
 

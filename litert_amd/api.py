@@ -160,6 +160,7 @@ def _load():
         "lrt_count_work": (i32, [P, P, P, u32, u32, u32, P]),
         "lrt_sync": (i32, [P]),
         "lrt_set_launch_blocks": (i32, [P, u32]),
+        "lrt_set_tile_culling": (i32, [P, i32]),
         "lrt_sbs_displaced_icosphere": (i32, [u32, f32, f32, u32, u32, u32, u32, u32, u32, i32, i32, C.POINTER(P)]),
         "lrt_sbs_grid": (i32, [u32, f32, f32, f32, u32, u32, u32, u32, i32, C.POINTER(P)]),
         "lrt_dr_optimize_grid": (i32, [P, u32, i32, P, u32, u32, i32, C.POINTER(P), C.POINTER(f32)]),
@@ -522,10 +523,10 @@ class MultiRenderer:
         return out
 
     def CountWorkEx(self, world_views, projs, width, height):
-        """(pairs, bricks, voxels, root iterations, MLP tiles, gated seeds) of one counted launch."""
+        """(pairs, bricks, voxels, root iterations, MLP tiles, gated seeds, culled 8x8 tiles) of one counted launch."""
         wv, pr = _f32(world_views).reshape(-1, 16), _f32(projs).reshape(-1, 16)
-        out = np.zeros(6, np.uint64)
-        _check(lib.lrt_count_work_ex(self._h, _ptr(wv), _ptr(pr), wv.shape[0], width, height, _ptr(out), 6), "CountWorkEx")
+        out = np.zeros(7, np.uint64)
+        _check(lib.lrt_count_work_ex(self._h, _ptr(wv), _ptr(pr), wv.shape[0], width, height, _ptr(out), 7), "CountWorkEx")
         return out
 
     # ---- learned root-find seeder (no reference counterpart) ----
@@ -626,6 +627,10 @@ class MultiRenderer:
 
     def SetLaunchBlocks(self, blocks):
         _check(lib.lrt_set_launch_blocks(self._h, blocks), "SetLaunchBlocks")
+
+    def SetTileCulling(self, on):
+        """lrt_set_tile_culling: skip 8x8 tiles that provably miss (identical images)."""
+        _check(lib.lrt_set_tile_culling(self._h, int(bool(on))), "SetTileCulling")
 
 
 class MultiRendererDR(MultiRenderer):

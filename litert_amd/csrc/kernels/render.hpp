@@ -25,6 +25,10 @@ struct RenderArgs {
   // 8x8 super-blocks per view (n_packed / 64) and per row (packed_w / 8), as
   // divisors of a wave's super-block index (set_image)
   FastDiv sb_view, sb_row;
+  // Tile culling (null: every tile is traced): bit s set when super-block s
+  // of the launch (view * sb_view.d + its row-major index) provably reaches
+  // no brick (k_tile_cull) and is written as misses without tracing.
+  const uint32_t* tile_mask = nullptr;
 };
 
 // Image size of a launch: W x H pixels, packed to multiples of 8.
@@ -59,9 +63,16 @@ struct BatchItem {
 
 namespace lrt {
 hipError_t zero_u32(uint32_t* d, uint32_t n, hipStream_t stream);
+// A.tile_mask non-null: k_tile_cull first fills it (tile_mask_words(A)
+// words) from the cull boxes at d_cull_boxes: n_cull (<= 8) coarse boxes
+// {min, max} with the [first, end) range of their fine boxes in the w
+// components (uint32 bits), then the fine boxes (at most 64).
 hipError_t launch_render(const lrtd::DevScene& S, const lrtd::TraceParams& P, const lrtd::Camera* d_cams, const lrtd::RenderArgs& A,
                          const lrtd::RenderOut& O, uint32_t* d_work_counter, uint32_t* d_spill, uint32_t blocks, bool rgba,
-                         unsigned long long* d_counts, hipStream_t stream);
+                         unsigned long long* d_counts, hipStream_t stream, const float4* d_cull_boxes = nullptr, uint32_t n_cull = 0);
+// 32-bit words of a launch's tile mask (one bit per 8x8 super-block, whole
+// 64-bit words)
+inline size_t tile_mask_words(const lrtd::RenderArgs& A) { return 2 * (((size_t)A.n_views * A.sb_view.d + 63) / 64); }
 // offsets: n_items + 1 packed-pixel offsets (offsets[n_items] = total);
 // chunk: pixels per chunk-counter atomic (a multiple of 256).
 hipError_t launch_render_batch(const lrtd::BatchItem* d_items, const uint64_t* d_offsets, uint32_t n_items, uint64_t total,

@@ -32,6 +32,7 @@ struct WorkCount {
   uint32_t iters = 0;   // root-finder iterations
   uint32_t tiles = 0;   // 16-sample MLP tiles evaluated (seeded kernel)
   uint32_t seeds = 0;   // brick entries that passed the confidence gate
+  uint32_t culled = 0;  // 8x8 tiles written as misses by the tile cull (once per tile)
 };
 
 // Node pair `left` (BVHNodePair, 64 B) as four 16-B loads through a buffer

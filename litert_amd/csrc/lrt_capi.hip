@@ -126,6 +126,56 @@ hipError_t Renderer::ensure_spill(uint32_t blocks, uint32_t lds_entries) {
   return d_spill.ensure(n * sizeof(uint32_t));
 }
 
+// Boxes for the tile cull, two levels of the BVH's child slots whose union
+// holds every brick: the coarse level (at most 8 slots: the root pair's two,
+// then every inner slot replaced by its pair's two, level by level, while
+// the list stays within 8) and below each coarse slot its descendants at the
+// fine level (the same expansion of the whole list while it stays within 64;
+// a leaf stays as it is). Layout, {min, max} float4 pairs: the n_coarse coarse
+// boxes, whose w components hold the [first, end) range of their fine boxes
+// as uint32 bits, then the fine boxes grouped by coarse box.
+std::vector<float> cull_boxes(const std::vector<BVHNodePair>& pairs, uint32_t* n_coarse) {
+  std::vector<float> out;
+  *n_coarse = 0;
+  if (pairs.empty()) return out;
+  struct Slot { const BVHNode* n; uint32_t root; };  // root: index of the coarse slot above
+  auto expand = [&](const std::vector<Slot>& cur, size_t limit, std::vector<Slot>& res) {
+    res = cur;
+    for (;;) {
+      std::vector<Slot> next;
+      for (const Slot& sl : res) {
+        if (sl.n->leftOffset & LRT_LEAF_BIT) next.push_back(sl);
+        else {
+          const BVHNodePair& p = pairs[sl.n->leftOffset];
+          next.push_back({&p.left, sl.root});
+          next.push_back({&p.right, sl.root});
+        }
+      }
+      if (next.size() > limit || next.size() == res.size()) break;
+      res.swap(next);
+    }
+  };
+  std::vector<Slot> coarse, fine;
+  expand({{&pairs[0].left, 0u}, {&pairs[0].right, 0u}}, 8, coarse);
+  for (uint32_t c = 0; c < coarse.size(); c++) coarse[c].root = c;
+  expand(coarse, 64, fine);  // fine slots come out grouped by coarse slot, in order
+  auto put = [&](const BVHNode* n, uint32_t w0, uint32_t w1) {
+    float b[8] = {n->boxMin[0], n->boxMin[1], n->boxMin[2], 0.0f, n->boxMax[0], n->boxMax[1], n->boxMax[2], 0.0f};
+    std::memcpy(&b[3], &w0, 4);
+    std::memcpy(&b[7], &w1, 4);
+    out.insert(out.end(), b, b + 8);
+  };
+  for (uint32_t c = 0; c < coarse.size(); c++) {
+    uint32_t first = (uint32_t)fine.size(), end = 0;
+    for (uint32_t f = 0; f < fine.size(); f++)
+      if (fine[f].root == c) { first = std::min(first, f); end = f + 1; }
+    put(coarse[c].n, (uint32_t)coarse.size() + first, (uint32_t)coarse.size() + end);
+  }
+  for (const Slot& sl : fine) put(sl.n, 0u, 0u);
+  *n_coarse = (uint32_t)coarse.size();
+  return out;
+}
+
 // Per-brick constants of the voxel walk (BVH2Common.cpp:785-795, 749-750),
 // evaluated once here with exactly the float operations the per-ray code
 // used to repeat: 12 floats per brick, {bmin.xyz, data_offset bits},
@@ -168,6 +218,18 @@ static bool packed_fits(uint32_t W, uint32_t H) {
   return (uint64_t)(8 * ((W + 7) / 8)) * (uint64_t)(8 * ((H + 7) / 8)) <= 0xFFFFFFFFull;
 }
 
+// Do the preset's sample offsets (sample_offset) stay within [0, 1]^2 of
+// their pixel? Random jitter does; the regular pattern does when spp is a
+// perfect square. The tile cull relies on it.
+static bool samples_in_pixel(const MultiRenderPreset& p) {
+  if (p.spp == 0) return false;
+  if (p.ray_gen_mode == LRT_RAY_GEN_MODE_RANDOM) return true;
+  const uint32_t spp = p.spp;
+  uint32_t q = 0;
+  while ((q + 1) * (q + 1) <= spp) q++;
+  return q * q == spp;
+}
+
 // Launches n_views cameras at W x H into the handle's output buffers.
 int Renderer::render_device(const lrtd::Camera* cams, uint32_t n_views, uint32_t W, uint32_t H, bool rgba, bool gbuf, bool force_normal,
                             const char* timer_name, unsigned long long* d_counts) {
@@ -204,6 +266,10 @@ int Renderer::render_device(const lrtd::Camera* cams, uint32_t n_views, uint32_t
   A.lights = light_set();
   const uint32_t blocks = grid_blocks();
   const bool seeded = seeding && mlp_ready;
+  if (tile_cull && !seeded && !rgba && n_cull > 0 && samples_in_pixel(preset)) {
+    if ((e = d_tile_mask.ensure(tile_mask_words(A) * 4)) != hipSuccess) return hip_err(e, "alloc tile mask");
+    A.tile_mask = (const uint32_t*)d_tile_mask.p;
+  }
   if ((e = ensure_spill(blocks, seeded ? lrtd::kStackLdsSeeded : lrtd::kStackLds)) != hipSuccess) return hip_err(e, "alloc spill");
   if (seeded && (e = ensure_brick_features()) != hipSuccess) return hip_err(e, "brick features");
   if ((e = hipEventRecord(ev0, stream)) != hipSuccess) return hip_err(e, "event");
@@ -214,7 +280,7 @@ int Renderer::render_device(const lrtd::Camera* cams, uint32_t n_views, uint32_t
                              stream);
   } else {
     e = launch_render(dev_scene(), trace_params(force_normal), (const lrtd::Camera*)d_cams.p, A, O, (uint32_t*)d_counter.p,
-                      (uint32_t*)d_spill.p, blocks, rgba, d_counts, stream);
+                      (uint32_t*)d_spill.p, blocks, rgba, d_counts, stream, (const float4*)d_cull_boxes.p, n_cull);
   }
   if (e != hipSuccess) return hip_err(e, "render launch");
   if ((e = hipEventRecord(ev1, stream)) != hipSuccess) return hip_err(e, "event");
@@ -344,6 +410,8 @@ int lrt_set_scene_sbs(lrt_handle h, const SdfSBSHeader* header, const SdfSBSNode
   BuildTree tree = build_sah(bmin, bmax, 1);
   r->pairs = to_fat_pairs(tree, bmin, bmax, &r->pair_depth);
   if ((e = r->d_pairs.upload(r->pairs.data(), r->pairs.size() * sizeof(BVHNodePair), r->stream)) != hipSuccess) return hip_err(e, "upload bvh");
+  const std::vector<float> cb = cull_boxes(r->pairs, &r->n_cull);
+  if ((e = r->d_cull_boxes.upload(cb.data(), cb.size() * sizeof(float), r->stream)) != hipSuccess) return hip_err(e, "upload cull boxes");
   if ((e = r->d_nodes.upload(r->nodes.data(), r->nodes.size() * sizeof(SdfSBSNode), r->stream)) != hipSuccess) return hip_err(e, "upload nodes");
   std::vector<float> recs = brick_records(r->nodes, r->header, r->quant_voxel_diag);
   if ((e = r->d_brecs.upload(recs.data(), recs.size() * sizeof(float), r->stream)) != hipSuccess) return hip_err(e, "upload bricks");
@@ -508,7 +576,7 @@ int lrt_count_work_ex(lrt_handle h, const float* wvs, const float* prs, uint32_t
                       uint32_t n_out) {
   int c = check_ready(h);
   if (c) return c;
-  if (!wvs || !prs || !out || n_views == 0 || n_out == 0 || n_out > 6) return set_err(LRT_E_INVALID, "bad arguments");
+  if (!wvs || !prs || !out || n_views == 0 || n_out == 0 || n_out > 7) return set_err(LRT_E_INVALID, "bad arguments");
   std::vector<lrtd::Camera> cams(n_views);
   for (uint32_t v = 0; v < n_views; v++) cams[v] = make_camera(wvs + 16 * v, prs + 16 * v);
   hipError_t e;
@@ -670,6 +738,12 @@ int lrt_sync(lrt_handle h) {
   if (!h) return set_err(LRT_E_INVALID, "null handle");
   hipError_t e = hipStreamSynchronize(h->stream);
   return e == hipSuccess ? LRT_OK : hip_err(e, "sync");
+}
+
+int lrt_set_tile_culling(lrt_handle h, int on) {
+  if (!h) return set_err(LRT_E_INVALID, "null handle");
+  h->tile_cull = on != 0;
+  return LRT_OK;
 }
 
 int lrt_set_launch_blocks(lrt_handle h, uint32_t blocks) {

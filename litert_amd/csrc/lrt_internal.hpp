@@ -74,6 +74,12 @@ struct Renderer {
   DevBuf d_cams, d_counter, d_spill, d_counts, d_rays, d_hits;
   DevBuf d_out_t, d_out_n, d_out_prim, d_out_rgba;
   uint32_t last_w = 0, last_h = 0, last_views = 0;
+  // tile culling of the unseeded G-buffer renders (on by default): the BVH
+  // boxes every tile's pyramid is tested against (cull_boxes) and the
+  // launch's culled-tile bits (k_tile_cull)
+  bool tile_cull = true;
+  uint32_t n_cull = 0;  // coarse cull boxes (the fine ones follow them in d_cull_boxes)
+  DevBuf d_cull_boxes, d_tile_mask;
 
   // differentiable rendering state (MultiRendererDR)
   uint32_t dr_n_ref = 0, dr_w = 0, dr_h = 0;  // reference views; render size (d_dr_ref holds the references at it)
@@ -135,7 +141,7 @@ struct Renderer {
                      &d_dr_pair_slot, &d_dr_tiled, &d_dr_occ, &d_dr_redist, &d_dr_losses, &d_dr_views, &d_dr_cull_pairs, &d_dr_fd_p0, &d_dr_fd_ids, &d_dr_fd_pos, &d_dr_fd_acc, &d_mlp_params,
                      &d_mlp_m, &d_mlp_v, &d_mlp_packed, &d_mlp_partial, &d_mlp_ploss, &d_mlp_grad, &d_mlp_loss,
                      &d_bfeat, &d_ds_feat, &d_ds_tgt, &d_ds_meta, &d_ds_out, &d_b_items, &d_b_offsets, &d_b_t,
-                     &d_b_n, &d_b_prim};
+                     &d_b_n, &d_b_prim, &d_cull_boxes, &d_tile_mask};
     for (DevBuf* b : all) b->release();
   }
 };

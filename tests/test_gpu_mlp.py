@@ -193,7 +193,9 @@ def test_trained_seeder_keeps_tolerances_and_saves_iterations(r_c1):
     for mode in (ST, NEWTON):
         wv, pr = L.orbit_camera(7, 2)
         r_c1.SetPreset(preset(mode))
+        r_c1.SetTileCulling(False)  # the seeded render traces every tile
         plain = r_c1.CountWorkEx(wv[None], pr[None], 256, 256)
+        r_c1.SetTileCulling(True)
         r_c1.SetSeeding(True, 0.9)
         try:
             seeded = r_c1.CountWorkEx(wv[None], pr[None], 256, 256)

@@ -99,6 +99,62 @@ def test_full_size_views_sampled_against_oracle():
     r.close()
 
 
+def _culled_and_traced(r, wvs, prs, W, H):
+    """G-buffers of one multi-view launch with the tile cull on and off, and
+    the culled-tile count."""
+    out = []
+    for on in (True, False):
+        r.SetTileCulling(on)
+        r.RenderViewsDevice(wvs, prs, W, H, 0)
+        out.append([r.DownloadView(v, W, H) for v in range(wvs.shape[0])])
+    r.SetTileCulling(True)
+    culled = int(r.CountWorkEx(wvs, prs, W, H)[6])
+    return out[0], out[1], culled
+
+
+@pytest.mark.parametrize("case", ["orbit512", "random512", "orbit128_st", "spp4_random", "spp4_regular", "spp2_regular"])
+def test_tile_culling_leaves_images_identical(case):
+    """Tiles culled by their ray pyramid are written exactly as tracing writes
+    them: G-buffers with the cull on and off are bit-identical (t, normal,
+    primitive) over whole images, for orbit cameras, cameras around and
+    inside the scene with any fov and aspect, an indexed FP32 scene with sphere
+    tracing, and jittered / regular multi-sample presets. Orbit views cull most
+    tiles; a regular pattern whose samples leave the pixel (spp 2) culls none."""
+    rng = np.random.default_rng(31)
+    sbs, mode, spp, gen = ico512_scene(), L.SDF_OCTREE_NODE_INTERSECT_NEWTON, 1, L.RAY_GEN_MODE_REGULAR
+    if case == "orbit128_st":
+        sbs, mode = ico128_scene(), L.SDF_OCTREE_NODE_INTERSECT_ST
+    if case.startswith("spp"):
+        spp = int(case[3])
+        gen = L.RAY_GEN_MODE_RANDOM if case.endswith("random") else L.RAY_GEN_MODE_REGULAR
+    if case == "random512":
+        cams = []
+        for _ in range(8):
+            d = rng.normal(size=3)
+            eye = d / np.linalg.norm(d) * rng.uniform(0.3, 4.0)
+            cams.append(L.look_at_camera(eye, rng.uniform(-0.3, 0.3, 3), rng.normal(size=3), fov=float(rng.uniform(20, 110)),
+                                         aspect=float(rng.uniform(0.5, 2.0))))
+        W, H = 320, 240
+    else:
+        cams = [L.orbit_camera(11, i) for i in range(4)]
+        W, H = 512, 384
+    wvs, prs = np.stack([c[0] for c in cams]), np.stack([c[1] for c in cams])
+    r = L.CreateMultiRenderer(L.DEVICE_GPU)
+    r.SetScene(sbs)
+    r.SetPreset(preset(mode, render_mode=L.MULTI_RENDER_MODE_NORMAL, spp=spp, ray_gen=gen))
+    on, off, culled = _culled_and_traced(r, wvs, prs, W, H)
+    r.close()
+    for v in range(len(cams)):
+        for a, b in zip(on[v], off[v]):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (case, v)
+    tiles = len(cams) * ((W + 7) // 8) * ((H + 7) // 8)
+    print(case, "culled", culled, "of", tiles)
+    if case == "spp2_regular":
+        assert culled == 0
+    elif case != "random512":
+        assert culled > 0.3 * tiles
+
+
 def test_random_cameras_sampled_against_oracle():
     """12 seeded look-at cameras around and inside the 512^3 scene (eye at
     radius 0.3-4, target jittered off the origin, random up vector, fov
@@ -408,11 +464,16 @@ def test_cast_rays_any_hit(r_ico128, tfar):
 
 
 def test_work_counters_match_oracle(r_ico128):
-    """GPU per-ray work counters equal the FP32 oracle's (same traversal order)."""
+    """GPU per-ray work counters equal the FP32 oracle's (same traversal order;
+    tile culling off, so every ray is traced as the reference traces it)."""
     sc = oracle_scene("ico128")
     wv, pr = L.orbit_camera(7, 0)
     r_ico128.SetPreset(preset())
-    g = r_ico128.CountWork(wv[None], pr[None], 256, 256)
+    r_ico128.SetTileCulling(False)
+    try:
+        g = r_ico128.CountWork(wv[None], pr[None], 256, 256)
+    finally:
+        r_ico128.SetTileCulling(True)
     o = sc.render(wv, pr, preset(), 256, 256, precision=32, counters=True)["counters"]
     assert abs(int(g[0]) - o["node_pairs"]) <= 1e-4 * o["node_pairs"]
     assert abs(int(g[1]) - o["bricks"]) <= 1e-4 * o["bricks"]
