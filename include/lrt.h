@@ -123,7 +123,7 @@ int lrt_sync(lrt_handle h);
 
 /* Persistent-grid size of the render kernels (0 = device default). */
 int lrt_set_launch_blocks(lrt_handle h, uint32_t blocks);
-/* Tile culling of the unseeded G-buffer renders (no reference counterpart: a
+/* Tile culling of the unseeded renders (no reference counterpart: a
  * performance switch, on by default). Each launch first bounds the rays of
  * every 8x8 pixel tile by a pyramid from the eye (k_tile_planes); a tile whose
  * pyramid reaches no box of the BVH's top levels is written as misses without

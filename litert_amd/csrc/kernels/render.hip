@@ -38,7 +38,7 @@ __device__ __forceinline__ void render_pixel(const DevScene& S, const TraceParam
   const uint32_t y = ((bhy * 2 + (lh >> 1)) << 2) + (l >> 2);
   const Camera& C = cams[view];
   // a tile the cull pass marked: written as tracing writes a miss
-  const bool culled = !RGBA && A.tile_mask && ((A.tile_mask[wsb >> 5] >> (wsb & 31u)) & 1u);
+  const bool culled = A.tile_mask && ((A.tile_mask[wsb >> 5] >> (wsb & 31u)) & 1u);
   if (x >= A.W || y >= A.H) return;
   const size_t pix = (size_t)view * A.W * A.H + (size_t)y * A.W + x;
   if (COUNT && culled && r == 0) wc.culled++;
@@ -76,7 +76,8 @@ __device__ __forceinline__ void render_pixel(const DevScene& S, const TraceParam
   }
   if (RGBA) {
     const float s = (float)A.spp;
-    O.rgba[pix] = make_float4(acc.x / s, acc.y / s, acc.z / s, acc.w / s);
+    // every sample of a culled tile shades as a miss, (0, 0, 0, 1)
+    O.rgba[pix] = culled ? make_float4(0.0f, 0.0f, 0.0f, 1.0f) : make_float4(acc.x / s, acc.y / s, acc.z / s, acc.w / s);
   }
 }
 

@@ -266,7 +266,7 @@ int Renderer::render_device(const lrtd::Camera* cams, uint32_t n_views, uint32_t
   A.lights = light_set();
   const uint32_t blocks = grid_blocks();
   const bool seeded = seeding && mlp_ready;
-  if (tile_cull && !seeded && !rgba && n_cull > 0 && samples_in_pixel(preset)) {
+  if (tile_cull && !seeded && n_cull > 0 && samples_in_pixel(preset)) {
     if ((e = d_tile_mask.ensure(tile_mask_words(A) * 4)) != hipSuccess) return hip_err(e, "alloc tile mask");
     A.tile_mask = (const uint32_t*)d_tile_mask.p;
   }

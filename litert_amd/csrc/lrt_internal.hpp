@@ -74,7 +74,7 @@ struct Renderer {
   DevBuf d_cams, d_counter, d_spill, d_counts, d_rays, d_hits;
   DevBuf d_out_t, d_out_n, d_out_prim, d_out_rgba;
   uint32_t last_w = 0, last_h = 0, last_views = 0;
-  // tile culling of the unseeded G-buffer renders (on by default): the BVH
+  // tile culling of the unseeded renders (on by default): the BVH
   // boxes every tile's pyramid is tested against (cull_boxes) and the
   // launch's culled-tile bits (k_tile_cull)
   bool tile_cull = true;

@@ -155,6 +155,26 @@ def test_tile_culling_leaves_images_identical(case):
         assert culled > 0.3 * tiles
 
 
+@pytest.mark.parametrize("rmode", [L.MULTI_RENDER_MODE_PHONG_NO_TEX, L.MULTI_RENDER_MODE_LAMBERT_NO_TEX, L.MULTI_RENDER_MODE_LINEAR_DEPTH])
+def test_tile_culling_leaves_colour_images_identical(rmode):
+    """RenderFloat (the shading path, shadow rays included) with the tile cull
+    on and off: bit-identical colour images; a culled tile shades as a miss."""
+    r = L.CreateMultiRenderer(L.DEVICE_GPU)
+    r.SetScene(ico512_scene())
+    p = preset(L.SDF_OCTREE_NODE_INTERSECT_NEWTON, render_mode=rmode)
+    imgs = []
+    for on in (True, False):
+        r.SetTileCulling(on)
+        imgs.append([r.RenderFloat(384, 256, *L.orbit_camera(13, i), p) for i in range(2)])
+    r.SetTileCulling(True)
+    r.SetPreset(p)
+    culled = int(r.CountWorkEx(*[np.stack(x) for x in zip(*[L.orbit_camera(13, i) for i in range(2)])], 384, 256)[6])
+    r.close()
+    for a, b in zip(*imgs):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert culled > 0
+
+
 def test_random_cameras_sampled_against_oracle():
     """12 seeded look-at cameras around and inside the 512^3 scene (eye at
     radius 0.3-4, target jittered off the origin, random up vector, fov
