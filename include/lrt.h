@@ -123,9 +123,10 @@ int lrt_sync(lrt_handle h);
 
 /* Persistent-grid size of the render kernels (0 = device default). */
 int lrt_set_launch_blocks(lrt_handle h, uint32_t blocks);
-/* Tile culling of the unseeded renders (no reference counterpart: a
- * performance switch, on by default). Each launch first bounds the rays of
- * every 8x8 pixel tile by a pyramid from the eye (k_tile_planes); a tile whose
+/* Tile culling of the single-scene renders, unseeded and MLP-seeded (no
+ * reference counterpart: a performance switch, on by default; the batched
+ * render does not cull). Each launch first bounds the rays of
+ * every 8x8 pixel tile by a pyramid from the eye (k_tile_cull); a tile whose
  * pyramid reaches no box of the BVH's top levels is written as misses without
  * tracing. The images are identical either way; lrt_count_work(_ex) counts
  * the work of the form that runs (out[6]: culled tiles). */

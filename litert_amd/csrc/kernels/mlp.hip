@@ -574,6 +574,7 @@ __device__ __forceinline__ void flush_counts(const WorkCount& wc, unsigned long 
   atomicAdd(&counts[3], (unsigned long long)wc.iters);
   atomicAdd(&counts[4], (unsigned long long)wc.tiles);
   atomicAdd(&counts[5], (unsigned long long)wc.seeds);
+  atomicAdd(&counts[6], (unsigned long long)wc.culled);
 }
 
 template <bool COUNT, bool RGBA, int RF, bool IDX>
@@ -608,6 +609,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
       const uint32_t tid = (uint32_t)(gtid - (uint64_t)view * A.n_packed);
       tid_to_xy(tid, A.packed_w, x, y);
       valid = x < A.W && y < A.H;
+    }
+    // A tile the cull pass marked (every ray provably misses every brick)
+    // is written as tracing writes a miss. The wave's 64 packed pixels are
+    // one 8x8 super-block (chunks start on multiples of 256), so the test is
+    // wave-uniform; waves past the launch's end (wsb >= its tile count) read
+    // no mask word.
+    const uint32_t wsb = __builtin_amdgcn_readfirstlane((uint32_t)(gtid >> 6));
+    if (A.tile_mask && (uint64_t)wsb * 64 < total && ((A.tile_mask[wsb >> 5] >> (wsb & 31u)) & 1u)) {
+      if (valid) {
+        const size_t pix = (size_t)view * A.W * A.H + (size_t)y * A.W + x;
+        if (O.t) O.t[pix] = 1e9f;
+        if (O.prim) O.prim[pix] = 0xFFFFFFFFu;
+        if (O.n) O.n[pix] = make_float2(0.0f, 0.0f);
+        if (RGBA) O.rgba[pix] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        if (COUNT && lane == 0) wc.culled++;
+      }
+      continue;
     }
     seeded_pixel<COUNT, RGBA, RF, IDX>(S, P, bfeat, packed, seed_logit, cams[view], A, O, view, x, y, valid, st, wc, fl[wave], ol[wave],
                                        lane);
@@ -735,9 +753,10 @@ struct SeededLaunch {
 hipError_t launch_render_seeded(const lrtd::DevScene& S, const lrtd::TraceParams& P, const float4* d_bfeat, const void* d_packed,
                                 float seed_logit, const lrtd::Camera* d_cams, const lrtd::RenderArgs& A, const lrtd::RenderOut& O,
                                 uint32_t* d_work_counter, uint32_t* d_spill, uint32_t blocks, bool rgba, unsigned long long* d_counts,
-                                hipStream_t stream) {
+                                hipStream_t stream, const float4* d_cull_boxes, uint32_t n_cull) {
   hipError_t e = zero_u32(d_work_counter, 1, stream);
   if (e != hipSuccess) return e;
+  if (A.tile_mask && (e = launch_tile_cull(d_cams, A, d_cull_boxes, n_cull, stream)) != hipSuccess) return e;
   SeededLaunch L{S, P, d_bfeat, d_packed, seed_logit, d_cams, A, O, d_work_counter, d_spill, blocks, d_counts, stream};
   const bool indexed = S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F || S.layout == LRT_SBS_LAYOUT_ID32F_IRGB32F_IN;
   if (indexed) return L.pick<-1, true>(rgba);

@@ -83,7 +83,7 @@ hipError_t launch_mlp_reduce_adam(float* d_params, float* d_m, float* d_v, const
 hipError_t launch_render_seeded(const lrtd::DevScene& S, const lrtd::TraceParams& P, const float4* d_bfeat, const void* d_packed,
                                 float seed_logit, const lrtd::Camera* d_cams, const lrtd::RenderArgs& A, const lrtd::RenderOut& O,
                                 uint32_t* d_work_counter, uint32_t* d_spill, uint32_t blocks, bool rgba, unsigned long long* d_counts,
-                                hipStream_t stream);
+                                hipStream_t stream, const float4* d_cull_boxes = nullptr, uint32_t n_cull = 0);
 hipError_t launch_render_batch_seeded(const lrtd::BatchItem* d_items, const uint64_t* d_offsets, uint32_t n_items, uint64_t total,
                                       const lrtd::TraceParams& P, bool indexed, const void* d_packed, float seed_logit, uint32_t chunk,
                                       uint32_t* d_work_counter, uint32_t* d_spill, uint32_t blocks, unsigned long long* d_counts,

@@ -379,18 +379,21 @@ struct RenderLaunch {
   }
 };
 
+hipError_t launch_tile_cull(const lrtd::Camera* d_cams, const lrtd::RenderArgs& A, const float4* d_cull_boxes, uint32_t n_cull,
+                            hipStream_t stream) {
+  if (!A.tile_mask || !d_cull_boxes || n_cull == 0 || n_cull > 8) return hipErrorInvalidValue;
+  const uint32_t n_tiles = A.n_views * A.sb_view.d;  // < 2^32: checked with the launch's pixel count
+  hipLaunchKernelGGL(lrtd::k_tile_cull, dim3((n_tiles + 255) / 256), dim3(256), 0, stream, d_cams, A, n_tiles, d_cull_boxes, n_cull,
+                     const_cast<uint32_t*>(A.tile_mask));
+  return hipGetLastError();
+}
+
 hipError_t launch_render(const lrtd::DevScene& S, const lrtd::TraceParams& P, const lrtd::Camera* d_cams, const lrtd::RenderArgs& A,
                          const lrtd::RenderOut& O, uint32_t* d_work_counter, uint32_t* d_spill, uint32_t blocks, bool rgba,
                          unsigned long long* d_counts, hipStream_t stream, const float4* d_cull_boxes, uint32_t n_cull) {
   hipError_t e = zero_u32(d_work_counter, lrtd::kXcds * lrtd::kCounterStride, stream);
   if (e != hipSuccess) return e;
-  if (A.tile_mask) {
-    if (!d_cull_boxes || n_cull == 0 || n_cull > 8) return hipErrorInvalidValue;
-    const uint32_t n_tiles = A.n_views * A.sb_view.d;  // < 2^32: checked with the launch's pixel count
-    hipLaunchKernelGGL(lrtd::k_tile_cull, dim3((n_tiles + 255) / 256), dim3(256), 0, stream, d_cams, A, n_tiles, d_cull_boxes, n_cull,
-                       const_cast<uint32_t*>(A.tile_mask));
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
+  if (A.tile_mask && (e = launch_tile_cull(d_cams, A, d_cull_boxes, n_cull, stream)) != hipSuccess) return e;
   RenderLaunch L{S, P, d_cams, A, O, d_work_counter, d_spill, blocks, d_counts, stream};
   return dispatch_variant(S, P, d_counts != nullptr, rgba, L);
 }

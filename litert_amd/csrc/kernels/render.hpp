@@ -63,6 +63,11 @@ struct BatchItem {
 
 namespace lrt {
 hipError_t zero_u32(uint32_t* d, uint32_t n, hipStream_t stream);
+// Fills A.tile_mask (tile_mask_words(A) words, one bit per 8x8 super-block:
+// set = every ray of the tile provably misses every brick) from the cull
+// boxes described at launch_render.
+hipError_t launch_tile_cull(const lrtd::Camera* d_cams, const lrtd::RenderArgs& A, const float4* d_cull_boxes, uint32_t n_cull,
+                            hipStream_t stream);
 // A.tile_mask non-null: k_tile_cull first fills it (tile_mask_words(A)
 // words) from the cull boxes at d_cull_boxes: n_cull (<= 8) coarse boxes
 // {min, max} with the [first, end) range of their fine boxes in the w

@@ -266,7 +266,7 @@ int Renderer::render_device(const lrtd::Camera* cams, uint32_t n_views, uint32_t
   A.lights = light_set();
   const uint32_t blocks = grid_blocks();
   const bool seeded = seeding && mlp_ready;
-  if (tile_cull && !seeded && n_cull > 0 && samples_in_pixel(preset)) {
+  if (tile_cull && n_cull > 0 && samples_in_pixel(preset)) {
     if ((e = d_tile_mask.ensure(tile_mask_words(A) * 4)) != hipSuccess) return hip_err(e, "alloc tile mask");
     A.tile_mask = (const uint32_t*)d_tile_mask.p;
   }
@@ -277,7 +277,7 @@ int Renderer::render_device(const lrtd::Camera* cams, uint32_t n_views, uint32_t
     const float logit = std::log(seed_threshold / (1.0f - seed_threshold));
     e = launch_render_seeded(dev_scene(), trace_params(force_normal), (const float4*)d_bfeat.p, d_mlp_packed.p, logit,
                              (const lrtd::Camera*)d_cams.p, A, O, (uint32_t*)d_counter.p, (uint32_t*)d_spill.p, blocks, rgba, d_counts,
-                             stream);
+                             stream, (const float4*)d_cull_boxes.p, n_cull);
   } else {
     e = launch_render(dev_scene(), trace_params(force_normal), (const lrtd::Camera*)d_cams.p, A, O, (uint32_t*)d_counter.p,
                       (uint32_t*)d_spill.p, blocks, rgba, d_counts, stream, (const float4*)d_cull_boxes.p, n_cull);

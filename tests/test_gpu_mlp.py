@@ -193,9 +193,7 @@ def test_trained_seeder_keeps_tolerances_and_saves_iterations(r_c1):
     for mode in (ST, NEWTON):
         wv, pr = L.orbit_camera(7, 2)
         r_c1.SetPreset(preset(mode))
-        r_c1.SetTileCulling(False)  # the seeded render traces every tile
         plain = r_c1.CountWorkEx(wv[None], pr[None], 256, 256)
-        r_c1.SetTileCulling(True)
         r_c1.SetSeeding(True, 0.9)
         try:
             seeded = r_c1.CountWorkEx(wv[None], pr[None], 256, 256)
@@ -204,6 +202,7 @@ def test_trained_seeder_keeps_tolerances_and_saves_iterations(r_c1):
             r_c1.SetSeeding(False)
         print(mode, "plain", plain, "seeded", seeded, res)
         assert seeded[0] == plain[0] and seeded[1] == plain[1]  # same traversal
+        assert seeded[6] == plain[6] > 0  # the same tiles culled
         assert res["mask_mismatch_fp64"] <= 1e-4 * res["n_pix"] and res["mismatch_all_grazing"]
         assert res["depth_rel_max"] <= 1e-5
         assert res["normal_bad"] == 0
@@ -216,6 +215,41 @@ def test_trained_seeder_keeps_tolerances_and_saves_iterations(r_c1):
         else:
             assert seeded[3] <= 1.01 * plain[3]
     r_c1.SetPreset(preset(NEWTON))
+
+
+@pytest.mark.parametrize("rmode", [None, L.MULTI_RENDER_MODE_PHONG_NO_TEX])
+def test_seeded_tile_culling_leaves_images_identical(r_c1, rmode):
+    """The seeded render skips the tiles the cull pass marks and writes them
+    as misses: G-buffers (t, normal, primitive) of a two-view launch, or a
+    shaded image with shadow rays, bit-identical with the cull on and off,
+    with confident seeds everywhere (adversarial network) and most tiles culled."""
+    r_c1.MlpSetParams(_adversarial_params(5))
+    cams = [L.orbit_camera(17, i) for i in range(2)]
+    wvs, prs = np.stack([c[0] for c in cams]), np.stack([c[1] for c in cams])
+    W, H = 320, 192
+    p = preset(NEWTON) if rmode is None else preset(NEWTON, render_mode=rmode)
+    r_c1.SetPreset(p)
+    r_c1.SetSeeding(True, 0.9)
+    out = []
+    try:
+        for on in (True, False):
+            r_c1.SetTileCulling(on)
+            if rmode is None:
+                r_c1.RenderViewsDevice(wvs, prs, W, H, 0)
+                out.append([a for v in range(2) for a in r_c1.DownloadView(v, W, H)])
+            else:
+                out.append([r_c1.RenderFloat(W, H, *c, p) for c in cams])
+        r_c1.SetTileCulling(True)
+        cnt = r_c1.CountWorkEx(wvs, prs, W, H)
+    finally:
+        r_c1.SetSeeding(False)
+        r_c1.SetTileCulling(True)
+        r_c1.SetPreset(preset(NEWTON))
+    for a, b in zip(*out):
+        assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
+    tiles = 2 * (W // 8) * (H // 8)
+    print("seeded culled", cnt[6], "of", tiles, "seeds", cnt[5])
+    assert cnt[6] > 0.3 * tiles and cnt[5] > 0
 
 
 def test_seeding_rejects_bad_arguments(r_c1):
