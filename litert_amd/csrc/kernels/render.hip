@@ -215,15 +215,15 @@ __global__ __launch_bounds__(256) void k_cast_rays(DevScene S, TraceParams P, co
 // max(1e-5, 16 * measured error): every point of the [-1, 1]^3 scene frame
 // lies within |e| + sqrt(3) of the eye. A box farther than the margin outside
 // one plane holds no point of any of the tile's rays. The fine cull boxes
-// together hold every brick, so a tile none of whose boxes reaches into its
+// (a cut through the BVH: every leaf lies under exactly one) together hold every brick, so a tile none of whose boxes reaches into its
 // pyramid traces to a miss on every ray. A tile whose cone is degenerate (corners not
 // in convex position, p.w changing sign) or whose eye lies far away
 // (|e| > 1e3) is traced.
 //
 // One lane per tile (64 tiles per wave): the pyramid, then the cull boxes
-// (read by the whole wave: the n_coarse coarse boxes, and the fine boxes of
-// a coarse box that reaches a lane's pyramid) against it; bit t of mask =
-// tile t culled.
+// against it, read by the whole wave: the n_coarse coarse boxes, the middle
+// boxes of a coarse box that reaches a lane's pyramid, and the fine boxes of
+// a middle box that does; bit t of mask = tile t culled.
 __global__ __launch_bounds__(256) void k_tile_cull(const Camera* __restrict__ cams, RenderArgs A, uint32_t n_tiles,
                                                    const float4* __restrict__ boxes, uint32_t n_coarse, uint32_t* __restrict__ mask) {
   const uint32_t t = blockIdx.x * 256 + threadIdx.x;
@@ -316,12 +316,19 @@ __global__ __launch_bounds__(256) void k_tile_cull(const Camera* __restrict__ ca
     const float4 clo = boxes[2 * c], chi = boxes[2 * c + 1];
     const bool cand = !touch && reaches(clo, chi);
     if (__builtin_amdgcn_ballot_w64(cand) == 0) continue;
-    const uint32_t f0 = __float_as_uint(clo.w), f1 = __float_as_uint(chi.w);
+    const uint32_t m0 = __float_as_uint(clo.w), m1 = __float_as_uint(chi.w);
     bool hit = false;
-    for (uint32_t f = f0; f < f1; f++) {
+    for (uint32_t m = m0; m < m1; m++) {
       if (__builtin_amdgcn_ballot_w64(cand && !hit) == 0) break;
-      const float4 lo = boxes[2 * f], hi = boxes[2 * f + 1];
-      hit |= cand && reaches(lo, hi);
+      const float4 mlo = boxes[2 * m], mhi = boxes[2 * m + 1];
+      const bool mc = cand && !hit && reaches(mlo, mhi);
+      if (__builtin_amdgcn_ballot_w64(mc) == 0) continue;
+      const uint32_t f0 = __float_as_uint(mlo.w), f1 = __float_as_uint(mhi.w);
+      for (uint32_t f = f0; f < f1; f++) {
+        if (__builtin_amdgcn_ballot_w64(mc && !hit) == 0) break;
+        const float4 lo = boxes[2 * f], hi = boxes[2 * f + 1];
+        hit |= mc && reaches(lo, hi);
+      }
     }
     touch |= hit;
   }

@@ -69,9 +69,11 @@ hipError_t zero_u32(uint32_t* d, uint32_t n, hipStream_t stream);
 hipError_t launch_tile_cull(const lrtd::Camera* d_cams, const lrtd::RenderArgs& A, const float4* d_cull_boxes, uint32_t n_cull,
                             hipStream_t stream);
 // A.tile_mask non-null: k_tile_cull first fills it (tile_mask_words(A)
-// words) from the cull boxes at d_cull_boxes: n_cull (<= 8) coarse boxes
-// {min, max} with the [first, end) range of their fine boxes in the w
-// components (uint32 bits), then the fine boxes (at most 64).
+// words) from the cull boxes at d_cull_boxes, three levels of BVH node boxes
+// {min, max}: n_cull (<= 8) coarse boxes, then the middle boxes (at most 64),
+// then the fine boxes (at most 512); a coarse or middle box holds the
+// [first, end) range of the boxes under it on the next level in its w
+// components (uint32 bits).
 hipError_t launch_render(const lrtd::DevScene& S, const lrtd::TraceParams& P, const lrtd::Camera* d_cams, const lrtd::RenderArgs& A,
                          const lrtd::RenderOut& O, uint32_t* d_work_counter, uint32_t* d_spill, uint32_t blocks, bool rgba,
                          unsigned long long* d_counts, hipStream_t stream, const float4* d_cull_boxes = nullptr, uint32_t n_cull = 0);

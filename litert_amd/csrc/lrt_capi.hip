@@ -138,7 +138,7 @@ std::vector<float> cull_boxes(const std::vector<BVHNodePair>& pairs, uint32_t* n
   std::vector<float> out;
   *n_coarse = 0;
   if (pairs.empty()) return out;
-  struct Slot { const BVHNode* n; uint32_t root; };  // root: index of the coarse slot above
+  struct Slot { const BVHNode* n; uint32_t root; };  // root: index of the slot above it on the level before
   auto expand = [&](const std::vector<Slot>& cur, size_t limit, std::vector<Slot>& res) {
     res = cur;
     for (;;) {
@@ -155,24 +155,46 @@ std::vector<float> cull_boxes(const std::vector<BVHNodePair>& pairs, uint32_t* n
       res.swap(next);
     }
   };
-  std::vector<Slot> coarse, fine;
+  // three cuts through the tree, each one below the last (a leaf stays in
+  // every later cut); expansion keeps children next to each other, so the
+  // slots under one upper slot come out contiguous and in order
+  std::vector<Slot> coarse, mid, fine;
   expand({{&pairs[0].left, 0u}, {&pairs[0].right, 0u}}, 8, coarse);
   for (uint32_t c = 0; c < coarse.size(); c++) coarse[c].root = c;
-  expand(coarse, 64, fine);  // fine slots come out grouped by coarse slot, in order
+  expand(coarse, 64, mid);
+  std::vector<uint32_t> mid_parent, fine_parent;
+  for (uint32_t m = 0; m < mid.size(); m++) mid_parent.push_back(mid[m].root), mid[m].root = m;
+  expand(mid, 512, fine);
+  for (const Slot& sl : fine) fine_parent.push_back(sl.root);
   auto put = [&](const BVHNode* n, uint32_t w0, uint32_t w1) {
     float b[8] = {n->boxMin[0], n->boxMin[1], n->boxMin[2], 0.0f, n->boxMax[0], n->boxMax[1], n->boxMax[2], 0.0f};
     std::memcpy(&b[3], &w0, 4);
     std::memcpy(&b[7], &w1, 4);
     out.insert(out.end(), b, b + 8);
   };
-  for (uint32_t c = 0; c < coarse.size(); c++) {
-    uint32_t first = (uint32_t)fine.size(), end = 0;
-    for (uint32_t f = 0; f < fine.size(); f++)
-      if (fine[f].root == c) { first = std::min(first, f); end = f + 1; }
-    put(coarse[c].n, (uint32_t)coarse.size() + first, (uint32_t)coarse.size() + end);
+  // [first, end) of the slots of a lower level (parent[f]: the upper slot
+  // above slot f) under upper slot u, as indices into the table (the lower
+  // level starts at `at`)
+  auto range = [](const std::vector<uint32_t>& parent, uint32_t u, uint32_t at, uint32_t* first, uint32_t* end) {
+    *first = (uint32_t)parent.size();
+    *end = 0;
+    for (uint32_t f = 0; f < parent.size(); f++)
+      if (parent[f] == u) { *first = std::min(*first, f); *end = f + 1; }
+    *first += at;
+    *end += at;
+  };
+  const uint32_t nc = (uint32_t)coarse.size(), nm = (uint32_t)mid.size();
+  uint32_t first, end;
+  for (uint32_t c = 0; c < nc; c++) {
+    range(mid_parent, c, nc, &first, &end);
+    put(coarse[c].n, first, end);
+  }
+  for (uint32_t m = 0; m < nm; m++) {
+    range(fine_parent, m, nc + nm, &first, &end);
+    put(mid[m].n, first, end);
   }
   for (const Slot& sl : fine) put(sl.n, 0u, 0u);
-  *n_coarse = (uint32_t)coarse.size();
+  *n_coarse = nc;
   return out;
 }
 
